@@ -1,0 +1,128 @@
+/*
+ * kestrel_amd.h -- C ABI of the MI355X (gfx950) pairwise Jacquard / kinship
+ * maximum-likelihood fit.  Plain C types only; every call returns 0 on success
+ * or a negative KESTREL_E* code, with a message in kestrel_last_error().
+ *
+ * Reference interfaces replaced (paths in the kestrel 0.1.2 source tree):
+ *   kestrel_cohort_create     wrapper.rs:89-207 up to the pair loop: allele
+ *                             frequencies by per-site SQP (allele/mod.rs:37-134,
+ *                             maf default 1/(2N) wrapper.rs:101), zero-frequency
+ *                             allele removal and non-segregating site drop
+ *                             (wrapper.rs:237-410), lane layout
+ *                             (lanearray2.rs:48-80); input = hard calls as the
+ *                             one-hot GL rows of vcf.rs:338-382.
+ *   kestrel_fit_pairs         coefficients.rs:40-141
+ *                             calculate_relatedness_coefficients_gl: every pair
+ *                             (x <= y) fitted by sqp.rs:49-93; Output records of
+ *                             coefficients.rs:143-151.
+ *   kestrel_evaluate          the operator boundary sqp.rs:8-14 implemented by
+ *                             JacquardCoefficientData (jacquard/mod.rs:6-21):
+ *                             objective + gradient + Hessian at given points.
+ *   kestrel_tune_default      sqp.rs:31-46 Tuneables::new(iterations, 10)
+ *                             (coefficients.rs:73).
+ *
+ * Ownership: the caller owns every host buffer; a cohort owns its device
+ * copies.  A cohort is used from one host thread at a time.
+ */
+#ifndef KESTREL_AMD_H
+#define KESTREL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KESTREL_OK 0
+#define KESTREL_EINVAL (-1)      /* bad argument (reference: assert!/panic!) */
+#define KESTREL_EDEVICE (-2)     /* HIP runtime error or no gfx950 device */
+#define KESTREL_ENOMEM (-3)      /* device allocation failed */
+#define KESTREL_EDATA (-4)       /* a site has no called sample (lanevector.rs:73-76) */
+
+/* per-pair status in kestrel_output.status */
+#define KESTREL_PAIR_OK 0u
+#define KESTREL_PAIR_NO_SITES 1u /* no usable site: the reference panics (lanevector.rs:133-138) */
+
+/* precision forms */
+#define KESTREL_FP64 0
+
+/* sqp.rs:16-29 Tuneables */
+typedef struct {
+    uint64_t sqp_max_iter;
+    double sqp_conv_tol;
+    uint64_t qp_max_iter;
+    double qp_conv_tol;
+    double qp_zero_search_tol;
+    uint64_t bls_max_iter;
+    double bls_sufficient_decrease;
+    double bls_step_size_reduce;
+    double epsilon;
+} kestrel_tune;
+
+/* coefficients.rs:143-151 Output + usable-site count n_xy + status (112 bytes) */
+typedef struct {
+    uint32_t x, y;
+    double jacquard[9];
+    double obj;
+    uint64_t iters;
+    double cond;
+    uint32_t n_sites;
+    uint32_t status;
+} kestrel_output;
+
+typedef struct kestrel_cohort kestrel_cohort;
+
+/* Defaults of sqp.rs:31-46 with sqp_max_iter = iterations (cli.rs:33-35) and
+ * qp_max_iter = 10 (coefficients.rs:73). */
+int kestrel_tune_default(kestrel_tune* tune, uint64_t iterations);
+
+/* Upload a cohort of 2-bit genotype codes and prepare it for pair fits.
+ *   packed      n_samples rows of row_stride_bytes; site s of a row is in bits
+ *               2*(s%4) of byte s/4; code 0 = 0/0, 1 = 0/1, 2 = 1/1 (the
+ *               unphased VCF index, coefficients.rs:18-38), 3 = missing.
+ *   af          NULL: estimate allele frequencies on the device as the
+ *               reference does for -f GL/PL (allele/mod.rs:37-134) with
+ *               af_iterations SQP iterations and minimum allele frequency maf
+ *               (maf < 0: 1/(2 n_samples), wrapper.rs:101).  Non-NULL: n_sites
+ *               x 2 frequencies (REF, ALT) used as given.
+ * Sites where either frequency is 0 are dropped (wrapper.rs:352-410).
+ * Returns KESTREL_EDATA if some site has no called sample. */
+int kestrel_cohort_create(kestrel_cohort** out, const uint8_t* packed, uint32_t n_samples, uint32_t n_sites,
+                          size_t row_stride_bytes, const double* af, double maf, uint64_t af_iterations,
+                          int precision);
+
+/* Frequencies of all input sites (n_sites x 2) and the kept-site mask. */
+int kestrel_cohort_allele_freqs(const kestrel_cohort* c, double* af_out, uint8_t* kept_out);
+/* Number of kept (segregating) sites. */
+int kestrel_cohort_n_kept(const kestrel_cohort* c, uint32_t* n_kept);
+
+/* Fit pairs.  pairs: n_pairs (x, y) sample indices, or NULL for the upper
+ * triangle including the diagonal of the first n_select samples in row-major
+ * order (coefficients.rs:57,87), n_pairs = n_select (n_select + 1) / 2.
+ * out has one record per pair, in pair order. Blocks until done. */
+int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_t* pairs, size_t n_pairs,
+                      uint32_t n_select, kestrel_output* out);
+
+/* Objective, gradient and Hessian of the Jacquard likelihood at n points:
+ * pairs n x 2, points n x 9; f n, g n x 9, h n x 81 (row-major), n_sites n. */
+int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* points, size_t n, double eps,
+                     double* f, double* g, double* h, uint32_t* n_sites);
+
+/* Tuning knobs: "segments" (site segments per sweep, 0 = auto), "slots"
+ * (pairs in flight, 0 = auto). */
+int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value);
+
+/* Timing of the last kestrel_fit_pairs: sweep-kernel milliseconds (sum of
+ * launch durations by HIP events), number of sweeps, wall milliseconds. */
+int kestrel_last_fit_stats(const kestrel_cohort* c, double* sweep_ms, uint64_t* sweeps, double* wall_ms);
+
+void kestrel_cohort_destroy(kestrel_cohort* c);
+
+const char* kestrel_last_error(void);
+const char* kestrel_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,145 @@
+"""Host-side interface of the MI355X Jacquard fit, mirroring the reference's
+batch boundary wrapper::calculate_relatedness_coefficients_gl (wrapper.rs:89-207):
+a cohort of hard calls in, one Output record per pair out (coefficients.rs:143-151).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import OUTPUT_DTYPE, Tune, check, ptr
+from .synth import pack2
+
+KINSHIP_VEC = np.array([1.0, 0.0, 0.5, 0.0, 0.5, 0.0, 0.5, 0.25, 0.0])  # cli.rs:150
+
+
+def tune_default(iterations: int = 100) -> Tune:
+    """sqp.rs:31-46 with sqp_max_iter = iterations and qp_max_iter = 10 (coefficients.rs:73)."""
+    t = Tune()
+    check(_lib.load().kestrel_tune_default(ctypes.byref(t), iterations))
+    return t
+
+
+class Cohort:
+    """Device-resident cohort of 2-bit hard calls (kestrel_cohort_create)."""
+
+    def __init__(self, codes: np.ndarray | None = None, *, packed: np.ndarray | None = None,
+                 n_sites: int | None = None, af: np.ndarray | None = None, maf: float | None = None,
+                 af_iterations: int = 100, segment_sites: int | None = None):
+        L = _lib.load()
+        if packed is None:
+            codes = np.ascontiguousarray(codes, np.uint8)
+            n_sites = codes.shape[1]
+            packed = pack2(codes)
+        packed = np.ascontiguousarray(packed, np.uint8)
+        self.n_samples = packed.shape[0]
+        self.n_sites = int(n_sites)
+        afp = None
+        if af is not None:
+            afp = np.ascontiguousarray(af, np.float64).reshape(self.n_sites, 2)
+        h = ctypes.c_void_p()
+        check(L.kestrel_cohort_create(ctypes.byref(h), ptr(packed), self.n_samples, self.n_sites, packed.shape[1],
+                                      ptr(afp), -1.0 if maf is None else float(maf), af_iterations, _lib.FP64))
+        self._h = h
+        if segment_sites is not None:
+            self.set_option("segment_sites", segment_sites)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().kestrel_cohort_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_option(self, key: str, value: int):
+        check(_lib.load().kestrel_set_option(self._h, key.encode(), int(value)))
+
+    @property
+    def n_kept(self) -> int:
+        v = ctypes.c_uint32()
+        check(_lib.load().kestrel_cohort_n_kept(self._h, ctypes.byref(v)))
+        return v.value
+
+    def allele_freqs(self):
+        af = np.zeros((self.n_sites, 2))
+        kept = np.zeros(self.n_sites, np.uint8)
+        check(_lib.load().kestrel_cohort_allele_freqs(self._h, ptr(af), ptr(kept)))
+        return af, kept.astype(bool)
+
+    def fit_pairs(self, pairs: np.ndarray | None = None, n_select: int | None = None,
+                  tune: Tune | None = None, iterations: int = 100) -> np.ndarray:
+        """Pairs (M, 2) or None for the upper triangle incl. the diagonal of
+        the first n_select samples (coefficients.rs:57, 87)."""
+        t = tune or tune_default(iterations)
+        if pairs is None:
+            n_select = self.n_samples if n_select is None else n_select
+            n = n_select * (n_select + 1) // 2
+            pp = None
+        else:
+            pp = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+            n = pp.shape[0]
+            n_select = 0
+        out = np.zeros(n, OUTPUT_DTYPE)
+        check(_lib.load().kestrel_fit_pairs(self._h, ctypes.byref(t), ptr(pp), n, n_select, ptr(out)))
+        return out
+
+    def evaluate(self, pairs: np.ndarray, points: np.ndarray, eps: float = 1e-8):
+        """Objective, gradient and Hessian at points (M, 9) for pairs (M, 2)."""
+        pp = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        pts = np.ascontiguousarray(points, np.float64).reshape(-1, 9)
+        n = pp.shape[0]
+        f, g, h = np.zeros(n), np.zeros((n, 9)), np.zeros((n, 9, 9))
+        ns = np.zeros(n, np.uint32)
+        check(_lib.load().kestrel_evaluate(self._h, ptr(pp), ptr(pts), n, eps, ptr(f), ptr(g), ptr(h), ptr(ns)))
+        return f, g, h, ns
+
+    def last_fit_stats(self):
+        ms, sw, wall = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_double()
+        check(_lib.load().kestrel_last_fit_stats(self._h, ctypes.byref(ms), ctypes.byref(sw), ctypes.byref(wall)))
+        return {"sweep_ms": ms.value, "sweeps": sw.value, "wall_ms": wall.value}
+
+
+def segment_bounds(n_kept: int, segment_sites: int = 4096, chunk: int = 32, pad: int = 256) -> np.ndarray:
+    """Site segment bounds the device uses for a cohort with n_kept sites (the
+    summation order of the sweep: sequential within a segment, segments folded
+    left to right)."""
+    v_pad = max(pad, (n_kept + pad - 1) // pad * pad)
+    chunks = v_pad // chunk
+    per = max(chunk, segment_sites) // chunk
+    nseg = max(1, (chunks + per - 1) // per)
+    b = [min(chunks, k * per) * chunk for k in range(nseg + 1)]
+    b[-1] = v_pad
+    return np.array(b, np.uint32)
+
+
+def kinship(jacquard: np.ndarray) -> np.ndarray:
+    """cli.rs:150,166: kinship = dot(delta, [1, 0, .5, 0, .5, 0, .5, .25, 0]) as an fma chain."""
+    j = np.asarray(jacquard, np.float64)
+    s = np.zeros(j.shape[:-1])
+    for k in range(9):
+        s = s + j[..., k] * KINSHIP_VEC[k]  # products with 0, .25, .5, 1 are exact
+    return s
+
+
+def build_arrays(outputs: np.ndarray, n: int):
+    """lib.rs:120-161: (N, N, 9) Jacquard array with delta3<->delta5 and
+    delta4<->delta6 swapped for (y, x), (N, N) kinship, (N,) inbreeding 2*theta_xx - 1."""
+    jac = np.full((n, n, 9), np.nan)
+    kin = np.full((n, n), np.nan)
+    swap = [0, 1, 4, 5, 2, 3, 6, 7, 8]
+    k = kinship(outputs["jacquard"])
+    for o, kk in zip(outputs, k):
+        x, y = int(o["x"]), int(o["y"])
+        jac[x, y] = o["jacquard"]
+        jac[y, x] = o["jacquard"][swap]
+        kin[x, y] = kin[y, x] = kk
+    inb = 2.0 * np.diagonal(kin) - 1.0
+    return jac, kin, inb

@@ -1,0 +1,417 @@
+// api.cpp -- host side of libkestrel_amd: cohort upload, allele frequencies,
+// kept-site repacking, and the sweep/step loop of the pair fit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <memory>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kestrel_internal.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail(e_ == hipErrorOutOfMemory ? KESTREL_ENOMEM : KESTREL_EDEVICE, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                             \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        release();
+        n = count;
+        return hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    }
+};
+
+kdev::Tune to_dev(const kestrel_tune& t) {
+    kdev::Tune d;
+    d.sqp_max_iter = t.sqp_max_iter;
+    d.sqp_conv_tol = t.sqp_conv_tol;
+    d.qp_max_iter = t.qp_max_iter;
+    d.qp_conv_tol = t.qp_conv_tol;
+    d.qp_zero_search_tol = t.qp_zero_search_tol;
+    d.bls_max_iter = t.bls_max_iter;
+    d.bls_sufficient_decrease = t.bls_sufficient_decrease;
+    d.bls_step_size_reduce = t.bls_step_size_reduce;
+    d.epsilon = t.epsilon;
+    return d;
+}
+
+}  // namespace
+
+struct kestrel_cohort {
+    int device = 0;
+    int n_cu = 0;
+    hipStream_t stream = nullptr;
+    uint32_t n_samples = 0, n_sites = 0, n_kept = 0, v_pad = 0, row_words = 0;
+    DevBuf<uint32_t> geno;  // n_samples x row_words packed kept sites
+    DevBuf<double> af;      // v_pad x 2 kept (p, q), padding (0.5, 0.5)
+    DevBuf<uint32_t> seg;   // segment bounds
+    uint32_t nseg = 0;
+    std::vector<double> af_all;
+    std::vector<uint8_t> kept_mask;
+    int64_t opt_slots = 0;
+    int64_t opt_segment_sites = 4096;
+    double last_sweep_ms = 0, last_wall_ms = 0;
+    uint64_t last_sweeps = 0;
+
+    int set_segments() {
+        const uint32_t chunks = v_pad / CHUNK;
+        uint32_t per = (uint32_t)std::max<int64_t>(CHUNK, opt_segment_sites) / CHUNK;
+        nseg = std::max<uint32_t>(1, (chunks + per - 1) / per);
+        std::vector<uint32_t> b(nseg + 1);
+        for (uint32_t k = 0; k <= nseg; k++) b[k] = std::min(chunks, k * per) * CHUNK;
+        b[nseg] = v_pad;
+        HIPCHK(seg.alloc(nseg + 1));
+        HIPCHK(hipMemcpy(seg.p, b.data(), b.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        return KESTREL_OK;
+    }
+
+    ~kestrel_cohort() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+extern "C" {
+
+const char* kestrel_last_error(void) { return g_err.c_str(); }
+const char* kestrel_version(void) { return "kestrel_amd 0.1.0 (gfx950)"; }
+
+int kestrel_tune_default(kestrel_tune* t, uint64_t iterations) {
+    if (!t || iterations == 0) return fail(KESTREL_EINVAL, "iterations must be non-zero");
+    t->sqp_max_iter = iterations;
+    t->sqp_conv_tol = 1e-8;
+    t->qp_max_iter = 10;
+    t->qp_conv_tol = 1e-10;
+    t->qp_zero_search_tol = 1e-12;
+    t->bls_max_iter = 10;
+    t->bls_sufficient_decrease = 1e-4;
+    t->bls_step_size_reduce = 0.9;
+    t->epsilon = 1e-8;
+    return KESTREL_OK;
+}
+
+int kestrel_cohort_create(kestrel_cohort** out, const uint8_t* packed, uint32_t n_samples, uint32_t n_sites,
+                          size_t row_stride_bytes, const double* af, double maf, uint64_t af_iterations,
+                          int precision) {
+    if (!out || !packed) return fail(KESTREL_EINVAL, "null argument");
+    *out = nullptr;
+    if (n_samples == 0 || n_sites == 0) return fail(KESTREL_EINVAL, "need at least one sample and one site");
+    if (row_stride_bytes < (n_sites + 3) / 4) return fail(KESTREL_EINVAL, "row stride too small");
+    if (precision != KESTREL_FP64) return fail(KESTREL_EINVAL, "unknown precision %d", precision);
+    if (maf < 0) maf = 1.0 / (2.0 * (double)n_samples);  // wrapper.rs:101
+    if (!(maf >= 0.0 && maf <= 1.0)) return fail(KESTREL_EINVAL, "Minimum allele frequency must be within 0 to 1");
+    if (!af && af_iterations == 0) return fail(KESTREL_EINVAL, "iterations must be non-zero");
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(KESTREL_EDEVICE, "no HIP device: libkestrel_amd needs an MI355X (gfx950) GPU");
+
+    auto c = new kestrel_cohort();
+    std::unique_ptr<kestrel_cohort> guard(c);
+    HIPCHK(hipGetDevice(&c->device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, c->device));
+    c->n_cu = prop.multiProcessorCount;
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->n_samples = n_samples;
+    c->n_sites = n_sites;
+
+    DevBuf<uint8_t> dpacked;
+    HIPCHK(dpacked.alloc((size_t)n_samples * row_stride_bytes));
+    HIPCHK(hipMemcpyAsync(dpacked.p, packed, (size_t)n_samples * row_stride_bytes, hipMemcpyHostToDevice,
+                          c->stream));
+
+    c->af_all.resize((size_t)n_sites * 2);
+    if (af) {
+        std::memcpy(c->af_all.data(), af, c->af_all.size() * sizeof(double));
+    } else {
+        DevBuf<double> daf;
+        DevBuf<int> dstatus;
+        HIPCHK(daf.alloc((size_t)n_sites * 2));
+        HIPCHK(dstatus.alloc(1));
+        HIPCHK(hipMemsetAsync(dstatus.p, 0, sizeof(int), c->stream));
+        kestrel_tune t;
+        kestrel_tune_default(&t, af_iterations);
+        t.qp_max_iter = 3;  // allele/mod.rs:93: A + 1
+        HIPCHK(klaunch::af(dpacked.p, row_stride_bytes, n_samples, n_sites, to_dev(t), maf, daf.p, dstatus.p,
+                           c->stream));
+        int status = 0;
+        HIPCHK(hipMemcpyAsync(c->af_all.data(), daf.p, c->af_all.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(&status, dstatus.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (status) return fail(KESTREL_EDATA, "Must have at least one row to analyze (a site has no called sample)");
+    }
+
+    // wrapper.rs:352-410: keep sites with two non-zero allele frequencies
+    c->kept_mask.assign(n_sites, 0);
+    std::vector<uint32_t> kept;
+    kept.reserve(n_sites);
+    for (uint32_t s = 0; s < n_sites; s++)
+        if (c->af_all[2 * s] > 0.0 && c->af_all[2 * s + 1] > 0.0) {
+            c->kept_mask[s] = 1;
+            kept.push_back(s);
+        }
+    c->n_kept = (uint32_t)kept.size();
+    c->v_pad = std::max<uint32_t>(SITE_PAD, (c->n_kept + SITE_PAD - 1) / SITE_PAD * SITE_PAD);
+    c->row_words = c->v_pad / 16;
+
+    std::vector<double> afk((size_t)c->v_pad * 2, 0.5);
+    for (uint32_t t = 0; t < c->n_kept; t++) {
+        afk[2 * t] = c->af_all[2 * kept[t]];
+        afk[2 * t + 1] = c->af_all[2 * kept[t] + 1];
+    }
+    HIPCHK(c->af.alloc(afk.size()));
+    HIPCHK(hipMemcpyAsync(c->af.p, afk.data(), afk.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    DevBuf<uint32_t> dkept;
+    HIPCHK(dkept.alloc(std::max<size_t>(kept.size(), 1)));
+    if (!kept.empty())
+        HIPCHK(hipMemcpyAsync(dkept.p, kept.data(), kept.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              c->stream));
+    HIPCHK(c->geno.alloc((size_t)n_samples * c->row_words));
+    HIPCHK(klaunch::gather(dpacked.p, row_stride_bytes, dkept.p, c->n_kept, n_samples, c->geno.p, c->row_words,
+                           c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int rc = c->set_segments();
+    if (rc) return rc;
+    *out = guard.release();
+    return KESTREL_OK;
+}
+
+int kestrel_cohort_allele_freqs(const kestrel_cohort* c, double* af_out, uint8_t* kept_out) {
+    if (!c) return fail(KESTREL_EINVAL, "null cohort");
+    if (af_out) std::memcpy(af_out, c->af_all.data(), c->af_all.size() * sizeof(double));
+    if (kept_out) std::memcpy(kept_out, c->kept_mask.data(), c->kept_mask.size());
+    return KESTREL_OK;
+}
+
+int kestrel_cohort_n_kept(const kestrel_cohort* c, uint32_t* n_kept) {
+    if (!c || !n_kept) return fail(KESTREL_EINVAL, "null argument");
+    *n_kept = c->n_kept;
+    return KESTREL_OK;
+}
+
+int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value) {
+    if (!c || !key) return fail(KESTREL_EINVAL, "null argument");
+    if (!strcmp(key, "slots")) {
+        if (value < 0) return fail(KESTREL_EINVAL, "slots must be >= 0");
+        c->opt_slots = value;
+        return KESTREL_OK;
+    }
+    if (!strcmp(key, "segment_sites")) {
+        if (value < CHUNK || value % CHUNK) return fail(KESTREL_EINVAL, "segment_sites must be a multiple of %d", CHUNK);
+        c->opt_segment_sites = value;
+        return c->set_segments();
+    }
+    return fail(KESTREL_EINVAL, "unknown option %s", key);
+}
+
+int kestrel_last_fit_stats(const kestrel_cohort* c, double* sweep_ms, uint64_t* sweeps, double* wall_ms) {
+    if (!c) return fail(KESTREL_EINVAL, "null cohort");
+    if (sweep_ms) *sweep_ms = c->last_sweep_ms;
+    if (sweeps) *sweeps = c->last_sweeps;
+    if (wall_ms) *wall_ms = c->last_wall_ms;
+    return KESTREL_OK;
+}
+
+int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_t* pairs, size_t n_pairs,
+                      uint32_t n_select, kestrel_output* out) {
+    if (!c || !tune || !out) return fail(KESTREL_EINVAL, "null argument");
+    if (tune->sqp_max_iter == 0 || tune->qp_max_iter == 0 || tune->bls_max_iter == 0)
+        return fail(KESTREL_EINVAL, "iteration limits must be non-zero");
+    std::vector<uint32_t> tri;
+    if (!pairs) {
+        if (n_select > c->n_samples)  // coefficients.rs:53-55
+            return fail(KESTREL_EINVAL, "Requested samples %u exceeds total samples %u", n_select, c->n_samples);
+        n_pairs = (size_t)n_select * (n_select + 1) / 2;
+        tri.resize(2 * n_pairs);
+        size_t i = 0;
+        for (uint32_t x = 0; x < n_select; x++)
+            for (uint32_t y = x; y < n_select; y++, i++) {
+                tri[2 * i] = x;
+                tri[2 * i + 1] = y;
+            }
+        pairs = tri.data();
+    } else {
+        for (size_t i = 0; i < 2 * n_pairs; i++)
+            if (pairs[i] >= c->n_samples) return fail(KESTREL_EINVAL, "pair %zu names sample %u of %u", i / 2, pairs[i],
+                                                      c->n_samples);
+    }
+    if (n_pairs == 0) return KESTREL_OK;
+    if (c->n_kept == 0) {
+        const double nan = std::nan("");
+        for (size_t i = 0; i < n_pairs; i++) {
+            kestrel_output& o = out[i];
+            o.x = pairs[2 * i];
+            o.y = pairs[2 * i + 1];
+            for (double& v : o.jacquard) v = nan;
+            o.obj = nan;
+            o.iters = 0;
+            o.cond = nan;
+            o.n_sites = 0;
+            o.status = KESTREL_PAIR_NO_SITES;
+        }
+        return KESTREL_OK;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+
+    // pairs in flight: fill the device (2 blocks of 256 lanes per CU) over
+    // nseg segments, never more than the pairs available
+    const uint64_t capacity = (uint64_t)std::max(1, c->n_cu) * 2 * SWEEP_TPB;
+    uint64_t nslots = c->opt_slots > 0 ? (uint64_t)c->opt_slots : std::max<uint64_t>(SWEEP_TPB, capacity / c->nseg);
+    nslots = std::min<uint64_t>(nslots, ((uint64_t)n_pairs + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB);
+    nslots = (nslots + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB;
+    const uint32_t ns = (uint32_t)nslots;
+
+    DevBuf<uint32_t> dpairs, dsx, dsy;
+    DevBuf<double> dpt, dpart;
+    DevBuf<int> dactive, dlive;
+    DevBuf<SlotState> dst;
+    DevBuf<unsigned long long> dnext;
+    DevBuf<kestrel_output> dout;
+    HIPCHK(dpairs.alloc(2 * n_pairs));
+    HIPCHK(dsx.alloc(ns));
+    HIPCHK(dsy.alloc(ns));
+    HIPCHK(dpt.alloc((size_t)9 * ns));
+    HIPCHK(dpart.alloc((size_t)c->nseg * NPART * ns));
+    HIPCHK(dactive.alloc(ns));
+    HIPCHK(dlive.alloc(1));
+    HIPCHK(dst.alloc(ns));
+    HIPCHK(dnext.alloc(1));
+    HIPCHK(dout.alloc(n_pairs));
+    hipStream_t st = c->stream;
+    HIPCHK(hipMemcpyAsync(dpairs.p, pairs, 2 * n_pairs * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(dst.p, 0, (size_t)ns * sizeof(SlotState), st));  // PH_IDLE
+    HIPCHK(hipMemsetAsync(dsx.p, 0, (size_t)ns * sizeof(uint32_t), st));
+    HIPCHK(hipMemsetAsync(dsy.p, 0, (size_t)ns * sizeof(uint32_t), st));
+    HIPCHK(hipMemsetAsync(dactive.p, 0, (size_t)ns * sizeof(int), st));
+    HIPCHK(hipMemsetAsync(dnext.p, 0, sizeof(unsigned long long), st));
+    int live = (int)ns;
+    HIPCHK(hipMemcpyAsync(dlive.p, &live, sizeof(int), hipMemcpyHostToDevice, st));
+    const kdev::Tune dt = to_dev(*tune);
+
+    hipEvent_t ev[2];
+    HIPCHK(hipEventCreate(&ev[0]));
+    HIPCHK(hipEventCreate(&ev[1]));
+    struct EvGuard {
+        hipEvent_t* e;
+        ~EvGuard() {
+            (void)hipEventDestroy(e[0]);
+            (void)hipEventDestroy(e[1]);
+        }
+    } evg{ev};
+
+    // claim the first pairs
+    HIPCHK(klaunch::step(dst.p, dsx.p, dsy.p, dpt.p, dactive.p, ns, dpart.p, c->nseg, dpairs.p, n_pairs, dnext.p,
+                         dout.p, dt, dlive.p, st));
+    double sweep_ms = 0;
+    uint64_t sweeps = 0;
+    // every pair needs at most 1 + sqp_max_iter * bls_max_iter evaluations
+    const uint64_t per_pair = 1 + tune->sqp_max_iter * tune->bls_max_iter;
+    const uint64_t max_sweeps = per_pair * (((uint64_t)n_pairs + ns - 1) / ns + 1) + 16;
+    for (;;) {
+        if (sweeps > max_sweeps) return fail(KESTREL_EDEVICE, "pair fit did not terminate after %llu sweeps",
+                                             (unsigned long long)sweeps);
+        HIPCHK(hipEventRecord(ev[0], st));
+        HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, dsx.p, dsy.p, dpt.p, dactive.p, ns,
+                              tune->epsilon, dpart.p, st));
+        HIPCHK(hipEventRecord(ev[1], st));
+        HIPCHK(klaunch::step(dst.p, dsx.p, dsy.p, dpt.p, dactive.p, ns, dpart.p, c->nseg, dpairs.p, n_pairs, dnext.p,
+                             dout.p, dt, dlive.p, st));
+        HIPCHK(hipMemcpyAsync(&live, dlive.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        sweep_ms += ms;
+        sweeps++;
+        if (live == 0) break;
+    }
+    HIPCHK(hipMemcpyAsync(out, dout.p, n_pairs * sizeof(kestrel_output), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    c->last_sweep_ms = sweep_ms;
+    c->last_sweeps = sweeps;
+    c->last_wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return KESTREL_OK;
+}
+
+int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* points, size_t n, double eps,
+                     double* f, double* g, double* h, uint32_t* n_sites) {
+    if (!c || !pairs || !points || !f || !g || !h || !n_sites) return fail(KESTREL_EINVAL, "null argument");
+    if (n == 0) return KESTREL_OK;
+    for (size_t i = 0; i < 2 * n; i++)
+        if (pairs[i] >= c->n_samples) return fail(KESTREL_EINVAL, "pair names sample %u of %u", pairs[i], c->n_samples);
+    const uint32_t ns = (uint32_t)((n + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB);
+    std::vector<uint32_t> hx(ns, 0), hy(ns, 0);
+    std::vector<int> hact(ns, 0);
+    std::vector<double> hpt((size_t)9 * ns, 0.0);
+    for (size_t i = 0; i < n; i++) {
+        hx[i] = pairs[2 * i];
+        hy[i] = pairs[2 * i + 1];
+        hact[i] = 1;
+        for (int k = 0; k < 9; k++) hpt[(size_t)k * ns + i] = points[i * 9 + k];
+    }
+    DevBuf<uint32_t> dsx, dsy, dn;
+    DevBuf<double> dpt, dpart, df, dg, dh;
+    DevBuf<int> dact;
+    HIPCHK(dsx.alloc(ns));
+    HIPCHK(dsy.alloc(ns));
+    HIPCHK(dact.alloc(ns));
+    HIPCHK(dpt.alloc((size_t)9 * ns));
+    HIPCHK(dpart.alloc((size_t)c->nseg * NPART * ns));
+    HIPCHK(df.alloc(n));
+    HIPCHK(dg.alloc(9 * n));
+    HIPCHK(dh.alloc(81 * n));
+    HIPCHK(dn.alloc(n));
+    hipStream_t st = c->stream;
+    HIPCHK(hipMemcpyAsync(dsx.p, hx.data(), ns * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dsy.p, hy.data(), ns * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dact.p, hact.data(), ns * sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dpt.p, hpt.data(), hpt.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, dsx.p, dsy.p, dpt.p, dact.p, ns, eps,
+                          dpart.p, st));
+    HIPCHK(klaunch::finalize(dpart.p, c->nseg, ns, (uint32_t)n, df.p, dg.p, dh.p, dn.p, st));
+    HIPCHK(hipMemcpyAsync(f, df.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(g, dg.p, 9 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(h, dh.p, 81 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(n_sites, dn.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return KESTREL_OK;
+}
+
+void kestrel_cohort_destroy(kestrel_cohort* c) { delete c; }
+
+}  // extern "C"

@@ -1,0 +1,583 @@
+// kernels.hip -- gfx950 kernels of the pairwise Jacquard maximum-likelihood fit.
+//
+//   af_kernel      per-site allele-frequency fit (allele/mod.rs:78-134), one
+//                  site per thread, 8-lane summation order of the reference.
+//   gather_kernel  repack the kept (segregating) sites of the 2-bit cohort.
+//   sweep_fp64     one evaluation pass: for 64 pairs per wavefront and one
+//                  segment of sites, the objective sum of log(eps + c_s.D)
+//                  (jacquard/objective.rs:42-72), the gradient sum of c_s / L
+//                  and the 45 Hessian sums of (c_s/L)(c_s/L)^T
+//                  (jacquard/grad_hess.rs:89-449), fused in one sweep.  c_s is
+//                  read from an LDS table built per 32-site chunk with the
+//                  exact expressions of mixture.rs:96-236.
+//   step_kernel    per-slot SQP state machine (sqp.rs:49-93, 270-310): folds
+//                  the segment partials, accepts or backtracks, runs the 9x9
+//                  active-set QP, writes finished pairs and claims the next
+//                  pair from an atomic counter (work stealing).
+//   finalize_eval  folds partials into (f, g, H) for explicit evaluation points.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kestrel_internal.hpp"
+#include "sqp_device.hpp"
+
+using namespace kdev;
+
+namespace {
+
+// ------------------------------------------------------------ allele fit
+
+__device__ __forceinline__ double radd8(const double* v) {
+    // Rust stdarch _mm512_reduce_add_pd: 256-bit halves, 128-bit halves, pair.
+    const double a0 = v[0] + v[4], a1 = v[1] + v[5], a2 = v[2] + v[6], a3 = v[3] + v[7];
+    const double b0 = a0 + a2, b1 = a1 + a3;
+    return b0 + b1;
+}
+
+struct AlleleSite {
+    const uint8_t* col;  // byte of this site in sample row 0
+    size_t stride;       // row stride in bytes
+    uint32_t shift;      // bit offset of the site inside its byte
+    uint32_t n_samples;
+    double eps;
+
+    __device__ __forceinline__ uint32_t code(uint32_t i) const { return (col[(size_t)i * stride] >> shift) & 3u; }
+
+    // allele/objective.rs:85-110 over the one-hot rows of the called samples
+    __device__ double obj(const double* x) const {
+        const double x00 = x[0] * x[0], x11 = x[1] * x[1], x01 = x[0] * x[1];
+        const double tx01 = x01 + x01;
+        double zs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        uint32_t buf = 0, cnt = 0, nb = 0;
+        auto row = [&](uint32_t c) {
+            double p = eps;
+            p = fma(x00, c == 0 ? 1.0 : 0.0, p);
+            p = fma(tx01, c == 1 ? 1.0 : 0.0, p);
+            p = fma(x11, c == 2 ? 1.0 : 0.0, p);
+            return sleef_log(p);
+        };
+        for (uint32_t i = 0; i < n_samples; i++) {
+            const uint32_t c = code(i);
+            if (c == 3) continue;
+            buf |= c << (2 * cnt);
+            if (++cnt == 8) {
+#pragma unroll
+                for (int j = 0; j < 8; j++) zs[j] = row((buf >> (2 * j)) & 3u) + zs[j];
+                buf = 0;
+                cnt = 0;
+                nb++;
+            }
+        }
+        double rs = 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if ((uint32_t)j < cnt) rs = row((buf >> (2 * j)) & 3u) + rs;
+        const double n = (double)(nb * 8 + cnt);
+        return -(radd8(zs) + rs) / n;
+    }
+
+    // allele/grad_hess.rs:107-166 + build_grad_hess 83-105
+    __device__ void gh(const double* x, double* g, double* h /* MX stride */) const {
+        double zg0[8], zg1[8], zh0[8], zh1[8], zh2[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) zg0[j] = zg1[j] = zh0[j] = zh1[j] = zh2[j] = 0.0;
+        const double x0 = x[0], x1 = x[1];
+        auto row = [&](uint32_t c, double& g0, double& g1, double& h0, double& h1, double& h2) {
+            const double l00 = c == 0 ? 1.0 : 0.0, l01 = c == 1 ? 1.0 : 0.0, l11 = c == 2 ? 1.0 : 0.0;
+            double lx0 = x0 * l00;
+            lx0 = fma(x1, l01, lx0);
+            double lx1 = x0 * l01;
+            lx1 = fma(x1, l11, lx1);
+            double d = eps;
+            d = fma(x0, lx0, d);
+            d = fma(x1, lx1, d);
+            const double inv = 1.0 / d;
+            const double a = lx0 * inv, b = lx1 * inv;
+            h0 = fma(2.0, a * a, h0);
+            h0 = fma(-l00, inv, h0);
+            h1 = fma(2.0, a * b, h1);
+            h1 = fma(-l01, inv, h1);
+            h2 = fma(2.0, b * b, h2);
+            h2 = fma(-l11, inv, h2);
+            g0 = fma(lx0, inv, g0);
+            g1 = fma(lx1, inv, g1);
+        };
+        uint32_t buf = 0, cnt = 0, nb = 0;
+        for (uint32_t i = 0; i < n_samples; i++) {
+            const uint32_t c = code(i);
+            if (c == 3) continue;
+            buf |= c << (2 * cnt);
+            if (++cnt == 8) {
+#pragma unroll
+                for (int j = 0; j < 8; j++) row((buf >> (2 * j)) & 3u, zg0[j], zg1[j], zh0[j], zh1[j], zh2[j]);
+                buf = 0;
+                cnt = 0;
+                nb++;
+            }
+        }
+        double rg0 = 0, rg1 = 0, rh0 = 0, rh1 = 0, rh2 = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if ((uint32_t)j < cnt) row((buf >> (2 * j)) & 3u, rg0, rg1, rh0, rh1, rh2);
+        const double n = (double)(nb * 8 + cnt);
+        g[0] = -2.0 * (radd8(zg0) + rg0) / n;
+        g[1] = -2.0 * (radd8(zg1) + rg1) / n;
+        const double h01 = 2.0 * (radd8(zh1) + rh1) / n;
+        h[0] = 2.0 * (radd8(zh0) + rh0) / n;
+        h[1] = h01;
+        h[MX] = h01;
+        h[MX + 1] = 2.0 * (radd8(zh2) + rh2) / n;
+    }
+};
+
+}  // namespace
+
+// allele/mod.rs:100-126: SQP from (1/2, 1/2) (sqp.rs:49-93 with QP cap 3),
+// then alleles with 0 < p < maf are zeroed and the pair renormalised.
+__global__ __launch_bounds__(64) void af_kernel(const uint8_t* __restrict__ packed, size_t row_bytes,
+                                                 uint32_t n_samples, uint32_t n_sites, Tune tune, double maf,
+                                                 double* __restrict__ af_out, int* __restrict__ status) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_sites) return;
+    AlleleSite site{packed + (s >> 2), row_bytes, (s & 3u) * 2u, n_samples, tune.epsilon};
+    uint32_t called = 0;
+    for (uint32_t i = 0; i < n_samples; i++) called += site.code(i) != 3;
+    if (called == 0) {  // lanevector.rs:73-76 would panic
+        af_out[2 * s] = 0.0;
+        af_out[2 * s + 1] = 0.0;
+        atomicOr(status, 1);
+        return;
+    }
+    double x0[2] = {0.5, 0.5}, x[2];
+    sum_to_one<2>(x0, x);
+    double f = site.obj(x);
+    uint64_t iter = 0;
+    for (; iter < tune.sqp_max_iter; iter++) {
+        double g[2], h[MX * 2];
+        site.gh(x, g, h);
+        if (converged<2>(x, g, tune.sqp_conv_tol)) break;
+        double c[2], y[2];
+        for (int i = 0; i < 2; i++) c[i] = g[i] - dotn(2, h + i * MX, x);
+        solve_qp<2>(h, c, x, tune, y);
+        double p[2] = {y[0] - x[0], y[1] - x[1]};
+        const double t = tune.bls_sufficient_decrease * dotn(2, g, p);
+        double alpha = 1.0, xn[2], fn;
+        uint64_t bi = 0;
+        for (;;) {
+            xn[0] = x[0] + alpha * p[0];
+            xn[1] = x[1] + alpha * p[1];
+            sum_to_one<2>(xn, xn);
+            fn = site.obj(xn);
+            if (fn <= f + alpha * t) break;
+            if (++bi == tune.bls_max_iter) break;
+            alpha *= tune.bls_step_size_reduce;
+        }
+        x[0] = xn[0];
+        x[1] = xn[1];
+        f = fn;
+    }
+    for (int i = 0; i < 2; i++)
+        if (x[i] > 0.0 && x[i] < maf) x[i] = 0.0;
+    double o[2];
+    sum_to_one<2>(x, o);
+    af_out[2 * s] = o[0];
+    af_out[2 * s + 1] = o[1];
+}
+
+// Repack kept sites: dst word w of sample i holds kept sites 16w..16w+15.
+__global__ __launch_bounds__(256) void gather_kernel(const uint8_t* __restrict__ src, size_t src_row_bytes,
+                                                      const uint32_t* __restrict__ kept, uint32_t n_kept,
+                                                      uint32_t n_samples, uint32_t* __restrict__ dst,
+                                                      uint32_t dst_row_words) {
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (size_t)n_samples * dst_row_words) return;
+    const uint32_t i = (uint32_t)(tid / dst_row_words), w = (uint32_t)(tid % dst_row_words);
+    const uint8_t* row = src + (size_t)i * src_row_bytes;
+    uint32_t word = 0;
+    for (int j = 0; j < 16; j++) {
+        const uint32_t t = w * 16 + j;
+        uint32_t c = 3;
+        if (t < n_kept) {
+            const uint32_t s = kept[t];
+            c = (row[s >> 2] >> ((s & 3u) * 2u)) & 3u;
+        }
+        word |= c << (2 * j);
+    }
+    dst[tid] = word;
+}
+
+// ------------------------------------------------------------ the sweep
+
+namespace {
+
+// Valid 4-bit combos (4*gx + gy with gx, gy in {0,1,2}): 0,1,2,4,5,6,8,9,10.
+constexpr uint32_t VALID_COMBOS = 0x0777u;
+
+// Rows of the c_s table for site (p, q): the non-zero block of mixture.rs
+// mixture_row2 for one-hot likelihoods of genotype pair (gx, gy), written in
+// the same expression order, so each row equals what mixture_row2 returns.
+__device__ __forceinline__ void lut_rows(double p, double q, int part, double* site_lut) {
+    const double p2 = p * p, p3 = p2 * p, q2 = q * q, q3 = q2 * q;
+    const double pq = p * q, p2q = p2 * q, pq2 = p * q2, p2q2 = p2 * q2;
+    const double p3q = p3 * q, pq3 = p * q3;
+    const double twopq = pq + pq, twop2q = p2q + p2q, twop3q = p3q + p3q, twopq2 = pq2 + pq2, twopq3 = pq3 + pq3;
+    double r[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int combo;
+    switch (part) {
+    case 0:  // (0,0)
+        combo = 0;
+        r[0] = p; r[1] = p2; r[2] = p2; r[3] = p3; r[4] = p2; r[5] = p3; r[6] = p2; r[7] = p3; r[8] = p2 * p2;
+        break;
+    case 1:  // (0,1)
+        combo = 1;
+        r[2] = pq; r[3] = twop2q; r[7] = p2q; r[8] = twop3q;
+        break;
+    case 2:  // (0,2)
+        combo = 2;
+        r[1] = pq; r[3] = pq2; r[5] = p2q; r[8] = p2q2;
+        break;
+    case 3:  // (1,0)
+        combo = 4;
+        r[4] = pq; r[5] = twop2q; r[7] = p2q; r[8] = twop3q;
+        break;
+    case 4:  // (1,1)
+        combo = 5;
+        r[6] = twopq; r[7] = pq; r[8] = twopq * twopq;
+        break;
+    case 5:  // (1,2)
+        combo = 6;
+        r[4] = pq; r[5] = twopq2; r[7] = pq2; r[8] = twopq3;
+        break;
+    case 6:  // (2,0)
+        combo = 8;
+        r[1] = pq; r[3] = p2q; r[5] = pq2; r[8] = p2q2;
+        break;
+    case 7:  // (2,1)
+        combo = 9;
+        r[2] = pq; r[3] = twopq2; r[7] = pq2; r[8] = twopq3;
+        break;
+    default:  // (2,2)
+        combo = 10;
+        r[0] = q; r[1] = q2; r[2] = q2; r[3] = q3; r[4] = q2; r[5] = q3; r[6] = q2; r[7] = q3; r[8] = q2 * q2;
+        break;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) site_lut[combo * 9 + k] = r[k];
+}
+
+}  // namespace
+
+// One evaluation pass.  blockIdx.y = site segment, blockIdx.x * 256 +
+// threadIdx.x = slot (one pair per lane).  Partials (f, g[9], h[45], n) go to
+// partial[(seg * NPART + k) * nslots + slot].
+__global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __restrict__ geno, uint32_t row_words,
+                                                            const double* __restrict__ af,
+                                                            const uint32_t* __restrict__ seg_bounds,
+                                                            const uint32_t* __restrict__ sx,
+                                                            const uint32_t* __restrict__ sy,
+                                                            const double* __restrict__ pt,
+                                                            const int* __restrict__ active, uint32_t nslots,
+                                                            double eps, double* __restrict__ partial) {
+    __shared__ double lut[2][CHUNK * 16 * 9];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t slot = blockIdx.x * SWEEP_TPB + tid;
+    const uint32_t seg = blockIdx.y;
+    const bool live = slot < nslots && active[slot] != 0;
+    if (!__syncthreads_or(live)) return;
+
+    // rows of invalid combos (a missing call on either side) stay zero
+    for (uint32_t i = tid; i < 2 * CHUNK * 16 * 9; i += SWEEP_TPB) (&lut[0][0])[i] = 0.0;
+    __syncthreads();
+
+    double D[9];
+    uint32_t x = 0, y = 0;
+    if (live) {
+        x = sx[slot];
+        y = sy[slot];
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k] = pt[(size_t)k * nslots + slot];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k] = 0.0;
+    }
+    const uint32_t* rowx = geno + (size_t)x * row_words;
+    const uint32_t* rowy = geno + (size_t)y * row_words;
+
+    double F = 0.0, N = 0.0, G[9], H[45];
+#pragma unroll
+    for (int k = 0; k < 9; k++) G[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 45; k++) H[k] = 0.0;
+
+    const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
+    const int lut_site = tid >> 3, lut_part = tid & 7;
+    auto build = [&](uint32_t cs, int b) {
+        const uint32_t s = cs + lut_site;
+        const double p = af[2 * s], q = af[2 * s + 1];
+        double* site_lut = &lut[b][lut_site * 16 * 9];
+        lut_rows(p, q, lut_part, site_lut);
+        if (lut_part == 0) lut_rows(p, q, 8, site_lut);
+    };
+    if (s0 < s1) build(s0, 0);
+    __syncthreads();
+    int b = 0;
+    for (uint32_t cs = s0; cs < s1; cs += CHUNK, b ^= 1) {
+        if (cs + CHUNK < s1) build(cs + CHUNK, b ^ 1);
+        const uint2 wx = *reinterpret_cast<const uint2*>(rowx + cs / 16);
+        const uint2 wy = *reinterpret_cast<const uint2*>(rowy + cs / 16);
+        const double* L0 = &lut[b][0];
+#pragma unroll
+        for (int k = 0; k < CHUNK; k++) {
+            const uint32_t ax = (k < 16 ? wx.x : wx.y) >> (2 * (k & 15));
+            const uint32_t ay = (k < 16 ? wy.x : wy.y) >> (2 * (k & 15));
+            const uint32_t combo = ((ax & 3u) << 2) | (ay & 3u);
+            const double* c = L0 + (k * 16 + combo) * 9;
+            double cv[9];
+#pragma unroll
+            for (int j = 0; j < 9; j++) cv[j] = c[j];
+            const double vd = (double)((VALID_COMBOS >> combo) & 1u);
+            // L = eps + c.D as an fma chain over columns 0..8 (objective.rs:58-63)
+            double L = eps;
+#pragma unroll
+            for (int j = 0; j < 9; j++) L = fma(D[j], cv[j], L);
+            F = fma(vd, sleef_log(L), F);
+            N += vd;
+            const double r = 1.0 / L;  // grad_hess.rs:122
+#pragma unroll
+            for (int j = 0; j < 9; j++) G[j] = fma(cv[j], r, G[j]);
+#pragma unroll
+            for (int j = 0; j < 9; j++) cv[j] = cv[j] * r;
+            int id = 0;
+#pragma unroll
+            for (int a = 0; a < 9; a++)
+#pragma unroll
+                for (int bb = a; bb < 9; bb++, id++) H[id] = fma(cv[a], cv[bb], H[id]);
+        }
+        __syncthreads();
+    }
+    if (live) {
+        double* out = partial + (size_t)seg * NPART * nslots + slot;
+        out[0] = F;
+#pragma unroll
+        for (int k = 0; k < 9; k++) out[(size_t)(1 + k) * nslots] = G[k];
+#pragma unroll
+        for (int k = 0; k < 45; k++) out[(size_t)(10 + k) * nslots] = H[k];
+        out[(size_t)55 * nslots] = N;
+    }
+}
+
+// ------------------------------------------------------------ SQP step
+
+namespace {
+
+__device__ __forceinline__ void fold(const double* __restrict__ partial, uint32_t nseg, uint32_t nslots,
+                                     uint32_t slot, double* acc /* NPART */) {
+    for (int k = 0; k < NPART; k++) acc[k] = partial[(size_t)k * nslots + slot];
+    for (uint32_t s = 1; s < nseg; s++) {
+        const double* p = partial + (size_t)s * NPART * nslots + slot;
+        for (int k = 0; k < NPART; k++) acc[k] = acc[k] + p[(size_t)k * nslots];
+    }
+}
+
+__device__ __forceinline__ void write_point(double* pt, uint32_t nslots, uint32_t slot, const double* v) {
+    for (int k = 0; k < 9; k++) pt[(size_t)k * nslots + slot] = v[k];
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(64) void step_kernel(SlotState* __restrict__ st, uint32_t* __restrict__ sx,
+                                                   uint32_t* __restrict__ sy, double* __restrict__ pt,
+                                                   int* __restrict__ active, uint32_t nslots,
+                                                   const double* __restrict__ partial, uint32_t nseg,
+                                                   const uint32_t* __restrict__ pairs, uint64_t npairs,
+                                                   unsigned long long* __restrict__ next_pair,
+                                                   kestrel_output* __restrict__ out, Tune tune,
+                                                   int* __restrict__ n_live) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= nslots) return;
+    SlotState s = st[slot];
+    if (s.phase == PH_RETIRED) return;
+    bool claim = s.phase == PH_IDLE;
+    if (!claim) {
+        double acc[NPART];
+        fold(partial, nseg, nslots, slot, acc);
+        double e[9];
+        for (int k = 0; k < 9; k++) e[k] = pt[(size_t)k * nslots + slot];
+        const double n = acc[55];
+        kestrel_output o;
+        o.x = sx[slot];
+        o.y = sy[slot];
+        bool done = false;
+        if (n == 0.0) {  // lanevector.rs:133-138 would panic: no usable site
+            const double nan = __longlong_as_double(0x7ff8000000000000LL);
+            for (int k = 0; k < 9; k++) o.jacquard[k] = nan;
+            o.obj = nan;
+            o.iters = 0;
+            o.cond = nan;
+            o.n_sites = 0;
+            o.status = KESTREL_PAIR_NO_SITES;
+            out[s.pair] = o;
+            claim = true;
+        } else {
+            const double fe = -acc[0] / n;
+            double g[9], h[MX * MX];
+            for (int k = 0; k < 9; k++) g[k] = -acc[1 + k] / n;
+            int id = 0;
+            for (int a = 0; a < 9; a++)
+                for (int b = a; b < 9; b++, id++) {
+                    const double v = acc[10 + id] / n;
+                    h[a * MX + b] = v;
+                    h[b * MX + a] = v;
+                }
+            bool accepted;
+            if (s.phase == PH_INIT) {
+                for (int k = 0; k < 9; k++) s.x[k] = e[k];
+                s.f = fe;
+                s.k = 0;
+                accepted = true;
+            } else if (fe <= s.f + s.alpha * s.t || s.bls + 1 == tune.bls_max_iter) {
+                for (int k = 0; k < 9; k++) s.x[k] = e[k];
+                s.f = fe;
+                s.k += 1;
+                accepted = true;
+            } else {
+                s.bls += 1;
+                s.alpha *= tune.bls_step_size_reduce;
+                double xn[9];
+                for (int k = 0; k < 9; k++) xn[k] = s.x[k] + s.alpha * s.p[k];
+                sum_to_one<9>(xn, xn);
+                write_point(pt, nslots, slot, xn);
+                accepted = false;
+            }
+            if (accepted) {
+                if (s.k == tune.sqp_max_iter) {
+                    o.iters = tune.sqp_max_iter;
+                    o.cond = __longlong_as_double(0x7ff8000000000000LL);
+                    done = true;
+                } else if (converged<9>(s.x, g, tune.sqp_conv_tol)) {
+                    o.iters = s.k;
+                    o.cond = condition<9>(s.x, g, h, tune.sqp_conv_tol);
+                    done = true;
+                } else {
+                    double c[9], yq[9];
+                    for (int i = 0; i < 9; i++) c[i] = g[i] - dotn(9, h + i * MX, s.x);
+                    solve_qp<9>(h, c, s.x, tune, yq);
+                    for (int i = 0; i < 9; i++) s.p[i] = yq[i] - s.x[i];
+                    s.t = tune.bls_sufficient_decrease * dotn(9, g, s.p);
+                    s.alpha = 1.0;
+                    s.bls = 0;
+                    double xn[9];
+                    for (int k = 0; k < 9; k++) xn[k] = s.x[k] + s.alpha * s.p[k];
+                    sum_to_one<9>(xn, xn);
+                    write_point(pt, nslots, slot, xn);
+                    s.phase = PH_TRIAL;
+                }
+                if (done) {
+                    for (int k = 0; k < 9; k++) o.jacquard[k] = s.x[k];
+                    o.obj = s.f;
+                    o.n_sites = (uint32_t)n;
+                    o.status = KESTREL_PAIR_OK;
+                    out[s.pair] = o;
+                    claim = true;
+                }
+            }
+        }
+    }
+    if (claim) {
+        const unsigned long long idx = atomicAdd(next_pair, 1ull);
+        if (idx < npairs) {
+            s.pair = (int64_t)idx;
+            const uint32_t px = pairs[2 * idx], py = pairs[2 * idx + 1];
+            sx[slot] = px;
+            sy[slot] = py;
+            // coefficients.rs:88-92 start point, normalised as sqp.rs:54
+            double x0[9];
+            for (int k = 0; k < 9; k++) x0[k] = px == py ? 0.0 : 1.0 / 9.0;
+            if (px == py) {
+                x0[0] = 0.5;
+                x0[6] = 0.5;
+            }
+            sum_to_one<9>(x0, x0);
+            write_point(pt, nslots, slot, x0);
+            s.phase = PH_INIT;
+            s.k = 0;
+            s.bls = 0;
+        } else {
+            s.phase = PH_RETIRED;
+            s.pair = -1;
+            atomicSub(n_live, 1);
+        }
+    }
+    active[slot] = (s.phase == PH_INIT || s.phase == PH_TRIAL) ? 1 : 0;
+    st[slot] = s;
+}
+
+// Explicit evaluation points: fold partials into f, g, H (objective.rs:34-39,
+// grad_hess.rs:20-38 normalisation).
+__global__ __launch_bounds__(64) void finalize_eval(const double* __restrict__ partial, uint32_t nseg,
+                                                     uint32_t nslots, uint32_t npts, double* __restrict__ f,
+                                                     double* __restrict__ g, double* __restrict__ h,
+                                                     uint32_t* __restrict__ nsites) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    double acc[NPART];
+    fold(partial, nseg, nslots, i, acc);
+    const double n = acc[55];
+    f[i] = -acc[0] / n;
+    for (int k = 0; k < 9; k++) g[(size_t)i * 9 + k] = -acc[1 + k] / n;
+    int id = 0;
+    for (int a = 0; a < 9; a++)
+        for (int b = a; b < 9; b++, id++) {
+            const double v = acc[10 + id] / n;
+            h[(size_t)i * 81 + a * 9 + b] = v;
+            h[(size_t)i * 81 + b * 9 + a] = v;
+        }
+    nsites[i] = (uint32_t)n;
+}
+
+// ------------------------------------------------------------ launchers
+
+namespace klaunch {
+
+hipError_t af(const uint8_t* packed, size_t row_bytes, uint32_t n_samples, uint32_t n_sites, const Tune& t,
+              double maf, double* af_out, int* status, hipStream_t st) {
+    if (n_sites == 0) return hipSuccess;
+    hipLaunchKernelGGL(af_kernel, dim3((n_sites + 63) / 64), dim3(64), 0, st, packed, row_bytes, n_samples, n_sites,
+                       t, maf, af_out, status);
+    return hipGetLastError();
+}
+
+hipError_t gather(const uint8_t* src, size_t src_row_bytes, const uint32_t* kept, uint32_t n_kept,
+                  uint32_t n_samples, uint32_t* dst, uint32_t dst_row_words, hipStream_t st) {
+    const size_t n = (size_t)n_samples * dst_row_words;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, src_row_bytes, kept,
+                       n_kept, n_samples, dst, dst_row_words);
+    return hipGetLastError();
+}
+
+hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, const uint32_t* seg_bounds,
+                 uint32_t nseg, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
+                 uint32_t nslots, double eps, double* partial, hipStream_t st) {
+    dim3 grid((nslots + SWEEP_TPB - 1) / SWEEP_TPB, nseg);
+    hipLaunchKernelGGL(sweep_fp64, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, af, seg_bounds, sx, sy, pt, active,
+                       nslots, eps, partial);
+    return hipGetLastError();
+}
+
+hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* active, uint32_t nslots,
+                const double* partial, uint32_t nseg, const uint32_t* pairs, uint64_t npairs,
+                unsigned long long* next_pair, kestrel_output* out, const Tune& t, int* n_live, hipStream_t st) {
+    hipLaunchKernelGGL(step_kernel, dim3((nslots + 63) / 64), dim3(64), 0, st, stt, sx, sy, pt, active, nslots,
+                       partial, nseg, pairs, npairs, next_pair, out, t, n_live);
+    return hipGetLastError();
+}
+
+hipError_t finalize(const double* partial, uint32_t nseg, uint32_t nslots, uint32_t npts, double* f, double* g,
+                    double* h, uint32_t* nsites, hipStream_t st) {
+    hipLaunchKernelGGL(finalize_eval, dim3((npts + 63) / 64), dim3(64), 0, st, partial, nseg, nslots, npts, f, g, h,
+                       nsites);
+    return hipGetLastError();
+}
+
+}  // namespace klaunch

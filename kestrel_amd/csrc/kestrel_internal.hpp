@@ -1,0 +1,45 @@
+// kestrel_internal.hpp -- shared device/host definitions of libkestrel_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/kestrel_amd.h"
+#include "sqp_device.hpp"
+
+constexpr int CHUNK = 32;       // sites per LDS table chunk
+constexpr int SWEEP_TPB = 256;  // threads (pairs) per sweep block
+constexpr int NPART = 56;       // f, g[9], h[45], n per segment partial
+constexpr uint32_t SITE_PAD = 256;  // rows padded to a multiple of this many sites
+
+enum : uint32_t { PH_IDLE = 0, PH_INIT = 1, PH_TRIAL = 2, PH_RETIRED = 3 };
+
+// Per-slot SQP state between sweeps.
+struct SlotState {
+    int64_t pair;    // index into the pair list, -1 when none
+    uint32_t phase;  // PH_*
+    uint32_t k;      // completed SQP iterations
+    uint32_t bls;    // rejected trials of the current line search
+    uint32_t pad;
+    double x[9];     // current iterate
+    double f;        // objective at x
+    double p[9];     // search direction
+    double t;        // sufficient-decrease slope c1 * g.p
+    double alpha;    // current step
+};
+
+namespace klaunch {
+hipError_t af(const uint8_t* packed, size_t row_bytes, uint32_t n_samples, uint32_t n_sites, const kdev::Tune& t,
+              double maf, double* af_out, int* status, hipStream_t st);
+hipError_t gather(const uint8_t* src, size_t src_row_bytes, const uint32_t* kept, uint32_t n_kept,
+                  uint32_t n_samples, uint32_t* dst, uint32_t dst_row_words, hipStream_t st);
+hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, const uint32_t* seg_bounds,
+                 uint32_t nseg, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
+                 uint32_t nslots, double eps, double* partial, hipStream_t st);
+hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* active, uint32_t nslots,
+                const double* partial, uint32_t nseg, const uint32_t* pairs, uint64_t npairs,
+                unsigned long long* next_pair, kestrel_output* out, const kdev::Tune& t, int* n_live,
+                hipStream_t st);
+hipError_t finalize(const double* partial, uint32_t nseg, uint32_t nslots, uint32_t npts, double* f, double* g,
+                    double* h, uint32_t* nsites, hipStream_t st);
+}  // namespace klaunch

@@ -1,0 +1,160 @@
+"""Seeded synthetic cohorts of biallelic hard calls.
+
+Genotype codes follow the unphased VCF index of the reference
+(coefficients.rs:18-38): 0 = 0/0, 1 = 0/1, 2 = 1/1, and 3 marks a missing
+call (vcf.rs:355,397 turn missing data into NaN, which drops the site for
+every pair that includes the sample, lanevector.rs:100-104).
+
+Allele frequencies p_s (of the REF allele) are Beta(0.5, 0.5) clamped to
+[maf, 1 - maf]; founders are drawn under Hardy-Weinberg, a fraction of them
+inbred (both alleles one identical-by-descent draw with probability F per
+site); relatives are made by gene-dropping with independent sites into
+parent-offspring, full-sib, half-sib and first-cousin pairs.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+REL_TYPES = ("PO", "FS", "HS", "FC")
+
+
+@dataclasses.dataclass
+class Cohort:
+    codes: np.ndarray  # (N, V) uint8, sample-major
+    p: np.ndarray  # (V,) generating REF-allele frequency
+    pairs: list  # [(x, y, relationship)] designated relative pairs
+    inbred: np.ndarray  # (N,) bool, founder drawn with F > 0
+    names: list
+
+    @property
+    def n_samples(self) -> int:
+        return self.codes.shape[0]
+
+    @property
+    def n_sites(self) -> int:
+        return self.codes.shape[1]
+
+
+def _founder(rng, q, f_inbred):
+    h0 = (rng.random(q.shape[0]) < q).astype(np.uint8)
+    h1 = (rng.random(q.shape[0]) < q).astype(np.uint8)
+    if f_inbred > 0:
+        ibd = rng.random(q.shape[0]) < f_inbred
+        h1 = np.where(ibd, h0, h1)
+    return np.stack([h0, h1])
+
+
+def _child(rng, mother, father):
+    v = mother.shape[1]
+    pick_m = rng.integers(0, 2, v, dtype=np.uint8)
+    pick_f = rng.integers(0, 2, v, dtype=np.uint8)
+    idx = np.arange(v)
+    return np.stack([mother[pick_m, idx], father[pick_f, idx]])
+
+
+def generate(n_samples: int, n_sites: int, seed: int, maf: float = 0.01, family_fraction: float = 0.2,
+             inbred_fraction: float = 0.02, f_inbred: float = 0.125, missing: float = 0.01,
+             missing_per_sample_max: float | None = None, n_family_pairs: int | None = None) -> Cohort:
+    """Generate a cohort.
+
+    n_family_pairs fixes the number of designated relative pairs (cycled over
+    PO/FS/HS/FC); otherwise pairs are added until family_fraction of the
+    samples belong to one.  missing_per_sample_max switches missingness from a
+    fixed MCAR rate to a per-sample rate ~ Uniform(0, max).
+    """
+    rng = np.random.default_rng(seed)
+    p = rng.beta(0.5, 0.5, n_sites)
+    p = np.clip(p, maf, 1.0 - maf)
+    q = 1.0 - p
+
+    haps = []  # list of (2, V) haplotype arrays of output samples
+    inbred = []
+    pairs = []
+
+    def founder():
+        inb = rng.random() < inbred_fraction
+        return _founder(rng, q, f_inbred if inb else 0.0), inb
+
+    def add(h, inb):
+        haps.append(h)
+        inbred.append(inb)
+        return len(haps) - 1
+
+    target = n_family_pairs
+    if target is None:
+        # each designated pair puts two output samples in a family
+        target = int(round(family_fraction * n_samples / 2))
+    k = 0
+    while len(pairs) < target and len(haps) + 2 <= n_samples:
+        kind = REL_TYPES[k % 4]
+        k += 1
+        if kind == "PO":
+            (a, ia), (b, _) = founder(), founder()
+            c = _child(rng, a, b)
+            x, y = add(a, ia), add(c, False)
+        elif kind == "FS":
+            (a, _), (b, _) = founder(), founder()
+            x, y = add(_child(rng, a, b), False), add(_child(rng, a, b), False)
+        elif kind == "HS":
+            (a, _), (b, _), (e, _) = founder(), founder(), founder()
+            x, y = add(_child(rng, a, b), False), add(_child(rng, a, e), False)
+        else:  # FC: children of two full sibs
+            (g1, _), (g2, _) = founder(), founder()
+            s1, s2 = _child(rng, g1, g2), _child(rng, g1, g2)
+            (m1, _), (m2, _) = founder(), founder()
+            x, y = add(_child(rng, s1, m1), False), add(_child(rng, s2, m2), False)
+        pairs.append((x, y, kind))
+    while len(haps) < n_samples:
+        h, inb = founder()
+        add(h, inb)
+
+    codes = np.empty((n_samples, n_sites), dtype=np.uint8)
+    for i, h in enumerate(haps):
+        codes[i] = h[0] + h[1]
+    if missing_per_sample_max is not None:
+        rates = rng.uniform(0.0, missing_per_sample_max, n_samples)
+    else:
+        rates = np.full(n_samples, missing)
+    for i in range(n_samples):
+        if rates[i] > 0:
+            codes[i, rng.random(n_sites) < rates[i]] = 3
+    names = [f"S{i:05d}" for i in range(n_samples)]
+    return Cohort(codes=codes, p=p, pairs=pairs, inbred=np.array(inbred), names=names)
+
+
+# Configurations of the benchmark workloads (sizes and generator settings).
+CONFIGS = {
+    "c1": dict(n_samples=64, n_sites=10_000, seed=0xC1, maf=0.01, missing=0.01, inbred_fraction=0.0,
+               n_family_pairs=16),
+    "c2": dict(n_samples=1000, n_sites=100_000, seed=0xC2, maf=0.01, missing=0.01),
+    "c3": dict(n_samples=4000, n_sites=50_000, seed=0xC3, maf=0.01, missing_per_sample_max=0.3),
+    "c4": dict(n_samples=2000, n_sites=100_000, seed=0xC4, maf=0.005, missing=0.01),
+    "c5": dict(n_samples=10_000, n_sites=100_000, seed=0xC5, maf=0.01, missing=0.01),
+}
+
+
+def config(name: str, **overrides) -> Cohort:
+    kw = dict(CONFIGS[name])
+    kw.update(overrides)
+    return generate(**kw)
+
+
+def pack2(codes: np.ndarray, row_sites: int | None = None) -> np.ndarray:
+    """Pack codes (N, V) into 2 bits per call, 4 calls per byte, site s of a
+    row in bits 2*(s%4) of byte s//4.  Rows are padded with code 3 (missing)
+    to row_sites sites (default: V rounded up to a multiple of 256)."""
+    n, v = codes.shape
+    if row_sites is None:
+        row_sites = (v + 255) // 256 * 256
+    padded = np.full((n, row_sites), 3, dtype=np.uint8)
+    padded[:, :v] = codes
+    r = padded.reshape(n, row_sites // 4, 4).astype(np.uint8)
+    return (r[..., 0] | (r[..., 1] << 2) | (r[..., 2] << 4) | (r[..., 3] << 6)).astype(np.uint8)
+
+
+def unpack2(packed: np.ndarray, n_sites: int) -> np.ndarray:
+    b = packed.astype(np.uint8)
+    out = np.stack([b & 3, (b >> 2) & 3, (b >> 4) & 3, (b >> 6) & 3], axis=-1).reshape(b.shape[0], -1)
+    return out[:, :n_sites].copy()

@@ -1,0 +1,127 @@
+/*
+ * kestrel_oracle.h -- CPU FP64 restatement of kestrel's pairwise Jacquard
+ * maximum-likelihood path.  TEST INFRASTRUCTURE ONLY: this library is the
+ * checker that tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * compare against.  The product (kestrel_amd/) never links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the kestrel 0.1.2 source tree).  Two summation orders are provided for the
+ * per-site reductions:
+ *   KO_ORDER_REF : the reference's AVX-512 order -- 8 interleaved lanes over
+ *                  the compacted (NaN-filtered) rows, 32-block tiles for
+ *                  grad/Hess, scalar remainder, _mm512_reduce_add_pd tree.
+ *   KO_ORDER_SEG : site-segmented sequential order -- each segment of sites is
+ *                  summed sequentially (masked sites add exact zeros) and the
+ *                  segment partials are folded left to right.  This is the
+ *                  order the device kernels use, so device results can be
+ *                  compared bit for bit.
+ */
+#ifndef KESTREL_ORACLE_H
+#define KESTREL_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { KO_ORDER_REF = 0, KO_ORDER_SEG = 1 };
+
+/* sqp.rs:16-47 Tuneables */
+typedef struct {
+    uint64_t sqp_max_iter;
+    double sqp_conv_tol;
+    uint64_t qp_max_iter;
+    double qp_conv_tol;
+    double qp_zero_search_tol;
+    uint64_t bls_max_iter;
+    double bls_sufficient_decrease;
+    double bls_step_size_reduce;
+    double epsilon;
+} ko_tune;
+
+/* coefficients.rs:143-151 Output, plus the usable-site count n_xy */
+typedef struct {
+    uint32_t x, y;
+    double jacquard[9];
+    double obj;
+    uint64_t iters;
+    double cond;
+    uint32_t n_sites;
+    uint32_t pad;
+} ko_output;
+
+void ko_tune_new(ko_tune* t, uint64_t sqp_max_iter, uint64_t qp_max_iter);
+
+/* log/scalar.rs:17-70 and log/avx512.rs:15-79 */
+double ko_log(double d);
+double ko_log_getmant(double d);
+
+/* mixture.rs:96-236 (one site, arbitrary likelihood rows) */
+void ko_mixture_row2(const double lx[3], const double ly[3], const double af[2], double out[9]);
+/* iis.rs:94-104 + conditional.rs:10-50 : IIS mode (1..15) of genotype pair */
+int ko_iis_mode(int i, int j, int k, int l);
+/* iis.rs:106-277 accumulate_iis over conditional.rs IIS_2 (the generic path) */
+void ko_mixture_generic2(const double lx[3], const double ly[3], const double af[2], double out[9]);
+/* iis.rs:279-462 conditional_probabilities */
+void ko_conditional_probabilities(double pi, double pj, double pk, double pl, int mode, double out[9]);
+
+/* jacquard/objective.rs:8-52 ; rows are compacted (n x 9, row-major) */
+double ko_jacq_obj_ref(const double* rows, size_t n, const double x[9], double eps);
+/* jacquard/grad_hess.rs:12-449 */
+void ko_jacq_gh_ref(const double* rows, size_t n, const double x[9], double eps, double g[9], double h[81]);
+
+/* segmented order: rows are per site (V x 9), valid[s] != 0 marks usable sites,
+ * seg has nseg+1 site boundaries. */
+double ko_jacq_obj_seg(const double* rows, const uint8_t* valid, size_t V, const uint32_t* seg,
+                       size_t nseg, const double x[9], double eps);
+void ko_jacq_gh_seg(const double* rows, const uint8_t* valid, size_t V, const uint32_t* seg, size_t nseg,
+                    const double x[9], double eps, double g[9], double h[81]);
+
+/* cholesky.rs:6-65, 72-102, 105-140 (matrices are 9x9 row-major, first n used) */
+void ko_modify_gmw(int n, const double* a, double* l, double* d);
+void ko_unmodified(int n, const double* a, double* l, double* d);
+void ko_solve_ldl(int n, const double* l, const double* d, double* b);
+
+/* sqp.rs:111-250 ; N-variable (N <= 9), matrices N x N row-major */
+uint64_t ko_solve_qp(int N, const double* q, const double* c, const double* y0, int modify, const ko_tune* t,
+                     double* y);
+/* sqp.rs:312-391 */
+double ko_condition(int N, const double* x, const double* g, const double* h, double tol);
+
+/* Full SQP of one pair (sqp.rs:49-93 with coefficients.rs:87-132).
+ * order = KO_ORDER_REF : rows compacted n x 9
+ * order = KO_ORDER_SEG : rows V x 9 with valid / seg */
+void ko_fit_pair_rows(int order, const double* rows, size_t n_or_V, const uint8_t* valid, const uint32_t* seg,
+                      size_t nseg, int self_pair, const ko_tune* t, ko_output* out);
+
+/* Build per-site c rows (V x 9) for a pair from 2-bit genotype codes (0,1,2 =
+ * index of the unphased genotype, coefficients.rs:18-38; 3 = missing) using
+ * mixture.rs:96-236 on one-hot likelihoods.  valid[s] = both called.
+ * Returns n_xy. */
+size_t ko_pair_rows_codes(const uint8_t* gx, const uint8_t* gy, const double* af, size_t V, double* rows,
+                          uint8_t* valid);
+
+/* allele/mod.rs:78-134 for biallelic sites from codes (one-hot GL rows, NaN
+ * for missing).  codes is sample-major (N x V).  af_out is V x 2.
+ * Returns 0, or -1 if some site has no called sample (lanevector.rs:73-76). */
+int ko_allele_freqs_codes(const uint8_t* codes, uint32_t N, uint32_t V, uint64_t iterations, double maf,
+                          int nthreads, double* af_out);
+/* one site, arbitrary likelihood rows (n x 3, NaN rows skipped) */
+int ko_allele_freq2_rows(const double* gl, size_t n, uint64_t iterations, double maf, double af[2],
+                         uint64_t* iters);
+
+/* Pair fits over a cohort of codes (N x V sample-major), af (V x 2).
+ * pairs: npairs (x,y) entries, or NULL for the upper triangle incl. diagonal of
+ * the first nsel samples, row-major (coefficients.rs:57,87).
+ * seg/nseg describe site segments for KO_ORDER_SEG (ignored for REF).
+ * Threads take pairs with an atomic counter. */
+int ko_fit_pairs_codes(int order, const uint8_t* codes, uint32_t N, uint32_t V, const double* af,
+                       const uint32_t* pairs, size_t npairs, uint32_t nsel, const uint32_t* seg, size_t nseg,
+                       uint64_t iterations, int nthreads, ko_output* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,286 @@
+"""ctypes binding of the oracle library (oracle/_build/libkestrel_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libkestrel_oracle.so")
+
+ORDER_REF = 0
+ORDER_SEG = 1
+
+
+class Tune(ctypes.Structure):
+    _fields_ = [
+        ("sqp_max_iter", ctypes.c_uint64),
+        ("sqp_conv_tol", ctypes.c_double),
+        ("qp_max_iter", ctypes.c_uint64),
+        ("qp_conv_tol", ctypes.c_double),
+        ("qp_zero_search_tol", ctypes.c_double),
+        ("bls_max_iter", ctypes.c_uint64),
+        ("bls_sufficient_decrease", ctypes.c_double),
+        ("bls_step_size_reduce", ctypes.c_double),
+        ("epsilon", ctypes.c_double),
+    ]
+
+
+OUTPUT_DTYPE = np.dtype([
+    ("x", np.uint32), ("y", np.uint32), ("jacquard", np.float64, 9), ("obj", np.float64),
+    ("iters", np.uint64), ("cond", np.float64), ("n_sites", np.uint32), ("pad", np.uint32),
+])
+assert OUTPUT_DTYPE.itemsize == 112
+
+_lib = None
+
+D = ctypes.c_double
+P = ctypes.c_void_p
+SZ = ctypes.c_size_t
+U32 = ctypes.c_uint32
+U64 = ctypes.c_uint64
+I = ctypes.c_int
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.ko_log.restype = D
+        L.ko_log.argtypes = [D]
+        L.ko_log_getmant.restype = D
+        L.ko_log_getmant.argtypes = [D]
+        L.ko_isa_name.restype = ctypes.c_char_p
+        L.ko_tune_new.argtypes = [P, U64, U64]
+        L.ko_mixture_row2.argtypes = [P, P, P, P]
+        L.ko_mixture_generic2.argtypes = [P, P, P, P]
+        L.ko_conditional_probabilities.argtypes = [D, D, D, D, I, P]
+        L.ko_iis_mode.argtypes = [I, I, I, I]
+        L.ko_iis_mode.restype = I
+        L.ko_jacq_obj_ref.restype = D
+        L.ko_jacq_obj_ref.argtypes = [P, SZ, P, D]
+        L.ko_jacq_gh_ref.argtypes = [P, SZ, P, D, P, P]
+        L.ko_jacq_obj_seg.restype = D
+        L.ko_jacq_obj_seg.argtypes = [P, P, SZ, P, SZ, P, D]
+        L.ko_jacq_gh_seg.argtypes = [P, P, SZ, P, SZ, P, D, P, P]
+        L.ko_modify_gmw.argtypes = [I, P, P, P]
+        L.ko_unmodified.argtypes = [I, P, P, P]
+        L.ko_solve_ldl.argtypes = [I, P, P, P]
+        L.ko_solve_qp.restype = U64
+        L.ko_solve_qp.argtypes = [I, P, P, P, I, P, P]
+        L.ko_condition.restype = D
+        L.ko_condition.argtypes = [I, P, P, P, D]
+        L.ko_fit_pair_rows.argtypes = [I, P, SZ, P, P, SZ, I, P, P]
+        L.ko_pair_rows_codes.restype = SZ
+        L.ko_pair_rows_codes.argtypes = [P, P, P, SZ, P, P]
+        L.ko_allele_freqs_codes.restype = I
+        L.ko_allele_freqs_codes.argtypes = [P, U32, U32, U64, D, I, P]
+        L.ko_allele_freq2_rows.restype = I
+        L.ko_allele_freq2_rows.argtypes = [P, SZ, U64, D, P, P]
+        L.ko_fit_pairs_codes.restype = I
+        L.ko_fit_pairs_codes.argtypes = [I, P, U32, U32, P, P, SZ, U32, P, SZ, U64, I, P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def tune(sqp_max_iter=100, qp_max_iter=10) -> Tune:
+    t = Tune()
+    lib().ko_tune_new(ctypes.byref(t), sqp_max_iter, qp_max_iter)
+    return t
+
+
+def isa() -> str:
+    return lib().ko_isa_name().decode()
+
+
+def log(d: float) -> float:
+    return lib().ko_log(d)
+
+
+def log_getmant(d: float) -> float:
+    return lib().ko_log_getmant(d)
+
+
+def mixture_row2(lx, ly, af) -> np.ndarray:
+    out = np.zeros(9)
+    lx, ly, af = f64(lx), f64(ly), f64(af)
+    lib().ko_mixture_row2(_p(lx), _p(ly), _p(af), _p(out))
+    return out
+
+
+def mixture_generic2(lx, ly, af) -> np.ndarray:
+    out = np.zeros(9)
+    lx, ly, af = f64(lx), f64(ly), f64(af)
+    lib().ko_mixture_generic2(_p(lx), _p(ly), _p(af), _p(out))
+    return out
+
+
+def conditional_probabilities(pi, pj, pk, pl, mode) -> np.ndarray:
+    out = np.zeros(9)
+    lib().ko_conditional_probabilities(pi, pj, pk, pl, mode, _p(out))
+    return out
+
+
+def iis_mode(i, j, k, l) -> int:
+    return lib().ko_iis_mode(i, j, k, l)
+
+
+def obj_ref(rows, x, eps=1e-8) -> float:
+    rows, x = f64(rows), f64(x)
+    return lib().ko_jacq_obj_ref(_p(rows), rows.shape[0], _p(x), eps)
+
+
+def gh_ref(rows, x, eps=1e-8):
+    rows, x = f64(rows), f64(x)
+    g, h = np.zeros(9), np.zeros((9, 9))
+    lib().ko_jacq_gh_ref(_p(rows), rows.shape[0], _p(x), eps, _p(g), _p(h))
+    return g, h
+
+
+def _seg(seg):
+    return np.ascontiguousarray(seg, dtype=np.uint32)
+
+
+def obj_seg(rows, valid, seg, x, eps=1e-8) -> float:
+    rows, x, valid, seg = f64(rows), f64(x), np.ascontiguousarray(valid, np.uint8), _seg(seg)
+    return lib().ko_jacq_obj_seg(_p(rows), _p(valid), rows.shape[0], _p(seg), len(seg) - 1, _p(x), eps)
+
+
+def gh_seg(rows, valid, seg, x, eps=1e-8):
+    rows, x, valid, seg = f64(rows), f64(x), np.ascontiguousarray(valid, np.uint8), _seg(seg)
+    g, h = np.zeros(9), np.zeros((9, 9))
+    lib().ko_jacq_gh_seg(_p(rows), _p(valid), rows.shape[0], _p(seg), len(seg) - 1, _p(x), eps, _p(g), _p(h))
+    return g, h
+
+
+def modify_gmw(a):
+    a = f64(a)
+    n = a.shape[0]
+    l, d = np.zeros((n, n)), np.zeros(n)
+    lib().ko_modify_gmw(n, _p(a), _p(l), _p(d))
+    return l, d
+
+
+def unmodified(a):
+    a = f64(a)
+    n = a.shape[0]
+    l, d = np.zeros((n, n)), np.zeros(n)
+    lib().ko_unmodified(n, _p(a), _p(l), _p(d))
+    return l, d
+
+
+def solve_ldl(l, d, b):
+    l, d, b = f64(l), f64(d), f64(b).copy()
+    lib().ko_solve_ldl(l.shape[0], _p(l), _p(d), _p(b))
+    return b
+
+
+def solve_qp(q, c, y0, modify=True, t: Tune | None = None):
+    q, c, y0 = f64(q), f64(c), f64(y0)
+    n = q.shape[0]
+    y = np.zeros(n)
+    t = t or tune(100, 10)
+    it = lib().ko_solve_qp(n, _p(q), _p(c), _p(y0), int(modify), ctypes.byref(t), _p(y))
+    return y, it
+
+
+def condition(x, g, h, tol=1e-8) -> float:
+    x, g, h = f64(x), f64(g), f64(h)
+    return lib().ko_condition(len(x), _p(x), _p(g), _p(h), tol)
+
+
+def fit_rows(rows, self_pair=False, order=ORDER_REF, valid=None, seg=None, iterations=100):
+    """Fit one pair from its c rows: compacted (REF) or per-site with valid (SEG)."""
+    rows = f64(rows)
+    out = np.zeros(1, OUTPUT_DTYPE)
+    t = tune(iterations, 10)
+    if order == ORDER_REF:
+        lib().ko_fit_pair_rows(ORDER_REF, _p(rows), rows.shape[0], None, None, 0, int(self_pair),
+                               ctypes.byref(t), _p(out))
+    else:
+        valid = np.ascontiguousarray(valid, np.uint8)
+        seg = _seg(seg)
+        lib().ko_fit_pair_rows(ORDER_SEG, _p(rows), rows.shape[0], _p(valid), _p(seg), len(seg) - 1,
+                               int(self_pair), ctypes.byref(t), _p(out))
+    return out[0]
+
+
+def pair_rows_codes(gx, gy, af):
+    gx, gy = np.ascontiguousarray(gx, np.uint8), np.ascontiguousarray(gy, np.uint8)
+    af = f64(af)
+    v = gx.shape[0]
+    rows = np.zeros((v, 9))
+    valid = np.zeros(v, np.uint8)
+    n = lib().ko_pair_rows_codes(_p(gx), _p(gy), _p(af), v, _p(rows), _p(valid))
+    return rows, valid, n
+
+
+def allele_freqs(codes, iterations=100, maf=None, nthreads=0):
+    """allele/mod.rs:37-134 on one-hot likelihoods of the codes (N, V)."""
+    codes = np.ascontiguousarray(codes, np.uint8)
+    n, v = codes.shape
+    if maf is None:
+        maf = 1.0 / (2.0 * n)  # wrapper.rs:101
+    af = np.zeros((v, 2))
+    rc = lib().ko_allele_freqs_codes(_p(codes), n, v, iterations, maf, nthreads, _p(af))
+    if rc != 0:
+        raise ValueError("a site has no called sample")
+    return af
+
+
+def allele_freq_rows(gl, iterations=100, maf=0.0):
+    gl = f64(gl)
+    af = np.zeros(2)
+    it = np.zeros(1, np.uint64)
+    rc = lib().ko_allele_freq2_rows(_p(gl), gl.shape[0], iterations, maf, _p(af), _p(it))
+    if rc != 0:
+        raise ValueError("no called sample")
+    return af, int(it[0])
+
+
+def keep_sites(af) -> np.ndarray:
+    """wrapper.rs:352-410: a biallelic site survives iff both frequencies are > 0."""
+    return (af[:, 0] > 0.0) & (af[:, 1] > 0.0)
+
+
+def fit_pairs(codes, af, pairs=None, nsel=None, order=ORDER_REF, seg=None, iterations=100, nthreads=0):
+    """Pair fits over kept sites: codes (N, V) and af (V, 2) restricted to kept sites."""
+    codes = np.ascontiguousarray(codes, np.uint8)
+    af = f64(af)
+    n, v = codes.shape
+    if nsel is None:
+        nsel = n
+    if pairs is not None:
+        pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        npairs = pairs.shape[0]
+    else:
+        npairs = nsel * (nsel + 1) // 2
+    out = np.zeros(npairs, OUTPUT_DTYPE)
+    if seg is None:
+        seg = np.array([0, v], np.uint32)
+    seg = _seg(seg)
+    rc = lib().ko_fit_pairs_codes(order, _p(codes), n, v, _p(af), _p(pairs), npairs, nsel, _p(seg), len(seg) - 1,
+                                  iterations, nthreads, _p(out))
+    if rc != 0:
+        raise ValueError("requested samples exceed total samples")
+    return out
