@@ -1,0 +1,153 @@
+"""Benchmark: pair-fits/sec of the pairwise Jacquard maximum-likelihood fit on
+one MI355X.
+
+Workload (default): 1,000 samples x 100,000 biallelic sites (seeded synthetic
+cohort, 20% of samples in relative pairs, 2% inbred founders, 1% missing),
+every pair x <= y fitted in FP64 (500,500 fits; the metric counts the 499,500
+off-diagonal ones).  One step = one full fit from the device-resident 2-bit
+cohort + allele frequencies to Output records in host memory.
+
+Prints one JSON line with the throughput, the roofline of the dominant kernel
+(the sweep: fused objective + gradient + Hessian passes, VALU FP64 bound) and a
+CPU baseline (the oracle's restatement of the reference CPU path, timed on a
+bounded sample of the same workload on this host's cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_VALU_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
+FLOP_PER_SITE_EVAL = 156  # 136 (gradient + Hessian) + 20 (objective, log counted as 1)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_flops(out) -> float:
+    """W_pair = 156 * n_xy * (iterations + 1); at the iteration cap iterations + 1 = cap."""
+    it = out["iters"].astype(np.float64)
+    return float((FLOP_PER_SITE_EVAL * out["n_sites"].astype(np.float64) * (it + 1)).sum())
+
+
+def cpu_threads() -> int:
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS")
+    if cap and cap.isdigit():
+        n = min(n, int(cap))
+    return max(1, n)
+
+
+def cpu_baseline(codes, af, budget_s: float):
+    """Oracle in the reference's order (8-lane AVX-512 loops, per-pair c_s buffer,
+    work stealing over pairs) on a seeded random sample of off-diagonal pairs."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    threads = cpu_threads()
+    n = codes.shape[0]
+    rng = np.random.default_rng(12345)
+
+    def sample(m):
+        x = rng.integers(0, n, m)
+        y = rng.integers(0, n, m)
+        y = np.where(y == x, (y + 1) % n, y)
+        return np.stack([np.minimum(x, y), np.maximum(x, y)], 1)
+
+    probe = sample(2 * threads)
+    t = time.perf_counter()
+    O.fit_pairs(codes, af, pairs=probe, order=O.ORDER_REF, nthreads=threads)
+    dt = time.perf_counter() - t
+    m = int(max(2 * threads, min(200_000, budget_s / max(dt, 1e-3) * len(probe))))
+    pairs = sample(m)
+    t = time.perf_counter()
+    O.fit_pairs(codes, af, pairs=pairs, order=O.ORDER_REF, nthreads=threads)
+    dt = time.perf_counter() - t
+    return {"value": m / dt, "unit": "pair-fits/s", "cores": threads, "kind": "port",
+            "isa": O.isa(), "sample": f"{m} random off-diagonal pairs of the same cohort, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work (0 = skip)")
+    ap.add_argument("--segment-sites", type=int, default=None)
+    ap.add_argument("--slots", type=int, default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from kestrel_amd import Cohort, synth
+
+    t = time.perf_counter()
+    c = synth.config(args.config)
+    n, v = c.codes.shape
+    log(f"cohort {args.config}: {n} x {v} generated in {time.perf_counter() - t:.1f}s")
+    t = time.perf_counter()
+    co = Cohort(c.codes)
+    if args.segment_sites:
+        co.set_option("segment_sites", args.segment_sites)
+    if args.slots:
+        co.set_option("slots", args.slots)
+    af, kept = co.allele_freqs()
+    log(f"device cohort + allele frequencies in {time.perf_counter() - t:.2f}s, kept {co.n_kept} sites")
+
+    for i in range(args.warmup):
+        t = time.perf_counter()
+        co.fit_pairs()
+        log(f"warmup {i}: {time.perf_counter() - t:.3f}s")
+    times, flops, sweep_ms, sweeps = [], 0.0, 0.0, 0
+    out = None
+    for i in range(args.steps):
+        t = time.perf_counter()
+        out = co.fit_pairs()
+        dt = time.perf_counter() - t
+        st = co.last_fit_stats()
+        times.append(dt)
+        flops += algorithmic_flops(out)
+        sweep_ms += st["sweep_ms"]
+        sweeps += st["sweeps"]
+        log(f"step {i}: {dt:.3f}s, {st['sweeps']} sweeps, sweep kernel {st['sweep_ms']:.1f} ms")
+    total = float(np.sum(times))
+    offdiag = int((out["x"] != out["y"]).sum())
+    value = offdiag * args.steps / total
+    achieved = flops / (sweep_ms / 1e3) / 1e12
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_VALU_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / FP64_VALU_PEAK_TFLOPS, 4), "traffic": None,
+                "kernel": "sweep_fp64", "avg_launch_ms": round(sweep_ms / sweeps, 4),
+                "algorithmic_flop_per_launch": flops / sweeps}
+    cpu = None
+    if args.cpu_budget > 0:
+        cpu = cpu_baseline(c.codes[:, kept], af[kept], args.cpu_budget)
+        log(f"cpu baseline: {cpu}")
+    line = {
+        "metric": "pair-fits/sec", "value": round(value, 1), "unit": "pair-fits/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(total / args.steps * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.config}: {n} samples x {v} sites ({co.n_kept} segregating), "
+                               f"{offdiag} off-diagonal pairs (+{len(out) - offdiag} self pairs), FP64",
+                   "samples": n, "sites": v, "pairs": offdiag, "precision": "fp64"},
+        "roofline": roofline, "cpu_baseline": cpu,
+        "iterations_mean": round(float(out["iters"].mean()), 3),
+        "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+    }
+    co.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -109,8 +109,9 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
 int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* points, size_t n, double eps,
                      double* f, double* g, double* h, uint32_t* n_sites);
 
-/* Tuning knobs: "segments" (site segments per sweep, 0 = auto), "slots"
- * (pairs in flight, 0 = auto). */
+/* Tuning knobs: "segment_sites" (sites per segment of a sweep, a multiple of
+ * 32, default 4096; fixes the summation order, so it is a cohort property),
+ * "slots" (pairs in flight, 0 = fill the device). */
 int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value);
 
 /* Timing of the last kestrel_fit_pairs: sweep-kernel milliseconds (sum of
@@ -118,6 +119,12 @@ int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value);
 int kestrel_last_fit_stats(const kestrel_cohort* c, double* sweep_ms, uint64_t* sweeps, double* wall_ms);
 
 void kestrel_cohort_destroy(kestrel_cohort* c);
+
+/* Device arithmetic self-test (no counterpart in the reference): compares, on
+ * n hashed inputs, the sweep's correctly rounded reciprocal with IEEE division
+ * and its positive-domain log with the full log; *mismatches = differing bit
+ * patterns (0 expected). */
+int kestrel_selftest(uint64_t n, uint64_t seed, uint64_t* mismatches);
 
 const char* kestrel_last_error(void);
 const char* kestrel_version(void);

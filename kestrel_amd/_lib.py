@@ -56,7 +56,7 @@ assert OUTPUT_DTYPE.itemsize == 112
 EXPORTS = (
     "kestrel_tune_default", "kestrel_cohort_create", "kestrel_cohort_allele_freqs", "kestrel_cohort_n_kept",
     "kestrel_fit_pairs", "kestrel_evaluate", "kestrel_set_option", "kestrel_last_fit_stats",
-    "kestrel_cohort_destroy", "kestrel_last_error", "kestrel_version",
+    "kestrel_cohort_destroy", "kestrel_last_error", "kestrel_version", "kestrel_selftest",
 )
 
 _lib = None
@@ -81,6 +81,7 @@ def load():
     L.kestrel_evaluate.argtypes = [P, P, P, ctypes.c_size_t, ctypes.c_double, P, P, P, P]
     L.kestrel_set_option.argtypes = [P, ctypes.c_char_p, ctypes.c_int64]
     L.kestrel_last_fit_stats.argtypes = [P, P, P, P]
+    L.kestrel_selftest.argtypes = [ctypes.c_uint64, ctypes.c_uint64, P]
     L.kestrel_cohort_destroy.argtypes = [P]
     L.kestrel_cohort_destroy.restype = None
     _lib = L

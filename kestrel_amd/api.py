@@ -113,8 +113,9 @@ def segment_bounds(n_kept: int, segment_sites: int = 4096, chunk: int = 32, pad:
     left to right)."""
     v_pad = max(pad, (n_kept + pad - 1) // pad * pad)
     chunks = v_pad // chunk
-    per = max(chunk, segment_sites) // chunk
-    nseg = max(1, (chunks + per - 1) // per)
+    target = max(chunk, segment_sites) // chunk
+    nseg = max(1, (chunks + target - 1) // target)
+    per = (chunks + nseg - 1) // nseg
     b = [min(chunks, k * per) * chunk for k in range(nseg + 1)]
     b[-1] = v_pad
     return np.array(b, np.uint32)

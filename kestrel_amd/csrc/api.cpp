@@ -89,9 +89,12 @@ struct kestrel_cohort {
     uint64_t last_sweeps = 0;
 
     int set_segments() {
+        // nseg = ceil(chunks / target); segments of equal length (the last
+        // may be shorter) so that every block of a sweep has the same work
         const uint32_t chunks = v_pad / CHUNK;
-        uint32_t per = (uint32_t)std::max<int64_t>(CHUNK, opt_segment_sites) / CHUNK;
-        nseg = std::max<uint32_t>(1, (chunks + per - 1) / per);
+        const uint32_t target = (uint32_t)std::max<int64_t>(CHUNK, opt_segment_sites) / CHUNK;
+        nseg = std::max<uint32_t>(1, (chunks + target - 1) / target);
+        const uint32_t per = (chunks + nseg - 1) / nseg;
         std::vector<uint32_t> b(nseg + 1);
         for (uint32_t k = 0; k <= nseg; k++) b[k] = std::min(chunks, k * per) * CHUNK;
         b[nseg] = v_pad;
@@ -291,8 +294,11 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
 
     // pairs in flight: fill the device (2 blocks of 256 lanes per CU) over
     // nseg segments, never more than the pairs available
-    const uint64_t capacity = (uint64_t)std::max(1, c->n_cu) * 2 * SWEEP_TPB;
-    uint64_t nslots = c->opt_slots > 0 ? (uint64_t)c->opt_slots : std::max<uint64_t>(SWEEP_TPB, capacity / c->nseg);
+    // resident sweep blocks: 2 per CU (LDS 72 KiB and 2 waves/SIMD each); size the
+    // slot count so (slot blocks x segments) fills them in one round
+    const uint64_t resident = (uint64_t)std::max(1, c->n_cu) * 2;
+    const uint64_t slot_blocks = std::max<uint64_t>(1, resident / c->nseg);
+    uint64_t nslots = c->opt_slots > 0 ? (uint64_t)c->opt_slots : slot_blocks * SWEEP_TPB;
     nslots = std::min<uint64_t>(nslots, ((uint64_t)n_pairs + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB);
     nslots = (nslots + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB;
     const uint32_t ns = (uint32_t)nslots;
@@ -413,5 +419,20 @@ int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* poi
 }
 
 void kestrel_cohort_destroy(kestrel_cohort* c) { delete c; }
+
+int kestrel_selftest(uint64_t n, uint64_t seed, uint64_t* mismatches) {
+    if (!mismatches) return fail(KESTREL_EINVAL, "null argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(KESTREL_EDEVICE, "no HIP device: libkestrel_amd needs an MI355X (gfx950) GPU");
+    DevBuf<unsigned long long> bad;
+    HIPCHK(bad.alloc(1));
+    HIPCHK(hipMemset(bad.p, 0, sizeof(unsigned long long)));
+    HIPCHK(klaunch::selftest(n, seed, bad.p, nullptr));
+    unsigned long long h = 0;
+    HIPCHK(hipMemcpy(&h, bad.p, sizeof h, hipMemcpyDeviceToHost));
+    *mismatches = h;
+    return KESTREL_OK;
+}
 
 }  // extern "C"

@@ -210,59 +210,66 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint8_t* __restrict__
 
 namespace {
 
-// Valid 4-bit combos (4*gx + gy with gx, gy in {0,1,2}): 0,1,2,4,5,6,8,9,10.
-constexpr uint32_t VALID_COMBOS = 0x0777u;
+// Table rows: row = min(4*gx + gy, 11) for the 2-bit codes gx, gy of a site;
+// rows 0,1,2,4,5,6,8,9,10 are the nine called genotype pairs, rows 3, 7 and 11
+// (a missing call on either side) stay zero.  A row holds the nine c_s values
+// and, in slot 9, 1.0 marking a usable site.
+constexpr int LUT_ROWS = 12;
+constexpr int LUT_W = 10;
+constexpr int LUT_BUF = LUT_ROWS * CHUNK * LUT_W;  // doubles per buffer
 
 // Rows of the c_s table for site (p, q): the non-zero block of mixture.rs
 // mixture_row2 for one-hot likelihoods of genotype pair (gx, gy), written in
 // the same expression order, so each row equals what mixture_row2 returns.
-__device__ __forceinline__ void lut_rows(double p, double q, int part, double* site_lut) {
+__device__ __forceinline__ void lut_rows(double p, double q, int part, double* buf, int site) {
     const double p2 = p * p, p3 = p2 * p, q2 = q * q, q3 = q2 * q;
     const double pq = p * q, p2q = p2 * q, pq2 = p * q2, p2q2 = p2 * q2;
     const double p3q = p3 * q, pq3 = p * q3;
     const double twopq = pq + pq, twop2q = p2q + p2q, twop3q = p3q + p3q, twopq2 = pq2 + pq2, twopq3 = pq3 + pq3;
     double r[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    int combo;
+    int row;
     switch (part) {
     case 0:  // (0,0)
-        combo = 0;
+        row = 0;
         r[0] = p; r[1] = p2; r[2] = p2; r[3] = p3; r[4] = p2; r[5] = p3; r[6] = p2; r[7] = p3; r[8] = p2 * p2;
         break;
     case 1:  // (0,1)
-        combo = 1;
+        row = 1;
         r[2] = pq; r[3] = twop2q; r[7] = p2q; r[8] = twop3q;
         break;
     case 2:  // (0,2)
-        combo = 2;
+        row = 2;
         r[1] = pq; r[3] = pq2; r[5] = p2q; r[8] = p2q2;
         break;
     case 3:  // (1,0)
-        combo = 4;
+        row = 4;
         r[4] = pq; r[5] = twop2q; r[7] = p2q; r[8] = twop3q;
         break;
     case 4:  // (1,1)
-        combo = 5;
+        row = 5;
         r[6] = twopq; r[7] = pq; r[8] = twopq * twopq;
         break;
     case 5:  // (1,2)
-        combo = 6;
+        row = 6;
         r[4] = pq; r[5] = twopq2; r[7] = pq2; r[8] = twopq3;
         break;
     case 6:  // (2,0)
-        combo = 8;
+        row = 8;
         r[1] = pq; r[3] = p2q; r[5] = pq2; r[8] = p2q2;
         break;
     case 7:  // (2,1)
-        combo = 9;
+        row = 9;
         r[2] = pq; r[3] = twopq2; r[7] = pq2; r[8] = twopq3;
         break;
     default:  // (2,2)
-        combo = 10;
+        row = 10;
         r[0] = q; r[1] = q2; r[2] = q2; r[3] = q3; r[4] = q2; r[5] = q3; r[6] = q2; r[7] = q3; r[8] = q2 * q2;
         break;
     }
+    double* dst = buf + (row * CHUNK + site) * LUT_W;
 #pragma unroll
-    for (int k = 0; k < 9; k++) site_lut[combo * 9 + k] = r[k];
+    for (int k = 0; k < 9; k++) dst[k] = r[k];
+    dst[9] = 1.0;
 }
 
 }  // namespace
@@ -278,15 +285,15 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
                                                             const double* __restrict__ pt,
                                                             const int* __restrict__ active, uint32_t nslots,
                                                             double eps, double* __restrict__ partial) {
-    __shared__ double lut[2][CHUNK * 16 * 9];
+    __shared__ __attribute__((aligned(16))) double lut[2 * LUT_BUF];
     const uint32_t tid = threadIdx.x;
     const uint32_t slot = blockIdx.x * SWEEP_TPB + tid;
     const uint32_t seg = blockIdx.y;
     const bool live = slot < nslots && active[slot] != 0;
     if (!__syncthreads_or(live)) return;
 
-    // rows of invalid combos (a missing call on either side) stay zero
-    for (uint32_t i = tid; i < 2 * CHUNK * 16 * 9; i += SWEEP_TPB) (&lut[0][0])[i] = 0.0;
+    // rows of a missing call on either side stay zero
+    for (uint32_t i = tid; i < 2 * LUT_BUF; i += SWEEP_TPB) lut[i] = 0.0;
     __syncthreads();
 
     double D[9];
@@ -314,9 +321,9 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
     auto build = [&](uint32_t cs, int b) {
         const uint32_t s = cs + lut_site;
         const double p = af[2 * s], q = af[2 * s + 1];
-        double* site_lut = &lut[b][lut_site * 16 * 9];
-        lut_rows(p, q, lut_part, site_lut);
-        if (lut_part == 0) lut_rows(p, q, 8, site_lut);
+        double* buf = lut + b * LUT_BUF;
+        lut_rows(p, q, lut_part, buf, lut_site);
+        if (lut_part == 0) lut_rows(p, q, 8, buf, lut_site);
     };
     if (s0 < s1) build(s0, 0);
     __syncthreads();
@@ -325,24 +332,28 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
         if (cs + CHUNK < s1) build(cs + CHUNK, b ^ 1);
         const uint2 wx = *reinterpret_cast<const uint2*>(rowx + cs / 16);
         const uint2 wy = *reinterpret_cast<const uint2*>(rowy + cs / 16);
-        const double* L0 = &lut[b][0];
+        const double* L0 = lut + b * LUT_BUF;
 #pragma unroll
         for (int k = 0; k < CHUNK; k++) {
             const uint32_t ax = (k < 16 ? wx.x : wx.y) >> (2 * (k & 15));
             const uint32_t ay = (k < 16 ? wy.x : wy.y) >> (2 * (k & 15));
-            const uint32_t combo = ((ax & 3u) << 2) | (ay & 3u);
-            const double* c = L0 + (k * 16 + combo) * 9;
-            double cv[9];
+            const uint32_t row = min(((ax & 3u) << 2) | (ay & 3u), 11u);
+            const double2* c2 = reinterpret_cast<const double2*>(L0 + (row * CHUNK + k) * LUT_W);
+            double cv[10];
 #pragma unroll
-            for (int j = 0; j < 9; j++) cv[j] = c[j];
-            const double vd = (double)((VALID_COMBOS >> combo) & 1u);
+            for (int j = 0; j < 5; j++) {
+                const double2 v = c2[j];
+                cv[2 * j] = v.x;
+                cv[2 * j + 1] = v.y;
+            }
+            const double vd = cv[9];
             // L = eps + c.D as an fma chain over columns 0..8 (objective.rs:58-63)
             double L = eps;
 #pragma unroll
             for (int j = 0; j < 9; j++) L = fma(D[j], cv[j], L);
-            F = fma(vd, sleef_log(L), F);
+            F = fma(vd, sleef_log_pos(L), F);
             N += vd;
-            const double r = 1.0 / L;  // grad_hess.rs:122
+            const double r = recip_rn(L);  // grad_hess.rs:122
 #pragma unroll
             for (int j = 0; j < 9; j++) G[j] = fma(cv[j], r, G[j]);
 #pragma unroll
@@ -420,15 +431,20 @@ __global__ __launch_bounds__(64) void step_kernel(SlotState* __restrict__ st, ui
             claim = true;
         } else {
             const double fe = -acc[0] / n;
-            double g[9], h[MX * MX];
+            double g[9], h[9][9];
+#pragma unroll
             for (int k = 0; k < 9; k++) g[k] = -acc[1 + k] / n;
-            int id = 0;
-            for (int a = 0; a < 9; a++)
-                for (int b = a; b < 9; b++, id++) {
-                    const double v = acc[10 + id] / n;
-                    h[a * MX + b] = v;
-                    h[b * MX + a] = v;
-                }
+            {
+                int id = 0;
+#pragma unroll
+                for (int a = 0; a < 9; a++)
+#pragma unroll
+                    for (int b = a; b < 9; b++, id++) {
+                        const double v = acc[10 + id] / n;
+                        h[a][b] = v;
+                        h[b][a] = v;
+                    }
+            }
             bool accepted;
             if (s.phase == PH_INIT) {
                 for (int k = 0; k < 9; k++) s.x[k] = e[k];
@@ -450,20 +466,34 @@ __global__ __launch_bounds__(64) void step_kernel(SlotState* __restrict__ st, ui
                 accepted = false;
             }
             if (accepted) {
+                double xr[9];
+#pragma unroll
+                for (int k = 0; k < 9; k++) xr[k] = s.x[k];
                 if (s.k == tune.sqp_max_iter) {
                     o.iters = tune.sqp_max_iter;
                     o.cond = __longlong_as_double(0x7ff8000000000000LL);
                     done = true;
-                } else if (converged<9>(s.x, g, tune.sqp_conv_tol)) {
+                } else if (converged<9>(xr, g, tune.sqp_conv_tol)) {
                     o.iters = s.k;
-                    o.cond = condition<9>(s.x, g, h, tune.sqp_conv_tol);
+                    o.cond = condition_reg<9>(xr, g, h, tune.sqp_conv_tol);
                     done = true;
                 } else {
                     double c[9], yq[9];
-                    for (int i = 0; i < 9; i++) c[i] = g[i] - dotn(9, h + i * MX, s.x);
-                    solve_qp<9>(h, c, s.x, tune, yq);
-                    for (int i = 0; i < 9; i++) s.p[i] = yq[i] - s.x[i];
-                    s.t = tune.bls_sufficient_decrease * dotn(9, g, s.p);
+#pragma unroll
+                    for (int i = 0; i < 9; i++) {
+                        double hx = 0.0;
+#pragma unroll
+                        for (int j = 0; j < 9; j++) hx = fma(h[i][j], xr[j], hx);
+                        c[i] = g[i] - hx;
+                    }
+                    solve_qp_reg<9>(h, c, xr, tune, yq);
+                    double gp = 0.0;
+#pragma unroll
+                    for (int i = 0; i < 9; i++) {
+                        s.p[i] = yq[i] - xr[i];
+                        gp = fma(g[i], s.p[i], gp);
+                    }
+                    s.t = tune.bls_sufficient_decrease * gp;
                     s.alpha = 1.0;
                     s.bls = 0;
                     double xn[9];
@@ -535,6 +565,27 @@ __global__ __launch_bounds__(64) void finalize_eval(const double* __restrict__ p
     nsites[i] = (uint32_t)n;
 }
 
+// Self-test of the sweep's arithmetic shortcuts: recip_rn against IEEE
+// division and sleef_log_pos against the full sleef_log, on n inputs drawn
+// from a counter-based hash (log-uniform over [1e-9, 1e4], and [1.75, 2.5),
+// the range of 1 + m inside the log).
+__global__ __launch_bounds__(256) void selftest_kernel(uint64_t n, uint64_t seed,
+                                                        unsigned long long* __restrict__ bad) {
+    unsigned long long local = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t h = (i + 1) * 0x9E3779B97F4A7C15ull ^ seed;
+        h ^= h >> 31;
+        h *= 0xBF58476D1CE4E5B9ull;
+        h ^= h >> 29;
+        const double u = (double)(h >> 11) * (1.0 / 9007199254740992.0);
+        const double x = (i & 1) ? exp2(-29.9 + u * 43.2) : 1.75 + 0.75 * u;
+        if (__double_as_longlong(recip_rn(x)) != __double_as_longlong(1.0 / x)) local++;
+        if (__double_as_longlong(sleef_log_pos(x)) != __double_as_longlong(sleef_log(x))) local++;
+    }
+    if (local) atomicAdd(bad, local);
+}
+
 // ------------------------------------------------------------ launchers
 
 namespace klaunch {
@@ -570,6 +621,11 @@ hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* act
                 unsigned long long* next_pair, kestrel_output* out, const Tune& t, int* n_live, hipStream_t st) {
     hipLaunchKernelGGL(step_kernel, dim3((nslots + 63) / 64), dim3(64), 0, st, stt, sx, sy, pt, active, nslots,
                        partial, nseg, pairs, npairs, next_pair, out, t, n_live);
+    return hipGetLastError();
+}
+
+hipError_t selftest(uint64_t n, uint64_t seed, unsigned long long* bad, hipStream_t st) {
+    hipLaunchKernelGGL(selftest_kernel, dim3(4096), dim3(256), 0, st, n, seed, bad);
     return hipGetLastError();
 }
 

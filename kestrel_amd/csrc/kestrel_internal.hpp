@@ -40,6 +40,7 @@ hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* act
                 const double* partial, uint32_t nseg, const uint32_t* pairs, uint64_t npairs,
                 unsigned long long* next_pair, kestrel_output* out, const kdev::Tune& t, int* n_live,
                 hipStream_t st);
+hipError_t selftest(uint64_t n, uint64_t seed, unsigned long long* bad, hipStream_t st);
 hipError_t finalize(const double* partial, uint32_t nseg, uint32_t nslots, uint32_t npts, double* f, double* g,
                     double* h, uint32_t* nsites, hipStream_t st);
 }  // namespace klaunch

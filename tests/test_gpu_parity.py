@@ -1,0 +1,261 @@
+"""GPU parity: libkestrel_amd (gfx950 kernels, called through the C ABI)
+against the oracle.
+
+Bars:
+  * allele frequencies, kept-site masks, n_xy, iterations: bit-exact;
+  * objective / gradient / Hessian and the whole fit: bit-exact against the
+    oracle evaluated in the device's segmented summation order;
+  * against the reference's own 8-lane order: max |dDelta_k| <= 1e-6,
+    |d kinship| <= 1e-6, relative objective <= 1e-9, log10 condition within
+    0.1, on pairs with log10(condition) < 12 (the reference's order differs
+    only in floating-point summation order).
+"""
+import numpy as np
+import pytest
+
+from kestrel_amd import Cohort, KestrelError, build_arrays, kinship, segment_bounds, synth
+
+pytestmark = pytest.mark.gpu
+
+TOL_DELTA = 1e-6
+TOL_KIN = 1e-6
+TOL_OBJ_REL = 1e-9
+TOL_LOG10_COND = 0.1
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def same(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return (a == b) | (np.isnan(a) & np.isnan(b))
+
+
+def seg_for(n_kept, V, segment_sites=4096):
+    return np.minimum(segment_bounds(n_kept, segment_sites), V).astype(np.uint32)
+
+
+def assert_bitexact(out, exp):
+    for k in ("x", "y", "jacquard", "obj", "iters", "n_sites", "cond"):
+        ok = same(out[k], exp[k]).reshape(len(out), -1).all(1)
+        assert ok.all(), f"{k}: {int((~ok).sum())} of {len(out)} pairs differ"
+
+
+def assert_within_tolerance(out, ref):
+    lc = np.log10(ref["cond"])
+    well = np.isfinite(lc) & (lc < 12)
+    d = np.abs(out["jacquard"] - ref["jacquard"]).max(1)
+    assert d[well].max() <= TOL_DELTA
+    assert np.abs(kinship(out["jacquard"]) - kinship(ref["jacquard"]))[well].max() <= TOL_KIN
+    assert (np.abs(out["obj"] - ref["obj"]) / np.abs(ref["obj"]))[well].max() <= TOL_OBJ_REL
+    lo = np.log10(out["cond"])
+    assert np.abs(lo - lc)[well].max() <= TOL_LOG10_COND
+
+
+@pytest.fixture(scope="module")
+def golden():
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "c1_small.npz"))
+    return {k: z[k] for k in z.files}
+
+
+def test_golden_allele_freqs_and_fit(golden):
+    say("case: golden c1_small (40 x 3000)")
+    with Cohort(golden["codes"], segment_sites=int(golden["segment_sites"])) as co:
+        af, kept = co.allele_freqs()
+        assert np.array_equal(af, golden["af"])
+        assert np.array_equal(kept, golden["kept"])
+        assert co.n_kept == int(golden["kept"].sum())
+        out = co.fit_pairs()
+    assert_bitexact(out, golden["out_seg"])
+    assert_within_tolerance(out, golden["out_ref"])
+
+
+@pytest.fixture(scope="module")
+def medium(O):
+    c = synth.generate(96, 15000, seed=31, n_family_pairs=16, missing=0.02, inbred_fraction=0.05)
+    co = Cohort(c.codes)
+    af, kept = co.allele_freqs()
+    codes, afk = c.codes[:, kept], af[kept]
+    yield c, co, codes, afk
+    co.close()
+
+
+def test_allele_freqs_bitexact(O, medium):
+    c, co, _, _ = medium
+    say("case: allele freqs 96 x 15000")
+    af, kept = co.allele_freqs()
+    exp = O.allele_freqs(c.codes)
+    assert np.array_equal(af, exp)
+    assert np.array_equal(kept, O.keep_sites(exp))
+
+
+def test_evaluate_matches_oracle(O, medium):
+    """The operator boundary (objective + gradient + Hessian at given points)."""
+    c, co, codes, afk = medium
+    say("case: evaluate 300 points")
+    rng = np.random.default_rng(0)
+    pairs = rng.integers(0, codes.shape[0], (300, 2))
+    pts = rng.dirichlet(np.ones(9), 300)
+    pts[:20] = np.eye(9)[rng.integers(0, 9, 20)] * 0.5 + 0.5 / 9  # points with large 1/L spread
+    f, g, h, ns = co.evaluate(pairs, pts)
+    seg = seg_for(co.n_kept, codes.shape[1])
+    for i in range(300):
+        rows, valid, n = O.pair_rows_codes(codes[pairs[i, 0]], codes[pairs[i, 1]], afk)
+        assert ns[i] == n
+        fe = O.obj_seg(rows, valid, seg, pts[i])
+        ge, he = O.gh_seg(rows, valid, seg, pts[i])
+        assert f[i] == fe and np.array_equal(g[i], ge) and np.array_equal(h[i], he), i
+        if i < 40:
+            comp = rows[valid == 1]
+            fr = O.obj_ref(comp, pts[i])
+            gr, hr = O.gh_ref(comp, pts[i])
+            assert abs(f[i] - fr) <= 1e-13 * abs(fr)
+            np.testing.assert_allclose(g[i], gr, rtol=1e-12, atol=1e-13 * np.abs(gr).max())
+            np.testing.assert_allclose(h[i], hr, rtol=1e-12, atol=1e-13 * np.abs(hr).max())
+
+
+def test_fit_all_pairs_bitexact_and_tolerance(O, medium):
+    c, co, codes, afk = medium
+    say("case: fit 96 x 15000 all 4656 pairs")
+    out = co.fit_pairs()
+    assert (out["status"] == 0).all()
+    seg = seg_for(co.n_kept, codes.shape[1])
+    assert_bitexact(out, O.fit_pairs(codes, afk, order=O.ORDER_SEG, seg=seg))
+    assert_within_tolerance(out, O.fit_pairs(codes, afk, order=O.ORDER_REF))
+
+
+def test_schedule_independence(O, medium):
+    """Results do not depend on how many pairs are in flight or their order."""
+    c, co, codes, afk = medium
+    say("case: schedule independence")
+    rng = np.random.default_rng(1)
+    pairs = rng.integers(0, codes.shape[0], (700, 2))
+    a = co.fit_pairs(pairs)
+    co.set_option("slots", 256)
+    b = co.fit_pairs(pairs)
+    perm = rng.permutation(700)
+    co.set_option("slots", 512)
+    cc = co.fit_pairs(pairs[perm])
+    co.set_option("slots", 0)
+    assert_bitexact(a, b)
+    assert_bitexact(a[perm], cc)
+
+
+def test_pair_order_and_subset(O, medium):
+    """(y, x) fits the swapped problem (lib.rs:137-149 swaps delta3<->delta5,
+    delta4<->delta6); n_select restricts to the first samples (cli.rs:41-45)."""
+    c, co, codes, afk = medium
+    say("case: pair order and subset")
+    pairs = np.array([[3, 17], [17, 3], [5, 5]])
+    out = co.fit_pairs(pairs)
+    swap = [0, 1, 4, 5, 2, 3, 6, 7, 8]
+    np.testing.assert_allclose(out[1]["jacquard"], out[0]["jacquard"][swap], atol=1e-6)
+    sub = co.fit_pairs(n_select=10)
+    assert len(sub) == 55 and sub["x"].max() < 10
+    full = co.fit_pairs(n_select=12)
+    idx = {(int(o["x"]), int(o["y"])): o for o in full}
+    for o in sub:
+        e = idx[(int(o["x"]), int(o["y"]))]
+        assert np.array_equal(o["jacquard"], e["jacquard"]) and o["iters"] == e["iters"]
+    jac, kin, inb = build_arrays(sub, 10)
+    assert np.array_equal(jac[3, 7][swap], jac[7, 3]) and kin[3, 7] == kin[7, 3]
+    with pytest.raises(KestrelError):
+        co.fit_pairs(n_select=97)
+
+
+def test_iteration_cap(O, medium):
+    c, co, codes, afk = medium
+    say("case: iteration cap 1 and 2")
+    for it in (1, 2):
+        out = co.fit_pairs(np.array([[0, 1], [2, 9], [4, 4]]), iterations=it)
+        exp = O.fit_pairs(codes, afk, pairs=np.array([[0, 1], [2, 9], [4, 4]]), order=O.ORDER_SEG,
+                          seg=seg_for(co.n_kept, codes.shape[1]), iterations=it)
+        assert_bitexact(out, exp)
+        assert (out["iters"] <= it).all()
+
+
+def test_edge_cases(O):
+    say("case: edge cases")
+    rng = np.random.default_rng(3)
+    codes = rng.integers(0, 3, (12, 700)).astype(np.uint8)
+    codes[:, :50] = 0          # monomorphic sites are dropped
+    codes[5, :] = 3            # a sample with no call: its pairs have no usable site
+    codes[6, ::2] = 3          # ragged missingness
+    with Cohort(codes) as co:
+        af, kept = co.allele_freqs()
+        assert not kept[:50].any()
+        out = co.fit_pairs()
+        exp_af = O.allele_freqs(codes)
+        assert np.array_equal(af, exp_af)
+        bad = (out["x"] == 5) | (out["y"] == 5)
+        assert (out["status"][bad] == 1).all() and (out["n_sites"][bad] == 0).all()
+        assert np.isnan(out["jacquard"][bad]).all()
+        assert (out["status"][~bad] == 0).all()
+        good = out[~bad]
+        kc, ka = codes[:, kept], af[kept]
+        exp = O.fit_pairs(kc, ka, pairs=np.stack([good["x"], good["y"]], 1), order=O.ORDER_SEG,
+                          seg=seg_for(co.n_kept, kc.shape[1]))
+        assert_bitexact(good, exp)
+    # all samples missing at one site: the reference panics; the library reports it
+    codes2 = codes.copy()
+    codes2[:, 60] = 3
+    with pytest.raises(KestrelError):
+        Cohort(codes2)
+    # a cohort with no segregating site: every pair has no usable site
+    with Cohort(np.zeros((4, 40), np.uint8)) as co:
+        assert co.n_kept == 0
+        out = co.fit_pairs()
+        assert (out["status"] == 1).all()
+
+
+def test_given_allele_frequencies(O):
+    say("case: given allele frequencies")
+    rng = np.random.default_rng(4)
+    codes = rng.integers(0, 3, (20, 900)).astype(np.uint8)
+    p = rng.uniform(0.05, 0.95, 900)
+    af = np.stack([p, 1 - p], 1)
+    af[:7, 1] = 0.0
+    af[:7, 0] = 1.0
+    with Cohort(codes, af=af) as co:
+        got, kept = co.allele_freqs()
+        assert np.array_equal(got, af) and not kept[:7].any() and kept[7:].all()
+        out = co.fit_pairs()
+    kc, ka = codes[:, kept], af[kept]
+    exp = O.fit_pairs(kc, ka, order=O.ORDER_SEG, seg=seg_for(int(kept.sum()), kc.shape[1]))
+    assert_bitexact(out, exp)
+
+
+def test_c2_sample_of_pairs(O):
+    """Full-size cohort (1,000 x 100,000): a seeded sample of pairs against the
+    oracle, and size-independent properties of every designated relative."""
+    say("case: C2 1000 x 100000 sampled pairs")
+    c = synth.config("c2")
+    with Cohort(c.codes) as co:
+        af, kept = co.allele_freqs()
+        exp_af = O.allele_freqs(c.codes)
+        assert np.array_equal(af, exp_af)
+        rng = np.random.default_rng(5)
+        pairs = np.concatenate([rng.integers(0, 1000, (120, 2)), np.array([(x, y) for x, y, _ in c.pairs])])
+        out = co.fit_pairs(pairs)
+    codes, afk = c.codes[:, kept], af[kept]
+    exp = O.fit_pairs(codes, afk, pairs=pairs, order=O.ORDER_SEG, seg=seg_for(int(kept.sum()), codes.shape[1]))
+    assert_bitexact(out, exp)
+    assert_within_tolerance(out, O.fit_pairs(codes, afk, pairs=pairs, order=O.ORDER_REF))
+    s = out["jacquard"].sum(1)
+    assert np.abs(s - 1).max() < 1e-12 and (out["jacquard"] >= 0).all()
+    expect = {"PO": 0.25, "FS": 0.25, "HS": 0.125, "FC": 0.0625}
+    kin = kinship(out["jacquard"][120:])
+    for (x, y, kind), k in zip(c.pairs, kin):
+        assert abs(k - expect[kind]) < 0.03, (kind, k)
+
+
+def test_device_arithmetic_selftest():
+    """recip_rn == IEEE 1/x and the positive-domain log == full log, bit for bit."""
+    import ctypes
+    from kestrel_amd import _lib
+    say("case: arithmetic self-test 2^30 inputs")
+    bad = ctypes.c_uint64()
+    _lib.check(_lib.load().kestrel_selftest(1 << 30, 12345, ctypes.byref(bad)))
+    assert bad.value == 0
