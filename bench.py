@@ -130,6 +130,17 @@ def main():
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_VALU_PEAK_TFLOPS, 4), "traffic": None,
                 "kernel": "sweep_fp64", "avg_launch_ms": round(sweep_ms / sweeps, 4),
                 "algorithmic_flop_per_launch": flops / sweeps}
+    measured_peak = None
+    try:
+        import ctypes
+        from kestrel_amd import _lib
+        pk = ctypes.c_double()
+        _lib.check(_lib.load().kestrel_measure_fp64_peak(ctypes.byref(pk)))
+        measured_peak = round(pk.value, 2)
+        roofline["measured_peak"] = measured_peak
+        roofline["frac_of_measured"] = round(achieved / pk.value, 4)
+    except Exception as e:  # the measurement is informative only
+        log(f"fp64 peak measurement failed: {e}")
     cpu = None
     if args.cpu_budget > 0:
         cpu = cpu_baseline(c.codes[:, kept], af[kept], args.cpu_budget)

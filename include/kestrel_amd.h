@@ -126,6 +126,16 @@ void kestrel_cohort_destroy(kestrel_cohort* c);
  * patterns (0 expected). */
 int kestrel_selftest(uint64_t n, uint64_t seed, uint64_t* mismatches);
 
+/* Achievable FP64 vector FMA throughput of this device in TFLOP/s (no
+ * counterpart in the reference): a kernel of independent fma chains, used to
+ * state the measured roofline next to the vendor peak. */
+int kestrel_measure_fp64_peak(double* tflops);
+
+/* Plain device copy of `bytes` (two launches, the second timed): HBM
+ * read+write GB/s (no counterpart in the reference; the reference kernel for
+ * scaling hardware byte counters). */
+int kestrel_measure_copy(uint64_t bytes, double* gbps);
+
 const char* kestrel_last_error(void);
 const char* kestrel_version(void);
 

@@ -420,6 +420,62 @@ int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* poi
 
 void kestrel_cohort_destroy(kestrel_cohort* c) { delete c; }
 
+int kestrel_measure_fp64_peak(double* tflops) {
+    if (!tflops) return fail(KESTREL_EINVAL, "null argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(KESTREL_EDEVICE, "no HIP device: libkestrel_amd needs an MI355X (gfx950) GPU");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    const uint32_t blocks = (uint32_t)prop.multiProcessorCount * 16;  // 16 workgroups of 4 waves per CU
+    const uint32_t iters = 1u << 15;
+    DevBuf<double> sink;
+    HIPCHK(sink.alloc(1));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(klaunch::calib_fp64(blocks, 1024, sink.p, nullptr));  // warm up clocks
+    HIPCHK(hipEventRecord(e0, nullptr));
+    HIPCHK(klaunch::calib_fp64(blocks, iters, sink.p, nullptr));
+    HIPCHK(hipEventRecord(e1, nullptr));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const double flop = (double)blocks * 256.0 * iters * 16.0 * 2.0;
+    *tflops = flop / (ms * 1e-3) / 1e12;
+    return KESTREL_OK;
+}
+
+int kestrel_measure_copy(uint64_t bytes, double* gbps) {
+    if (!gbps || bytes < 16) return fail(KESTREL_EINVAL, "bad argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(KESTREL_EDEVICE, "no HIP device: libkestrel_amd needs an MI355X (gfx950) GPU");
+    const uint64_t n16 = bytes / 16;
+    DevBuf<uint4> a, b;
+    HIPCHK(a.alloc(n16));
+    HIPCHK(b.alloc(n16));
+    HIPCHK(hipMemset(a.p, 1, n16 * 16));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(klaunch::copy(a.p, b.p, n16, nullptr));
+    HIPCHK(hipEventRecord(e0, nullptr));
+    HIPCHK(klaunch::copy(a.p, b.p, n16, nullptr));
+    HIPCHK(hipEventRecord(e1, nullptr));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *gbps = 2.0 * (double)(n16 * 16) / (ms * 1e-3) / 1e9;
+    return KESTREL_OK;
+}
+
 int kestrel_selftest(uint64_t n, uint64_t seed, uint64_t* mismatches) {
     if (!mismatches) return fail(KESTREL_EINVAL, "null argument");
     int ndev = 0;

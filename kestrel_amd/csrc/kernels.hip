@@ -310,7 +310,7 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
     const uint32_t* rowx = geno + (size_t)x * row_words;
     const uint32_t* rowy = geno + (size_t)y * row_words;
 
-    double F = 0.0, N = 0.0, G[9], H[45];
+    double F = 0.0, N = 0.0, P = 1.0, G[9], H[45];
 #pragma unroll
     for (int k = 0; k < 9; k++) G[k] = 0.0;
 #pragma unroll
@@ -351,7 +351,13 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
             double L = eps;
 #pragma unroll
             for (int j = 0; j < 9; j++) L = fma(D[j], cv[j], L);
-            F = fma(vd, sleef_log_pos(L), F);
+            // sum of log L over a 16-site group = log of the product of the
+            // usable sites' L (a missing site multiplies by exactly 1.0)
+            P = P * fma(vd, L, 1.0 - vd);
+            if ((k & (LOG_GROUP - 1)) == LOG_GROUP - 1) {
+                F = F + sleef_log_pos(P);
+                P = 1.0;
+            }
             N += vd;
             const double r = recip_rn(L);  // grad_hess.rs:122
 #pragma unroll
@@ -586,6 +592,29 @@ __global__ __launch_bounds__(256) void selftest_kernel(uint64_t n, uint64_t seed
     if (local) atomicAdd(bad, local);
 }
 
+// Achievable FP64 FMA throughput: 16 independent fma chains per lane.
+__global__ __launch_bounds__(256) void calib_fp64_kernel(uint32_t iters, double a, double b,
+                                                          double* __restrict__ sink) {
+    double acc[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) acc[k] = (double)(threadIdx.x + k);
+    for (uint32_t i = 0; i < iters; i++) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) acc[k] = fma(acc[k], a, b);
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) s += acc[k];
+    if (s == 1.2345) sink[0] = s;  // keeps the chains live
+}
+
+// Plain copy of n16 16-byte words (the reference kernel for HBM byte counters).
+__global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                    uint64_t n16) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // ------------------------------------------------------------ launchers
 
 namespace klaunch {
@@ -626,6 +655,16 @@ hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* act
 
 hipError_t selftest(uint64_t n, uint64_t seed, unsigned long long* bad, hipStream_t st) {
     hipLaunchKernelGGL(selftest_kernel, dim3(4096), dim3(256), 0, st, n, seed, bad);
+    return hipGetLastError();
+}
+
+hipError_t calib_fp64(uint32_t blocks, uint32_t iters, double* sink, hipStream_t st) {
+    hipLaunchKernelGGL(calib_fp64_kernel, dim3(blocks), dim3(256), 0, st, iters, 0.999999, 1e-7, sink);
+    return hipGetLastError();
+}
+
+hipError_t copy(const void* src, void* dst, uint64_t n16, hipStream_t st) {
+    hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(256), 0, st, (const uint4*)src, (uint4*)dst, n16);
     return hipGetLastError();
 }
 

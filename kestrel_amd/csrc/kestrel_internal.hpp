@@ -8,6 +8,7 @@
 #include "sqp_device.hpp"
 
 constexpr int CHUNK = 32;       // sites per LDS table chunk
+constexpr int LOG_GROUP = 16;   // sites per product in the objective (one log per group)
 constexpr int SWEEP_TPB = 256;  // threads (pairs) per sweep block
 constexpr int NPART = 56;       // f, g[9], h[45], n per segment partial
 constexpr uint32_t SITE_PAD = 256;  // rows padded to a multiple of this many sites
@@ -41,6 +42,8 @@ hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* act
                 unsigned long long* next_pair, kestrel_output* out, const kdev::Tune& t, int* n_live,
                 hipStream_t st);
 hipError_t selftest(uint64_t n, uint64_t seed, unsigned long long* bad, hipStream_t st);
+hipError_t calib_fp64(uint32_t blocks, uint32_t iters, double* sink, hipStream_t st);
+hipError_t copy(const void* src, void* dst, uint64_t n16, hipStream_t st);
 hipError_t finalize(const double* partial, uint32_t nseg, uint32_t nslots, uint32_t npts, double* f, double* g,
                     double* h, uint32_t* nsites, hipStream_t st);
 }  // namespace klaunch
