@@ -27,6 +27,9 @@ sys.path.insert(0, ROOT)
 
 FP64_VALU_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
 FLOP_PER_SITE_EVAL = 156  # 136 (gradient + Hessian) + 20 (objective, log counted as 1)
+# HBM bytes per sweep_fp64 launch on the default workload, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE scaled by a plain 256 MiB copy (profiles/sweep_fp64_c2.md)
+SWEEP_TRAFFIC_BYTES_C2 = 1.593e8
 
 
 def log(*a):
@@ -127,7 +130,8 @@ def main():
     value = offdiag * args.steps / total
     achieved = flops / (sweep_ms / 1e3) / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_VALU_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP64_VALU_PEAK_TFLOPS, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / FP64_VALU_PEAK_TFLOPS, 4),
+                "traffic": SWEEP_TRAFFIC_BYTES_C2 if args.config == "c2" and not args.segment_sites else None,
                 "kernel": "sweep_fp64", "avg_launch_ms": round(sweep_ms / sweeps, 4),
                 "algorithmic_flop_per_launch": flops / sweeps}
     measured_peak = None
