@@ -109,6 +109,24 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
 int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* points, size_t n, double eps,
                      double* f, double* g, double* h, uint32_t* n_sites);
 
+/* The CLS / Brier estimator of the CLI default -f GT.
+ *   kestrel_cohort_create_gt  replaces wrapper.rs:22-87 up to the pair loop:
+ *                             sites with a missing call in any sample are
+ *                             dropped (vcf.rs:217-221), frequencies are
+ *                             counted (wrapper.rs:412-428) unless given
+ *                             (n_sites x 2, -a info), every other site is kept.
+ *   kestrel_fit_pairs_gt      replaces coefficients.rs:166-278
+ *                             calculate_coefficients_gt: one 9x9 Gram matrix Q
+ *                             (quadratic.rs, coefficients.rs:280-320), per pair
+ *                             c = -(1/V) sum_s m_s[gx][gy] (cls.rs:28-49) in site
+ *                             order, the QP (2 variables for a self pair), brier
+ *                             in obj, QP iterations in iters (cap = iterations).
+ *                             q_out (81 doubles, may be NULL) receives Q. */
+int kestrel_cohort_create_gt(kestrel_cohort** out, const uint8_t* packed, uint32_t n_samples, uint32_t n_sites,
+                             size_t row_stride_bytes, const double* af);
+int kestrel_fit_pairs_gt(kestrel_cohort* c, uint64_t iterations, const uint32_t* pairs, size_t n_pairs,
+                         uint32_t n_select, kestrel_output* out, double* q_out);
+
 /* Tuning knobs: "segment_sites" (sites per segment of a sweep, a multiple of
  * 32, default 4096; fixes the summation order, so it is a cohort property),
  * "slots" (pairs in flight, 0 = fill the device). */

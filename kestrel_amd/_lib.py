@@ -57,7 +57,7 @@ EXPORTS = (
     "kestrel_tune_default", "kestrel_cohort_create", "kestrel_cohort_allele_freqs", "kestrel_cohort_n_kept",
     "kestrel_fit_pairs", "kestrel_evaluate", "kestrel_set_option", "kestrel_last_fit_stats",
     "kestrel_cohort_destroy", "kestrel_last_error", "kestrel_version", "kestrel_selftest",
-    "kestrel_measure_fp64_peak", "kestrel_measure_copy",
+    "kestrel_measure_fp64_peak", "kestrel_measure_copy", "kestrel_cohort_create_gt", "kestrel_fit_pairs_gt",
 )
 
 _lib = None
@@ -83,6 +83,8 @@ def load():
     L.kestrel_set_option.argtypes = [P, ctypes.c_char_p, ctypes.c_int64]
     L.kestrel_last_fit_stats.argtypes = [P, P, P, P]
     L.kestrel_selftest.argtypes = [ctypes.c_uint64, ctypes.c_uint64, P]
+    L.kestrel_cohort_create_gt.argtypes = [P, P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_size_t, P]
+    L.kestrel_fit_pairs_gt.argtypes = [P, ctypes.c_uint64, P, ctypes.c_size_t, ctypes.c_uint32, P, P]
     L.kestrel_measure_fp64_peak.argtypes = [P]
     L.kestrel_measure_copy.argtypes = [ctypes.c_uint64, P]
     L.kestrel_cohort_destroy.argtypes = [P]

@@ -46,6 +46,42 @@ class Cohort:
         if segment_sites is not None:
             self.set_option("segment_sites", segment_sites)
 
+    @classmethod
+    def gt(cls, codes: np.ndarray | None = None, *, packed: np.ndarray | None = None, n_sites: int | None = None,
+           af: np.ndarray | None = None) -> "Cohort":
+        """Cohort for the CLS / Brier estimator of -f GT (kestrel_cohort_create_gt):
+        sites with a missing call are dropped, frequencies counted unless given."""
+        L = _lib.load()
+        self = cls.__new__(cls)
+        if packed is None:
+            codes = np.ascontiguousarray(codes, np.uint8)
+            n_sites = codes.shape[1]
+            packed = pack2(codes)
+        packed = np.ascontiguousarray(packed, np.uint8)
+        self.n_samples = packed.shape[0]
+        self.n_sites = int(n_sites)
+        afp = None if af is None else np.ascontiguousarray(af, np.float64).reshape(self.n_sites, 2)
+        h = ctypes.c_void_p()
+        check(L.kestrel_cohort_create_gt(ctypes.byref(h), ptr(packed), self.n_samples, self.n_sites,
+                                         packed.shape[1], ptr(afp)))
+        self._h = h
+        return self
+
+    def fit_pairs_gt(self, pairs: np.ndarray | None = None, n_select: int | None = None, iterations: int = 100):
+        """CLS fit of pairs (coefficients.rs:166-278); returns (outputs, Q)."""
+        if pairs is None:
+            n_select = self.n_samples if n_select is None else n_select
+            n = n_select * (n_select + 1) // 2
+            pp = None
+        else:
+            pp = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+            n = pp.shape[0]
+            n_select = 0
+        out = np.zeros(n, OUTPUT_DTYPE)
+        q = np.zeros((9, 9))
+        check(_lib.load().kestrel_fit_pairs_gt(self._h, iterations, ptr(pp), n, n_select, ptr(out), ptr(q)))
+        return out, q
+
     def close(self):
         if getattr(self, "_h", None):
             _lib.load().kestrel_cohort_destroy(self._h)

@@ -83,6 +83,9 @@ struct kestrel_cohort {
     uint32_t nseg = 0;
     std::vector<double> af_all;
     std::vector<uint8_t> kept_mask;
+    bool gt = false;        // created by kestrel_cohort_create_gt (CLS path)
+    DevBuf<double> qmat;    // CLS Gram matrix (computed on first use)
+    bool have_q = false;
     int64_t opt_slots = 0;
     int64_t opt_segment_sites = 4096;
     double last_sweep_ms = 0, last_wall_ms = 0;
@@ -107,6 +110,40 @@ struct kestrel_cohort {
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
+
+namespace {
+
+// Repack the kept sites of the uploaded 2-bit rows into c->geno (rows of
+// v_pad / 16 words, padding code 3) and set n_kept / v_pad / row_words.
+int repack_kept(kestrel_cohort* c, const uint8_t* dpacked, size_t row_stride_bytes, const std::vector<uint32_t>& kept) {
+    c->n_kept = (uint32_t)kept.size();
+    c->v_pad = std::max<uint32_t>(SITE_PAD, (c->n_kept + SITE_PAD - 1) / SITE_PAD * SITE_PAD);
+    c->row_words = c->v_pad / 16;
+    DevBuf<uint32_t> dkept;
+    HIPCHK(dkept.alloc(std::max<size_t>(kept.size(), 1)));
+    if (!kept.empty())
+        HIPCHK(hipMemcpyAsync(dkept.p, kept.data(), kept.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              c->stream));
+    HIPCHK(c->geno.alloc((size_t)c->n_samples * c->row_words));
+    HIPCHK(klaunch::gather(dpacked, row_stride_bytes, dkept.p, c->n_kept, c->n_samples, c->geno.p, c->row_words,
+                           c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return KESTREL_OK;
+}
+
+int open_device(kestrel_cohort* c) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(KESTREL_EDEVICE, "no HIP device: libkestrel_amd needs an MI355X (gfx950) GPU");
+    HIPCHK(hipGetDevice(&c->device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, c->device));
+    c->n_cu = prop.multiProcessorCount;
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    return KESTREL_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -139,17 +176,10 @@ int kestrel_cohort_create(kestrel_cohort** out, const uint8_t* packed, uint32_t 
     if (!(maf >= 0.0 && maf <= 1.0)) return fail(KESTREL_EINVAL, "Minimum allele frequency must be within 0 to 1");
     if (!af && af_iterations == 0) return fail(KESTREL_EINVAL, "iterations must be non-zero");
 
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
-        return fail(KESTREL_EDEVICE, "no HIP device: libkestrel_amd needs an MI355X (gfx950) GPU");
-
     auto c = new kestrel_cohort();
     std::unique_ptr<kestrel_cohort> guard(c);
-    HIPCHK(hipGetDevice(&c->device));
-    hipDeviceProp_t prop;
-    HIPCHK(hipGetDeviceProperties(&prop, c->device));
-    c->n_cu = prop.multiProcessorCount;
-    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    int rc = open_device(c);
+    if (rc) return rc;
     c->n_samples = n_samples;
     c->n_sites = n_sites;
 
@@ -189,10 +219,8 @@ int kestrel_cohort_create(kestrel_cohort** out, const uint8_t* packed, uint32_t 
             c->kept_mask[s] = 1;
             kept.push_back(s);
         }
-    c->n_kept = (uint32_t)kept.size();
-    c->v_pad = std::max<uint32_t>(SITE_PAD, (c->n_kept + SITE_PAD - 1) / SITE_PAD * SITE_PAD);
-    c->row_words = c->v_pad / 16;
-
+    rc = repack_kept(c, dpacked.p, row_stride_bytes, kept);
+    if (rc) return rc;
     std::vector<double> afk((size_t)c->v_pad * 2, 0.5);
     for (uint32_t t = 0; t < c->n_kept; t++) {
         afk[2 * t] = c->af_all[2 * kept[t]];
@@ -200,16 +228,8 @@ int kestrel_cohort_create(kestrel_cohort** out, const uint8_t* packed, uint32_t 
     }
     HIPCHK(c->af.alloc(afk.size()));
     HIPCHK(hipMemcpyAsync(c->af.p, afk.data(), afk.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    DevBuf<uint32_t> dkept;
-    HIPCHK(dkept.alloc(std::max<size_t>(kept.size(), 1)));
-    if (!kept.empty())
-        HIPCHK(hipMemcpyAsync(dkept.p, kept.data(), kept.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
-                              c->stream));
-    HIPCHK(c->geno.alloc((size_t)n_samples * c->row_words));
-    HIPCHK(klaunch::gather(dpacked.p, row_stride_bytes, dkept.p, c->n_kept, n_samples, c->geno.p, c->row_words,
-                           c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    int rc = c->set_segments();
+    rc = c->set_segments();
     if (rc) return rc;
     *out = guard.release();
     return KESTREL_OK;
@@ -254,6 +274,7 @@ int kestrel_last_fit_stats(const kestrel_cohort* c, double* sweep_ms, uint64_t* 
 int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_t* pairs, size_t n_pairs,
                       uint32_t n_select, kestrel_output* out) {
     if (!c || !tune || !out) return fail(KESTREL_EINVAL, "null argument");
+    if (c->gt) return fail(KESTREL_EINVAL, "a GT cohort is fitted with kestrel_fit_pairs_gt");
     if (tune->sqp_max_iter == 0 || tune->qp_max_iter == 0 || tune->bls_max_iter == 0)
         return fail(KESTREL_EINVAL, "iteration limits must be non-zero");
     std::vector<uint32_t> tri;
@@ -488,6 +509,117 @@ int kestrel_selftest(uint64_t n, uint64_t seed, uint64_t* mismatches) {
     unsigned long long h = 0;
     HIPCHK(hipMemcpy(&h, bad.p, sizeof h, hipMemcpyDeviceToHost));
     *mismatches = h;
+    return KESTREL_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int kestrel_cohort_create_gt(kestrel_cohort** out, const uint8_t* packed, uint32_t n_samples, uint32_t n_sites,
+                             size_t row_stride_bytes, const double* af) {
+    if (!out || !packed) return fail(KESTREL_EINVAL, "null argument");
+    *out = nullptr;
+    if (n_samples == 0 || n_sites == 0) return fail(KESTREL_EINVAL, "need at least one sample and one site");
+    if (row_stride_bytes < (n_sites + 3) / 4) return fail(KESTREL_EINVAL, "row stride too small");
+    auto c = new kestrel_cohort();
+    std::unique_ptr<kestrel_cohort> guard(c);
+    int rc = open_device(c);
+    if (rc) return rc;
+    c->gt = true;
+    c->n_samples = n_samples;
+    c->n_sites = n_sites;
+    DevBuf<uint8_t> dpacked, dflag;
+    HIPCHK(dpacked.alloc((size_t)n_samples * row_stride_bytes));
+    HIPCHK(dflag.alloc(n_sites));
+    HIPCHK(hipMemcpyAsync(dpacked.p, packed, (size_t)n_samples * row_stride_bytes, hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(klaunch::complete_sites(dpacked.p, row_stride_bytes, n_samples, n_sites, dflag.p, c->stream));
+    c->kept_mask.resize(n_sites);
+    HIPCHK(hipMemcpyAsync(c->kept_mask.data(), dflag.p, n_sites, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<uint32_t> kept;
+    for (uint32_t s = 0; s < n_sites; s++)
+        if (c->kept_mask[s]) kept.push_back(s);
+    rc = repack_kept(c, dpacked.p, row_stride_bytes, kept);
+    if (rc) return rc;
+    HIPCHK(c->af.alloc((size_t)c->v_pad * 2));
+    HIPCHK(hipMemsetAsync(c->af.p, 0, (size_t)c->v_pad * 2 * sizeof(double), c->stream));
+    c->af_all.assign((size_t)n_sites * 2, std::nan(""));
+    if (af) {
+        std::vector<double> afk((size_t)c->v_pad * 2, 0.0);
+        for (uint32_t t = 0; t < c->n_kept; t++) {
+            afk[2 * t] = af[2 * kept[t]];
+            afk[2 * t + 1] = af[2 * kept[t] + 1];
+        }
+        HIPCHK(hipMemcpyAsync(c->af.p, afk.data(), afk.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    } else {
+        HIPCHK(klaunch::gt_count(c->geno.p, c->row_words, n_samples, c->n_kept, c->af.p, c->stream));
+    }
+    std::vector<double> afk((size_t)c->v_pad * 2);
+    HIPCHK(hipMemcpyAsync(afk.data(), c->af.p, afk.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (uint32_t t = 0; t < c->n_kept; t++) {
+        c->af_all[2 * kept[t]] = afk[2 * t];
+        c->af_all[2 * kept[t] + 1] = afk[2 * t + 1];
+    }
+    rc = c->set_segments();
+    if (rc) return rc;
+    *out = guard.release();
+    return KESTREL_OK;
+}
+
+int kestrel_fit_pairs_gt(kestrel_cohort* c, uint64_t iterations, const uint32_t* pairs, size_t n_pairs,
+                         uint32_t n_select, kestrel_output* out, double* q_out) {
+    if (!c || !out) return fail(KESTREL_EINVAL, "null argument");
+    if (!c->gt) return fail(KESTREL_EINVAL, "cohort was not created by kestrel_cohort_create_gt");
+    if (iterations == 0) return fail(KESTREL_EINVAL, "iterations must be non-zero");
+    if (c->n_kept == 0) return fail(KESTREL_EINVAL, "Must have at least one site");
+    std::vector<uint32_t> tri;
+    if (!pairs) {
+        if (n_select > c->n_samples)
+            return fail(KESTREL_EINVAL, "Requested samples %u exceeds total samples %u", n_select, c->n_samples);
+        n_pairs = (size_t)n_select * (n_select + 1) / 2;
+        tri.resize(2 * n_pairs);
+        size_t i = 0;
+        for (uint32_t x = 0; x < n_select; x++)
+            for (uint32_t y = x; y < n_select; y++, i++) {
+                tri[2 * i] = x;
+                tri[2 * i + 1] = y;
+            }
+        pairs = tri.data();
+    } else {
+        for (size_t i = 0; i < 2 * n_pairs; i++)
+            if (pairs[i] >= c->n_samples) return fail(KESTREL_EINVAL, "pair names sample %u of %u", pairs[i], c->n_samples);
+    }
+    hipStream_t st = c->stream;
+    if (!c->have_q) {
+        const uint64_t rows = (uint64_t)c->n_kept * 9, n_tiles = (rows / 8 + 31) / 32;
+        DevBuf<double> z;
+        HIPCHK(z.alloc(std::max<uint64_t>(1, n_tiles) * 45 * 8));
+        HIPCHK(c->qmat.alloc(81));
+        HIPCHK(klaunch::gram(c->af.p, c->n_kept, z.p, c->qmat.p, st));
+        HIPCHK(hipStreamSynchronize(st));
+        c->have_q = true;
+    }
+    if (q_out) HIPCHK(hipMemcpy(q_out, c->qmat.p, 81 * sizeof(double), hipMemcpyDeviceToHost));
+    if (n_pairs == 0) return KESTREL_OK;
+    const auto t0 = std::chrono::steady_clock::now();
+    kestrel_tune t;
+    kestrel_tune_default(&t, iterations);
+    t.qp_max_iter = iterations;  // coefficients.rs:207: Tuneables::new(iterations, iterations)
+    DevBuf<uint32_t> dpairs;
+    DevBuf<kestrel_output> dout;
+    HIPCHK(dpairs.alloc(2 * n_pairs));
+    HIPCHK(dout.alloc(n_pairs));
+    HIPCHK(hipMemcpyAsync(dpairs.p, pairs, 2 * n_pairs * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(klaunch::cls_pairs(c->geno.p, c->row_words, c->af.p, c->n_kept, c->qmat.p, dpairs.p, n_pairs, to_dev(t),
+                              dout.p, st));
+    HIPCHK(hipMemcpyAsync(out, dout.p, n_pairs * sizeof(kestrel_output), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    c->last_wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    c->last_sweeps = 1;
+    c->last_sweep_ms = 0;
     return KESTREL_OK;
 }
 

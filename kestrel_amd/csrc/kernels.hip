@@ -615,6 +615,256 @@ __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src
         dst[i] = src[i];
 }
 
+// ------------------------------------------------------------ CLS (-f GT)
+
+namespace {
+
+// iis.rs:279-462 conditional_probabilities for the modes two alleles reach.
+__device__ __forceinline__ void cond_prob(double pi, double pj, double pk, double pl, int mode, double* o) {
+    const double pi2 = pi * pi, pi3 = pi2 * pi, pj2 = pj * pj, pk2 = pk * pk;
+#pragma unroll
+    for (int i = 0; i < 9; i++) o[i] = 0.0;
+    switch (mode) {
+    case 1:
+        o[0] = pi; o[1] = pi2; o[2] = pi2; o[3] = pi3; o[4] = pi2; o[5] = pi3; o[6] = pi2; o[7] = pi3;
+        o[8] = pi2 * pi2;
+        break;
+    case 2: o[1] = pi * pk; o[3] = pi * pk2; o[5] = pi2 * pk; o[8] = pi2 * pk2; break;
+    case 3: o[2] = pi * pl; o[3] = (2.0 * pi2) * pl; o[7] = pi2 * pl; o[8] = (2.0 * pi3) * pl; break;
+    case 4: o[2] = pi * pk; o[3] = (2.0 * pi2) * pk; o[7] = pi2 * pk; o[8] = (2.0 * pi3) * pk; break;
+    case 6: o[4] = pi * pj; o[5] = (2.0 * pi2) * pj; o[7] = pi2 * pj; o[8] = (2.0 * pi3) * pj; break;
+    case 7: o[4] = pi * pj; o[5] = (2.0 * pj2) * pi; o[7] = pj2 * pi; o[8] = (2.0 * (pj2 * pj)) * pi; break;
+    default:  // 9: ij-ij
+        o[6] = (2.0 * pi) * pj; o[7] = (pi * pj) * (pi + pj); o[8] = (4.0 * pi2) * pj2;
+        break;
+    }
+}
+
+// conditional.rs IIS_2: genotype pair (gx, gy) in VCF order -> alleles and mode
+__device__ __forceinline__ void m_row(double p, double q, int gx, int gy, double* o) {
+    const int A[3][2] = {{0, 0}, {0, 1}, {1, 1}};
+    const int MODE[3][3] = {{1, 3, 2}, {6, 9, 7}, {2, 4, 1}};
+    const double f[2] = {p, q};
+    cond_prob(f[A[gx][0]], f[A[gx][1]], f[A[gy][0]], f[A[gy][1]], MODE[gx][gy], o);
+}
+
+}  // namespace
+
+// vcf.rs:217-221: a site with a missing call in any sample is dropped on the
+// GT path; flag[s] = 1 when every sample is called.
+__global__ __launch_bounds__(64) void complete_sites_kernel(const uint8_t* __restrict__ packed, size_t row_bytes,
+                                                             uint32_t n_samples, uint32_t n_sites,
+                                                             uint8_t* __restrict__ flag) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_sites) return;
+    uint8_t ok = 1;
+    for (uint32_t i = 0; i < n_samples; i++)
+        if (((packed[(size_t)i * row_bytes + (s >> 2)] >> ((s & 3u) * 2u)) & 3u) == 3u) ok = 0;
+    flag[s] = ok;
+}
+
+// wrapper.rs:412-428: allele frequencies by counting over all samples.
+__global__ __launch_bounds__(64) void gt_count_kernel(const uint32_t* __restrict__ geno, uint32_t row_words,
+                                                       uint32_t n_samples, uint32_t n_sites,
+                                                       double* __restrict__ af) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_sites) return;
+    uint64_t f0 = 0, f1 = 0;
+    for (uint32_t i = 0; i < n_samples; i++) {
+        const uint32_t g = (geno[(size_t)i * row_words + s / 16] >> (2 * (s % 16))) & 3u;
+        f0 += g == 0 ? 2 : g == 1 ? 1 : 0;
+        f1 += g == 2 ? 2 : g == 1 ? 1 : 0;
+    }
+    const double tot = (double)(2 * (uint64_t)n_samples);
+    af[2 * s] = (double)f0 / tot;
+    af[2 * s + 1] = (double)f1 / tot;
+}
+
+// quadratic.rs:68-365 per tile: thread (tile t, lane j) sums its 32 rows
+// r = 256 t + 8 b + j of the 9V rows (site s = r / 9, genotype pair r % 9)
+// from zero into z[t][45][8].
+__global__ __launch_bounds__(64) void gram_tile_kernel(const double* __restrict__ af, uint64_t n_blocks,
+                                                        uint64_t n_tiles, double* __restrict__ z) {
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= n_tiles * 8) return;
+    const uint64_t t = tid / 8;
+    const int j = (int)(tid % 8);
+    double acc[45];
+#pragma unroll
+    for (int k = 0; k < 45; k++) acc[k] = 0.0;
+    const uint64_t b0 = t * 32, b1 = b0 + 32 < n_blocks ? b0 + 32 : n_blocks;
+    for (uint64_t b = b0; b < b1; b++) {
+        const uint64_t r = b * 8 + j;
+        const uint64_t s = r / 9;
+        const int combo = (int)(r % 9);
+        double m[9];
+        m_row(af[2 * s], af[2 * s + 1], combo / 3, combo % 3, m);
+        int id = 0;
+#pragma unroll
+        for (int a = 0; a < 9; a++)
+#pragma unroll
+            for (int c = a; c < 9; c++, id++) acc[id] = fma(m[a], m[c], acc[id]);
+    }
+#pragma unroll
+    for (int k = 0; k < 45; k++) z[(t * 45 + k) * 8 + j] = acc[k];
+}
+
+// Running lane sums over tiles (h = h + z_t), radd, the scalar remainder rows
+// as one partial tile, divided by the number of sites (quadratic.rs:12-33).
+__global__ __launch_bounds__(64) void gram_final_kernel(const double* __restrict__ af, const double* __restrict__ z,
+                                                         uint64_t n_tiles, uint64_t n_blocks, uint32_t n_rem,
+                                                         uint32_t n_sites, double* __restrict__ q) {
+    const int id = threadIdx.x;
+    if (id >= 45) return;
+    int a = 0, c = 0;
+    for (int k = 0, row = 0; row < 9; row++)
+        for (int col = row; col < 9; col++, k++)
+            if (k == id) {
+                a = row;
+                c = col;
+            }
+    double h[8];
+    for (int j = 0; j < 8; j++) h[j] = 0.0;
+    for (uint64_t t = 0; t < n_tiles; t++)
+        for (int j = 0; j < 8; j++) h[j] = h[j] + z[(t * 45 + id) * 8 + j];
+    double rz = 0.0;
+    for (uint32_t r = 0; r < n_rem; r++) {
+        const uint64_t row = n_blocks * 8 + r;
+        const uint64_t s = row / 9;
+        const int combo = (int)(row % 9);
+        double m[9];
+        m_row(af[2 * s], af[2 * s + 1], combo / 3, combo % 3, m);
+        rz = fma(m[a], m[c], rz);
+    }
+    const double v = (radd8(h) + (0.0 + rz)) / (double)n_sites;
+    q[a * 9 + c] = v;
+    q[c * 9 + a] = v;
+}
+
+// coefficients.rs:216-276, one pair per lane: c = sum over all sites of
+// m[gx][gy] in site order (cls.rs:28-49), c / -V, the QP from 1/9 (a self pair
+// solves the 2-variable (delta1, delta7) QP from (1/2, 1/2)), brier, gradient
+// and condition.
+__global__ __launch_bounds__(256) void cls_pairs_kernel(const uint32_t* __restrict__ geno, uint32_t row_words,
+                                                         const double* __restrict__ af, uint32_t n_sites,
+                                                         const double* __restrict__ qmat,
+                                                         const uint32_t* __restrict__ pairs, uint64_t npairs,
+                                                         Tune tune, kestrel_output* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) double lut[2 * LUT_BUF];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t idx = (uint64_t)blockIdx.x * 256 + tid;
+    const bool live = idx < npairs;
+    for (uint32_t i = tid; i < 2 * LUT_BUF; i += 256) lut[i] = 0.0;
+    uint32_t x = 0, y = 0;
+    if (live) {
+        x = pairs[2 * idx];
+        y = pairs[2 * idx + 1];
+    }
+    const uint32_t* rowx = geno + (size_t)x * row_words;
+    const uint32_t* rowy = geno + (size_t)y * row_words;
+    const int lut_site = tid >> 3, lut_part = tid & 7;
+    auto build = [&](uint32_t cs, int b) {
+        const uint32_t s = cs + lut_site;
+        double* buf = lut + b * LUT_BUF;
+        const int parts[2] = {lut_part, 8};
+        for (int w = 0; w < (lut_part == 0 ? 2 : 1); w++) {
+            const int pr = parts[w];
+            const int gx = pr / 3, gy = pr % 3;
+            double* dst = buf + ((gx * 4 + gy) * CHUNK + lut_site) * LUT_W;
+            if (s < n_sites) m_row(af[2 * s], af[2 * s + 1], gx, gy, dst);
+            else
+                for (int k = 0; k < 9; k++) dst[k] = 0.0;
+        }
+    };
+    double c[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) c[k] = 0.0;
+    const uint32_t v_pad = row_words * 16;
+    __syncthreads();
+    build(0, 0);
+    __syncthreads();
+    int b = 0;
+    for (uint32_t cs = 0; cs < v_pad; cs += CHUNK, b ^= 1) {
+        if (cs + CHUNK < v_pad) build(cs + CHUNK, b ^ 1);
+        const uint2 wx = *reinterpret_cast<const uint2*>(rowx + cs / 16);
+        const uint2 wy = *reinterpret_cast<const uint2*>(rowy + cs / 16);
+        const double* L0 = lut + b * LUT_BUF;
+        // padding sites past n_sites carry code 3 and select a zero row (x + 0 = x)
+#pragma unroll
+        for (int k = 0; k < CHUNK; k++) {
+            {
+                const uint32_t ax = (k < 16 ? wx.x : wx.y) >> (2 * (k & 15));
+                const uint32_t ay = (k < 16 ? wy.x : wy.y) >> (2 * (k & 15));
+                const uint32_t row = min(((ax & 3u) << 2) | (ay & 3u), 11u);
+                const double2* r2 = reinterpret_cast<const double2*>(L0 + (row * CHUNK + k) * LUT_W);
+                double m[10];
+#pragma unroll
+                for (int j = 0; j < 5; j++) {
+                    const double2 v = r2[j];
+                    m[2 * j] = v.x;
+                    m[2 * j + 1] = v.y;
+                }
+#pragma unroll
+                for (int j = 0; j < 9; j++) c[j] = c[j] + m[j];
+            }
+        }
+        __syncthreads();
+    }
+    if (!live) return;
+    const double nv = -(double)n_sites;
+#pragma unroll
+    for (int k = 0; k < 9; k++) c[k] = c[k] / nv;
+    double Q[9][9];
+#pragma unroll
+    for (int i = 0; i < 9; i++)
+#pragma unroll
+        for (int j = 0; j < 9; j++) Q[i][j] = qmat[i * 9 + j];
+    double delta[9];
+    uint64_t iters;
+    if (x == y) {
+        const double Q2[2][2] = {{Q[0][0], Q[0][6]}, {Q[6][0], Q[6][6]}};
+        const double c2[2] = {c[0], c[6]};
+        const double y0[2] = {0.5, 0.5};
+        double y2[2];
+        iters = solve_qp_reg<2>(Q2, c2, y0, tune, y2);
+#pragma unroll
+        for (int k = 0; k < 9; k++) delta[k] = 0.0;
+        delta[0] = y2[0];
+        delta[6] = y2[1];
+    } else {
+        double y0[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) y0[k] = 1.0 / 9.0;
+        iters = solve_qp_reg<9>(Q, c, y0, tune, delta);
+    }
+    double qd[9], g[9], dq = 0.0, dc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < 9; j++) s = fma(Q[i][j], delta[j], s);
+        qd[i] = s;
+    }
+#pragma unroll
+    for (int i = 0; i < 9; i++) dq = fma(delta[i], qd[i], dq);
+#pragma unroll
+    for (int i = 0; i < 9; i++) dc = fma(c[i], delta[i], dc);
+    const double brier = dq + 2.0 * dc + 1.0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) g[i] = qd[i] + c[i];
+    kestrel_output o;
+    o.x = x;
+    o.y = y;
+#pragma unroll
+    for (int k = 0; k < 9; k++) o.jacquard[k] = delta[k];
+    o.obj = brier;
+    o.iters = iters;
+    o.cond = condition_reg<9>(delta, g, Q, tune.sqp_conv_tol);
+    o.n_sites = n_sites;
+    o.status = KESTREL_PAIR_OK;
+    out[idx] = o;
+}
+
 // ------------------------------------------------------------ launchers
 
 namespace klaunch {
@@ -665,6 +915,40 @@ hipError_t calib_fp64(uint32_t blocks, uint32_t iters, double* sink, hipStream_t
 
 hipError_t copy(const void* src, void* dst, uint64_t n16, hipStream_t st) {
     hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(256), 0, st, (const uint4*)src, (uint4*)dst, n16);
+    return hipGetLastError();
+}
+
+hipError_t complete_sites(const uint8_t* packed, size_t row_bytes, uint32_t n_samples, uint32_t n_sites, uint8_t* flag,
+                          hipStream_t st) {
+    if (n_sites == 0) return hipSuccess;
+    hipLaunchKernelGGL(complete_sites_kernel, dim3((n_sites + 63) / 64), dim3(64), 0, st, packed, row_bytes, n_samples,
+                       n_sites, flag);
+    return hipGetLastError();
+}
+
+hipError_t gt_count(const uint32_t* geno, uint32_t row_words, uint32_t n_samples, uint32_t n_sites, double* af,
+                    hipStream_t st) {
+    if (n_sites == 0) return hipSuccess;
+    hipLaunchKernelGGL(gt_count_kernel, dim3((n_sites + 63) / 64), dim3(64), 0, st, geno, row_words, n_samples,
+                       n_sites, af);
+    return hipGetLastError();
+}
+
+hipError_t gram(const double* af, uint32_t n_sites, double* z, double* q, hipStream_t st) {
+    const uint64_t rows = (uint64_t)n_sites * 9, n_blocks = rows / 8, n_tiles = (n_blocks + 31) / 32;
+    const uint32_t n_rem = (uint32_t)(rows % 8);
+    if (n_tiles)
+        hipLaunchKernelGGL(gram_tile_kernel, dim3((unsigned)((n_tiles * 8 + 63) / 64)), dim3(64), 0, st, af, n_blocks,
+                           n_tiles, z);
+    hipLaunchKernelGGL(gram_final_kernel, dim3(1), dim3(64), 0, st, af, z, n_tiles, n_blocks, n_rem, n_sites, q);
+    return hipGetLastError();
+}
+
+hipError_t cls_pairs(const uint32_t* geno, uint32_t row_words, const double* af, uint32_t n_sites, const double* q,
+                     const uint32_t* pairs, uint64_t npairs, const Tune& t, kestrel_output* out, hipStream_t st) {
+    if (npairs == 0) return hipSuccess;
+    hipLaunchKernelGGL(cls_pairs_kernel, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, st, geno, row_words, af,
+                       n_sites, q, pairs, npairs, t, out);
     return hipGetLastError();
 }
 

@@ -44,6 +44,13 @@ hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* act
 hipError_t selftest(uint64_t n, uint64_t seed, unsigned long long* bad, hipStream_t st);
 hipError_t calib_fp64(uint32_t blocks, uint32_t iters, double* sink, hipStream_t st);
 hipError_t copy(const void* src, void* dst, uint64_t n16, hipStream_t st);
+hipError_t complete_sites(const uint8_t* packed, size_t row_bytes, uint32_t n_samples, uint32_t n_sites, uint8_t* flag,
+                          hipStream_t st);
+hipError_t gt_count(const uint32_t* geno, uint32_t row_words, uint32_t n_samples, uint32_t n_sites, double* af,
+                    hipStream_t st);
+hipError_t gram(const double* af, uint32_t n_sites, double* z, double* q, hipStream_t st);
+hipError_t cls_pairs(const uint32_t* geno, uint32_t row_words, const double* af, uint32_t n_sites, const double* q,
+                     const uint32_t* pairs, uint64_t npairs, const kdev::Tune& t, kestrel_output* out, hipStream_t st);
 hipError_t finalize(const double* partial, uint32_t nseg, uint32_t nslots, uint32_t npts, double* f, double* g,
                     double* h, uint32_t* nsites, hipStream_t st);
 }  // namespace klaunch

@@ -158,3 +158,35 @@ def unpack2(packed: np.ndarray, n_sites: int) -> np.ndarray:
     b = packed.astype(np.uint8)
     out = np.stack([b & 3, (b >> 2) & 3, (b >> 4) & 3, (b >> 6) & 3], axis=-1).reshape(b.shape[0], -1)
     return out[:, :n_sites].copy()
+
+
+def write_vcf(cohort: Cohort, path: str, field: str = "GT", af_info: bool = False, names=None) -> None:
+    """Write the cohort as a VCF (plain text, or gzip when path ends in .gz).
+    GT: 0/0, 0/1, 1/1, ./. for a missing call.  GL / PL: one-hot likelihoods
+    (log10 likelihood 0 for the call and -1000 otherwise; Phred 0 and 10000),
+    '.,.,.' for a missing call.  af_info writes INFO AF = 1 - p."""
+    import gzip
+
+    codes = cohort.codes
+    n, v = codes.shape
+    names = names or cohort.names
+    gt = np.array(["0/0", "0/1", "1/1", "./."])
+    if field == "GL":
+        lik = np.array(["0,-1000,-1000", "-1000,0,-1000", "-1000,-1000,0", ".,.,."])
+    else:
+        lik = np.array(["0,10000,10000", "10000,0,10000", "10000,10000,0", ".,.,."])
+    opener = gzip.open if path.endswith(".gz") else open
+    bases = "ACGT"
+    with opener(path, "wt") as f:
+        f.write("##fileformat=VCFv4.2\n")
+        f.write('##INFO=<ID=AF,Number=A,Type=Float,Description="Allele frequency">\n')
+        f.write('##FORMAT=<ID=GT,Number=1,Type=String,Description="Genotype">\n')
+        f.write('##FORMAT=<ID=GL,Number=G,Type=Float,Description="Genotype likelihoods">\n')
+        f.write('##FORMAT=<ID=PL,Number=G,Type=Integer,Description="Phred-scaled genotype likelihoods">\n')
+        f.write("#CHROM\tPOS\tID\tREF\tALT\tQUAL\tFILTER\tINFO\tFORMAT\t" + "\t".join(names) + "\n")
+        table = gt if field == "GT" else lik
+        for s in range(v):
+            ref, alt = bases[s % 4], bases[(s + 1) % 4]
+            inf = f"AF={1.0 - cohort.p[s]:.6g}" if af_info else "."
+            calls = "\t".join(table[codes[:, s]])
+            f.write(f"1\t{s + 1}\t.\t{ref}\t{alt}\t.\tPASS\t{inf}\t{field}\t{calls}\n")

@@ -121,6 +121,14 @@ int ko_fit_pairs_codes(int order, const uint8_t* codes, uint32_t N, uint32_t V, 
                        const uint32_t* pairs, size_t npairs, uint32_t nsel, const uint32_t* seg, size_t nseg,
                        uint64_t iterations, int nthreads, ko_output* out);
 
+/* CLS / Brier path of -f GT (coefficients.rs:166-320, cls.rs, quadratic.rs,
+ * wrapper.rs:412-428).  codes N x V without missing calls. */
+void ko_gt_allele_counts(const uint8_t* codes, uint32_t N, uint32_t V, double* af);
+void ko_cls_m_matrices(const double* af, uint32_t V, double* m);
+void ko_cls_gram(const double* m, uint32_t V, double* q);
+int ko_cls_fit_pairs(const uint8_t* codes, uint32_t N, uint32_t V, const double* af, const uint32_t* pairs,
+                     size_t npairs, uint32_t nsel, uint64_t iterations, int nthreads, ko_output* out, double* q_out);
+
 #ifdef __cplusplus
 }
 #endif

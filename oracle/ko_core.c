@@ -20,11 +20,11 @@
 /* ---------------------------------------------------------------- dispatch */
 
 static const ko_lane_ops OPS_AVX512 = {ko_lane_log_avx512, ko_jacq_obj_lv_avx512, ko_jacq_gh_lv_avx512,
-                                       ko_allele_obj2_avx512, ko_allele_gh2_avx512, "avx512"};
+                                       ko_allele_obj2_avx512, ko_allele_gh2_avx512, ko_gram_lv_avx512, "avx512"};
 static const ko_lane_ops OPS_AVX2 = {ko_lane_log_avx2, ko_jacq_obj_lv_avx2, ko_jacq_gh_lv_avx2,
-                                     ko_allele_obj2_avx2, ko_allele_gh2_avx2, "avx2"};
+                                     ko_allele_obj2_avx2, ko_allele_gh2_avx2, ko_gram_lv_avx2, "avx2"};
 static const ko_lane_ops OPS_BASE = {ko_lane_log_base, ko_jacq_obj_lv_base, ko_jacq_gh_lv_base,
-                                     ko_allele_obj2_base, ko_allele_gh2_base, "base"};
+                                     ko_allele_obj2_base, ko_allele_gh2_base, ko_gram_lv_base, "base"};
 
 /* simd/simd.rs:15-50: AVX-512 (F/BW/CD/DQ/VL), else AVX2+FMA, else scalar.
  * KO_ISA=base|avx2|avx512 in the environment pins a variant (for tests). */
@@ -1031,5 +1031,170 @@ int ko_fit_pairs_codes(int order, const uint8_t* codes, uint32_t N, uint32_t V, 
     for (int i = 0; i < nt; i++) pthread_create(&th[i], NULL, fit_worker, &j);
     for (int i = 0; i < nt; i++) pthread_join(th[i], NULL);
     free(th);
+    return 0;
+}
+
+/* ------------------------------------------------- CLS (Brier) path, -f GT */
+
+/* wrapper.rs:412-428: allele frequencies by counting (no missing calls on
+ * this path: vcf.rs:217-221 drops such sites). codes N x V. af V x 2. */
+void ko_gt_allele_counts(const uint8_t* codes, uint32_t N, uint32_t V, double* af) {
+    for (uint32_t s = 0; s < V; s++) {
+        uint64_t f0 = 0, f1 = 0;
+        for (uint32_t i = 0; i < N; i++) {
+            const uint8_t g = codes[(size_t)i * V + s];
+            /* genotype index 0 = (0,0), 1 = (0,1), 2 = (1,1) */
+            f0 += g == 0 ? 2 : g == 1 ? 1 : 0;
+            f1 += g == 2 ? 2 : g == 1 ? 1 : 0;
+        }
+        const double tot = (double)(2 * (uint64_t)N);
+        af[2 * s] = (double)f0 / tot;
+        af[2 * s + 1] = (double)f1 / tot;
+    }
+}
+
+/* cls.rs:6-26 with conditional.rs IIS_2: m[s][gx][gy][9] */
+void ko_cls_m_matrices(const double* af, uint32_t V, double* m) {
+    static const int GI[3][2] = {{0, 0}, {0, 1}, {1, 1}};
+    for (uint32_t s = 0; s < V; s++)
+        for (int gx = 0; gx < 3; gx++)
+            for (int gy = 0; gy < 3; gy++) {
+                const int i = GI[gx][0], j = GI[gx][1], k = GI[gy][0], l = GI[gy][1];
+                ko_conditional_probabilities(af[2 * s + i], af[2 * s + j], af[2 * s + k], af[2 * s + l],
+                                             ko_iis_mode(i, j, k, l), m + ((size_t)s * 9 + gx * 3 + gy) * 9);
+            }
+}
+
+/* coefficients.rs:280-320 compute_quadratic_matrix for biallelic sites: all
+ * 9 rows of every site pushed in (gx, gy) order into an 8-lane vector, then
+ * quadratic.rs:12-33 divided by V. */
+void ko_cls_gram(const double* m, uint32_t V, double* q) {
+    const size_t rows = (size_t)V * 9;
+    ko_lanevec lv;
+    lv.nblocks = rows / 8;
+    lv.nrem = rows % 8;
+    lv.blocks = (double*)malloc((lv.nblocks ? lv.nblocks : 1) * 72 * sizeof(double));
+    for (size_t b = 0; b < lv.nblocks; b++)
+        for (int j = 0; j < 8; j++)
+            for (int c = 0; c < 9; c++) lv.blocks[b * 72 + c * 8 + j] = m[(b * 8 + j) * 9 + c];
+    for (size_t r = 0; r < lv.nrem; r++)
+        for (int c = 0; c < 9; c++) lv.rem[r * 9 + c] = m[(lv.nblocks * 8 + r) * 9 + c];
+    ko_ops()->gram(&lv, (double)V, q);
+    free(lv.blocks);
+}
+
+/* coefficients.rs:216-276 for one pair: c = sum over sites of m[gx][gy]
+ * (cls.rs:28-49, sequential), scaled by -1/V; QP from 1/9 (or the 2-variable
+ * (delta1, delta7) QP from (1/2, 1/2) for a self pair); brier, gradient and
+ * condition.  q is 9 x 9 row-major; qp cap = iterations. */
+void ko_cls_pair(const double* m, const double* q, const uint8_t* gx, const uint8_t* gy, uint32_t V, int self_pair,
+                 uint64_t iterations, ko_output* out) {
+    double c[9] = {0};
+    for (uint32_t s = 0; s < V; s++) {
+        const double* row = m + ((size_t)s * 9 + gx[s] * 3 + gy[s]) * 9;
+        for (int k = 0; k < 9; k++) c[k] = c[k] + row[k];
+    }
+    const double nv = -(double)V;
+    for (int k = 0; k < 9; k++) c[k] = c[k] / nv;
+    ko_tune t;
+    ko_tune_new(&t, iterations, iterations);
+    double Q[MX * MX];
+    memcpy(Q, q, sizeof Q);
+    double delta[9];
+    uint64_t iters;
+    if (self_pair) {
+        double q2[MX * MX] = {0};
+        q2[0] = Q[0];
+        q2[1] = Q[6];
+        q2[MX] = Q[6 * MX];
+        q2[MX + 1] = Q[6 * MX + 6];
+        double c2[2] = {c[0], c[6]}, y0[2] = {0.5, 0.5}, y[2];
+        iters = qp(2, q2, c2, y0, 1, &t, y);
+        for (int k = 0; k < 9; k++) delta[k] = 0.0;
+        delta[0] = y[0];
+        delta[6] = y[1];
+    } else {
+        double y0[9];
+        for (int k = 0; k < 9; k++) y0[k] = 1.0 / 9.0;
+        iters = qp(9, Q, c, y0, 1, &t, delta);
+    }
+    double qd[9], g[9];
+    for (int i = 0; i < 9; i++) qd[i] = dotn(9, Q + i * MX, delta);
+    const double brier = dotn(9, delta, qd) + 2.0 * dotn(9, c, delta) + 1.0;
+    for (int i = 0; i < 9; i++) g[i] = qd[i] + c[i];
+    out->cond = condition(9, delta, g, Q, t.sqp_conv_tol);
+    memcpy(out->jacquard, delta, sizeof delta);
+    out->obj = brier;
+    out->iters = iters;
+    out->n_sites = V;
+}
+
+typedef struct {
+    const uint8_t* codes;
+    uint32_t N, V;
+    const double* m;
+    const double* q;
+    const uint32_t* pairs;
+    size_t npairs;
+    uint32_t nsel;
+    uint64_t iterations;
+    ko_output* out;
+    atomic_size_t next;
+} cls_job;
+
+static void* cls_worker(void* vj) {
+    cls_job* j = (cls_job*)vj;
+    for (;;) {
+        size_t i = atomic_fetch_add(&j->next, 1);
+        if (i >= j->npairs) break;
+        uint32_t x, y;
+        if (j->pairs) {
+            x = j->pairs[2 * i];
+            y = j->pairs[2 * i + 1];
+        } else
+            tri_pair(i, j->nsel, &x, &y);
+        ko_output* o = j->out + i;
+        memset(o, 0, sizeof *o);
+        o->x = x;
+        o->y = y;
+        ko_cls_pair(j->m, j->q, j->codes + (size_t)x * j->V, j->codes + (size_t)y * j->V, j->V, x == y,
+                    j->iterations, o);
+    }
+    return NULL;
+}
+
+/* coefficients.rs:166-278 calculate_coefficients_gt for biallelic codes
+ * (no missing calls). af V x 2 (NULL = counted). */
+int ko_cls_fit_pairs(const uint8_t* codes, uint32_t N, uint32_t V, const double* af_in, const uint32_t* pairs,
+                     size_t npairs, uint32_t nsel, uint64_t iterations, int nthreads, ko_output* out, double* q_out) {
+    if (nsel > N) return -1;
+    double* af = (double*)malloc(sizeof(double) * 2 * (V ? V : 1));
+    if (af_in) memcpy(af, af_in, sizeof(double) * 2 * V);
+    else ko_gt_allele_counts(codes, N, V, af);
+    double* m = (double*)malloc(sizeof(double) * 81 * (V ? V : 1));
+    ko_cls_m_matrices(af, V, m);
+    double q[81];
+    ko_cls_gram(m, V, q);
+    if (q_out) memcpy(q_out, q, sizeof q);
+    cls_job j;
+    memset(&j, 0, sizeof j);
+    j.codes = codes;
+    j.N = N;
+    j.V = V;
+    j.m = m;
+    j.q = q;
+    j.pairs = pairs;
+    j.nsel = nsel;
+    j.npairs = pairs ? npairs : (size_t)nsel * (nsel + 1) / 2;
+    j.iterations = iterations;
+    j.out = out;
+    atomic_init(&j.next, 0);
+    int nt = nthreads_default(nthreads);
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nt);
+    for (int i = 0; i < nt; i++) pthread_create(&th[i], NULL, cls_worker, &j);
+    for (int i = 0; i < nt; i++) pthread_join(th[i], NULL);
+    free(th);
+    free(m);
+    free(af);
     return 0;
 }

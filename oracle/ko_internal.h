@@ -20,7 +20,8 @@ typedef struct {
     double ko_allele_obj2_##sfx(const double* blocks, size_t nb, const double* rem, size_t nrem,              \
                                 const double x[2], double eps);                                                 \
     void ko_allele_gh2_##sfx(const double* blocks, size_t nb, const double* rem, size_t nrem, const double x[2], \
-                             double eps, double g[2], double h[4]);
+                             double eps, double g[2], double h[4]);                                          \
+    void ko_gram_lv_##sfx(const ko_lanevec* lv, double n, double out[81]);
 
 KO_DECL_VARIANT(avx512)
 KO_DECL_VARIANT(avx2)
@@ -32,6 +33,7 @@ typedef struct {
     void (*jacq_gh)(const ko_lanevec*, const double*, double, double*, double*);
     double (*allele_obj)(const double*, size_t, const double*, size_t, const double*, double);
     void (*allele_gh)(const double*, size_t, const double*, size_t, const double*, double, double*, double*);
+    void (*gram)(const ko_lanevec*, double, double*);
     const char* name;
 } ko_lane_ops;
 

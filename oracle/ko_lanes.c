@@ -297,3 +297,34 @@ void FN(ko_allele_gh2_)(const double* blocks, size_t nb, const double* rem, size
     h[2] = h01;
     h[3] = h11;
 }
+
+/* quadratic.rs:12-365 compute_gram_matrix: lanes part in tiles of 32 blocks
+ * (per-lane partials from zero, added into running lane sums), radd, scalar
+ * remainder as one partial tile; divided by n (the number of sites, not rows). */
+void FN(ko_gram_lv_)(const ko_lanevec* lv, double n, double out[81]) {
+    double h[45][8];
+    memset(h, 0, sizeof h);
+    size_t full = lv->nblocks / 32, part = lv->nblocks % 32;
+    for (size_t t = 0; t <= full; t++) {
+        const size_t nb = t < full ? 32 : part;
+        const double* blocks = lv->blocks + t * 32 * 72;
+        int id = 0;
+        for (int a = 0; a < 9; a++)
+            for (int c = a; c < 9; c++, id++) {
+                double z[8];
+                for (int j = 0; j < 8; j++) z[j] = 0.0;
+                for (size_t b = 0; b < nb; b++)
+                    for (int j = 0; j < 8; j++) z[j] = fma(blocks[b * 72 + a * 8 + j], blocks[b * 72 + c * 8 + j], z[j]);
+                for (int j = 0; j < 8; j++) h[id][j] = h[id][j] + z[j];
+            }
+    }
+    int id = 0;
+    for (int a = 0; a < 9; a++)
+        for (int c = a; c < 9; c++, id++) {
+            double z = 0.0;
+            for (size_t r = 0; r < lv->nrem; r++) z = fma(lv->rem[r * 9 + a], lv->rem[r * 9 + c], z);
+            const double v = (radd8(h[id]) + (0.0 + z)) / n;
+            out[a * 9 + c] = v;
+            out[c * 9 + a] = v;
+        }
+}

@@ -284,3 +284,38 @@ def fit_pairs(codes, af, pairs=None, nsel=None, order=ORDER_REF, seg=None, itera
     if rc != 0:
         raise ValueError("requested samples exceed total samples")
     return out
+
+
+def gt_allele_counts(codes):
+    """wrapper.rs:412-428 allele frequencies by counting (GT path)."""
+    codes = np.ascontiguousarray(codes, np.uint8)
+    n, v = codes.shape
+    af = np.zeros((v, 2))
+    L = lib()
+    L.ko_gt_allele_counts.argtypes = [P, U32, U32, P]
+    L.ko_gt_allele_counts(_p(codes), n, v, _p(af))
+    return af
+
+
+def cls_fit_pairs(codes, af=None, pairs=None, nsel=None, iterations=100, nthreads=0):
+    """coefficients.rs:166-278 (CLS / Brier estimator of -f GT) for biallelic codes
+    without missing calls.  Returns (outputs, Q)."""
+    codes = np.ascontiguousarray(codes, np.uint8)
+    n, v = codes.shape
+    if nsel is None:
+        nsel = n
+    if pairs is not None:
+        pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        npairs = pairs.shape[0]
+    else:
+        npairs = nsel * (nsel + 1) // 2
+    afp = f64(af) if af is not None else None
+    out = np.zeros(npairs, OUTPUT_DTYPE)
+    q = np.zeros((9, 9))
+    L = lib()
+    L.ko_cls_fit_pairs.restype = I
+    L.ko_cls_fit_pairs.argtypes = [P, U32, U32, P, P, SZ, U32, U64, I, P, P]
+    rc = L.ko_cls_fit_pairs(_p(codes), n, v, _p(afp), _p(pairs), npairs, nsel, iterations, nthreads, _p(out), _p(q))
+    if rc != 0:
+        raise ValueError("requested samples exceed total samples")
+    return out, q

@@ -1,0 +1,119 @@
+"""CPU tests of the kestrel CLI's VCF decoding (vcf.rs:96-482 rules) and of
+its TSV float text (ryu, the csv crate's f64 writer).  Decoding is compared
+bit-exactly: codes 0/1/2 = unphased genotype index (coefficients.rs:18-38),
+3 = missing."""
+import gzip
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from kestrel_amd import synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+KESTREL = os.path.join(ROOT, "kestrel_amd", "kestrel")
+
+HEADER = ("##fileformat=VCFv4.2\n#CHROM\tPOS\tID\tREF\tALT\tQUAL\tFILTER\tINFO\tFORMAT\tA\tB\tC\n")
+
+
+def dump(path, field="GT", af_info=False, expect_ok=True):
+    cmd = [KESTREL, path, "/dev/null", "-f", field, "--dump-codes"] + (["-a", "info"] if af_info else [])
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if not expect_ok:
+        assert r.returncode != 0
+        return r.stderr
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.rstrip("\n").split("\n")
+    samples = lines[0].split("\t")
+    sites = [ln.split("\t") for ln in lines[1:]]
+    counts = dict(kv.split("=") for kv in r.stderr.strip().split("\n")[-1].split())
+    return samples, sites, {k: int(v) for k, v in counts.items()}
+
+
+def write(tmp_path, body, name="t.vcf"):
+    p = tmp_path / name
+    p.write_text(HEADER + body)
+    return str(p)
+
+
+def test_gt_decoding_rules(tmp_path):
+    body = (
+        "1\t1\t.\tA\tG\t.\tPASS\t.\tGT\t0/0\t0|1\t1/1\n"       # kept
+        "1\t2\t.\tA\tG\t.\tPASS\t.\tGT:DP\t1/0:5\t1|1:3\t0/0:1\n"  # unphased index, extra FORMAT key
+        "1\t3\t.\tAT\tG\t.\tPASS\t.\tGT\t0/0\t0/1\t1/1\n"      # REF not a single base: not a SNP
+        "1\t4\t.\tA\tGC\t.\tPASS\t.\tGT\t0/0\t0/1\t1/1\n"      # ALT not a single base
+        "1\t5\t.\tA\t.\t.\tPASS\t.\tGT\t0/0\t0/0\t0/0\n"       # no ALT: one allele
+        "1\t6\t.\ta\tt\t.\tPASS\t.\tGT\t./.\t0/1\t1/1\n"       # missing call drops the site (vcf.rs:217-221)
+        "1\t7\t.\tC\tT\t.\tPASS\t.\tGT\t0/1\t.|1\t0/0\n"       # half-missing too
+        "1\t8\t.\tc\tg\t.\tPASS\t.\tGT\t1/1\t0/1\t0/1\n"       # lower-case bases are SNPs
+    )
+    samples, sites, counts = dump(write(tmp_path, body))
+    assert samples == ["A", "B", "C"]
+    assert [s[0] for s in sites] == ["012", "120", "211"]
+    assert counts == {"total": 8, "not_snp": 3, "skipped_missing": 2}
+
+
+def test_gt_errors(tmp_path):
+    err = dump(write(tmp_path, "1\t1\t.\tA\tG\t.\tPASS\t.\tGT\t0\t0/1\t1/1\n"), expect_ok=False)
+    assert "not diploid" in err
+    err = dump(write(tmp_path, "1\t1\t.\tA\tG\t.\tPASS\t.\tGT\t0/1/1\t0/1\t1/1\n"), expect_ok=False)
+    assert "not diploid" in err
+    err = dump(write(tmp_path, "1\t1\t.\tA\tG,T\t.\tPASS\t.\tGT\t0/2\t0/1\t1/1\n"), expect_ok=False)
+    assert "tri- and tetra-allelic" in err
+    err = dump(write(tmp_path, "1\t1\t.\tA\tG\t.\tPASS\t.\tDP\t1\t2\t3\n"), expect_ok=False)
+    assert "No GT data found" in err
+
+
+def test_gl_pl_decoding(tmp_path):
+    body = (
+        "1\t1\t.\tA\tG\t.\tPASS\tAF=0.25\tGL:PL\t0,-1000,-1000:0,9999,9999\t-1000,0,-1000:9999,0,9999\t"
+        "-2000,-2000,-1000:9999,9999,0\n"
+        "1\t2\t.\tA\tG\t.\tPASS\tAF=0.5\tGL:PL\t.,.,.:.,.,.\t-1000,0,-1000:9999,0,9999\t0,-1000,.:0,9999,.\n"
+    )
+    p = write(tmp_path, body)
+    for field in ("GL", "PL"):
+        _, sites, _ = dump(p, field, af_info=True)
+        # a NaN in any likelihood drops the row (vcf.rs:355,397; lanevector.rs:100-104)
+        assert [s[0] for s in sites] == ["012", "313"]
+        assert sites[0][1:] == ["0.75", "0.25"] and sites[1][1:] == ["0.5", "0.5"]
+    soft = write(tmp_path, "1\t1\t.\tA\tG\t.\tPASS\t.\tPL\t0,30,300\t0,9999,9999\t0,9999,9999\n", "s.vcf")
+    assert "not one-hot" in dump(soft, "PL", expect_ok=False)
+    bad = write(tmp_path, "1\t1\t.\tA\tG\t.\tPASS\t.\tGL\t0,-1000\t0,-1,-1\t0,-1,-1\n", "b.vcf")
+    assert "Not enough genotype likelihoods" in dump(bad, "GL", expect_ok=False)
+
+
+def test_synthetic_vcf_roundtrip_gz(tmp_path):
+    c = synth.generate(17, 300, seed=9, missing=0.05)
+    for field in ("GT", "GL", "PL"):
+        path = str(tmp_path / f"c.{field}.vcf.gz")
+        synth.write_vcf(c, path, field)
+        samples, sites, counts = dump(path, field)
+        assert samples == c.names
+        got = np.array([[int(ch) for ch in s[0]] for s in sites], np.uint8).T
+        if field == "GT":
+            keep = (c.codes != 3).all(0)
+            assert np.array_equal(got, c.codes[:, keep])
+            assert counts["skipped_missing"] == int((~keep).sum())
+        else:
+            assert np.array_equal(got, c.codes)
+
+
+def test_float_text_matches_ryu():
+    vals = ["0.25", "1e-8", "1.5e-7", "100", "1e15", "1e16", "0.00001", "0.000001", "123456789012345680",
+            "-0.5", "nan", "inf", "-inf", "0.1", "0.30000000000000004", "5e-324", "1.7976931348623157e308"]
+    exp = ["0.25", "1e-8", "1.5e-7", "100.0", "1000000000000000.0", "1e16", "0.00001", "1e-6",
+           "1.2345678901234568e17", "-0.5", "NaN", "inf", "-inf", "0.1", "0.30000000000000004", "5e-324",
+           "1.7976931348623157e308"]
+    r = subprocess.run([KESTREL, "--format-floats"], input="\n".join(vals) + "\n", capture_output=True, text=True,
+                       check=True)
+    assert r.stdout.split("\n")[:-1] == exp
+
+
+def test_usage_errors():
+    r = subprocess.run([KESTREL, "only_one_arg"], capture_output=True, text=True)
+    assert r.returncode == 2 and "required" in r.stderr
+    r = subprocess.run([KESTREL, "a", "b", "-f", "XX"], capture_output=True, text=True)
+    assert r.returncode == 2 and "invalid value" in r.stderr
+    r = subprocess.run([KESTREL, "/nonexistent.vcf", "out.tsv"], capture_output=True, text=True)
+    assert r.returncode == 1 and "No such file" in r.stderr

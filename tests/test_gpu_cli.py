@@ -1,0 +1,88 @@
+"""End-to-end drop-in test of the kestrel CLI on the GPU: VCF in, TSV out
+(cli.rs:129-181 columns), against the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from kestrel_amd import segment_bounds, synth
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+KESTREL = os.path.join(ROOT, "kestrel_amd", "kestrel")
+KIN = np.array([1, 0, .5, 0, .5, 0, .5, .25, 0])
+COLS = ["sample1", "sample2", "kinship"] + [f"jacquard{i}" for i in range(1, 10)] + ["iterations", "objective",
+                                                                                        "condition"]
+
+
+def run(vcf, out, *args):
+    r = subprocess.run([KESTREL, vcf, out, *args], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    lines = open(out).read().rstrip("\n").split("\n")
+    assert lines[0].split("\t") == COLS
+    rows = [ln.split("\t") for ln in lines[1:]]
+    return rows, r.stderr
+
+
+def as_arrays(rows, names):
+    idx = {n: i for i, n in enumerate(names)}
+    x = np.array([idx[r[0]] for r in rows])
+    y = np.array([idx[r[1]] for r in rows])
+    kin = np.array([float(r[2]) for r in rows])
+    jac = np.array([[float(v) for v in r[3:12]] for r in rows])
+    it = np.array([int(r[12]) for r in rows])
+    obj = np.array([float(r[13]) for r in rows])
+    cond = np.array([float(r[14]) for r in rows])
+    return x, y, kin, jac, it, obj, cond
+
+
+def test_cli_gt_cls_bitexact(O, tmp_path):
+    print("case: cli -f GT", flush=True)
+    c = synth.generate(30, 2500, seed=21, n_family_pairs=6, missing=0.002)
+    vcf = str(tmp_path / "c.vcf.gz")
+    synth.write_vcf(c, vcf, "GT")
+    rows, log = run(vcf, str(tmp_path / "o.tsv"), "-i", "100")
+    x, y, kin, jac, it, obj, cond = as_arrays(rows, c.names)
+    keep = (c.codes != 3).all(0)
+    exp, _ = O.cls_fit_pairs(c.codes[:, keep])
+    assert np.array_equal(x, exp["x"]) and np.array_equal(y, exp["y"])
+    assert np.array_equal(jac, exp["jacquard"]) and np.array_equal(it, exp["iters"])
+    assert np.array_equal(obj, exp["obj"])
+    assert np.array_equal(cond, np.log10(exp["cond"]), equal_nan=True)
+    kin_exp = np.zeros(len(exp))
+    for k in range(9):
+        kin_exp = kin_exp + exp["jacquard"][:, k] * KIN[k]
+    assert np.array_equal(kin, kin_exp)
+    assert "Wall Time" in log
+
+
+@pytest.mark.parametrize("field", ["GL", "PL"])
+def test_cli_ml_matches_oracle(O, tmp_path, field):
+    print(f"case: cli -f {field}", flush=True)
+    c = synth.generate(28, 3000, seed=22, n_family_pairs=6, missing=0.02)
+    vcf = str(tmp_path / f"c.{field}.vcf")
+    synth.write_vcf(c, vcf, field)
+    rows, _ = run(vcf, str(tmp_path / "o.tsv"), "-f", field, "-s", "20")
+    x, y, kin, jac, it, obj, cond = as_arrays(rows, c.names)
+    assert len(rows) == 20 * 21 // 2 and x.max() < 20
+    af = O.allele_freqs(c.codes)  # frequencies use every sample (cli.rs:41-45)
+    kept = O.keep_sites(af)
+    codes, afk = c.codes[:, kept], af[kept]
+    seg = np.minimum(segment_bounds(int(kept.sum())), codes.shape[1]).astype(np.uint32)
+    exp = O.fit_pairs(codes, afk, nsel=20, order=O.ORDER_SEG, seg=seg)
+    assert np.array_equal(jac, exp["jacquard"]) and np.array_equal(it, exp["iters"])
+    assert np.array_equal(obj, exp["obj"])
+    assert np.array_equal(cond, np.log10(exp["cond"]), equal_nan=True)
+    ref = O.fit_pairs(codes, afk, nsel=20, order=O.ORDER_REF)
+    lc = np.log10(ref["cond"])
+    well = np.isfinite(lc) & (lc < 12)
+    assert np.abs(jac - ref["jacquard"]).max(1)[well].max() <= 1e-6
+
+
+def test_cli_samples_beyond_cohort(tmp_path):
+    c = synth.generate(5, 200, seed=23, missing=0.0)
+    vcf = str(tmp_path / "c.vcf")
+    synth.write_vcf(c, vcf, "GT")
+    r = subprocess.run([KESTREL, vcf, str(tmp_path / "o.tsv"), "-s", "6"], capture_output=True, text=True)
+    assert r.returncode != 0 and "exceeds total samples" in r.stderr
