@@ -136,6 +136,56 @@ int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value);
  * launch durations by HIP events), number of sweeps, wall milliseconds. */
 int kestrel_last_fit_stats(const kestrel_cohort* c, double* sweep_ms, uint64_t* sweeps, double* wall_ms);
 
+/* Serving: the cohort stays device-resident and answers requests of query
+ * samples (new 2-bit rows over the cohort's n_sites, fitted with the cohort's
+ * allele frequencies and kept sites) against the whole cohort or a subset of
+ * it.  The reference fits one batch per run and has no counterpart; each pair
+ * is the fit of coefficients.rs:40-141 (start point 1/9, sqp.rs:49-93).
+ *
+ * Requests are admitted while others run (continuous batching), first come
+ * first served, when a request slot, max_query_rows rows and max_pairs output
+ * records (the device memory fixed at kestrel_serve_start) have room; the
+ * admitted requests share the device pair by pair.  A pair's result does not
+ * depend on what else is running: it is bit-identical alone and in any mix.
+ *
+ *   kestrel_serve_start     starts the engine thread (tune fixed per engine).
+ *   kestrel_submit_queries  query_packed: n_query rows of row_stride_bytes;
+ *                           cohort_subset: n_subset cohort sample indices, or
+ *                           NULL for all samples.  out (caller-owned, kept
+ *                           alive until the request is done) receives
+ *                           n_query x n_subset records, out[q * n_subset + j]
+ *                           with x = q (query index) and y = cohort index.
+ *                           The query rows are copied; returns at once.
+ *   kestrel_poll            *done = 1 once out is written; with latency_ms
+ *                           (submission to completion).  A finished request is
+ *                           forgotten after it was reported done once.
+ *   kestrel_wait            blocks until the request is done.
+ *   kestrel_serve_stop      fails pending and running requests, stops the
+ *                           engine, frees its device memory (also done by
+ *                           kestrel_cohort_destroy).
+ * Submit, poll, wait and stats are internally synchronised and may come from
+ * any thread; kestrel_serve_stop and kestrel_cohort_destroy must not overlap
+ * another call on the cohort. */
+typedef struct {
+    uint64_t requests_done;
+    uint64_t pairs_done;
+    uint64_t sweeps;
+    uint64_t busy_slot_sweeps; /* sum over sweeps of the slots holding a pair */
+    uint64_t device_bytes;     /* device memory held by the engine */
+    uint32_t slots;            /* pairs in flight at most */
+    uint32_t live;             /* admitted, unfinished requests */
+    uint32_t pending;          /* submitted, not yet admitted */
+    uint32_t reserved;
+} kestrel_serve_stats;
+
+int kestrel_serve_start(kestrel_cohort* c, const kestrel_tune* tune, uint32_t max_query_rows, uint64_t max_pairs);
+int kestrel_submit_queries(kestrel_cohort* c, const uint8_t* query_packed, uint32_t n_query, size_t row_stride_bytes,
+                           const uint32_t* cohort_subset, size_t n_subset, kestrel_output* out, uint64_t* request_id);
+int kestrel_poll(kestrel_cohort* c, uint64_t request_id, int* done, double* latency_ms);
+int kestrel_wait(kestrel_cohort* c, uint64_t request_id, double* latency_ms);
+int kestrel_serve_get_stats(kestrel_cohort* c, kestrel_serve_stats* stats);
+int kestrel_serve_stop(kestrel_cohort* c);
+
 void kestrel_cohort_destroy(kestrel_cohort* c);
 
 /* Device arithmetic self-test (no counterpart in the reference): compares, on

@@ -58,7 +58,24 @@ EXPORTS = (
     "kestrel_fit_pairs", "kestrel_evaluate", "kestrel_set_option", "kestrel_last_fit_stats",
     "kestrel_cohort_destroy", "kestrel_last_error", "kestrel_version", "kestrel_selftest",
     "kestrel_measure_fp64_peak", "kestrel_measure_copy", "kestrel_cohort_create_gt", "kestrel_fit_pairs_gt",
+    "kestrel_serve_start", "kestrel_submit_queries", "kestrel_poll", "kestrel_wait", "kestrel_serve_get_stats",
+    "kestrel_serve_stop",
 )
+
+
+class ServeStats(ctypes.Structure):
+    """kestrel_serve_stats."""
+    _fields_ = [
+        ("requests_done", ctypes.c_uint64),
+        ("pairs_done", ctypes.c_uint64),
+        ("sweeps", ctypes.c_uint64),
+        ("busy_slot_sweeps", ctypes.c_uint64),
+        ("device_bytes", ctypes.c_uint64),
+        ("slots", ctypes.c_uint32),
+        ("live", ctypes.c_uint32),
+        ("pending", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
 
 _lib = None
 P = ctypes.c_void_p
@@ -87,6 +104,12 @@ def load():
     L.kestrel_fit_pairs_gt.argtypes = [P, ctypes.c_uint64, P, ctypes.c_size_t, ctypes.c_uint32, P, P]
     L.kestrel_measure_fp64_peak.argtypes = [P]
     L.kestrel_measure_copy.argtypes = [ctypes.c_uint64, P]
+    L.kestrel_serve_start.argtypes = [P, P, ctypes.c_uint32, ctypes.c_uint64]
+    L.kestrel_submit_queries.argtypes = [P, P, ctypes.c_uint32, ctypes.c_size_t, P, ctypes.c_size_t, P, P]
+    L.kestrel_poll.argtypes = [P, ctypes.c_uint64, P, P]
+    L.kestrel_wait.argtypes = [P, ctypes.c_uint64, P]
+    L.kestrel_serve_get_stats.argtypes = [P, P]
+    L.kestrel_serve_stop.argtypes = [P]
     L.kestrel_cohort_destroy.argtypes = [P]
     L.kestrel_cohort_destroy.restype = None
     _lib = L

@@ -137,6 +137,65 @@ class Cohort:
         check(_lib.load().kestrel_evaluate(self._h, ptr(pp), ptr(pts), n, eps, ptr(f), ptr(g), ptr(h), ptr(ns)))
         return f, g, h, ns
 
+    # ---- serving (kestrel_serve_start / kestrel_submit_queries / kestrel_wait)
+
+    def serve_start(self, max_query_rows: int = 1024, max_pairs: int = 1 << 24, tune: Tune | None = None,
+                    iterations: int = 100):
+        """Start the serving engine: the cohort answers requests of query
+        samples while others run (continuous batching)."""
+        self._serve_tune = tune or tune_default(iterations)
+        self._requests = {}
+        check(_lib.load().kestrel_serve_start(self._h, ctypes.byref(self._serve_tune), int(max_query_rows),
+                                              int(max_pairs)))
+
+    def submit(self, query_codes: np.ndarray | None = None, subset: np.ndarray | None = None, *,
+               query_packed: np.ndarray | None = None) -> int:
+        """Queue query samples (codes (Q, n_sites) or 2-bit rows) against the
+        cohort subset (indices) or the whole cohort; returns the request id.
+        The result of wait(id) is (Q * len(subset)) records, x = query index,
+        y = cohort index, in query-major order."""
+        if query_packed is None:
+            query_packed = pack2(np.ascontiguousarray(query_codes, np.uint8))
+        qp = np.ascontiguousarray(query_packed, np.uint8)
+        sub = None if subset is None else np.ascontiguousarray(subset, np.uint32)
+        n_sub = self.n_samples if sub is None else sub.shape[0]
+        out = np.zeros(qp.shape[0] * n_sub, OUTPUT_DTYPE)
+        rid = ctypes.c_uint64()
+        check(_lib.load().kestrel_submit_queries(self._h, ptr(qp), qp.shape[0], qp.shape[1], ptr(sub),
+                                                 0 if sub is None else sub.shape[0], ptr(out), ctypes.byref(rid)))
+        self._requests[rid.value] = out  # the engine writes into out until the request is done
+        return rid.value
+
+    def poll(self, request_id: int):
+        """(done, latency_ms, outputs or None)."""
+        done, lat = ctypes.c_int(), ctypes.c_double()
+        try:
+            check(_lib.load().kestrel_poll(self._h, request_id, ctypes.byref(done), ctypes.byref(lat)))
+        except Exception:
+            self._requests.pop(request_id, None)
+            raise
+        if not done.value:
+            return False, None, None
+        return True, lat.value, self._requests.pop(request_id)
+
+    def wait(self, request_id: int):
+        """Block until the request is done; returns (outputs, latency_ms)."""
+        lat = ctypes.c_double()
+        try:
+            check(_lib.load().kestrel_wait(self._h, request_id, ctypes.byref(lat)))
+        except Exception:
+            self._requests.pop(request_id, None)
+            raise
+        return self._requests.pop(request_id), lat.value
+
+    def serve_stats(self) -> dict:
+        st = _lib.ServeStats()
+        check(_lib.load().kestrel_serve_get_stats(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_ if k != "reserved"}
+
+    def serve_stop(self):
+        check(_lib.load().kestrel_serve_stop(self._h))
+
     def last_fit_stats(self):
         ms, sw, wall = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_double()
         check(_lib.load().kestrel_last_fit_stats(self._h, ctypes.byref(ms), ctypes.byref(sw), ctypes.byref(wall)))

@@ -188,7 +188,8 @@ __global__ __launch_bounds__(64) void af_kernel(const uint8_t* __restrict__ pack
 __global__ __launch_bounds__(256) void gather_kernel(const uint8_t* __restrict__ src, size_t src_row_bytes,
                                                       const uint32_t* __restrict__ kept, uint32_t n_kept,
                                                       uint32_t n_samples, uint32_t* __restrict__ dst,
-                                                      uint32_t dst_row_words) {
+                                                      uint32_t dst_row_words,
+                                                      const uint32_t* __restrict__ dst_rows) {
     const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= (size_t)n_samples * dst_row_words) return;
     const uint32_t i = (uint32_t)(tid / dst_row_words), w = (uint32_t)(tid % dst_row_words);
@@ -203,7 +204,8 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint8_t* __restrict__
         }
         word |= c << (2 * j);
     }
-    dst[tid] = word;
+    const size_t drow = dst_rows ? dst_rows[i] : i;
+    dst[drow * dst_row_words + w] = word;
 }
 
 // ------------------------------------------------------------ the sweep
@@ -409,12 +411,12 @@ __global__ __launch_bounds__(64) void step_kernel(SlotState* __restrict__ st, ui
                                                    const uint32_t* __restrict__ pairs, uint64_t npairs,
                                                    unsigned long long* __restrict__ next_pair,
                                                    kestrel_output* __restrict__ out, Tune tune,
-                                                   int* __restrict__ n_live) {
+                                                   int* __restrict__ n_live, PairQueue q) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= nslots) return;
     SlotState s = st[slot];
     if (s.phase == PH_RETIRED) return;
-    bool claim = s.phase == PH_IDLE;
+    bool claim = s.phase == PH_IDLE || s.phase == PH_WAIT;
     if (!claim) {
         double acc[NPART];
         fold(partial, nseg, nslots, slot, acc);
@@ -434,6 +436,10 @@ __global__ __launch_bounds__(64) void step_kernel(SlotState* __restrict__ st, ui
             o.n_sites = 0;
             o.status = KESTREL_PAIR_NO_SITES;
             out[s.pair] = o;
+            if (q.cap) {
+                atomicAdd(&q.req_done[s.req], 1u);
+                atomicSub(q.busy, 1);
+            }
             claim = true;
         } else {
             const double fe = -acc[0] / n;
@@ -514,16 +520,48 @@ __global__ __launch_bounds__(64) void step_kernel(SlotState* __restrict__ st, ui
                     o.n_sites = (uint32_t)n;
                     o.status = KESTREL_PAIR_OK;
                     out[s.pair] = o;
+                    if (q.cap) {
+                        atomicAdd(&q.req_done[s.req], 1u);
+                        atomicSub(q.busy, 1);
+                    }
                     claim = true;
                 }
             }
         }
     }
     if (claim) {
-        const unsigned long long idx = atomicAdd(next_pair, 1ull);
-        if (idx < npairs) {
-            s.pair = (int64_t)idx;
-            const uint32_t px = pairs[2 * idx], py = pairs[2 * idx + 1];
+        bool got = false;
+        uint32_t px = 0, py = 0;
+        if (q.cap) {
+            const unsigned long long idx =
+                s.phase == PH_WAIT ? (unsigned long long)s.pair : atomicAdd(next_pair, 1ull);
+            const unsigned long long tail = __atomic_load_n(q.tail, __ATOMIC_RELAXED);
+            if (idx < tail) {
+                const uint4 e = *reinterpret_cast<const uint4*>(q.ent + (idx % q.cap) * 4);
+                px = e.x;
+                py = e.y;
+                s.pair = (int64_t)e.z;
+                s.req = e.w;
+                got = true;
+                atomicAdd(q.busy, 1);
+            } else {
+                s.phase = PH_WAIT;
+                s.pair = (int64_t)idx;
+            }
+        } else {
+            const unsigned long long idx = atomicAdd(next_pair, 1ull);
+            if (idx < npairs) {
+                s.pair = (int64_t)idx;
+                px = pairs[2 * idx];
+                py = pairs[2 * idx + 1];
+                got = true;
+            } else {
+                s.phase = PH_RETIRED;
+                s.pair = -1;
+                atomicSub(n_live, 1);
+            }
+        }
+        if (got) {
             sx[slot] = px;
             sy[slot] = py;
             // coefficients.rs:88-92 start point, normalised as sqp.rs:54
@@ -538,10 +576,6 @@ __global__ __launch_bounds__(64) void step_kernel(SlotState* __restrict__ st, ui
             s.phase = PH_INIT;
             s.k = 0;
             s.bls = 0;
-        } else {
-            s.phase = PH_RETIRED;
-            s.pair = -1;
-            atomicSub(n_live, 1);
         }
     }
     active[slot] = (s.phase == PH_INIT || s.phase == PH_TRIAL) ? 1 : 0;
@@ -878,11 +912,11 @@ hipError_t af(const uint8_t* packed, size_t row_bytes, uint32_t n_samples, uint3
 }
 
 hipError_t gather(const uint8_t* src, size_t src_row_bytes, const uint32_t* kept, uint32_t n_kept,
-                  uint32_t n_samples, uint32_t* dst, uint32_t dst_row_words, hipStream_t st) {
+                  uint32_t n_samples, uint32_t* dst, uint32_t dst_row_words, hipStream_t st, const uint32_t* dst_rows) {
     const size_t n = (size_t)n_samples * dst_row_words;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, src_row_bytes, kept,
-                       n_kept, n_samples, dst, dst_row_words);
+                       n_kept, n_samples, dst, dst_row_words, dst_rows);
     return hipGetLastError();
 }
 
@@ -897,9 +931,10 @@ hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, con
 
 hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* active, uint32_t nslots,
                 const double* partial, uint32_t nseg, const uint32_t* pairs, uint64_t npairs,
-                unsigned long long* next_pair, kestrel_output* out, const Tune& t, int* n_live, hipStream_t st) {
+                unsigned long long* next_pair, kestrel_output* out, const Tune& t, int* n_live, const PairQueue& q,
+                hipStream_t st) {
     hipLaunchKernelGGL(step_kernel, dim3((nslots + 63) / 64), dim3(64), 0, st, stt, sx, sy, pt, active, nslots,
-                       partial, nseg, pairs, npairs, next_pair, out, t, n_live);
+                       partial, nseg, pairs, npairs, next_pair, out, t, n_live, q);
     return hipGetLastError();
 }
 

@@ -13,15 +13,17 @@ constexpr int SWEEP_TPB = 256;  // threads (pairs) per sweep block
 constexpr int NPART = 56;       // f, g[9], h[45], n per segment partial
 constexpr uint32_t SITE_PAD = 256;  // rows padded to a multiple of this many sites
 
-enum : uint32_t { PH_IDLE = 0, PH_INIT = 1, PH_TRIAL = 2, PH_RETIRED = 3 };
+// PH_WAIT: the slot holds a reservation (SlotState::pair) of a serving-queue
+// entry that is not published yet
+enum : uint32_t { PH_IDLE = 0, PH_INIT = 1, PH_TRIAL = 2, PH_RETIRED = 3, PH_WAIT = 4 };
 
 // Per-slot SQP state between sweeps.
 struct SlotState {
-    int64_t pair;    // index into the pair list, -1 when none
+    int64_t pair;    // output record of the pair in flight, -1 when none
     uint32_t phase;  // PH_*
     uint32_t k;      // completed SQP iterations
     uint32_t bls;    // rejected trials of the current line search
-    uint32_t pad;
+    uint32_t req;    // request slot (serving queue)
     double x[9];     // current iterate
     double f;        // objective at x
     double p[9];     // search direction
@@ -29,18 +31,34 @@ struct SlotState {
     double alpha;    // current step
 };
 
+// Serving queue: a ring of pair entries (x row, y row, output record, request
+// slot) published up to *tail; cap == 0 selects the batch pair list instead.
+// Slots reserve entry indices with one atomic increment of the claim counter,
+// which may run ahead of the tail by at most the number of slots; a slot whose
+// index is not yet published waits (PH_WAIT) and reads it at a later step.
+// After any step, every entry below min(claim counter, tail) has been read, so
+// the host may overwrite ring positions up to that point.
+struct PairQueue {
+    const uint32_t* ent;
+    uint64_t cap;
+    const unsigned long long* tail;
+    uint32_t* req_done;  // finished pairs per request slot
+    int* busy;           // slots holding a pair
+};
+
 namespace klaunch {
 hipError_t af(const uint8_t* packed, size_t row_bytes, uint32_t n_samples, uint32_t n_sites, const kdev::Tune& t,
               double maf, double* af_out, int* status, hipStream_t st);
 hipError_t gather(const uint8_t* src, size_t src_row_bytes, const uint32_t* kept, uint32_t n_kept,
-                  uint32_t n_samples, uint32_t* dst, uint32_t dst_row_words, hipStream_t st);
+                  uint32_t n_samples, uint32_t* dst, uint32_t dst_row_words, hipStream_t st,
+                  const uint32_t* dst_rows = nullptr);
 hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, const uint32_t* seg_bounds,
                  uint32_t nseg, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
                  uint32_t nslots, double eps, double* partial, hipStream_t st);
 hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* active, uint32_t nslots,
                 const double* partial, uint32_t nseg, const uint32_t* pairs, uint64_t npairs,
                 unsigned long long* next_pair, kestrel_output* out, const kdev::Tune& t, int* n_live,
-                hipStream_t st);
+                const PairQueue& q, hipStream_t st);
 hipError_t selftest(uint64_t n, uint64_t seed, unsigned long long* bad, hipStream_t st);
 hipError_t calib_fp64(uint32_t blocks, uint32_t iters, double* sink, hipStream_t st);
 hipError_t copy(const void* src, void* dst, uint64_t n16, hipStream_t st);
