@@ -79,6 +79,120 @@ def cpu_baseline(codes, af, budget_s: float):
             "isa": O.isa(), "sample": f"{m} random off-diagonal pairs of the same cohort, {dt:.1f} s"}
 
 
+def percentile(a, q):
+    return float(np.percentile(np.asarray(a, np.float64), q)) if len(a) else None
+
+
+def serving(args):
+    """Serving workload: a device-resident cohort (default 10,000 x 100,000)
+    answers requests of 1-256 new query samples against the whole cohort or a
+    random subset of 500-5,000 samples (even odds).  Phase 1 submits every
+    request at once (saturated: pair-fits/s capacity); phase 2 submits the same
+    number of fresh requests at Poisson times with a mean rate of `load` x the
+    phase-1 capacity (request latency p50/p99)."""
+    import threading
+
+    from kestrel_amd import Cohort, synth
+
+    t = time.perf_counter()
+    n_query_pool = 256
+    kw = dict(synth.CONFIGS[args.config])
+    kw["n_samples"] += n_query_pool
+    c = synth.generate(**kw)
+    n = c.codes.shape[0] - n_query_pool
+    v = c.codes.shape[1]
+    packed = synth.pack2(c.codes)
+    cohort_packed, pool = packed[:n], packed[n:]
+    del c
+    log(f"cohort {args.config}: {n} x {v} (+{n_query_pool} query samples) generated in {time.perf_counter() - t:.1f}s")
+    t = time.perf_counter()
+    co = Cohort(packed=cohort_packed, n_sites=v)
+    if args.slots:
+        co.set_option("slots", args.slots)
+    co.set_option("serve_policy", {"rr": 0, "srpt": 1}[args.policy])
+    co.serve_start(max_query_rows=1024, max_pairs=1 << 24)
+    st0 = co.serve_stats()
+    log(f"device cohort, allele frequencies and engine in {time.perf_counter() - t:.2f}s, kept {co.n_kept} sites, "
+        f"{st0['slots']} slots, engine holds {st0['device_bytes'] / 2**30:.2f} GiB")
+    rng = np.random.default_rng(0xC5)
+
+    def make_request():
+        nq = int(rng.integers(1, 257))
+        q0 = int(rng.integers(0, n_query_pool - nq + 1))
+        sub = None if rng.random() < 0.5 else np.sort(rng.choice(n, int(rng.integers(500, 5001)), replace=False))
+        return pool[q0:q0 + nq], sub
+
+    def run(reqs, arrivals):
+        """Submit reqs at arrival offsets (s); returns (wall s, pairs, latencies ms)."""
+        lat = []
+        done = threading.Event()
+
+        def waiter():
+            for i in iter(ids_q.get, None):
+                out, ms = co.wait(i)
+                lat.append(ms)
+                del out
+            done.set()
+
+        import queue
+        ids_q = queue.Queue()
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        pairs = 0
+        for (q, sub), at in zip(reqs, arrivals):
+            d = t0 + at - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            ids_q.put(co.submit(query_packed=q, subset=sub))
+            pairs += q.shape[0] * (n if sub is None else len(sub))
+        ids_q.put(None)
+        done.wait()
+        th.join()
+        return time.perf_counter() - t0, pairs, lat
+
+    # warm up: one mid-size request
+    t = time.perf_counter()
+    w = co.wait(co.submit(query_packed=pool[:16]))[0]
+    log(f"warmup request 16 x {n}: {time.perf_counter() - t:.2f}s")
+    del w
+    k = args.requests
+    sat = [make_request() for _ in range(k)]
+    dt, pairs, lat_sat = run(sat, [0.0] * k)
+    capacity = pairs / dt
+    log(f"saturated: {k} requests, {pairs} pairs in {dt:.2f}s = {capacity:.0f} pair-fits/s")
+    reqs = [make_request() for _ in range(k)]
+    mean_pairs = np.mean([q.shape[0] * (n if s is None else len(s)) for q, s in reqs])
+    rate = args.load * capacity / mean_pairs
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, k))
+    arrivals -= arrivals[0]
+    dt2, pairs2, lat = run(reqs, arrivals)
+    st = co.serve_stats()
+    occupancy = st["busy_slot_sweeps"] / max(1, st["sweeps"]) / st["slots"]
+    n_kept = co.n_kept
+    log(f"poisson: {k} requests at {rate:.3f}/s, {pairs2} pairs in {dt2:.2f}s, "
+        f"p50 {percentile(lat, 50):.0f} ms p99 {percentile(lat, 99):.0f} ms")
+    co.close()
+    line = {
+        "metric": "pair-fits/sec (serving)", "value": round(capacity, 1), "unit": "pair-fits/s", "n_gpus": 1,
+        "steps": 2 * k, "warmup": 1, "ms_per_step": round(dt / k * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.config} serving: cohort {n} samples x {v} sites ({n_kept} segregating) "
+                               f"device-resident; requests of 1-256 queries vs all or 500-5000 samples",
+                   "samples": n, "sites": v, "requests": k, "precision": "fp64", "policy": args.policy},
+        "saturated": {"requests": k, "pairs": int(pairs), "seconds": round(dt, 3),
+                      "pair_fits_per_s": round(capacity, 1),
+                      "latency_ms_p50": round(percentile(lat_sat, 50), 1),
+                      "latency_ms_p99": round(percentile(lat_sat, 99), 1)},
+        "poisson": {"requests": k, "load": args.load, "rate_per_s": round(rate, 4), "pairs": int(pairs2),
+                    "seconds": round(dt2, 3), "pair_fits_per_s": round(pairs2 / dt2, 1),
+                    "latency_ms_p50": round(percentile(lat, 50), 1),
+                    "latency_ms_p99": round(percentile(lat, 99), 1)},
+        "slot_occupancy": round(occupancy, 4),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,11 +202,21 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work (0 = skip)")
     ap.add_argument("--segment-sites", type=int, default=None)
     ap.add_argument("--slots", type=int, default=None)
+    ap.add_argument("--serving", action="store_true", help="serving workload (default config c5)")
+    ap.add_argument("--requests", type=int, default=12, help="requests per serving phase")
+    ap.add_argument("--load", type=float, default=0.7, help="Poisson offered load / saturated capacity")
+    ap.add_argument("--policy", choices=("srpt", "rr"), default="srpt",
+                    help="serving feed order: shortest remaining first or round robin")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
+    if args.serving:
+        if args.config == "c2":
+            args.config = "c5"
+        serving(args)
+        return
 
     from kestrel_amd import Cohort, synth
 

@@ -129,7 +129,9 @@ int kestrel_fit_pairs_gt(kestrel_cohort* c, uint64_t iterations, const uint32_t*
 
 /* Tuning knobs: "segment_sites" (sites per segment of a sweep, a multiple of
  * 32, default 4096; fixes the summation order, so it is a cohort property),
- * "slots" (pairs in flight, 0 = fill the device). */
+ * "slots" (pairs in flight, 0 = fill the device), "serve_policy" (order in
+ * which a serving engine started afterwards feeds admitted requests to the
+ * device: 1 = shortest remaining first (default), 0 = round robin). */
 int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value);
 
 /* Timing of the last kestrel_fit_pairs: sweep-kernel milliseconds (sum of

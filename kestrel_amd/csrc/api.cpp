@@ -195,6 +195,11 @@ int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value) {
         c->opt_segment_sites = value;
         return c->set_segments();
     }
+    if (!strcmp(key, "serve_policy")) {
+        if (value != 0 && value != 1) return fail(KESTREL_EINVAL, "serve_policy must be 0 or 1");
+        c->opt_serve_policy = value;
+        return KESTREL_OK;
+    }
     return fail(KESTREL_EINVAL, "unknown option %s", key);
 }
 

@@ -66,6 +66,7 @@ struct kestrel_cohort {
     bool have_q = false;
     int64_t opt_slots = 0;
     int64_t opt_segment_sites = 4096;
+    int64_t opt_serve_policy = 1;  // 0 round robin, 1 shortest remaining first
     double last_sweep_ms = 0, last_wall_ms = 0;
     uint64_t last_sweeps = 0;
 

@@ -5,9 +5,9 @@
 //   geno      the cohort's kept-site rows followed by max_query_rows rows for
 //             queries (rows are lent to admitted requests)
 //   ring      pair entries (x row, y row, output record, request slot); the
-//             host appends entries round-robin over the admitted requests, a
-//             bounded backlog at a time, so a small request is not queued
-//             behind a large one
+//             host appends entries of the admitted requests (shortest
+//             remaining first, or round robin) a bounded backlog at a time,
+//             so a small request is not queued behind a large one
 //   outpool   max_pairs output records, a contiguous range per request
 //   ctrl      published tail, claim counter, busy slots, done count per
 //             request slot
@@ -96,6 +96,7 @@ struct ServeState {
     hipStream_t stream = nullptr;
     uint32_t qmax = 0, n_cohort = 0, row_words = 0, nseg = 0, ns = 0;
     uint64_t out_cap = 0, ring_cap = 0, backlog = 0;
+    int policy = 1;  // 0 round robin, 1 shortest remaining first
     size_t rb = 0;  // bytes of one query row over n_sites
 
     DevBuf<uint32_t> geno, kept, seg, qrows, ring, sx, sy;
@@ -177,23 +178,16 @@ int upload(ServeState* S, Request* r) {
     return KESTREL_OK;
 }
 
-// Append entries round-robin over the live requests until the unread backlog
-// reaches S->backlog, then publish the new tail.
+// Append entries until the unread backlog reaches S->backlog, then publish
+// the new tail.  Round robin gives every live request FEED_CHUNK entries in
+// turn (processor sharing); shortest remaining first feeds the request with
+// the fewest unfed entries first (lower mean latency under load).
 int feed(ServeState* S, const std::vector<Request*>& live) {
     uint64_t room = S->backlog > S->tail - S->consumed ? S->backlog - (S->tail - S->consumed) : 0;
     if (room == 0 || live.empty()) return KESTREL_OK;
     uint64_t n = 0;
     const uint64_t t0 = S->tail;
-    size_t idle = 0;
-    while (room > 0 && idle < live.size()) {
-        Request* r = live[S->rr % live.size()];
-        S->rr++;
-        const uint64_t k = std::min<uint64_t>({FEED_CHUNK, room, r->npairs - r->fed});
-        if (k == 0) {
-            idle++;
-            continue;
-        }
-        idle = 0;
+    auto append = [&](Request* r, uint64_t k) {
         for (uint64_t i = r->fed; i < r->fed + k; i++, n++) {
             const uint64_t q = i / r->n_sub, j = i % r->n_sub;
             uint32_t* e = S->hent + 4 * n;
@@ -204,6 +198,28 @@ int feed(ServeState* S, const std::vector<Request*>& live) {
         }
         r->fed += k;
         room -= k;
+    };
+    if (S->policy == 1) {
+        std::vector<Request*> order(live);
+        std::stable_sort(order.begin(), order.end(),
+                         [](const Request* a, const Request* b) { return a->npairs - a->fed < b->npairs - b->fed; });
+        for (Request* r : order) {
+            if (room == 0) break;
+            append(r, std::min<uint64_t>(room, r->npairs - r->fed));
+        }
+    } else {
+        size_t idle = 0;
+        while (room > 0 && idle < live.size()) {
+            Request* r = live[S->rr % live.size()];
+            S->rr++;
+            const uint64_t k = std::min<uint64_t>({FEED_CHUNK, room, r->npairs - r->fed});
+            if (k == 0) {
+                idle++;
+                continue;
+            }
+            idle = 0;
+            append(r, k);
+        }
     }
     if (n == 0) return KESTREL_OK;
     // ring positions t0 .. t0+n-1 (mod ring_cap), in at most two copies
@@ -392,6 +408,7 @@ int kestrel_serve_start(kestrel_cohort* c, const kestrel_tune* tune, uint32_t ma
     S->n_cohort = c->n_samples;
     S->row_words = c->row_words;
     S->out_cap = max_pairs;
+    S->policy = (int)c->opt_serve_policy;
     S->rb = (c->n_sites + 3) / 4;
     HIPCHK(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
 

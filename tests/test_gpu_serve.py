@@ -190,3 +190,21 @@ def test_stop_fails_pending(served):
     co.serve_start(max_query_rows=48, max_pairs=40_000)
     out, _ = co.wait(co.submit(queries[:1], np.arange(3)))
     assert (out["status"] == 0).all()
+
+
+def test_round_robin_policy_same_results(served):
+    co, cohort, queries, af, kept = served
+    say("case: round-robin feed order, mix vs shortest-remaining-first alone")
+    reqs = [(queries[0:6], None), (queries[6:7], np.arange(0, N_COHORT, 3)), (queries[7:12], np.arange(50))]
+    alone = [co.wait(co.submit(q, s))[0] for q, s in reqs]
+    co.serve_stop()
+    co.set_option("serve_policy", 0)
+    co.serve_start(max_query_rows=48, max_pairs=40_000)
+    try:
+        ids = [co.submit(q, s) for q, s in reqs]
+        for i, a in zip(ids, alone):
+            assert_bitexact(co.wait(i)[0], a)
+    finally:
+        co.serve_stop()
+        co.set_option("serve_policy", 1)
+        co.serve_start(max_query_rows=48, max_pairs=40_000)
