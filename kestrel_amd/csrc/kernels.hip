@@ -218,12 +218,17 @@ namespace {
 // and, in slot 9, 1.0 marking a usable site.
 constexpr int LUT_ROWS = 12;
 constexpr int LUT_W = 10;
-constexpr int LUT_BUF = LUT_ROWS * CHUNK * LUT_W;  // doubles per buffer
+// Row stride padded by 16 bytes: rows of one site then start in different
+// LDS banks, so lanes reading different genotype-pair rows of the same site
+// do not conflict (an unpadded stride of 2560 bytes puts all 12 rows on the
+// same banks).
+constexpr int LUT_RS = CHUNK * LUT_W + 2;
+constexpr int LUT_BUF = LUT_ROWS * LUT_RS;  // doubles per buffer
 
 // Rows of the c_s table for site (p, q): the non-zero block of mixture.rs
 // mixture_row2 for one-hot likelihoods of genotype pair (gx, gy), written in
 // the same expression order, so each row equals what mixture_row2 returns.
-__device__ __forceinline__ void lut_rows(double p, double q, int part, double* buf, int site) {
+__device__ __forceinline__ void lut_rows(double p, double q, int part, double* buf, int site, double eps) {
     const double p2 = p * p, p3 = p2 * p, q2 = q * q, q3 = q2 * q;
     const double pq = p * q, p2q = p2 * q, pq2 = p * q2, p2q2 = p2 * q2;
     const double p3q = p3 * q, pq3 = p * q3;
@@ -268,10 +273,10 @@ __device__ __forceinline__ void lut_rows(double p, double q, int part, double* b
         r[0] = q; r[1] = q2; r[2] = q2; r[3] = q3; r[4] = q2; r[5] = q3; r[6] = q2; r[7] = q3; r[8] = q2 * q2;
         break;
     }
-    double* dst = buf + (row * CHUNK + site) * LUT_W;
+    double* dst = buf + row * LUT_RS + site * LUT_W;
 #pragma unroll
     for (int k = 0; k < 9; k++) dst[k] = r[k];
-    dst[9] = 1.0;
+    dst[9] = eps;  // start of the L = eps + c.D chain
 }
 
 }  // namespace
@@ -294,8 +299,13 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
     const bool live = slot < nslots && active[slot] != 0;
     if (!__syncthreads_or(live)) return;
 
-    // rows of a missing call on either side stay zero
-    for (uint32_t i = tid; i < 2 * LUT_BUF; i += SWEEP_TPB) lut[i] = 0.0;
+    // rows of a missing call on either side (3, 7, 11) stay c = 0 with 1.0
+    // as the start of the L chain, so such a site has L = 1 exactly and adds
+    // nothing to the log product, the gradient or the Hessian
+    for (uint32_t i = tid; i < 2 * LUT_BUF; i += SWEEP_TPB) {
+        const uint32_t off = i % LUT_BUF, row = off / LUT_RS, w = off % LUT_RS;
+        lut[i] = (row & 3u) == 3u && w < CHUNK * LUT_W && w % LUT_W == 9 ? 1.0 : 0.0;
+    }
     __syncthreads();
 
     double D[9];
@@ -324,8 +334,8 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
         const uint32_t s = cs + lut_site;
         const double p = af[2 * s], q = af[2 * s + 1];
         double* buf = lut + b * LUT_BUF;
-        lut_rows(p, q, lut_part, buf, lut_site);
-        if (lut_part == 0) lut_rows(p, q, 8, buf, lut_site);
+        lut_rows(p, q, lut_part, buf, lut_site, eps);
+        if (lut_part == 0) lut_rows(p, q, 8, buf, lut_site, eps);
     };
     if (s0 < s1) build(s0, 0);
     __syncthreads();
@@ -335,12 +345,19 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
         const uint2 wx = *reinterpret_cast<const uint2*>(rowx + cs / 16);
         const uint2 wy = *reinterpret_cast<const uint2*>(rowy + cs / 16);
         const double* L0 = lut + b * LUT_BUF;
+        // usable sites of the chunk: neither call missing (code 3), counted
+        // exactly as integers (lanevector.rs:100-104 NaN mask)
+        {
+            const uint32_t ux = ~((wx.x & (wx.x >> 1)) | (wy.x & (wy.x >> 1))) & 0x55555555u;
+            const uint32_t uy = ~((wx.y & (wx.y >> 1)) | (wy.y & (wy.y >> 1))) & 0x55555555u;
+            N += (double)(__popc(ux) + __popc(uy));
+        }
 #pragma unroll
         for (int k = 0; k < CHUNK; k++) {
             const uint32_t ax = (k < 16 ? wx.x : wx.y) >> (2 * (k & 15));
             const uint32_t ay = (k < 16 ? wy.x : wy.y) >> (2 * (k & 15));
             const uint32_t row = min(((ax & 3u) << 2) | (ay & 3u), 11u);
-            const double2* c2 = reinterpret_cast<const double2*>(L0 + (row * CHUNK + k) * LUT_W);
+            const double2* c2 = reinterpret_cast<const double2*>(L0 + row * LUT_RS + k * LUT_W);
             double cv[10];
 #pragma unroll
             for (int j = 0; j < 5; j++) {
@@ -348,19 +365,18 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
                 cv[2 * j] = v.x;
                 cv[2 * j + 1] = v.y;
             }
-            const double vd = cv[9];
-            // L = eps + c.D as an fma chain over columns 0..8 (objective.rs:58-63)
-            double L = eps;
+            // L = eps + c.D as an fma chain over columns 0..8 (objective.rs:58-63);
+            // column 9 holds eps, or 1.0 on a missing row
+            double L = cv[9];
 #pragma unroll
             for (int j = 0; j < 9; j++) L = fma(D[j], cv[j], L);
             // sum of log L over a 16-site group = log of the product of the
             // usable sites' L (a missing site multiplies by exactly 1.0)
-            P = P * fma(vd, L, 1.0 - vd);
+            P = P * L;
             if ((k & (LOG_GROUP - 1)) == LOG_GROUP - 1) {
                 F = F + sleef_log_pos(P);
                 P = 1.0;
             }
-            N += vd;
             const double r = recip_rn(L);  // grad_hess.rs:122
 #pragma unroll
             for (int j = 0; j < 9; j++) G[j] = fma(cv[j], r, G[j]);
@@ -804,7 +820,7 @@ __global__ __launch_bounds__(256) void cls_pairs_kernel(const uint32_t* __restri
         for (int w = 0; w < (lut_part == 0 ? 2 : 1); w++) {
             const int pr = parts[w];
             const int gx = pr / 3, gy = pr % 3;
-            double* dst = buf + ((gx * 4 + gy) * CHUNK + lut_site) * LUT_W;
+            double* dst = buf + (gx * 4 + gy) * LUT_RS + lut_site * LUT_W;
             if (s < n_sites) m_row(af[2 * s], af[2 * s + 1], gx, gy, dst);
             else
                 for (int k = 0; k < 9; k++) dst[k] = 0.0;
@@ -830,7 +846,7 @@ __global__ __launch_bounds__(256) void cls_pairs_kernel(const uint32_t* __restri
                 const uint32_t ax = (k < 16 ? wx.x : wx.y) >> (2 * (k & 15));
                 const uint32_t ay = (k < 16 ? wy.x : wy.y) >> (2 * (k & 15));
                 const uint32_t row = min(((ax & 3u) << 2) | (ay & 3u), 11u);
-                const double2* r2 = reinterpret_cast<const double2*>(L0 + (row * CHUNK + k) * LUT_W);
+                const double2* r2 = reinterpret_cast<const double2*>(L0 + row * LUT_RS + k * LUT_W);
                 double m[10];
 #pragma unroll
                 for (int j = 0; j < 5; j++) {
