@@ -330,20 +330,38 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
 
     const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
     const int lut_site = tid >> 3, lut_part = tid & 7;
-    auto build = [&](uint32_t cs, int b) {
-        const uint32_t s = cs + lut_site;
-        const double p = af[2 * s], q = af[2 * s + 1];
+    auto build = [&](double p, double q, int b) {
         double* buf = lut + b * LUT_BUF;
         lut_rows(p, q, lut_part, buf, lut_site, eps);
         if (lut_part == 0) lut_rows(p, q, 8, buf, lut_site, eps);
     };
-    if (s0 < s1) build(s0, 0);
+    // the genotype words of a chunk and the frequencies of the LUT built for
+    // the next one are loaded a chunk ahead of their use
+    uint2 wx = make_uint2(0u, 0u), wy = make_uint2(0u, 0u);
+    double pn = 0.5, qn = 0.5;
+    if (s0 < s1) {
+        build(af[2 * (s0 + lut_site)], af[2 * (s0 + lut_site) + 1], 0);
+        wx = *reinterpret_cast<const uint2*>(rowx + s0 / 16);
+        wy = *reinterpret_cast<const uint2*>(rowy + s0 / 16);
+    }
+    if (s0 + CHUNK < s1) {
+        pn = af[2 * (s0 + CHUNK + lut_site)];
+        qn = af[2 * (s0 + CHUNK + lut_site) + 1];
+    }
     __syncthreads();
     int b = 0;
     for (uint32_t cs = s0; cs < s1; cs += CHUNK, b ^= 1) {
-        if (cs + CHUNK < s1) build(cs + CHUNK, b ^ 1);
-        const uint2 wx = *reinterpret_cast<const uint2*>(rowx + cs / 16);
-        const uint2 wy = *reinterpret_cast<const uint2*>(rowy + cs / 16);
+        uint2 nwx = wx, nwy = wy;
+        double pnn = 0.5, qnn = 0.5;
+        if (cs + CHUNK < s1) {
+            nwx = *reinterpret_cast<const uint2*>(rowx + (cs + CHUNK) / 16);
+            nwy = *reinterpret_cast<const uint2*>(rowy + (cs + CHUNK) / 16);
+            build(pn, qn, b ^ 1);
+        }
+        if (cs + 2 * CHUNK < s1) {
+            pnn = af[2 * (cs + 2 * CHUNK + lut_site)];
+            qnn = af[2 * (cs + 2 * CHUNK + lut_site) + 1];
+        }
         const double* L0 = lut + b * LUT_BUF;
         // usable sites of the chunk: neither call missing (code 3), counted
         // exactly as integers (lanevector.rs:100-104 NaN mask)
@@ -388,6 +406,10 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
 #pragma unroll
                 for (int bb = a; bb < 9; bb++, id++) H[id] = fma(cv[a], cv[bb], H[id]);
         }
+        wx = nwx;
+        wy = nwy;
+        pn = pnn;
+        qn = qnn;
         __syncthreads();
     }
     if (live) {
