@@ -287,45 +287,60 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     HIPCHK(hipMemsetAsync(dsy.p, 0, (size_t)ns * sizeof(uint32_t), st));
     HIPCHK(hipMemsetAsync(dactive.p, 0, (size_t)ns * sizeof(int), st));
     HIPCHK(hipMemsetAsync(dnext.p, 0, sizeof(unsigned long long), st));
-    int live = (int)ns;
-    HIPCHK(hipMemcpyAsync(dlive.p, &live, sizeof(int), hipMemcpyHostToDevice, st));
+    const int live0 = (int)ns;
+    HIPCHK(hipMemcpyAsync(dlive.p, &live0, sizeof(int), hipMemcpyHostToDevice, st));
     const kdev::Tune dt = to_dev(*tune);
 
-    hipEvent_t ev[2];
-    HIPCHK(hipEventCreate(&ev[0]));
-    HIPCHK(hipEventCreate(&ev[1]));
+    // sweeps and steps are queued BATCH at a time between host synchronisations;
+    // each step adds the slots still live after it to hist[i], so the host
+    // counts (and times) only sweeps that ran with live slots
+    constexpr int BATCH = 8;
+    std::vector<hipEvent_t> ev(2 * BATCH, nullptr);
     struct EvGuard {
-        hipEvent_t* e;
+        std::vector<hipEvent_t>& e;
         ~EvGuard() {
-            (void)hipEventDestroy(e[0]);
-            (void)hipEventDestroy(e[1]);
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
         }
     } evg{ev};
+    for (hipEvent_t& x : ev) HIPCHK(hipEventCreate(&x));
+    DevBuf<int> dhist;
+    HIPCHK(dhist.alloc(BATCH));
 
     // claim the first pairs
     HIPCHK(klaunch::step(dst.p, dsx.p, dsy.p, dpt.p, dactive.p, ns, dpart.p, c->nseg, dpairs.p, n_pairs, dnext.p,
-                         dout.p, dt, dlive.p, PairQueue{}, st));
+                         dout.p, dt, dlive.p, PairQueue{}, nullptr, st));
     double sweep_ms = 0;
-    uint64_t sweeps = 0;
+    uint64_t sweeps = 0, launched = 0;
+    int hist[BATCH];
+    int live_before = (int)ns;
     // every pair needs at most 1 + sqp_max_iter * bls_max_iter evaluations
     const uint64_t per_pair = 1 + tune->sqp_max_iter * tune->bls_max_iter;
     const uint64_t max_sweeps = per_pair * (((uint64_t)n_pairs + ns - 1) / ns + 1) + 16;
-    for (;;) {
-        if (sweeps > max_sweeps) return fail(KESTREL_EDEVICE, "pair fit did not terminate after %llu sweeps",
-                                             (unsigned long long)sweeps);
-        HIPCHK(hipEventRecord(ev[0], st));
-        HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, dsx.p, dsy.p, dpt.p, dactive.p, ns,
-                              tune->epsilon, dpart.p, st));
-        HIPCHK(hipEventRecord(ev[1], st));
-        HIPCHK(klaunch::step(dst.p, dsx.p, dsy.p, dpt.p, dactive.p, ns, dpart.p, c->nseg, dpairs.p, n_pairs, dnext.p,
-                             dout.p, dt, dlive.p, PairQueue{}, st));
-        HIPCHK(hipMemcpyAsync(&live, dlive.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    while (live_before > 0) {
+        if (launched > max_sweeps) return fail(KESTREL_EDEVICE, "pair fit did not terminate after %llu sweeps",
+                                               (unsigned long long)launched);
+        HIPCHK(hipMemsetAsync(dhist.p, 0, BATCH * sizeof(int), st));
+        for (int i = 0; i < BATCH; i++) {
+            HIPCHK(hipEventRecord(ev[2 * i], st));
+            HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, dsx.p, dsy.p, dpt.p,
+                                  dactive.p, ns, tune->epsilon, dpart.p, st));
+            HIPCHK(hipEventRecord(ev[2 * i + 1], st));
+            HIPCHK(klaunch::step(dst.p, dsx.p, dsy.p, dpt.p, dactive.p, ns, dpart.p, c->nseg, dpairs.p, n_pairs,
+                                 dnext.p, dout.p, dt, dlive.p, PairQueue{}, dhist.p + i, st));
+        }
+        launched += BATCH;
+        HIPCHK(hipMemcpyAsync(hist, dhist.p, BATCH * sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
-        sweep_ms += ms;
-        sweeps++;
-        if (live == 0) break;
+        for (int i = 0; i < BATCH; i++) {
+            if (live_before > 0) {
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+                sweep_ms += ms;
+                sweeps++;
+            }
+            live_before = hist[i];
+        }
     }
     HIPCHK(hipMemcpyAsync(out, dout.p, n_pairs * sizeof(kestrel_output), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));

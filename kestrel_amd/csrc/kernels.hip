@@ -442,22 +442,20 @@ __device__ __forceinline__ void write_point(double* pt, uint32_t nslots, uint32_
 
 }  // namespace
 
-__global__ __launch_bounds__(64) void step_kernel(SlotState* __restrict__ st, uint32_t* __restrict__ sx,
-                                                   uint32_t* __restrict__ sy, double* __restrict__ pt,
-                                                   int* __restrict__ active, uint32_t nslots,
-                                                   const double* __restrict__ partial, uint32_t nseg,
-                                                   const uint32_t* __restrict__ pairs, uint64_t npairs,
-                                                   unsigned long long* __restrict__ next_pair,
-                                                   kestrel_output* __restrict__ out, Tune tune,
-                                                   int* __restrict__ n_live, PairQueue q) {
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= nslots) return;
-    SlotState s = st[slot];
-    if (s.phase == PH_RETIRED) return;
+__device__ __forceinline__ void step_slot(SlotState* __restrict__ st, uint32_t* __restrict__ sx,
+                                          uint32_t* __restrict__ sy, double* __restrict__ pt,
+                                          int* __restrict__ active, uint32_t nslots,
+                                          const double* __restrict__ partial, uint32_t nseg,
+                                          const uint32_t* __restrict__ pairs, uint64_t npairs,
+                                          unsigned long long* __restrict__ next_pair,
+                                          kestrel_output* __restrict__ out, const Tune& tune,
+                                          int* __restrict__ n_live, const PairQueue& q, uint32_t slot,
+                                          SlotState& s) {
     bool claim = s.phase == PH_IDLE || s.phase == PH_WAIT;
     if (!claim) {
+        // segment sums, left in segment 0 by fold_kernel
         double acc[NPART];
-        fold(partial, nseg, nslots, slot, acc);
+        for (int k = 0; k < NPART; k++) acc[k] = partial[(size_t)k * nslots + slot];
         double e[9];
         for (int k = 0; k < 9; k++) e[k] = pt[(size_t)k * nslots + slot];
         const double n = acc[55];
@@ -618,6 +616,41 @@ __global__ __launch_bounds__(64) void step_kernel(SlotState* __restrict__ st, ui
     }
     active[slot] = (s.phase == PH_INIT || s.phase == PH_TRIAL) ? 1 : 0;
     st[slot] = s;
+}
+
+// Sums the segment partials of every swept slot in segment order (the
+// summation order of fold()) into segment 0, one thread per (partial, slot).
+__global__ __launch_bounds__(256) void fold_kernel(double* __restrict__ partial, uint32_t nseg, uint32_t nslots,
+                                                    const int* __restrict__ active) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)NPART * nslots || !active[i % nslots]) return;
+    double* p = partial + i;  // i = k * nslots + slot
+    double a = p[0];
+#pragma unroll 4
+    for (uint32_t sg = 1; sg < nseg; sg++) a = a + p[(size_t)sg * NPART * nslots];
+    p[0] = a;
+}
+
+__global__ __launch_bounds__(64) void step_kernel(SlotState* __restrict__ st, uint32_t* __restrict__ sx,
+                                                   uint32_t* __restrict__ sy, double* __restrict__ pt,
+                                                   int* __restrict__ active, uint32_t nslots,
+                                                   const double* __restrict__ partial, uint32_t nseg,
+                                                   const uint32_t* __restrict__ pairs, uint64_t npairs,
+                                                   unsigned long long* __restrict__ next_pair,
+                                                   kestrel_output* __restrict__ out, Tune tune,
+                                                   int* __restrict__ n_live, PairQueue q, int* __restrict__ live_out) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= nslots) return;
+    SlotState s = st[slot];
+    if (s.phase == PH_RETIRED) return;
+    step_slot(st, sx, sy, pt, active, nslots, partial, nseg, pairs, npairs, next_pair, out, tune, n_live, q, slot, s);
+    if (live_out) {
+        // slots still holding or claiming pairs after this step, one atomic
+        // per wave (retired lanes have returned above)
+        const unsigned long long act = __ballot(1);
+        const unsigned long long m = __ballot(s.phase != PH_RETIRED);
+        if (__lane_id() == __ffsll(act) - 1 && m) atomicAdd(live_out, (int)__popcll(m));
+    }
 }
 
 // Explicit evaluation points: fold partials into f, g, H (objective.rs:34-39,
@@ -968,11 +1001,16 @@ hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, con
 }
 
 hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* active, uint32_t nslots,
-                const double* partial, uint32_t nseg, const uint32_t* pairs, uint64_t npairs,
+                double* partial, uint32_t nseg, const uint32_t* pairs, uint64_t npairs,
                 unsigned long long* next_pair, kestrel_output* out, const Tune& t, int* n_live, const PairQueue& q,
-                hipStream_t st) {
+                int* live_out, hipStream_t st) {
+    if (nseg > 1) {
+        const size_t n = (size_t)NPART * nslots;
+        hipLaunchKernelGGL(fold_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, partial, nseg, nslots,
+                           active);
+    }
     hipLaunchKernelGGL(step_kernel, dim3((nslots + 63) / 64), dim3(64), 0, st, stt, sx, sy, pt, active, nslots,
-                       partial, nseg, pairs, npairs, next_pair, out, t, n_live, q);
+                       partial, nseg, pairs, npairs, next_pair, out, t, n_live, q, live_out);
     return hipGetLastError();
 }
 
