@@ -227,15 +227,91 @@ def kinship(jacquard: np.ndarray) -> np.ndarray:
 
 def build_arrays(outputs: np.ndarray, n: int):
     """lib.rs:120-161: (N, N, 9) Jacquard array with delta3<->delta5 and
-    delta4<->delta6 swapped for (y, x), (N, N) kinship, (N,) inbreeding 2*theta_xx - 1."""
-    jac = np.full((n, n, 9), np.nan)
-    kin = np.full((n, n), np.nan)
+    delta4<->delta6 swapped for (y, x), (N, N) kinship, (N,) inbreeding
+    2*theta_xx - 1; entries without a fitted pair stay 0 as in the reference."""
+    jac = np.zeros((n, n, 9))
+    kin = np.zeros((n, n))
+    inb = np.zeros(n)
     swap = [0, 1, 4, 5, 2, 3, 6, 7, 8]
     k = kinship(outputs["jacquard"])
     for o, kk in zip(outputs, k):
         x, y = int(o["x"]), int(o["y"])
         jac[x, y] = o["jacquard"]
-        jac[y, x] = o["jacquard"][swap]
-        kin[x, y] = kin[y, x] = kk
-    inb = 2.0 * np.diagonal(kin) - 1.0
+        kin[x, y] = kk
+        if x != y:
+            jac[y, x] = o["jacquard"][swap]
+            kin[y, x] = kk
+        else:
+            inb[x] = 2.0 * kk - 1.0
     return jac, kin, inb
+
+
+class Coefficients:
+    """lib.rs:36-44: the arrays returned by relatedness_coefficients_gt."""
+
+    def __init__(self, jacquard, kinship, inbreeding):
+        self.jacquard = jacquard
+        self.kinship = kinship
+        self.inbreeding = inbreeding
+
+
+def _gt_codes(genotypes) -> np.ndarray:
+    """(V, N, 2) allele indices -> (N, V) unphased genotype index
+    (coefficients.rs:18-38); only biallelic sites are supported."""
+    g = np.asarray(genotypes)
+    if g.ndim != 3 or g.shape[2] != 2:
+        raise ValueError("genotypes must have shape (sites, samples, 2)")
+    g = g.astype(np.uint8)
+    lo, hi = np.minimum(g[..., 0], g[..., 1]), np.maximum(g[..., 0], g[..., 1])
+    if hi.size and hi.max() > 1:
+        raise ValueError("tri- and tetra-allelic sites are not supported by this build")
+    return np.ascontiguousarray((hi * (hi + 1) // 2 + lo).T)
+
+
+def relatedness_coefficients_gt(genotypes, allele_frequencies=None, threads=None, iterations: int = 100):
+    """lib.rs:46-73 (a stub in the reference): the CLS / Brier estimator of
+    wrapper.rs:22-87 over every pair of samples.  genotypes: (V, N, 2) allele
+    indices; allele_frequencies: (V, 2) or None (counted).  threads is
+    accepted for the reference's signature and unused (the fit runs on the
+    GPU)."""
+    if iterations < 1:
+        raise ValueError("iterations must be non-zero")
+    codes = _gt_codes(genotypes)
+    af = None if allele_frequencies is None else np.asarray(allele_frequencies, np.float64)
+    co = Cohort.gt(codes, af=af)
+    try:
+        out, _ = co.fit_pairs_gt(iterations=iterations)
+    finally:
+        co.close()
+    return Coefficients(*build_arrays(out, codes.shape[0]))
+
+
+def relatedness_coefficients_gl(genotype_likelihoods, allele_frequencies=None, threads=None,
+                                iterations: int = 100, minimum_allele_frequency=None):
+    """lib.rs:75-109 (a stub in the reference): the maximum-likelihood fit of
+    wrapper.rs:89-207 over every pair of samples.  genotype_likelihoods:
+    (V, N, 3) linear-scale likelihoods of 0/0, 0/1, 1/1; a row with a NaN is a
+    missing call.  Rows must be one-hot (hard calls); given allele frequencies
+    are rejected as in wrapper.rs:113-122.  Returns (jacquard, kinship,
+    inbreeding)."""
+    if allele_frequencies is not None:
+        raise NotImplementedError("allele frequencies with genotype likelihoods (wrapper.rs:113-122)")
+    if iterations < 1:
+        raise ValueError("iterations must be non-zero")
+    gl = np.asarray(genotype_likelihoods, np.float64)
+    if gl.ndim != 3:
+        raise ValueError("genotype_likelihoods must have shape (sites, samples, genotypes)")
+    if gl.shape[2] != 3:
+        raise ValueError("tri- and tetra-allelic sites are not supported by this build")
+    missing = np.isnan(gl).any(axis=2)
+    one_hot = ((gl == 1.0).sum(axis=2) == 1) & ((gl == 0.0).sum(axis=2) == 2)
+    if not (missing | one_hot).all():
+        raise ValueError("genotype likelihoods are not one-hot (soft likelihoods are not supported by this build)")
+    codes = np.where(missing, 3, np.argmax(np.nan_to_num(gl, nan=0.0), axis=2)).astype(np.uint8)
+    codes = np.ascontiguousarray(codes.T)
+    co = Cohort(codes, maf=minimum_allele_frequency, af_iterations=iterations)
+    try:
+        out = co.fit_pairs(iterations=iterations)
+    finally:
+        co.close()
+    return build_arrays(out, codes.shape[0])

@@ -1,0 +1,47 @@
+"""Python array API (lib.rs:26-161): input conversion and array assembly,
+checked on the CPU without a device."""
+import numpy as np
+import pytest
+
+from kestrel_amd import OUTPUT_DTYPE, build_arrays, relatedness_coefficients_gl
+from kestrel_amd.api import _gt_codes
+
+
+def test_gt_codes_unphased_index():
+    # coefficients.rs:27-32: j(j+1)/2 + i with i = min, j = max
+    g = np.array([[[0, 0], [0, 1], [1, 0], [1, 1]]], np.uint8)  # one site, four samples
+    assert _gt_codes(g).tolist() == [[0], [1], [1], [2]]
+
+
+def test_gt_codes_rejects_multiallelic_and_bad_shape():
+    with pytest.raises(ValueError):
+        _gt_codes(np.array([[[0, 2]]], np.uint8))
+    with pytest.raises(ValueError):
+        _gt_codes(np.zeros((3, 4), np.uint8))
+
+
+def test_gl_input_checks_before_any_device_call():
+    soft = np.full((2, 3, 3), 1.0 / 3)
+    with pytest.raises(ValueError, match="one-hot"):
+        relatedness_coefficients_gl(soft)
+    with pytest.raises(NotImplementedError):
+        relatedness_coefficients_gl(np.eye(3)[None].repeat(2, 0), allele_frequencies=np.full((2, 2), 0.5))
+    with pytest.raises(ValueError, match="tri- and tetra"):
+        relatedness_coefficients_gl(np.zeros((2, 3, 6)))
+
+
+def test_build_arrays_matches_reference_assembly():
+    # lib.rs:120-161: swap delta3<->delta5, delta4<->delta6 for (y, x), kinship
+    # symmetric, inbreeding from the diagonal, zeros elsewhere
+    out = np.zeros(3, OUTPUT_DTYPE)
+    d = np.arange(1, 10) / 45.0
+    out[0]["x"], out[0]["y"], out[0]["jacquard"] = 0, 1, d
+    out[1]["x"], out[1]["y"], out[1]["jacquard"] = 1, 1, np.eye(9)[0]
+    out[2]["x"], out[2]["y"], out[2]["jacquard"] = 0, 0, np.eye(9)[8]
+    jac, kin, inb = build_arrays(out, 3)
+    assert np.array_equal(jac[0, 1], d)
+    assert np.array_equal(jac[1, 0], d[[0, 1, 4, 5, 2, 3, 6, 7, 8]])
+    k01 = d @ np.array([1.0, 0.0, 0.5, 0.0, 0.5, 0.0, 0.5, 0.25, 0.0])
+    assert kin[0, 1] == kin[1, 0] and abs(kin[0, 1] - k01) < 1e-15
+    assert inb[1] == 1.0 and inb[0] == -1.0 and inb[2] == 0.0
+    assert (jac[2] == 0).all() and (kin[2] == 0).all()
