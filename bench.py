@@ -29,7 +29,7 @@ FP64_VALU_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
 FLOP_PER_SITE_EVAL = 156  # 136 (gradient + Hessian) + 20 (objective, log counted as 1)
 # HBM bytes per sweep_fp64 launch on the default workload, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE scaled by a plain 256 MiB copy (profiles/sweep_fp64_c2.md)
-SWEEP_TRAFFIC_BYTES_C2 = 1.593e8
+SWEEP_TRAFFIC_BYTES_C2 = 1.612e8
 
 
 def log(*a):
