@@ -129,7 +129,9 @@ int kestrel_fit_pairs_gt(kestrel_cohort* c, uint64_t iterations, const uint32_t*
 
 /* Tuning knobs: "segment_sites" (sites per segment of a sweep, a multiple of
  * 32, default 4096; fixes the summation order, so it is a cohort property),
- * "slots" (pairs in flight, 0 = fill the device), "serve_policy" (order in
+ * "slots" (pairs in flight per slot group, 0 = fill the device), "streams"
+ * (1 or 2 slot groups on separate HIP streams, default 2: one group's fold and
+ * step run beside the other's sweep; results do not change), "serve_policy" (order in
  * which a serving engine started afterwards feeds admitted requests to the
  * device: 1 = shortest remaining first (default), 0 = round robin). */
 int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value);
@@ -137,6 +139,11 @@ int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value);
 /* Timing of the last kestrel_fit_pairs: sweep-kernel milliseconds (sum of
  * launch durations by HIP events), number of sweeps, wall milliseconds. */
 int kestrel_last_fit_stats(const kestrel_cohort* c, double* sweep_ms, uint64_t* sweeps, double* wall_ms);
+
+/* The last fit's sweep-busy time (milliseconds with at least one sweep
+ * running; the launches of the two slot groups overlap) and its number of
+ * slot groups (option "streams"). */
+int kestrel_last_fit_busy(const kestrel_cohort* c, double* busy_ms, uint32_t* groups);
 
 /* Serving: the cohort stays device-resident and answers requests of query
  * samples (new 2-bit rows over the cohort's n_sites, fitted with the cohort's

@@ -199,7 +199,10 @@ class Cohort:
     def last_fit_stats(self):
         ms, sw, wall = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_double()
         check(_lib.load().kestrel_last_fit_stats(self._h, ctypes.byref(ms), ctypes.byref(sw), ctypes.byref(wall)))
-        return {"sweep_ms": ms.value, "sweeps": sw.value, "wall_ms": wall.value}
+        busy, groups = ctypes.c_double(), ctypes.c_uint32()
+        check(_lib.load().kestrel_last_fit_busy(self._h, ctypes.byref(busy), ctypes.byref(groups)))
+        return {"sweep_ms": ms.value, "sweeps": sw.value, "wall_ms": wall.value, "busy_ms": busy.value,
+                "groups": groups.value}
 
 
 def segment_bounds(n_kept: int, segment_sites: int = 4096, chunk: int = 32, pad: int = 256) -> np.ndarray:

@@ -75,6 +75,7 @@ int open_device(kestrel_cohort* c) {
     HIPCHK(hipGetDeviceProperties(&prop, c->device));
     c->n_cu = prop.multiProcessorCount;
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
     return KESTREL_OK;
 }
 
@@ -195,6 +196,11 @@ int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value) {
         c->opt_segment_sites = value;
         return c->set_segments();
     }
+    if (!strcmp(key, "streams")) {
+        if (value != 1 && value != 2) return fail(KESTREL_EINVAL, "streams must be 1 or 2");
+        c->opt_streams = value;
+        return KESTREL_OK;
+    }
     if (!strcmp(key, "serve_policy")) {
         if (value != 0 && value != 1) return fail(KESTREL_EINVAL, "serve_policy must be 0 or 1");
         c->opt_serve_policy = value;
@@ -208,6 +214,13 @@ int kestrel_last_fit_stats(const kestrel_cohort* c, double* sweep_ms, uint64_t* 
     if (sweep_ms) *sweep_ms = c->last_sweep_ms;
     if (sweeps) *sweeps = c->last_sweeps;
     if (wall_ms) *wall_ms = c->last_wall_ms;
+    return KESTREL_OK;
+}
+
+int kestrel_last_fit_busy(const kestrel_cohort* c, double* busy_ms, uint32_t* groups) {
+    if (!c) return fail(KESTREL_EINVAL, "null cohort");
+    if (busy_ms) *busy_ms = c->last_busy_ms;
+    if (groups) *groups = (uint32_t)c->last_groups;
     return KESTREL_OK;
 }
 
@@ -253,41 +266,32 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     }
     const auto t0 = std::chrono::steady_clock::now();
 
-    // pairs in flight: fill the device (2 blocks of 256 lanes per CU) over
-    // nseg segments, never more than the pairs available
-    // resident sweep blocks: 2 per CU (LDS 72 KiB and 2 waves/SIMD each); size the
-    // slot count so (slot blocks x segments) fills them in one round
+    // pairs in flight per group: fill the device (2 blocks of 256 lanes per
+    // CU) over nseg segments, never more than the pairs available
     const uint64_t resident = (uint64_t)std::max(1, c->n_cu) * 2;
     const uint64_t slot_blocks = std::max<uint64_t>(1, resident / c->nseg);
     uint64_t nslots = c->opt_slots > 0 ? (uint64_t)c->opt_slots : slot_blocks * SWEEP_TPB;
     nslots = std::min<uint64_t>(nslots, ((uint64_t)n_pairs + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB);
     nslots = (nslots + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB;
     const uint32_t ns = (uint32_t)nslots;
+    // Two slot groups on two streams when there is work for both: while one
+    // group folds and steps, the other's sweep keeps the device busy.  Both
+    // claim from one pair counter; a pair's result does not depend on the
+    // group or slot that fits it.
+    const int ngroups = (c->opt_streams >= 2 && (uint64_t)n_pairs >= 4 * nslots) ? 2 : 1;
 
-    DevBuf<uint32_t> dpairs, dsx, dsy;
-    DevBuf<double> dpt, dpart;
-    DevBuf<int> dactive, dlive;
-    DevBuf<SlotState> dst;
+    DevBuf<uint32_t> dpairs;
+    DevBuf<int> dlive;
     DevBuf<unsigned long long> dnext;
     DevBuf<kestrel_output> dout;
     HIPCHK(dpairs.alloc(2 * n_pairs));
-    HIPCHK(dsx.alloc(ns));
-    HIPCHK(dsy.alloc(ns));
-    HIPCHK(dpt.alloc((size_t)9 * ns));
-    HIPCHK(dpart.alloc((size_t)c->nseg * NPART * ns));
-    HIPCHK(dactive.alloc(ns));
     HIPCHK(dlive.alloc(1));
-    HIPCHK(dst.alloc(ns));
     HIPCHK(dnext.alloc(1));
     HIPCHK(dout.alloc(n_pairs));
     hipStream_t st = c->stream;
     HIPCHK(hipMemcpyAsync(dpairs.p, pairs, 2 * n_pairs * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemsetAsync(dst.p, 0, (size_t)ns * sizeof(SlotState), st));  // PH_IDLE
-    HIPCHK(hipMemsetAsync(dsx.p, 0, (size_t)ns * sizeof(uint32_t), st));
-    HIPCHK(hipMemsetAsync(dsy.p, 0, (size_t)ns * sizeof(uint32_t), st));
-    HIPCHK(hipMemsetAsync(dactive.p, 0, (size_t)ns * sizeof(int), st));
     HIPCHK(hipMemsetAsync(dnext.p, 0, sizeof(unsigned long long), st));
-    const int live0 = (int)ns;
+    const int live0 = (int)ns * ngroups;
     HIPCHK(hipMemcpyAsync(dlive.p, &live0, sizeof(int), hipMemcpyHostToDevice, st));
     const kdev::Tune dt = to_dev(*tune);
 
@@ -295,53 +299,125 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     // each step adds the slots still live after it to hist[i], so the host
     // counts (and times) only sweeps that ran with live slots
     constexpr int BATCH = 8;
-    std::vector<hipEvent_t> ev(2 * BATCH, nullptr);
-    struct EvGuard {
-        std::vector<hipEvent_t>& e;
-        ~EvGuard() {
-            for (hipEvent_t x : e)
+    struct Group {
+        hipStream_t stream = nullptr;
+        DevBuf<uint32_t> sx, sy;
+        DevBuf<double> pt, part;
+        DevBuf<int> active, hist;
+        DevBuf<SlotState> st;
+        std::vector<hipEvent_t> ev;
+        int hist_h[BATCH] = {};
+        int live_before = 0;
+        ~Group() {
+            for (hipEvent_t x : ev)
                 if (x) (void)hipEventDestroy(x);
         }
-    } evg{ev};
-    for (hipEvent_t& x : ev) HIPCHK(hipEventCreate(&x));
-    DevBuf<int> dhist;
-    HIPCHK(dhist.alloc(BATCH));
-
+    };
+    Group grp[2];
+    hipEvent_t ref = nullptr;
+    struct RefGuard {
+        hipEvent_t& e;
+        ~RefGuard() {
+            if (e) (void)hipEventDestroy(e);
+        }
+    } refg{ref};
+    HIPCHK(hipEventCreate(&ref));
+    HIPCHK(hipStreamSynchronize(st));  // pair list and counters are set before either stream starts
+    for (int g = 0; g < ngroups; g++) {
+        Group& G = grp[g];
+        G.stream = g == 0 ? c->stream : c->stream2;
+        HIPCHK(G.sx.alloc(ns));
+        HIPCHK(G.sy.alloc(ns));
+        HIPCHK(G.pt.alloc((size_t)9 * ns));
+        HIPCHK(G.part.alloc((size_t)c->nseg * NPART * ns));
+        HIPCHK(G.active.alloc(ns));
+        HIPCHK(G.hist.alloc(BATCH));
+        HIPCHK(G.st.alloc(ns));
+        G.ev.assign(2 * BATCH, nullptr);
+        for (hipEvent_t& x : G.ev) HIPCHK(hipEventCreate(&x));
+        HIPCHK(hipMemsetAsync(G.st.p, 0, (size_t)ns * sizeof(SlotState), G.stream));  // PH_IDLE
+        HIPCHK(hipMemsetAsync(G.sx.p, 0, (size_t)ns * sizeof(uint32_t), G.stream));
+        HIPCHK(hipMemsetAsync(G.sy.p, 0, (size_t)ns * sizeof(uint32_t), G.stream));
+        HIPCHK(hipMemsetAsync(G.active.p, 0, (size_t)ns * sizeof(int), G.stream));
+        G.live_before = (int)ns;
+    }
+    HIPCHK(hipEventRecord(ref, st));
+    if (ngroups == 2) HIPCHK(hipStreamWaitEvent(grp[1].stream, ref, 0));
     // claim the first pairs
-    HIPCHK(klaunch::step(dst.p, dsx.p, dsy.p, dpt.p, dactive.p, ns, dpart.p, c->nseg, dpairs.p, n_pairs, dnext.p,
-                         dout.p, dt, dlive.p, PairQueue{}, nullptr, st));
+    for (int g = 0; g < ngroups; g++) {
+        Group& G = grp[g];
+        HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, c->nseg, dpairs.p, n_pairs,
+                             dnext.p, dout.p, dt, dlive.p, PairQueue{}, nullptr, G.stream));
+    }
     double sweep_ms = 0;
     uint64_t sweeps = 0, launched = 0;
-    int hist[BATCH];
-    int live_before = (int)ns;
+    std::vector<std::pair<float, float>> spans;  // [start, end] of live sweeps, ms after ref
     // every pair needs at most 1 + sqp_max_iter * bls_max_iter evaluations
     const uint64_t per_pair = 1 + tune->sqp_max_iter * tune->bls_max_iter;
     const uint64_t max_sweeps = per_pair * (((uint64_t)n_pairs + ns - 1) / ns + 1) + 16;
-    while (live_before > 0) {
+    auto any_live = [&] {
+        for (int g = 0; g < ngroups; g++)
+            if (grp[g].live_before > 0) return true;
+        return false;
+    };
+    while (any_live()) {
         if (launched > max_sweeps) return fail(KESTREL_EDEVICE, "pair fit did not terminate after %llu sweeps",
                                                (unsigned long long)launched);
-        HIPCHK(hipMemsetAsync(dhist.p, 0, BATCH * sizeof(int), st));
-        for (int i = 0; i < BATCH; i++) {
-            HIPCHK(hipEventRecord(ev[2 * i], st));
-            HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, dsx.p, dsy.p, dpt.p,
-                                  dactive.p, ns, tune->epsilon, dpart.p, st));
-            HIPCHK(hipEventRecord(ev[2 * i + 1], st));
-            HIPCHK(klaunch::step(dst.p, dsx.p, dsy.p, dpt.p, dactive.p, ns, dpart.p, c->nseg, dpairs.p, n_pairs,
-                                 dnext.p, dout.p, dt, dlive.p, PairQueue{}, dhist.p + i, st));
-        }
-        launched += BATCH;
-        HIPCHK(hipMemcpyAsync(hist, dhist.p, BATCH * sizeof(int), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        for (int i = 0; i < BATCH; i++) {
-            if (live_before > 0) {
-                float ms = 0;
-                HIPCHK(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
-                sweep_ms += ms;
-                sweeps++;
+        for (int g = 0; g < ngroups; g++)
+            if (grp[g].live_before > 0)
+                HIPCHK(hipMemsetAsync(grp[g].hist.p, 0, BATCH * sizeof(int), grp[g].stream));
+        for (int i = 0; i < BATCH; i++)
+            for (int g = 0; g < ngroups; g++) {
+                Group& G = grp[g];
+                if (G.live_before <= 0) continue;
+                HIPCHK(hipEventRecord(G.ev[2 * i], G.stream));
+                HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, G.sx.p, G.sy.p, G.pt.p,
+                                      G.active.p, ns, tune->epsilon, G.part.p, G.stream));
+                HIPCHK(hipEventRecord(G.ev[2 * i + 1], G.stream));
+                HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, c->nseg, dpairs.p,
+                                     n_pairs, dnext.p, dout.p, dt, dlive.p, PairQueue{}, G.hist.p + i, G.stream));
             }
-            live_before = hist[i];
+        launched += BATCH;
+        for (int g = 0; g < ngroups; g++) {
+            Group& G = grp[g];
+            if (G.live_before <= 0) continue;
+            HIPCHK(hipMemcpyAsync(G.hist_h, G.hist.p, BATCH * sizeof(int), hipMemcpyDeviceToHost, G.stream));
+        }
+        for (int g = 0; g < ngroups; g++)
+            if (grp[g].live_before > 0) HIPCHK(hipStreamSynchronize(grp[g].stream));
+        for (int g = 0; g < ngroups; g++) {
+            Group& G = grp[g];
+            if (G.live_before <= 0) continue;
+            for (int i = 0; i < BATCH; i++) {
+                if (G.live_before > 0) {
+                    float ms = 0, t_start = 0, t_end = 0;
+                    HIPCHK(hipEventElapsedTime(&ms, G.ev[2 * i], G.ev[2 * i + 1]));
+                    HIPCHK(hipEventElapsedTime(&t_start, ref, G.ev[2 * i]));
+                    HIPCHK(hipEventElapsedTime(&t_end, ref, G.ev[2 * i + 1]));
+                    sweep_ms += ms;
+                    sweeps++;
+                    spans.emplace_back(t_start, t_end);
+                }
+                G.live_before = G.hist_h[i];
+            }
         }
     }
+    // device time with at least one sweep running (the launches of the two
+    // groups overlap)
+    std::sort(spans.begin(), spans.end());
+    double busy = 0, cur_s = -1, cur_e = -1;
+    for (const auto& sp : spans) {
+        if (sp.first > cur_e) {
+            if (cur_e > cur_s) busy += cur_e - cur_s;
+            cur_s = sp.first;
+            cur_e = sp.second;
+        } else if (sp.second > cur_e) {
+            cur_e = sp.second;
+        }
+    }
+    if (cur_e > cur_s) busy += cur_e - cur_s;
+    c->last_busy_ms = busy;
+    c->last_groups = ngroups;
     HIPCHK(hipMemcpyAsync(out, dout.p, n_pairs * sizeof(kestrel_output), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     c->last_sweep_ms = sweep_ms;

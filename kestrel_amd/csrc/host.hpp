@@ -67,8 +67,11 @@ struct kestrel_cohort {
     int64_t opt_slots = 0;
     int64_t opt_segment_sites = 4096;
     int64_t opt_serve_policy = 1;  // 0 round robin, 1 shortest remaining first
-    double last_sweep_ms = 0, last_wall_ms = 0;
+    int64_t opt_streams = 2;       // slot groups on separate streams in a fit
+    hipStream_t stream2 = nullptr; // second stream of a fit
+    double last_sweep_ms = 0, last_wall_ms = 0, last_busy_ms = 0;
     uint64_t last_sweeps = 0;
+    int last_groups = 1;
 
     int set_segments() {
         // nseg = ceil(chunks / target); segments of equal length (the last
@@ -90,5 +93,6 @@ struct kestrel_cohort {
     ~kestrel_cohort() {
         serve_release(serve);
         if (stream) (void)hipStreamDestroy(stream);
+        if (stream2) (void)hipStreamDestroy(stream2);
     }
 };

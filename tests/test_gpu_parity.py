@@ -141,6 +141,17 @@ def test_schedule_independence(O, medium):
     co.set_option("slots", 0)
     assert_bitexact(a, b)
     assert_bitexact(a[perm], cc)
+    # two slot groups on two streams (used when there are >= 4 x slots pairs)
+    co.set_option("slots", 256)
+    many = rng.integers(0, codes.shape[0], (3000, 2))
+    two = co.fit_pairs(many)
+    assert co.last_fit_stats()["groups"] == 2
+    co.set_option("streams", 1)
+    one = co.fit_pairs(many)
+    assert co.last_fit_stats()["groups"] == 1
+    co.set_option("streams", 2)
+    co.set_option("slots", 0)
+    assert_bitexact(two, one)
 
 
 def test_pair_order_and_subset(O, medium):
