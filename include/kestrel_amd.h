@@ -93,6 +93,19 @@ int kestrel_cohort_create(kestrel_cohort** out, const uint8_t* packed, uint32_t 
                           int precision);
 
 /* Frequencies of all input sites (n_sites x 2) and the kept-site mask. */
+/* A cohort of soft genotype likelihoods for the maximum-likelihood fit.
+ * gl: n_sites x n_samples x 3 doubles, site-major as the reference's
+ * GenotypeLikelihoods (vcf.rs:321-323), each triple the linear-scale
+ * likelihoods of 0/0, 0/1, 1/1 (10^(gl - max gl) or 10^(-pl/10), vcf.rs:368,
+ * 407); a triple with a NaN is a missing call.  Replaces wrapper.rs:89-207 up
+ * to the pair loop: allele frequencies by the per-site SQP on these rows
+ * (allele/mod.rs:37-134, maf < 0 = 1/(2N)) unless af (n_sites x 2) is given,
+ * kept sites as kestrel_cohort_create.  kestrel_fit_pairs / kestrel_evaluate
+ * then build c_s per pair and site as mixture_row2 (mixture.rs:96-236).
+ * Biallelic sites only. */
+int kestrel_cohort_create_gl(kestrel_cohort** out, const double* gl, uint32_t n_samples, uint32_t n_sites,
+                             const double* af, double maf, uint64_t af_iterations);
+
 int kestrel_cohort_allele_freqs(const kestrel_cohort* c, double* af_out, uint8_t* kept_out);
 /* Number of kept (segregating) sites. */
 int kestrel_cohort_n_kept(const kestrel_cohort* c, uint32_t* n_kept);

@@ -67,6 +67,27 @@ class Cohort:
         self._h = h
         return self
 
+    @classmethod
+    def gl(cls, gl: np.ndarray, *, af: np.ndarray | None = None, maf: float | None = None,
+           af_iterations: int = 100, segment_sites: int | None = None) -> "Cohort":
+        """Cohort of soft genotype likelihoods (kestrel_cohort_create_gl): gl is
+        (V, N, 3) site-major linear-scale likelihoods of 0/0, 0/1, 1/1, a NaN
+        marking a missing call."""
+        L = _lib.load()
+        self = cls.__new__(cls)
+        g = np.ascontiguousarray(gl, np.float64)
+        if g.ndim != 3 or g.shape[2] != 3:
+            raise ValueError("gl must have shape (sites, samples, 3)")
+        self.n_sites, self.n_samples = int(g.shape[0]), int(g.shape[1])
+        afp = None if af is None else np.ascontiguousarray(af, np.float64).reshape(self.n_sites, 2)
+        h = ctypes.c_void_p()
+        check(L.kestrel_cohort_create_gl(ctypes.byref(h), ptr(g), self.n_samples, self.n_sites, ptr(afp),
+                                         -1.0 if maf is None else float(maf), af_iterations))
+        self._h = h
+        if segment_sites is not None:
+            self.set_option("segment_sites", segment_sites)
+        return self
+
     def fit_pairs_gt(self, pairs: np.ndarray | None = None, n_select: int | None = None, iterations: int = 100):
         """CLS fit of pairs (coefficients.rs:166-278); returns (outputs, Q)."""
         if pairs is None:
@@ -294,9 +315,10 @@ def relatedness_coefficients_gl(genotype_likelihoods, allele_frequencies=None, t
     """lib.rs:75-109 (a stub in the reference): the maximum-likelihood fit of
     wrapper.rs:89-207 over every pair of samples.  genotype_likelihoods:
     (V, N, 3) linear-scale likelihoods of 0/0, 0/1, 1/1; a row with a NaN is a
-    missing call.  Rows must be one-hot (hard calls); given allele frequencies
-    are rejected as in wrapper.rs:113-122.  Returns (jacquard, kinship,
-    inbreeding)."""
+    missing call.  Hard calls (one-hot rows) take the 2-bit path, other rows
+    the likelihood path; both give the reference's records.  Given allele
+    frequencies are rejected as in wrapper.rs:113-122.  Returns (jacquard,
+    kinship, inbreeding)."""
     if allele_frequencies is not None:
         raise NotImplementedError("allele frequencies with genotype likelihoods (wrapper.rs:113-122)")
     if iterations < 1:
@@ -308,13 +330,14 @@ def relatedness_coefficients_gl(genotype_likelihoods, allele_frequencies=None, t
         raise ValueError("tri- and tetra-allelic sites are not supported by this build")
     missing = np.isnan(gl).any(axis=2)
     one_hot = ((gl == 1.0).sum(axis=2) == 1) & ((gl == 0.0).sum(axis=2) == 2)
-    if not (missing | one_hot).all():
-        raise ValueError("genotype likelihoods are not one-hot (soft likelihoods are not supported by this build)")
-    codes = np.where(missing, 3, np.argmax(np.nan_to_num(gl, nan=0.0), axis=2)).astype(np.uint8)
-    codes = np.ascontiguousarray(codes.T)
-    co = Cohort(codes, maf=minimum_allele_frequency, af_iterations=iterations)
+    n = gl.shape[1]
+    if (missing | one_hot).all():
+        codes = np.where(missing, 3, np.argmax(np.nan_to_num(gl, nan=0.0), axis=2)).astype(np.uint8)
+        co = Cohort(np.ascontiguousarray(codes.T), maf=minimum_allele_frequency, af_iterations=iterations)
+    else:
+        co = Cohort.gl(gl, maf=minimum_allele_frequency, af_iterations=iterations)
     try:
         out = co.fit_pairs(iterations=iterations)
     finally:
         co.close()
-    return build_arrays(out, codes.shape[0])
+    return build_arrays(out, n)

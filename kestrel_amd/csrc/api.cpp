@@ -171,6 +171,82 @@ int kestrel_cohort_create(kestrel_cohort** out, const uint8_t* packed, uint32_t 
     return KESTREL_OK;
 }
 
+int kestrel_cohort_create_gl(kestrel_cohort** out, const double* gl, uint32_t n_samples, uint32_t n_sites,
+                             const double* af, double maf, uint64_t af_iterations) {
+    if (!out || !gl) return fail(KESTREL_EINVAL, "null argument");
+    *out = nullptr;
+    if (n_samples == 0 || n_sites == 0) return fail(KESTREL_EINVAL, "need at least one sample and one site");
+    if (maf < 0) maf = 1.0 / (2.0 * (double)n_samples);  // wrapper.rs:101
+    if (!(maf >= 0.0 && maf <= 1.0)) return fail(KESTREL_EINVAL, "Minimum allele frequency must be within 0 to 1");
+    if (!af && af_iterations == 0) return fail(KESTREL_EINVAL, "iterations must be non-zero");
+
+    auto c = new kestrel_cohort();
+    std::unique_ptr<kestrel_cohort> guard(c);
+    int rc = open_device(c);
+    if (rc) return rc;
+    c->soft = true;
+    c->n_samples = n_samples;
+    c->n_sites = n_sites;
+    const size_t n_vals = (size_t)n_sites * n_samples * 3;
+    DevBuf<double> dgl;
+    HIPCHK(dgl.alloc(n_vals));
+    HIPCHK(hipMemcpyAsync(dgl.p, gl, n_vals * sizeof(double), hipMemcpyHostToDevice, c->stream));
+
+    c->af_all.resize((size_t)n_sites * 2);
+    if (af) {
+        std::memcpy(c->af_all.data(), af, c->af_all.size() * sizeof(double));
+    } else {
+        DevBuf<double> daf;
+        DevBuf<int> dstatus;
+        HIPCHK(daf.alloc((size_t)n_sites * 2));
+        HIPCHK(dstatus.alloc(1));
+        HIPCHK(hipMemsetAsync(dstatus.p, 0, sizeof(int), c->stream));
+        kestrel_tune t;
+        kestrel_tune_default(&t, af_iterations);
+        t.qp_max_iter = 3;  // allele/mod.rs:93: A + 1
+        HIPCHK(klaunch::af_gl(dgl.p, n_samples, n_sites, to_dev(t), maf, daf.p, dstatus.p, c->stream));
+        int status = 0;
+        HIPCHK(hipMemcpyAsync(c->af_all.data(), daf.p, c->af_all.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(&status, dstatus.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (status) return fail(KESTREL_EDATA, "Must have at least one row to analyze (a site has no called sample)");
+    }
+
+    // wrapper.rs:352-410: keep sites with two non-zero allele frequencies
+    c->kept_mask.assign(n_sites, 0);
+    std::vector<uint32_t> kept;
+    for (uint32_t s = 0; s < n_sites; s++)
+        if (c->af_all[2 * s] > 0.0 && c->af_all[2 * s + 1] > 0.0) {
+            c->kept_mask[s] = 1;
+            kept.push_back(s);
+        }
+    c->n_kept = (uint32_t)kept.size();
+    c->v_pad = std::max<uint32_t>(SITE_PAD, (c->n_kept + SITE_PAD - 1) / SITE_PAD * SITE_PAD);
+    c->row_words = c->v_pad / 16;
+    DevBuf<uint32_t> dkept;
+    HIPCHK(dkept.alloc(std::max<size_t>(kept.size(), 1)));
+    if (!kept.empty())
+        HIPCHK(hipMemcpyAsync(dkept.p, kept.data(), kept.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              c->stream));
+    HIPCHK(c->geno.alloc((size_t)n_samples * c->row_words));
+    HIPCHK(c->glrows.alloc((size_t)n_samples * c->v_pad * 3));
+    HIPCHK(klaunch::gather_gl(dgl.p, n_samples, dkept.p, c->n_kept, c->glrows.p, c->v_pad, c->geno.p, c->row_words,
+                              c->stream));
+    std::vector<double> afk((size_t)c->v_pad * 2, 0.5);
+    for (uint32_t t = 0; t < c->n_kept; t++) {
+        afk[2 * t] = c->af_all[2 * kept[t]];
+        afk[2 * t + 1] = c->af_all[2 * kept[t] + 1];
+    }
+    HIPCHK(c->af.alloc(afk.size()));
+    HIPCHK(hipMemcpyAsync(c->af.p, afk.data(), afk.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    rc = c->set_segments();
+    if (rc) return rc;
+    *out = guard.release();
+    return KESTREL_OK;
+}
+
 int kestrel_cohort_allele_freqs(const kestrel_cohort* c, double* af_out, uint8_t* kept_out) {
     if (!c) return fail(KESTREL_EINVAL, "null cohort");
     if (af_out) std::memcpy(af_out, c->af_all.data(), c->af_all.size() * sizeof(double));
@@ -372,7 +448,8 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
                 if (G.live_before <= 0) continue;
                 HIPCHK(hipEventRecord(G.ev[2 * i], G.stream));
                 HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, G.sx.p, G.sy.p, G.pt.p,
-                                      G.active.p, ns, tune->epsilon, G.part.p, G.stream));
+                                      G.active.p, ns, tune->epsilon, G.part.p, G.stream,
+                                      c->soft ? c->glrows.p : nullptr, c->v_pad));
                 HIPCHK(hipEventRecord(G.ev[2 * i + 1], G.stream));
                 HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, c->nseg, dpairs.p,
                                      n_pairs, dnext.p, dout.p, dt, dlive.p, PairQueue{}, G.hist.p + i, G.stream));
@@ -460,7 +537,7 @@ int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* poi
     HIPCHK(hipMemcpyAsync(dact.p, hact.data(), ns * sizeof(int), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dpt.p, hpt.data(), hpt.size() * sizeof(double), hipMemcpyHostToDevice, st));
     HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, dsx.p, dsy.p, dpt.p, dact.p, ns, eps,
-                          dpart.p, st));
+                          dpart.p, st, c->soft ? c->glrows.p : nullptr, c->v_pad));
     HIPCHK(klaunch::finalize(dpart.p, c->nseg, ns, (uint32_t)n, df.p, dg.p, dh.p, dn.p, st));
     HIPCHK(hipMemcpyAsync(f, df.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(g, dg.p, 9 * n * sizeof(double), hipMemcpyDeviceToHost, st));

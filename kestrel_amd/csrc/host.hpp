@@ -62,6 +62,8 @@ struct kestrel_cohort {
     std::vector<double> af_all;
     std::vector<uint8_t> kept_mask;
     bool gt = false;        // created by kestrel_cohort_create_gt (CLS path)
+    bool soft = false;      // created by kestrel_cohort_create_gl: likelihood rows in glrows
+    DevBuf<double> glrows;  // n_samples x v_pad x 3 likelihoods of kept sites (0 when missing)
     DevBuf<double> qmat;    // CLS Gram matrix (computed on first use)
     bool have_q = false;
     int64_t opt_slots = 0;

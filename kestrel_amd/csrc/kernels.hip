@@ -34,46 +34,87 @@ __device__ __forceinline__ double radd8(const double* v) {
     return b0 + b1;
 }
 
-struct AlleleSite {
+// Likelihood rows of one site: codes (one-hot rows of the 2-bit calls) or
+// linear-scale likelihoods (site-major, samples consecutive); load() returns
+// false for a row that lanevector.rs:51-53 skips (a missing call / a NaN).
+struct CodeRows {
     const uint8_t* col;  // byte of this site in sample row 0
     size_t stride;       // row stride in bytes
     uint32_t shift;      // bit offset of the site inside its byte
+    __device__ __forceinline__ bool load(uint32_t i, double* l) const {
+        const uint32_t c = (col[(size_t)i * stride] >> shift) & 3u;
+        l[0] = c == 0 ? 1.0 : 0.0;
+        l[1] = c == 1 ? 1.0 : 0.0;
+        l[2] = c == 2 ? 1.0 : 0.0;
+        return c != 3;
+    }
+};
+
+struct GlRows {
+    const double* base;  // likelihood triple of sample 0 at this site
+    __device__ __forceinline__ bool load(uint32_t i, double* l) const {
+        l[0] = base[3 * (size_t)i];
+        l[1] = base[3 * (size_t)i + 1];
+        l[2] = base[3 * (size_t)i + 2];
+        return !(isnan(l[0]) || isnan(l[1]) || isnan(l[2]));
+    }
+};
+
+template <class Rows>
+struct AlleleSite {
+    Rows rows;
     uint32_t n_samples;
     double eps;
 
-    __device__ __forceinline__ uint32_t code(uint32_t i) const { return (col[(size_t)i * stride] >> shift) & 3u; }
-
-    // allele/objective.rs:85-110 over the one-hot rows of the called samples
-    __device__ double obj(const double* x) const {
-        const double x00 = x[0] * x[0], x11 = x[1] * x[1], x01 = x[0] * x[1];
-        const double tx01 = x01 + x01;
-        double zs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        uint32_t buf = 0, cnt = 0, nb = 0;
-        auto row = [&](uint32_t c) {
-            double p = eps;
-            p = fma(x00, c == 0 ? 1.0 : 0.0, p);
-            p = fma(tx01, c == 1 ? 1.0 : 0.0, p);
-            p = fma(x11, c == 2 ? 1.0 : 0.0, p);
-            return sleef_log(p);
-        };
+    // Visits the usable rows as LaneVectorBuilder lays them out
+    // (lanevector.rs:99-139): full blocks of 8 in order, lane j taking the
+    // j-th row of a block, then the remainder rows.
+    template <class Lane, class Rem>
+    __device__ __forceinline__ uint32_t visit(Lane lane, Rem rem) const {
+        uint32_t idx[8];
+        uint32_t cnt = 0, nb = 0;
         for (uint32_t i = 0; i < n_samples; i++) {
-            const uint32_t c = code(i);
-            if (c == 3) continue;
-            buf |= c << (2 * cnt);
+            double l[3];
+            if (!rows.load(i, l)) continue;
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                if ((uint32_t)j == cnt) idx[j] = i;
             if (++cnt == 8) {
 #pragma unroll
-                for (int j = 0; j < 8; j++) zs[j] = row((buf >> (2 * j)) & 3u) + zs[j];
-                buf = 0;
+                for (int j = 0; j < 8; j++) {
+                    rows.load(idx[j], l);
+                    lane(j, l);
+                }
                 cnt = 0;
                 nb++;
             }
         }
-        double rs = 0.0;
 #pragma unroll
         for (int j = 0; j < 8; j++)
-            if ((uint32_t)j < cnt) rs = row((buf >> (2 * j)) & 3u) + rs;
-        const double n = (double)(nb * 8 + cnt);
-        return -(radd8(zs) + rs) / n;
+            if ((uint32_t)j < cnt) {
+                double l[3];
+                rows.load(idx[j], l);
+                rem(l);
+            }
+        return nb * 8 + cnt;
+    }
+
+    // allele/objective.rs:85-110
+    __device__ double obj(const double* x) const {
+        const double x00 = x[0] * x[0], x11 = x[1] * x[1], x01 = x[0] * x[1];
+        const double tx01 = x01 + x01;
+        double zs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        double rs = 0.0;
+        auto row = [&](const double* l) {
+            double p = eps;
+            p = fma(x00, l[0], p);
+            p = fma(tx01, l[1], p);
+            p = fma(x11, l[2], p);
+            return sleef_log(p);
+        };
+        const uint32_t n =
+            visit([&](int j, const double* l) { zs[j] = row(l) + zs[j]; }, [&](const double* l) { rs = row(l) + rs; });
+        return -(radd8(zs) + rs) / (double)n;
     }
 
     // allele/grad_hess.rs:107-166 + build_grad_hess 83-105
@@ -81,9 +122,10 @@ struct AlleleSite {
         double zg0[8], zg1[8], zh0[8], zh1[8], zh2[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) zg0[j] = zg1[j] = zh0[j] = zh1[j] = zh2[j] = 0.0;
+        double rg0 = 0, rg1 = 0, rh0 = 0, rh1 = 0, rh2 = 0;
         const double x0 = x[0], x1 = x[1];
-        auto row = [&](uint32_t c, double& g0, double& g1, double& h0, double& h1, double& h2) {
-            const double l00 = c == 0 ? 1.0 : 0.0, l01 = c == 1 ? 1.0 : 0.0, l11 = c == 2 ? 1.0 : 0.0;
+        auto row = [&](const double* l, double& g0, double& g1, double& h0, double& h1, double& h2) {
+            const double l00 = l[0], l01 = l[1], l11 = l[2];
             double lx0 = x0 * l00;
             lx0 = fma(x1, l01, lx0);
             double lx1 = x0 * l01;
@@ -102,47 +144,34 @@ struct AlleleSite {
             g0 = fma(lx0, inv, g0);
             g1 = fma(lx1, inv, g1);
         };
-        uint32_t buf = 0, cnt = 0, nb = 0;
-        for (uint32_t i = 0; i < n_samples; i++) {
-            const uint32_t c = code(i);
-            if (c == 3) continue;
-            buf |= c << (2 * cnt);
-            if (++cnt == 8) {
-#pragma unroll
-                for (int j = 0; j < 8; j++) row((buf >> (2 * j)) & 3u, zg0[j], zg1[j], zh0[j], zh1[j], zh2[j]);
-                buf = 0;
-                cnt = 0;
-                nb++;
-            }
-        }
-        double rg0 = 0, rg1 = 0, rh0 = 0, rh1 = 0, rh2 = 0;
-#pragma unroll
-        for (int j = 0; j < 8; j++)
-            if ((uint32_t)j < cnt) row((buf >> (2 * j)) & 3u, rg0, rg1, rh0, rh1, rh2);
-        const double n = (double)(nb * 8 + cnt);
-        g[0] = -2.0 * (radd8(zg0) + rg0) / n;
-        g[1] = -2.0 * (radd8(zg1) + rg1) / n;
-        const double h01 = 2.0 * (radd8(zh1) + rh1) / n;
-        h[0] = 2.0 * (radd8(zh0) + rh0) / n;
+        const uint32_t n = visit([&](int j, const double* l) { row(l, zg0[j], zg1[j], zh0[j], zh1[j], zh2[j]); },
+                                 [&](const double* l) { row(l, rg0, rg1, rh0, rh1, rh2); });
+        const double nd = (double)n;
+        g[0] = -2.0 * (radd8(zg0) + rg0) / nd;
+        g[1] = -2.0 * (radd8(zg1) + rg1) / nd;
+        const double h01 = 2.0 * (radd8(zh1) + rh1) / nd;
+        h[0] = 2.0 * (radd8(zh0) + rh0) / nd;
         h[1] = h01;
         h[MX] = h01;
-        h[MX + 1] = 2.0 * (radd8(zh2) + rh2) / n;
+        h[MX + 1] = 2.0 * (radd8(zh2) + rh2) / nd;
+    }
+
+    __device__ uint32_t called() const {
+        uint32_t n = 0;
+        for (uint32_t i = 0; i < n_samples; i++) {
+            double l[3];
+            n += rows.load(i, l);
+        }
+        return n;
     }
 };
 
-}  // namespace
-
 // allele/mod.rs:100-126: SQP from (1/2, 1/2) (sqp.rs:49-93 with QP cap 3),
 // then alleles with 0 < p < maf are zeroed and the pair renormalised.
-__global__ __launch_bounds__(64) void af_kernel(const uint8_t* __restrict__ packed, size_t row_bytes,
-                                                 uint32_t n_samples, uint32_t n_sites, Tune tune, double maf,
-                                                 double* __restrict__ af_out, int* __restrict__ status) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_sites) return;
-    AlleleSite site{packed + (s >> 2), row_bytes, (s & 3u) * 2u, n_samples, tune.epsilon};
-    uint32_t called = 0;
-    for (uint32_t i = 0; i < n_samples; i++) called += site.code(i) != 3;
-    if (called == 0) {  // lanevector.rs:73-76 would panic
+template <class Rows>
+__device__ void fit_site(const AlleleSite<Rows>& site, uint32_t s, const Tune& tune, double maf,
+                         double* __restrict__ af_out, int* __restrict__ status) {
+    if (site.called() == 0) {  // lanevector.rs:73-76 would panic
         af_out[2 * s] = 0.0;
         af_out[2 * s + 1] = 0.0;
         atomicOr(status, 1);
@@ -184,6 +213,28 @@ __global__ __launch_bounds__(64) void af_kernel(const uint8_t* __restrict__ pack
     af_out[2 * s + 1] = o[1];
 }
 
+}  // namespace
+
+// One site per thread: codes (2-bit sample-major rows) or likelihood rows
+// (site-major, V x N x 3).
+__global__ __launch_bounds__(64) void af_kernel(const uint8_t* __restrict__ packed, size_t row_bytes,
+                                                 uint32_t n_samples, uint32_t n_sites, Tune tune, double maf,
+                                                 double* __restrict__ af_out, int* __restrict__ status) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_sites) return;
+    const AlleleSite<CodeRows> site{{packed + (s >> 2), row_bytes, (s & 3u) * 2u}, n_samples, tune.epsilon};
+    fit_site(site, s, tune, maf, af_out, status);
+}
+
+__global__ __launch_bounds__(64) void af_gl_kernel(const double* __restrict__ gl, uint32_t n_samples,
+                                                    uint32_t n_sites, Tune tune, double maf,
+                                                    double* __restrict__ af_out, int* __restrict__ status) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_sites) return;
+    const AlleleSite<GlRows> site{{gl + (size_t)s * n_samples * 3}, n_samples, tune.epsilon};
+    fit_site(site, s, tune, maf, af_out, status);
+}
+
 // Repack kept sites: dst word w of sample i holds kept sites 16w..16w+15.
 __global__ __launch_bounds__(256) void gather_kernel(const uint8_t* __restrict__ src, size_t src_row_bytes,
                                                       const uint32_t* __restrict__ kept, uint32_t n_kept,
@@ -211,6 +262,33 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint8_t* __restrict__
 // ------------------------------------------------------------ the sweep
 
 namespace {
+
+// One site of one pair: L = L0 + c.D as an fma chain over columns 0..8
+// (objective.rs:58-63; L0 = eps, or 1.0 for a missing site whose c is 0),
+// the log product (the sum of log L over a 16-site group is the log of the
+// product of its L; a missing site multiplies by exactly 1.0), r = 1/L
+// (grad_hess.rs:122), g += c r, c' = c r, H += c' c'^T (upper 45).
+__device__ __forceinline__ void site_update(double* cv, double L0, const double* D, int k, double& P, double& F,
+                                            double* G, double* H) {
+    double L = L0;
+#pragma unroll
+    for (int j = 0; j < 9; j++) L = fma(D[j], cv[j], L);
+    P = P * L;
+    if ((k & (LOG_GROUP - 1)) == LOG_GROUP - 1) {
+        F = F + sleef_log_pos(P);
+        P = 1.0;
+    }
+    const double r = recip_rn(L);
+#pragma unroll
+    for (int j = 0; j < 9; j++) G[j] = fma(cv[j], r, G[j]);
+#pragma unroll
+    for (int j = 0; j < 9; j++) cv[j] = cv[j] * r;
+    int id = 0;
+#pragma unroll
+    for (int a = 0; a < 9; a++)
+#pragma unroll
+        for (int bb = a; bb < 9; bb++, id++) H[id] = fma(cv[a], cv[bb], H[id]);
+}
 
 // Table rows: row = min(4*gx + gy, 11) for the 2-bit codes gx, gy of a site;
 // rows 0,1,2,4,5,6,8,9,10 are the nine called genotype pairs, rows 3, 7 and 11
@@ -383,28 +461,8 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
                 cv[2 * j] = v.x;
                 cv[2 * j + 1] = v.y;
             }
-            // L = eps + c.D as an fma chain over columns 0..8 (objective.rs:58-63);
             // column 9 holds eps, or 1.0 on a missing row
-            double L = cv[9];
-#pragma unroll
-            for (int j = 0; j < 9; j++) L = fma(D[j], cv[j], L);
-            // sum of log L over a 16-site group = log of the product of the
-            // usable sites' L (a missing site multiplies by exactly 1.0)
-            P = P * L;
-            if ((k & (LOG_GROUP - 1)) == LOG_GROUP - 1) {
-                F = F + sleef_log_pos(P);
-                P = 1.0;
-            }
-            const double r = recip_rn(L);  // grad_hess.rs:122
-#pragma unroll
-            for (int j = 0; j < 9; j++) G[j] = fma(cv[j], r, G[j]);
-#pragma unroll
-            for (int j = 0; j < 9; j++) cv[j] = cv[j] * r;
-            int id = 0;
-#pragma unroll
-            for (int a = 0; a < 9; a++)
-#pragma unroll
-                for (int bb = a; bb < 9; bb++, id++) H[id] = fma(cv[a], cv[bb], H[id]);
+            site_update(cv, cv[9], D, k, P, F, G, H);
         }
         wx = nwx;
         wy = nwy;
@@ -421,6 +479,236 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
         for (int k = 0; k < 45; k++) out[(size_t)(10 + k) * nslots] = H[k];
         out[(size_t)55 * nslots] = N;
     }
+}
+
+// Soft likelihoods: the same pass with c_s built per pair and site from the two
+// likelihood triples in the operation order of mixture_row2 (mixture.rs:96-236):
+// 9 products lx[i] ly[j], then 45 fma into c (each block (i, j) in the
+// reference's order) with the site's 18 frequency products, which the
+// workgroup computes once per site into LDS (K_*).  gl holds sample-major rows
+// of v_pad triples (0 for a missing call); the 2-bit mask rows (code 3 =
+// missing) select the missing sites and count n_xy.
+enum : int {
+    K_P, K_P2, K_P3, K_P4, K_Q, K_Q2, K_Q3, K_Q4, K_PQ, K_2P2Q, K_P2Q, K_2P3Q, K_PQ2, K_P2Q2, K_2PQ, K_2PQSQ,
+    K_2PQ2, K_2PQ3, K_N
+};
+
+__device__ __forceinline__ void site_coefficients(double p, double q, double* K) {
+    const double p2 = p * p, p3 = p2 * p, q2 = q * q, q3 = q2 * q;
+    const double pq = p * q, p2q = p2 * q, pq2 = p * q2, p2q2 = p2 * q2;
+    const double p3q = p3 * q, pq3 = p * q3;
+    const double twopq = pq + pq, twop2q = p2q + p2q, twop3q = p3q + p3q, twopq2 = pq2 + pq2, twopq3 = pq3 + pq3;
+    K[K_P] = p;
+    K[K_P2] = p2;
+    K[K_P3] = p3;
+    K[K_P4] = p2 * p2;
+    K[K_Q] = q;
+    K[K_Q2] = q2;
+    K[K_Q3] = q3;
+    K[K_Q4] = q2 * q2;
+    K[K_PQ] = pq;
+    K[K_2P2Q] = twop2q;
+    K[K_P2Q] = p2q;
+    K[K_2P3Q] = twop3q;
+    K[K_PQ2] = pq2;
+    K[K_P2Q2] = p2q2;
+    K[K_2PQ] = twopq;
+    K[K_2PQSQ] = twopq * twopq;
+    K[K_2PQ2] = twopq2;
+    K[K_2PQ3] = twopq3;
+}
+
+// mixture.rs:138-233 block by block
+__device__ __forceinline__ void mixture_soft(const double* lx, const double* ly, const double* K, double* a) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) a[k] = 0.0;
+    double g = lx[0] * ly[0];  // (0,0) (0,0)
+    a[0] = fma(g, K[K_P], a[0]);
+    a[1] = fma(g, K[K_P2], a[1]);
+    a[2] = fma(g, K[K_P2], a[2]);
+    a[3] = fma(g, K[K_P3], a[3]);
+    a[4] = fma(g, K[K_P2], a[4]);
+    a[5] = fma(g, K[K_P3], a[5]);
+    a[6] = fma(g, K[K_P2], a[6]);
+    a[7] = fma(g, K[K_P3], a[7]);
+    a[8] = fma(g, K[K_P4], a[8]);
+    g = lx[0] * ly[1];  // (0,0) (0,1)
+    a[2] = fma(g, K[K_PQ], a[2]);
+    a[3] = fma(g, K[K_2P2Q], a[3]);
+    a[7] = fma(g, K[K_P2Q], a[7]);
+    a[8] = fma(g, K[K_2P3Q], a[8]);
+    g = lx[0] * ly[2];  // (0,0) (1,1)
+    a[1] = fma(g, K[K_PQ], a[1]);
+    a[3] = fma(g, K[K_PQ2], a[3]);
+    a[5] = fma(g, K[K_P2Q], a[5]);
+    a[8] = fma(g, K[K_P2Q2], a[8]);
+    g = lx[1] * ly[0];  // (0,1) (0,0)
+    a[4] = fma(g, K[K_PQ], a[4]);
+    a[5] = fma(g, K[K_2P2Q], a[5]);
+    a[7] = fma(g, K[K_P2Q], a[7]);
+    a[8] = fma(g, K[K_2P3Q], a[8]);
+    g = lx[1] * ly[1];  // (0,1) (0,1)
+    a[6] = fma(g, K[K_2PQ], a[6]);
+    a[7] = fma(g, K[K_PQ], a[7]);
+    a[8] = fma(g, K[K_2PQSQ], a[8]);
+    g = lx[1] * ly[2];  // (0,1) (1,1)
+    a[4] = fma(g, K[K_PQ], a[4]);
+    a[5] = fma(g, K[K_2PQ2], a[5]);
+    a[7] = fma(g, K[K_PQ2], a[7]);
+    a[8] = fma(g, K[K_2PQ3], a[8]);
+    g = lx[2] * ly[0];  // (1,1) (0,0)
+    a[1] = fma(g, K[K_PQ], a[1]);
+    a[3] = fma(g, K[K_P2Q], a[3]);
+    a[5] = fma(g, K[K_PQ2], a[5]);
+    a[8] = fma(g, K[K_P2Q2], a[8]);
+    g = lx[2] * ly[1];  // (1,1) (0,1)
+    a[2] = fma(g, K[K_PQ], a[2]);
+    a[3] = fma(g, K[K_2PQ2], a[3]);
+    a[7] = fma(g, K[K_PQ2], a[7]);
+    a[8] = fma(g, K[K_2PQ3], a[8]);
+    g = lx[2] * ly[2];  // (1,1) (1,1)
+    a[0] = fma(g, K[K_Q], a[0]);
+    a[1] = fma(g, K[K_Q2], a[1]);
+    a[2] = fma(g, K[K_Q2], a[2]);
+    a[3] = fma(g, K[K_Q3], a[3]);
+    a[4] = fma(g, K[K_Q2], a[4]);
+    a[5] = fma(g, K[K_Q3], a[5]);
+    a[6] = fma(g, K[K_Q2], a[6]);
+    a[7] = fma(g, K[K_Q3], a[7]);
+    a[8] = fma(g, K[K_Q4], a[8]);
+}
+
+constexpr int KT_BUF = CHUNK * K_N;  // doubles per coefficient buffer
+
+__global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_gl(const uint32_t* __restrict__ geno, uint32_t row_words,
+                                                          const double* __restrict__ gl, uint32_t v_pad,
+                                                          const double* __restrict__ af,
+                                                          const uint32_t* __restrict__ seg_bounds,
+                                                          const uint32_t* __restrict__ sx,
+                                                          const uint32_t* __restrict__ sy,
+                                                          const double* __restrict__ pt,
+                                                          const int* __restrict__ active, uint32_t nslots,
+                                                          double eps, double* __restrict__ partial) {
+    __shared__ __attribute__((aligned(16))) double kt[2 * KT_BUF];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t slot = blockIdx.x * SWEEP_TPB + tid;
+    const uint32_t seg = blockIdx.y;
+    const bool live = slot < nslots && active[slot] != 0;
+    if (!__syncthreads_or(live)) return;
+
+    double D[9];
+    uint32_t x = 0, y = 0;
+    if (live) {
+        x = sx[slot];
+        y = sy[slot];
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k] = pt[(size_t)k * nslots + slot];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k] = 0.0;
+    }
+    const uint32_t* rowx = geno + (size_t)x * row_words;
+    const uint32_t* rowy = geno + (size_t)y * row_words;
+    const double* glx = gl + (size_t)x * v_pad * 3;
+    const double* gly = gl + (size_t)y * v_pad * 3;
+
+    double F = 0.0, N = 0.0, P = 1.0, G[9], H[45];
+#pragma unroll
+    for (int k = 0; k < 9; k++) G[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 45; k++) H[k] = 0.0;
+
+    const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
+    auto build = [&](uint32_t cs, int b) {
+        if (tid < CHUNK) site_coefficients(af[2 * (cs + tid)], af[2 * (cs + tid) + 1], kt + b * KT_BUF + tid * K_N);
+    };
+    if (s0 < s1) build(s0, 0);
+    __syncthreads();
+    int b = 0;
+    for (uint32_t cs = s0; cs < s1; cs += CHUNK, b ^= 1) {
+        if (cs + CHUNK < s1) build(cs + CHUNK, b ^ 1);
+        const uint2 wx = *reinterpret_cast<const uint2*>(rowx + cs / 16);
+        const uint2 wy = *reinterpret_cast<const uint2*>(rowy + cs / 16);
+        {
+            const uint32_t ux = ~((wx.x & (wx.x >> 1)) | (wy.x & (wy.x >> 1))) & 0x55555555u;
+            const uint32_t uy = ~((wx.y & (wx.y >> 1)) | (wy.y & (wy.y >> 1))) & 0x55555555u;
+            N += (double)(__popc(ux) + __popc(uy));
+        }
+        const double* K0 = kt + b * KT_BUF;
+#pragma unroll
+        for (int k = 0; k < CHUNK; k += 2) {
+            // two sites of likelihood triples = 48 bytes, 16-byte aligned
+            const double2* px = reinterpret_cast<const double2*>(glx + (size_t)(cs + k) * 3);
+            const double2* py = reinterpret_cast<const double2*>(gly + (size_t)(cs + k) * 3);
+            const double2 x0 = px[0], x1 = px[1], x2 = px[2];
+            const double2 y0 = py[0], y1 = py[1], y2 = py[2];
+            const double lx[2][3] = {{x0.x, x0.y, x1.x}, {x1.y, x2.x, x2.y}};
+            const double ly[2][3] = {{y0.x, y0.y, y1.x}, {y1.y, y2.x, y2.y}};
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int kk = k + h;
+                const uint32_t ax = (kk < 16 ? wx.x : wx.y) >> (2 * (kk & 15));
+                const uint32_t ay = (kk < 16 ? wy.x : wy.y) >> (2 * (kk & 15));
+                const bool missing = (ax & 3u) == 3u || (ay & 3u) == 3u;
+                double K[K_N];
+                const double2* k2 = reinterpret_cast<const double2*>(K0 + kk * K_N);
+#pragma unroll
+                for (int j = 0; j < K_N / 2; j++) {
+                    const double2 v = k2[j];
+                    K[2 * j] = v.x;
+                    K[2 * j + 1] = v.y;
+                }
+                double cv[9];
+                mixture_soft(lx[h], ly[h], K, cv);
+                site_update(cv, missing ? 1.0 : eps, D, kk, P, F, G, H);
+            }
+        }
+        __syncthreads();
+    }
+    if (live) {
+        double* out = partial + (size_t)seg * NPART * nslots + slot;
+        out[0] = F;
+#pragma unroll
+        for (int k = 0; k < 9; k++) out[(size_t)(1 + k) * nslots] = G[k];
+#pragma unroll
+        for (int k = 0; k < 45; k++) out[(size_t)(10 + k) * nslots] = H[k];
+        out[(size_t)55 * nslots] = N;
+    }
+}
+
+// Soft rows of the kept sites: thread (sample i, word w) moves sites
+// 16w..16w+15 from the site-major input (V x N x 3) into the sample-major
+// rows (0 for a missing call or a padding site) and writes the 2-bit mask word
+// (3 = a NaN in the triple, lanevector.rs:100-104; padding = 3).
+__global__ __launch_bounds__(256) void gather_gl_kernel(const double* __restrict__ gl_site, uint32_t n_samples,
+                                                         const uint32_t* __restrict__ kept, uint32_t n_kept,
+                                                         double* __restrict__ rows, uint32_t v_pad,
+                                                         uint32_t* __restrict__ mask, uint32_t row_words) {
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (size_t)n_samples * row_words) return;
+    const uint32_t i = (uint32_t)(tid / row_words), w = (uint32_t)(tid % row_words);
+    uint32_t word = 0;
+    for (int j = 0; j < 16; j++) {
+        const uint32_t t = w * 16 + j;
+        double l[3] = {0.0, 0.0, 0.0};
+        uint32_t code = 3;
+        if (t < n_kept) {
+            const double* src = gl_site + ((size_t)kept[t] * n_samples + i) * 3;
+            const double a = src[0], b2 = src[1], c = src[2];
+            if (!(isnan(a) || isnan(b2) || isnan(c))) {
+                l[0] = a;
+                l[1] = b2;
+                l[2] = c;
+                code = 0;
+            }
+        }
+        double* dst = rows + ((size_t)i * v_pad + t) * 3;
+        dst[0] = l[0];
+        dst[1] = l[1];
+        dst[2] = l[2];
+        word |= code << (2 * j);
+    }
+    mask[(size_t)i * row_words + w] = word;
 }
 
 // ------------------------------------------------------------ SQP step
@@ -993,10 +1281,30 @@ hipError_t gather(const uint8_t* src, size_t src_row_bytes, const uint32_t* kept
 
 hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, const uint32_t* seg_bounds,
                  uint32_t nseg, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
-                 uint32_t nslots, double eps, double* partial, hipStream_t st) {
+                 uint32_t nslots, double eps, double* partial, hipStream_t st, const double* gl, uint32_t v_pad) {
     dim3 grid((nslots + SWEEP_TPB - 1) / SWEEP_TPB, nseg);
-    hipLaunchKernelGGL(sweep_fp64, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, af, seg_bounds, sx, sy, pt, active,
-                       nslots, eps, partial);
+    if (gl)
+        hipLaunchKernelGGL(sweep_gl, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, af, seg_bounds, sx, sy,
+                           pt, active, nslots, eps, partial);
+    else
+        hipLaunchKernelGGL(sweep_fp64, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, af, seg_bounds, sx, sy, pt,
+                           active, nslots, eps, partial);
+    return hipGetLastError();
+}
+
+hipError_t af_gl(const double* gl, uint32_t n_samples, uint32_t n_sites, const Tune& t, double maf, double* af_out,
+                 int* status, hipStream_t st) {
+    hipLaunchKernelGGL(af_gl_kernel, dim3((n_sites + 63) / 64), dim3(64), 0, st, gl, n_samples, n_sites, t, maf, af_out,
+                       status);
+    return hipGetLastError();
+}
+
+hipError_t gather_gl(const double* gl_site, uint32_t n_samples, const uint32_t* kept, uint32_t n_kept, double* rows,
+                     uint32_t v_pad, uint32_t* mask, uint32_t row_words, hipStream_t st) {
+    const size_t n = (size_t)n_samples * row_words;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_gl_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, gl_site, n_samples, kept,
+                       n_kept, rows, v_pad, mask, row_words);
     return hipGetLastError();
 }
 

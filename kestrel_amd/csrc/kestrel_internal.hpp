@@ -49,12 +49,19 @@ struct PairQueue {
 namespace klaunch {
 hipError_t af(const uint8_t* packed, size_t row_bytes, uint32_t n_samples, uint32_t n_sites, const kdev::Tune& t,
               double maf, double* af_out, int* status, hipStream_t st);
+// soft likelihoods: allele frequencies from site-major V x N x 3 rows, and
+// the kept sites into sample-major rows (v_pad triples) + 2-bit mask rows
+hipError_t af_gl(const double* gl, uint32_t n_samples, uint32_t n_sites, const kdev::Tune& t, double maf,
+                 double* af_out, int* status, hipStream_t st);
+hipError_t gather_gl(const double* gl_site, uint32_t n_samples, const uint32_t* kept, uint32_t n_kept, double* rows,
+                     uint32_t v_pad, uint32_t* mask, uint32_t row_words, hipStream_t st);
 hipError_t gather(const uint8_t* src, size_t src_row_bytes, const uint32_t* kept, uint32_t n_kept,
                   uint32_t n_samples, uint32_t* dst, uint32_t dst_row_words, hipStream_t st,
                   const uint32_t* dst_rows = nullptr);
 hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, const uint32_t* seg_bounds,
                  uint32_t nseg, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
-                 uint32_t nslots, double eps, double* partial, hipStream_t st);
+                 uint32_t nslots, double eps, double* partial, hipStream_t st,
+                 const double* gl = nullptr, uint32_t v_pad = 0);
 hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* active, uint32_t nslots,
                 double* partial, uint32_t nseg, const uint32_t* pairs, uint64_t npairs,
                 unsigned long long* next_pair, kestrel_output* out, const kdev::Tune& t, int* n_live,

@@ -141,6 +141,28 @@ def config(name: str, **overrides) -> Cohort:
     return generate(**kw)
 
 
+def soft_likelihoods(codes: np.ndarray, seed: int, mean_depth: float = 4.0, error: float = 0.01) -> np.ndarray:
+    """Linear-scale genotype likelihoods (V, N, 3) of 0/0, 0/1, 1/1, site-major
+    like the reference's GenotypeLikelihoods: each call gets Poisson(mean_depth)
+    reads, each read shows the ALT allele with probability error, 1/2 or
+    1 - error under the true genotype, and the binomial likelihoods are scaled
+    by their maximum (vcf.rs:368 normalises GL the same way), so one entry is
+    exactly 1.  A call with no read, or a missing code, is a NaN row."""
+    rng = np.random.default_rng(seed)
+    n, v = codes.shape
+    depth = rng.poisson(mean_depth, (n, v))
+    alt_p = np.array([error, 0.5, 1.0 - error, 0.0])[codes]
+    alt = rng.binomial(depth, alt_p)
+    ref = depth - alt
+    # log P(reads | G) up to a constant, G = 0/0, 0/1, 1/1
+    pa = np.array([error, 0.5, 1.0 - error])
+    ll = alt[..., None] * np.log(pa) + ref[..., None] * np.log1p(-pa)
+    ll -= ll.max(axis=-1, keepdims=True)
+    gl = np.exp(ll)
+    gl[(depth == 0) | (codes == 3)] = np.nan
+    return np.ascontiguousarray(gl.transpose(1, 0, 2))
+
+
 def pack2(codes: np.ndarray, row_sites: int | None = None) -> np.ndarray:
     """Pack codes (N, V) into 2 bits per call, 4 calls per byte, site s of a
     row in bits 2*(s%4) of byte s//4.  Rows are padded with code 3 (missing)

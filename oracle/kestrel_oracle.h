@@ -121,6 +121,18 @@ int ko_fit_pairs_codes(int order, const uint8_t* codes, uint32_t N, uint32_t V, 
                        const uint32_t* pairs, size_t npairs, uint32_t nsel, const uint32_t* seg, size_t nseg,
                        uint64_t iterations, int nthreads, ko_output* out);
 
+/* Soft likelihoods (vcf.rs:338-410 rows: 10^(gl - max) or 10^(-pl/10), NaN =
+ * missing), sample-major N x V x 3.  Pair rows of one pair (V x 3 each), the
+ * allele-frequency fit (allele/mod.rs:37-134) and the pair fits
+ * (coefficients.rs:40-141) as their code counterparts. */
+size_t ko_pair_rows_gl(const double* lx, const double* ly, const double* af, size_t V, double* rows,
+                       uint8_t* valid);
+int ko_allele_freqs_gl(const double* gl, uint32_t N, uint32_t V, uint64_t iterations, double maf, int nthreads,
+                       double* af_out);
+int ko_fit_pairs_gl(int order, const double* gl, uint32_t N, uint32_t V, const double* af, const uint32_t* pairs,
+                    size_t npairs, uint32_t nsel, const uint32_t* seg, size_t nseg, uint64_t iterations, int nthreads,
+                    ko_output* out);
+
 /* CLS / Brier path of -f GT (coefficients.rs:166-320, cls.rs, quadratic.rs,
  * wrapper.rs:412-428).  codes N x V without missing calls. */
 void ko_gt_allele_counts(const uint8_t* codes, uint32_t N, uint32_t V, double* af);

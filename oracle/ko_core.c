@@ -796,6 +796,24 @@ size_t ko_pair_rows_codes(const uint8_t* gx, const uint8_t* gy, const double* af
     return n;
 }
 
+/* Soft likelihoods: lx, ly are V x 3 linear-scale likelihood rows (vcf.rs:
+ * 10^(gl - max) or 10^(-pl/10)); a row with a NaN drops the site for the pair. */
+size_t ko_pair_rows_gl(const double* lx, const double* ly, const double* af, size_t V, double* rows,
+                       uint8_t* valid) {
+    size_t n = 0;
+    for (size_t s = 0; s < V; s++) {
+        double c[9];
+        ko_mixture_row2(lx + 3 * s, ly + 3 * s, af + 2 * s, c);
+        int ok = 1;
+        for (int k = 0; k < 9; k++)
+            if (c[k] != c[k]) ok = 0; /* lanevector.rs:100-104 */
+        valid[s] = (uint8_t)ok;
+        for (int k = 0; k < 9; k++) rows[s * 9 + k] = ok ? c[k] : 0.0;
+        n += ok;
+    }
+    return n;
+}
+
 /* ------------------------------------------------- allele frequencies */
 
 typedef struct {
@@ -866,6 +884,7 @@ int ko_allele_freq2_rows(const double* gl, size_t n, uint64_t iterations, double
 
 typedef struct {
     const uint8_t* codes;
+    const double* gl; /* N x V x 3 likelihood rows, instead of codes when set */
     uint32_t N, V;
     uint64_t iterations;
     double maf;
@@ -883,7 +902,9 @@ static void* af_worker(void* vj) {
         if (s0 >= j->V) break;
         size_t s1 = s0 + 256 < j->V ? s0 + 256 : j->V;
         for (size_t s = s0; s < s1; s++) {
-            for (uint32_t i = 0; i < j->N; i++) memcpy(gl + 3 * i, ONEHOT[j->codes[(size_t)i * j->V + s] & 3], 24);
+            for (uint32_t i = 0; i < j->N; i++)
+                memcpy(gl + 3 * i, j->gl ? j->gl + ((size_t)i * j->V + s) * 3 : ONEHOT[j->codes[(size_t)i * j->V + s] & 3],
+                       24);
             if (allele_site(gl, j->N, blocks, j->iterations, j->maf, j->af + 2 * s, NULL)) atomic_store(&j->err, -1);
         }
     }
@@ -898,10 +919,11 @@ static int nthreads_default(int n) {
     return c > 0 ? (int)c : 1;
 }
 
-int ko_allele_freqs_codes(const uint8_t* codes, uint32_t N, uint32_t V, uint64_t iterations, double maf,
-                          int nthreads, double* af_out) {
+static int allele_freqs(const uint8_t* codes, const double* glrows, uint32_t N, uint32_t V, uint64_t iterations,
+                        double maf, int nthreads, double* af_out) {
     af_job j;
     j.codes = codes;
+    j.gl = glrows;
     j.N = N;
     j.V = V;
     j.iterations = iterations;
@@ -917,11 +939,22 @@ int ko_allele_freqs_codes(const uint8_t* codes, uint32_t N, uint32_t V, uint64_t
     return atomic_load(&j.err);
 }
 
+int ko_allele_freqs_codes(const uint8_t* codes, uint32_t N, uint32_t V, uint64_t iterations, double maf,
+                          int nthreads, double* af_out) {
+    return allele_freqs(codes, NULL, N, V, iterations, maf, nthreads, af_out);
+}
+
+int ko_allele_freqs_gl(const double* gl, uint32_t N, uint32_t V, uint64_t iterations, double maf, int nthreads,
+                       double* af_out) {
+    return allele_freqs(NULL, gl, N, V, iterations, maf, nthreads, af_out);
+}
+
 /* ------------------------------------------------------- pair driver */
 
 typedef struct {
     int order;
     const uint8_t* codes;
+    const double* gl; /* N x V x 3 likelihood rows, instead of codes when set */
     uint32_t N, V;
     const double* af;
     const uint32_t* pairs;
@@ -965,13 +998,13 @@ static void* fit_worker(void* vj) {
             y = j->pairs[2 * i + 1];
         } else
             tri_pair(i, j->nsel, &x, &y);
-        const uint8_t* gx = j->codes + (size_t)x * V;
-        const uint8_t* gy = j->codes + (size_t)y * V;
         ko_output* o = j->out + i;
         memset(o, 0, sizeof *o);
         o->x = x;
         o->y = y;
-        size_t n = ko_pair_rows_codes(gx, gy, j->af, V, rows, valid);
+        size_t n = j->gl ? ko_pair_rows_gl(j->gl + (size_t)x * V * 3, j->gl + (size_t)y * V * 3, j->af, V, rows, valid)
+                         : ko_pair_rows_codes(j->codes + (size_t)x * V, j->codes + (size_t)y * V, j->af, V, rows,
+                                              valid);
         o->n_sites = (uint32_t)n;
         if (n == 0) { /* lanevector.rs:133-138 would panic; report NaN */
             for (int k = 0; k < 9; k++) o->jacquard[k] = NAN;
@@ -1007,14 +1040,15 @@ static void* fit_worker(void* vj) {
     return NULL;
 }
 
-int ko_fit_pairs_codes(int order, const uint8_t* codes, uint32_t N, uint32_t V, const double* af,
-                       const uint32_t* pairs, size_t npairs, uint32_t nsel, const uint32_t* seg, size_t nseg,
-                       uint64_t iterations, int nthreads, ko_output* out) {
+static int fit_pairs(int order, const uint8_t* codes, const double* glrows, uint32_t N, uint32_t V,
+                     const double* af, const uint32_t* pairs, size_t npairs, uint32_t nsel, const uint32_t* seg,
+                     size_t nseg, uint64_t iterations, int nthreads, ko_output* out) {
     if (nsel > N) return -1; /* coefficients.rs:53-55 */
     fit_job j;
     memset(&j, 0, sizeof j);
     j.order = order;
     j.codes = codes;
+    j.gl = glrows;
     j.N = N;
     j.V = V;
     j.af = af;
@@ -1032,6 +1066,18 @@ int ko_fit_pairs_codes(int order, const uint8_t* codes, uint32_t N, uint32_t V, 
     for (int i = 0; i < nt; i++) pthread_join(th[i], NULL);
     free(th);
     return 0;
+}
+
+int ko_fit_pairs_codes(int order, const uint8_t* codes, uint32_t N, uint32_t V, const double* af,
+                       const uint32_t* pairs, size_t npairs, uint32_t nsel, const uint32_t* seg, size_t nseg,
+                       uint64_t iterations, int nthreads, ko_output* out) {
+    return fit_pairs(order, codes, NULL, N, V, af, pairs, npairs, nsel, seg, nseg, iterations, nthreads, out);
+}
+
+int ko_fit_pairs_gl(int order, const double* gl, uint32_t N, uint32_t V, const double* af, const uint32_t* pairs,
+                    size_t npairs, uint32_t nsel, const uint32_t* seg, size_t nseg, uint64_t iterations, int nthreads,
+                    ko_output* out) {
+    return fit_pairs(order, NULL, gl, N, V, af, pairs, npairs, nsel, seg, nseg, iterations, nthreads, out);
 }
 
 /* ------------------------------------------------- CLS (Brier) path, -f GT */

@@ -91,6 +91,12 @@ def lib():
         L.ko_allele_freq2_rows.argtypes = [P, SZ, U64, D, P, P]
         L.ko_fit_pairs_codes.restype = I
         L.ko_fit_pairs_codes.argtypes = [I, P, U32, U32, P, P, SZ, U32, P, SZ, U64, I, P]
+        L.ko_allele_freqs_gl.restype = I
+        L.ko_allele_freqs_gl.argtypes = [P, U32, U32, U64, D, I, P]
+        L.ko_fit_pairs_gl.restype = I
+        L.ko_fit_pairs_gl.argtypes = [I, P, U32, U32, P, P, SZ, U32, P, SZ, U64, I, P]
+        L.ko_pair_rows_gl.restype = SZ
+        L.ko_pair_rows_gl.argtypes = [P, P, P, SZ, P, P]
         _lib = L
     return _lib
 
@@ -281,6 +287,51 @@ def fit_pairs(codes, af, pairs=None, nsel=None, order=ORDER_REF, seg=None, itera
     seg = _seg(seg)
     rc = lib().ko_fit_pairs_codes(order, _p(codes), n, v, _p(af), _p(pairs), npairs, nsel, _p(seg), len(seg) - 1,
                                   iterations, nthreads, _p(out))
+    if rc != 0:
+        raise ValueError("requested samples exceed total samples")
+    return out
+
+
+def allele_freqs_gl(gl, iterations=100, maf=None, nthreads=0):
+    """allele/mod.rs:37-134 on likelihood rows gl (N, V, 3), NaN = missing."""
+    gl = f64(gl)
+    n, v, _ = gl.shape
+    if maf is None:
+        maf = 1.0 / (2.0 * n)  # wrapper.rs:101
+    af = np.zeros((v, 2))
+    rc = lib().ko_allele_freqs_gl(_p(gl), n, v, iterations, maf, nthreads, _p(af))
+    if rc != 0:
+        raise ValueError("a site has no called sample")
+    return af
+
+
+def pair_rows_gl(lx, ly, af):
+    lx, ly, af = f64(lx), f64(ly), f64(af)
+    v = lx.shape[0]
+    rows = np.zeros((v, 9))
+    valid = np.zeros(v, np.uint8)
+    n = lib().ko_pair_rows_gl(_p(lx), _p(ly), _p(af), v, _p(rows), _p(valid))
+    return rows, valid, n
+
+
+def fit_pairs_gl(gl, af, pairs=None, nsel=None, order=ORDER_REF, seg=None, iterations=100, nthreads=0):
+    """Pair fits over kept sites from likelihood rows gl (N, V, 3) and af (V, 2)."""
+    gl = f64(gl)
+    af = f64(af)
+    n, v, _ = gl.shape
+    if nsel is None:
+        nsel = n
+    if pairs is not None:
+        pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        npairs = pairs.shape[0]
+    else:
+        npairs = nsel * (nsel + 1) // 2
+    out = np.zeros(npairs, OUTPUT_DTYPE)
+    if seg is None:
+        seg = np.array([0, v], np.uint32)
+    seg = _seg(seg)
+    rc = lib().ko_fit_pairs_gl(order, _p(gl), n, v, _p(af), _p(pairs), npairs, nsel, _p(seg), len(seg) - 1,
+                               iterations, nthreads, _p(out))
     if rc != 0:
         raise ValueError("requested samples exceed total samples")
     return out

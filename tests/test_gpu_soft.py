@@ -1,0 +1,112 @@
+"""Soft genotype likelihoods on the device (kestrel_cohort_create_gl, sweep_gl)
+against the oracle.  Bars as the hard-call parity: allele frequencies, masks,
+n_xy, Delta, objective, iterations and condition bit-exact in the device's
+summation order; the reference's order within the stated tolerance; one-hot
+rows through the likelihood path equal the 2-bit path bit for bit."""
+import numpy as np
+import pytest
+
+from kestrel_amd import Cohort, build_arrays, relatedness_coefficients_gl, segment_bounds, synth
+
+pytestmark = pytest.mark.gpu
+
+TOL_DELTA = 1e-6
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def same(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return (a == b) | (np.isnan(a) & np.isnan(b))
+
+
+def assert_bitexact(out, exp):
+    for k in ("x", "y", "jacquard", "obj", "iters", "n_sites", "cond"):
+        ok = same(out[k], exp[k]).reshape(len(out), -1).all(1)
+        assert ok.all(), f"{k}: {int((~ok).sum())} of {len(out)} pairs differ"
+
+
+@pytest.fixture(scope="module")
+def soft():
+    c = synth.generate(64, 9000, seed=31, n_family_pairs=10, missing=0.02)
+    gl = synth.soft_likelihoods(c.codes, seed=32, mean_depth=4.0)  # (V, N, 3)
+    gl[5, 7, 1] = np.nan  # one value missing: the whole call is missing (lanevector.rs:100-104)
+    co = Cohort.gl(gl)
+    yield c, gl, co
+    co.close()
+
+
+def test_one_hot_rows_equal_the_2bit_path():
+    say("case: one-hot likelihood rows vs 2-bit codes, 40 x 4000")
+    c = synth.generate(40, 4000, seed=33, missing=0.03)
+    gl = np.eye(3)[np.minimum(c.codes, 2)]
+    gl[c.codes == 3] = np.nan
+    with Cohort.gl(np.ascontiguousarray(gl.transpose(1, 0, 2))) as a, Cohort(c.codes) as b:
+        afa, ka = a.allele_freqs()
+        afb, kb = b.allele_freqs()
+        assert np.array_equal(afa, afb) and np.array_equal(ka, kb)
+        assert_bitexact(a.fit_pairs(), b.fit_pairs())
+
+
+def test_soft_allele_freqs_bitexact(O, soft):
+    c, gl, co = soft
+    say("case: soft allele frequencies 64 x 9000")
+    af, kept = co.allele_freqs()
+    exp = O.allele_freqs_gl(gl.transpose(1, 0, 2))
+    assert np.array_equal(af, exp)
+    assert np.array_equal(kept, O.keep_sites(exp))
+
+
+def test_soft_fit_bitexact_and_tolerance(O, soft):
+    c, gl, co = soft
+    say("case: soft fit, all 2080 pairs")
+    af, kept = co.allele_freqs()
+    g = np.ascontiguousarray(gl.transpose(1, 0, 2)[:, kept])
+    out = co.fit_pairs()
+    seg = np.minimum(segment_bounds(co.n_kept), g.shape[1]).astype(np.uint32)
+    assert_bitexact(out, O.fit_pairs_gl(g, af[kept], order=O.ORDER_SEG, seg=seg))
+    ref = O.fit_pairs_gl(g, af[kept], order=O.ORDER_REF)
+    lc = np.log10(ref["cond"])
+    well = np.isfinite(lc) & (lc < 12)
+    assert np.abs(out["jacquard"] - ref["jacquard"]).max(1)[well].max() <= TOL_DELTA
+    assert (np.abs(out["obj"] - ref["obj"]) / np.abs(ref["obj"]))[well].max() <= 1e-9
+
+
+def test_soft_evaluate_bitexact(O, soft):
+    c, gl, co = soft
+    say("case: soft evaluate, 100 points")
+    af, kept = co.allele_freqs()
+    g = gl.transpose(1, 0, 2)[:, kept]
+    rng = np.random.default_rng(1)
+    pairs = rng.integers(0, 64, (100, 2))
+    pairs[0] = (7, 9)  # sample 7 has the partly missing call
+    pts = rng.dirichlet(np.ones(9), 100)
+    f, gr, h, ns = co.evaluate(pairs, pts)
+    seg = np.minimum(segment_bounds(co.n_kept), g.shape[1]).astype(np.uint32)
+    for i in range(100):
+        rows, valid, n = O.pair_rows_gl(g[pairs[i, 0]], g[pairs[i, 1]], af[kept])
+        assert ns[i] == n
+        assert f[i] == O.obj_seg(rows, valid, seg, pts[i])
+        ge, he = O.gh_seg(rows, valid, seg, pts[i])
+        assert np.array_equal(gr[i], ge) and np.array_equal(h[i], he)
+
+
+def test_relatedness_coefficients_gl_soft(O):
+    say("case: relatedness_coefficients_gl on soft rows, 20 x 2500")
+    c = synth.generate(20, 2500, seed=34, missing=0.01)
+    gl = synth.soft_likelihoods(c.codes, seed=35)
+    jac, kin, inb = relatedness_coefficients_gl(gl)
+    g = gl.transpose(1, 0, 2)
+    af = O.allele_freqs_gl(g)
+    k = O.keep_sites(af)
+    seg = np.minimum(segment_bounds(int(k.sum())), int(k.sum())).astype(np.uint32)
+    ej, ek, ei = build_arrays(O.fit_pairs_gl(g[:, k], af[k], order=O.ORDER_SEG, seg=seg), 20)
+    assert np.array_equal(jac, ej) and np.array_equal(kin, ek) and np.array_equal(inb, ei)
+
+
+def test_serving_rejects_soft_cohort(soft):
+    c, gl, co = soft
+    with pytest.raises(Exception):
+        co.serve_start(max_query_rows=4, max_pairs=100)
