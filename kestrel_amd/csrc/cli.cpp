@@ -6,8 +6,9 @@
 //           [-i ITERATIONS] [-m MAF] [-s SAMPLES]
 //
 // -f GT (default) runs the CLS / Brier estimator (coefficients.rs:166-278);
-// -f GL / PL run the maximum-likelihood fit (coefficients.rs:40-141) on
-// hard-call likelihoods.
+// -f GL / PL run the maximum-likelihood fit (coefficients.rs:40-141): on the
+// 2-bit cohort when every likelihood row is one-hot (hard calls), else on the
+// likelihood rows.
 #include <sys/resource.h>
 
 #include <charconv>
@@ -138,6 +139,7 @@ struct Args {
     bool have_maf = false;
     long samples = -1;
     bool dump_codes = false;     // print the decoded calls and stop (no GPU)
+    std::string dump_probs;      // write the GL / PL likelihood rows to this file and stop (no GPU)
     bool format_floats = false;  // print stdin doubles as TSV text and stop
 };
 
@@ -166,6 +168,10 @@ Args parse(int argc, char** argv) {
         };
         if (s == "--dump-codes") {
             a.dump_codes = true;
+            continue;
+        }
+        if (s == "--dump-probs" && i + 1 < argc) {  // GL / PL likelihood rows as raw doubles
+            a.dump_probs = argv[++i];
             continue;
         }
         if (s == "--format-floats") {
@@ -244,6 +250,14 @@ int main(int argc, char** argv) {
     info("Found 0 tri-allelic sites");
     info("Found 0 tetra-allelic sites");
     const uint32_t n = (uint32_t)d.samples.size();
+    if (!a.dump_probs.empty()) {
+        // n_sites x n_samples x 3 doubles, site-major (the rows the likelihood path receives)
+        FILE* f = fopen(a.dump_probs.c_str(), "wb");
+        if (!f || fwrite(d.probs.data(), sizeof(double), d.probs.size(), f) != d.probs.size())
+            return die("cannot write " + a.dump_probs);
+        fclose(f);
+        return 0;
+    }
     if (a.dump_codes) {
         // samples, then one line per kept site: codes (0/1/2, 3 = missing) and AF
         for (uint32_t i = 0; i < n; i++) printf("%s%s", i ? "\t" : "", d.samples[i].c_str());
@@ -293,7 +307,12 @@ int main(int argc, char** argv) {
         }
         info("Calculating allele frequencies for " + std::to_string(d.n_sites) + " sites using " + std::to_string(n) +
              " samples");
-        rc = kestrel_cohort_create(&c, packed.data(), n, d.n_sites, row_bytes, af, maf, a.iterations, KESTREL_FP64);
+        // hard calls take the 2-bit path; any soft row, the likelihood path
+        if (d.one_hot)
+            rc = kestrel_cohort_create(&c, packed.data(), n, d.n_sites, row_bytes, af, maf, a.iterations,
+                                       KESTREL_FP64);
+        else
+            rc = kestrel_cohort_create_gl(&c, d.probs.data(), n, d.n_sites, af, maf, a.iterations);
         if (rc) return die(kestrel_last_error());
         tracker.checkpoint();
         uint32_t kept = 0;

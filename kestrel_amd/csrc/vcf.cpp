@@ -113,6 +113,7 @@ bool read(const std::string& path, Field field, bool parse_af, Data& out, std::s
     std::vector<std::pair<const char*, const char*>> cols, fmt, sub, vals;
     const char* key = field == Field::GT ? "GT" : field == Field::GL ? "GL" : "PL";
     std::vector<uint8_t> codes;
+    std::vector<double> site_probs;
     uint64_t recno = 0;
     while (r.line(line)) {
         if (line.empty()) continue;
@@ -160,6 +161,7 @@ bool read(const std::string& path, Field field, bool parse_af, Data& out, std::s
             }
         }
         codes.assign(n, 3);
+        if (field != Field::GT) site_probs.assign(3 * n, NAN);
         bool skip = false;
         for (size_t i = 0; i < n && !skip; i++) {
             split(cols[9 + i].first, cols[9 + i].second, ':', sub);
@@ -256,14 +258,14 @@ bool read(const std::string& path, Field field, bool parse_af, Data& out, std::s
                     else if (prob[k] == 1.0 && one < 0) one = k;
                     else if (prob[k] == 0.0) zeros++;
                 }
+                for (int k = 0; k < 3; k++) site_probs[3 * i + k] = prob[k];
                 if (any_nan) {
                     codes[i] = 3;  // the row is dropped for every pair with this sample
                 } else if (one >= 0 && zeros == 2) {
                     codes[i] = (uint8_t)one;
                 } else {
-                    err = "record " + std::to_string(recno) + ", sample " + out.samples[i] +
-                          ": genotype likelihoods are not one-hot (soft likelihoods are not supported by this build)";
-                    return false;
+                    codes[i] = 0;  // a soft row: only the likelihood path reads it
+                    out.one_hot = false;
                 }
             }
         }
@@ -302,6 +304,7 @@ bool read(const std::string& path, Field field, bool parse_af, Data& out, std::s
             }
         }
         out.site_codes.insert(out.site_codes.end(), codes.begin(), codes.end());
+        if (field != Field::GT) out.probs.insert(out.probs.end(), site_probs.begin(), site_probs.end());
         out.n_sites++;
     }
     if (!header) {

@@ -15,6 +15,8 @@ struct Data {
     uint32_t n_sites = 0;
     std::vector<uint8_t> site_codes;  // n_sites x n_samples (site-major), 0/1/2 genotype index, 3 missing
     std::vector<double> af;           // n_sites x 2 when INFO AF was requested
+    std::vector<double> probs;        // GL / PL: n_sites x n_samples x 3 linear-scale likelihoods (NaN = missing)
+    bool one_hot = true;              // GL / PL: every called row has one 1 and two 0
     uint64_t total = 0, not_snp = 0, skipped_missing = 0, multiallelic = 0;
 
     // 2-bit rows, sample-major, site s of a row in bits 2*(s%4) of byte s/4
@@ -23,9 +25,10 @@ struct Data {
 
 // Reads `path`; returns false with `err` set on failure (the reference's
 // anyhow errors).  GT: a site with a missing call is skipped (vcf.rs:217-221).
-// GL / PL: likelihood rows become codes when they are one-hot (the called
-// genotype has probability 1, the others 0) and code 3 when a value is
-// missing (vcf.rs:355, 397).
+// GL / PL: every row is kept as its linear-scale likelihoods (vcf.rs:368, 407)
+// in `probs`; a row with a NaN (a missing value, vcf.rs:355, 397) has code 3,
+// a one-hot row (the called genotype 1, the others 0) the called genotype's
+// code; `one_hot` says whether every called row was one-hot.
 bool read(const std::string& path, Field field, bool parse_af, Data& out, std::string& err);
 
 }  // namespace kvcf

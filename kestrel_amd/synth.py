@@ -182,11 +182,14 @@ def unpack2(packed: np.ndarray, n_sites: int) -> np.ndarray:
     return out[:, :n_sites].copy()
 
 
-def write_vcf(cohort: Cohort, path: str, field: str = "GT", af_info: bool = False, names=None) -> None:
+def write_vcf(cohort: Cohort, path: str, field: str = "GT", af_info: bool = False, names=None,
+              gl: np.ndarray | None = None) -> None:
     """Write the cohort as a VCF (plain text, or gzip when path ends in .gz).
     GT: 0/0, 0/1, 1/1, ./. for a missing call.  GL / PL: one-hot likelihoods
     (log10 likelihood 0 for the call and -1000 otherwise; Phred 0 and 10000),
-    '.,.,.' for a missing call.  af_info writes INFO AF = 1 - p."""
+    '.,.,.' for a missing call; or, with gl (V, N, 3) linear-scale
+    likelihoods, their log10 (GL, 7 significant digits) or Phred-scaled
+    integers (PL).  af_info writes INFO AF = 1 - p."""
     import gzip
 
     codes = cohort.codes
@@ -210,5 +213,15 @@ def write_vcf(cohort: Cohort, path: str, field: str = "GT", af_info: bool = Fals
         for s in range(v):
             ref, alt = bases[s % 4], bases[(s + 1) % 4]
             inf = f"AF={1.0 - cohort.p[s]:.6g}" if af_info else "."
-            calls = "\t".join(table[codes[:, s]])
+            if gl is not None and field != "GT":
+                with np.errstate(divide="ignore"):
+                    lg = np.log10(gl[s])
+                if field == "GL":
+                    cells = [".,.,." if np.isnan(r).any() else ",".join(f"{x:.7g}" for x in r) for r in lg]
+                else:
+                    cells = [".,.,." if np.isnan(r).any() else ",".join(str(int(min(round(-10 * x), 99999)))
+                                                                         for x in r) for r in lg]
+                calls = "\t".join(cells)
+            else:
+                calls = "\t".join(table[codes[:, s]])
             f.write(f"1\t{s + 1}\t.\t{ref}\t{alt}\t.\tPASS\t{inf}\t{field}\t{calls}\n")

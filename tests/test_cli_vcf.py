@@ -3,6 +3,7 @@ its TSV float text (ryu, the csv crate's f64 writer).  Decoding is compared
 bit-exactly: codes 0/1/2 = unphased genotype index (coefficients.rs:18-38),
 3 = missing."""
 import gzip
+import math
 import os
 import subprocess
 
@@ -77,8 +78,24 @@ def test_gl_pl_decoding(tmp_path):
         # a NaN in any likelihood drops the row (vcf.rs:355,397; lanevector.rs:100-104)
         assert [s[0] for s in sites] == ["012", "313"]
         assert sites[0][1:] == ["0.75", "0.25"] and sites[1][1:] == ["0.5", "0.5"]
-    soft = write(tmp_path, "1\t1\t.\tA\tG\t.\tPASS\t.\tPL\t0,30,300\t0,9999,9999\t0,9999,9999\n", "s.vcf")
-    assert "not one-hot" in dump(soft, "PL", expect_ok=False)
+    # soft rows are kept as likelihoods: PL -> 10^(-pl/10) (vcf.rs:407), GL ->
+    # 10^(gl - max gl) over the float32 values (vcf.rs:349-368); libm pow
+    soft = write(tmp_path, "1\t1\t.\tA\tG\t.\tPASS\t.\tGL:PL\t-0.3,0,-2.7:0,30,300\t"
+                           "-1.25,-0.5,.:3,0,7\t0,-1000,-1000:9999,0,9999\n", "s.vcf")
+    f32 = lambda t: float(np.float32(t))
+    exp = {"PL": [[10 ** -0.0, math.pow(10.0, -3.0), math.pow(10.0, -30.0)],
+                  [math.pow(10.0, -0.3), 1.0, math.pow(10.0, -0.7)],
+                  [math.pow(10.0, -999.9), 1.0, math.pow(10.0, -999.9)]],
+           "GL": [[math.pow(10.0, f32("-0.3") - 0.0), 1.0, math.pow(10.0, f32("-2.7") - 0.0)],
+                  [math.pow(10.0, f32("-1.25") - f32("-0.5")), 1.0, math.nan],
+                  [1.0, 0.0, 0.0]]}
+    for field in ("GL", "PL"):
+        out = str(tmp_path / f"p.{field}.bin")
+        r = subprocess.run([KESTREL, soft, "/dev/null", "-f", field, "--dump-probs", out], capture_output=True,
+                           text=True)
+        assert r.returncode == 0, r.stderr
+        got = np.fromfile(out, np.float64).reshape(1, 3, 3)[0]
+        assert np.array_equal(got, np.array(exp[field]), equal_nan=True), (field, got)
     bad = write(tmp_path, "1\t1\t.\tA\tG\t.\tPASS\t.\tGL\t0,-1000\t0,-1,-1\t0,-1,-1\n", "b.vcf")
     assert "Not enough genotype likelihoods" in dump(bad, "GL", expect_ok=False)
 

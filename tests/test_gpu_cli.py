@@ -92,3 +92,28 @@ def test_cli_samples_beyond_cohort(tmp_path):
     synth.write_vcf(c, vcf, "GT")
     r = subprocess.run([KESTREL, vcf, str(tmp_path / "o.tsv"), "-s", "6"], capture_output=True, text=True)
     assert r.returncode != 0 and "exceeds total samples" in r.stderr
+
+
+@pytest.mark.parametrize("field", ["GL", "PL"])
+def test_cli_soft_likelihoods_match_oracle(O, tmp_path, field):
+    """Soft GL / PL rows take the likelihood path; the oracle fits the rows the
+    CLI parsed (written back by --dump-probs)."""
+    print(f"case: cli -f {field} soft", flush=True)
+    c = synth.generate(24, 2500, seed=24, n_family_pairs=5, missing=0.01)
+    gl = synth.soft_likelihoods(c.codes, seed=25, mean_depth=5.0)
+    vcf = str(tmp_path / f"s.{field}.vcf.gz")
+    synth.write_vcf(c, vcf, field, gl=gl)
+    probs = str(tmp_path / "p.bin")
+    r = subprocess.run([KESTREL, vcf, "/dev/null", "-f", field, "--dump-probs", probs], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    g = np.fromfile(probs, np.float64).reshape(gl.shape).transpose(1, 0, 2)  # (N, V, 3)
+    rows, _ = run(vcf, str(tmp_path / "o.tsv"), "-f", field)
+    x, y, kin, jac, it, obj, cond = as_arrays(rows, c.names)
+    af = O.allele_freqs_gl(g)
+    kept = O.keep_sites(af)
+    gk = np.ascontiguousarray(g[:, kept])
+    seg = np.minimum(segment_bounds(int(kept.sum())), gk.shape[1]).astype(np.uint32)
+    exp = O.fit_pairs_gl(gk, af[kept], order=O.ORDER_SEG, seg=seg)
+    assert np.array_equal(jac, exp["jacquard"]) and np.array_equal(it, exp["iters"])
+    assert np.array_equal(obj, exp["obj"])
+    assert np.array_equal(cond, log10(exp["cond"]), equal_nan=True)
