@@ -50,14 +50,20 @@ def cpu_threads() -> int:
     return max(1, n)
 
 
-def cpu_baseline(codes, af, budget_s: float):
+def cpu_baseline(codes, af, budget_s: float, gl=None):
     """Oracle in the reference's order (8-lane AVX-512 loops, per-pair c_s buffer,
-    work stealing over pairs) on a seeded random sample of off-diagonal pairs."""
+    work stealing over pairs) on a seeded random sample of off-diagonal pairs;
+    gl (N, V, 3) likelihood rows instead of codes when given."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     threads = cpu_threads()
-    n = codes.shape[0]
+    n = codes.shape[0] if gl is None else gl.shape[0]
+
+    def fit(pairs):
+        if gl is None:
+            return O.fit_pairs(codes, af, pairs=pairs, order=O.ORDER_REF, nthreads=threads)
+        return O.fit_pairs_gl(gl, af, pairs=pairs, order=O.ORDER_REF, nthreads=threads)
     rng = np.random.default_rng(12345)
 
     def sample(m):
@@ -68,12 +74,12 @@ def cpu_baseline(codes, af, budget_s: float):
 
     probe = sample(2 * threads)
     t = time.perf_counter()
-    O.fit_pairs(codes, af, pairs=probe, order=O.ORDER_REF, nthreads=threads)
+    fit(probe)
     dt = time.perf_counter() - t
     m = int(max(2 * threads, min(200_000, budget_s / max(dt, 1e-3) * len(probe))))
     pairs = sample(m)
     t = time.perf_counter()
-    O.fit_pairs(codes, af, pairs=pairs, order=O.ORDER_REF, nthreads=threads)
+    fit(pairs)
     dt = time.perf_counter() - t
     return {"value": m / dt, "unit": "pair-fits/s", "cores": threads, "kind": "port",
             "isa": O.isa(), "sample": f"{m} random off-diagonal pairs of the same cohort, {dt:.1f} s"}
@@ -204,6 +210,8 @@ def main():
     ap.add_argument("--slots", type=int, default=None)
     ap.add_argument("--streams", type=int, default=None, help="slot groups on separate streams (1 or 2)")
     ap.add_argument("--serving", action="store_true", help="serving workload (default config c5)")
+    ap.add_argument("--soft", action="store_true", help="soft genotype likelihoods instead of hard calls")
+    ap.add_argument("--depth", type=float, default=4.0, help="mean read depth of --soft likelihoods")
     ap.add_argument("--requests", type=int, default=12, help="requests per serving phase")
     ap.add_argument("--load", type=float, default=0.7, help="Poisson offered load / saturated capacity")
     ap.add_argument("--policy", choices=("srpt", "rr"), default="srpt",
@@ -224,9 +232,13 @@ def main():
     t = time.perf_counter()
     c = synth.config(args.config)
     n, v = c.codes.shape
-    log(f"cohort {args.config}: {n} x {v} generated in {time.perf_counter() - t:.1f}s")
+    gl = None
+    if args.soft:
+        gl = synth.soft_likelihoods(c.codes, seed=0x50F7, mean_depth=args.depth)
+    log(f"cohort {args.config}: {n} x {v}{' soft likelihoods' if args.soft else ''} generated in "
+        f"{time.perf_counter() - t:.1f}s")
     t = time.perf_counter()
-    co = Cohort(c.codes)
+    co = Cohort.gl(gl) if args.soft else Cohort(c.codes)
     if args.segment_sites:
         co.set_option("segment_sites", args.segment_sites)
     if args.slots:
@@ -263,8 +275,9 @@ def main():
     achieved = flops / (busy_ms / 1e3) / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_VALU_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_VALU_PEAK_TFLOPS, 4),
-                "traffic": SWEEP_TRAFFIC_BYTES_C2 if args.config == "c2" and not args.segment_sites else None,
-                "kernel": "sweep_fp64", "avg_launch_ms": round(sweep_ms / sweeps, 4),
+                "traffic": (SWEEP_TRAFFIC_BYTES_C2 if args.config == "c2" and not args.segment_sites
+                            and not args.soft else None),
+                "kernel": "sweep_gl" if args.soft else "sweep_fp64", "avg_launch_ms": round(sweep_ms / sweeps, 4),
                 "concurrent_launches": groups, "sweep_busy_ms_per_step": round(busy_ms / args.steps, 2),
                 "algorithmic_flop_per_launch": flops / sweeps}
     measured_peak = None
@@ -280,15 +293,18 @@ def main():
         log(f"fp64 peak measurement failed: {e}")
     cpu = None
     if args.cpu_budget > 0:
-        cpu = cpu_baseline(c.codes[:, kept], af[kept], args.cpu_budget)
+        glk = None if gl is None else np.ascontiguousarray(gl.transpose(1, 0, 2)[:, kept])
+        cpu = cpu_baseline(c.codes[:, kept], af[kept], args.cpu_budget, gl=glk)
         log(f"cpu baseline: {cpu}")
     line = {
         "metric": "pair-fits/sec", "value": round(value, 1), "unit": "pair-fits/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(total / args.steps * 1e3, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{args.config}: {n} samples x {v} sites ({co.n_kept} segregating), "
-                               f"{offdiag} off-diagonal pairs (+{len(out) - offdiag} self pairs), FP64",
-                   "samples": n, "sites": v, "pairs": offdiag, "precision": "fp64"},
+                               f"{offdiag} off-diagonal pairs (+{len(out) - offdiag} self pairs), FP64"
+                               + (f", soft likelihoods (mean depth {args.depth})" if args.soft else ""),
+                   "samples": n, "sites": v, "pairs": offdiag, "precision": "fp64",
+                   "input": "soft genotype likelihoods" if args.soft else "hard calls"},
         "roofline": roofline, "cpu_baseline": cpu,
         "iterations_mean": round(float(out["iters"].mean()), 3),
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
