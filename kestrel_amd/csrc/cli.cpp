@@ -48,7 +48,7 @@ void help() {
         "                                     [possible values: estimate, info]\n"
         "  -f, --field <FIELD>                Which FORMAT field to compute kinship from [default: GT]\n"
         "                                     [possible values: GT, GL, PL]\n"
-        "  -t, --threads <THREADS>            Number of threads (accepted; the fit runs on the GPU)\n"
+        "  -t, --threads <THREADS>            Number of threads reading the VCF (the fit runs on the GPU)\n"
         "  -i, --iterations <ITERATIONS>      Maximum number of iterations for minimization algorithm [default: 100]\n"
         "  -m, --minimum-allele-frequency <MINIMUM_ALLELE_FREQUENCY>\n"
         "                                     Allele frequencies below this threshold are set to zero\n"
@@ -242,7 +242,7 @@ int main(int argc, char** argv) {
     info("Parsing VCF \"" + a.input + "\"");
     kvcf::Data d;
     std::string err;
-    if (!kvcf::read(a.input, a.field, a.af_info, d, err)) return die(err);
+    if (!kvcf::read(a.input, a.field, a.af_info, d, err, a.threads > 0 ? (unsigned)a.threads : 0)) return die(err);
     info("Parsed " + std::to_string(d.total) + " total sites");
     info("Skipped " + std::to_string(d.not_snp) + " sites that were not SNPs");
     if (a.field == kvcf::Field::GT) info("Skipped " + std::to_string(d.skipped_missing) + " sites with missing data");
@@ -262,10 +262,13 @@ int main(int argc, char** argv) {
         // samples, then one line per kept site: codes (0/1/2, 3 = missing) and AF
         for (uint32_t i = 0; i < n; i++) printf("%s%s", i ? "\t" : "", d.samples[i].c_str());
         printf("\n");
+        std::string line;
         for (uint32_t s = 0; s < d.n_sites; s++) {
-            for (uint32_t i = 0; i < n; i++) putchar('0' + d.site_codes[(size_t)s * n + i]);
-            if (a.af_info) printf("\t%s\t%s", fmt_f64(d.af[2 * s]).c_str(), fmt_f64(d.af[2 * s + 1]).c_str());
-            putchar('\n');
+            line.assign(n, '0');
+            for (uint32_t i = 0; i < n; i++) line[i] = (char)('0' + d.site_codes[(size_t)s * n + i]);
+            if (a.af_info) line += "\t" + fmt_f64(d.af[2 * s]) + "\t" + fmt_f64(d.af[2 * s + 1]);
+            line += '\n';
+            fwrite(line.data(), 1, line.size(), stdout);
         }
         fprintf(stderr, "total=%llu not_snp=%llu skipped_missing=%llu\n", (unsigned long long)d.total,
                 (unsigned long long)d.not_snp, (unsigned long long)d.skipped_missing);
@@ -278,7 +281,7 @@ int main(int argc, char** argv) {
         fprintf(stderr, "Requested samples %u exceeds total samples %u\n", nsel, n);
         return 1;
     }
-    if (a.threads) info("Using thread pool with " + std::to_string(a.threads) + " threads (host); the fit runs on the GPU");
+    if (a.threads) info("Read the VCF with " + std::to_string(a.threads) + " threads; the fit runs on the GPU");
     size_t row_bytes = 0;
     std::vector<uint8_t> packed = d.pack(&row_bytes);
     const double* af = a.af_info ? d.af.data() : nullptr;

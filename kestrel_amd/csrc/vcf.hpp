@@ -29,6 +29,8 @@ struct Data {
 // in `probs`; a row with a NaN (a missing value, vcf.rs:355, 397) has code 3,
 // a one-hot row (the called genotype 1, the others 0) the called genotype's
 // code; `one_hot` says whether every called row was one-hot.
-bool read(const std::string& path, Field field, bool parse_af, Data& out, std::string& err);
+// threads: parser / inflater threads (0 = one per hardware thread); the
+// result does not depend on it.
+bool read(const std::string& path, Field field, bool parse_af, Data& out, std::string& err, unsigned threads = 0);
 
 }  // namespace kvcf

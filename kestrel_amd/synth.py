@@ -182,14 +182,55 @@ def unpack2(packed: np.ndarray, n_sites: int) -> np.ndarray:
     return out[:, :n_sites].copy()
 
 
+class _BgzfWriter:
+    """BGZF (htslib's blocked gzip): gzip members of at most 65,280 input bytes,
+    each with a 'BC' extra field holding the block size, then the empty EOF
+    block."""
+
+    BLOCK = 65280
+
+    def __init__(self, path):
+        self.f = open(path, "wb")
+        self.buf = bytearray()
+
+    @staticmethod
+    def _block(data: bytes) -> bytes:
+        import struct
+        import zlib
+
+        c = zlib.compressobj(6, zlib.DEFLATED, -15)
+        comp = c.compress(data) + c.flush()
+        head = struct.pack("<BBBBIBBHBBHH", 0x1F, 0x8B, 8, 4, 0, 0, 0xFF, 6, ord("B"), ord("C"), 2, len(comp) + 25)
+        return head + comp + struct.pack("<II", zlib.crc32(data) & 0xFFFFFFFF, len(data))
+
+    def write(self, text: str):
+        self.buf += text.encode()
+        while len(self.buf) >= self.BLOCK:
+            self.f.write(self._block(bytes(self.buf[:self.BLOCK])))
+            del self.buf[:self.BLOCK]
+
+    def close(self):
+        if self.buf:
+            self.f.write(self._block(bytes(self.buf)))
+        self.f.write(self._block(b""))
+        self.f.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 def write_vcf(cohort: Cohort, path: str, field: str = "GT", af_info: bool = False, names=None,
-              gl: np.ndarray | None = None) -> None:
+              gl: np.ndarray | None = None, bgzf: bool = False) -> None:
     """Write the cohort as a VCF (plain text, or gzip when path ends in .gz).
     GT: 0/0, 0/1, 1/1, ./. for a missing call.  GL / PL: one-hot likelihoods
     (log10 likelihood 0 for the call and -1000 otherwise; Phred 0 and 10000),
     '.,.,.' for a missing call; or, with gl (V, N, 3) linear-scale
     likelihoods, their log10 (GL, 7 significant digits) or Phred-scaled
-    integers (PL).  af_info writes INFO AF = 1 - p."""
+    integers (PL).  af_info writes INFO AF = 1 - p.  bgzf: BGZF blocks (as
+    bgzip writes them) instead of one gzip stream."""
     import gzip
 
     codes = cohort.codes
@@ -200,7 +241,14 @@ def write_vcf(cohort: Cohort, path: str, field: str = "GT", af_info: bool = Fals
         lik = np.array(["0,-1000,-1000", "-1000,0,-1000", "-1000,-1000,0", ".,.,."])
     else:
         lik = np.array(["0,10000,10000", "10000,0,10000", "10000,10000,0", ".,.,."])
-    opener = gzip.open if path.endswith(".gz") else open
+    if bgzf:
+        def opener(p, mode):
+            return _BgzfWriter(p)
+    elif path.endswith(".gz"):
+        def opener(p, mode):
+            return gzip.open(p, mode, compresslevel=1)
+    else:
+        opener = open
     bases = "ACGT"
     with opener(path, "wt") as f:
         f.write("##fileformat=VCFv4.2\n")

@@ -134,3 +134,49 @@ def test_usage_errors():
     assert r.returncode == 2 and "invalid value" in r.stderr
     r = subprocess.run([KESTREL, "/nonexistent.vcf", "out.tsv"], capture_output=True, text=True)
     assert r.returncode == 1 and "No such file" in r.stderr
+
+
+def _dump_all(path, field, threads):
+    r = subprocess.run([KESTREL, path, "/dev/null", "-f", field, "--dump-codes", "-t", str(threads)],
+                       capture_output=True, text=True)
+    return r.returncode, r.stdout, r.stderr.strip().split("\n")[-1]
+
+
+@pytest.mark.parametrize("field", ["GT", "PL"])
+def test_parallel_reader_matches_serial(tmp_path, field):
+    """The same records for 1 or 8 threads and plain, gzip or BGZF input."""
+    c = synth.generate(100, 30000, seed=41, missing=0.00002)
+    files = []
+    for name, kw in (("a.vcf", {}), ("b.vcf.gz", {}), ("c.vcf.gz", {"bgzf": True})):
+        p = str(tmp_path / name)
+        synth.write_vcf(c, p, field, **kw)
+        files.append(p)
+    base = _dump_all(files[0], field, 1)
+    assert base[0] == 0 and base[1].count("\n") > 1000
+    for p in files:
+        for t in (1, 3, 8):
+            assert _dump_all(p, field, t) == base, (p, t)
+
+
+def test_parallel_reader_reports_first_error(tmp_path):
+    """An error deep in the file is found and numbered as a serial read would."""
+    c = synth.generate(30, 20000, seed=42, missing=0.0)
+    p = str(tmp_path / "e.vcf")
+    synth.write_vcf(c, p, "GT")
+    lines = open(p).read().split("\n")
+    hdr = sum(1 for ln in lines if ln.startswith("#"))
+    lines[hdr + 15000] = lines[hdr + 15000].replace("0/1", "0/x", 1) if "0/1" in lines[hdr + 15000] else \
+        lines[hdr + 15000].replace("0/0", "0/x", 1)
+    lines[hdr + 17000] = "\t".join(lines[hdr + 17000].split("\t")[:5])  # a later, different error
+    open(p, "w").write("\n".join(lines))
+    outs = {t: _dump_all(p, "GT", t) for t in (1, 8)}
+    assert outs[1] == outs[8]
+    assert outs[1][0] != 0
+    r = subprocess.run([KESTREL, p, "/dev/null", "--dump-codes", "-t", "8"], capture_output=True, text=True)
+    assert "Error reading genotype" in r.stderr
+
+
+def test_last_line_without_newline(tmp_path):
+    p = write(tmp_path, "1\t1\t.\tA\tG\t.\tPASS\t.\tGT\t0/0\t0/1\t1/1", "n.vcf")
+    _, sites, counts = dump(p)
+    assert [s[0] for s in sites] == ["012"] and counts["total"] == 1
