@@ -121,6 +121,16 @@ int ko_fit_pairs_codes(int order, const uint8_t* codes, uint32_t N, uint32_t V, 
                        const uint32_t* pairs, size_t npairs, uint32_t nsel, const uint32_t* seg, size_t nseg,
                        uint64_t iterations, int nthreads, ko_output* out);
 
+/* Sites with A = 2, 3 or 4 alleles (G = A(A+1)/2 likelihoods in VCF order):
+ * mixture.rs:8-60 over IIS_A, the allele-frequency fit of one site
+ * (allele/objective.rs, grad_hess.rs lane2/3/4; QP cap A + 1; maf zeroing),
+ * and the pair rows of one pair (lx, ly V x G, af V x A). */
+void ko_mixture_generic(int A, const double* lx, const double* ly, const double* af, double out[9]);
+int ko_allele_freqA_rows(int A, const double* gl, size_t n, uint64_t iterations, double maf, double* af,
+                         uint64_t* iters);
+size_t ko_pair_rows_generic(int A, const double* lx, const double* ly, const double* af, size_t V, double* rows,
+                            uint8_t* valid);
+
 /* Soft likelihoods (vcf.rs:338-410 rows: 10^(gl - max) or 10^(-pl/10), NaN =
  * missing), sample-major N x V x 3.  Pair rows of one pair (V x 3 each), the
  * allele-frequency fit (allele/mod.rs:37-134) and the pair fits

@@ -236,6 +236,33 @@ void ko_mixture_generic2(const double lx[3], const double ly[3], const double af
     memcpy(out, a, sizeof a);
 }
 
+/* mixture.rs:8-60 calculate_mixture_component_matrix over IIS_A
+ * (conditional.rs:10-50): genotypes in VCF order, index j(j+1)/2 + i for
+ * alleles i <= j; for every (gx, gy) in that order g = lx[gx] ly[gy] and
+ * accumulate_iis with the four alleles' frequencies. */
+static void genotype_alleles(int A, int gi[10], int gj[10]) {
+    int g = 0;
+    for (int j = 0; j < A; j++)
+        for (int i = 0; i <= j; i++, g++) {
+            gi[g] = i;
+            gj[g] = j;
+        }
+}
+
+void ko_mixture_generic(int A, const double* lx, const double* ly, const double* af, double out[9]) {
+    int gi[10], gj[10];
+    genotype_alleles(A, gi, gj);
+    const int G = A * (A + 1) / 2;
+    double a[9] = {0};
+    for (int gx = 0; gx < G; gx++)
+        for (int gy = 0; gy < G; gy++) {
+            const double g = lx[gx] * ly[gy];
+            const int i = gi[gx], j = gj[gx], k = gi[gy], l = gj[gy];
+            accumulate_iis(a, g, af[i], af[j], af[k], af[l], ko_iis_mode(i, j, k, l));
+        }
+    memcpy(out, a, sizeof a);
+}
+
 /* iis.rs:279-462 */
 void ko_conditional_probabilities(double pi, double pj, double pk, double pl, int mode, double o[9]) {
     const double pi2 = pi * pi, pi3 = pi2 * pi, pj2 = pj * pj, pk2 = pk * pk;
@@ -880,6 +907,158 @@ int ko_allele_freq2_rows(const double* gl, size_t n, uint64_t iterations, double
     int rc = allele_site(gl, n, blocks, iterations, maf, af, iters);
     free(blocks);
     return rc;
+}
+
+/* allele/objective.rs:114-209 and grad_hess.rs:169-396 for A alleles (the
+ * lane2/3/4 functions are one pattern): G = A(A+1)/2 likelihoods per row in
+ * VCF order, rows as 8-lane blocks [G][8] + remainder [G]. */
+static inline double radd8c(const double v[8]) {
+    double a[4], b[2];
+    for (int i = 0; i < 4; i++) a[i] = v[i] + v[i + 4];
+    for (int i = 0; i < 2; i++) b[i] = a[i] + a[i + 2];
+    return b[0] + b[1];
+}
+
+typedef struct {
+    int A, G;
+    const double* blocks;
+    size_t nb;
+    const double* rem;
+    size_t nrem;
+} allele_ctxA;
+
+static double allele_rowA_obj(const allele_ctxA* c, const double* w, const double* l, size_t stride, double eps) {
+    double p = eps;
+    for (int g = 0; g < c->G; g++) p = fma(w[g], l[g * stride], p);
+    return ko_log(p);
+}
+
+static double allele_objA(void* vc, const double* x, double eps) {
+    const allele_ctxA* c = (const allele_ctxA*)vc;
+    int gi[10], gj[10];
+    genotype_alleles(c->A, gi, gj);
+    double w[10];
+    for (int g = 0; g < c->G; g++) {
+        if (gi[g] == gj[g]) w[g] = x[gi[g]] * x[gi[g]];
+        else {
+            const double t = x[gi[g]] * x[gj[g]];
+            w[g] = t + t;
+        }
+    }
+    double zs[8] = {0};
+    for (size_t b = 0; b < c->nb; b++)
+        for (int j = 0; j < 8; j++) zs[j] = allele_rowA_obj(c, w, c->blocks + b * c->G * 8 + j, 8, eps) + zs[j];
+    double rs = 0.0;
+    for (size_t r = 0; r < c->nrem; r++) rs = allele_rowA_obj(c, w, c->rem + r * c->G, 1, eps) + rs;
+    return -(radd8c(zs) + rs) / (double)(c->nb * 8 + c->nrem);
+}
+
+static void allele_rowA_gh(const allele_ctxA* c, const int* gi, const int* gj, const double* x, const double* l,
+                           size_t stride, double eps, double* zg, double* zh) {
+    const int A = c->A;
+    double lx[4], lxd[4];
+    for (int a = 0; a < A; a++) {
+        /* lx_a = sum_b x_b l_{ab}: first a product, then fma over b */
+        for (int b = 0; b < A; b++) {
+            const int i = a < b ? a : b, j = a < b ? b : a;
+            const double lab = l[(j * (j + 1) / 2 + i) * stride];
+            lx[a] = b == 0 ? x[0] * lab : fma(x[b], lab, lx[a]);
+        }
+    }
+    double d = eps;
+    for (int a = 0; a < A; a++) d = fma(x[a], lx[a], d);
+    const double inv = 1.0 / d;
+    for (int a = 0; a < A; a++) lxd[a] = lx[a] * inv;
+    for (int g = 0; g < c->G; g++) {
+        zh[g] = fma(2.0, lxd[gi[g]] * lxd[gj[g]], zh[g]);
+        zh[g] = fma(-l[g * stride], inv, zh[g]);
+    }
+    for (int a = 0; a < A; a++) zg[a] = fma(lx[a], inv, zg[a]);
+}
+
+static void allele_ghA(void* vc, const double* x, double eps, double* g, double* h) {
+    const allele_ctxA* c = (const allele_ctxA*)vc;
+    int gi[10], gj[10];
+    genotype_alleles(c->A, gi, gj);
+    double zg[8][4], zh[8][10], rg[4] = {0}, rh[10] = {0};
+    memset(zg, 0, sizeof zg);
+    memset(zh, 0, sizeof zh);
+    for (size_t b = 0; b < c->nb; b++)
+        for (int j = 0; j < 8; j++)
+            allele_rowA_gh(c, gi, gj, x, c->blocks + b * c->G * 8 + j, 8, eps, zg[j], zh[j]);
+    for (size_t r = 0; r < c->nrem; r++) allele_rowA_gh(c, gi, gj, x, c->rem + r * c->G, 1, eps, rg, rh);
+    const double n = (double)(c->nb * 8 + c->nrem);
+    for (int a = 0; a < c->A; a++) {
+        double v[8];
+        for (int j = 0; j < 8; j++) v[j] = zg[j][a];
+        g[a] = -2.0 * (radd8c(v) + rg[a]) / n;
+    }
+    for (int gg = 0; gg < c->G; gg++) {
+        double v[8];
+        for (int j = 0; j < 8; j++) v[j] = zh[j][gg];
+        const double val = 2.0 * (radd8c(v) + rh[gg]) / n;
+        h[gi[gg] * MX + gj[gg]] = val;
+        h[gj[gg] * MX + gi[gg]] = val;
+    }
+}
+
+int ko_allele_freqA_rows(int A, const double* gl, size_t n, uint64_t iterations, double maf, double* af,
+                         uint64_t* iters) {
+    const int G = A * (A + 1) / 2;
+    double* blocks = (double*)malloc(((n / 8) + 1) * G * 8 * sizeof(double));
+    double rem[8 * 10], blk[10 * 8];
+    size_t nb = 0, cur = 0, cnt = 0;
+    for (size_t i = 0; i < n; i++) {
+        const double* r = gl + (size_t)G * i;
+        int nan = 0;
+        for (int k = 0; k < G; k++) nan |= r[k] != r[k];
+        if (nan) continue; /* lanevector.rs:51-53 */
+        for (int k = 0; k < G; k++) blk[k * 8 + cur] = r[k];
+        cur++;
+        cnt++;
+        if (cur == 8) {
+            memcpy(blocks + nb * G * 8, blk, (size_t)G * 8 * sizeof(double));
+            nb++;
+            cur = 0;
+        }
+    }
+    for (size_t r = 0; r < cur; r++)
+        for (int k = 0; k < G; k++) rem[r * G + k] = blk[k * 8 + r];
+    if (cnt == 0) {
+        free(blocks);
+        return -1; /* lanevector.rs:73-76 */
+    }
+    allele_ctxA ctx = {A, G, blocks, nb, rem, cur};
+    ko_tune t;
+    ko_tune_new(&t, iterations, (uint64_t)A + 1); /* allele/mod.rs:93 */
+    double x0[4], x[4], f, cond;
+    for (int a = 0; a < A; a++) x0[a] = 1.0 / (double)A; /* allele/mod.rs:106 */
+    uint64_t it;
+    sqp(A, allele_objA, allele_ghA, &ctx, x0, &t, &f, x, &it, &cond);
+    for (int a = 0; a < A; a++)
+        if (x[a] > 0.0 && x[a] < maf) x[a] = 0.0;
+    sum_to_one(A, x, af);
+    if (iters) *iters = it;
+    free(blocks);
+    return 0;
+}
+
+/* Pair rows of one pair over sites with A alleles: lx, ly V x G, af V x A. */
+size_t ko_pair_rows_generic(int A, const double* lx, const double* ly, const double* af, size_t V, double* rows,
+                            uint8_t* valid) {
+    const int G = A * (A + 1) / 2;
+    size_t n = 0;
+    for (size_t s = 0; s < V; s++) {
+        double c[9];
+        ko_mixture_generic(A, lx + (size_t)G * s, ly + (size_t)G * s, af + (size_t)A * s, c);
+        int ok = 1;
+        for (int k = 0; k < 9; k++)
+            if (c[k] != c[k]) ok = 0; /* lanevector.rs:100-104 */
+        valid[s] = (uint8_t)ok;
+        for (int k = 0; k < 9; k++) rows[s * 9 + k] = ok ? c[k] : 0.0;
+        n += ok;
+    }
+    return n;
 }
 
 typedef struct {

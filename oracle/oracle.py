@@ -97,6 +97,12 @@ def lib():
         L.ko_fit_pairs_gl.argtypes = [I, P, U32, U32, P, P, SZ, U32, P, SZ, U64, I, P]
         L.ko_pair_rows_gl.restype = SZ
         L.ko_pair_rows_gl.argtypes = [P, P, P, SZ, P, P]
+        L.ko_mixture_generic.restype = None
+        L.ko_mixture_generic.argtypes = [I, P, P, P, P]
+        L.ko_allele_freqA_rows.restype = I
+        L.ko_allele_freqA_rows.argtypes = [I, P, SZ, U64, D, P, P]
+        L.ko_pair_rows_generic.restype = SZ
+        L.ko_pair_rows_generic.argtypes = [I, P, P, P, SZ, P, P]
         _lib = L
     return _lib
 
@@ -334,6 +340,133 @@ def fit_pairs_gl(gl, af, pairs=None, nsel=None, order=ORDER_REF, seg=None, itera
                                iterations, nthreads, _p(out))
     if rc != 0:
         raise ValueError("requested samples exceed total samples")
+    return out
+
+
+def mixture_generic(A, lx, ly, af) -> np.ndarray:
+    """mixture.rs:8-60 over IIS_A for one site."""
+    out = np.zeros(9)
+    lx, ly, af = f64(lx), f64(ly), f64(af)
+    lib().ko_mixture_generic(A, _p(lx), _p(ly), _p(af), _p(out))
+    return out
+
+
+def allele_freqA_rows(A, gl, iterations=100, maf=0.0):
+    """allele/mod.rs:78-134 for one site with A alleles: gl (n, A(A+1)/2)."""
+    gl = f64(gl)
+    af = np.zeros(A)
+    it = np.zeros(1, np.uint64)
+    rc = lib().ko_allele_freqA_rows(A, _p(gl), gl.shape[0], iterations, maf, _p(af), _p(it))
+    if rc != 0:
+        raise ValueError("no called sample")
+    return af, int(it[0])
+
+
+def pair_rows_generic(A, lx, ly, af):
+    lx, ly, af = f64(lx), f64(ly), f64(af)
+    v = lx.shape[0]
+    rows = np.zeros((v, 9))
+    valid = np.zeros(v, np.uint8)
+    n = lib().ko_pair_rows_generic(A, _p(lx), _p(ly), _p(af), v, _p(rows), _p(valid))
+    return rows, valid, n
+
+
+def _gl_index(a, b):
+    i, j = min(a, b), max(a, b)
+    return j * (j + 1) // 2 + i
+
+
+def multi_prepare(buckets, iterations=100, maf=None):
+    """wrapper.rs:89-130 + split_gl_af / remove_zero_frequencies (wrapper.rs:237-410)
+    for likelihood buckets {2: (V2, N, 3), 3: (V3, N, 6), 4: (V4, N, 10)}
+    (site-major as vcf.rs builds them): allele frequencies per site, then
+    alleles with frequency 0 removed and every site re-bucketed by its
+    remaining allele count (processing the 2-, 3- then 4-allele buckets in
+    site order; one allele left drops the site).  Returns {A: (af (V'_A, A),
+    gl (N, V'_A, G'_A) sample-major)} and the frequencies per input bucket."""
+    n = next(b.shape[1] for b in buckets.values() if b is not None and b.size)
+    if maf is None:
+        maf = 1.0 / (2.0 * n)  # wrapper.rs:101
+    af_in, out_af, out_gl = {}, {2: [], 3: [], 4: []}, {2: [], 3: [], 4: []}
+    for A in (2, 3, 4):
+        b = buckets.get(A)
+        if b is None or b.shape[0] == 0:
+            af_in[A] = np.zeros((0, A))
+            continue
+        af_in[A] = np.array([allele_freqA_rows(A, b[s], iterations, maf)[0] for s in range(b.shape[0])])
+        for s in range(b.shape[0]):
+            nz = [a for a in range(A) if af_in[A][s, a] > 0.0]
+            if len(nz) < 2:
+                continue
+            idx = [_gl_index(nz[i], nz[j]) for j in range(len(nz)) for i in range(j + 1)]
+            out_af[len(nz)].append(af_in[A][s, nz])
+            out_gl[len(nz)].append(b[s][:, idx])
+    res = {}
+    for A in (2, 3, 4):
+        G = A * (A + 1) // 2
+        res[A] = (np.array(out_af[A]).reshape(-1, A),
+                  np.ascontiguousarray(np.array(out_gl[A]).reshape(-1, n, G).transpose(1, 0, 2)))
+    return res, af_in
+
+
+def multi_pair_rows(parts, x, y, order, pads=None):
+    """The c rows of pair (x, y) over the re-bucketed sites.  REF: the order of
+    coefficients.rs:97-107 (the 8-lane parts of the 2-, 3-, 4-allele sites,
+    then their remainders), compacted to the usable rows.  SEG: the device's
+    order, each part padded to pads[A] sites; returns (rows, valid)."""
+    chunks = {}
+    for A in (2, 3, 4):
+        af, gl = parts[A]
+        if af.shape[0] == 0:
+            chunks[A] = (np.zeros((0, 9)), np.zeros(0, np.uint8))
+            continue
+        rows, valid, _ = pair_rows_generic(A, gl[x], gl[y], af) if A > 2 else pair_rows_gl(gl[x], gl[y], af)
+        chunks[A] = (rows, valid)
+    if order == ORDER_REF:
+        seq = []
+        for A in (2, 3, 4):
+            r, v = chunks[A]
+            m = (len(v) // 8) * 8
+            seq.append((r[:m], v[:m]))
+        for A in (2, 3, 4):
+            r, v = chunks[A]
+            m = (len(v) // 8) * 8
+            seq.append((r[m:], v[m:]))
+        rows = np.vstack([r for r, _ in seq])
+        valid = np.concatenate([v for _, v in seq])
+        return rows[valid == 1], None
+    rows, valid = [], []
+    for A in (2, 3, 4):
+        r, v = chunks[A]
+        pad = pads[A] if pads and A in pads else len(v)
+        if pad == 0:
+            continue
+        rr = np.zeros((pad, 9))
+        vv = np.zeros(pad, np.uint8)
+        rr[:len(v)] = r
+        vv[:len(v)] = v
+        rows.append(rr)
+        valid.append(vv)
+    return np.vstack(rows), np.concatenate(valid)
+
+
+def multi_fit_pairs(parts, pairs, order=ORDER_REF, pads=None, seg=None, iterations=100):
+    """Pair fits over re-bucketed multi-allelic sites (coefficients.rs:40-141)."""
+    out = np.zeros(len(pairs), OUTPUT_DTYPE)
+    for i, (x, y) in enumerate(pairs):
+        rows, valid = multi_pair_rows(parts, int(x), int(y), order, pads)
+        n = len(rows) if order == ORDER_REF else int(valid.sum())
+        if n == 0:
+            o = out[i]
+            o["x"], o["y"], o["n_sites"] = x, y, 0
+            o["jacquard"], o["obj"], o["cond"] = np.nan, np.nan, np.nan
+            continue
+        if order == ORDER_REF:
+            o = fit_rows(rows, self_pair=x == y, order=ORDER_REF, iterations=iterations)
+        else:
+            o = fit_rows(rows, self_pair=x == y, order=ORDER_SEG, valid=valid, seg=seg, iterations=iterations)
+        out[i] = o
+        out[i]["x"], out[i]["y"] = x, y
     return out
 
 
