@@ -106,6 +106,25 @@ int kestrel_cohort_create(kestrel_cohort** out, const uint8_t* packed, uint32_t 
 int kestrel_cohort_create_gl(kestrel_cohort** out, const double* gl, uint32_t n_samples, uint32_t n_sites,
                              const double* af, double maf, uint64_t af_iterations);
 
+/* Soft likelihoods of sites with 2, 3 and 4 alleles, the buckets of the
+ * reference's GenotypeLikelihoods {gl2, gl3, gl4} (vcf.rs:270-330):
+ * glA is nA x n_samples x G doubles (G = 3, 6, 10 likelihoods in VCF genotype
+ * order), site-major, a NaN marking a missing call; NULL when nA = 0.
+ * Replaces wrapper.rs:89-130 + split_gl_af (wrapper.rs:237-410): allele
+ * frequencies per site by the A-allele SQP (allele/mod.rs:37-134, QP cap
+ * A + 1, maf < 0 = 1/(2N)), alleles of frequency 0 removed and every site
+ * re-bucketed by its remaining allele count (sites left with one allele are
+ * dropped).  Pair fits then add the 3- and 4-allele sites' c_s from the generic
+ * mixture over IIS_3 / IIS_4 (mixture.rs:8-60) after the 2-allele sites. */
+int kestrel_cohort_create_gl_multi(kestrel_cohort** out, uint32_t n_samples, const double* gl2, uint32_t n2,
+                                   const double* gl3, uint32_t n3, const double* gl4, uint32_t n4, double maf,
+                                   uint64_t af_iterations);
+/* Of a kestrel_cohort_create_gl_multi cohort: part_sites = sites with 2, 3, 4
+ * alleles after zero-frequency removal; afA (nA x A, may be NULL) = the
+ * frequencies fitted for the input buckets. */
+int kestrel_cohort_multi_info(const kestrel_cohort* c, uint32_t part_sites[3], double* af2, double* af3,
+                              double* af4);
+
 int kestrel_cohort_allele_freqs(const kestrel_cohort* c, double* af_out, uint8_t* kept_out);
 /* Number of kept (segregating) sites. */
 int kestrel_cohort_n_kept(const kestrel_cohort* c, uint32_t* n_kept);

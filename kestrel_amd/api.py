@@ -88,6 +88,50 @@ class Cohort:
             self.set_option("segment_sites", segment_sites)
         return self
 
+    @classmethod
+    def gl_multi(cls, gl2: np.ndarray | None = None, gl3: np.ndarray | None = None, gl4: np.ndarray | None = None,
+                 *, maf: float | None = None, af_iterations: int = 100, segment_sites: int | None = None) -> "Cohort":
+        """Soft likelihoods of sites with 2, 3 and 4 alleles
+        (kestrel_cohort_create_gl_multi): glA is (V_A, N, G) site-major,
+        G = 3, 6, 10 in VCF genotype order."""
+        L = _lib.load()
+        self = cls.__new__(cls)
+        arrs, counts, n = [], [], None
+        for A, g in ((2, gl2), (3, gl3), (4, gl4)):
+            G = A * (A + 1) // 2
+            if g is None:
+                arrs.append(None)
+                counts.append(0)
+                continue
+            g = np.ascontiguousarray(g, np.float64)
+            if g.ndim != 3 or g.shape[2] != G:
+                raise ValueError(f"gl{A} must have shape (sites, samples, {G})")
+            if n is not None and g.shape[1] != n:
+                raise ValueError("buckets must have the same samples")
+            n = g.shape[1]
+            arrs.append(g)
+            counts.append(g.shape[0])
+        if n is None:
+            raise ValueError("no likelihoods given")
+        self.n_samples, self.n_sites = int(n), int(sum(counts))
+        self._bucket_sites = counts
+        h = ctypes.c_void_p()
+        check(L.kestrel_cohort_create_gl_multi(ctypes.byref(h), self.n_samples, ptr(arrs[0]), counts[0],
+                                               ptr(arrs[1]), counts[1], ptr(arrs[2]), counts[2],
+                                               -1.0 if maf is None else float(maf), af_iterations))
+        self._h = h
+        if segment_sites is not None:
+            self.set_option("segment_sites", segment_sites)
+        return self
+
+    def multi_info(self):
+        """(sites with 2, 3, 4 alleles after zero-frequency removal, the
+        fitted frequencies of the input buckets {A: (V_A, A)})."""
+        parts = np.zeros(3, np.uint32)
+        afs = [np.zeros((v, A)) for A, v in zip((2, 3, 4), self._bucket_sites)]
+        check(_lib.load().kestrel_cohort_multi_info(self._h, ptr(parts), ptr(afs[0]), ptr(afs[1]), ptr(afs[2])))
+        return tuple(int(v) for v in parts), {2: afs[0], 3: afs[1], 4: afs[2]}
+
     def fit_pairs_gt(self, pairs: np.ndarray | None = None, n_select: int | None = None, iterations: int = 100):
         """CLS fit of pairs (coefficients.rs:166-278); returns (outputs, Q)."""
         if pairs is None:
@@ -326,8 +370,16 @@ def relatedness_coefficients_gl(genotype_likelihoods, allele_frequencies=None, t
     gl = np.asarray(genotype_likelihoods, np.float64)
     if gl.ndim != 3:
         raise ValueError("genotype_likelihoods must have shape (sites, samples, genotypes)")
+    if gl.shape[2] in (6, 10):  # every site tri- or tetra-allelic
+        co = Cohort.gl_multi(**{("gl3" if gl.shape[2] == 6 else "gl4"): gl}, maf=minimum_allele_frequency,
+                             af_iterations=iterations)
+        try:
+            out = co.fit_pairs(iterations=iterations)
+        finally:
+            co.close()
+        return build_arrays(out, gl.shape[1])
     if gl.shape[2] != 3:
-        raise ValueError("tri- and tetra-allelic sites are not supported by this build")
+        raise ValueError("genotype_likelihoods need 3, 6 or 10 genotypes (2, 3 or 4 alleles)")
     missing = np.isnan(gl).any(axis=2)
     one_hot = ((gl == 1.0).sum(axis=2) == 1) & ((gl == 0.0).sum(axis=2) == 2)
     n = gl.shape[1]

@@ -66,6 +66,21 @@ int repack_kept(kestrel_cohort* c, const uint8_t* dpacked, size_t row_stride_byt
     return KESTREL_OK;
 }
 
+// One evaluation pass over every part of the cohort: the main part's segments,
+// then those of the tri/tetra-allelic parts, into consecutive partials.
+int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
+                  uint32_t ns, double eps, double* partial, hipStream_t st) {
+    HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, sx, sy, pt, active, ns, eps, partial, st,
+                          c->soft ? c->glrows.p : nullptr, c->v_pad));
+    size_t base = c->nseg;
+    for (const auto& p : c->parts) {
+        HIPCHK(klaunch::sweep_multi(p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->af.p, p->seg.p,
+                                    p->nseg, sx, sy, pt, active, ns, eps, partial + base * NPART * ns, st));
+        base += p->nseg;
+    }
+    return KESTREL_OK;
+}
+
 int open_device(kestrel_cohort* c) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -247,8 +262,144 @@ int kestrel_cohort_create_gl(kestrel_cohort** out, const double* gl, uint32_t n_
     return KESTREL_OK;
 }
 
+int kestrel_cohort_create_gl_multi(kestrel_cohort** out, uint32_t n_samples, const double* gl2, uint32_t n2,
+                                   const double* gl3, uint32_t n3, const double* gl4, uint32_t n4, double maf,
+                                   uint64_t af_iterations) {
+    if (!out) return fail(KESTREL_EINVAL, "null argument");
+    *out = nullptr;
+    const uint32_t nin[3] = {n2, n3, n4};
+    const double* gin[3] = {gl2, gl3, gl4};
+    if (n_samples == 0 || (uint64_t)n2 + n3 + n4 == 0)
+        return fail(KESTREL_EINVAL, "need at least one sample and one site");
+    for (int b = 0; b < 3; b++)
+        if (nin[b] && !gin[b]) return fail(KESTREL_EINVAL, "null likelihoods for %u sites", nin[b]);
+    if (maf < 0) maf = 1.0 / (2.0 * (double)n_samples);  // wrapper.rs:101
+    if (!(maf >= 0.0 && maf <= 1.0)) return fail(KESTREL_EINVAL, "Minimum allele frequency must be within 0 to 1");
+    if (af_iterations == 0) return fail(KESTREL_EINVAL, "iterations must be non-zero");
+
+    auto c = new kestrel_cohort();
+    std::unique_ptr<kestrel_cohort> guard(c);
+    int rc = open_device(c);
+    if (rc) return rc;
+    c->soft = true;
+    c->multi = true;
+    c->n_samples = n_samples;
+    c->n_sites = n2 + n3 + n4;
+
+    // allele frequencies per input bucket (allele/mod.rs:37-134)
+    DevBuf<double> dgl[3];
+    DevBuf<int> dstatus;
+    HIPCHK(dstatus.alloc(1));
+    HIPCHK(hipMemsetAsync(dstatus.p, 0, sizeof(int), c->stream));
+    for (int b = 0; b < 3; b++) {
+        const int A = b + 2, G = A * (A + 1) / 2;
+        const size_t vals = (size_t)nin[b] * n_samples * G;
+        HIPCHK(dgl[b].alloc(vals));
+        c->af_in[b].assign((size_t)nin[b] * A, 0.0);
+        if (!nin[b]) continue;
+        HIPCHK(hipMemcpyAsync(dgl[b].p, gin[b], vals * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        DevBuf<double> daf;
+        HIPCHK(daf.alloc((size_t)nin[b] * A));
+        kestrel_tune t;
+        kestrel_tune_default(&t, af_iterations);
+        t.qp_max_iter = (uint64_t)A + 1;  // allele/mod.rs:93
+        HIPCHK(klaunch::af_gl(dgl[b].p, n_samples, nin[b], to_dev(t), maf, daf.p, dstatus.p, c->stream, A));
+        HIPCHK(hipMemcpyAsync(c->af_in[b].data(), daf.p, c->af_in[b].size() * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    int status = 0;
+    HIPCHK(hipMemcpy(&status, dstatus.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (status) return fail(KESTREL_EDATA, "Must have at least one row to analyze (a site has no called sample)");
+
+    // wrapper.rs:237-410 remove_zero_frequencies: drop alleles of frequency 0,
+    // re-bucket every site by its remaining allele count (buckets of 2, 3, 4
+    // alleles in turn, sites in order; one allele left drops the site)
+    struct PartMap {
+        std::vector<uint32_t> site;
+        std::vector<uint8_t> alleles, gmap;
+        std::vector<double> af;
+    } pm[3];
+    for (int b = 0; b < 3; b++) {
+        const int A = b + 2;
+        for (uint32_t s = 0; s < nin[b]; s++) {
+            int nz[4], m = 0;
+            for (int a = 0; a < A; a++)
+                if (c->af_in[b][(size_t)A * s + a] > 0.0) nz[m++] = a;
+            if (m < 2) continue;
+            PartMap& p = pm[m - 2];
+            p.site.push_back(s);
+            p.alleles.push_back((uint8_t)A);
+            for (int jj = 0; jj < m; jj++)
+                for (int ii = 0; ii <= jj; ii++) p.gmap.push_back((uint8_t)(nz[jj] * (nz[jj] + 1) / 2 + nz[ii]));
+            for (int q = 0; q < m; q++) p.af.push_back(c->af_in[b][(size_t)A * s + nz[q]]);
+        }
+    }
+    for (int q = 0; q < 3; q++) c->part_sites[q] = (uint32_t)pm[q].site.size();
+
+    auto build_part = [&](int q, uint32_t v_pad, DevBuf<double>& rows, DevBuf<uint32_t>& mask,
+                          DevBuf<double>& af) -> int {
+        const int A = q + 2, G = A * (A + 1) / 2;
+        const uint32_t n_t = (uint32_t)pm[q].site.size(), row_words = v_pad / 16;
+        DevBuf<uint32_t> dsite;
+        DevBuf<uint8_t> dal, dgm;
+        HIPCHK(dsite.alloc(std::max<uint32_t>(1, n_t)));
+        HIPCHK(dal.alloc(std::max<uint32_t>(1, n_t)));
+        HIPCHK(dgm.alloc(std::max<size_t>(1, pm[q].gmap.size())));
+        if (n_t) {
+            HIPCHK(hipMemcpyAsync(dsite.p, pm[q].site.data(), n_t * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(dal.p, pm[q].alleles.data(), n_t, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(dgm.p, pm[q].gmap.data(), pm[q].gmap.size(), hipMemcpyHostToDevice, c->stream));
+        }
+        HIPCHK(rows.alloc((size_t)n_samples * v_pad * G));
+        HIPCHK(mask.alloc((size_t)n_samples * row_words));
+        HIPCHK(klaunch::gather_multi(dgl[0].p, dgl[1].p, dgl[2].p, n_samples, dsite.p, dal.p, dgm.p, n_t, G, rows.p,
+                                     v_pad, mask.p, row_words, c->stream));
+        std::vector<double> afp((size_t)v_pad * A, 1.0 / (double)A);  // padding sites are missing for every sample
+        std::copy(pm[q].af.begin(), pm[q].af.end(), afp.begin());
+        HIPCHK(af.alloc(afp.size()));
+        HIPCHK(hipMemcpyAsync(af.p, afp.data(), afp.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));  // the maps and host vectors go out of scope
+        return KESTREL_OK;
+    };
+    // the 2-allele part is the main part (sweep_gl); 3 and 4 alleles follow
+    c->n_kept = c->part_sites[0];
+    c->v_pad = std::max<uint32_t>(SITE_PAD, (c->n_kept + SITE_PAD - 1) / SITE_PAD * SITE_PAD);
+    c->row_words = c->v_pad / 16;
+    rc = build_part(0, c->v_pad, c->glrows, c->geno, c->af);
+    if (rc) return rc;
+    for (int q = 1; q < 3; q++) {
+        if (!c->part_sites[q]) continue;
+        std::unique_ptr<SitePart> p(new SitePart());
+        p->alleles = q + 2;
+        p->n_sites = c->part_sites[q];
+        p->v_pad = (p->n_sites + SITE_PAD - 1) / SITE_PAD * SITE_PAD;
+        p->row_words = p->v_pad / 16;
+        rc = build_part(q, p->v_pad, p->rows, p->mask, p->af);
+        if (rc) return rc;
+        c->parts.push_back(std::move(p));
+    }
+    rc = c->set_segments();
+    if (rc) return rc;
+    *out = guard.release();
+    return KESTREL_OK;
+}
+
+int kestrel_cohort_multi_info(const kestrel_cohort* c, uint32_t part_sites[3], double* af2, double* af3,
+                              double* af4) {
+    if (!c) return fail(KESTREL_EINVAL, "null cohort");
+    if (!c->multi) return fail(KESTREL_EINVAL, "cohort was not created by kestrel_cohort_create_gl_multi");
+    if (part_sites)
+        for (int q = 0; q < 3; q++) part_sites[q] = c->part_sites[q];
+    double* dst[3] = {af2, af3, af4};
+    for (int b = 0; b < 3; b++)
+        if (dst[b]) std::copy(c->af_in[b].begin(), c->af_in[b].end(), dst[b]);
+    return KESTREL_OK;
+}
+
 int kestrel_cohort_allele_freqs(const kestrel_cohort* c, double* af_out, uint8_t* kept_out) {
     if (!c) return fail(KESTREL_EINVAL, "null cohort");
+    if (c->multi) return fail(KESTREL_EINVAL, "a multi-allelic cohort reports frequencies by kestrel_cohort_multi_info");
     if (af_out) std::memcpy(af_out, c->af_all.data(), c->af_all.size() * sizeof(double));
     if (kept_out) std::memcpy(kept_out, c->kept_mask.data(), c->kept_mask.size());
     return KESTREL_OK;
@@ -325,7 +476,9 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
                                                       c->n_samples);
     }
     if (n_pairs == 0) return KESTREL_OK;
-    if (c->n_kept == 0) {
+    uint64_t usable_sites = c->n_kept;
+    for (const auto& p : c->parts) usable_sites += p->n_sites;
+    if (usable_sites == 0) {
         const double nan = std::nan("");
         for (size_t i = 0; i < n_pairs; i++) {
             kestrel_output& o = out[i];
@@ -405,7 +558,7 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
         HIPCHK(G.sx.alloc(ns));
         HIPCHK(G.sy.alloc(ns));
         HIPCHK(G.pt.alloc((size_t)9 * ns));
-        HIPCHK(G.part.alloc((size_t)c->nseg * NPART * ns));
+        HIPCHK(G.part.alloc((size_t)c->nseg_total() * NPART * ns));
         HIPCHK(G.active.alloc(ns));
         HIPCHK(G.hist.alloc(BATCH));
         HIPCHK(G.st.alloc(ns));
@@ -422,8 +575,8 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     // claim the first pairs
     for (int g = 0; g < ngroups; g++) {
         Group& G = grp[g];
-        HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, c->nseg, dpairs.p, n_pairs,
-                             dnext.p, dout.p, dt, dlive.p, PairQueue{}, nullptr, G.stream));
+        HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, c->nseg_total(), dpairs.p,
+                             n_pairs, dnext.p, dout.p, dt, dlive.p, PairQueue{}, nullptr, G.stream));
     }
     double sweep_ms = 0;
     uint64_t sweeps = 0, launched = 0;
@@ -447,11 +600,10 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
                 Group& G = grp[g];
                 if (G.live_before <= 0) continue;
                 HIPCHK(hipEventRecord(G.ev[2 * i], G.stream));
-                HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, G.sx.p, G.sy.p, G.pt.p,
-                                      G.active.p, ns, tune->epsilon, G.part.p, G.stream,
-                                      c->soft ? c->glrows.p : nullptr, c->v_pad));
+                int rs = launch_sweeps(c, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, tune->epsilon, G.part.p, G.stream);
+                if (rs) return rs;
                 HIPCHK(hipEventRecord(G.ev[2 * i + 1], G.stream));
-                HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, c->nseg, dpairs.p,
+                HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, c->nseg_total(), dpairs.p,
                                      n_pairs, dnext.p, dout.p, dt, dlive.p, PairQueue{}, G.hist.p + i, G.stream));
             }
         launched += BATCH;
@@ -526,7 +678,7 @@ int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* poi
     HIPCHK(dsy.alloc(ns));
     HIPCHK(dact.alloc(ns));
     HIPCHK(dpt.alloc((size_t)9 * ns));
-    HIPCHK(dpart.alloc((size_t)c->nseg * NPART * ns));
+    HIPCHK(dpart.alloc((size_t)c->nseg_total() * NPART * ns));
     HIPCHK(df.alloc(n));
     HIPCHK(dg.alloc(9 * n));
     HIPCHK(dh.alloc(81 * n));
@@ -536,9 +688,9 @@ int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* poi
     HIPCHK(hipMemcpyAsync(dsy.p, hy.data(), ns * sizeof(uint32_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dact.p, hact.data(), ns * sizeof(int), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dpt.p, hpt.data(), hpt.size() * sizeof(double), hipMemcpyHostToDevice, st));
-    HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, dsx.p, dsy.p, dpt.p, dact.p, ns, eps,
-                          dpart.p, st, c->soft ? c->glrows.p : nullptr, c->v_pad));
-    HIPCHK(klaunch::finalize(dpart.p, c->nseg, ns, (uint32_t)n, df.p, dg.p, dh.p, dn.p, st));
+    int rs = launch_sweeps(c, dsx.p, dsy.p, dpt.p, dact.p, ns, eps, dpart.p, st);
+    if (rs) return rs;
+    HIPCHK(klaunch::finalize(dpart.p, c->nseg_total(), ns, (uint32_t)n, df.p, dg.p, dh.p, dn.p, st));
     HIPCHK(hipMemcpyAsync(f, df.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(g, dg.p, 9 * n * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(h, dh.p, 81 * n * sizeof(double), hipMemcpyDeviceToHost, st));

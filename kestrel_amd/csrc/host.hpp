@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -47,6 +48,31 @@ struct DevBuf {
 kdev::Tune to_dev(const kestrel_tune& t);
 
 struct ServeState;
+
+// Segment bounds of a part of v_pad sites: nseg = ceil(chunks / target),
+// segments of equal length (the last may be shorter) so that every block of a
+// sweep has the same work.
+inline std::vector<uint32_t> segment_bounds(uint32_t v_pad, int64_t segment_sites) {
+    const uint32_t chunks = v_pad / CHUNK;
+    const uint32_t target = (uint32_t)std::max<int64_t>(CHUNK, segment_sites) / CHUNK;
+    const uint32_t nseg = std::max<uint32_t>(1, (chunks + target - 1) / target);
+    const uint32_t per = (chunks + nseg - 1) / nseg;
+    std::vector<uint32_t> b(nseg + 1);
+    for (uint32_t k = 0; k <= nseg; k++) b[k] = std::min(chunks, k * per) * CHUNK;
+    b[nseg] = v_pad;
+    return b;
+}
+
+// Sites with 3 or 4 alleles after zero-frequency removal (likelihood rows of
+// A(A+1)/2 genotypes); their segments follow the main part's in a sweep.
+struct SitePart {
+    int alleles = 0;
+    uint32_t n_sites = 0, v_pad = 0, row_words = 0, nseg = 0;
+    DevBuf<double> rows;    // n_samples x v_pad x G
+    DevBuf<uint32_t> mask;  // n_samples x row_words 2-bit rows, 3 = missing
+    DevBuf<double> af;      // v_pad x alleles
+    DevBuf<uint32_t> seg;
+};
 // Stops the serving engine of a cohort (if any) and frees its buffers.
 void serve_release(ServeState* s);
 
@@ -75,18 +101,29 @@ struct kestrel_cohort {
     uint64_t last_sweeps = 0;
     int last_groups = 1;
 
+    // tri/tetra-allelic cohorts (kestrel_cohort_create_gl_multi)
+    std::vector<std::unique_ptr<SitePart>> parts;
+    bool multi = false;
+    uint32_t part_sites[3] = {0, 0, 0};   // sites with 2, 3, 4 alleles after zero-frequency removal
+    std::vector<double> af_in[3];         // frequencies of the input buckets (2, 3, 4 alleles)
+
+    uint32_t nseg_total() const {
+        uint32_t n = nseg;
+        for (const auto& p : parts) n += p->nseg;
+        return n;
+    }
+
     int set_segments() {
-        // nseg = ceil(chunks / target); segments of equal length (the last
-        // may be shorter) so that every block of a sweep has the same work
-        const uint32_t chunks = v_pad / CHUNK;
-        const uint32_t target = (uint32_t)std::max<int64_t>(CHUNK, opt_segment_sites) / CHUNK;
-        nseg = std::max<uint32_t>(1, (chunks + target - 1) / target);
-        const uint32_t per = (chunks + nseg - 1) / nseg;
-        std::vector<uint32_t> b(nseg + 1);
-        for (uint32_t k = 0; k <= nseg; k++) b[k] = std::min(chunks, k * per) * CHUNK;
-        b[nseg] = v_pad;
+        std::vector<uint32_t> b = segment_bounds(v_pad, opt_segment_sites);
+        nseg = (uint32_t)b.size() - 1;
         HIPCHK(seg.alloc(nseg + 1));
         HIPCHK(hipMemcpy(seg.p, b.data(), b.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        for (auto& p : parts) {
+            std::vector<uint32_t> pb = segment_bounds(p->v_pad, opt_segment_sites);
+            p->nseg = (uint32_t)pb.size() - 1;
+            HIPCHK(p->seg.alloc(p->nseg + 1));
+            HIPCHK(hipMemcpy(p->seg.p, pb.data(), pb.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        }
         return KESTREL_OK;
     }
 

@@ -50,18 +50,43 @@ struct CodeRows {
     }
 };
 
+// Likelihood rows of a site with G genotypes, samples consecutive.
+template <int G>
 struct GlRows {
-    const double* base;  // likelihood triple of sample 0 at this site
+    const double* base;  // likelihoods of sample 0 at this site
     __device__ __forceinline__ bool load(uint32_t i, double* l) const {
-        l[0] = base[3 * (size_t)i];
-        l[1] = base[3 * (size_t)i + 1];
-        l[2] = base[3 * (size_t)i + 2];
-        return !(isnan(l[0]) || isnan(l[1]) || isnan(l[2]));
+        bool nan = false;
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+            l[k] = base[(size_t)G * i + k];
+            nan |= isnan(l[k]);
+        }
+        return !nan;
     }
 };
 
-template <class Rows>
+// allele/objective.rs and allele/grad_hess.rs for A alleles (the lane2/3/4
+// functions are one pattern): G = A(A+1)/2 likelihoods per row in VCF order,
+// genotype index j(j+1)/2 + i for alleles i <= j.
+template <int A>
+struct Genotypes {
+    static constexpr int G = A * (A + 1) / 2;
+    int gi[G], gj[G];
+    constexpr Genotypes() : gi(), gj() {
+        int g = 0;
+        for (int j = 0; j < A; j++)
+            for (int i = 0; i <= j; i++, g++) {
+                gi[g] = i;
+                gj[g] = j;
+            }
+    }
+};
+
+__host__ __device__ constexpr int gl_index(int a, int b) { return a < b ? b * (b + 1) / 2 + a : a * (a + 1) / 2 + b; }
+
+template <class Rows, int A>
 struct AlleleSite {
+    static constexpr int G = A * (A + 1) / 2;
     Rows rows;
     uint32_t n_samples;
     double eps;
@@ -74,7 +99,7 @@ struct AlleleSite {
         uint32_t idx[8];
         uint32_t cnt = 0, nb = 0;
         for (uint32_t i = 0; i < n_samples; i++) {
-            double l[3];
+            double l[G];
             if (!rows.load(i, l)) continue;
 #pragma unroll
             for (int j = 0; j < 8; j++)
@@ -92,24 +117,33 @@ struct AlleleSite {
 #pragma unroll
         for (int j = 0; j < 8; j++)
             if ((uint32_t)j < cnt) {
-                double l[3];
+                double l[G];
                 rows.load(idx[j], l);
                 rem(l);
             }
         return nb * 8 + cnt;
     }
 
-    // allele/objective.rs:85-110
+    // allele/objective.rs:85-209: p = eps + sum_g w_g l_g, w = x_i x_i or
+    // 2 x_i x_j, in genotype order
     __device__ double obj(const double* x) const {
-        const double x00 = x[0] * x[0], x11 = x[1] * x[1], x01 = x[0] * x[1];
-        const double tx01 = x01 + x01;
+        constexpr Genotypes<A> gt;
+        double w[G];
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            if (gt.gi[g] == gt.gj[g]) {
+                w[g] = x[gt.gi[g]] * x[gt.gi[g]];
+            } else {
+                const double t = x[gt.gi[g]] * x[gt.gj[g]];
+                w[g] = t + t;
+            }
+        }
         double zs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         double rs = 0.0;
         auto row = [&](const double* l) {
             double p = eps;
-            p = fma(x00, l[0], p);
-            p = fma(tx01, l[1], p);
-            p = fma(x11, l[2], p);
+#pragma unroll
+            for (int g = 0; g < G; g++) p = fma(w[g], l[g], p);
             return sleef_log(p);
         };
         const uint32_t n =
@@ -117,121 +151,140 @@ struct AlleleSite {
         return -(radd8(zs) + rs) / (double)n;
     }
 
-    // allele/grad_hess.rs:107-166 + build_grad_hess 83-105
+    // allele/grad_hess.rs:107-396 + build_grad_hess 83-105
     __device__ void gh(const double* x, double* g, double* h /* MX stride */) const {
-        double zg0[8], zg1[8], zh0[8], zh1[8], zh2[8];
+        constexpr Genotypes<A> gt;
+        double zg[A][8], zh[G][8], rg[A], rh[G];
 #pragma unroll
-        for (int j = 0; j < 8; j++) zg0[j] = zg1[j] = zh0[j] = zh1[j] = zh2[j] = 0.0;
-        double rg0 = 0, rg1 = 0, rh0 = 0, rh1 = 0, rh2 = 0;
-        const double x0 = x[0], x1 = x[1];
-        auto row = [&](const double* l, double& g0, double& g1, double& h0, double& h1, double& h2) {
-            const double l00 = l[0], l01 = l[1], l11 = l[2];
-            double lx0 = x0 * l00;
-            lx0 = fma(x1, l01, lx0);
-            double lx1 = x0 * l01;
-            lx1 = fma(x1, l11, lx1);
+        for (int a = 0; a < A; a++) {
+            rg[a] = 0.0;
+#pragma unroll
+            for (int j = 0; j < 8; j++) zg[a][j] = 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < G; q++) {
+            rh[q] = 0.0;
+#pragma unroll
+            for (int j = 0; j < 8; j++) zh[q][j] = 0.0;
+        }
+        auto row = [&](const double* l, int j) {
+            double lx[A];
+#pragma unroll
+            for (int a = 0; a < A; a++) {
+                lx[a] = x[0] * l[gl_index(a, 0)];
+#pragma unroll
+                for (int b = 1; b < A; b++) lx[a] = fma(x[b], l[gl_index(a, b)], lx[a]);
+            }
             double d = eps;
-            d = fma(x0, lx0, d);
-            d = fma(x1, lx1, d);
+#pragma unroll
+            for (int a = 0; a < A; a++) d = fma(x[a], lx[a], d);
             const double inv = 1.0 / d;
-            const double a = lx0 * inv, b = lx1 * inv;
-            h0 = fma(2.0, a * a, h0);
-            h0 = fma(-l00, inv, h0);
-            h1 = fma(2.0, a * b, h1);
-            h1 = fma(-l01, inv, h1);
-            h2 = fma(2.0, b * b, h2);
-            h2 = fma(-l11, inv, h2);
-            g0 = fma(lx0, inv, g0);
-            g1 = fma(lx1, inv, g1);
+            double lxd[A];
+#pragma unroll
+            for (int a = 0; a < A; a++) lxd[a] = lx[a] * inv;
+#pragma unroll
+            for (int q = 0; q < G; q++) {
+                double& z = j < 0 ? rh[q] : zh[q][j];
+                z = fma(2.0, lxd[gt.gi[q]] * lxd[gt.gj[q]], z);
+                z = fma(-l[q], inv, z);
+            }
+#pragma unroll
+            for (int a = 0; a < A; a++) {
+                double& z = j < 0 ? rg[a] : zg[a][j];
+                z = fma(lx[a], inv, z);
+            }
         };
-        const uint32_t n = visit([&](int j, const double* l) { row(l, zg0[j], zg1[j], zh0[j], zh1[j], zh2[j]); },
-                                 [&](const double* l) { row(l, rg0, rg1, rh0, rh1, rh2); });
+        const uint32_t n = visit([&](int j, const double* l) { row(l, j); }, [&](const double* l) { row(l, -1); });
         const double nd = (double)n;
-        g[0] = -2.0 * (radd8(zg0) + rg0) / nd;
-        g[1] = -2.0 * (radd8(zg1) + rg1) / nd;
-        const double h01 = 2.0 * (radd8(zh1) + rh1) / nd;
-        h[0] = 2.0 * (radd8(zh0) + rh0) / nd;
-        h[1] = h01;
-        h[MX] = h01;
-        h[MX + 1] = 2.0 * (radd8(zh2) + rh2) / nd;
+#pragma unroll
+        for (int a = 0; a < A; a++) g[a] = -2.0 * (radd8(zg[a]) + rg[a]) / nd;
+#pragma unroll
+        for (int q = 0; q < G; q++) {
+            const double v = 2.0 * (radd8(zh[q]) + rh[q]) / nd;
+            h[gt.gi[q] * MX + gt.gj[q]] = v;
+            h[gt.gj[q] * MX + gt.gi[q]] = v;
+        }
     }
 
     __device__ uint32_t called() const {
         uint32_t n = 0;
         for (uint32_t i = 0; i < n_samples; i++) {
-            double l[3];
+            double l[G];
             n += rows.load(i, l);
         }
         return n;
     }
 };
 
-// allele/mod.rs:100-126: SQP from (1/2, 1/2) (sqp.rs:49-93 with QP cap 3),
-// then alleles with 0 < p < maf are zeroed and the pair renormalised.
-template <class Rows>
-__device__ void fit_site(const AlleleSite<Rows>& site, uint32_t s, const Tune& tune, double maf,
+// allele/mod.rs:100-126: SQP from the uniform start (sqp.rs:49-93 with QP cap
+// A + 1, set by the caller in tune), then alleles with 0 < p < maf are zeroed
+// and the frequencies renormalised.
+template <class Rows, int A>
+__device__ void fit_site(const AlleleSite<Rows, A>& site, uint32_t s, const Tune& tune, double maf,
                          double* __restrict__ af_out, int* __restrict__ status) {
     if (site.called() == 0) {  // lanevector.rs:73-76 would panic
-        af_out[2 * s] = 0.0;
-        af_out[2 * s + 1] = 0.0;
+#pragma unroll
+        for (int a = 0; a < A; a++) af_out[(size_t)A * s + a] = 0.0;
         atomicOr(status, 1);
         return;
     }
-    double x0[2] = {0.5, 0.5}, x[2];
-    sum_to_one<2>(x0, x);
+    double x0[A], x[A];
+#pragma unroll
+    for (int a = 0; a < A; a++) x0[a] = 1.0 / (double)A;
+    sum_to_one<A>(x0, x);
     double f = site.obj(x);
     uint64_t iter = 0;
     for (; iter < tune.sqp_max_iter; iter++) {
-        double g[2], h[MX * 2];
+        double g[A], h[MX * A];
         site.gh(x, g, h);
-        if (converged<2>(x, g, tune.sqp_conv_tol)) break;
-        double c[2], y[2];
-        for (int i = 0; i < 2; i++) c[i] = g[i] - dotn(2, h + i * MX, x);
-        solve_qp<2>(h, c, x, tune, y);
-        double p[2] = {y[0] - x[0], y[1] - x[1]};
-        const double t = tune.bls_sufficient_decrease * dotn(2, g, p);
-        double alpha = 1.0, xn[2], fn;
+        if (converged<A>(x, g, tune.sqp_conv_tol)) break;
+        double c[A], y[A];
+        for (int i = 0; i < A; i++) c[i] = g[i] - dotn(A, h + i * MX, x);
+        solve_qp<A>(h, c, x, tune, y);
+        double p[A];
+        for (int i = 0; i < A; i++) p[i] = y[i] - x[i];
+        const double t = tune.bls_sufficient_decrease * dotn(A, g, p);
+        double alpha = 1.0, xn[A], fn;
         uint64_t bi = 0;
         for (;;) {
-            xn[0] = x[0] + alpha * p[0];
-            xn[1] = x[1] + alpha * p[1];
-            sum_to_one<2>(xn, xn);
+            for (int i = 0; i < A; i++) xn[i] = x[i] + alpha * p[i];
+            sum_to_one<A>(xn, xn);
             fn = site.obj(xn);
             if (fn <= f + alpha * t) break;
             if (++bi == tune.bls_max_iter) break;
             alpha *= tune.bls_step_size_reduce;
         }
-        x[0] = xn[0];
-        x[1] = xn[1];
+        for (int i = 0; i < A; i++) x[i] = xn[i];
         f = fn;
     }
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < A; i++)
         if (x[i] > 0.0 && x[i] < maf) x[i] = 0.0;
-    double o[2];
-    sum_to_one<2>(x, o);
-    af_out[2 * s] = o[0];
-    af_out[2 * s + 1] = o[1];
+    double o[A];
+    sum_to_one<A>(x, o);
+    for (int a = 0; a < A; a++) af_out[(size_t)A * s + a] = o[a];
 }
 
 }  // namespace
 
 // One site per thread: codes (2-bit sample-major rows) or likelihood rows
-// (site-major, V x N x 3).
+// (site-major, V x N x G); af_out holds A frequencies per site.
 __global__ __launch_bounds__(64) void af_kernel(const uint8_t* __restrict__ packed, size_t row_bytes,
                                                  uint32_t n_samples, uint32_t n_sites, Tune tune, double maf,
                                                  double* __restrict__ af_out, int* __restrict__ status) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n_sites) return;
-    const AlleleSite<CodeRows> site{{packed + (s >> 2), row_bytes, (s & 3u) * 2u}, n_samples, tune.epsilon};
+    const AlleleSite<CodeRows, 2> site{{packed + (s >> 2), row_bytes, (s & 3u) * 2u}, n_samples, tune.epsilon};
     fit_site(site, s, tune, maf, af_out, status);
 }
 
+template <int A>
 __global__ __launch_bounds__(64) void af_gl_kernel(const double* __restrict__ gl, uint32_t n_samples,
                                                     uint32_t n_sites, Tune tune, double maf,
                                                     double* __restrict__ af_out, int* __restrict__ status) {
+    constexpr int G = A * (A + 1) / 2;
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n_sites) return;
-    const AlleleSite<GlRows> site{{gl + (size_t)s * n_samples * 3}, n_samples, tune.epsilon};
+    const AlleleSite<GlRows<G>, A> site{{gl + (size_t)s * n_samples * G}, n_samples, tune.epsilon};
     fit_site(site, s, tune, maf, af_out, status);
 }
 
@@ -674,6 +727,266 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_gl(const uint32_t* __restr
         for (int k = 0; k < 45; k++) out[(size_t)(10 + k) * nslots] = H[k];
         out[(size_t)55 * nslots] = N;
     }
+}
+
+// ------------------------------------------------ tri/tetra-allelic sites
+
+// iis.rs:26-104: identity mode of the alleles (i, j) of x and (k, l) of y.
+__host__ __device__ constexpr int iis_mode(int i, int j, int k, int l) {
+    const int key = ((i == j) << 5) | ((i == k) << 4) | ((i == l) << 3) | ((j == k) << 2) | ((j == l) << 1) | (k == l);
+    return key == 0x3f ? 1 : key == 0x21 ? 2 : key == 0x34 ? 3 : key == 0x2a ? 4 : key == 0x20 ? 5 : key == 0x19 ? 6
+         : key == 0x07 ? 7 : key == 0x01 ? 8 : key == 0x12 ? 9 : key == 0x0c ? 10 : key == 0x10 ? 11
+         : key == 0x08 ? 12 : key == 0x04 ? 13 : key == 0x02 ? 14 : key == 0x00 ? 15 : 0;
+}
+
+// iis.rs:106-277 accumulate_iis1..15, in their operation order; mode is a
+// constant wherever the genotype loops are unrolled.
+__device__ __forceinline__ void accumulate_iis(double* a, double g, double pi, double pj, double pk, double pl,
+                                               int mode) {
+    switch (mode) {
+    case 1: {
+        const double pi2 = pi * pi, pi3 = pi2 * pi;
+        a[0] = fma(g, pi, a[0]);
+        a[1] = fma(g, pi2, a[1]);
+        a[2] = fma(g, pi2, a[2]);
+        a[3] = fma(g, pi3, a[3]);
+        a[4] = fma(g, pi2, a[4]);
+        a[5] = fma(g, pi3, a[5]);
+        a[6] = fma(g, pi2, a[6]);
+        a[7] = fma(g, pi3, a[7]);
+        a[8] = fma(g, pi2 * pi2, a[8]);
+    } break;
+    case 2: {
+        const double pipk = pi * pk;
+        a[1] = fma(g, pipk, a[1]);
+        a[3] = fma(g, pipk * pk, a[3]);
+        a[5] = fma(g, pi * pipk, a[5]);
+        a[8] = fma(g, pipk * pipk, a[8]);
+    } break;
+    case 3: {
+        const double pipl = pi * pl, pi2pl = pi * pipl, two = pi2pl + pi2pl;
+        a[2] = fma(g, pipl, a[2]);
+        a[3] = fma(g, two, a[3]);
+        a[7] = fma(g, pi2pl, a[7]);
+        a[8] = fma(g, two * pi, a[8]);
+    } break;
+    case 4: {
+        const double pipk = pi * pk, pi2pk = pi * pipk, two = pi2pk + pi2pk;
+        a[2] = fma(g, pipk, a[2]);
+        a[3] = fma(g, two, a[3]);
+        a[7] = fma(g, pi2pk, a[7]);
+        a[8] = fma(g, two * pi, a[8]);
+    } break;
+    case 5: {
+        const double v = (pi * pk) * pl, two = v + v;
+        a[3] = fma(g, two, a[3]);
+        a[8] = fma(g, two * pi, a[8]);
+    } break;
+    case 6: {
+        const double pipj = pi * pj, pi2pj = pi * pipj, two = pi2pj + pi2pj;
+        a[4] = fma(g, pipj, a[4]);
+        a[5] = fma(g, two, a[5]);
+        a[7] = fma(g, pi2pj, a[7]);
+        a[8] = fma(g, two * pi, a[8]);
+    } break;
+    case 7: {
+        const double pipj = pi * pj, pj2pi = pj * pipj, two = pj2pi + pj2pi;
+        a[4] = fma(g, pipj, a[4]);
+        a[5] = fma(g, two, a[5]);
+        a[7] = fma(g, pj2pi, a[7]);
+        a[8] = fma(g, two * pj, a[8]);
+    } break;
+    case 8: {
+        const double v = (pi * pj) * pk, two = v + v;
+        a[5] = fma(g, two, a[5]);
+        a[8] = fma(g, two * pk, a[8]);
+    } break;
+    case 9:
+    case 10: {  // mode 10 is mode 9 (iis.rs:213-215)
+        const double pipj = pi * pj, two = pipj + pipj;
+        a[6] = fma(g, two, a[6]);
+        a[7] = fma(g, pipj * (pi + pj), a[7]);
+        a[8] = fma(g, two * two, a[8]);
+    } break;
+    case 11: {
+        const double v = (pi * pj) * pl;
+        a[7] = fma(g, v, a[7]);
+        a[8] = fma(g, (4.0 * v) * pi, a[8]);
+    } break;
+    case 12: {
+        const double v = (pi * pj) * pk;
+        a[7] = fma(g, v, a[7]);
+        a[8] = fma(g, (4.0 * v) * pi, a[8]);
+    } break;
+    case 13: {
+        const double v = (pi * pj) * pl;
+        a[7] = fma(g, v, a[7]);
+        a[8] = fma(g, (4.0 * v) * pj, a[8]);
+    } break;
+    case 14: {
+        const double v = (pi * pj) * pk;
+        a[7] = fma(g, v, a[7]);
+        a[8] = fma(g, (4.0 * v) * pj, a[8]);
+    } break;
+    case 15: {
+        const double v = (pi * pj) * (pk * pl);
+        a[8] = fma(g, 4.0 * v, a[8]);
+    } break;
+    default:
+        break;
+    }
+}
+
+// mixture.rs:8-60 over IIS_A: for every (gx, gy) in genotype order,
+// g = lx[gx] ly[gy] and accumulate_iis with the four alleles' frequencies.
+template <int A>
+__device__ __forceinline__ void mixture_generic(const double* lx, const double* ly, const double* p, double* a) {
+    constexpr Genotypes<A> gt;
+    constexpr int G = Genotypes<A>::G;
+#pragma unroll
+    for (int k = 0; k < 9; k++) a[k] = 0.0;
+#pragma unroll
+    for (int gx = 0; gx < G; gx++)
+#pragma unroll
+        for (int gy = 0; gy < G; gy++) {
+            const int i = gt.gi[gx], j = gt.gj[gx], k = gt.gi[gy], l = gt.gj[gy];
+            accumulate_iis(a, lx[gx] * ly[gy], p[i], p[j], p[k], p[l], iis_mode(i, j, k, l));
+        }
+}
+
+// The sweep over sites with A = 3 or 4 alleles: as sweep_gl with rows of
+// G = A(A+1)/2 likelihoods per site and the generic mixture.
+template <int A>
+__global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __restrict__ geno, uint32_t row_words,
+                                                           const double* __restrict__ gl, uint32_t v_pad,
+                                                           const double* __restrict__ af,
+                                                           const uint32_t* __restrict__ seg_bounds,
+                                                           const uint32_t* __restrict__ sx,
+                                                           const uint32_t* __restrict__ sy,
+                                                           const double* __restrict__ pt,
+                                                           const int* __restrict__ active, uint32_t nslots,
+                                                           double eps, double* __restrict__ partial) {
+    constexpr int G = A * (A + 1) / 2;
+    __shared__ __attribute__((aligned(16))) double kt[2 * CHUNK * 4];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t slot = blockIdx.x * SWEEP_TPB + tid;
+    const uint32_t seg = blockIdx.y;
+    const bool live = slot < nslots && active[slot] != 0;
+    if (!__syncthreads_or(live)) return;
+
+    double D[9];
+    uint32_t x = 0, y = 0;
+    if (live) {
+        x = sx[slot];
+        y = sy[slot];
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k] = pt[(size_t)k * nslots + slot];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k] = 0.0;
+    }
+    const uint32_t* rowx = geno + (size_t)x * row_words;
+    const uint32_t* rowy = geno + (size_t)y * row_words;
+    const double* glx = gl + (size_t)x * v_pad * G;
+    const double* gly = gl + (size_t)y * v_pad * G;
+
+    double F = 0.0, N = 0.0, P = 1.0, Gr[9], H[45];
+#pragma unroll
+    for (int k = 0; k < 9; k++) Gr[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 45; k++) H[k] = 0.0;
+
+    const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
+    auto build = [&](uint32_t cs, int b) {
+        if (tid < CHUNK)
+#pragma unroll
+            for (int a = 0; a < 4; a++) kt[b * CHUNK * 4 + tid * 4 + a] = a < A ? af[(size_t)A * (cs + tid) + a] : 0.0;
+    };
+    if (s0 < s1) build(s0, 0);
+    __syncthreads();
+    int b = 0;
+    for (uint32_t cs = s0; cs < s1; cs += CHUNK, b ^= 1) {
+        if (cs + CHUNK < s1) build(cs + CHUNK, b ^ 1);
+        const uint2 wx = *reinterpret_cast<const uint2*>(rowx + cs / 16);
+        const uint2 wy = *reinterpret_cast<const uint2*>(rowy + cs / 16);
+        {
+            const uint32_t ux = ~((wx.x & (wx.x >> 1)) | (wy.x & (wy.x >> 1))) & 0x55555555u;
+            const uint32_t uy = ~((wx.y & (wx.y >> 1)) | (wy.y & (wy.y >> 1))) & 0x55555555u;
+            N += (double)(__popc(ux) + __popc(uy));
+        }
+        const double* K0 = kt + b * CHUNK * 4;
+#pragma unroll 2
+        for (int k = 0; k < CHUNK; k++) {
+            const uint32_t ax = (k < 16 ? wx.x : wx.y) >> (2 * (k & 15));
+            const uint32_t ay = (k < 16 ? wy.x : wy.y) >> (2 * (k & 15));
+            const bool missing = (ax & 3u) == 3u || (ay & 3u) == 3u;
+            double lx[G], ly[G];
+            const double2* px = reinterpret_cast<const double2*>(glx + (size_t)(cs + k) * G);
+            const double2* py = reinterpret_cast<const double2*>(gly + (size_t)(cs + k) * G);
+#pragma unroll
+            for (int q = 0; q < G / 2; q++) {
+                const double2 vx = px[q], vy = py[q];
+                lx[2 * q] = vx.x;
+                lx[2 * q + 1] = vx.y;
+                ly[2 * q] = vy.x;
+                ly[2 * q + 1] = vy.y;
+            }
+            const double p[4] = {K0[k * 4], K0[k * 4 + 1], K0[k * 4 + 2], K0[k * 4 + 3]};
+            double cv[9];
+            mixture_generic<A>(lx, ly, p, cv);
+            site_update(cv, missing ? 1.0 : eps, D, k, P, F, Gr, H);
+        }
+        __syncthreads();
+    }
+    if (live) {
+        double* out = partial + (size_t)seg * NPART * nslots + slot;
+        out[0] = F;
+#pragma unroll
+        for (int k = 0; k < 9; k++) out[(size_t)(1 + k) * nslots] = Gr[k];
+#pragma unroll
+        for (int k = 0; k < 45; k++) out[(size_t)(10 + k) * nslots] = H[k];
+        out[(size_t)55 * nslots] = N;
+    }
+}
+
+// Re-bucketed rows (wrapper.rs:237-410 remove_zero_frequencies): new site t of
+// a part with Gp genotypes takes, from site map_site[t] of the input bucket
+// with map_alleles[t] alleles, the likelihoods at gmap[t][0..Gp) (the
+// genotypes of the alleles with non-zero frequency); thread (sample i, word w)
+// writes sites 16w..16w+15 of row i (0 for a missing call, a NaN among the
+// selected entries) and the 2-bit mask word (3 = missing, padding = 3).
+__global__ __launch_bounds__(256) void gather_multi_kernel(const double* __restrict__ src2,
+                                                            const double* __restrict__ src3,
+                                                            const double* __restrict__ src4, uint32_t n_samples,
+                                                            const uint32_t* __restrict__ map_site,
+                                                            const uint8_t* __restrict__ map_alleles,
+                                                            const uint8_t* __restrict__ gmap, uint32_t n_t, int gp,
+                                                            double* __restrict__ rows, uint32_t v_pad,
+                                                            uint32_t* __restrict__ mask, uint32_t row_words) {
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (size_t)n_samples * row_words) return;
+    const uint32_t i = (uint32_t)(tid / row_words), w = (uint32_t)(tid % row_words);
+    uint32_t word = 0;
+    for (int j = 0; j < 16; j++) {
+        const uint32_t t = w * 16 + j;
+        double* dst = rows + ((size_t)i * v_pad + t) * gp;
+        uint32_t code = 3;
+        if (t < n_t) {
+            const int na = map_alleles[t];
+            const int gs = na * (na + 1) / 2;
+            const double* src = (na == 2 ? src2 : na == 3 ? src3 : src4) + ((size_t)map_site[t] * n_samples + i) * gs;
+            bool nan = false;
+            for (int k = 0; k < gp; k++) nan |= isnan(src[gmap[(size_t)t * gp + k]]);
+            if (!nan) {
+                code = 0;
+                for (int k = 0; k < gp; k++) dst[k] = src[gmap[(size_t)t * gp + k]];
+            }
+        }
+        if (code == 3)
+            for (int k = 0; k < gp; k++) dst[k] = 0.0;
+        word |= code << (2 * j);
+    }
+    mask[(size_t)i * row_words + w] = word;
 }
 
 // Soft rows of the kept sites: thread (sample i, word w) moves sites
@@ -1293,9 +1606,43 @@ hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, con
 }
 
 hipError_t af_gl(const double* gl, uint32_t n_samples, uint32_t n_sites, const Tune& t, double maf, double* af_out,
-                 int* status, hipStream_t st) {
-    hipLaunchKernelGGL(af_gl_kernel, dim3((n_sites + 63) / 64), dim3(64), 0, st, gl, n_samples, n_sites, t, maf, af_out,
-                       status);
+                 int* status, hipStream_t st, int alleles) {
+    if (n_sites == 0) return hipSuccess;
+    const dim3 grid((n_sites + 63) / 64);
+    if (alleles == 2)
+        hipLaunchKernelGGL(af_gl_kernel<2>, grid, dim3(64), 0, st, gl, n_samples, n_sites, t, maf, af_out, status);
+    else if (alleles == 3)
+        hipLaunchKernelGGL(af_gl_kernel<3>, grid, dim3(64), 0, st, gl, n_samples, n_sites, t, maf, af_out, status);
+    else if (alleles == 4)
+        hipLaunchKernelGGL(af_gl_kernel<4>, grid, dim3(64), 0, st, gl, n_samples, n_sites, t, maf, af_out, status);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t sweep_multi(int alleles, const uint32_t* geno, uint32_t row_words, const double* gl, uint32_t v_pad,
+                       const double* af, const uint32_t* seg_bounds, uint32_t nseg, const uint32_t* sx,
+                       const uint32_t* sy, const double* pt, const int* active, uint32_t nslots, double eps,
+                       double* partial, hipStream_t st) {
+    dim3 grid((nslots + SWEEP_TPB - 1) / SWEEP_TPB, nseg);
+    if (alleles == 3)
+        hipLaunchKernelGGL(sweep_glm<3>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, af, seg_bounds, sx,
+                           sy, pt, active, nslots, eps, partial);
+    else if (alleles == 4)
+        hipLaunchKernelGGL(sweep_glm<4>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, af, seg_bounds, sx,
+                           sy, pt, active, nslots, eps, partial);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t gather_multi(const double* src2, const double* src3, const double* src4, uint32_t n_samples,
+                        const uint32_t* map_site, const uint8_t* map_alleles, const uint8_t* gmap, uint32_t n_t,
+                        int gp, double* rows, uint32_t v_pad, uint32_t* mask, uint32_t row_words, hipStream_t st) {
+    const size_t n = (size_t)n_samples * row_words;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_multi_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src2, src3, src4,
+                       n_samples, map_site, map_alleles, gmap, n_t, gp, rows, v_pad, mask, row_words);
     return hipGetLastError();
 }
 

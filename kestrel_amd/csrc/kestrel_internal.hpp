@@ -52,9 +52,18 @@ hipError_t af(const uint8_t* packed, size_t row_bytes, uint32_t n_samples, uint3
 // soft likelihoods: allele frequencies from site-major V x N x 3 rows, and
 // the kept sites into sample-major rows (v_pad triples) + 2-bit mask rows
 hipError_t af_gl(const double* gl, uint32_t n_samples, uint32_t n_sites, const kdev::Tune& t, double maf,
-                 double* af_out, int* status, hipStream_t st);
+                 double* af_out, int* status, hipStream_t st, int alleles = 2);
 hipError_t gather_gl(const double* gl_site, uint32_t n_samples, const uint32_t* kept, uint32_t n_kept, double* rows,
                      uint32_t v_pad, uint32_t* mask, uint32_t row_words, hipStream_t st);
+// tri/tetra-allelic parts: the sweep over a part of sites with 3 or 4 alleles
+// (rows of A(A+1)/2 likelihoods, af of A per site), and the re-bucketing gather
+hipError_t sweep_multi(int alleles, const uint32_t* geno, uint32_t row_words, const double* gl, uint32_t v_pad,
+                       const double* af, const uint32_t* seg_bounds, uint32_t nseg, const uint32_t* sx,
+                       const uint32_t* sy, const double* pt, const int* active, uint32_t nslots, double eps,
+                       double* partial, hipStream_t st);
+hipError_t gather_multi(const double* src2, const double* src3, const double* src4, uint32_t n_samples,
+                        const uint32_t* map_site, const uint8_t* map_alleles, const uint8_t* gmap, uint32_t n_t,
+                        int gp, double* rows, uint32_t v_pad, uint32_t* mask, uint32_t row_words, hipStream_t st);
 hipError_t gather(const uint8_t* src, size_t src_row_bytes, const uint32_t* kept, uint32_t n_kept,
                   uint32_t n_samples, uint32_t* dst, uint32_t dst_row_words, hipStream_t st,
                   const uint32_t* dst_rows = nullptr);

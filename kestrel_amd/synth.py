@@ -163,6 +163,48 @@ def soft_likelihoods(codes: np.ndarray, seed: int, mean_depth: float = 4.0, erro
     return np.ascontiguousarray(gl.transpose(1, 0, 2))
 
 
+def multi_likelihoods(n_samples: int, n_sites: tuple, seed: int, mean_depth: float = 5.0, error: float = 0.01,
+                      absent: float = 0.3, missing: float = 0.02):
+    """Soft likelihood buckets (gl2, gl3, gl4) for sites with 2, 3, 4 alleles
+    (each (V_A, N, A(A+1)/2), VCF genotype order), site-major like the
+    reference's GenotypeLikelihoods.  Genotypes follow Hardy-Weinberg from
+    Dirichlet allele frequencies; with probability `absent` one allele of a
+    3/4-allele site is never carried (its fitted frequency is 0, so the site
+    moves to a smaller bucket).  Reads: Poisson(mean_depth) per call, each the
+    carried allele (one of the two at random) with probability 1 - error, else
+    another allele; likelihoods scaled by their maximum; no read or a
+    `missing` draw gives a NaN row."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for A, v in zip((2, 3, 4), n_sites):
+        G = A * (A + 1) // 2
+        gi = [i for j in range(A) for i in range(j + 1)]
+        gj = [j for j in range(A) for i in range(j + 1)]
+        gl = np.empty((v, n_samples, G))
+        for s in range(v):
+            p = rng.dirichlet(np.full(A, 0.7))
+            if A > 2 and rng.random() < absent:
+                p[rng.integers(A)] = 0.0
+                p /= p.sum()
+            alleles = rng.choice(A, (n_samples, 2), p=p)
+            depth = rng.poisson(mean_depth, n_samples)
+            ll = np.zeros((n_samples, G))
+            for r in range(n_samples):
+                for _ in range(depth[r]):
+                    carried = alleles[r, rng.integers(2)]
+                    seen = carried if rng.random() >= error else rng.choice([a for a in range(A) if a != carried])
+                    for g in range(G):
+                        pi = (1 - error) if seen == gi[g] else error / (A - 1)
+                        pj = (1 - error) if seen == gj[g] else error / (A - 1)
+                        ll[r, g] += np.log(0.5 * pi + 0.5 * pj)
+            ll -= ll.max(axis=1, keepdims=True)
+            row = np.exp(ll)
+            row[(depth == 0) | (rng.random(n_samples) < missing)] = np.nan
+            gl[s] = row
+        out.append(gl)
+    return tuple(out)
+
+
 def pack2(codes: np.ndarray, row_sites: int | None = None) -> np.ndarray:
     """Pack codes (N, V) into 2 bits per call, 4 calls per byte, site s of a
     row in bits 2*(s%4) of byte s//4.  Rows are padded with code 3 (missing)

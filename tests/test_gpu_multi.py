@@ -1,0 +1,113 @@
+"""Tri/tetra-allelic sites on the device (kestrel_cohort_create_gl_multi,
+sweep_glm) against the oracle: input-bucket allele frequencies and the
+re-bucketed part sizes bit-exact; pair records bit-exact in the device's
+summation order (2-allele sites, then 3, then 4, each padded to 256 and
+segmented); the reference's order within the stated tolerance."""
+import numpy as np
+import pytest
+
+from kestrel_amd import Cohort, relatedness_coefficients_gl, segment_bounds, synth
+
+pytestmark = pytest.mark.gpu
+
+N = 20
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def same(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return (a == b) | (np.isnan(a) & np.isnan(b))
+
+
+def assert_bitexact(out, exp):
+    for k in ("x", "y", "jacquard", "obj", "iters", "n_sites", "cond"):
+        ok = same(out[k], exp[k]).reshape(len(out), -1).all(1)
+        assert ok.all(), f"{k}: {int((~ok).sum())} of {len(out)} pairs differ"
+
+
+def device_order(parts):
+    """Pads and segment bounds of the device's parts (each part's sites
+    rounded up to 256, the 2-allele part at least 256)."""
+    pads, seg, base = {}, [0], 0
+    for A in (2, 3, 4):
+        n = parts[A][0].shape[0]
+        if A > 2 and n == 0:
+            continue
+        b = segment_bounds(n)
+        pads[A] = int(b[-1])
+        seg += [int(v) + base for v in b[1:]]
+        base += int(b[-1])
+    return pads, np.array(seg, np.uint32)
+
+
+@pytest.fixture(scope="module")
+def multi(O):
+    b2, b3, b4 = synth.multi_likelihoods(N, (900, 260, 120), seed=51)
+    co = Cohort.gl_multi(b2, b3, b4)
+    parts, af_in = O.multi_prepare({2: b2, 3: b3, 4: b4})
+    yield co, parts, af_in, (b2, b3, b4)
+    co.close()
+
+
+def test_multi_frequencies_and_parts(multi):
+    co, parts, af_in, _ = multi
+    say("case: multi-allelic frequencies and re-bucketing")
+    counts, af = co.multi_info()
+    assert counts == tuple(parts[A][0].shape[0] for A in (2, 3, 4))
+    assert counts[0] > 900 and counts[1] > 0 and counts[2] > 0  # 3/4-allele sites moved down
+    for A in (2, 3, 4):
+        assert np.array_equal(af[A], af_in[A]), A
+
+
+def test_multi_fit_bitexact_and_tolerance(O, multi):
+    co, parts, af_in, _ = multi
+    say(f"case: multi-allelic fit, {N * (N + 1) // 2} pairs")
+    out = co.fit_pairs()
+    pairs = np.stack([out["x"], out["y"]], 1)
+    pads, seg = device_order(parts)
+    assert_bitexact(out, O.multi_fit_pairs(parts, pairs, order=O.ORDER_SEG, pads=pads, seg=seg))
+    ref = O.multi_fit_pairs(parts, pairs, order=O.ORDER_REF)
+    lc = np.log10(ref["cond"])
+    well = np.isfinite(lc) & (lc < 12)
+    assert well.sum() > 100
+    assert np.abs(out["jacquard"] - ref["jacquard"]).max(1)[well].max() <= 1e-6
+    assert (np.abs(out["obj"] - ref["obj"]) / np.abs(ref["obj"]))[well].max() <= 1e-9
+
+
+def test_multi_evaluate_bitexact(O, multi):
+    co, parts, af_in, _ = multi
+    say("case: multi-allelic evaluate")
+    rng = np.random.default_rng(2)
+    pairs = rng.integers(0, N, (40, 2))
+    pts = rng.dirichlet(np.ones(9), 40)
+    f, g, h, ns = co.evaluate(pairs, pts)
+    pads, seg = device_order(parts)
+    for i in range(40):
+        rows, valid = O.multi_pair_rows(parts, int(pairs[i, 0]), int(pairs[i, 1]), O.ORDER_SEG, pads)
+        assert ns[i] == valid.sum()
+        assert f[i] == O.obj_seg(rows, valid, seg, pts[i])
+        ge, he = O.gh_seg(rows, valid, seg, pts[i])
+        assert np.array_equal(g[i], ge) and np.array_equal(h[i], he)
+
+
+def test_biallelic_bucket_alone_equals_create_gl(multi):
+    co, parts, af_in, (b2, _, _) = multi
+    say("case: gl_multi with only 2-allele sites vs create_gl")
+    with Cohort.gl_multi(b2) as a, Cohort.gl(b2) as b:
+        assert_bitexact(a.fit_pairs(), b.fit_pairs())
+
+
+def test_relatedness_coefficients_gl_tetra_only(O):
+    say("case: relatedness_coefficients_gl on 4-allele sites")
+    _, _, b4 = synth.multi_likelihoods(12, (0, 0, 400), seed=52)
+    jac, kin, inb = relatedness_coefficients_gl(b4)
+    parts, _ = O.multi_prepare({4: b4})
+    tri = [(x, y) for x in range(12) for y in range(x, 12)]
+    pads, seg = device_order(parts)
+    exp = O.multi_fit_pairs(parts, tri, order=O.ORDER_SEG, pads=pads, seg=seg)
+    from kestrel_amd import build_arrays
+    ej, ek, ei = build_arrays(exp, 12)
+    assert np.array_equal(jac, ej) and np.array_equal(kin, ek) and np.array_equal(inb, ei)

@@ -23,8 +23,8 @@ def test_gt_codes_rejects_multiallelic_and_bad_shape():
 def test_gl_input_checks_before_any_device_call():
     with pytest.raises(NotImplementedError):
         relatedness_coefficients_gl(np.eye(3)[None].repeat(2, 0), allele_frequencies=np.full((2, 2), 0.5))
-    with pytest.raises(ValueError, match="tri- and tetra"):
-        relatedness_coefficients_gl(np.zeros((2, 3, 6)))
+    with pytest.raises(ValueError, match="3, 6 or 10 genotypes"):
+        relatedness_coefficients_gl(np.zeros((2, 3, 5)))
 
 
 def test_build_arrays_matches_reference_assembly():
