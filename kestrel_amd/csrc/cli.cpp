@@ -247,15 +247,19 @@ int main(int argc, char** argv) {
     info("Skipped " + std::to_string(d.not_snp) + " sites that were not SNPs");
     if (a.field == kvcf::Field::GT) info("Skipped " + std::to_string(d.skipped_missing) + " sites with missing data");
     info("Found " + std::to_string(d.n_sites) + " di-allelic sites");
-    info("Found 0 tri-allelic sites");
-    info("Found 0 tetra-allelic sites");
+    info("Found " + std::to_string(d.n_sites3) + " tri-allelic sites");
+    info("Found " + std::to_string(d.n_sites4) + " tetra-allelic sites");
     const uint32_t n = (uint32_t)d.samples.size();
     if (!a.dump_probs.empty()) {
-        // n_sites x n_samples x 3 doubles, site-major (the rows the likelihood path receives)
+        // the 2-, 3- and 4-allele buckets in turn as raw doubles, site-major
+        // (n_sites x n_samples x 3, then x 6, then x 10: the rows the
+        // likelihood paths receive)
         FILE* f = fopen(a.dump_probs.c_str(), "wb");
-        if (!f || fwrite(d.probs.data(), sizeof(double), d.probs.size(), f) != d.probs.size())
-            return die("cannot write " + a.dump_probs);
-        fclose(f);
+        bool ok = f != nullptr;
+        for (const std::vector<double>* v : {&d.probs, &d.probs3, &d.probs4})
+            ok = ok && fwrite(v->data(), sizeof(double), v->size(), f) == v->size();
+        if (f) fclose(f);
+        if (!ok) return die("cannot write " + a.dump_probs);
         return 0;
     }
     if (a.dump_codes) {
@@ -274,7 +278,7 @@ int main(int argc, char** argv) {
                 (unsigned long long)d.not_snp, (unsigned long long)d.skipped_missing);
         return 0;
     }
-    if (d.n_sites == 0) return die("Must have at least one site");
+    if ((uint64_t)d.n_sites + d.n_sites3 + d.n_sites4 == 0) return die("Must have at least one site");
     if (n == 0) return die("Must have at least one sample");
     const uint32_t nsel = a.samples < 0 ? n : (uint32_t)a.samples;
     if (nsel > n) {
@@ -310,20 +314,31 @@ int main(int argc, char** argv) {
         }
         info("Calculating allele frequencies for " + std::to_string(d.n_sites) + " sites using " + std::to_string(n) +
              " samples");
-        // hard calls take the 2-bit path; any soft row, the likelihood path
-        if (d.one_hot)
+        // hard calls take the 2-bit path; any soft row, the likelihood path;
+        // sites with 3 or 4 alleles, the multi-allelic likelihood path
+        const bool multi = d.n_sites3 || d.n_sites4;
+        if (multi)
+            rc = kestrel_cohort_create_gl_multi(&c, n, d.n_sites ? d.probs.data() : nullptr, d.n_sites,
+                                                d.n_sites3 ? d.probs3.data() : nullptr, d.n_sites3,
+                                                d.n_sites4 ? d.probs4.data() : nullptr, d.n_sites4, maf, a.iterations);
+        else if (d.one_hot)
             rc = kestrel_cohort_create(&c, packed.data(), n, d.n_sites, row_bytes, af, maf, a.iterations,
                                        KESTREL_FP64);
         else
             rc = kestrel_cohort_create_gl(&c, d.probs.data(), n, d.n_sites, af, maf, a.iterations);
         if (rc) return die(kestrel_last_error());
         tracker.checkpoint();
-        uint32_t kept = 0;
-        kestrel_cohort_n_kept(c, &kept);
-        info("Identified " + std::to_string(kept) + " di-allelic sites");
-        info("Identified 0 tri-allelic sites");
-        info("Identified 0 tetra-allelic sites");
-        info("Discarded " + std::to_string(d.n_sites - kept) + " non-segregating sites");
+        uint32_t parts[3] = {0, 0, 0};
+        if (multi)
+            kestrel_cohort_multi_info(c, parts, nullptr, nullptr, nullptr);
+        else
+            kestrel_cohort_n_kept(c, &parts[0]);
+        const uint64_t input = (uint64_t)d.n_sites + d.n_sites3 + d.n_sites4;
+        const uint32_t kept = parts[0] + parts[1] + parts[2];
+        info("Identified " + std::to_string(parts[0]) + " di-allelic sites");
+        info("Identified " + std::to_string(parts[1]) + " tri-allelic sites");
+        info("Identified " + std::to_string(parts[2]) + " tetra-allelic sites");
+        info("Discarded " + std::to_string(input - kept) + " non-segregating sites");
         info("Calculating Jacquard coefficients for " + std::to_string(npairs) + " pairs using " +
              std::to_string(kept) + " sites");
         kestrel_tune t;

@@ -280,7 +280,8 @@ struct Batch {
     std::vector<uint8_t> codes;  // kept sites x n
     std::vector<double> probs;   // kept sites x n x 3 (GL / PL)
     std::vector<double> af;      // kept sites x 2 (INFO AF)
-    uint32_t kept = 0;
+    std::vector<double> probs3, probs4;  // GL / PL sites with 3 and 4 alleles
+    uint32_t kept = 0, kept3 = 0, kept4 = 0;
     uint64_t total = 0, not_snp = 0, skipped_missing = 0, multiallelic = 0;
     bool one_hot = true;
     std::string err;  // the first error of the batch (parsing stops there)
@@ -311,10 +312,16 @@ bool parse_record(const char* lb, const char* le, uint64_t recno, const Ctx& ctx
         out.not_snp++;
         return true;
     }
-    if (nb > 2) {
+    if (nb > 2 && field == Field::GT) {
         out.multiallelic++;
         out.err = "record " + std::to_string(recno) + " (" + str(cols[0]) + ":" + str(cols[1]) + ") has " +
-                  std::to_string(nb) + " alleles: tri- and tetra-allelic sites are not supported by this build";
+                  std::to_string(nb) + " alleles: -f GT takes biallelic sites in this build (use -f GL or PL)";
+        return false;
+    }
+    const int ng = nb * (nb + 1) / 2;  // likelihoods per sample (vcf.rs:270-305)
+    if (nb > 2 && ctx.parse_af) {  // wrapper.rs:113-122 leaves GL with given frequencies unimplemented
+        out.err = "record " + std::to_string(recno) + ": INFO allele frequencies with tri- and tetra-allelic "
+                  "likelihoods are not supported";
         return false;
     }
     const char* key = field == Field::GT ? "GT" : field == Field::GL ? "GL" : "PL";
@@ -332,7 +339,7 @@ bool parse_record(const char* lb, const char* le, uint64_t recno, const Ctx& ctx
     auto& codes = sc.codes;
     auto& site_probs = sc.probs;
     codes.assign(n, 3);
-    if (field != Field::GT) site_probs.assign(3 * n, NAN);
+    if (field != Field::GT) site_probs.assign((size_t)ng * n, NAN);
     bool skip = false;
     for (size_t i = 0; i < n && !skip; i++) {
         split(cols[9 + i].first, cols[9 + i].second, ':', sub);
@@ -387,13 +394,13 @@ bool parse_record(const char* lb, const char* le, uint64_t recno, const Ctx& ctx
                 return false;
             }
             split(b, e, ',', vals);
-            if (vals.size() != 3) {
-                out.err = vals.size() > 3 ? "Too many genotype likelihoods!" : "Not enough genotype likelihoods!";
+            if (vals.size() != (size_t)ng) {
+                out.err = vals.size() > (size_t)ng ? "Too many genotype likelihoods!" : "Not enough genotype likelihoods!";
                 return false;
             }
-            double v[3], prob[3];
+            double v[10], prob[10];
             bool any_nan = false;
-            for (int k = 0; k < 3; k++) {
+            for (int k = 0; k < ng; k++) {
                 bool miss, ok;
                 if (field == Field::GL) {
                     v[k] = parse_f32(vals[k].first, vals[k].second, &miss, &ok);
@@ -414,18 +421,19 @@ bool parse_record(const char* lb, const char* le, uint64_t recno, const Ctx& ctx
             }
             if (field == Field::GL) {  // vcf.rs:349-368: normalise by the maximum GL
                 double mx = -INFINITY;
-                for (int k = 0; k < 3; k++) mx = std::fmax(mx, v[k]);
-                for (int k = 0; k < 3; k++) prob[k] = std::pow(10.0, v[k] - mx);
+                for (int k = 0; k < ng; k++) mx = std::fmax(mx, v[k]);
+                for (int k = 0; k < ng; k++) prob[k] = std::pow(10.0, v[k] - mx);
             } else {  // vcf.rs:407
-                for (int k = 0; k < 3; k++) prob[k] = std::pow(10.0, -v[k] / 10.0);
+                for (int k = 0; k < ng; k++) prob[k] = std::pow(10.0, -v[k] / 10.0);
             }
+            for (int k = 0; k < ng; k++) site_probs[(size_t)ng * i + k] = prob[k];
+            if (ng != 3) continue;  // 3 / 4 alleles: the likelihood rows are all the path reads
             int one = -1, zeros = 0;
             for (int k = 0; k < 3; k++) {
                 if (prob[k] != prob[k]) any_nan = true;
                 else if (prob[k] == 1.0 && one < 0) one = k;
                 else if (prob[k] == 0.0) zeros++;
             }
-            for (int k = 0; k < 3; k++) site_probs[3 * i + k] = prob[k];
             if (any_nan) {
                 codes[i] = 3;  // the row is dropped for every pair with this sample
             } else if (one >= 0 && zeros == 2) {
@@ -469,6 +477,16 @@ bool parse_record(const char* lb, const char* le, uint64_t recno, const Ctx& ctx
             out.err = "No AF info found";
             return false;
         }
+    }
+    if (nb == 3) {
+        out.probs3.insert(out.probs3.end(), site_probs.begin(), site_probs.end());
+        out.kept3++;
+        return true;
+    }
+    if (nb == 4) {
+        out.probs4.insert(out.probs4.end(), site_probs.begin(), site_probs.end());
+        out.kept4++;
+        return true;
     }
     out.codes.insert(out.codes.end(), codes.begin(), codes.end());
     if (field != Field::GT) out.probs.insert(out.probs.end(), site_probs.begin(), site_probs.end());
@@ -561,7 +579,11 @@ bool read(const std::string& path, Field field, bool parse_af, Data& out, std::s
         out.site_codes.insert(out.site_codes.end(), b.codes.begin(), b.codes.end());
         out.probs.insert(out.probs.end(), b.probs.begin(), b.probs.end());
         out.af.insert(out.af.end(), b.af.begin(), b.af.end());
+        out.probs3.insert(out.probs3.end(), b.probs3.begin(), b.probs3.end());
+        out.probs4.insert(out.probs4.end(), b.probs4.begin(), b.probs4.end());
         out.n_sites += b.kept;
+        out.n_sites3 += b.kept3;
+        out.n_sites4 += b.kept4;
         if (!b.err.empty()) {
             failed = true;
             err = b.err;

@@ -205,6 +205,45 @@ def multi_likelihoods(n_samples: int, n_sites: tuple, seed: int, mean_depth: flo
     return tuple(out)
 
 
+def write_vcf_multi(path: str, buckets, field: str = "GL", names=None) -> None:
+    """A VCF of likelihood buckets (gl2, gl3, gl4) as multi_likelihoods makes
+    them: records interleaved 2-, 3-, 4-allele site by site (each bucket keeps
+    its order), GL as log10 with 7 significant digits or PL as Phred integers,
+    '.' values for a NaN row."""
+    import gzip
+
+    n = next(b.shape[1] for b in buckets if b is not None and b.size)
+    names = names or [f"S{i:05d}" for i in range(n)]
+    alts = {2: "C", 3: "C,G", 4: "C,G,T"}
+
+    def cells(rows):
+        with np.errstate(divide="ignore"):
+            lg = np.log10(rows)
+        out = []
+        for r in lg:
+            if np.isnan(r).any():
+                out.append(",".join(["."] * len(r)))
+            elif field == "GL":
+                out.append(",".join(f"{x:.7g}" for x in r))
+            else:
+                out.append(",".join(str(int(min(round(-10 * x), 99999))) for x in r))
+        return "\t".join(out)
+
+    opener = gzip.open if path.endswith(".gz") else open
+    with opener(path, "wt") as f:
+        f.write("##fileformat=VCFv4.2\n")
+        f.write(f'##FORMAT=<ID={field},Number=G,Type={"Float" if field == "GL" else "Integer"},Description="">\n')
+        f.write("#CHROM\tPOS\tID\tREF\tALT\tQUAL\tFILTER\tINFO\tFORMAT\t" + "\t".join(names) + "\n")
+        pos = 0
+        longest = max(b.shape[0] for b in buckets if b is not None)
+        for k in range(longest):
+            for A, b in zip((2, 3, 4), buckets):
+                if b is None or k >= b.shape[0]:
+                    continue
+                pos += 1
+                f.write(f"1\t{pos}\t.\tA\t{alts[A]}\t.\tPASS\t.\t{field}\t{cells(b[k])}\n")
+
+
 def pack2(codes: np.ndarray, row_sites: int | None = None) -> np.ndarray:
     """Pack codes (N, V) into 2 bits per call, 4 calls per byte, site s of a
     row in bits 2*(s%4) of byte s//4.  Rows are padded with code 3 (missing)

@@ -61,7 +61,7 @@ def test_gt_errors(tmp_path):
     err = dump(write(tmp_path, "1\t1\t.\tA\tG\t.\tPASS\t.\tGT\t0/1/1\t0/1\t1/1\n"), expect_ok=False)
     assert "not diploid" in err
     err = dump(write(tmp_path, "1\t1\t.\tA\tG,T\t.\tPASS\t.\tGT\t0/2\t0/1\t1/1\n"), expect_ok=False)
-    assert "tri- and tetra-allelic" in err
+    assert "biallelic" in err
     err = dump(write(tmp_path, "1\t1\t.\tA\tG\t.\tPASS\t.\tDP\t1\t2\t3\n"), expect_ok=False)
     assert "No GT data found" in err
 
@@ -180,3 +180,27 @@ def test_last_line_without_newline(tmp_path):
     p = write(tmp_path, "1\t1\t.\tA\tG\t.\tPASS\t.\tGT\t0/0\t0/1\t1/1", "n.vcf")
     _, sites, counts = dump(p)
     assert [s[0] for s in sites] == ["012"] and counts["total"] == 1
+
+
+def test_multiallelic_likelihoods_are_bucketed(tmp_path):
+    """GL / PL records with 3 and 4 alleles keep their 6 / 10 likelihoods in
+    their own buckets (vcf.rs:270-305), after the biallelic rows."""
+    body = ("1\t1\t.\tA\tG\t.\tPASS\t.\tPL\t0,10,20\t5,0,5\t20,10,0\n"
+            "1\t2\t.\tA\tG,T\t.\tPASS\t.\tPL\t0,1,2,3,4,5\t9,8,7,6,5,0\t.,.,.,.,.,.\n"
+            "1\t3\t.\tA\tC,G,T\t.\tPASS\t.\tPL\t" + ",".join(str(i) for i in range(10)) + "\t"
+            + ",".join(str(20 - i) for i in range(10)) + "\t0,0,0,0,0,0,0,0,0,0\n")
+    p = write(tmp_path, body, "m.vcf")
+    out = str(tmp_path / "p.bin")
+    r = subprocess.run([KESTREL, p, "/dev/null", "-f", "PL", "--dump-probs", out], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(out, np.float64)
+    assert got.size == 3 * 3 + 3 * 6 + 3 * 10
+    pl2 = [[0, 10, 20], [5, 0, 5], [20, 10, 0]]
+    pl3 = [[0, 1, 2, 3, 4, 5], [9, 8, 7, 6, 5, 0]]
+    pl4 = [list(range(10)), [20 - i for i in range(10)], [0] * 10]
+    exp = [math.pow(10.0, -v / 10.0) for row in pl2 for v in row]
+    exp += [math.pow(10.0, -v / 10.0) for row in pl3 for v in row] + [math.nan] * 6
+    exp += [math.pow(10.0, -v / 10.0) for row in pl4 for v in row]
+    assert np.array_equal(got, np.array(exp), equal_nan=True)
+    bad = write(tmp_path, "1\t1\t.\tA\tG,T\t.\tPASS\t.\tGL\t0,0,0\t0,0,0,0,0,0\t0,0,0,0,0,0\n", "b.vcf")
+    assert "Not enough genotype likelihoods" in dump(bad, "GL", expect_ok=False)

@@ -117,3 +117,42 @@ def test_cli_soft_likelihoods_match_oracle(O, tmp_path, field):
     assert np.array_equal(jac, exp["jacquard"]) and np.array_equal(it, exp["iters"])
     assert np.array_equal(obj, exp["obj"])
     assert np.array_equal(cond, log10(exp["cond"]), equal_nan=True)
+
+
+@pytest.mark.parametrize("field", ["GL", "PL"])
+def test_cli_multiallelic_matches_oracle(O, tmp_path, field):
+    """2-, 3- and 4-allele records: the CLI buckets, re-buckets and fits them
+    as the oracle does on the rows it parsed."""
+    print(f"case: cli -f {field} multi-allelic", flush=True)
+    b = synth.multi_likelihoods(16, (600, 150, 60), seed=61)
+    vcf = str(tmp_path / f"m.{field}.vcf")
+    synth.write_vcf_multi(vcf, b, field)
+    probs = str(tmp_path / "p.bin")
+    r = subprocess.run([KESTREL, vcf, "/dev/null", "-f", field, "--dump-probs", probs], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(probs, np.float64)
+    shapes = [(600, 16, 3), (150, 16, 6), (60, 16, 10)]
+    parsed, off = [], 0
+    for s in shapes:
+        k = int(np.prod(s))
+        parsed.append(raw[off:off + k].reshape(s))
+        off += k
+    assert off == raw.size
+    rows, log = run(vcf, str(tmp_path / "o.tsv"), "-f", field)
+    assert "Found 150 tri-allelic sites" in log and "Found 60 tetra-allelic sites" in log
+    x, y, kin, jac, it, obj, cond = as_arrays(rows, [f"S{i:05d}" for i in range(16)])
+    parts, _ = O.multi_prepare({2: parsed[0], 3: parsed[1], 4: parsed[2]})
+    pads, seg, base = {}, [0], 0
+    for A in (2, 3, 4):
+        nA = parts[A][0].shape[0]
+        if A > 2 and nA == 0:
+            continue
+        bnd = segment_bounds(nA)
+        pads[A] = int(bnd[-1])
+        seg += [int(v) + base for v in bnd[1:]]
+        base += int(bnd[-1])
+    exp = O.multi_fit_pairs(parts, np.stack([x, y], 1), order=O.ORDER_SEG, pads=pads, seg=np.array(seg, np.uint32))
+    assert np.array_equal(jac, exp["jacquard"]) and np.array_equal(it, exp["iters"])
+    assert np.array_equal(obj, exp["obj"])
+    assert np.array_equal(cond, log10(exp["cond"]), equal_nan=True)
+    assert f"Identified {parts[3][0].shape[0]} tri-allelic sites" in log
