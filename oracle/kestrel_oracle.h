@@ -150,6 +150,15 @@ void ko_cls_m_matrices(const double* af, uint32_t V, double* m);
 void ko_cls_gram(const double* m, uint32_t V, double* q);
 int ko_cls_fit_pairs(const uint8_t* codes, uint32_t N, uint32_t V, const double* af, const uint32_t* pairs,
                      size_t npairs, uint32_t nsel, uint64_t iterations, int nthreads, ko_output* out, double* q_out);
+/* Tri/tetra-allelic form: genotype indices in VCF order per bucket. */
+void ko_gt_allele_counts_generic(int A, const uint8_t* idx, uint32_t N, uint32_t V, double* af);
+void ko_cls_m_generic(int A, const double* af, uint32_t V, double* m);
+void ko_cls_gram_multi(const double* m2, uint32_t V2, const double* m3, uint32_t V3, const double* m4, uint32_t V4,
+                       double* q);
+int ko_cls_fit_pairs_multi(uint32_t N, const uint8_t* idx2, uint32_t V2, const uint8_t* idx3, uint32_t V3,
+                           const uint8_t* idx4, uint32_t V4, const double* af2, const double* af3, const double* af4,
+                           const uint32_t* pairs, size_t npairs, uint32_t nsel, uint64_t iterations, int nthreads,
+                           ko_output* out, double* q_out);
 
 #ifdef __cplusplus
 }

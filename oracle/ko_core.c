@@ -1261,66 +1261,111 @@ int ko_fit_pairs_gl(int order, const double* gl, uint32_t N, uint32_t V, const d
 
 /* ------------------------------------------------- CLS (Brier) path, -f GT */
 
-/* wrapper.rs:412-428: allele frequencies by counting (no missing calls on
- * this path: vcf.rs:217-221 drops such sites). codes N x V. af V x 2. */
-void ko_gt_allele_counts(const uint8_t* codes, uint32_t N, uint32_t V, double* af) {
+/* wrapper.rs:412-428 calculate_allele_frequencies::<A>: frequencies by
+ * counting both alleles of every call (no missing calls on this path:
+ * vcf.rs:217-221 drops such sites).  idx N x V genotype indices in VCF order
+ * (j(j+1)/2 + i, coefficients.rs:19-38); af V x A. */
+void ko_gt_allele_counts_generic(int A, const uint8_t* idx, uint32_t N, uint32_t V, double* af) {
+    int gi[10], gj[10];
+    genotype_alleles(A, gi, gj);
     for (uint32_t s = 0; s < V; s++) {
-        uint64_t f0 = 0, f1 = 0;
+        uint64_t f[4] = {0, 0, 0, 0};
         for (uint32_t i = 0; i < N; i++) {
-            const uint8_t g = codes[(size_t)i * V + s];
-            /* genotype index 0 = (0,0), 1 = (0,1), 2 = (1,1) */
-            f0 += g == 0 ? 2 : g == 1 ? 1 : 0;
-            f1 += g == 2 ? 2 : g == 1 ? 1 : 0;
+            const uint8_t g = idx[(size_t)i * V + s];
+            f[gi[g]]++;
+            f[gj[g]]++;
         }
         const double tot = (double)(2 * (uint64_t)N);
-        af[2 * s] = (double)f0 / tot;
-        af[2 * s + 1] = (double)f1 / tot;
+        for (int a = 0; a < A; a++) af[(size_t)s * A + a] = (double)f[a] / tot;
     }
 }
 
-/* cls.rs:6-26 with conditional.rs IIS_2: m[s][gx][gy][9] */
-void ko_cls_m_matrices(const double* af, uint32_t V, double* m) {
-    static const int GI[3][2] = {{0, 0}, {0, 1}, {1, 1}};
-    for (uint32_t s = 0; s < V; s++)
-        for (int gx = 0; gx < 3; gx++)
-            for (int gy = 0; gy < 3; gy++) {
-                const int i = GI[gx][0], j = GI[gx][1], k = GI[gy][0], l = GI[gy][1];
-                ko_conditional_probabilities(af[2 * s + i], af[2 * s + j], af[2 * s + k], af[2 * s + l],
-                                             ko_iis_mode(i, j, k, l), m + ((size_t)s * 9 + gx * 3 + gy) * 9);
-            }
+void ko_gt_allele_counts(const uint8_t* codes, uint32_t N, uint32_t V, double* af) {
+    ko_gt_allele_counts_generic(2, codes, N, V, af);
 }
 
-/* coefficients.rs:280-320 compute_quadratic_matrix for biallelic sites: all
- * 9 rows of every site pushed in (gx, gy) order into an 8-lane vector, then
- * quadratic.rs:12-33 divided by V. */
-void ko_cls_gram(const double* m, uint32_t V, double* q) {
-    const size_t rows = (size_t)V * 9;
+/* cls.rs:6-26 calculate_m_matrices over IIS_A (conditional.rs:10-50): for
+ * every (gx, gy) in genotype order the alleles (i, j), (k, l) of the two
+ * genotypes, their IBS mode and iis.rs:279-462 conditional_probabilities.
+ * af V x A; m[s][gx][gy][9], G = A(A+1)/2. */
+void ko_cls_m_generic(int A, const double* af, uint32_t V, double* m) {
+    int gi[10], gj[10];
+    genotype_alleles(A, gi, gj);
+    const int G = A * (A + 1) / 2;
+    for (uint32_t s = 0; s < V; s++) {
+        const double* p = af + (size_t)s * A;
+        for (int gx = 0; gx < G; gx++)
+            for (int gy = 0; gy < G; gy++) {
+                const int i = gi[gx], j = gj[gx], k = gi[gy], l = gj[gy];
+                ko_conditional_probabilities(p[i], p[j], p[k], p[l], ko_iis_mode(i, j, k, l),
+                                             m + (((size_t)s * G + gx) * G + gy) * 9);
+            }
+    }
+}
+
+void ko_cls_m_matrices(const double* af, uint32_t V, double* m) { ko_cls_m_generic(2, af, V, m); }
+
+/* coefficients.rs:280-320 compute_quadratic_matrix: every row of every
+ * 2-allele site's m (gx, gy order), then of the 3-allele sites, then of the
+ * 4-allele sites, pushed into one 8-lane vector (lanevector.rs:87-139; rows
+ * holding a NaN are skipped); quadratic.rs:12-33 Gram matrix divided by the
+ * number of sites V = V2 + V3 + V4. */
+void ko_cls_gram_multi(const double* m2, uint32_t V2, const double* m3, uint32_t V3, const double* m4, uint32_t V4,
+                       double* q) {
+    const size_t total = (size_t)V2 * 9 + (size_t)V3 * 36 + (size_t)V4 * 100;
+    double* rows = (double*)malloc((total ? total : 1) * 9 * sizeof(double));
+    size_t n = 0;
+    const double* src[3] = {m2, m3, m4};
+    const size_t cnt[3] = {(size_t)V2 * 9, (size_t)V3 * 36, (size_t)V4 * 100};
+    for (int b = 0; b < 3; b++)
+        for (size_t r = 0; r < cnt[b]; r++) {
+            const double* row = src[b] + r * 9;
+            int nan = 0;
+            for (int c = 0; c < 9; c++) nan |= row[c] != row[c];
+            if (nan) continue;
+            memcpy(rows + n * 9, row, 9 * sizeof(double));
+            n++;
+        }
     ko_lanevec lv;
-    lv.nblocks = rows / 8;
-    lv.nrem = rows % 8;
+    lv.nblocks = n / 8;
+    lv.nrem = n % 8;
     lv.blocks = (double*)malloc((lv.nblocks ? lv.nblocks : 1) * 72 * sizeof(double));
     for (size_t b = 0; b < lv.nblocks; b++)
         for (int j = 0; j < 8; j++)
-            for (int c = 0; c < 9; c++) lv.blocks[b * 72 + c * 8 + j] = m[(b * 8 + j) * 9 + c];
+            for (int c = 0; c < 9; c++) lv.blocks[b * 72 + c * 8 + j] = rows[(b * 8 + j) * 9 + c];
     for (size_t r = 0; r < lv.nrem; r++)
-        for (int c = 0; c < 9; c++) lv.rem[r * 9 + c] = m[(lv.nblocks * 8 + r) * 9 + c];
-    ko_ops()->gram(&lv, (double)V, q);
+        for (int c = 0; c < 9; c++) lv.rem[r * 9 + c] = rows[(lv.nblocks * 8 + r) * 9 + c];
+    ko_ops()->gram(&lv, (double)V2 + (double)V3 + (double)V4, q);
     free(lv.blocks);
+    free(rows);
 }
 
-/* coefficients.rs:216-276 for one pair: c = sum over sites of m[gx][gy]
- * (cls.rs:28-49, sequential), scaled by -1/V; QP from 1/9 (or the 2-variable
- * (delta1, delta7) QP from (1/2, 1/2) for a self pair); brier, gradient and
- * condition.  q is 9 x 9 row-major; qp cap = iterations. */
-void ko_cls_pair(const double* m, const double* q, const uint8_t* gx, const uint8_t* gy, uint32_t V, int self_pair,
-                 uint64_t iterations, ko_output* out) {
-    double c[9] = {0};
+void ko_cls_gram(const double* m, uint32_t V, double* q) { ko_cls_gram_multi(m, V, NULL, 0, NULL, 0, q); }
+
+/* cls.rs:28-49 calculate_quadratic_c: sum of m_s[gx_s][gy_s] over the sites of
+ * one bucket, in site order, from zero. */
+static void cls_bucket_sum(const double* m, int G, const uint8_t* gx, const uint8_t* gy, uint32_t V, double c[9]) {
+    for (int k = 0; k < 9; k++) c[k] = 0.0;
     for (uint32_t s = 0; s < V; s++) {
-        const double* row = m + ((size_t)s * 9 + gx[s] * 3 + gy[s]) * 9;
+        const double* row = m + (((size_t)s * G + gx[s]) * G + gy[s]) * 9;
         for (int k = 0; k < 9; k++) c[k] = c[k] + row[k];
     }
-    const double nv = -(double)V;
-    for (int k = 0; k < 9; k++) c[k] = c[k] / nv;
+}
+
+/* coefficients.rs:216-276 for one pair: c = (c2 + (c3 + c4)) / -V, QP from 1/9
+ * (or the 2-variable (delta1, delta7) QP from (1/2, 1/2) for a self pair);
+ * brier, gradient and condition.  q is 9 x 9 row-major; qp cap = iterations.
+ * m[b], x[b], y[b], V[b]: the m-matrices and the two genotype-index rows of
+ * the 2-, 3- and 4-allele sites. */
+static void cls_pair_multi(const double* const m[3], const double* q, const uint8_t* const x[3],
+                           const uint8_t* const y[3], const uint32_t V[3], int self_pair, uint64_t iterations,
+                           ko_output* out) {
+    static const int G[3] = {3, 6, 10};
+    double cb[3][9], c[9];
+    for (int b = 0; b < 3; b++) cls_bucket_sum(m[b], G[b], x[b], y[b], V[b], cb[b]);
+    const uint32_t vt = V[0] + V[1] + V[2];
+    const double nv = -(double)vt;
+    for (int k = 0; k < 9; k++) c[k] = (cb[0][k] + (cb[1][k] + cb[2][k])) / nv;
     ko_tune t;
     ko_tune_new(&t, iterations, iterations);
     double Q[MX * MX];
@@ -1333,11 +1378,11 @@ void ko_cls_pair(const double* m, const double* q, const uint8_t* gx, const uint
         q2[1] = Q[6];
         q2[MX] = Q[6 * MX];
         q2[MX + 1] = Q[6 * MX + 6];
-        double c2[2] = {c[0], c[6]}, y0[2] = {0.5, 0.5}, y[2];
-        iters = qp(2, q2, c2, y0, 1, &t, y);
+        double c2[2] = {c[0], c[6]}, y0[2] = {0.5, 0.5}, yv[2];
+        iters = qp(2, q2, c2, y0, 1, &t, yv);
         for (int k = 0; k < 9; k++) delta[k] = 0.0;
-        delta[0] = y[0];
-        delta[6] = y[1];
+        delta[0] = yv[0];
+        delta[6] = yv[1];
     } else {
         double y0[9];
         for (int k = 0; k < 9; k++) y0[k] = 1.0 / 9.0;
@@ -1351,13 +1396,22 @@ void ko_cls_pair(const double* m, const double* q, const uint8_t* gx, const uint
     memcpy(out->jacquard, delta, sizeof delta);
     out->obj = brier;
     out->iters = iters;
-    out->n_sites = V;
+    out->n_sites = vt;
+}
+
+void ko_cls_pair(const double* m, const double* q, const uint8_t* gx, const uint8_t* gy, uint32_t V, int self_pair,
+                 uint64_t iterations, ko_output* out) {
+    const double* mm[3] = {m, NULL, NULL};
+    const uint8_t* xx[3] = {gx, NULL, NULL};
+    const uint8_t* yy[3] = {gy, NULL, NULL};
+    const uint32_t vv[3] = {V, 0, 0};
+    cls_pair_multi(mm, q, xx, yy, vv, self_pair, iterations, out);
 }
 
 typedef struct {
-    const uint8_t* codes;
-    uint32_t N, V;
-    const double* m;
+    const uint8_t* idx[3];
+    uint32_t N, V[3];
+    const double* m[3];
     const double* q;
     const uint32_t* pairs;
     size_t npairs;
@@ -1382,31 +1436,53 @@ static void* cls_worker(void* vj) {
         memset(o, 0, sizeof *o);
         o->x = x;
         o->y = y;
-        ko_cls_pair(j->m, j->q, j->codes + (size_t)x * j->V, j->codes + (size_t)y * j->V, j->V, x == y,
-                    j->iterations, o);
+        const uint8_t* rx[3];
+        const uint8_t* ry[3];
+        for (int b = 0; b < 3; b++) {
+            rx[b] = j->idx[b] ? j->idx[b] + (size_t)x * j->V[b] : NULL;
+            ry[b] = j->idx[b] ? j->idx[b] + (size_t)y * j->V[b] : NULL;
+        }
+        cls_pair_multi(j->m, j->q, rx, ry, j->V, x == y, j->iterations, o);
     }
     return NULL;
 }
 
-/* coefficients.rs:166-278 calculate_coefficients_gt for biallelic codes
- * (no missing calls). af V x 2 (NULL = counted). */
-int ko_cls_fit_pairs(const uint8_t* codes, uint32_t N, uint32_t V, const double* af_in, const uint32_t* pairs,
-                     size_t npairs, uint32_t nsel, uint64_t iterations, int nthreads, ko_output* out, double* q_out) {
+/* coefficients.rs:166-278 calculate_coefficients_gt with wrapper.rs:22-87:
+ * idx2 / idx3 / idx4 are N x V2 / V3 / V4 genotype indices (VCF order, no
+ * missing calls) of the 2-, 3- and 4-allele sites; af2 / af3 / af4 are V_A x A
+ * frequencies (all NULL = counted per bucket). */
+int ko_cls_fit_pairs_multi(uint32_t N, const uint8_t* idx2, uint32_t V2, const uint8_t* idx3, uint32_t V3,
+                           const uint8_t* idx4, uint32_t V4, const double* af2, const double* af3, const double* af4,
+                           const uint32_t* pairs, size_t npairs, uint32_t nsel, uint64_t iterations, int nthreads,
+                           ko_output* out, double* q_out) {
     if (nsel > N) return -1;
-    double* af = (double*)malloc(sizeof(double) * 2 * (V ? V : 1));
-    if (af_in) memcpy(af, af_in, sizeof(double) * 2 * V);
-    else ko_gt_allele_counts(codes, N, V, af);
-    double* m = (double*)malloc(sizeof(double) * 81 * (V ? V : 1));
-    ko_cls_m_matrices(af, V, m);
+    const uint8_t* idx[3] = {idx2, idx3, idx4};
+    const uint32_t V[3] = {V2, V3, V4};
+    const double* af_in[3] = {af2, af3, af4};
+    const int given = af2 || af3 || af4;
+    double* af[3];
+    double* m[3];
+    for (int b = 0; b < 3; b++) {
+        const int A = b + 2, G = A * (A + 1) / 2;
+        af[b] = (double*)malloc(sizeof(double) * A * (V[b] ? V[b] : 1));
+        if (given) {
+            if (V[b]) memcpy(af[b], af_in[b], sizeof(double) * A * V[b]);
+        } else
+            ko_gt_allele_counts_generic(A, idx[b], N, V[b], af[b]);
+        m[b] = (double*)malloc(sizeof(double) * 9 * G * G * (V[b] ? V[b] : 1));
+        ko_cls_m_generic(A, af[b], V[b], m[b]);
+    }
     double q[81];
-    ko_cls_gram(m, V, q);
+    ko_cls_gram_multi(m[0], V2, m[1], V3, m[2], V4, q);
     if (q_out) memcpy(q_out, q, sizeof q);
     cls_job j;
     memset(&j, 0, sizeof j);
-    j.codes = codes;
+    for (int b = 0; b < 3; b++) {
+        j.idx[b] = idx[b];
+        j.V[b] = V[b];
+        j.m[b] = m[b];
+    }
     j.N = N;
-    j.V = V;
-    j.m = m;
     j.q = q;
     j.pairs = pairs;
     j.nsel = nsel;
@@ -1419,7 +1495,17 @@ int ko_cls_fit_pairs(const uint8_t* codes, uint32_t N, uint32_t V, const double*
     for (int i = 0; i < nt; i++) pthread_create(&th[i], NULL, cls_worker, &j);
     for (int i = 0; i < nt; i++) pthread_join(th[i], NULL);
     free(th);
-    free(m);
-    free(af);
+    for (int b = 0; b < 3; b++) {
+        free(m[b]);
+        free(af[b]);
+    }
     return 0;
+}
+
+/* The biallelic form: codes N x V (0, 1, 2 = genotype index), af V x 2 or
+ * NULL (counted). */
+int ko_cls_fit_pairs(const uint8_t* codes, uint32_t N, uint32_t V, const double* af_in, const uint32_t* pairs,
+                     size_t npairs, uint32_t nsel, uint64_t iterations, int nthreads, ko_output* out, double* q_out) {
+    return ko_cls_fit_pairs_multi(N, codes, V, NULL, 0, NULL, 0, af_in, NULL, NULL, pairs, npairs, nsel, iterations,
+                                  nthreads, out, q_out);
 }

@@ -503,3 +503,66 @@ def cls_fit_pairs(codes, af=None, pairs=None, nsel=None, iterations=100, nthread
     if rc != 0:
         raise ValueError("requested samples exceed total samples")
     return out, q
+
+
+def gt_allele_counts_generic(A, idx):
+    """wrapper.rs:412-428 calculate_allele_frequencies::<A> over genotype
+    indices (N x V, VCF order).  Returns V x A."""
+    idx = np.ascontiguousarray(idx, np.uint8)
+    n, v = idx.shape
+    af = np.zeros((v, A))
+    L = lib()
+    L.ko_gt_allele_counts_generic.argtypes = [I, P, U32, U32, P]
+    L.ko_gt_allele_counts_generic(A, _p(idx), n, v, _p(af))
+    return af
+
+
+def cls_m_generic(A, af):
+    """cls.rs:6-26 m-matrices over IIS_A: V x G x G x 9."""
+    af = f64(af).reshape(-1, A)
+    v = af.shape[0]
+    G = A * (A + 1) // 2
+    m = np.zeros((v, G, G, 9))
+    L = lib()
+    L.ko_cls_m_generic.argtypes = [I, P, U32, P]
+    L.ko_cls_m_generic(A, _p(af), v, _p(m))
+    return m
+
+
+def cls_gram_multi(m2, m3, m4):
+    """coefficients.rs:280-320: Gram matrix of every m row (2-, then 3-, then
+    4-allele sites) in the 8-lane order, divided by the number of sites."""
+    ms = [f64(m) for m in (m2, m3, m4)]
+    q = np.zeros((9, 9))
+    L = lib()
+    L.ko_cls_gram_multi.argtypes = [P, U32, P, U32, P, U32, P]
+    L.ko_cls_gram_multi(_p(ms[0]), ms[0].shape[0], _p(ms[1]), ms[1].shape[0], _p(ms[2]), ms[2].shape[0], _p(q))
+    return q
+
+
+def cls_fit_pairs_multi(idx2, idx3, idx4, af=None, pairs=None, nsel=None, iterations=100, nthreads=0):
+    """coefficients.rs:166-278 with 2-, 3- and 4-allele sites: idxA are N x V_A
+    genotype indices (VCF order, no missing calls); af = (af2, af3, af4) or
+    None (counted per bucket).  Returns (outputs, Q)."""
+    idx = [np.ascontiguousarray(a, np.uint8) for a in (idx2, idx3, idx4)]
+    n = idx[0].shape[0]
+    assert all(a.shape[0] == n for a in idx)
+    if nsel is None:
+        nsel = n
+    if pairs is not None:
+        pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        npairs = pairs.shape[0]
+    else:
+        npairs = nsel * (nsel + 1) // 2
+    afs = [None, None, None] if af is None else [f64(a) for a in af]
+    out = np.zeros(npairs, OUTPUT_DTYPE)
+    q = np.zeros((9, 9))
+    L = lib()
+    L.ko_cls_fit_pairs_multi.restype = I
+    L.ko_cls_fit_pairs_multi.argtypes = [U32, P, U32, P, U32, P, U32, P, P, P, P, SZ, U32, U64, I, P, P]
+    rc = L.ko_cls_fit_pairs_multi(n, _p(idx[0]), idx[0].shape[1], _p(idx[1]), idx[1].shape[1], _p(idx[2]),
+                                  idx[2].shape[1], _p(afs[0]), _p(afs[1]), _p(afs[2]), _p(pairs), npairs, nsel,
+                                  iterations, nthreads, _p(out), _p(q))
+    if rc != 0:
+        raise ValueError("requested samples exceed total samples")
+    return out, q
