@@ -121,7 +121,9 @@ int kestrel_cohort_create_gl_multi(kestrel_cohort** out, uint32_t n_samples, con
                                    uint64_t af_iterations);
 /* Of a kestrel_cohort_create_gl_multi cohort: part_sites = sites with 2, 3, 4
  * alleles after zero-frequency removal; afA (nA x A, may be NULL) = the
- * frequencies fitted for the input buckets. */
+ * frequencies fitted for the input buckets.  Of a
+ * kestrel_cohort_create_gt_multi cohort: part_sites = kept sites of each
+ * bucket; afA = the frequencies of the input sites (NaN for a dropped one). */
 int kestrel_cohort_multi_info(const kestrel_cohort* c, uint32_t part_sites[3], double* af2, double* af3,
                               double* af4);
 
@@ -158,6 +160,23 @@ int kestrel_cohort_create_gt(kestrel_cohort** out, const uint8_t* packed, uint32
                              size_t row_stride_bytes, const double* af);
 int kestrel_fit_pairs_gt(kestrel_cohort* c, uint64_t iterations, const uint32_t* pairs, size_t n_pairs,
                          uint32_t n_select, kestrel_output* out, double* q_out);
+
+/* kestrel_cohort_create_gt with sites of 3 and 4 alleles: replaces
+ * vcf.rs:95-191 parse_vcf_gt's three buckets and wrapper.rs:22-87 up to the
+ * pair loop.  packed2 / n2 / row_stride_bytes: the 2-allele sites as for
+ * kestrel_cohort_create_gt; gt3 (n_samples x n3) and gt4 (n_samples x n4):
+ * genotype indices of the 3- and 4-allele sites, sample-major, VCF order
+ * (j(j+1)/2 + i for alleles i <= j, coefficients.rs:19-38), 255 = a missing
+ * call (the site is dropped, vcf.rs:217-221); any bucket may be empty.
+ * af2 / af3 / af4 (n_A x A, reference allele first; -a info) are all given or
+ * all NULL (counted per bucket, wrapper.rs:36-40).  kestrel_fit_pairs_gt then
+ * fits the reference's calculate_coefficients_gt: Q from the m rows of the
+ * 2-, 3- and 4-allele sites in that order, c = (c2 + (c3 + c4)) / -V over all
+ * V kept sites.  kestrel_cohort_multi_info reports the kept sites of each
+ * bucket and its frequencies (input sites, NaN for a dropped one). */
+int kestrel_cohort_create_gt_multi(kestrel_cohort** out, uint32_t n_samples, const uint8_t* packed2, uint32_t n2,
+                                   size_t row_stride_bytes, const uint8_t* gt3, uint32_t n3, const uint8_t* gt4,
+                                   uint32_t n4, const double* af2, const double* af3, const double* af4);
 
 /* Tuning knobs: "segment_sites" (sites per segment of a sweep, a multiple of
  * 32, default 4096; fixes the summation order, so it is a cohort property),

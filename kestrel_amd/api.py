@@ -68,6 +68,50 @@ class Cohort:
         return self
 
     @classmethod
+    def gt_multi(cls, gt2: np.ndarray | None = None, gt3: np.ndarray | None = None, gt4: np.ndarray | None = None,
+                 *, af=None) -> "Cohort":
+        """Cohort for the CLS / Brier estimator of -f GT with sites of 2, 3 and 4
+        alleles (kestrel_cohort_create_gt_multi): gtA is (N, V_A) genotype
+        indices in VCF order (j(j+1)/2 + i), 255 = a missing call (the site is
+        dropped); af = (af2, af3, af4) with afA (V_A, A), or None (counted)."""
+        L = _lib.load()
+        self = cls.__new__(cls)
+        arrs, counts, n = [], [], None
+        for A, g in ((2, gt2), (3, gt3), (4, gt4)):
+            if g is None:
+                arrs.append(None)
+                counts.append(0)
+                continue
+            g = np.ascontiguousarray(g, np.uint8)
+            if g.ndim != 2:
+                raise ValueError(f"gt{A} must have shape (samples, sites)")
+            if n is not None and g.shape[0] != n:
+                raise ValueError("buckets must have the same samples")
+            n = g.shape[0]
+            arrs.append(g)
+            counts.append(g.shape[1])
+        if n is None:
+            raise ValueError("no genotypes given")
+        packed = None
+        if arrs[0] is not None and counts[0]:
+            g2 = arrs[0]
+            if ((g2 > 2) & (g2 != 255)).any():
+                raise ValueError("gt2 holds genotype indices 0, 1, 2 or 255")
+            packed = pack2(np.where(g2 == 255, 3, g2).astype(np.uint8))
+        afs = [None, None, None]
+        if af is not None:
+            afs = [None if v == 0 or a is None else np.ascontiguousarray(a, np.float64).reshape(v, A)
+                   for A, v, a in zip((2, 3, 4), counts, af)]
+        self.n_samples, self.n_sites = int(n), int(sum(counts))
+        self._bucket_sites = counts
+        h = ctypes.c_void_p()
+        check(L.kestrel_cohort_create_gt_multi(ctypes.byref(h), self.n_samples, ptr(packed), counts[0],
+                                               0 if packed is None else packed.shape[1], ptr(arrs[1]), counts[1],
+                                               ptr(arrs[2]), counts[2], ptr(afs[0]), ptr(afs[1]), ptr(afs[2])))
+        self._h = h
+        return self
+
+    @classmethod
     def gl(cls, gl: np.ndarray, *, af: np.ndarray | None = None, maf: float | None = None,
            af_iterations: int = 100, segment_sites: int | None = None) -> "Cohort":
         """Cohort of soft genotype likelihoods (kestrel_cohort_create_gl): gl is
@@ -126,7 +170,9 @@ class Cohort:
 
     def multi_info(self):
         """(sites with 2, 3, 4 alleles after zero-frequency removal, the
-        fitted frequencies of the input buckets {A: (V_A, A)})."""
+        fitted frequencies of the input buckets {A: (V_A, A)}); for a gt_multi
+        cohort the kept sites per bucket and the frequencies of the input
+        sites (NaN rows for dropped sites)."""
         parts = np.zeros(3, np.uint32)
         afs = [np.zeros((v, A)) for A, v in zip((2, 3, 4), self._bucket_sites)]
         check(_lib.load().kestrel_cohort_multi_info(self._h, ptr(parts), ptr(afs[0]), ptr(afs[1]), ptr(afs[2])))
@@ -323,35 +369,53 @@ class Coefficients:
         self.inbreeding = inbreeding
 
 
-def _gt_codes(genotypes) -> np.ndarray:
+def _gt_indices(genotypes) -> np.ndarray:
     """(V, N, 2) allele indices -> (N, V) unphased genotype index
-    (coefficients.rs:18-38); only biallelic sites are supported."""
+    (coefficients.rs:18-38)."""
     g = np.asarray(genotypes)
     if g.ndim != 3 or g.shape[2] != 2:
         raise ValueError("genotypes must have shape (sites, samples, 2)")
-    g = g.astype(np.uint8)
+    g = g.astype(np.int64)
     lo, hi = np.minimum(g[..., 0], g[..., 1]), np.maximum(g[..., 0], g[..., 1])
-    if hi.size and hi.max() > 1:
-        raise ValueError("tri- and tetra-allelic sites are not supported by this build")
-    return np.ascontiguousarray((hi * (hi + 1) // 2 + lo).T)
+    if hi.size and hi.max() > 3:
+        raise ValueError("alleles beyond the fourth are not supported (vcf.rs:126-160 buckets 2, 3, 4 alleles)")
+    return np.ascontiguousarray((hi * (hi + 1) // 2 + lo).T.astype(np.uint8))
 
 
-def relatedness_coefficients_gt(genotypes, allele_frequencies=None, threads=None, iterations: int = 100):
+def relatedness_coefficients_gt(genotypes, allele_frequencies=None, threads=None, iterations: int = 100,
+                                alleles=None):
     """lib.rs:46-73 (a stub in the reference): the CLS / Brier estimator of
     wrapper.rs:22-87 over every pair of samples.  genotypes: (V, N, 2) allele
-    indices; allele_frequencies: (V, 2) or None (counted).  threads is
+    indices; alleles: (V,) allele count of each site (2, 3 or 4; default: one
+    more than the largest allele index seen, at least 2), which puts the site in
+    the reference's 2-, 3- or 4-allele bucket; allele_frequencies: (V, A_max)
+    with the first A columns of a site used, or None (counted).  threads is
     accepted for the reference's signature and unused (the fit runs on the
     GPU)."""
     if iterations < 1:
         raise ValueError("iterations must be non-zero")
-    codes = _gt_codes(genotypes)
-    af = None if allele_frequencies is None else np.asarray(allele_frequencies, np.float64)
-    co = Cohort.gt(codes, af=af)
+    idx = _gt_indices(genotypes)
+    g = np.asarray(genotypes)
+    n_v = idx.shape[1]
+    if alleles is None:
+        alleles = np.maximum(2, g.max(axis=(1, 2)).astype(np.int64) + 1) if n_v else np.zeros(0, np.int64)
+    alleles = np.asarray(alleles, np.int64).reshape(n_v)
+    if n_v and (alleles.min() < 2 or alleles.max() > 4):
+        raise ValueError("sites must have 2, 3 or 4 alleles")
+    if n_v and (g.max(axis=(1, 2)) >= alleles).any():
+        raise ValueError("an allele index exceeds its site's allele count")
+    af = None if allele_frequencies is None else np.asarray(allele_frequencies, np.float64).reshape(n_v, -1)
+    if (alleles == 2).all():
+        co = Cohort.gt(idx, af=None if af is None else af[:, :2])
+    else:
+        sel = [np.flatnonzero(alleles == A) for A in (2, 3, 4)]
+        co = Cohort.gt_multi(*(idx[:, s] for s in sel),
+                             af=None if af is None else [af[s, :A] for A, s in zip((2, 3, 4), sel)])
     try:
         out, _ = co.fit_pairs_gt(iterations=iterations)
     finally:
         co.close()
-    return Coefficients(*build_arrays(out, codes.shape[0]))
+    return Coefficients(*build_arrays(out, idx.shape[0]))
 
 
 def relatedness_coefficients_gl(genotype_likelihoods, allele_frequencies=None, threads=None,

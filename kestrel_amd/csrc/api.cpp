@@ -388,7 +388,7 @@ int kestrel_cohort_create_gl_multi(kestrel_cohort** out, uint32_t n_samples, con
 int kestrel_cohort_multi_info(const kestrel_cohort* c, uint32_t part_sites[3], double* af2, double* af3,
                               double* af4) {
     if (!c) return fail(KESTREL_EINVAL, "null cohort");
-    if (!c->multi) return fail(KESTREL_EINVAL, "cohort was not created by kestrel_cohort_create_gl_multi");
+    if (!c->multi) return fail(KESTREL_EINVAL, "cohort was not created by kestrel_cohort_create_gl_multi or _gt_multi");
     if (part_sites)
         for (int q = 0; q < 3; q++) part_sites[q] = c->part_sites[q];
     double* dst[3] = {af2, af3, af4};
@@ -658,6 +658,8 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
 int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* points, size_t n, double eps,
                      double* f, double* g, double* h, uint32_t* n_sites) {
     if (!c || !pairs || !points || !f || !g || !h || !n_sites) return fail(KESTREL_EINVAL, "null argument");
+    if (c->gt_parts[0] || c->gt_parts[1])
+        return fail(KESTREL_EINVAL, "a -f GT cohort with 3- or 4-allele sites has no likelihood evaluation");
     if (n == 0) return KESTREL_OK;
     for (size_t i = 0; i < 2 * n; i++)
         if (pairs[i] >= c->n_samples) return fail(KESTREL_EINVAL, "pair names sample %u of %u", pairs[i], c->n_samples);
@@ -774,34 +776,29 @@ int kestrel_selftest(uint64_t n, uint64_t seed, uint64_t* mismatches) {
 
 }  // extern "C"
 
-extern "C" {
+namespace {
 
-int kestrel_cohort_create_gt(kestrel_cohort** out, const uint8_t* packed, uint32_t n_samples, uint32_t n_sites,
-                             size_t row_stride_bytes, const double* af) {
-    if (!out || !packed) return fail(KESTREL_EINVAL, "null argument");
-    *out = nullptr;
-    if (n_samples == 0 || n_sites == 0) return fail(KESTREL_EINVAL, "need at least one sample and one site");
-    if (row_stride_bytes < (n_sites + 3) / 4) return fail(KESTREL_EINVAL, "row stride too small");
-    auto c = new kestrel_cohort();
-    std::unique_ptr<kestrel_cohort> guard(c);
-    int rc = open_device(c);
-    if (rc) return rc;
-    c->gt = true;
-    c->n_samples = n_samples;
+// The 2-allele sites of a -f GT cohort (wrapper.rs:22-87): sites with a
+// missing call in any sample are dropped (vcf.rs:217-221), frequencies are
+// counted (wrapper.rs:412-428) unless given (n_sites x 2).
+int init_gt2(kestrel_cohort* c, const uint8_t* packed, uint32_t n_sites, size_t row_stride_bytes, const double* af) {
+    const uint32_t n_samples = c->n_samples;
     c->n_sites = n_sites;
     DevBuf<uint8_t> dpacked, dflag;
     HIPCHK(dpacked.alloc((size_t)n_samples * row_stride_bytes));
     HIPCHK(dflag.alloc(n_sites));
-    HIPCHK(hipMemcpyAsync(dpacked.p, packed, (size_t)n_samples * row_stride_bytes, hipMemcpyHostToDevice,
-                          c->stream));
+    if (n_sites)
+        HIPCHK(hipMemcpyAsync(dpacked.p, packed, (size_t)n_samples * row_stride_bytes, hipMemcpyHostToDevice,
+                              c->stream));
     HIPCHK(klaunch::complete_sites(dpacked.p, row_stride_bytes, n_samples, n_sites, dflag.p, c->stream));
     c->kept_mask.resize(n_sites);
-    HIPCHK(hipMemcpyAsync(c->kept_mask.data(), dflag.p, n_sites, hipMemcpyDeviceToHost, c->stream));
+    if (n_sites)
+        HIPCHK(hipMemcpyAsync(c->kept_mask.data(), dflag.p, n_sites, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     std::vector<uint32_t> kept;
     for (uint32_t s = 0; s < n_sites; s++)
         if (c->kept_mask[s]) kept.push_back(s);
-    rc = repack_kept(c, dpacked.p, row_stride_bytes, kept);
+    int rc = repack_kept(c, dpacked.p, row_stride_bytes, kept);
     if (rc) return rc;
     HIPCHK(c->af.alloc((size_t)c->v_pad * 2));
     HIPCHK(hipMemsetAsync(c->af.p, 0, (size_t)c->v_pad * 2 * sizeof(double), c->stream));
@@ -823,7 +820,123 @@ int kestrel_cohort_create_gt(kestrel_cohort** out, const uint8_t* packed, uint32
         c->af_all[2 * kept[t]] = afk[2 * t];
         c->af_all[2 * kept[t] + 1] = afk[2 * t + 1];
     }
-    rc = c->set_segments();
+    return c->set_segments();
+}
+
+// The 3- or 4-allele sites of a -f GT cohort: idx n_samples x n_in genotype
+// indices (VCF order, 255 = missing call); sites with a missing call are
+// dropped, the rest packed into 4-bit rows; frequencies counted on the device
+// unless given (n_in x A); then the sites' m tables.  af_all receives the
+// frequencies of every input site (NaN for a dropped one).
+int init_gt_part(kestrel_cohort* c, int A, const uint8_t* idx, uint32_t n_in, const double* af,
+                 std::vector<double>& af_all, uint32_t* n_kept) {
+    const uint32_t n = c->n_samples, G = (uint32_t)(A * (A + 1) / 2);
+    std::vector<uint32_t> kept;
+    for (uint32_t s = 0; s < n_in; s++) {
+        bool ok = true;
+        for (uint32_t i = 0; i < n; i++) {
+            const uint8_t g = idx[(size_t)i * n_in + s];
+            if (g == 255) ok = false;
+            else if (g >= G)
+                return fail(KESTREL_EINVAL, "genotype index %u of sample %u at %u-allele site %u exceeds %u", g, i, A,
+                            s, G - 1);
+        }
+        if (ok) kept.push_back(s);
+    }
+    *n_kept = (uint32_t)kept.size();
+    af_all.assign((size_t)n_in * A, std::nan(""));
+    if (kept.empty()) return KESTREL_OK;
+    auto part = std::make_unique<GtPart>();
+    part->alleles = A;
+    part->n_sites = (uint32_t)kept.size();
+    part->row_words = (part->n_sites + 7) / 8;
+    std::vector<uint32_t> rows((size_t)n * part->row_words, 0xffffffffu);
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t* row = rows.data() + (size_t)i * part->row_words;
+        for (uint32_t t = 0; t < part->n_sites; t++) {
+            const uint32_t g = idx[(size_t)i * n_in + kept[t]], sh = 4 * (t % 8);
+            row[t / 8] = (row[t / 8] & ~(15u << sh)) | (g << sh);
+        }
+    }
+    HIPCHK(part->geno.alloc(rows.size()));
+    HIPCHK(hipMemcpyAsync(part->geno.p, rows.data(), rows.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(part->af.alloc((size_t)part->n_sites * A));
+    std::vector<double> afk((size_t)part->n_sites * A);
+    if (af) {
+        for (uint32_t t = 0; t < part->n_sites; t++)
+            for (int a = 0; a < A; a++) afk[(size_t)t * A + a] = af[(size_t)kept[t] * A + a];
+        HIPCHK(hipMemcpyAsync(part->af.p, afk.data(), afk.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    } else {
+        HIPCHK(klaunch::gt_count_multi(A, part->geno.p, part->row_words, n, part->n_sites, part->af.p, c->stream));
+    }
+    HIPCHK(part->mtab.alloc((size_t)part->n_sites * G * G * 9));
+    HIPCHK(klaunch::cls_m(A, part->af.p, part->n_sites, part->mtab.p, c->stream));
+    HIPCHK(hipMemcpyAsync(afk.data(), part->af.p, afk.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (uint32_t t = 0; t < part->n_sites; t++)
+        for (int a = 0; a < A; a++) af_all[(size_t)kept[t] * A + a] = afk[(size_t)t * A + a];
+    c->gt_parts[A - 3] = std::move(part);
+    return KESTREL_OK;
+}
+
+bool all_finite(const double* v, size_t n) {
+    for (size_t i = 0; i < n; i++)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kestrel_cohort_create_gt(kestrel_cohort** out, const uint8_t* packed, uint32_t n_samples, uint32_t n_sites,
+                             size_t row_stride_bytes, const double* af) {
+    if (!out || !packed) return fail(KESTREL_EINVAL, "null argument");
+    *out = nullptr;
+    if (n_samples == 0 || n_sites == 0) return fail(KESTREL_EINVAL, "need at least one sample and one site");
+    if (row_stride_bytes < (n_sites + 3) / 4) return fail(KESTREL_EINVAL, "row stride too small");
+    if (af && !all_finite(af, (size_t)n_sites * 2)) return fail(KESTREL_EINVAL, "allele frequencies must be finite");
+    auto c = new kestrel_cohort();
+    std::unique_ptr<kestrel_cohort> guard(c);
+    int rc = open_device(c);
+    if (rc) return rc;
+    c->gt = true;
+    c->n_samples = n_samples;
+    rc = init_gt2(c, packed, n_sites, row_stride_bytes, af);
+    if (rc) return rc;
+    *out = guard.release();
+    return KESTREL_OK;
+}
+
+int kestrel_cohort_create_gt_multi(kestrel_cohort** out, uint32_t n_samples, const uint8_t* packed2, uint32_t n2,
+                                   size_t row_stride_bytes, const uint8_t* gt3, uint32_t n3, const uint8_t* gt4,
+                                   uint32_t n4, const double* af2, const double* af3, const double* af4) {
+    if (!out) return fail(KESTREL_EINVAL, "null argument");
+    *out = nullptr;
+    if (n_samples == 0 || (uint64_t)n2 + n3 + n4 == 0) return fail(KESTREL_EINVAL, "need at least one sample and one site");
+    if ((n2 && !packed2) || (n3 && !gt3) || (n4 && !gt4)) return fail(KESTREL_EINVAL, "null genotype bucket");
+    if (n2 && row_stride_bytes < (n2 + 3) / 4) return fail(KESTREL_EINVAL, "row stride too small");
+    const bool given = af2 || af3 || af4;
+    if (given && ((n2 && !af2) || (n3 && !af3) || (n4 && !af4)))
+        return fail(KESTREL_EINVAL, "allele frequencies must be given for every bucket or none");
+    if (given && ((af2 && !all_finite(af2, (size_t)n2 * 2)) || (af3 && !all_finite(af3, (size_t)n3 * 3)) ||
+                  (af4 && !all_finite(af4, (size_t)n4 * 4))))
+        return fail(KESTREL_EINVAL, "allele frequencies must be finite");
+    auto c = new kestrel_cohort();
+    std::unique_ptr<kestrel_cohort> guard(c);
+    int rc = open_device(c);
+    if (rc) return rc;
+    c->gt = true;
+    c->multi = true;
+    c->n_samples = n_samples;
+    rc = init_gt2(c, packed2, n2, n2 ? row_stride_bytes : 0, af2);
+    if (rc) return rc;
+    c->part_sites[0] = c->n_kept;
+    c->af_in[0] = c->af_all;
+    rc = init_gt_part(c, 3, gt3, n3, af3, c->af_in[1], &c->part_sites[1]);
+    if (rc) return rc;
+    rc = init_gt_part(c, 4, gt4, n4, af4, c->af_in[2], &c->part_sites[2]);
     if (rc) return rc;
     *out = guard.release();
     return KESTREL_OK;
@@ -834,7 +947,11 @@ int kestrel_fit_pairs_gt(kestrel_cohort* c, uint64_t iterations, const uint32_t*
     if (!c || !out) return fail(KESTREL_EINVAL, "null argument");
     if (!c->gt) return fail(KESTREL_EINVAL, "cohort was not created by kestrel_cohort_create_gt");
     if (iterations == 0) return fail(KESTREL_EINVAL, "iterations must be non-zero");
-    if (c->n_kept == 0) return fail(KESTREL_EINVAL, "Must have at least one site");
+    const GtPart* p3 = c->gt_parts[0].get();
+    const GtPart* p4 = c->gt_parts[1].get();
+    const uint32_t n3 = p3 ? p3->n_sites : 0, n4 = p4 ? p4->n_sites : 0;
+    const uint32_t v_total = c->n_kept + n3 + n4;
+    if (v_total == 0) return fail(KESTREL_EINVAL, "Must have at least one site");
     std::vector<uint32_t> tri;
     if (!pairs) {
         if (n_select > c->n_samples)
@@ -854,11 +971,11 @@ int kestrel_fit_pairs_gt(kestrel_cohort* c, uint64_t iterations, const uint32_t*
     }
     hipStream_t st = c->stream;
     if (!c->have_q) {
-        const uint64_t rows = (uint64_t)c->n_kept * 9, n_tiles = (rows / 8 + 31) / 32;
         DevBuf<double> z;
-        HIPCHK(z.alloc(std::max<uint64_t>(1, n_tiles) * 45 * 8));
+        HIPCHK(z.alloc(std::max<uint64_t>(1, klaunch::gram_tiles(c->n_kept, n3, n4)) * 45 * 8));
         HIPCHK(c->qmat.alloc(81));
-        HIPCHK(klaunch::gram(c->af.p, c->n_kept, z.p, c->qmat.p, st));
+        HIPCHK(klaunch::gram(c->af.p, c->n_kept, p3 ? p3->mtab.p : nullptr, n3, p4 ? p4->mtab.p : nullptr, n4, z.p,
+                             c->qmat.p, st));
         HIPCHK(hipStreamSynchronize(st));
         c->have_q = true;
     }
@@ -870,11 +987,20 @@ int kestrel_fit_pairs_gt(kestrel_cohort* c, uint64_t iterations, const uint32_t*
     t.qp_max_iter = iterations;  // coefficients.rs:207: Tuneables::new(iterations, iterations)
     DevBuf<uint32_t> dpairs;
     DevBuf<kestrel_output> dout;
+    DevBuf<double> c34;
     HIPCHK(dpairs.alloc(2 * n_pairs));
     HIPCHK(dout.alloc(n_pairs));
     HIPCHK(hipMemcpyAsync(dpairs.p, pairs, 2 * n_pairs * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-    HIPCHK(klaunch::cls_pairs(c->geno.p, c->row_words, c->af.p, c->n_kept, c->qmat.p, dpairs.p, n_pairs, to_dev(t),
-                              dout.p, st));
+    if (p3 || p4) {
+        HIPCHK(c34.alloc(n_pairs * 9));
+        if (p3)
+            HIPCHK(klaunch::cls_multi(3, p3->geno.p, p3->row_words, p3->mtab.p, n3, dpairs.p, n_pairs, 0, c34.p, st));
+        if (p4)
+            HIPCHK(klaunch::cls_multi(4, p4->geno.p, p4->row_words, p4->mtab.p, n4, dpairs.p, n_pairs, p3 ? 1 : 0,
+                                      c34.p, st));
+    }
+    HIPCHK(klaunch::cls_pairs(c->geno.p, c->row_words, c->af.p, c->n_kept, c34.p, v_total, c->qmat.p, dpairs.p,
+                              n_pairs, to_dev(t), dout.p, st));
     HIPCHK(hipMemcpyAsync(out, dout.p, n_pairs * sizeof(kestrel_output), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     c->last_wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -274,6 +274,21 @@ int main(int argc, char** argv) {
             line += '\n';
             fwrite(line.data(), 1, line.size(), stdout);
         }
+        // GT sites with 3 and 4 alleles: "A:" then their genotype indices (and AF)
+        for (int A = 3; A <= 4; A++) {
+            const uint32_t v = A == 3 ? d.n_sites3 : d.n_sites4;
+            const std::vector<uint8_t>& codes = A == 3 ? d.site_codes3 : d.site_codes4;
+            const std::vector<double>& afs = A == 3 ? d.af3 : d.af4;
+            if (a.field != kvcf::Field::GT) break;
+            for (uint32_t s = 0; s < v; s++) {
+                line = std::to_string(A) + ":";
+                for (uint32_t i = 0; i < n; i++) line += (char)('0' + codes[(size_t)s * n + i]);
+                if (a.af_info)
+                    for (int k = 0; k < A; k++) line += "\t" + fmt_f64(afs[(size_t)s * A + k]);
+                line += '\n';
+                fwrite(line.data(), 1, line.size(), stdout);
+            }
+        }
         fprintf(stderr, "total=%llu not_snp=%llu skipped_missing=%llu\n", (unsigned long long)d.total,
                 (unsigned long long)d.not_snp, (unsigned long long)d.skipped_missing);
         return 0;
@@ -296,10 +311,19 @@ int main(int argc, char** argv) {
     std::vector<kestrel_output> out(npairs);
     int rc;
     if (a.field == kvcf::Field::GT) {
-        rc = kestrel_cohort_create_gt(&c, packed.data(), n, d.n_sites, row_bytes, af);
+        if (d.n_sites3 || d.n_sites4) {  // coefficients.rs:166-320 over the three buckets
+            const std::vector<uint8_t> gt3 = d.sample_major(3), gt4 = d.sample_major(4);
+            rc = kestrel_cohort_create_gt_multi(&c, n, d.n_sites ? packed.data() : nullptr, d.n_sites, row_bytes,
+                                                d.n_sites3 ? gt3.data() : nullptr, d.n_sites3,
+                                                d.n_sites4 ? gt4.data() : nullptr, d.n_sites4,
+                                                a.af_info ? d.af.data() : nullptr, a.af_info ? d.af3.data() : nullptr,
+                                                a.af_info ? d.af4.data() : nullptr);
+        } else {
+            rc = kestrel_cohort_create_gt(&c, packed.data(), n, d.n_sites, row_bytes, af);
+        }
         if (rc) return die(kestrel_last_error());
         info("Calculating Jacquard coefficients for " + std::to_string(npairs) + " pairs using " +
-             std::to_string(d.n_sites) + " sites");
+             std::to_string((uint64_t)d.n_sites + d.n_sites3 + d.n_sites4) + " sites");
         rc = kestrel_fit_pairs_gt(c, a.iterations, nullptr, 0, nsel, out.data(), nullptr);
         if (rc) return die(kestrel_last_error());
         for (const auto& o : out)

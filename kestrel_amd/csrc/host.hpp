@@ -73,6 +73,16 @@ struct SitePart {
     DevBuf<double> af;      // v_pad x alleles
     DevBuf<uint32_t> seg;
 };
+// The 3- or 4-allele sites of a -f GT cohort (kestrel_cohort_create_gt_multi):
+// genotype indices in 4-bit rows (8 sites per word, sample-major, padding 15),
+// frequencies and the sites' m tables (n_sites x G x G x 9).
+struct GtPart {
+    int alleles = 0;
+    uint32_t n_sites = 0, row_words = 0;
+    DevBuf<uint32_t> geno;
+    DevBuf<double> af;
+    DevBuf<double> mtab;
+};
 // Stops the serving engine of a cohort (if any) and frees its buffers.
 void serve_release(ServeState* s);
 
@@ -106,6 +116,7 @@ struct kestrel_cohort {
     bool multi = false;
     uint32_t part_sites[3] = {0, 0, 0};   // sites with 2, 3, 4 alleles after zero-frequency removal
     std::vector<double> af_in[3];         // frequencies of the input buckets (2, 3, 4 alleles)
+    std::unique_ptr<GtPart> gt_parts[2];  // -f GT cohorts: the 3- and 4-allele sites (null when none)
 
     uint32_t nseg_total() const {
         uint32_t n = nseg;

@@ -1325,7 +1325,8 @@ __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src
 
 namespace {
 
-// iis.rs:279-462 conditional_probabilities for the modes two alleles reach.
+// iis.rs:279-462 conditional_probabilities, every IBS mode, in the
+// reference's operation order.
 __device__ __forceinline__ void cond_prob(double pi, double pj, double pk, double pl, int mode, double* o) {
     const double pi2 = pi * pi, pi3 = pi2 * pi, pj2 = pj * pj, pk2 = pk * pk;
 #pragma unroll
@@ -1338,21 +1339,67 @@ __device__ __forceinline__ void cond_prob(double pi, double pj, double pk, doubl
     case 2: o[1] = pi * pk; o[3] = pi * pk2; o[5] = pi2 * pk; o[8] = pi2 * pk2; break;
     case 3: o[2] = pi * pl; o[3] = (2.0 * pi2) * pl; o[7] = pi2 * pl; o[8] = (2.0 * pi3) * pl; break;
     case 4: o[2] = pi * pk; o[3] = (2.0 * pi2) * pk; o[7] = pi2 * pk; o[8] = (2.0 * pi3) * pk; break;
+    case 5: o[3] = ((2.0 * pi) * pk) * pl; o[8] = ((2.0 * pi2) * pk) * pl; break;
     case 6: o[4] = pi * pj; o[5] = (2.0 * pi2) * pj; o[7] = pi2 * pj; o[8] = (2.0 * pi3) * pj; break;
     case 7: o[4] = pi * pj; o[5] = (2.0 * pj2) * pi; o[7] = pj2 * pi; o[8] = (2.0 * (pj2 * pj)) * pi; break;
-    default:  // 9: ij-ij
+    case 8: o[5] = ((2.0 * pi) * pj) * pk; o[8] = ((2.0 * pk2) * pi) * pj; break;
+    case 9:
+    case 10:  // ij-ij and ij-ji
         o[6] = (2.0 * pi) * pj; o[7] = (pi * pj) * (pi + pj); o[8] = (4.0 * pi2) * pj2;
+        break;
+    case 11: o[7] = (pi * pj) * pl; o[8] = ((4.0 * pi2) * pj) * pl; break;
+    case 12: o[7] = (pi * pj) * pk; o[8] = ((4.0 * pi2) * pj) * pk; break;
+    case 13: o[7] = (pi * pj) * pl; o[8] = ((4.0 * pi) * pj2) * pl; break;
+    case 14: o[7] = (pi * pj) * pk; o[8] = ((4.0 * pi) * pj2) * pk; break;
+    default:  // 15: ij-kl
+        o[8] = (((4.0 * pi) * pj) * pk) * pl;
         break;
     }
 }
 
-// conditional.rs IIS_2: genotype pair (gx, gy) in VCF order -> alleles and mode
-__device__ __forceinline__ void m_row(double p, double q, int gx, int gy, double* o) {
-    const int A[3][2] = {{0, 0}, {0, 1}, {1, 1}};
-    const int MODE[3][3] = {{1, 3, 2}, {6, 9, 7}, {2, 4, 1}};
-    const double f[2] = {p, q};
-    cond_prob(f[A[gx][0]], f[A[gx][1]], f[A[gy][0]], f[A[gy][1]], MODE[gx][gy], o);
+// m_s[gx][gy] (cls.rs:6-26) over IIS_A (conditional.rs:10-50): genotype
+// indices in VCF order, alleles (i, j) of gx and (k, l) of gy, their IBS mode.
+__device__ __forceinline__ void m_row_generic(const double* p, int gi_x, int gj_x, int gi_y, int gj_y, double* o) {
+    cond_prob(p[gi_x], p[gj_x], p[gi_y], p[gj_y], iis_mode(gi_x, gj_x, gi_y, gj_y), o);
 }
+
+// Alleles of VCF genotype index g (j(j+1)/2 + i, i <= j), g < 10.
+__device__ __forceinline__ void index_alleles(int g, int* i, int* j) {
+    const int jj = g >= 6 ? 3 : g >= 3 ? 2 : g >= 1 ? 1 : 0;
+    *j = jj;
+    *i = g - jj * (jj + 1) / 2;
+}
+
+// The biallelic m row (IIS_2) from (p, q).
+__device__ __forceinline__ void m_row(double p, double q, int gx, int gy, double* o) {
+    const double f[2] = {p, q};
+    int i, j, k, l;
+    index_alleles(gx, &i, &j);
+    index_alleles(gy, &k, &l);
+    m_row_generic(f, i, j, k, l, o);
+}
+
+// Row r of the reference's concatenated m rows (coefficients.rs:280-320):
+// 9 per 2-allele site (frequencies af2), then the precomputed 36-row tables of
+// the 3-allele sites and 100-row tables of the 4-allele sites.
+struct GramRows {
+    const double* af2;
+    const double* m3;
+    const double* m4;
+    uint64_t rows2, rows3;
+    __device__ __forceinline__ void row(uint64_t r, double* m) const {
+        if (r < rows2) {
+            const uint64_t s = r / 9;
+            const int combo = (int)(r % 9);
+            m_row(af2[2 * s], af2[2 * s + 1], combo / 3, combo % 3, m);
+            return;
+        }
+        r -= rows2;
+        const double* src = r < rows3 ? m3 + r * 9 : m4 + (r - rows3) * 9;
+#pragma unroll
+        for (int k = 0; k < 9; k++) m[k] = src[k];
+    }
+};
 
 }  // namespace
 
@@ -1387,10 +1434,10 @@ __global__ __launch_bounds__(64) void gt_count_kernel(const uint32_t* __restrict
 }
 
 // quadratic.rs:68-365 per tile: thread (tile t, lane j) sums its 32 rows
-// r = 256 t + 8 b + j of the 9V rows (site s = r / 9, genotype pair r % 9)
-// from zero into z[t][45][8].
-__global__ __launch_bounds__(64) void gram_tile_kernel(const double* __restrict__ af, uint64_t n_blocks,
-                                                        uint64_t n_tiles, double* __restrict__ z) {
+// r = 256 t + 8 b + j of the concatenated m rows (GramRows) from zero into
+// z[t][45][8].
+__global__ __launch_bounds__(64) void gram_tile_kernel(GramRows gr, uint64_t n_blocks, uint64_t n_tiles,
+                                                        double* __restrict__ z) {
     const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= n_tiles * 8) return;
     const uint64_t t = tid / 8;
@@ -1400,11 +1447,8 @@ __global__ __launch_bounds__(64) void gram_tile_kernel(const double* __restrict_
     for (int k = 0; k < 45; k++) acc[k] = 0.0;
     const uint64_t b0 = t * 32, b1 = b0 + 32 < n_blocks ? b0 + 32 : n_blocks;
     for (uint64_t b = b0; b < b1; b++) {
-        const uint64_t r = b * 8 + j;
-        const uint64_t s = r / 9;
-        const int combo = (int)(r % 9);
         double m[9];
-        m_row(af[2 * s], af[2 * s + 1], combo / 3, combo % 3, m);
+        gr.row(b * 8 + j, m);
         int id = 0;
 #pragma unroll
         for (int a = 0; a < 9; a++)
@@ -1417,9 +1461,9 @@ __global__ __launch_bounds__(64) void gram_tile_kernel(const double* __restrict_
 
 // Running lane sums over tiles (h = h + z_t), radd, the scalar remainder rows
 // as one partial tile, divided by the number of sites (quadratic.rs:12-33).
-__global__ __launch_bounds__(64) void gram_final_kernel(const double* __restrict__ af, const double* __restrict__ z,
-                                                         uint64_t n_tiles, uint64_t n_blocks, uint32_t n_rem,
-                                                         uint32_t n_sites, double* __restrict__ q) {
+__global__ __launch_bounds__(64) void gram_final_kernel(GramRows gr, const double* __restrict__ z, uint64_t n_tiles,
+                                                         uint64_t n_blocks, uint32_t n_rem, uint32_t n_sites,
+                                                         double* __restrict__ q) {
     const int id = threadIdx.x;
     if (id >= 45) return;
     int a = 0, c = 0;
@@ -1435,16 +1479,111 @@ __global__ __launch_bounds__(64) void gram_final_kernel(const double* __restrict
         for (int j = 0; j < 8; j++) h[j] = h[j] + z[(t * 45 + id) * 8 + j];
     double rz = 0.0;
     for (uint32_t r = 0; r < n_rem; r++) {
-        const uint64_t row = n_blocks * 8 + r;
-        const uint64_t s = row / 9;
-        const int combo = (int)(row % 9);
         double m[9];
-        m_row(af[2 * s], af[2 * s + 1], combo / 3, combo % 3, m);
+        gr.row(n_blocks * 8 + r, m);
         rz = fma(m[a], m[c], rz);
     }
     const double v = (radd8(h) + (0.0 + rz)) / (double)n_sites;
     q[a * 9 + c] = v;
     q[c * 9 + a] = v;
+}
+
+// wrapper.rs:412-428 for the sites with A alleles: genotype indices in
+// 4-bit rows (8 sites per word, sample-major); af n_sites x A.
+template <int A>
+__global__ __launch_bounds__(64) void gt_count_multi_kernel(const uint32_t* __restrict__ geno, uint32_t row_words,
+                                                             uint32_t n_samples, uint32_t n_sites,
+                                                             double* __restrict__ af) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_sites) return;
+    uint64_t f[A];
+#pragma unroll
+    for (int a = 0; a < A; a++) f[a] = 0;
+    for (uint32_t n = 0; n < n_samples; n++) {
+        const int g = (int)((geno[(size_t)n * row_words + s / 8] >> (4 * (s % 8))) & 15u);
+        int i, j;
+        index_alleles(g, &i, &j);
+#pragma unroll
+        for (int a = 0; a < A; a++) f[a] += (uint64_t)(a == i) + (uint64_t)(a == j);
+    }
+    const double tot = (double)(2 * (uint64_t)n_samples);
+#pragma unroll
+    for (int a = 0; a < A; a++) af[(size_t)s * A + a] = (double)f[a] / tot;
+}
+
+// cls.rs:6-26 m-matrices of the sites with A alleles: m[s][gx][gy][9], one
+// thread per (site, genotype pair).
+template <int A>
+__global__ __launch_bounds__(64) void cls_m_kernel(const double* __restrict__ af, uint32_t n_sites,
+                                                    double* __restrict__ m) {
+    constexpr int G = A * (A + 1) / 2;
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (uint64_t)n_sites * G * G) return;
+    const uint64_t s = e / (G * G);
+    const int combo = (int)(e % (G * G));
+    int i, j, k, l;
+    index_alleles(combo / G, &i, &j);
+    index_alleles(combo % G, &k, &l);
+    double o[9];
+    m_row_generic(af + s * A, i, j, k, l, o);
+#pragma unroll
+    for (int t = 0; t < 9; t++) m[e * 9 + t] = o[t];
+}
+
+// cls.rs:28-49 over the sites with A alleles, one pair per lane: c_A = sum of
+// m_s[gx][gy] in site order from zero.  The block stages S sites' m tables in
+// LDS (row gx G + gy; row G G is zero and takes the padding nibbles 15).
+// A = 3 stores c3; A = 4 stores c3 + c4 (cls.rs via coefficients.rs:226-228)
+// with c3 read back from out, or 0 + c4 when there are no 3-allele sites.
+template <int A>
+__global__ __launch_bounds__(256) void cls_multi_kernel(const uint32_t* __restrict__ geno, uint32_t row_words,
+                                                         const double* __restrict__ mtab, uint32_t n_sites,
+                                                         const uint32_t* __restrict__ pairs, uint64_t npairs,
+                                                         int add_prev, double* __restrict__ out) {
+    constexpr int G = A * (A + 1) / 2, GG = G * G, S = A == 4 ? 4 : 8, W = 10, RS = S * W + 2;
+    __shared__ __attribute__((aligned(16))) double lut[(GG + 1) * RS];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t idx = (uint64_t)blockIdx.x * 256 + tid;
+    const bool live = idx < npairs;
+    for (uint32_t i = tid; i < RS; i += 256) lut[GG * RS + i] = 0.0;
+    uint32_t x = 0, y = 0;
+    if (live) {
+        x = pairs[2 * idx];
+        y = pairs[2 * idx + 1];
+    }
+    const uint32_t* rowx = geno + (size_t)x * row_words;
+    const uint32_t* rowy = geno + (size_t)y * row_words;
+    double c[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) c[k] = 0.0;
+    for (uint32_t cs = 0; cs < n_sites; cs += S) {
+        __syncthreads();
+        for (uint32_t e = tid; e < (uint32_t)(GG * S); e += 256) {
+            const uint32_t k = e / GG, row = e % GG, site = cs + k;
+            double* dst = lut + row * RS + k * W;
+            if (site < n_sites) {
+                const double* src = mtab + ((size_t)site * GG + row) * 9;
+                for (int t = 0; t < 9; t++) dst[t] = src[t];
+            } else {
+                for (int t = 0; t < 9; t++) dst[t] = 0.0;
+            }
+        }
+        __syncthreads();
+        const uint32_t wx = rowx[cs / 8] >> (4 * (cs % 8));
+        const uint32_t wy = rowy[cs / 8] >> (4 * (cs % 8));
+#pragma unroll
+        for (int k = 0; k < S; k++) {
+            const uint32_t gx = (wx >> (4 * k)) & 15u, gy = (wy >> (4 * k)) & 15u;
+            const uint32_t row = gx < (uint32_t)G ? gx * G + gy : (uint32_t)GG;
+            const double* r = lut + row * RS + k * W;
+#pragma unroll
+            for (int t = 0; t < 9; t++) c[t] = c[t] + r[t];
+        }
+    }
+    if (!live) return;
+    double* o = out + idx * 9;
+#pragma unroll
+    for (int t = 0; t < 9; t++) o[t] = A == 4 ? (add_prev ? o[t] : 0.0) + c[t] : c[t];
 }
 
 // coefficients.rs:216-276, one pair per lane: c = sum over all sites of
@@ -1453,6 +1592,7 @@ __global__ __launch_bounds__(64) void gram_final_kernel(const double* __restrict
 // and condition.
 __global__ __launch_bounds__(256) void cls_pairs_kernel(const uint32_t* __restrict__ geno, uint32_t row_words,
                                                          const double* __restrict__ af, uint32_t n_sites,
+                                                         const double* __restrict__ c34, uint32_t v_total,
                                                          const double* __restrict__ qmat,
                                                          const uint32_t* __restrict__ pairs, uint64_t npairs,
                                                          Tune tune, kestrel_output* __restrict__ out) {
@@ -1517,9 +1657,10 @@ __global__ __launch_bounds__(256) void cls_pairs_kernel(const uint32_t* __restri
         __syncthreads();
     }
     if (!live) return;
-    const double nv = -(double)n_sites;
+    // coefficients.rs:226-228: (c2 + (c3 + c4)) / -V over all V sites
+    const double nv = -(double)v_total;
 #pragma unroll
-    for (int k = 0; k < 9; k++) c[k] = c[k] / nv;
+    for (int k = 0; k < 9; k++) c[k] = (c[k] + (c34 ? c34[idx * 9 + k] : 0.0 + 0.0)) / nv;
     double Q[9][9];
 #pragma unroll
     for (int i = 0; i < 9; i++)
@@ -1566,7 +1707,7 @@ __global__ __launch_bounds__(256) void cls_pairs_kernel(const uint32_t* __restri
     o.obj = brier;
     o.iters = iters;
     o.cond = condition_reg<9>(delta, g, Q, tune.sqp_conv_tol);
-    o.n_sites = n_sites;
+    o.n_sites = v_total;
     o.status = KESTREL_PAIR_OK;
     out[idx] = o;
 }
@@ -1700,21 +1841,64 @@ hipError_t gt_count(const uint32_t* geno, uint32_t row_words, uint32_t n_samples
     return hipGetLastError();
 }
 
-hipError_t gram(const double* af, uint32_t n_sites, double* z, double* q, hipStream_t st) {
-    const uint64_t rows = (uint64_t)n_sites * 9, n_blocks = rows / 8, n_tiles = (n_blocks + 31) / 32;
+hipError_t gram(const double* af2, uint32_t n2, const double* m3, uint32_t n3, const double* m4, uint32_t n4,
+                double* z, double* q, hipStream_t st) {
+    const GramRows gr{af2, m3, m4, (uint64_t)n2 * 9, (uint64_t)n3 * 36};
+    const uint64_t rows = gr.rows2 + gr.rows3 + (uint64_t)n4 * 100, n_blocks = rows / 8, n_tiles = (n_blocks + 31) / 32;
     const uint32_t n_rem = (uint32_t)(rows % 8);
     if (n_tiles)
-        hipLaunchKernelGGL(gram_tile_kernel, dim3((unsigned)((n_tiles * 8 + 63) / 64)), dim3(64), 0, st, af, n_blocks,
+        hipLaunchKernelGGL(gram_tile_kernel, dim3((unsigned)((n_tiles * 8 + 63) / 64)), dim3(64), 0, st, gr, n_blocks,
                            n_tiles, z);
-    hipLaunchKernelGGL(gram_final_kernel, dim3(1), dim3(64), 0, st, af, z, n_tiles, n_blocks, n_rem, n_sites, q);
+    hipLaunchKernelGGL(gram_final_kernel, dim3(1), dim3(64), 0, st, gr, z, n_tiles, n_blocks, n_rem, n2 + n3 + n4, q);
     return hipGetLastError();
 }
 
-hipError_t cls_pairs(const uint32_t* geno, uint32_t row_words, const double* af, uint32_t n_sites, const double* q,
-                     const uint32_t* pairs, uint64_t npairs, const Tune& t, kestrel_output* out, hipStream_t st) {
+uint64_t gram_tiles(uint32_t n2, uint32_t n3, uint32_t n4) {
+    const uint64_t rows = (uint64_t)n2 * 9 + (uint64_t)n3 * 36 + (uint64_t)n4 * 100;
+    return (rows / 8 + 31) / 32;
+}
+
+hipError_t gt_count_multi(int alleles, const uint32_t* geno, uint32_t row_words, uint32_t n_samples, uint32_t n_sites,
+                          double* af, hipStream_t st) {
+    if (n_sites == 0) return hipSuccess;
+    const dim3 grid((n_sites + 63) / 64);
+    if (alleles == 3)
+        hipLaunchKernelGGL(gt_count_multi_kernel<3>, grid, dim3(64), 0, st, geno, row_words, n_samples, n_sites, af);
+    else
+        hipLaunchKernelGGL(gt_count_multi_kernel<4>, grid, dim3(64), 0, st, geno, row_words, n_samples, n_sites, af);
+    return hipGetLastError();
+}
+
+hipError_t cls_m(int alleles, const double* af, uint32_t n_sites, double* m, hipStream_t st) {
+    if (n_sites == 0) return hipSuccess;
+    const uint64_t gg = alleles == 3 ? 36 : 100, n = (uint64_t)n_sites * gg;
+    const dim3 grid((unsigned)((n + 63) / 64));
+    if (alleles == 3)
+        hipLaunchKernelGGL(cls_m_kernel<3>, grid, dim3(64), 0, st, af, n_sites, m);
+    else
+        hipLaunchKernelGGL(cls_m_kernel<4>, grid, dim3(64), 0, st, af, n_sites, m);
+    return hipGetLastError();
+}
+
+hipError_t cls_multi(int alleles, const uint32_t* geno, uint32_t row_words, const double* mtab, uint32_t n_sites,
+                     const uint32_t* pairs, uint64_t npairs, int add_prev, double* out, hipStream_t st) {
+    if (npairs == 0) return hipSuccess;
+    const dim3 grid((unsigned)((npairs + 255) / 256));
+    if (alleles == 3)
+        hipLaunchKernelGGL(cls_multi_kernel<3>, grid, dim3(256), 0, st, geno, row_words, mtab, n_sites, pairs, npairs,
+                           add_prev, out);
+    else
+        hipLaunchKernelGGL(cls_multi_kernel<4>, grid, dim3(256), 0, st, geno, row_words, mtab, n_sites, pairs, npairs,
+                           add_prev, out);
+    return hipGetLastError();
+}
+
+hipError_t cls_pairs(const uint32_t* geno, uint32_t row_words, const double* af, uint32_t n_sites, const double* c34,
+                     uint32_t v_total, const double* q, const uint32_t* pairs, uint64_t npairs, const Tune& t,
+                     kestrel_output* out, hipStream_t st) {
     if (npairs == 0) return hipSuccess;
     hipLaunchKernelGGL(cls_pairs_kernel, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, st, geno, row_words, af,
-                       n_sites, q, pairs, npairs, t, out);
+                       n_sites, c34, v_total, q, pairs, npairs, t, out);
     return hipGetLastError();
 }
 

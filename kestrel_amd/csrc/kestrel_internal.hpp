@@ -82,9 +82,19 @@ hipError_t complete_sites(const uint8_t* packed, size_t row_bytes, uint32_t n_sa
                           hipStream_t st);
 hipError_t gt_count(const uint32_t* geno, uint32_t row_words, uint32_t n_samples, uint32_t n_sites, double* af,
                     hipStream_t st);
-hipError_t gram(const double* af, uint32_t n_sites, double* z, double* q, hipStream_t st);
-hipError_t cls_pairs(const uint32_t* geno, uint32_t row_words, const double* af, uint32_t n_sites, const double* q,
-                     const uint32_t* pairs, uint64_t npairs, const kdev::Tune& t, kestrel_output* out, hipStream_t st);
+// CLS Gram matrix over the 2-allele sites' m rows (from af2), then the 3- and
+// 4-allele sites' precomputed tables m3 / m4; z holds gram_tiles(...) x 45 x 8.
+hipError_t gram(const double* af2, uint32_t n2, const double* m3, uint32_t n3, const double* m4, uint32_t n4,
+                double* z, double* q, hipStream_t st);
+uint64_t gram_tiles(uint32_t n2, uint32_t n3, uint32_t n4);
+hipError_t gt_count_multi(int alleles, const uint32_t* geno, uint32_t row_words, uint32_t n_samples, uint32_t n_sites,
+                          double* af, hipStream_t st);
+hipError_t cls_m(int alleles, const double* af, uint32_t n_sites, double* m, hipStream_t st);
+hipError_t cls_multi(int alleles, const uint32_t* geno, uint32_t row_words, const double* mtab, uint32_t n_sites,
+                     const uint32_t* pairs, uint64_t npairs, int add_prev, double* out, hipStream_t st);
+hipError_t cls_pairs(const uint32_t* geno, uint32_t row_words, const double* af, uint32_t n_sites, const double* c34,
+                     uint32_t v_total, const double* q, const uint32_t* pairs, uint64_t npairs, const kdev::Tune& t,
+                     kestrel_output* out, hipStream_t st);
 hipError_t finalize(const double* partial, uint32_t nseg, uint32_t nslots, uint32_t npts, double* f, double* g,
                     double* h, uint32_t* nsites, hipStream_t st);
 }  // namespace klaunch

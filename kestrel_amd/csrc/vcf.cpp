@@ -281,6 +281,8 @@ struct Batch {
     std::vector<double> probs;   // kept sites x n x 3 (GL / PL)
     std::vector<double> af;      // kept sites x 2 (INFO AF)
     std::vector<double> probs3, probs4;  // GL / PL sites with 3 and 4 alleles
+    std::vector<uint8_t> codes3, codes4; // GT sites with 3 and 4 alleles: genotype indices
+    std::vector<double> af3, af4;        // their INFO AF (x 3, x 4)
     uint32_t kept = 0, kept3 = 0, kept4 = 0;
     uint64_t total = 0, not_snp = 0, skipped_missing = 0, multiallelic = 0;
     bool one_hot = true;
@@ -312,14 +314,9 @@ bool parse_record(const char* lb, const char* le, uint64_t recno, const Ctx& ctx
         out.not_snp++;
         return true;
     }
-    if (nb > 2 && field == Field::GT) {
-        out.multiallelic++;
-        out.err = "record " + std::to_string(recno) + " (" + str(cols[0]) + ":" + str(cols[1]) + ") has " +
-                  std::to_string(nb) + " alleles: -f GT takes biallelic sites in this build (use -f GL or PL)";
-        return false;
-    }
     const int ng = nb * (nb + 1) / 2;  // likelihoods per sample (vcf.rs:270-305)
-    if (nb > 2 && ctx.parse_af) {  // wrapper.rs:113-122 leaves GL with given frequencies unimplemented
+    if (nb > 2) out.multiallelic++;
+    if (nb > 2 && ctx.parse_af && field != Field::GT) {  // wrapper.rs:113-122 leaves GL with given frequencies unimplemented
         out.err = "record " + std::to_string(recno) + ": INFO allele frequencies with tri- and tetra-allelic "
                   "likelihoods are not supported";
         return false;
@@ -455,20 +452,29 @@ bool parse_record(const char* lb, const char* le, uint64_t recno, const Ctx& ctx
         for (auto& t : sub) {
             if (t.second - t.first > 3 && !strncmp(t.first, "AF=", 3)) {
                 split(t.first + 3, t.second, ',', vals);
-                if (vals.size() != 1) {
-                    out.err = vals.size() > 1 ? "Too many allele frequencies!" : "Not enough allele frequencies!";
+                double afv[4] = {0.0, 0.0, 0.0, 0.0}, alt_sum = 0.0;
+                int num_af = 1;
+                for (auto& v : vals) {
+                    if (num_af >= nb) {
+                        out.err = "Too many allele frequencies!";
+                        return false;
+                    }
+                    bool miss, ok;
+                    const double a = parse_f32(v.first, v.second, &miss, &ok);
+                    if (miss || !ok) {
+                        out.err = miss ? "No AF data found" : "Error reading AF";
+                        return false;
+                    }
+                    alt_sum += a;
+                    afv[num_af++] = a;
+                }
+                if (num_af != nb) {
+                    out.err = "Not enough allele frequencies!";
                     return false;
                 }
-                bool miss, ok;
-                const double a = parse_f32(vals[0].first, vals[0].second, &miss, &ok);
-                if (miss || !ok) {
-                    out.err = miss ? "No AF data found" : "Error reading AF";
-                    return false;
-                }
-                double alt_sum = 0.0;
-                alt_sum += a;
-                out.af.push_back(1.0 - std::fmin(std::fmax(alt_sum, 0.0), 1.0));
-                out.af.push_back(a);
+                afv[0] = 1.0 - std::fmin(std::fmax(alt_sum, 0.0), 1.0);
+                std::vector<double>& dst = nb == 2 ? out.af : nb == 3 ? out.af3 : out.af4;
+                dst.insert(dst.end(), afv, afv + nb);
                 found = true;
                 break;
             }
@@ -479,12 +485,14 @@ bool parse_record(const char* lb, const char* le, uint64_t recno, const Ctx& ctx
         }
     }
     if (nb == 3) {
-        out.probs3.insert(out.probs3.end(), site_probs.begin(), site_probs.end());
+        if (field == Field::GT) out.codes3.insert(out.codes3.end(), codes.begin(), codes.end());
+        else out.probs3.insert(out.probs3.end(), site_probs.begin(), site_probs.end());
         out.kept3++;
         return true;
     }
     if (nb == 4) {
-        out.probs4.insert(out.probs4.end(), site_probs.begin(), site_probs.end());
+        if (field == Field::GT) out.codes4.insert(out.codes4.end(), codes.begin(), codes.end());
+        else out.probs4.insert(out.probs4.end(), site_probs.begin(), site_probs.end());
         out.kept4++;
         return true;
     }
@@ -537,6 +545,16 @@ std::vector<uint8_t> Data::pack(size_t* row_bytes) const {
     return out;
 }
 
+std::vector<uint8_t> Data::sample_major(int alleles) const {
+    const size_t n = samples.size();
+    const std::vector<uint8_t>& src = alleles == 3 ? site_codes3 : site_codes4;
+    const size_t v = alleles == 3 ? n_sites3 : n_sites4;
+    std::vector<uint8_t> out(n * v);
+    for (size_t s = 0; s < v; s++)
+        for (size_t i = 0; i < n; i++) out[i * v + s] = src[s * n + i];
+    return out;
+}
+
 bool read(const std::string& path, Field field, bool parse_af, Data& out, std::string& err, unsigned threads) {
     if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
     FILE* f = fopen(path.c_str(), "rb");
@@ -581,6 +599,10 @@ bool read(const std::string& path, Field field, bool parse_af, Data& out, std::s
         out.af.insert(out.af.end(), b.af.begin(), b.af.end());
         out.probs3.insert(out.probs3.end(), b.probs3.begin(), b.probs3.end());
         out.probs4.insert(out.probs4.end(), b.probs4.begin(), b.probs4.end());
+        out.site_codes3.insert(out.site_codes3.end(), b.codes3.begin(), b.codes3.end());
+        out.site_codes4.insert(out.site_codes4.end(), b.codes4.begin(), b.codes4.end());
+        out.af3.insert(out.af3.end(), b.af3.begin(), b.af3.end());
+        out.af4.insert(out.af4.end(), b.af4.begin(), b.af4.end());
         out.n_sites += b.kept;
         out.n_sites3 += b.kept3;
         out.n_sites4 += b.kept4;

@@ -21,17 +21,23 @@ struct Data {
     // n_sites3 x n_samples x 6 and n_sites4 x n_samples x 10
     uint32_t n_sites3 = 0, n_sites4 = 0;
     std::vector<double> probs3, probs4;
+    // GT sites with 3 and 4 alleles: site-major genotype indices (VCF order) and
+    // INFO AF (n_sites3 x 3, n_sites4 x 4; reference allele first)
+    std::vector<uint8_t> site_codes3, site_codes4;
+    std::vector<double> af3, af4;
     uint64_t total = 0, not_snp = 0, skipped_missing = 0, multiallelic = 0;
 
     // 2-bit rows, sample-major, site s of a row in bits 2*(s%4) of byte s/4
     std::vector<uint8_t> pack(size_t* row_bytes) const;
+    // The GT genotype indices of the 3- or 4-allele sites, sample-major (N x V_A)
+    std::vector<uint8_t> sample_major(int alleles) const;
 };
 
 // Reads `path`; returns false with `err` set on failure (the reference's
 // anyhow errors).  GT: a site with a missing call is skipped (vcf.rs:217-221).
 // GL / PL: every row is kept as its linear-scale likelihoods (vcf.rs:368, 407)
-// in `probs` (sites with 3 or 4 alleles in `probs3` / `probs4`; GT records with
-// more than 2 alleles are an error: the -f GT estimator here is biallelic); a row with a NaN (a missing value, vcf.rs:355, 397) has code 3,
+// in `probs` (sites with 3 or 4 alleles in `probs3` / `probs4`; GT sites with 3
+// or 4 alleles in `site_codes3` / `site_codes4`); a row with a NaN (a missing value, vcf.rs:355, 397) has code 3,
 // a one-hot row (the called genotype 1, the others 0) the called genotype's
 // code; `one_hot` says whether every called row was one-hot.
 // threads: parser / inflater threads (0 = one per hardware thread); the

@@ -244,6 +244,72 @@ def write_vcf_multi(path: str, buckets, field: str = "GL", names=None) -> None:
                 f.write(f"1\t{pos}\t.\tA\t{alts[A]}\t.\tPASS\t.\t{field}\t{cells(b[k])}\n")
 
 
+def multi_genotypes(n_samples: int, n_sites: tuple, seed: int, family_fraction: float = 0.4,
+                    missing: float = 0.0):
+    """Hard calls at sites with 2, 3 and 4 alleles: one (N, V_A) uint8 array of
+    genotype indices per bucket (VCF order, j(j+1)/2 + i for alleles i <= j;
+    255 = a missing call), sample-major.  Allele frequencies are
+    Dirichlet(0.7) per site; the first samples are founders drawing both
+    alleles from them, and a `family_fraction` of the samples are children of
+    two earlier samples (one allele of each parent, per site), so the cohort
+    holds parent-child, sibling and more distant pairs.  A call is missing
+    with probability `missing`."""
+    rng = np.random.default_rng(seed)
+    n_child = int(round(family_fraction * n_samples))
+    n_found = max(2, n_samples - n_child)
+    parents = [(int(rng.integers(k)), int(rng.integers(k))) for k in range(n_found, n_samples)]
+    out = []
+    for A, v in zip((2, 3, 4), n_sites):
+        al = np.empty((n_samples, v, 2), np.int64)
+        for s in range(v):
+            p = rng.dirichlet(np.full(A, 0.7))
+            al[:n_found, s] = rng.choice(A, (n_found, 2), p=p)
+        for k, (mo, fa) in zip(range(n_found, n_samples), parents):
+            pick_m = rng.integers(2, size=v)
+            pick_f = rng.integers(2, size=v)
+            al[k, :, 0] = al[mo, np.arange(v), pick_m]
+            al[k, :, 1] = al[fa, np.arange(v), pick_f]
+        lo, hi = al.min(axis=2), al.max(axis=2)
+        idx = (hi * (hi + 1) // 2 + lo).astype(np.uint8)
+        if missing > 0:
+            idx[rng.random(idx.shape) < missing] = 255
+        out.append(idx)
+    return tuple(out)
+
+
+def write_vcf_gt_multi(path: str, buckets, names=None, af=None) -> None:
+    """A GT VCF of genotype-index buckets (idx2, idx3, idx4) as multi_genotypes
+    makes them: records interleaved 2-, 3-, 4-allele site by site (each bucket
+    keeps its order), calls 'i/j' (or '.|.' for 255); af = (af2, af3, af4)
+    writes INFO AF with the alternate frequencies (V_A x A arrays, 7
+    significant digits)."""
+    import gzip
+
+    n = next(b.shape[0] for b in buckets if b is not None)
+    names = names or [f"S{i:05d}" for i in range(n)]
+    alts = {2: "C", 3: "C,G", 4: "C,G,T"}
+    calls = {}
+    for A in (2, 3, 4):
+        t = [f"{i}/{j}" for j in range(A) for i in range(j + 1)]
+        calls[A] = np.array(t + ["."] * (256 - len(t)), dtype=object)
+        calls[A][255] = ".|."
+    opener = gzip.open if path.endswith(".gz") else open
+    with opener(path, "wt") as f:
+        f.write("##fileformat=VCFv4.2\n")
+        f.write('##INFO=<ID=AF,Number=A,Type=Float,Description="Allele frequency">\n')
+        f.write('##FORMAT=<ID=GT,Number=1,Type=String,Description="Genotype">\n')
+        f.write("#CHROM\tPOS\tID\tREF\tALT\tQUAL\tFILTER\tINFO\tFORMAT\t" + "\t".join(names) + "\n")
+        pos = 0
+        longest = max(b.shape[1] for b in buckets if b is not None)
+        for k in range(longest):
+            for bi, (A, b) in enumerate(zip((2, 3, 4), buckets)):
+                if b is None or k >= b.shape[1]:
+                    continue
+                pos += 1
+                inf = "AF=" + ",".join(f"{x:.7g}" for x in af[bi][k, 1:]) if af is not None else "."
+                f.write(f"1\t{pos}\t.\tA\t{alts[A]}\t.\tPASS\t{inf}\tGT\t" + "\t".join(calls[A][b[:, k]]) + "\n")
+
+
 def pack2(codes: np.ndarray, row_sites: int | None = None) -> np.ndarray:
     """Pack codes (N, V) into 2 bits per call, 4 calls per byte, site s of a
     row in bits 2*(s%4) of byte s//4.  Rows are padded with code 3 (missing)

@@ -60,8 +60,10 @@ def test_gt_errors(tmp_path):
     assert "not diploid" in err
     err = dump(write(tmp_path, "1\t1\t.\tA\tG\t.\tPASS\t.\tGT\t0/1/1\t0/1\t1/1\n"), expect_ok=False)
     assert "not diploid" in err
-    err = dump(write(tmp_path, "1\t1\t.\tA\tG,T\t.\tPASS\t.\tGT\t0/2\t0/1\t1/1\n"), expect_ok=False)
-    assert "biallelic" in err
+    err = dump(write(tmp_path, "1\t1\t.\tA\tG,T\t.\tPASS\t.\tGT\t0/3\t0/1\t1/1\n"), expect_ok=False)
+    assert "allele index 3 exceeds" in err
+    err = dump(write(tmp_path, "1\t1\t.\tA\tG\t.\tPASS\t.\tGT\t0/2\t0/1\t1/1\n"), expect_ok=False)
+    assert "allele index 2 exceeds" in err
     err = dump(write(tmp_path, "1\t1\t.\tA\tG\t.\tPASS\t.\tDP\t1\t2\t3\n"), expect_ok=False)
     assert "No GT data found" in err
 
@@ -204,3 +206,58 @@ def test_multiallelic_likelihoods_are_bucketed(tmp_path):
     assert np.array_equal(got, np.array(exp), equal_nan=True)
     bad = write(tmp_path, "1\t1\t.\tA\tG,T\t.\tPASS\t.\tGL\t0,0,0\t0,0,0,0,0,0\t0,0,0,0,0,0\n", "b.vcf")
     assert "Not enough genotype likelihoods" in dump(bad, "GL", expect_ok=False)
+
+
+def dump_multi(path, af_info=False):
+    """--dump-codes of a GT file: (2-allele lines, {3: lines, 4: lines})."""
+    r = subprocess.run([KESTREL, path, "/dev/null", "-f", "GT", "--dump-codes"] + (["-a", "info"] if af_info else []),
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.rstrip("\n").split("\n")[1:]
+    two = [ln.split("\t") for ln in lines if ":" not in ln.split("\t")[0]]
+    multi = {3: [], 4: []}
+    for ln in lines:
+        head = ln.split("\t")[0]
+        if ":" in head:
+            A, codes = head.split(":")
+            multi[int(A)].append([codes] + ln.split("\t")[1:])
+    return two, multi
+
+
+def test_gt_multiallelic_sites_are_bucketed(tmp_path):
+    """GT records with 3 and 4 alleles go to their own buckets (vcf.rs:126-160)
+    as genotype indices j(j+1)/2 + i (coefficients.rs:19-38); a missing call
+    drops the site in every bucket; INFO AF holds A - 1 values and the
+    reference allele gets 1 - clamp(sum) (vcf.rs:446-482)."""
+    body = ("1\t1\t.\tA\tG\t.\tPASS\tAF=0.25\tGT\t0/0\t0/1\t1/1\n"
+            "1\t2\t.\tA\tG,T\t.\tPASS\tAF=0.25,0.5\tGT\t0/2\t2|1\t2/2\n"
+            "1\t3\t.\tA\tC,G,T\t.\tPASS\tAF=0.1,0.2,0.3\tGT\t3/0\t1/3\t3/3\n"
+            "1\t4\t.\tA\tG,T\t.\tPASS\tAF=0.5,0.5\tGT\t0/0\t./2\t1/1\n"
+            "1\t5\t.\tA\tG,T\t.\tPASS\tAF=0.75,0.5\tGT\t1/0\t1/1\t0/0\n")
+    p = write(tmp_path, body, "m.vcf")
+    two, multi = dump_multi(p, af_info=True)
+    assert [t[0] for t in two] == ["012"]
+    assert [m[0] for m in multi[3]] == ["345", "120"]
+    assert [m[0] for m in multi[4]] == ["679"]
+    f32 = lambda x: float(np.float32(x))
+    assert [float(v) for v in multi[3][0][1:]] == [1.0 - (f32(0.25) + f32(0.5)), f32(0.25), f32(0.5)]
+    assert [float(v) for v in multi[3][1][1:]] == [0.0, f32(0.75), f32(0.5)]  # clamped alt sum
+    assert [float(v) for v in multi[4][0][1:]] == [1.0 - (f32(0.1) + f32(0.2) + f32(0.3)), f32(0.1), f32(0.2),
+                                                   f32(0.3)]
+    _, counts = dump(p)[1:]
+    assert counts == {"total": 5, "not_snp": 0, "skipped_missing": 1}
+    for af, msg in (("AF=0.1", "Not enough allele frequencies"), ("AF=0.1,0.2,0.3", "Too many allele frequencies")):
+        bad = write(tmp_path, f"1\t1\t.\tA\tG,T\t.\tPASS\t{af}\tGT\t0/2\t0/1\t1/1\n", "b.vcf")
+        assert msg in dump(bad, "GT", af_info=True, expect_ok=False)
+
+
+def test_gt_multiallelic_synthetic_roundtrip(tmp_path):
+    idx = synth.multi_genotypes(9, (25, 12, 7), seed=4, missing=0.02)
+    p = str(tmp_path / "g.vcf.gz")
+    synth.write_vcf_gt_multi(p, idx)
+    two, multi = dump_multi(p)
+    for A, got, b in ((2, [t[0] for t in two], idx[0]), (3, [m[0] for m in multi[3]], idx[1]),
+                      (4, [m[0] for m in multi[4]], idx[2])):
+        keep = ~(b == 255).any(axis=0)
+        exp = ["".join(str(v) for v in b[:, s]) for s in np.flatnonzero(keep)]
+        assert got == exp, A

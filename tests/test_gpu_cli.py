@@ -63,6 +63,35 @@ def test_cli_gt_cls_bitexact(O, tmp_path):
     assert "Wall Time" in log
 
 
+@pytest.mark.parametrize("af_info", [False, True])
+def test_cli_gt_multiallelic_cls_bitexact(O, tmp_path, af_info):
+    """-f GT with 2-, 3- and 4-allele records: the reader's buckets, the
+    three-bucket CLS fit (coefficients.rs:166-320), TSV text."""
+    print(f"case: cli -f GT multi-allelic af_info={af_info}", flush=True)
+    idx = synth.multi_genotypes(22, (400, 60, 30), seed=23, missing=0.002)
+    rng = np.random.default_rng(4)
+    af = [rng.dirichlet(np.ones(A), v) for A, v in zip((2, 3, 4), (400, 60, 30))] if af_info else None
+    vcf = str(tmp_path / "m.vcf")
+    synth.write_vcf_gt_multi(vcf, idx, af=af)
+    rows, log = run(vcf, str(tmp_path / "o.tsv"), *(["-a", "info"] if af_info else []))
+    names = [f"S{i:05d}" for i in range(22)]
+    x, y, kin, jac, it, obj, cond = as_arrays(rows, names)
+    keep = [~(b == 255).any(axis=0) for b in idx]
+    afk = None
+    if af_info:  # the text the writer put in INFO, read as f32 (vcf.rs:446-482)
+        afk = []
+        for a, k in zip(af, keep):
+            alt = np.array([[float(np.float32(float(f"{v:.7g}"))) for v in row[1:]] for row in a[k]])
+            ref = 1.0 - np.clip(np.array([sum(r, 0.0) for r in alt]), 0.0, 1.0)
+            afk.append(np.column_stack([ref, alt]))
+    exp, _ = O.cls_fit_pairs_multi(*(b[:, k] for b, k in zip(idx, keep)), af=afk)
+    assert np.array_equal(x, exp["x"]) and np.array_equal(y, exp["y"])
+    assert np.array_equal(jac, exp["jacquard"]) and np.array_equal(it, exp["iters"])
+    assert np.array_equal(obj, exp["obj"])
+    assert np.array_equal(cond, log10(exp["cond"]), equal_nan=True)
+    assert "tri-allelic" in log
+
+
 @pytest.mark.parametrize("field", ["GL", "PL"])
 def test_cli_ml_matches_oracle(O, tmp_path, field):
     print(f"case: cli -f {field}", flush=True)

@@ -4,20 +4,32 @@ import numpy as np
 import pytest
 
 from kestrel_amd import OUTPUT_DTYPE, build_arrays, relatedness_coefficients_gl
-from kestrel_amd.api import _gt_codes
+from kestrel_amd import relatedness_coefficients_gt
+from kestrel_amd.api import _gt_indices
 
 
-def test_gt_codes_unphased_index():
+def test_gt_indices_unphased_index():
     # coefficients.rs:27-32: j(j+1)/2 + i with i = min, j = max
     g = np.array([[[0, 0], [0, 1], [1, 0], [1, 1]]], np.uint8)  # one site, four samples
-    assert _gt_codes(g).tolist() == [[0], [1], [1], [2]]
+    assert _gt_indices(g).tolist() == [[0], [1], [1], [2]]
+    g = np.array([[[0, 2], [2, 1], [2, 2], [3, 0], [1, 3], [3, 3]]], np.uint8)
+    assert _gt_indices(g).ravel().tolist() == [3, 4, 5, 6, 7, 9]
 
 
-def test_gt_codes_rejects_multiallelic_and_bad_shape():
+def test_gt_indices_rejects_fifth_allele_and_bad_shape():
     with pytest.raises(ValueError):
-        _gt_codes(np.array([[[0, 2]]], np.uint8))
+        _gt_indices(np.array([[[0, 4]]], np.uint8))
     with pytest.raises(ValueError):
-        _gt_codes(np.zeros((3, 4), np.uint8))
+        _gt_indices(np.zeros((3, 4), np.uint8))
+
+
+def test_gt_allele_counts_checked_before_any_device_call():
+    g = np.zeros((2, 3, 2), np.uint8)
+    g[1, 0] = (0, 2)
+    with pytest.raises(ValueError, match="exceeds"):
+        relatedness_coefficients_gt(g, alleles=[2, 2])
+    with pytest.raises(ValueError, match="2, 3 or 4"):
+        relatedness_coefficients_gt(g, alleles=[2, 5])
 
 
 def test_gl_input_checks_before_any_device_call():
