@@ -259,6 +259,11 @@ int kestrel_selftest(uint64_t n, uint64_t seed, uint64_t* mismatches);
  * state the measured roofline next to the vendor peak. */
 int kestrel_measure_fp64_peak(double* tflops);
 
+/* The same for FP32 FMA chains as this library compiles FP32 code (no packed
+ * FP32 instructions): the rate an FP32 form of the per-site terms could use
+ * (no counterpart in the reference). */
+int kestrel_measure_fp32_peak(double* tflops);
+
 /* Plain device copy of `bytes` (two launches, the second timed): HBM
  * read+write GB/s (no counterpart in the reference; the reference kernel for
  * scaling hardware byte counters). */

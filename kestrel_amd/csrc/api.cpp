@@ -703,7 +703,13 @@ int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* poi
 
 void kestrel_cohort_destroy(kestrel_cohort* c) { delete c; }
 
-int kestrel_measure_fp64_peak(double* tflops) {
+}  // extern "C"
+
+namespace {
+
+// Times one launch of the FMA-chain kernel (after a short warm-up launch).
+template <class T, class Launch>
+int measure_fma(Launch launch, double* tflops) {
     if (!tflops) return fail(KESTREL_EINVAL, "null argument");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -714,14 +720,14 @@ int kestrel_measure_fp64_peak(double* tflops) {
     HIPCHK(hipGetDeviceProperties(&prop, dev));
     const uint32_t blocks = (uint32_t)prop.multiProcessorCount * 16;  // 16 workgroups of 4 waves per CU
     const uint32_t iters = 1u << 15;
-    DevBuf<double> sink;
+    DevBuf<T> sink;
     HIPCHK(sink.alloc(1));
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    HIPCHK(klaunch::calib_fp64(blocks, 1024, sink.p, nullptr));  // warm up clocks
+    HIPCHK(launch(blocks, 1024u, sink.p));  // warm up clocks
     HIPCHK(hipEventRecord(e0, nullptr));
-    HIPCHK(klaunch::calib_fp64(blocks, iters, sink.p, nullptr));
+    HIPCHK(launch(blocks, iters, sink.p));
     HIPCHK(hipEventRecord(e1, nullptr));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;
@@ -731,6 +737,20 @@ int kestrel_measure_fp64_peak(double* tflops) {
     const double flop = (double)blocks * 256.0 * iters * 16.0 * 2.0;
     *tflops = flop / (ms * 1e-3) / 1e12;
     return KESTREL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kestrel_measure_fp64_peak(double* tflops) {
+    return measure_fma<double>(
+        [](uint32_t b, uint32_t it, double* s) { return klaunch::calib_fp64(b, it, s, nullptr); }, tflops);
+}
+
+int kestrel_measure_fp32_peak(double* tflops) {
+    return measure_fma<float>(
+        [](uint32_t b, uint32_t it, float* s) { return klaunch::calib_fp32(b, it, s, nullptr); }, tflops);
 }
 
 int kestrel_measure_copy(uint64_t bytes, double* gbps) {

@@ -1298,20 +1298,21 @@ __global__ __launch_bounds__(256) void selftest_kernel(uint64_t n, uint64_t seed
     if (local) atomicAdd(bad, local);
 }
 
-// Achievable FP64 FMA throughput: 16 independent fma chains per lane.
-__global__ __launch_bounds__(256) void calib_fp64_kernel(uint32_t iters, double a, double b,
-                                                          double* __restrict__ sink) {
-    double acc[16];
+// Achievable vector FMA throughput: 16 independent fma chains per lane, in
+// FP64 or (as this library compiles it: no packed FP32) FP32.
+template <class T>
+__global__ __launch_bounds__(256) void calib_fma_kernel(uint32_t iters, T a, T b, T* __restrict__ sink) {
+    T acc[16];
 #pragma unroll
-    for (int k = 0; k < 16; k++) acc[k] = (double)(threadIdx.x + k);
+    for (int k = 0; k < 16; k++) acc[k] = (T)(threadIdx.x + k);
     for (uint32_t i = 0; i < iters; i++) {
 #pragma unroll
         for (int k = 0; k < 16; k++) acc[k] = fma(acc[k], a, b);
     }
-    double s = 0.0;
+    T s = 0;
 #pragma unroll
     for (int k = 0; k < 16; k++) s += acc[k];
-    if (s == 1.2345) sink[0] = s;  // keeps the chains live
+    if (s == (T)1.2345) sink[0] = s;  // keeps the chains live
 }
 
 // Plain copy of n16 16-byte words (the reference kernel for HBM byte counters).
@@ -1815,8 +1816,13 @@ hipError_t selftest(uint64_t n, uint64_t seed, unsigned long long* bad, hipStrea
     return hipGetLastError();
 }
 
+hipError_t calib_fp32(uint32_t blocks, uint32_t iters, float* sink, hipStream_t st) {
+    hipLaunchKernelGGL(calib_fma_kernel<float>, dim3(blocks), dim3(256), 0, st, iters, 0.999999f, 1e-7f, sink);
+    return hipGetLastError();
+}
+
 hipError_t calib_fp64(uint32_t blocks, uint32_t iters, double* sink, hipStream_t st) {
-    hipLaunchKernelGGL(calib_fp64_kernel, dim3(blocks), dim3(256), 0, st, iters, 0.999999, 1e-7, sink);
+    hipLaunchKernelGGL(calib_fma_kernel<double>, dim3(blocks), dim3(256), 0, st, iters, 0.999999, 1e-7, sink);
     return hipGetLastError();
 }
 
