@@ -209,6 +209,9 @@ def main():
     ap.add_argument("--segment-sites", type=int, default=None)
     ap.add_argument("--slots", type=int, default=None)
     ap.add_argument("--streams", type=int, default=None, help="slot groups on separate streams (1 or 2)")
+    ap.add_argument("--batch", type=int, default=None, help="iterations queued per host synchronisation")
+    ap.add_argument("--autotune", type=int, default=0, metavar="PAIRS",
+                    help="before the warmup, tune slots/streams/batch on this many random pairs (0 = off)")
     ap.add_argument("--serving", action="store_true", help="serving workload (default config c5)")
     ap.add_argument("--soft", action="store_true", help="soft genotype likelihoods instead of hard calls")
     ap.add_argument("--depth", type=float, default=4.0, help="mean read depth of --soft likelihoods")
@@ -245,8 +248,15 @@ def main():
         co.set_option("slots", args.slots)
     if args.streams:
         co.set_option("streams", args.streams)
+    if args.batch:
+        co.set_option("batch", args.batch)
     af, kept = co.allele_freqs()
     log(f"device cohort + allele frequencies in {time.perf_counter() - t:.2f}s, kept {co.n_kept} sites")
+    tuned = None
+    if args.autotune:
+        t = time.perf_counter()
+        tuned = co.autotune(sample=args.autotune)
+        log(f"autotune ({time.perf_counter() - t:.1f}s, untimed): {tuned}")
 
     for i in range(args.warmup):
         t = time.perf_counter()
@@ -304,7 +314,8 @@ def main():
                                f"{offdiag} off-diagonal pairs (+{len(out) - offdiag} self pairs), FP64"
                                + (f", soft likelihoods (mean depth {args.depth})" if args.soft else ""),
                    "samples": n, "sites": v, "pairs": offdiag, "precision": "fp64",
-                   "input": "soft genotype likelihoods" if args.soft else "hard calls"},
+                   "input": "soft genotype likelihoods" if args.soft else "hard calls",
+                   "launch_shape": ({k: tuned[k] for k in ("slots", "streams", "batch")} if tuned else "default")},
         "roofline": roofline, "cpu_baseline": cpu,
         "iterations_mean": round(float(out["iters"].mean()), 3),
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,

@@ -182,10 +182,31 @@ int kestrel_cohort_create_gt_multi(kestrel_cohort** out, uint32_t n_samples, con
  * 32, default 4096; fixes the summation order, so it is a cohort property),
  * "slots" (pairs in flight per slot group, 0 = fill the device), "streams"
  * (1 or 2 slot groups on separate HIP streams, default 2: one group's fold and
- * step run beside the other's sweep; results do not change), "serve_policy" (order in
+ * step run beside the other's sweep; results do not change), "batch" (sweep
+ * and step iterations queued per host synchronisation, 1 to 64, default 8;
+ * results do not change), "serve_policy" (order in
  * which a serving engine started afterwards feeds admitted requests to the
  * device: 1 = shortest remaining first (default), 0 = round robin). */
 int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value);
+
+/* Launch-shape autotuner (no counterpart in the reference; the reference's
+ * thread pool has no shape to tune).  Fits the sample `pairs` (n_pairs x 2, a
+ * random sample of the job, at least a few times the device's pairs in
+ * flight) twice with each candidate shape -- "slots" at 1/8 to 3 device
+ * fills, "streams" 1 and 2, "batch" 4, 8, 16; a shape only when the sample
+ * holds 4 times its pairs in flight -- and predicts the time of a job of
+ * job_pairs pairs (0 = n_pairs) from the sample's steady rate (pairs done
+ * while every slot was busy) plus its drain and host time.  Keeps the
+ * fastest prediction and sets those options on the cohort.  None of them
+ * changes a fit's results. */
+typedef struct {
+    int64_t slots, streams, batch;  /* the shape chosen (slots 0 = fill the device) */
+    double pairs_per_s;             /* its predicted job throughput */
+    double default_pairs_per_s;     /* that of the shape in force before */
+    uint32_t trials;                /* sample fits run */
+} kestrel_autotune_result;
+int kestrel_autotune(kestrel_cohort* c, const kestrel_tune* tune, const uint32_t* pairs, size_t n_pairs,
+                     uint64_t job_pairs, kestrel_autotune_result* res);
 
 /* Timing of the last kestrel_fit_pairs: sweep-kernel milliseconds (sum of
  * launch durations by HIP events), number of sweeps, wall milliseconds. */

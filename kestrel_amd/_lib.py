@@ -60,8 +60,20 @@ EXPORTS = (
     "kestrel_measure_fp64_peak", "kestrel_measure_copy", "kestrel_cohort_create_gt", "kestrel_fit_pairs_gt",
     "kestrel_serve_start", "kestrel_submit_queries", "kestrel_poll", "kestrel_wait", "kestrel_serve_get_stats",
     "kestrel_serve_stop", "kestrel_last_fit_busy", "kestrel_cohort_create_gl", "kestrel_cohort_create_gl_multi",
-    "kestrel_cohort_multi_info", "kestrel_cohort_create_gt_multi", "kestrel_measure_fp32_peak",
+    "kestrel_cohort_multi_info", "kestrel_cohort_create_gt_multi", "kestrel_measure_fp32_peak", "kestrel_autotune",
 )
+
+
+class AutotuneResult(ctypes.Structure):
+    """kestrel_autotune_result."""
+    _fields_ = [
+        ("slots", ctypes.c_int64),
+        ("streams", ctypes.c_int64),
+        ("batch", ctypes.c_int64),
+        ("pairs_per_s", ctypes.c_double),
+        ("default_pairs_per_s", ctypes.c_double),
+        ("trials", ctypes.c_uint32),
+    ]
 
 
 class ServeStats(ctypes.Structure):
@@ -113,6 +125,7 @@ def load():
     L.kestrel_fit_pairs_gt.argtypes = [P, ctypes.c_uint64, P, ctypes.c_size_t, ctypes.c_uint32, P, P]
     L.kestrel_measure_fp64_peak.argtypes = [P]
     L.kestrel_measure_fp32_peak.argtypes = [P]
+    L.kestrel_autotune.argtypes = [P, P, P, ctypes.c_size_t, ctypes.c_uint64, P]
     L.kestrel_measure_copy.argtypes = [ctypes.c_uint64, P]
     L.kestrel_serve_start.argtypes = [P, P, ctypes.c_uint32, ctypes.c_uint64]
     L.kestrel_submit_queries.argtypes = [P, P, ctypes.c_uint32, ctypes.c_size_t, P, ctypes.c_size_t, P, P]

@@ -238,6 +238,34 @@ class Cohort:
         check(_lib.load().kestrel_fit_pairs(self._h, ctypes.byref(t), ptr(pp), n, n_select, ptr(out)))
         return out
 
+    def autotune(self, pairs: np.ndarray | None = None, sample: int = 65536, seed: int = 0,
+                 job_pairs: int | None = None, tune: Tune | None = None, iterations: int = 100) -> dict:
+        """Launch-shape autotuner (kestrel_autotune): fits a sample of pairs
+        (default: `sample` random off-diagonal pairs) with each candidate
+        slots / streams / batch shape, predicts the time of a job of
+        job_pairs pairs (default: every pair of the cohort) from each, keeps
+        the fastest on this cohort and returns it with the predicted
+        throughputs.  Results of later fits do not depend on the shape."""
+        t = tune or tune_default(iterations)
+        if pairs is None:
+            rng = np.random.default_rng(seed)
+            n = self.n_samples
+            if n < 2:
+                raise ValueError("tuning needs at least two samples")
+            m = sample  # drawn with replacement: a timing sample
+            x = rng.integers(0, n, 2 * m + 16)
+            y = rng.integers(0, n, 2 * m + 16)
+            keep = x != y
+            pairs = np.stack([np.minimum(x, y), np.maximum(x, y)], 1)[keep][:m]
+        pp = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        if job_pairs is None:
+            job_pairs = self.n_samples * (self.n_samples + 1) // 2
+        r = _lib.AutotuneResult()
+        check(_lib.load().kestrel_autotune(self._h, ctypes.byref(t), ptr(pp), pp.shape[0], int(job_pairs),
+                                           ctypes.byref(r)))
+        return {"slots": r.slots, "streams": r.streams, "batch": r.batch, "pairs_per_s": r.pairs_per_s,
+                "default_pairs_per_s": r.default_pairs_per_s, "trials": r.trials, "sample_pairs": int(pp.shape[0])}
+
     def evaluate(self, pairs: np.ndarray, points: np.ndarray, eps: float = 1e-8):
         """Objective, gradient and Hessian at points (M, 9) for pairs (M, 2)."""
         pp = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)

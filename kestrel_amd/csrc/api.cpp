@@ -81,6 +81,13 @@ int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, con
     return KESTREL_OK;
 }
 
+// Pairs in flight per slot group that fill the device: 2 blocks of 256 lanes
+// per CU over the cohort's segments.
+uint64_t default_slots(const kestrel_cohort* c) {
+    const uint64_t resident = (uint64_t)std::max(1, c->n_cu) * 2;
+    return std::max<uint64_t>(1, resident / std::max<uint32_t>(1, c->nseg)) * SWEEP_TPB;
+}
+
 int open_device(kestrel_cohort* c) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -428,6 +435,11 @@ int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value) {
         c->opt_streams = value;
         return KESTREL_OK;
     }
+    if (!strcmp(key, "batch")) {
+        if (value < 1 || value > 64) return fail(KESTREL_EINVAL, "batch must be within 1 to 64");
+        c->opt_batch = value;
+        return KESTREL_OK;
+    }
     if (!strcmp(key, "serve_policy")) {
         if (value != 0 && value != 1) return fail(KESTREL_EINVAL, "serve_policy must be 0 or 1");
         c->opt_serve_policy = value;
@@ -495,11 +507,9 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     }
     const auto t0 = std::chrono::steady_clock::now();
 
-    // pairs in flight per group: fill the device (2 blocks of 256 lanes per
-    // CU) over nseg segments, never more than the pairs available
-    const uint64_t resident = (uint64_t)std::max(1, c->n_cu) * 2;
-    const uint64_t slot_blocks = std::max<uint64_t>(1, resident / c->nseg);
-    uint64_t nslots = c->opt_slots > 0 ? (uint64_t)c->opt_slots : slot_blocks * SWEEP_TPB;
+    // pairs in flight per group: fill the device unless set, never more than
+    // the pairs available
+    uint64_t nslots = c->opt_slots > 0 ? (uint64_t)c->opt_slots : default_slots(c);
     nslots = std::min<uint64_t>(nslots, ((uint64_t)n_pairs + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB);
     nslots = (nslots + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB;
     const uint32_t ns = (uint32_t)nslots;
@@ -527,7 +537,7 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     // sweeps and steps are queued BATCH at a time between host synchronisations;
     // each step adds the slots still live after it to hist[i], so the host
     // counts (and times) only sweeps that ran with live slots
-    constexpr int BATCH = 8;
+    const int BATCH = (int)c->opt_batch;
     struct Group {
         hipStream_t stream = nullptr;
         DevBuf<uint32_t> sx, sy;
@@ -535,7 +545,7 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
         DevBuf<int> active, hist;
         DevBuf<SlotState> st;
         std::vector<hipEvent_t> ev;
-        int hist_h[BATCH] = {};
+        std::vector<int> hist_h;
         int live_before = 0;
         ~Group() {
             for (hipEvent_t x : ev)
@@ -563,6 +573,7 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
         HIPCHK(G.hist.alloc(BATCH));
         HIPCHK(G.st.alloc(ns));
         G.ev.assign(2 * BATCH, nullptr);
+        G.hist_h.assign(BATCH, 0);
         for (hipEvent_t& x : G.ev) HIPCHK(hipEventCreate(&x));
         HIPCHK(hipMemsetAsync(G.st.p, 0, (size_t)ns * sizeof(SlotState), G.stream));  // PH_IDLE
         HIPCHK(hipMemsetAsync(G.sx.p, 0, (size_t)ns * sizeof(uint32_t), G.stream));
@@ -578,7 +589,7 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
         HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, c->nseg_total(), dpairs.p,
                              n_pairs, dnext.p, dout.p, dt, dlive.p, PairQueue{}, nullptr, G.stream));
     }
-    double sweep_ms = 0;
+    double sweep_ms = 0, exhaust_ms = INFINITY, end_ms = 0;
     uint64_t sweeps = 0, launched = 0;
     std::vector<std::pair<float, float>> spans;  // [start, end] of live sweeps, ms after ref
     // every pair needs at most 1 + sqp_max_iter * bls_max_iter evaluations
@@ -610,7 +621,7 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
         for (int g = 0; g < ngroups; g++) {
             Group& G = grp[g];
             if (G.live_before <= 0) continue;
-            HIPCHK(hipMemcpyAsync(G.hist_h, G.hist.p, BATCH * sizeof(int), hipMemcpyDeviceToHost, G.stream));
+            HIPCHK(hipMemcpyAsync(G.hist_h.data(), G.hist.p, BATCH * sizeof(int), hipMemcpyDeviceToHost, G.stream));
         }
         for (int g = 0; g < ngroups; g++)
             if (grp[g].live_before > 0) HIPCHK(hipStreamSynchronize(grp[g].stream));
@@ -626,6 +637,9 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
                     sweep_ms += ms;
                     sweeps++;
                     spans.emplace_back(t_start, t_end);
+                    // a slot goes idle only when no pair is left to claim
+                    if (G.hist_h[i] < (int)ns && t_end < exhaust_ms) exhaust_ms = t_end;
+                    end_ms = std::max<double>(end_ms, t_end);
                 }
                 G.live_before = G.hist_h[i];
             }
@@ -647,11 +661,102 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     if (cur_e > cur_s) busy += cur_e - cur_s;
     c->last_busy_ms = busy;
     c->last_groups = ngroups;
+    c->last_slots = ns;
+    c->last_exhaust_ms = std::isfinite(exhaust_ms) ? exhaust_ms : end_ms;
+    c->last_end_ms = end_ms;
     HIPCHK(hipMemcpyAsync(out, dout.p, n_pairs * sizeof(kestrel_output), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     c->last_sweep_ms = sweep_ms;
     c->last_sweeps = sweeps;
     c->last_wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return KESTREL_OK;
+}
+
+int kestrel_autotune(kestrel_cohort* c, const kestrel_tune* tune, const uint32_t* pairs, size_t n_pairs,
+                     uint64_t job_pairs, kestrel_autotune_result* res) {
+    if (!c || !tune || !pairs || !res) return fail(KESTREL_EINVAL, "null argument");
+    if (c->gt) return fail(KESTREL_EINVAL, "a GT cohort is fitted with kestrel_fit_pairs_gt");
+    if (c->serve) return fail(KESTREL_EINVAL, "stop the serving engine before tuning");
+    if (n_pairs < SWEEP_TPB) return fail(KESTREL_EINVAL, "need at least %d sample pairs", SWEEP_TPB);
+    if (job_pairs == 0) job_pairs = n_pairs;
+    const int64_t keep_slots = c->opt_slots, keep_streams = c->opt_streams, keep_batch = c->opt_batch;
+    std::vector<kestrel_output> out(n_pairs);
+    uint32_t trials = 0;
+    // Predicted milliseconds of a job of job_pairs pairs with one shape, from
+    // the better of two fits of the sample: while pairs are left to claim
+    // every slot is busy, so the sample's pairs completed by the time the
+    // first slot goes idle over that time is the shape's steady rate; the rest
+    // of the sample's fit (the drain of the last pairs in flight, and the
+    // fixed host work around the device time) is added as measured.
+    auto trial = [&](int64_t slots, int64_t streams, int64_t batch, double* ms) {
+        c->opt_slots = slots;
+        c->opt_streams = streams;
+        c->opt_batch = batch;
+        *ms = INFINITY;
+        for (int r = 0; r < 2; r++) {
+            const int rc = kestrel_fit_pairs(c, tune, pairs, n_pairs, 0, out.data());
+            if (rc) return rc;
+            trials++;
+            const double inflight = (double)c->last_slots * c->last_groups;
+            const double done = (double)n_pairs - inflight;
+            double predicted = c->last_wall_ms;
+            if (done > 0 && c->last_exhaust_ms > 0) {
+                const double rate = done / c->last_exhaust_ms;  // pairs per ms while every slot is busy
+                predicted = std::max(0.0, (double)job_pairs - inflight) / rate +
+                            (c->last_wall_ms - c->last_exhaust_ms);
+            }
+            *ms = std::min(*ms, predicted);
+        }
+        return KESTREL_OK;
+    };
+    auto restore = [&] {
+        c->opt_slots = keep_slots;
+        c->opt_streams = keep_streams;
+        c->opt_batch = keep_batch;
+    };
+    double ms0 = 0;
+    int rc = trial(keep_slots, keep_streams, keep_batch, &ms0);  // also warms the device up
+    if (!rc) rc = trial(keep_slots, keep_streams, keep_batch, &ms0);
+    if (rc) {
+        restore();
+        return rc;
+    }
+    // candidates: 1/8 to 3 device fills per group, one or two groups, 4 / 8 /
+    // 16 iterations per synchronisation; a shape is tried only when the
+    // sample holds at least 4 times its pairs in flight
+    const uint64_t base = default_slots(c);
+    double best_ms = ms0;
+    int64_t best[3] = {keep_slots, keep_streams, keep_batch};
+    for (const double fill : {0.125, 0.25, 0.5, 1.0, 2.0, 3.0}) {
+        const int64_t slots = (int64_t)std::max<uint64_t>(
+            SWEEP_TPB, ((uint64_t)(fill * (double)base) + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB);
+        for (const int64_t streams : {1, 2}) {
+            if ((uint64_t)slots * (uint64_t)streams * 4 > n_pairs) continue;
+            for (const int64_t batch : {4, 8, 16}) {
+                double ms = 0;
+                rc = trial(slots, streams, batch, &ms);
+                if (rc) {
+                    restore();
+                    return rc;
+                }
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best[0] = slots;
+                    best[1] = streams;
+                    best[2] = batch;
+                }
+            }
+        }
+    }
+    c->opt_slots = best[0];
+    c->opt_streams = best[1];
+    c->opt_batch = best[2];
+    res->slots = best[0];
+    res->streams = best[1];
+    res->batch = best[2];
+    res->pairs_per_s = (double)job_pairs / (best_ms * 1e-3);
+    res->default_pairs_per_s = (double)job_pairs / (ms0 * 1e-3);
+    res->trials = trials;
     return KESTREL_OK;
 }
 

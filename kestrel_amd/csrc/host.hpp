@@ -106,10 +106,13 @@ struct kestrel_cohort {
     int64_t opt_segment_sites = 4096;
     int64_t opt_serve_policy = 1;  // 0 round robin, 1 shortest remaining first
     int64_t opt_streams = 2;       // slot groups on separate streams in a fit
+    int64_t opt_batch = 8;         // sweep/step iterations queued per host synchronisation
     hipStream_t stream2 = nullptr; // second stream of a fit
     double last_sweep_ms = 0, last_wall_ms = 0, last_busy_ms = 0;
     uint64_t last_sweeps = 0;
     int last_groups = 1;
+    uint32_t last_slots = 0;                        // pairs in flight per group in the last fit
+    double last_exhaust_ms = 0, last_end_ms = 0;    // device ms until the first slot went idle / the last sweep
 
     // tri/tetra-allelic cohorts (kestrel_cohort_create_gl_multi)
     std::vector<std::unique_ptr<SitePart>> parts;

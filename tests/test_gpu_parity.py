@@ -152,6 +152,30 @@ def test_schedule_independence(O, medium):
     co.set_option("streams", 2)
     co.set_option("slots", 0)
     assert_bitexact(two, one)
+    # iterations queued per host synchronisation
+    for batch in (1, 3, 16):
+        co.set_option("batch", batch)
+        assert_bitexact(co.fit_pairs(many), two)
+    co.set_option("batch", 8)
+
+
+def test_autotune_sets_a_shape_that_keeps_results(medium):
+    """kestrel_autotune tries launch shapes on a sample, keeps the fastest;
+    the fit afterwards is bit-identical to the default shape's."""
+    c, co, codes, afk = medium
+    say("case: autotune")
+    rng = np.random.default_rng(11)
+    many = rng.integers(0, codes.shape[0], (4000, 2))
+    before = co.fit_pairs(many)
+    r = co.autotune(sample=30000)
+    assert r["trials"] >= 6 and r["pairs_per_s"] > 0 and r["default_pairs_per_s"] > 0
+    assert r["streams"] in (1, 2) and r["batch"] in (4, 8, 16)
+    assert_bitexact(co.fit_pairs(many), before)
+    co.set_option("slots", 0)
+    co.set_option("streams", 2)
+    co.set_option("batch", 8)
+    with pytest.raises(KestrelError):
+        co.autotune(pairs=np.zeros((10, 2), np.uint32))  # fewer than 256 sample pairs
 
 
 def test_pair_order_and_subset(O, medium):
