@@ -210,6 +210,7 @@ def main():
     ap.add_argument("--slots", type=int, default=None)
     ap.add_argument("--streams", type=int, default=None, help="slot groups on separate streams (1 or 2)")
     ap.add_argument("--batch", type=int, default=None, help="iterations queued per host synchronisation")
+    ap.add_argument("--graph", action="store_true", help="relaunch each batch from a captured HIP graph")
     ap.add_argument("--autotune", type=int, default=0, metavar="PAIRS",
                     help="before the warmup, tune slots/streams/batch on this many random pairs (0 = off)")
     ap.add_argument("--serving", action="store_true", help="serving workload (default config c5)")
@@ -250,6 +251,8 @@ def main():
         co.set_option("streams", args.streams)
     if args.batch:
         co.set_option("batch", args.batch)
+    if args.graph:
+        co.set_option("graph", 1)
     af, kept = co.allele_freqs()
     log(f"device cohort + allele frequencies in {time.perf_counter() - t:.2f}s, kept {co.n_kept} sites")
     tuned = None
@@ -315,7 +318,8 @@ def main():
                                + (f", soft likelihoods (mean depth {args.depth})" if args.soft else ""),
                    "samples": n, "sites": v, "pairs": offdiag, "precision": "fp64",
                    "input": "soft genotype likelihoods" if args.soft else "hard calls",
-                   "launch_shape": ({k: tuned[k] for k in ("slots", "streams", "batch")} if tuned else "default")},
+                   "launch_shape": ({k: tuned[k] for k in ("slots", "streams", "batch")} if tuned else "default"),
+                   "hip_graph": bool(args.graph)},
         "roofline": roofline, "cpu_baseline": cpu,
         "iterations_mean": round(float(out["iters"].mean()), 3),
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,

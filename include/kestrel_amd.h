@@ -184,7 +184,9 @@ int kestrel_cohort_create_gt_multi(kestrel_cohort** out, uint32_t n_samples, con
  * (1 or 2 slot groups on separate HIP streams, default 2: one group's fold and
  * step run beside the other's sweep; results do not change), "batch" (sweep
  * and step iterations queued per host synchronisation, 1 to 64, default 8;
- * results do not change), "serve_policy" (order in
+ * results do not change), "graph" (1: a fit captures each slot group's batch
+ * of sweeps and steps once into a HIP graph and relaunches it; timing stats
+ * are then per batch; results do not change), "serve_policy" (order in
  * which a serving engine started afterwards feeds admitted requests to the
  * device: 1 = shortest remaining first (default), 0 = round robin). */
 int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value);

@@ -435,6 +435,11 @@ int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value) {
         c->opt_streams = value;
         return KESTREL_OK;
     }
+    if (!strcmp(key, "graph")) {
+        if (value != 0 && value != 1) return fail(KESTREL_EINVAL, "graph must be 0 or 1");
+        c->opt_graph = value;
+        return KESTREL_OK;
+    }
     if (!strcmp(key, "batch")) {
         if (value < 1 || value > 64) return fail(KESTREL_EINVAL, "batch must be within 1 to 64");
         c->opt_batch = value;
@@ -547,9 +552,13 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
         std::vector<hipEvent_t> ev;
         std::vector<int> hist_h;
         int live_before = 0;
+        hipGraph_t graph = nullptr;     // option "graph": the BATCH sweeps and steps, captured once
+        hipGraphExec_t exec = nullptr;
         ~Group() {
             for (hipEvent_t x : ev)
                 if (x) (void)hipEventDestroy(x);
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
         }
     };
     Group grp[2];
@@ -589,6 +598,30 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
         HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, c->nseg_total(), dpairs.p,
                              n_pairs, dnext.p, dout.p, dt, dlive.p, PairQueue{}, nullptr, G.stream));
     }
+    // option "graph": each group's BATCH sweeps and steps are captured once
+    // into a HIP graph (kernel nodes only; the histogram is cleared outside
+    // it) and relaunched per synchronisation; timing is then per batch
+    const bool use_graph = c->opt_graph != 0;
+    if (use_graph) {
+        for (int g = 0; g < ngroups; g++) {
+            Group& G = grp[g];
+            HIPCHK(hipStreamBeginCapture(G.stream, hipStreamCaptureModeThreadLocal));
+            int rs = KESTREL_OK;
+            for (int i = 0; i < BATCH && rs == KESTREL_OK; i++) {
+                rs = launch_sweeps(c, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, tune->epsilon, G.part.p, G.stream);
+                if (rs == KESTREL_OK) {
+                    const hipError_t e = klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p,
+                                                       c->nseg_total(), dpairs.p, n_pairs, dnext.p, dout.p, dt,
+                                                       dlive.p, PairQueue{}, G.hist.p + i, G.stream);
+                    if (e != hipSuccess) rs = fail(KESTREL_EDEVICE, "HIP error %s", hipGetErrorString(e));
+                }
+            }
+            const hipError_t ec = hipStreamEndCapture(G.stream, &G.graph);
+            if (rs) return rs;
+            HIPCHK(ec);
+            HIPCHK(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+        }
+    }
     double sweep_ms = 0, exhaust_ms = INFINITY, end_ms = 0;
     uint64_t sweeps = 0, launched = 0;
     std::vector<std::pair<float, float>> spans;  // [start, end] of live sweeps, ms after ref
@@ -606,7 +639,14 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
         for (int g = 0; g < ngroups; g++)
             if (grp[g].live_before > 0)
                 HIPCHK(hipMemsetAsync(grp[g].hist.p, 0, BATCH * sizeof(int), grp[g].stream));
-        for (int i = 0; i < BATCH; i++)
+        for (int g = 0; g < ngroups && use_graph; g++) {
+            Group& G = grp[g];
+            if (G.live_before <= 0) continue;
+            HIPCHK(hipEventRecord(G.ev[0], G.stream));
+            HIPCHK(hipGraphLaunch(G.exec, G.stream));
+            HIPCHK(hipEventRecord(G.ev[1], G.stream));
+        }
+        for (int i = 0; i < BATCH && !use_graph; i++)
             for (int g = 0; g < ngroups; g++) {
                 Group& G = grp[g];
                 if (G.live_before <= 0) continue;
@@ -625,7 +665,25 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
         }
         for (int g = 0; g < ngroups; g++)
             if (grp[g].live_before > 0) HIPCHK(hipStreamSynchronize(grp[g].stream));
-        for (int g = 0; g < ngroups; g++) {
+        for (int g = 0; g < ngroups && use_graph; g++) {
+            Group& G = grp[g];
+            if (G.live_before <= 0) continue;
+            float ms = 0, t_start = 0, t_end = 0;
+            HIPCHK(hipEventElapsedTime(&ms, G.ev[0], G.ev[1]));
+            HIPCHK(hipEventElapsedTime(&t_start, ref, G.ev[0]));
+            HIPCHK(hipEventElapsedTime(&t_end, ref, G.ev[1]));
+            sweep_ms += ms;  // the whole batch: sweeps, folds and steps
+            spans.emplace_back(t_start, t_end);
+            end_ms = std::max<double>(end_ms, t_end);
+            for (int i = 0; i < BATCH; i++) {
+                if (G.live_before > 0) {
+                    sweeps++;
+                    if (G.hist_h[i] < (int)ns && t_end < exhaust_ms) exhaust_ms = t_end;
+                }
+                G.live_before = G.hist_h[i];
+            }
+        }
+        for (int g = 0; g < ngroups && !use_graph; g++) {
             Group& G = grp[g];
             if (G.live_before <= 0) continue;
             for (int i = 0; i < BATCH; i++) {

@@ -75,6 +75,12 @@ def test_multi_fit_bitexact_and_tolerance(O, multi):
     assert well.sum() > 100
     assert np.abs(out["jacquard"] - ref["jacquard"]).max(1)[well].max() <= 1e-6
     assert (np.abs(out["obj"] - ref["obj"]) / np.abs(ref["obj"]))[well].max() <= 1e-9
+    say("case: multi-allelic fit with the batch in a HIP graph")
+    co.set_option("graph", 1)
+    try:
+        assert_bitexact(co.fit_pairs(), out)
+    finally:
+        co.set_option("graph", 0)
 
 
 def test_multi_evaluate_bitexact(O, multi):

@@ -156,6 +156,18 @@ def test_schedule_independence(O, medium):
     for batch in (1, 3, 16):
         co.set_option("batch", batch)
         assert_bitexact(co.fit_pairs(many), two)
+    # the batch captured once per fit into a HIP graph and relaunched: every
+    # fit (many relaunches each) equals the eager one, with eager fits between
+    co.set_option("graph", 1)
+    for batch in (8, 3):
+        co.set_option("batch", batch)
+        assert_bitexact(co.fit_pairs(many), two)
+        co.set_option("graph", 0)
+        assert_bitexact(co.fit_pairs(many[:700]), two[:700])
+        co.set_option("graph", 1)
+    co.set_option("slots", 0)
+    assert_bitexact(co.fit_pairs(many), two)
+    co.set_option("graph", 0)
     co.set_option("batch", 8)
 
 
