@@ -10,7 +10,10 @@ cohort + allele frequencies to Output records in host memory.
 Prints one JSON line with the throughput, the roofline of the dominant kernel
 (the sweep: fused objective + gradient + Hessian passes, VALU FP64 bound) and a
 CPU baseline (the oracle's restatement of the reference CPU path, timed on a
-bounded sample of the same workload on this host's cores).
+bounded sample of the same workload on this host's cores).  Other modes:
+--serving (device-resident cohort answering query requests, p50/p99),
+--soft (likelihood input), --cli (the kestrel binary end to end on the cohort
+written as a VCF), --autotune / --graph / --batch (launch shape).
 """
 from __future__ import annotations
 
@@ -199,6 +202,62 @@ def serving(args):
     print(json.dumps(line), flush=True)
 
 
+def cli_end_to_end(args):
+    """The drop-in CLI on the workload's cohort written as a VCF: one-hot GL
+    records (the ML path, -f GL), BGZF, in a temporary directory; times the
+    kestrel binary (parse with the reader's threads, device frequencies and
+    fit, TSV write) and reports its phase line."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+
+    from kestrel_amd import synth
+
+    c = synth.config(args.config)
+    n, v = c.codes.shape
+    root = os.path.dirname(os.path.abspath(__file__))
+    tmp = tempfile.mkdtemp(prefix="kestrel_cli_")
+    try:
+        path = os.path.join(tmp, "cohort.vcf.gz")
+        t = time.perf_counter()
+        table = np.array(["0,-1000,-1000", "-1000,0,-1000", "-1000,-1000,0", ".,.,."])
+        with synth._BgzfWriter(path) as f:
+            f.write("##fileformat=VCFv4.2\n")
+            f.write("#CHROM\tPOS\tID\tREF\tALT\tQUAL\tFILTER\tINFO\tFORMAT\t" + "\t".join(c.names) + "\n")
+            buf = []
+            for s in range(v):
+                buf.append(f"1\t{s + 1}\t.\tA\tC\t.\tPASS\t.\tGL\t" + "\t".join(table[c.codes[:, s]]) + "\n")
+                if len(buf) == 2000:
+                    f.write("".join(buf))
+                    buf = []
+                if (s + 1) % 25000 == 0:
+                    log(f"VCF: {s + 1} of {v} sites written ({time.perf_counter() - t:.1f}s)")
+            f.write("".join(buf))
+        vcf_bytes = os.path.getsize(path)
+        tsv = os.path.join(tmp, "out.tsv")
+        threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
+        t = time.perf_counter()
+        r = subprocess.run([os.path.join(root, "kestrel_amd", "kestrel"), path, tsv, "-f", "GL", "-t", str(threads)],
+                           capture_output=True, text=True)
+        wall = time.perf_counter() - t
+        if r.returncode:
+            log(r.stderr)
+            sys.exit(r.returncode)
+        m = re.search(r"Phases: parse ([0-9.]+) s, device setup and fit ([0-9.]+) s, write ([0-9.]+) s", r.stderr)
+        rows = sum(1 for _ in open(tsv)) - 1
+        line = {"metric": "end-to-end CLI seconds (VCF in, TSV out)", "value": round(wall, 3), "unit": "s",
+                "n_gpus": 1, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+                "config": {"workload": f"{args.config}: {n} samples x {v} sites as a one-hot GL VCF (BGZF), "
+                                       f"kestrel -f GL -t {threads}", "vcf_bytes": vcf_bytes, "pairs": rows},
+                "phases_s": ({"parse": float(m.group(1)), "frequencies_and_fit": float(m.group(2)),
+                              "write": float(m.group(3))} if m else None),
+                "pair_fits_per_s_end_to_end": round((rows - n) / wall, 1)}
+        print(json.dumps(line), flush=True)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,6 +279,7 @@ def main():
     ap.add_argument("--load", type=float, default=0.7, help="Poisson offered load / saturated capacity")
     ap.add_argument("--policy", choices=("srpt", "rr"), default="srpt",
                     help="serving feed order: shortest remaining first or round robin")
+    ap.add_argument("--cli", action="store_true", help="end-to-end kestrel CLI run on the workload as a VCF")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -229,6 +289,9 @@ def main():
         if args.config == "c2":
             args.config = "c5"
         serving(args)
+        return
+    if args.cli:
+        cli_end_to_end(args)
         return
 
     from kestrel_amd import Cohort, synth

@@ -1014,18 +1014,20 @@ int init_gt2(kestrel_cohort* c, const uint8_t* packed, uint32_t n_sites, size_t 
 int init_gt_part(kestrel_cohort* c, int A, const uint8_t* idx, uint32_t n_in, const double* af,
                  std::vector<double>& af_all, uint32_t* n_kept) {
     const uint32_t n = c->n_samples, G = (uint32_t)(A * (A + 1) / 2);
-    std::vector<uint32_t> kept;
-    for (uint32_t s = 0; s < n_in; s++) {
-        bool ok = true;
-        for (uint32_t i = 0; i < n; i++) {
-            const uint8_t g = idx[(size_t)i * n_in + s];
-            if (g == 255) ok = false;
+    std::vector<uint8_t> complete(n_in, 1);
+    for (uint32_t i = 0; i < n; i++) {  // row by row (sample-major input)
+        const uint8_t* row = idx + (size_t)i * n_in;
+        for (uint32_t s = 0; s < n_in; s++) {
+            const uint8_t g = row[s];
+            if (g == 255) complete[s] = 0;
             else if (g >= G)
                 return fail(KESTREL_EINVAL, "genotype index %u of sample %u at %u-allele site %u exceeds %u", g, i, A,
                             s, G - 1);
         }
-        if (ok) kept.push_back(s);
     }
+    std::vector<uint32_t> kept;
+    for (uint32_t s = 0; s < n_in; s++)
+        if (complete[s]) kept.push_back(s);
     *n_kept = (uint32_t)kept.size();
     af_all.assign((size_t)n_in * A, std::nan(""));
     if (kept.empty()) return KESTREL_OK;

@@ -240,9 +240,11 @@ int main(int argc, char** argv) {
     }
     info(std::string("Kestrel ") + VERSION + " (" + kestrel_version() + ")");
     info("Parsing VCF \"" + a.input + "\"");
+    const auto t_start = std::chrono::steady_clock::now();
     kvcf::Data d;
     std::string err;
     if (!kvcf::read(a.input, a.field, a.af_info, d, err, a.threads > 0 ? (unsigned)a.threads : 0)) return die(err);
+    const auto t_parsed = std::chrono::steady_clock::now();
     info("Parsed " + std::to_string(d.total) + " total sites");
     info("Skipped " + std::to_string(d.not_snp) + " sites that were not SNPs");
     if (a.field == kvcf::Field::GT) info("Skipped " + std::to_string(d.skipped_missing) + " sites with missing data");
@@ -379,6 +381,7 @@ int main(int argc, char** argv) {
     }
     tracker.checkpoint();
     kestrel_cohort_destroy(c);
+    const auto t_fitted = std::chrono::steady_clock::now();
 
     // cli.rs:129-181
     FILE* f = fopen(a.output.c_str(), "w");
@@ -397,5 +400,12 @@ int main(int argc, char** argv) {
         fputs(row.c_str(), f);
     }
     if (fclose(f) != 0) return die("failed to write " + a.output);
+    auto secs = [](std::chrono::steady_clock::time_point x, std::chrono::steady_clock::time_point y) {
+        char b[32];
+        snprintf(b, sizeof b, "%.3f s", std::chrono::duration<double>(y - x).count());
+        return std::string(b);
+    };
+    info("Phases: parse " + secs(t_start, t_parsed) + ", device setup and fit " + secs(t_parsed, t_fitted) +
+         ", write " + secs(t_fitted, std::chrono::steady_clock::now()));
     return 0;
 }

@@ -550,8 +550,11 @@ std::vector<uint8_t> Data::sample_major(int alleles) const {
     const std::vector<uint8_t>& src = alleles == 3 ? site_codes3 : site_codes4;
     const size_t v = alleles == 3 ? n_sites3 : n_sites4;
     std::vector<uint8_t> out(n * v);
-    for (size_t s = 0; s < v; s++)
-        for (size_t i = 0; i < n; i++) out[i * v + s] = src[s * n + i];
+    constexpr size_t B = 64;  // transpose in 64 x 64 tiles
+    for (size_t s0 = 0; s0 < v; s0 += B)
+        for (size_t i0 = 0; i0 < n; i0 += B)
+            for (size_t s = s0; s < std::min(v, s0 + B); s++)
+                for (size_t i = i0; i < std::min(n, i0 + B); i++) out[i * v + s] = src[s * n + i];
     return out;
 }
 
