@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/kestrel_amd.h"
@@ -258,6 +259,7 @@ int main(int argc, char** argv) {
         // likelihood paths receive)
         FILE* f = fopen(a.dump_probs.c_str(), "wb");
         bool ok = f != nullptr;
+        d.join_probs();
         for (const std::vector<double>* v : {&d.probs, &d.probs3, &d.probs4})
             ok = ok && fwrite(v->data(), sizeof(double), v->size(), f) == v->size();
         if (f) fclose(f);
@@ -343,6 +345,7 @@ int main(int argc, char** argv) {
         // hard calls take the 2-bit path; any soft row, the likelihood path;
         // sites with 3 or 4 alleles, the multi-allelic likelihood path
         const bool multi = d.n_sites3 || d.n_sites4;
+        if (multi || !d.one_hot) d.join_probs();
         if (multi)
             rc = kestrel_cohort_create_gl_multi(&c, n, d.n_sites ? d.probs.data() : nullptr, d.n_sites,
                                                 d.n_sites3 ? d.probs3.data() : nullptr, d.n_sites3,
@@ -390,14 +393,46 @@ int main(int argc, char** argv) {
           "jacquard8\tjacquard9\titerations\tobjective\tcondition\n",
           f);
     static const double KIN[9] = {1.0, 0.0, 0.5, 0.0, 0.5, 0.0, 0.5, 0.25, 0.0};
-    std::string row;
-    for (const auto& o : out) {
-        double kin = 0.0;
-        for (int k = 0; k < 9; k++) kin = std::fma(o.jacquard[k], KIN[k], kin);  // algebra.rs dot
-        row = tsv_field(d.samples[o.x]) + "\t" + tsv_field(d.samples[o.y]) + "\t" + fmt_f64(kin);
-        for (int k = 0; k < 9; k++) row += "\t" + fmt_f64(o.jacquard[k]);
-        row += "\t" + std::to_string(o.iters) + "\t" + fmt_f64(o.obj) + "\t" + fmt_f64(std::log10(o.cond)) + "\n";
-        fputs(row.c_str(), f);
+    auto format_rows = [&](size_t lo, size_t hi, std::string& buf) {
+        for (size_t i = lo; i < hi; i++) {
+            const kestrel_output& o = out[i];
+            double kin = 0.0;
+            for (int k = 0; k < 9; k++) kin = std::fma(o.jacquard[k], KIN[k], kin);  // algebra.rs dot
+            buf += tsv_field(d.samples[o.x]);
+            buf += '\t';
+            buf += tsv_field(d.samples[o.y]);
+            buf += '\t';
+            buf += fmt_f64(kin);
+            for (int k = 0; k < 9; k++) {
+                buf += '\t';
+                buf += fmt_f64(o.jacquard[k]);
+            }
+            buf += '\t';
+            buf += std::to_string(o.iters);
+            buf += '\t';
+            buf += fmt_f64(o.obj);
+            buf += '\t';
+            buf += fmt_f64(std::log10(o.cond));
+            buf += '\n';
+        }
+    };
+    // rows are formatted in parallel chunks and written in order
+    const size_t nthreads = std::max<size_t>(1, a.threads > 0 ? (size_t)a.threads
+                                                              : (size_t)std::thread::hardware_concurrency());
+    const size_t chunk = 16384;
+    for (size_t base = 0; base < out.size(); base += chunk * nthreads) {
+        std::vector<std::string> bufs(nthreads);
+        std::vector<std::thread> pool;
+        for (size_t t = 0; t < nthreads; t++) {
+            const size_t lo = std::min(out.size(), base + t * chunk), hi = std::min(out.size(), lo + chunk);
+            if (lo < hi) pool.emplace_back(format_rows, lo, hi, std::ref(bufs[t]));
+        }
+        for (auto& th : pool) th.join();
+        for (const auto& b : bufs)
+            if (!b.empty() && fwrite(b.data(), 1, b.size(), f) != b.size()) {
+                fclose(f);
+                return die("failed to write " + a.output);
+            }
     }
     if (fclose(f) != 0) return die("failed to write " + a.output);
     auto secs = [](std::chrono::steady_clock::time_point x, std::chrono::steady_clock::time_point y) {

@@ -9,6 +9,7 @@
 
 #include <zlib.h>
 
+#include <charconv>
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
@@ -112,18 +113,36 @@ double parse_f32(const char* b, const char* e, bool* missing, bool* ok) {
     *missing = (e - b == 1 && *b == '.');
     *ok = true;
     if (*missing) return NAN;
-    char buf[64];
-    const size_t n = (size_t)(e - b);
-    if (n == 0 || n >= sizeof buf) {
+    const char* p = b;
+    if (p < e && *p == '+') p++;  // a leading '+' is accepted, as by Rust's f32 parser
+    float v = 0.0f;
+    const auto r = std::from_chars(p, e, v, std::chars_format::general);  // correctly rounded
+    if (p == e || r.ptr != e) {
         *ok = false;
         return 0.0;
     }
-    memcpy(buf, b, n);
-    buf[n] = 0;
-    char* end = nullptr;
-    float v = strtof(buf, &end);
-    if (end == buf || *end) *ok = false;
+    if (r.ec == std::errc::result_out_of_range) {  // overflow to inf, underflow to 0 (rare): strtof's result
+        char buf[64];
+        const size_t n = (size_t)(e - b);
+        if (n >= sizeof buf) {
+            *ok = false;
+            return 0.0;
+        }
+        memcpy(buf, b, n);
+        buf[n] = 0;
+        return (double)strtof(buf, nullptr);
+    }
+    if (r.ec != std::errc()) *ok = false;
     return (double)v;
+}
+
+// pow(10, x) with libm's results where they are fixed: exactly 1 at x = 0 (C
+// pow(x, +-0) = 1) and exactly 0 far below the smallest subnormal (10^-400);
+// one-hot likelihood rows (0 and -1000, PL 0 and 10000) take only these.
+inline double pow10(double x) {
+    if (x == 0.0) return 1.0;
+    if (x <= -400.0) return 0.0;
+    return std::pow(10.0, x);
 }
 
 // Decimal digits only (a GT allele), with an optional leading '+' as strtol
@@ -419,9 +438,9 @@ bool parse_record(const char* lb, const char* le, uint64_t recno, const Ctx& ctx
             if (field == Field::GL) {  // vcf.rs:349-368: normalise by the maximum GL
                 double mx = -INFINITY;
                 for (int k = 0; k < ng; k++) mx = std::fmax(mx, v[k]);
-                for (int k = 0; k < ng; k++) prob[k] = std::pow(10.0, v[k] - mx);
+                for (int k = 0; k < ng; k++) prob[k] = pow10(v[k] - mx);
             } else {  // vcf.rs:407
-                for (int k = 0; k < ng; k++) prob[k] = std::pow(10.0, -v[k] / 10.0);
+                for (int k = 0; k < ng; k++) prob[k] = pow10(-v[k] / 10.0);
             }
             for (int k = 0; k < ng; k++) site_probs[(size_t)ng * i + k] = prob[k];
             if (ng != 3) continue;  // 3 / 4 alleles: the likelihood rows are all the path reads
@@ -545,6 +564,17 @@ std::vector<uint8_t> Data::pack(size_t* row_bytes) const {
     return out;
 }
 
+void Data::join_probs() {
+    size_t total = probs.size();
+    for (const auto& p : probs_parts) total += p.size();
+    probs.reserve(total);
+    for (auto& p : probs_parts) {
+        probs.insert(probs.end(), p.begin(), p.end());
+        std::vector<double>().swap(p);
+    }
+    probs_parts.clear();
+}
+
 std::vector<uint8_t> Data::sample_major(int alleles) const {
     const size_t n = samples.size();
     const std::vector<uint8_t>& src = alleles == 3 ? site_codes3 : site_codes4;
@@ -598,7 +628,7 @@ bool read(const std::string& path, Field field, bool parse_af, Data& out, std::s
         out.multiallelic += b.multiallelic;
         out.one_hot = out.one_hot && b.one_hot;
         out.site_codes.insert(out.site_codes.end(), b.codes.begin(), b.codes.end());
-        out.probs.insert(out.probs.end(), b.probs.begin(), b.probs.end());
+        if (!b.probs.empty()) out.probs_parts.push_back(std::move(b.probs));  // joined on demand
         out.af.insert(out.af.end(), b.af.begin(), b.af.end());
         out.probs3.insert(out.probs3.end(), b.probs3.begin(), b.probs3.end());
         out.probs4.insert(out.probs4.end(), b.probs4.begin(), b.probs4.end());

@@ -15,7 +15,9 @@ struct Data {
     uint32_t n_sites = 0;
     std::vector<uint8_t> site_codes;  // n_sites x n_samples (site-major), 0/1/2 genotype index, 3 missing
     std::vector<double> af;           // n_sites x 2 when INFO AF was requested
-    std::vector<double> probs;        // GL / PL: n_sites x n_samples x 3 linear-scale likelihoods (NaN = missing)
+    std::vector<double> probs;        // GL / PL: n_sites x n_samples x 3 linear-scale likelihoods (NaN = missing),
+                                      // filled by join_probs()
+    std::vector<std::vector<double>> probs_parts;  // the same rows as read, batch by batch
     bool one_hot = true;              // GL / PL: every called row has one 1 and two 0
     // GL / PL sites with 3 and 4 alleles (vcf.rs:270-305 buckets), site-major
     // n_sites3 x n_samples x 6 and n_sites4 x n_samples x 10
@@ -29,6 +31,9 @@ struct Data {
 
     // 2-bit rows, sample-major, site s of a row in bits 2*(s%4) of byte s/4
     std::vector<uint8_t> pack(size_t* row_bytes) const;
+    // Moves probs_parts into probs (one copy; the 2-bit path of one-hot rows
+    // never needs it)
+    void join_probs();
     // The GT genotype indices of the 3- or 4-allele sites, sample-major (N x V_A)
     std::vector<uint8_t> sample_major(int alleles) const;
 };

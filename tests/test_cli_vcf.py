@@ -261,3 +261,20 @@ def test_gt_multiallelic_synthetic_roundtrip(tmp_path):
         keep = ~(b == 255).any(axis=0)
         exp = ["".join(str(v) for v in b[:, s]) for s in np.flatnonzero(keep)]
         assert got == exp, A
+
+
+def test_float_fields_parse_as_f32(tmp_path):
+    """INFO AF (and GL) values are f32 text (noodles' Float): a leading '+' is
+    accepted, values round correctly to f32, below f32's range give 0 and
+    above it inf; a malformed value is an error."""
+    body = ("1\t1\t.\tA\tG\t.\tPASS\tAF=+0.25\tGT\t0/0\t0/1\t1/1\n"
+            "1\t2\t.\tA\tG\t.\tPASS\tAF=1e-50\tGT\t0/0\t0/1\t1/1\n"
+            "1\t3\t.\tA\tG\t.\tPASS\tAF=1e39\tGT\t0/0\t0/1\t1/1\n"
+            "1\t4\t.\tA\tG\t.\tPASS\tAF=0.1\tGT\t0/0\t0/1\t1/1\n")
+    _, sites, _ = dump(write(tmp_path, body), af_info=True)
+    alt = [float(s[2]) for s in sites]
+    assert alt[0] == 0.25 and alt[1] == 0.0 and math.isinf(alt[2]) and alt[3] == float(np.float32(0.1))
+    assert [float(s[1]) for s in sites] == [0.75, 1.0, 0.0, 1.0 - float(np.float32(0.1))]
+    err = dump(write(tmp_path, "1\t1\t.\tA\tG\t.\tPASS\tAF=0.1x\tGT\t0/0\t0/1\t1/1\n", "b.vcf"), af_info=True,
+               expect_ok=False)
+    assert "Error reading AF" in err
