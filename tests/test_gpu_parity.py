@@ -50,7 +50,7 @@ def assert_within_tolerance(out, ref):
     assert np.abs(kinship(out["jacquard"]) - kinship(ref["jacquard"]))[well].max() <= TOL_KIN
     assert (np.abs(out["obj"] - ref["obj"]) / np.abs(ref["obj"]))[well].max() <= TOL_OBJ_REL
     lo = np.log10(out["cond"])
-    assert np.abs(lo - lc)[well].max() <= TOL_LOG10_COND
+    assert np.abs(lo[well] - lc[well]).max() <= TOL_LOG10_COND
 
 
 @pytest.fixture(scope="module")
