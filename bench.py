@@ -270,6 +270,7 @@ def main():
     ap.add_argument("--streams", type=int, default=None, help="slot groups on separate streams (1 or 2)")
     ap.add_argument("--batch", type=int, default=None, help="iterations queued per host synchronisation")
     ap.add_argument("--graph", action="store_true", help="relaunch each batch from a captured HIP graph")
+    ap.add_argument("--pair-tile", type=int, default=None, help="claim order in tiles of this many samples")
     ap.add_argument("--autotune", type=int, default=0, metavar="PAIRS",
                     help="before the warmup, tune slots/streams/batch on this many random pairs (0 = off)")
     ap.add_argument("--serving", action="store_true", help="serving workload (default config c5)")
@@ -316,6 +317,8 @@ def main():
         co.set_option("batch", args.batch)
     if args.graph:
         co.set_option("graph", 1)
+    if args.pair_tile is not None:
+        co.set_option("pair_tile", args.pair_tile)
     af, kept = co.allele_freqs()
     log(f"device cohort + allele frequencies in {time.perf_counter() - t:.2f}s, kept {co.n_kept} sites")
     tuned = None

@@ -186,7 +186,12 @@ int kestrel_cohort_create_gt_multi(kestrel_cohort** out, uint32_t n_samples, con
  * and step iterations queued per host synchronisation, 1 to 64, default 8;
  * results do not change), "graph" (1: a fit captures each slot group's batch
  * of sweeps and steps once into a HIP graph and relaunches it; timing stats
- * are then per batch; results do not change), "serve_policy" (order in
+ * are then per batch; results do not change), "pair_tile" (claim order: pairs
+ * are taken in tiles of this many samples so that few sample rows are in
+ * flight; 0 = list order, -1 = automatic: sqrt(pairs in flight), within 32
+ * to 256, for likelihood-row cohorts, list order for 2-bit ones; records stay
+ * in list order and do not change),
+ * "serve_policy" (order in
  * which a serving engine started afterwards feeds admitted requests to the
  * device: 1 = shortest remaining first (default), 0 = round robin). */
 int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value);

@@ -81,6 +81,23 @@ int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, con
     return KESTREL_OK;
 }
 
+// Claim order of a pair list in sample tiles: a stable counting sort by the
+// tiles (min(x, y) / T, max(x, y) / T) of its samples, tile pairs in row-major
+// order.  The pairs in flight at any time then come from a few tiles, so few
+// sample rows are streamed per sweep (it pays where rows are likelihoods).
+void tile_order(const uint32_t* pairs, size_t n, uint32_t n_samples, uint32_t T, std::vector<uint64_t>& order) {
+    const uint64_t nt = (n_samples + T - 1) / T;
+    std::vector<uint64_t> start(nt * nt + 1, 0);
+    auto key = [&](size_t i) {
+        const uint32_t a = std::min(pairs[2 * i], pairs[2 * i + 1]) / T, b = std::max(pairs[2 * i], pairs[2 * i + 1]) / T;
+        return (uint64_t)a * nt + b;
+    };
+    for (size_t i = 0; i < n; i++) start[key(i) + 1]++;
+    for (uint64_t k = 0; k < nt * nt; k++) start[k + 1] += start[k];
+    order.resize(n);
+    for (size_t i = 0; i < n; i++) order[start[key(i)]++] = i;
+}
+
 // Pairs in flight per slot group that fill the device: 2 blocks of 256 lanes
 // per CU over the cohort's segments.
 uint64_t default_slots(const kestrel_cohort* c) {
@@ -435,6 +452,11 @@ int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value) {
         c->opt_streams = value;
         return KESTREL_OK;
     }
+    if (!strcmp(key, "pair_tile")) {
+        if (value < -1 || value > (int64_t)1 << 30) return fail(KESTREL_EINVAL, "pair_tile must be -1, 0 or a tile size");
+        c->opt_pair_tile = value;
+        return KESTREL_OK;
+    }
     if (!strcmp(key, "graph")) {
         if (value != 0 && value != 1) return fail(KESTREL_EINVAL, "graph must be 0 or 1");
         c->opt_graph = value;
@@ -534,6 +556,24 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     HIPCHK(dout.alloc(n_pairs));
     hipStream_t st = c->stream;
     HIPCHK(hipMemcpyAsync(dpairs.p, pairs, 2 * n_pairs * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    // claim order (option "pair_tile"; automatic where the sweep streams
+    // likelihood rows: tiles of sqrt(pairs in flight) samples, so that the
+    // pairs in flight span about one tile pair (measured on C2 soft: 16 ->
+    // 245k, 64 -> 273k, 128 -> 276k, 256 -> 263k pair fits/s, list order 218k);
+    // list order for 2-bit rows, which stay in cache)
+    const int64_t auto_tile = std::min<int64_t>(
+        256, std::max<int64_t>(32, (int64_t)std::lround(std::sqrt((double)ns * ngroups))));
+    const int64_t tile =
+        c->opt_pair_tile >= 0 ? c->opt_pair_tile : (c->soft || !c->parts.empty() ? auto_tile : 0);
+    std::vector<uint64_t> order;
+    DevBuf<uint64_t> dorder;
+    const uint64_t* dorder_p = nullptr;
+    if (tile > 0 && n_pairs > ns) {
+        tile_order(pairs, n_pairs, c->n_samples, (uint32_t)tile, order);
+        HIPCHK(dorder.alloc(n_pairs));
+        HIPCHK(hipMemcpyAsync(dorder.p, order.data(), n_pairs * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+        dorder_p = dorder.p;
+    }
     HIPCHK(hipMemsetAsync(dnext.p, 0, sizeof(unsigned long long), st));
     const int live0 = (int)ns * ngroups;
     HIPCHK(hipMemcpyAsync(dlive.p, &live0, sizeof(int), hipMemcpyHostToDevice, st));
@@ -596,7 +636,7 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     for (int g = 0; g < ngroups; g++) {
         Group& G = grp[g];
         HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, c->nseg_total(), dpairs.p,
-                             n_pairs, dnext.p, dout.p, dt, dlive.p, PairQueue{}, nullptr, G.stream));
+                             n_pairs, dorder_p, dnext.p, dout.p, dt, dlive.p, PairQueue{}, nullptr, G.stream));
     }
     // option "graph": each group's BATCH sweeps and steps are captured once
     // into a HIP graph (kernel nodes only; the histogram is cleared outside
@@ -611,7 +651,8 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
                 rs = launch_sweeps(c, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, tune->epsilon, G.part.p, G.stream);
                 if (rs == KESTREL_OK) {
                     const hipError_t e = klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p,
-                                                       c->nseg_total(), dpairs.p, n_pairs, dnext.p, dout.p, dt,
+                                                       c->nseg_total(), dpairs.p, n_pairs, dorder_p, dnext.p,
+                                                       dout.p, dt,
                                                        dlive.p, PairQueue{}, G.hist.p + i, G.stream);
                     if (e != hipSuccess) rs = fail(KESTREL_EDEVICE, "HIP error %s", hipGetErrorString(e));
                 }
@@ -655,7 +696,7 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
                 if (rs) return rs;
                 HIPCHK(hipEventRecord(G.ev[2 * i + 1], G.stream));
                 HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, c->nseg_total(), dpairs.p,
-                                     n_pairs, dnext.p, dout.p, dt, dlive.p, PairQueue{}, G.hist.p + i, G.stream));
+                                     n_pairs, dorder_p, dnext.p, dout.p, dt, dlive.p, PairQueue{}, G.hist.p + i, G.stream));
             }
         launched += BATCH;
         for (int g = 0; g < ngroups; g++) {

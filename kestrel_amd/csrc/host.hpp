@@ -108,6 +108,7 @@ struct kestrel_cohort {
     int64_t opt_streams = 2;       // slot groups on separate streams in a fit
     int64_t opt_batch = 8;         // sweep/step iterations queued per host synchronisation
     int64_t opt_graph = 0;         // 1: the batch is captured once into a HIP graph per fit and relaunched
+    int64_t opt_pair_tile = -1;    // claim order in tiles of this many samples (0 list order, -1 automatic)
     hipStream_t stream2 = nullptr; // second stream of a fit
     double last_sweep_ms = 0, last_wall_ms = 0, last_busy_ms = 0;
     uint64_t last_sweeps = 0;

@@ -1048,6 +1048,7 @@ __device__ __forceinline__ void step_slot(SlotState* __restrict__ st, uint32_t* 
                                           int* __restrict__ active, uint32_t nslots,
                                           const double* __restrict__ partial, uint32_t nseg,
                                           const uint32_t* __restrict__ pairs, uint64_t npairs,
+                                          const uint64_t* __restrict__ order,
                                           unsigned long long* __restrict__ next_pair,
                                           kestrel_output* __restrict__ out, const Tune& tune,
                                           int* __restrict__ n_live, const PairQueue& q, uint32_t slot,
@@ -1188,9 +1189,13 @@ __device__ __forceinline__ void step_slot(SlotState* __restrict__ st, uint32_t* 
         } else {
             const unsigned long long idx = atomicAdd(next_pair, 1ull);
             if (idx < npairs) {
-                s.pair = (int64_t)idx;
-                px = pairs[2 * idx];
-                py = pairs[2 * idx + 1];
+                // the idx-th claim fits pair order[idx] (a claim order that
+                // keeps the sample rows in flight few); its record goes to its
+                // own position
+                const uint64_t p = order ? order[idx] : (uint64_t)idx;
+                s.pair = (int64_t)p;
+                px = pairs[2 * p];
+                py = pairs[2 * p + 1];
                 got = true;
             } else {
                 s.phase = PH_RETIRED;
@@ -1237,6 +1242,7 @@ __global__ __launch_bounds__(64) void step_kernel(SlotState* __restrict__ st, ui
                                                    int* __restrict__ active, uint32_t nslots,
                                                    const double* __restrict__ partial, uint32_t nseg,
                                                    const uint32_t* __restrict__ pairs, uint64_t npairs,
+                                                   const uint64_t* __restrict__ order,
                                                    unsigned long long* __restrict__ next_pair,
                                                    kestrel_output* __restrict__ out, Tune tune,
                                                    int* __restrict__ n_live, PairQueue q, int* __restrict__ live_out) {
@@ -1244,7 +1250,8 @@ __global__ __launch_bounds__(64) void step_kernel(SlotState* __restrict__ st, ui
     if (slot >= nslots) return;
     SlotState s = st[slot];
     if (s.phase == PH_RETIRED) return;
-    step_slot(st, sx, sy, pt, active, nslots, partial, nseg, pairs, npairs, next_pair, out, tune, n_live, q, slot, s);
+    step_slot(st, sx, sy, pt, active, nslots, partial, nseg, pairs, npairs, order, next_pair, out, tune, n_live, q, slot,
+              s);
     if (live_out) {
         // slots still holding or claiming pairs after this step, one atomic
         // per wave (retired lanes have returned above)
@@ -1798,7 +1805,7 @@ hipError_t gather_gl(const double* gl_site, uint32_t n_samples, const uint32_t* 
 }
 
 hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* active, uint32_t nslots,
-                double* partial, uint32_t nseg, const uint32_t* pairs, uint64_t npairs,
+                double* partial, uint32_t nseg, const uint32_t* pairs, uint64_t npairs, const uint64_t* order,
                 unsigned long long* next_pair, kestrel_output* out, const Tune& t, int* n_live, const PairQueue& q,
                 int* live_out, hipStream_t st) {
     if (nseg > 1) {
@@ -1807,7 +1814,7 @@ hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* act
                            active);
     }
     hipLaunchKernelGGL(step_kernel, dim3((nslots + 63) / 64), dim3(64), 0, st, stt, sx, sy, pt, active, nslots,
-                       partial, nseg, pairs, npairs, next_pair, out, t, n_live, q, live_out);
+                       partial, nseg, pairs, npairs, order, next_pair, out, t, n_live, q, live_out);
     return hipGetLastError();
 }
 

@@ -72,7 +72,7 @@ hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, con
                  uint32_t nslots, double eps, double* partial, hipStream_t st,
                  const double* gl = nullptr, uint32_t v_pad = 0);
 hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* active, uint32_t nslots,
-                double* partial, uint32_t nseg, const uint32_t* pairs, uint64_t npairs,
+                double* partial, uint32_t nseg, const uint32_t* pairs, uint64_t npairs, const uint64_t* order,
                 unsigned long long* next_pair, kestrel_output* out, const kdev::Tune& t, int* n_live,
                 const PairQueue& q, int* live_out, hipStream_t st);
 hipError_t selftest(uint64_t n, uint64_t seed, unsigned long long* bad, hipStream_t st);

@@ -273,7 +273,7 @@ int engine_round(ServeState* S, bool* idle) {
                               S->active.p, S->ns, S->tune.epsilon, S->partial.p, S->stream));
     const PairQueue q{S->ring.p, S->ring_cap, ctrl_tail(S), ctrl_done(S), ctrl_busy(S)};
     HIPCHK(klaunch::step(S->slots.p, S->sx.p, S->sy.p, S->pt.p, S->active.p, S->ns, S->partial.p, S->nseg, nullptr,
-                         0, ctrl_next(S), S->outpool.p, S->dt, nullptr, q, nullptr, S->stream));
+                         0, nullptr, ctrl_next(S), S->outpool.p, S->dt, nullptr, q, nullptr, S->stream));
     uint32_t max_slot = 0;
     for (Request* r : live) max_slot = std::max(max_slot, r->slot + 1);
     HIPCHK(hipMemcpyAsync(S->hctrl + 8, S->ctrl.p + 8, CTRL_DONE_OFF - 8 + 4 * (size_t)max_slot,

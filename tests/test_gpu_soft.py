@@ -50,6 +50,29 @@ def test_one_hot_rows_equal_the_2bit_path():
         assert_bitexact(a.fit_pairs(), b.fit_pairs())
 
 
+def test_claim_order_tiles_keep_records(soft):
+    """Pairs claimed in sample tiles (option pair_tile, automatic for
+    likelihood rows) give the same records, in list order, as list order."""
+    c, gl, co = soft
+    say("case: soft fit, claim order in tiles of 64 / 5 samples vs list order")
+    co.set_option("slots", 256)  # fewer slots than pairs: the claim order matters
+    try:
+        co.set_option("pair_tile", 0)
+        base = co.fit_pairs()
+        for tile in (-1, 5):
+            co.set_option("pair_tile", tile)
+            assert_bitexact(co.fit_pairs(), base)
+        rng = np.random.default_rng(3)
+        pairs = rng.integers(0, 64, (3000, 2))
+        co.set_option("pair_tile", 0)
+        lst = co.fit_pairs(pairs)
+        co.set_option("pair_tile", 8)
+        assert_bitexact(co.fit_pairs(pairs), lst)
+    finally:
+        co.set_option("pair_tile", -1)
+        co.set_option("slots", 0)
+
+
 def test_soft_allele_freqs_bitexact(O, soft):
     c, gl, co = soft
     say("case: soft allele frequencies 64 x 9000")
