@@ -258,6 +258,71 @@ def cli_end_to_end(args):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def cls_bench(args):
+    """The CLS / Brier estimator of -f GT (kestrel_fit_pairs_gt) on the
+    workload's sizes without missing calls (the GT path drops every site with
+    a missing call, so the workload's 1 % missingness would leave no site).
+    One step = one fit of every pair from the device-resident cohort; the
+    9 x 9 Gram matrix is built once per cohort (in the warmup)."""
+
+    from kestrel_amd import Cohort, synth
+
+    t = time.perf_counter()
+    c = synth.config(args.config, missing=0.0)
+    n, v = c.codes.shape
+    log(f"cohort {args.config} without missing calls: {n} x {v} in {time.perf_counter() - t:.1f}s")
+    co = Cohort.gt(c.codes)
+    for i in range(args.warmup):
+        t = time.perf_counter()
+        co.fit_pairs_gt()
+        log(f"warmup {i}: {time.perf_counter() - t:.3f}s")
+    times, kms = [], 0.0
+    out = None
+    for i in range(args.steps):
+        t = time.perf_counter()
+        out, _ = co.fit_pairs_gt()
+        times.append(time.perf_counter() - t)
+        kms += co.last_fit_stats()["sweep_ms"]
+        log(f"step {i}: {times[-1]:.3f}s, per-pair kernels {co.last_fit_stats()['sweep_ms']:.1f} ms")
+    total = float(np.sum(times))
+    offdiag = int((out["x"] != out["y"]).sum())
+    value = offdiag * args.steps / total
+    # c_xy = sum over sites of m_s[gx][gy]: 9 FP64 adds and an 80-byte LDS row per pair and site
+    pair_sites = float(len(out)) * v * args.steps
+    lds_tbs = pair_sites * 80 / (kms * 1e-3) / 1e12
+    cpu = None
+    if args.cpu_budget > 0:  # the oracle's CLS (reference order) on a sample of pairs
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        rng = np.random.default_rng(0xC15)
+        m = 2000
+        pairs = np.stack([rng.integers(0, n, m), rng.integers(0, n, m)], 1).astype(np.uint32)
+        pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+        threads = cpu_threads()
+        t = time.perf_counter()
+        O.cls_fit_pairs(c.codes, pairs=pairs, nthreads=threads)
+        dt = time.perf_counter() - t
+        cpu = {"value": len(pairs) / dt, "unit": "pair-fits/s", "cores": threads, "kind": "port",
+               "sample": f"{len(pairs)} random off-diagonal pairs of the same cohort, {dt:.1f} s"}
+        log(f"cpu baseline: {cpu}")
+    line = {
+        "metric": "pair-fits/sec (CLS, -f GT)", "value": round(value, 1), "unit": "pair-fits/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(total / args.steps * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.config} without missing calls: {n} samples x {v} sites, {offdiag} "
+                               f"off-diagonal pairs (+{len(out) - offdiag} self pairs), CLS / Brier estimator",
+                   "samples": n, "sites": v, "pairs": offdiag},
+        "roofline": {"bound": "lds", "achieved": round(lds_tbs, 3), "peak": 78.6, "unit": "TB/s",
+                     "frac": round(lds_tbs / 78.6, 4), "traffic": None, "kernel": "cls_pairs_kernel",
+                     "kernel_ms_per_step": round(kms / args.steps, 2),
+                     "note": "LDS bytes: one 80-byte m row per pair and site; peak 128 B/clk/CU x 256 CUs x 2.4 GHz"},
+        "cpu_baseline": cpu, "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+    }
+    co.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -281,6 +346,7 @@ def main():
     ap.add_argument("--policy", choices=("srpt", "rr"), default="srpt",
                     help="serving feed order: shortest remaining first or round robin")
     ap.add_argument("--cli", action="store_true", help="end-to-end kestrel CLI run on the workload as a VCF")
+    ap.add_argument("--cls", action="store_true", help="the CLS / Brier estimator of -f GT")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -293,6 +359,9 @@ def main():
         return
     if args.cli:
         cli_end_to_end(args)
+        return
+    if args.cls:
+        cls_bench(args)
         return
 
     from kestrel_amd import Cohort, synth

@@ -216,7 +216,8 @@ int kestrel_autotune(kestrel_cohort* c, const kestrel_tune* tune, const uint32_t
                      uint64_t job_pairs, kestrel_autotune_result* res);
 
 /* Timing of the last kestrel_fit_pairs: sweep-kernel milliseconds (sum of
- * launch durations by HIP events), number of sweeps, wall milliseconds. */
+ * launch durations by HIP events), number of sweeps, wall milliseconds.  After
+ * kestrel_fit_pairs_gt: the milliseconds of its per-pair kernels, 1, wall. */
 int kestrel_last_fit_stats(const kestrel_cohort* c, double* sweep_ms, uint64_t* sweeps, double* wall_ms);
 
 /* The last fit's sweep-busy time (milliseconds with at least one sweep

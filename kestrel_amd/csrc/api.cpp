@@ -1217,6 +1217,18 @@ int kestrel_fit_pairs_gt(kestrel_cohort* c, uint64_t iterations, const uint32_t*
     HIPCHK(dpairs.alloc(2 * n_pairs));
     HIPCHK(dout.alloc(n_pairs));
     HIPCHK(hipMemcpyAsync(dpairs.p, pairs, 2 * n_pairs * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    struct EvGuard {
+        hipEvent_t& a;
+        hipEvent_t& b;
+        ~EvGuard() {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } evg{e0, e1};
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, st));
     if (p3 || p4) {
         HIPCHK(c34.alloc(n_pairs * 9));
         if (p3)
@@ -1227,11 +1239,16 @@ int kestrel_fit_pairs_gt(kestrel_cohort* c, uint64_t iterations, const uint32_t*
     }
     HIPCHK(klaunch::cls_pairs(c->geno.p, c->row_words, c->af.p, c->n_kept, c34.p, v_total, c->qmat.p, dpairs.p,
                               n_pairs, to_dev(t), dout.p, st));
+    HIPCHK(hipEventRecord(e1, st));
     HIPCHK(hipMemcpyAsync(out, dout.p, n_pairs * sizeof(kestrel_output), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    float kms = 0;
+    HIPCHK(hipEventElapsedTime(&kms, e0, e1));
     c->last_wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     c->last_sweeps = 1;
-    c->last_sweep_ms = 0;
+    c->last_sweep_ms = kms;  // the per-pair CLS kernels (c3/c4 and cls_pairs_kernel)
+    c->last_busy_ms = kms;
+    c->last_groups = 1;
     return KESTREL_OK;
 }
 
