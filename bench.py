@@ -29,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_VALU_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
+LDS_PEAK_TBS = 157.3  # gfx950 LDS reads: 256 B/clk/CU x 256 CUs x 2.4 GHz
 FLOP_PER_SITE_EVAL = 156  # 136 (gradient + Hessian) + 20 (objective, log counted as 1)
 # HBM bytes per sweep_fp64 launch on the default workload, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE scaled by a plain 256 MiB copy (profiles/sweep_fp64_c2.md)
@@ -290,6 +291,11 @@ def cls_bench(args):
     # c_xy = sum over sites of m_s[gx][gy]: 9 FP64 adds and an 80-byte LDS row per pair and site
     pair_sites = float(len(out)) * v * args.steps
     lds_tbs = pair_sites * 80 / (kms * 1e-3) / 1e12
+    import ctypes
+
+    from kestrel_amd import _lib
+    measured = ctypes.c_double()
+    _lib.check(_lib.load().kestrel_measure_lds(ctypes.byref(measured)))
     cpu = None
     if args.cpu_budget > 0:  # the oracle's CLS (reference order) on a sample of pairs
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -313,10 +319,12 @@ def cls_bench(args):
         "config": {"workload": f"{args.config} without missing calls: {n} samples x {v} sites, {offdiag} "
                                f"off-diagonal pairs (+{len(out) - offdiag} self pairs), CLS / Brier estimator",
                    "samples": n, "sites": v, "pairs": offdiag},
-        "roofline": {"bound": "lds", "achieved": round(lds_tbs, 3), "peak": 78.6, "unit": "TB/s",
-                     "frac": round(lds_tbs / 78.6, 4), "traffic": None, "kernel": "cls_pairs_kernel",
-                     "kernel_ms_per_step": round(kms / args.steps, 2),
-                     "note": "LDS bytes: one 80-byte m row per pair and site; peak 128 B/clk/CU x 256 CUs x 2.4 GHz"},
+        "roofline": {"bound": "lds", "achieved": round(lds_tbs, 3), "peak": LDS_PEAK_TBS, "unit": "TB/s",
+                     "frac": round(lds_tbs / LDS_PEAK_TBS, 4), "traffic": None, "kernel": "cls_pairs_kernel",
+                     "kernel_ms_per_step": round(kms / args.steps, 2), "measured_peak": round(measured.value, 2),
+                     "frac_of_measured": round(lds_tbs / measured.value, 4),
+                     "note": "LDS bytes: one 80-byte m row per pair and site; peak 256 B/clk/CU x 256 CUs x 2.4 GHz; "
+                             "measured: conflict-free 16-byte reads (kestrel_measure_lds)"},
         "cpu_baseline": cpu, "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
     }
     co.close()

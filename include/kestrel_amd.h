@@ -293,6 +293,11 @@ int kestrel_measure_fp64_peak(double* tflops);
  * (no counterpart in the reference). */
 int kestrel_measure_fp32_peak(double* tflops);
 
+/* Achievable LDS read bandwidth of this device in TB/s (no counterpart in the
+ * reference): conflict-free 16-byte reads by every lane, the bound of the CLS
+ * per-pair kernel. */
+int kestrel_measure_lds(double* tbs);
+
 /* Plain device copy of `bytes` (two launches, the second timed): HBM
  * read+write GB/s (no counterpart in the reference; the reference kernel for
  * scaling hardware byte counters). */

@@ -61,6 +61,7 @@ EXPORTS = (
     "kestrel_serve_start", "kestrel_submit_queries", "kestrel_poll", "kestrel_wait", "kestrel_serve_get_stats",
     "kestrel_serve_stop", "kestrel_last_fit_busy", "kestrel_cohort_create_gl", "kestrel_cohort_create_gl_multi",
     "kestrel_cohort_multi_info", "kestrel_cohort_create_gt_multi", "kestrel_measure_fp32_peak", "kestrel_autotune",
+    "kestrel_measure_lds",
 )
 
 
@@ -125,6 +126,7 @@ def load():
     L.kestrel_fit_pairs_gt.argtypes = [P, ctypes.c_uint64, P, ctypes.c_size_t, ctypes.c_uint32, P, P]
     L.kestrel_measure_fp64_peak.argtypes = [P]
     L.kestrel_measure_fp32_peak.argtypes = [P]
+    L.kestrel_measure_lds.argtypes = [P]
     L.kestrel_autotune.argtypes = [P, P, P, ctypes.c_size_t, ctypes.c_uint64, P]
     L.kestrel_measure_copy.argtypes = [ctypes.c_uint64, P]
     L.kestrel_serve_start.argtypes = [P, P, ctypes.c_uint32, ctypes.c_uint64]

@@ -957,6 +957,17 @@ int kestrel_measure_fp32_peak(double* tflops) {
         [](uint32_t b, uint32_t it, float* s) { return klaunch::calib_fp32(b, it, s, nullptr); }, tflops);
 }
 
+int kestrel_measure_lds(double* tbs) {
+    // the FMA timer's launch shape; 8 loads of 16 bytes per lane and step
+    double per = 0;
+    const int rc = measure_fma<double>(
+        [](uint32_t b, uint32_t it, double* s) { return klaunch::calib_lds(b, it, s, nullptr); }, &per);
+    if (rc) return rc;
+    // measure_fma counts 16 chains x 2 FLOP per lane and step; this kernel moves 8 x 16 bytes
+    *tbs = per * (8.0 * 16.0) / (16.0 * 2.0);
+    return KESTREL_OK;
+}
+
 int kestrel_measure_copy(uint64_t bytes, double* gbps) {
     if (!gbps || bytes < 16) return fail(KESTREL_EINVAL, "bad argument");
     int ndev = 0;

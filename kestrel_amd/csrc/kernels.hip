@@ -1322,6 +1322,33 @@ __global__ __launch_bounds__(256) void calib_fma_kernel(uint32_t iters, T a, T b
     if (s == (T)1.2345) sink[0] = s;  // keeps the chains live
 }
 
+// LDS read bandwidth: each lane reads 16 bytes per load, a wave 1 KB of
+// consecutive addresses (no bank conflicts); the 8 loads of a step read rows
+// that change every step, so none can be hoisted out of the loop.
+__global__ __launch_bounds__(256) void calib_lds_kernel(uint32_t iters, double* __restrict__ sink) {
+    __shared__ __attribute__((aligned(16))) double buf[8192];  // 64 KB: 64 rows of 64 double2
+    for (uint32_t i = threadIdx.x; i < 8192; i += 256) buf[i] = (double)i;
+    __syncthreads();
+    const double2* b2 = reinterpret_cast<const double2*>(buf);
+    const uint32_t lane = threadIdx.x & 63u;
+    double2 acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc[k] = make_double2(0.0, 0.0);
+    for (uint32_t it = 0; it < iters; it++) {
+        const uint32_t row = (it * 13u) & 63u;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const double2 v = b2[((row + 8u * k) & 63u) * 64u + lane];
+            acc[k].x += v.x;
+            acc[k].y += v.y;
+        }
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) s += acc[k].x + acc[k].y;
+    if (s == 1.2345) sink[0] = s;  // keeps the loads live
+}
+
 // Plain copy of n16 16-byte words (the reference kernel for HBM byte counters).
 __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                     uint64_t n16) {
@@ -1820,6 +1847,11 @@ hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* act
 
 hipError_t selftest(uint64_t n, uint64_t seed, unsigned long long* bad, hipStream_t st) {
     hipLaunchKernelGGL(selftest_kernel, dim3(4096), dim3(256), 0, st, n, seed, bad);
+    return hipGetLastError();
+}
+
+hipError_t calib_lds(uint32_t blocks, uint32_t iters, double* sink, hipStream_t st) {
+    hipLaunchKernelGGL(calib_lds_kernel, dim3(blocks), dim3(256), 0, st, iters, sink);
     return hipGetLastError();
 }
 

@@ -78,6 +78,7 @@ hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* act
 hipError_t selftest(uint64_t n, uint64_t seed, unsigned long long* bad, hipStream_t st);
 hipError_t calib_fp64(uint32_t blocks, uint32_t iters, double* sink, hipStream_t st);
 hipError_t calib_fp32(uint32_t blocks, uint32_t iters, float* sink, hipStream_t st);
+hipError_t calib_lds(uint32_t blocks, uint32_t iters, double* sink, hipStream_t st);
 hipError_t copy(const void* src, void* dst, uint64_t n16, hipStream_t st);
 hipError_t complete_sites(const uint8_t* packed, size_t row_bytes, uint32_t n_samples, uint32_t n_sites, uint8_t* flag,
                           hipStream_t st);
