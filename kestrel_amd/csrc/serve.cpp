@@ -99,12 +99,23 @@ struct ServeState {
     int policy = 1;  // 0 round robin, 1 shortest remaining first
     size_t rb = 0;  // bytes of one query row over n_sites
 
-    DevBuf<uint32_t> geno, kept, seg, qrows, ring, sx, sy;
+    DevBuf<uint32_t> geno, kept, seg, qrows, ring;
     DevBuf<uint8_t> qstage, ctrl;
     DevBuf<kestrel_output> outpool;
-    DevBuf<SlotState> slots;
-    DevBuf<double> pt, partial;
-    DevBuf<int> active;
+    // slot groups: each sweeps and steps its own slots on its own stream,
+    // all claim from the one ring (as the slot groups of a batch fit)
+    struct Group {
+        hipStream_t stream = nullptr;  // group 0 uses ServeState::stream
+        DevBuf<uint32_t> sx, sy;
+        DevBuf<SlotState> slots;
+        DevBuf<double> pt, partial;
+        DevBuf<int> active;
+    };
+    Group grp[2];
+    int ngroups = 1;
+    int iters = 2;                 // sweep + step iterations per engine round
+    hipStream_t stream2 = nullptr;
+    hipEvent_t fed = nullptr, swept = nullptr;
     uint8_t* hctrl = nullptr;     // pinned read-back of ctrl
     uint32_t* hent = nullptr;     // pinned staging of appended entries
     uint64_t device_bytes = 0;
@@ -131,7 +142,10 @@ struct ServeState {
     ~ServeState() {
         if (hctrl) (void)hipHostFree(hctrl);
         if (hent) (void)hipHostFree(hent);
+        if (fed) (void)hipEventDestroy(fed);
+        if (swept) (void)hipEventDestroy(swept);
         if (stream) (void)hipStreamDestroy(stream);
+        if (stream2) (void)hipStreamDestroy(stream2);
     }
 };
 
@@ -268,12 +282,26 @@ int engine_round(ServeState* S, bool* idle) {
     }
     int rc = feed(S, live);
     if (rc) return rc;
-    if (S->busy > 0)
-        HIPCHK(klaunch::sweep(S->geno.p, S->row_words, S->c->af.p, S->seg.p, S->nseg, S->sx.p, S->sy.p, S->pt.p,
-                              S->active.p, S->ns, S->tune.epsilon, S->partial.p, S->stream));
+    // the other group's stream starts after the ring and its tail are written
+    if (S->ngroups == 2) {
+        HIPCHK(hipEventRecord(S->fed, S->stream));
+        HIPCHK(hipStreamWaitEvent(S->grp[1].stream, S->fed, 0));
+    }
     const PairQueue q{S->ring.p, S->ring_cap, ctrl_tail(S), ctrl_done(S), ctrl_busy(S)};
-    HIPCHK(klaunch::step(S->slots.p, S->sx.p, S->sy.p, S->pt.p, S->active.p, S->ns, S->partial.p, S->nseg, nullptr,
-                         0, nullptr, ctrl_next(S), S->outpool.p, S->dt, nullptr, q, nullptr, S->stream));
+    const bool sweep = S->busy > 0 || S->tail > S->consumed;
+    for (int it = 0; it < S->iters; it++)
+        for (int g = 0; g < S->ngroups; g++) {
+            ServeState::Group& G = S->grp[g];
+            if (sweep)
+                HIPCHK(klaunch::sweep(S->geno.p, S->row_words, S->c->af.p, S->seg.p, S->nseg, G.sx.p, G.sy.p, G.pt.p,
+                                      G.active.p, S->ns, S->tune.epsilon, G.partial.p, G.stream));
+            HIPCHK(klaunch::step(G.slots.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, S->ns, G.partial.p, S->nseg, nullptr,
+                                 0, nullptr, ctrl_next(S), S->outpool.p, S->dt, nullptr, q, nullptr, G.stream));
+        }
+    if (S->ngroups == 2) {
+        HIPCHK(hipEventRecord(S->swept, S->grp[1].stream));
+        HIPCHK(hipStreamWaitEvent(S->stream, S->swept, 0));
+    }
     uint32_t max_slot = 0;
     for (Request* r : live) max_slot = std::max(max_slot, r->slot + 1);
     HIPCHK(hipMemcpyAsync(S->hctrl + 8, S->ctrl.p + 8, CTRL_DONE_OFF - 8 + 4 * (size_t)max_slot,
@@ -294,9 +322,9 @@ int engine_round(ServeState* S, bool* idle) {
     }
     {
         std::lock_guard<std::mutex> lk(S->mu);
-        if (was_busy > 0) {
-            S->st_sweeps++;
-            S->st_busy += (uint64_t)was_busy;
+        if (was_busy > 0) {  // busy slots as seen at the round's end, for each of its sweeps
+            S->st_sweeps += (uint64_t)S->iters;
+            S->st_busy += (uint64_t)was_busy * (uint64_t)S->iters;
         }
         const auto now = Clock::now();
         for (Request* r : finished) {
@@ -413,16 +441,19 @@ int kestrel_serve_start(kestrel_cohort* c, const kestrel_tune* tune, uint32_t ma
     S->rb = (c->n_sites + 3) / 4;
     HIPCHK(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
 
-    // slots as kestrel_fit_pairs: the resident sweep blocks over the segments
+    // slots as kestrel_fit_pairs: the resident sweep blocks over the segments,
+    // per group; two groups on two streams unless option "streams" is 1
     const uint64_t resident = (uint64_t)std::max(1, c->n_cu) * 2;
     S->nseg = c->nseg;
     const uint64_t slot_blocks = std::max<uint64_t>(1, resident / S->nseg);
     uint64_t ns = c->opt_slots > 0 ? (uint64_t)c->opt_slots : slot_blocks * SWEEP_TPB;
     ns = (ns + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB;
     S->ns = (uint32_t)ns;
-    // a backlog of a quarter of the slots refills finished slots every step
-    // while keeping a newly admitted request close to the front
-    S->backlog = std::max<uint64_t>(FEED_CHUNK, ns / 4);
+    S->ngroups = c->opt_streams >= 2 ? 2 : 1;
+    // a backlog of a quarter of the slots per sweep of a round refills
+    // finished slots every step while keeping a newly admitted request close
+    // to the front
+    S->backlog = std::max<uint64_t>(FEED_CHUNK, ns / 4 * S->ngroups * S->iters);
     S->ring_cap = 2 * S->backlog;
 
     const size_t rows = (size_t)c->n_samples + max_query_rows;
@@ -441,23 +472,32 @@ int kestrel_serve_start(kestrel_cohort* c, const kestrel_tune* tune, uint32_t ma
     HIPCHK(S->ring.alloc(4 * S->ring_cap));
     HIPCHK(S->outpool.alloc(max_pairs));
     HIPCHK(S->ctrl.alloc(CTRL_DONE_OFF + 4 * (size_t)MAX_REQ));
-    HIPCHK(S->slots.alloc(ns));
-    HIPCHK(S->sx.alloc(ns));
-    HIPCHK(S->sy.alloc(ns));
-    HIPCHK(S->pt.alloc(9 * ns));
-    HIPCHK(S->partial.alloc((size_t)S->nseg * NPART * ns));
-    HIPCHK(S->active.alloc(ns));
     HIPCHK(hipMemsetAsync(S->ctrl.p, 0, S->ctrl.n, S->stream));
-    HIPCHK(hipMemsetAsync(S->slots.p, 0, ns * sizeof(SlotState), S->stream));  // PH_IDLE
-    HIPCHK(hipMemsetAsync(S->sx.p, 0, ns * sizeof(uint32_t), S->stream));
-    HIPCHK(hipMemsetAsync(S->sy.p, 0, ns * sizeof(uint32_t), S->stream));
-    HIPCHK(hipMemsetAsync(S->active.p, 0, ns * sizeof(int), S->stream));
+    if (S->ngroups == 2) {
+        HIPCHK(hipStreamCreateWithFlags(&S->stream2, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&S->fed, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&S->swept, hipEventDisableTiming));
+    }
+    for (int g = 0; g < S->ngroups; g++) {
+        ServeState::Group& G = S->grp[g];
+        G.stream = g == 0 ? S->stream : S->stream2;
+        HIPCHK(G.slots.alloc(ns));
+        HIPCHK(G.sx.alloc(ns));
+        HIPCHK(G.sy.alloc(ns));
+        HIPCHK(G.pt.alloc(9 * ns));
+        HIPCHK(G.partial.alloc((size_t)S->nseg * NPART * ns));
+        HIPCHK(G.active.alloc(ns));
+        HIPCHK(hipMemsetAsync(G.slots.p, 0, ns * sizeof(SlotState), S->stream));  // PH_IDLE
+        HIPCHK(hipMemsetAsync(G.sx.p, 0, ns * sizeof(uint32_t), S->stream));
+        HIPCHK(hipMemsetAsync(G.sy.p, 0, ns * sizeof(uint32_t), S->stream));
+        HIPCHK(hipMemsetAsync(G.active.p, 0, ns * sizeof(int), S->stream));
+    }
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&S->hctrl), S->ctrl.n, hipHostMallocDefault));
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&S->hent), S->backlog * 16, hipHostMallocDefault));
     HIPCHK(hipStreamSynchronize(S->stream));
     S->device_bytes = rows * S->row_words * 4 + kept.size() * 4 + (S->nseg + 1) * 4 + S->qstage.n + 4 * S->qrows.n +
-                      16 * S->ring_cap + max_pairs * sizeof(kestrel_output) + S->ctrl.n + ns * sizeof(SlotState) +
-                      8 * ns + 72 * ns + (size_t)S->nseg * NPART * ns * 8 + 4 * ns;
+                      16 * S->ring_cap + max_pairs * sizeof(kestrel_output) + S->ctrl.n +
+                      S->ngroups * (ns * sizeof(SlotState) + 8 * ns + 72 * ns + (size_t)S->nseg * NPART * ns * 8 + 4 * ns);
 
     for (uint32_t i = 0; i < MAX_REQ; i++) S->free_slots.push_back(MAX_REQ - 1 - i);
     for (uint32_t i = 0; i < max_query_rows; i++) S->free_rows.push_back(c->n_samples + max_query_rows - 1 - i);
@@ -545,7 +585,7 @@ int kestrel_serve_get_stats(kestrel_cohort* c, kestrel_serve_stats* st) {
     st->sweeps = S->st_sweeps;
     st->busy_slot_sweeps = S->st_busy;
     st->device_bytes = S->device_bytes;
-    st->slots = S->ns;
+    st->slots = S->ns * (uint32_t)S->ngroups;
     st->live = (uint32_t)S->live.size();
     st->pending = (uint32_t)S->pending.size();
     return KESTREL_OK;
