@@ -71,7 +71,7 @@ int repack_kept(kestrel_cohort* c, const uint8_t* dpacked, size_t row_stride_byt
 int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
                   uint32_t ns, double eps, double* partial, hipStream_t st) {
     HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, sx, sy, pt, active, ns, eps, partial, st,
-                          c->soft ? c->glrows.p : nullptr, c->v_pad));
+                          c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride));
     size_t base = c->nseg;
     for (const auto& p : c->parts) {
         HIPCHK(klaunch::sweep_multi(p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->af.p, p->seg.p,
@@ -269,9 +269,25 @@ int kestrel_cohort_create_gl(kestrel_cohort** out, const double* gl, uint32_t n_
         HIPCHK(hipMemcpyAsync(dkept.p, kept.data(), kept.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                               c->stream));
     HIPCHK(c->geno.alloc((size_t)n_samples * c->row_words));
-    HIPCHK(c->glrows.alloc((size_t)n_samples * c->v_pad * 3));
+    // 16-byte rows when every called row holds an exact 1.0 (likelihoods
+    // scaled by their maximum, as vcf.rs:349-368 makes GL rows and a PL of 0
+    // makes a PL row), else rows of three likelihoods
+    DevBuf<int> no_one;
+    HIPCHK(no_one.alloc(1));
+    HIPCHK(hipMemsetAsync(no_one.p, 0, sizeof(int), c->stream));
+    HIPCHK(c->glrows.alloc((size_t)n_samples * c->v_pad * 2));
     HIPCHK(klaunch::gather_gl(dgl.p, n_samples, dkept.p, c->n_kept, c->glrows.p, c->v_pad, c->geno.p, c->row_words,
-                              c->stream));
+                              1, no_one.p, c->stream));
+    int lacks = 0;
+    HIPCHK(hipMemcpyAsync(&lacks, no_one.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->gl_stride = 2;
+    if (lacks) {
+        HIPCHK(c->glrows.alloc((size_t)n_samples * c->v_pad * 3));
+        HIPCHK(klaunch::gather_gl(dgl.p, n_samples, dkept.p, c->n_kept, c->glrows.p, c->v_pad, c->geno.p,
+                                  c->row_words, 0, no_one.p, c->stream));
+        c->gl_stride = 3;
+    }
     std::vector<double> afk((size_t)c->v_pad * 2, 0.5);
     for (uint32_t t = 0; t < c->n_kept; t++) {
         afk[2 * t] = c->af_all[2 * kept[t]];

@@ -99,7 +99,8 @@ struct kestrel_cohort {
     std::vector<uint8_t> kept_mask;
     bool gt = false;        // created by kestrel_cohort_create_gt (CLS path)
     bool soft = false;      // created by kestrel_cohort_create_gl: likelihood rows in glrows
-    DevBuf<double> glrows;  // n_samples x v_pad x 3 likelihoods of kept sites (0 when missing)
+    DevBuf<double> glrows;  // n_samples x v_pad x gl_stride likelihoods of kept sites (0 when missing)
+    int gl_stride = 3;      // 3: the three likelihoods; 2: the two beside an exact 1.0 (its position in geno)
     DevBuf<double> qmat;    // CLS Gram matrix (computed on first use)
     bool have_q = false;
     int64_t opt_slots = 0;

@@ -633,6 +633,10 @@ __device__ __forceinline__ void mixture_soft(const double* lx, const double* ly,
 
 constexpr int KT_BUF = CHUNK * K_N;  // doubles per coefficient buffer
 
+// STRIDE 3: rows of three likelihoods (24 bytes per sample and site); STRIDE
+// 2: 16-byte rows of the two entries beside an exact 1.0 whose position is the
+// site's 2-bit code (gather_gl_kernel), rebuilt here into the same triple.
+template <int STRIDE>
 __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_gl(const uint32_t* __restrict__ geno, uint32_t row_words,
                                                           const double* __restrict__ gl, uint32_t v_pad,
                                                           const double* __restrict__ af,
@@ -662,8 +666,8 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_gl(const uint32_t* __restr
     }
     const uint32_t* rowx = geno + (size_t)x * row_words;
     const uint32_t* rowy = geno + (size_t)y * row_words;
-    const double* glx = gl + (size_t)x * v_pad * 3;
-    const double* gly = gl + (size_t)y * v_pad * 3;
+    const double* glx = gl + (size_t)x * v_pad * STRIDE;
+    const double* gly = gl + (size_t)y * v_pad * STRIDE;
 
     double F = 0.0, N = 0.0, P = 1.0, G[9], H[45];
 #pragma unroll
@@ -690,13 +694,35 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_gl(const uint32_t* __restr
         const double* K0 = kt + b * KT_BUF;
 #pragma unroll
         for (int k = 0; k < CHUNK; k += 2) {
-            // two sites of likelihood triples = 48 bytes, 16-byte aligned
-            const double2* px = reinterpret_cast<const double2*>(glx + (size_t)(cs + k) * 3);
-            const double2* py = reinterpret_cast<const double2*>(gly + (size_t)(cs + k) * 3);
-            const double2 x0 = px[0], x1 = px[1], x2 = px[2];
-            const double2 y0 = py[0], y1 = py[1], y2 = py[2];
-            const double lx[2][3] = {{x0.x, x0.y, x1.x}, {x1.y, x2.x, x2.y}};
-            const double ly[2][3] = {{y0.x, y0.y, y1.x}, {y1.y, y2.x, y2.y}};
+            double lx[2][3], ly[2][3];
+            if (STRIDE == 3) {
+                // two sites of likelihood triples = 48 bytes, 16-byte aligned
+                const double2* px = reinterpret_cast<const double2*>(glx + (size_t)(cs + k) * 3);
+                const double2* py = reinterpret_cast<const double2*>(gly + (size_t)(cs + k) * 3);
+                const double2 x0 = px[0], x1 = px[1], x2 = px[2];
+                const double2 y0 = py[0], y1 = py[1], y2 = py[2];
+                lx[0][0] = x0.x; lx[0][1] = x0.y; lx[0][2] = x1.x;
+                lx[1][0] = x1.y; lx[1][1] = x2.x; lx[1][2] = x2.y;
+                ly[0][0] = y0.x; ly[0][1] = y0.y; ly[0][2] = y1.x;
+                ly[1][0] = y1.y; ly[1][1] = y2.x; ly[1][2] = y2.y;
+            } else {
+                // two sites of 16-byte rows: put the exact 1.0 back at its position
+                const double2* px = reinterpret_cast<const double2*>(glx + (size_t)(cs + k) * 2);
+                const double2* py = reinterpret_cast<const double2*>(gly + (size_t)(cs + k) * 2);
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int kk = k + h;
+                    const uint32_t cx = ((kk < 16 ? wx.x : wx.y) >> (2 * (kk & 15))) & 3u;
+                    const uint32_t cy = ((kk < 16 ? wy.x : wy.y) >> (2 * (kk & 15))) & 3u;
+                    const double2 vx = px[h], vy = py[h];
+                    lx[h][0] = cx == 0 ? 1.0 : vx.x;
+                    lx[h][1] = cx == 0 ? vx.x : cx == 1 ? 1.0 : vx.y;
+                    lx[h][2] = cx == 2 ? 1.0 : vx.y;
+                    ly[h][0] = cy == 0 ? 1.0 : vy.x;
+                    ly[h][1] = cy == 0 ? vy.x : cy == 1 ? 1.0 : vy.y;
+                    ly[h][2] = cy == 2 ? 1.0 : vy.y;
+                }
+            }
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int kk = k + h;
@@ -996,11 +1022,13 @@ __global__ __launch_bounds__(256) void gather_multi_kernel(const double* __restr
 __global__ __launch_bounds__(256) void gather_gl_kernel(const double* __restrict__ gl_site, uint32_t n_samples,
                                                          const uint32_t* __restrict__ kept, uint32_t n_kept,
                                                          double* __restrict__ rows, uint32_t v_pad,
-                                                         uint32_t* __restrict__ mask, uint32_t row_words) {
+                                                         uint32_t* __restrict__ mask, uint32_t row_words, int compact,
+                                                         int* __restrict__ no_one) {
     const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= (size_t)n_samples * row_words) return;
     const uint32_t i = (uint32_t)(tid / row_words), w = (uint32_t)(tid % row_words);
     uint32_t word = 0;
+    int lacks_one = 0;
     for (int j = 0; j < 16; j++) {
         const uint32_t t = w * 16 + j;
         double l[3] = {0.0, 0.0, 0.0};
@@ -1015,13 +1043,30 @@ __global__ __launch_bounds__(256) void gather_gl_kernel(const double* __restrict
                 code = 0;
             }
         }
-        double* dst = rows + ((size_t)i * v_pad + t) * 3;
-        dst[0] = l[0];
-        dst[1] = l[1];
-        dst[2] = l[2];
+        if (compact) {
+            // 16-byte row: the two entries other than the first exact 1.0, in
+            // order; the 1.0's position is the row's 2-bit code (3 = missing:
+            // the stored pair is 0, 0)
+            double* dst = rows + ((size_t)i * v_pad + t) * 2;
+            if (code != 3) {
+                code = l[0] == 1.0 ? 0u : l[1] == 1.0 ? 1u : l[2] == 1.0 ? 2u : 3u;
+                if (code == 3) {  // no entry is exactly 1: the caller falls back to 24-byte rows
+                    lacks_one = 1;
+                    code = 0;
+                }
+            }
+            dst[0] = code == 0 ? l[1] : l[0];
+            dst[1] = code == 2 ? l[1] : l[2];
+        } else {
+            double* dst = rows + ((size_t)i * v_pad + t) * 3;
+            dst[0] = l[0];
+            dst[1] = l[1];
+            dst[2] = l[2];
+        }
         word |= code << (2 * j);
     }
     mask[(size_t)i * row_words + w] = word;
+    if (lacks_one) atomicOr(no_one, 1);
 }
 
 // ------------------------------------------------------------ SQP step
@@ -1770,11 +1815,15 @@ hipError_t gather(const uint8_t* src, size_t src_row_bytes, const uint32_t* kept
 
 hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, const uint32_t* seg_bounds,
                  uint32_t nseg, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
-                 uint32_t nslots, double eps, double* partial, hipStream_t st, const double* gl, uint32_t v_pad) {
+                 uint32_t nslots, double eps, double* partial, hipStream_t st, const double* gl, uint32_t v_pad,
+                 int gl_stride) {
     dim3 grid((nslots + SWEEP_TPB - 1) / SWEEP_TPB, nseg);
-    if (gl)
-        hipLaunchKernelGGL(sweep_gl, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, af, seg_bounds, sx, sy,
-                           pt, active, nslots, eps, partial);
+    if (gl && gl_stride == 2)
+        hipLaunchKernelGGL(sweep_gl<2>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, af, seg_bounds, sx,
+                           sy, pt, active, nslots, eps, partial);
+    else if (gl)
+        hipLaunchKernelGGL(sweep_gl<3>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, af, seg_bounds, sx,
+                           sy, pt, active, nslots, eps, partial);
     else
         hipLaunchKernelGGL(sweep_fp64, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, af, seg_bounds, sx, sy, pt,
                            active, nslots, eps, partial);
@@ -1823,11 +1872,11 @@ hipError_t gather_multi(const double* src2, const double* src3, const double* sr
 }
 
 hipError_t gather_gl(const double* gl_site, uint32_t n_samples, const uint32_t* kept, uint32_t n_kept, double* rows,
-                     uint32_t v_pad, uint32_t* mask, uint32_t row_words, hipStream_t st) {
+                     uint32_t v_pad, uint32_t* mask, uint32_t row_words, int compact, int* no_one, hipStream_t st) {
     const size_t n = (size_t)n_samples * row_words;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(gather_gl_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, gl_site, n_samples, kept,
-                       n_kept, rows, v_pad, mask, row_words);
+                       n_kept, rows, v_pad, mask, row_words, compact, no_one);
     return hipGetLastError();
 }
 

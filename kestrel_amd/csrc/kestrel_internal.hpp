@@ -53,8 +53,10 @@ hipError_t af(const uint8_t* packed, size_t row_bytes, uint32_t n_samples, uint3
 // the kept sites into sample-major rows (v_pad triples) + 2-bit mask rows
 hipError_t af_gl(const double* gl, uint32_t n_samples, uint32_t n_sites, const kdev::Tune& t, double maf,
                  double* af_out, int* status, hipStream_t st, int alleles = 2);
+// compact: 16-byte rows (the entries beside an exact 1.0, its position in the
+// 2-bit code); *no_one is set when a called row has no entry equal to 1.
 hipError_t gather_gl(const double* gl_site, uint32_t n_samples, const uint32_t* kept, uint32_t n_kept, double* rows,
-                     uint32_t v_pad, uint32_t* mask, uint32_t row_words, hipStream_t st);
+                     uint32_t v_pad, uint32_t* mask, uint32_t row_words, int compact, int* no_one, hipStream_t st);
 // tri/tetra-allelic parts: the sweep over a part of sites with 3 or 4 alleles
 // (rows of A(A+1)/2 likelihoods, af of A per site), and the re-bucketing gather
 hipError_t sweep_multi(int alleles, const uint32_t* geno, uint32_t row_words, const double* gl, uint32_t v_pad,
@@ -70,7 +72,7 @@ hipError_t gather(const uint8_t* src, size_t src_row_bytes, const uint32_t* kept
 hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, const uint32_t* seg_bounds,
                  uint32_t nseg, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
                  uint32_t nslots, double eps, double* partial, hipStream_t st,
-                 const double* gl = nullptr, uint32_t v_pad = 0);
+                 const double* gl = nullptr, uint32_t v_pad = 0, int gl_stride = 3);
 hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* active, uint32_t nslots,
                 double* partial, uint32_t nseg, const uint32_t* pairs, uint64_t npairs, const uint64_t* order,
                 unsigned long long* next_pair, kestrel_output* out, const kdev::Tune& t, int* n_live,

@@ -97,6 +97,24 @@ def test_soft_fit_bitexact_and_tolerance(O, soft):
     assert (np.abs(out["obj"] - ref["obj"]) / np.abs(ref["obj"]))[well].max() <= 1e-9
 
 
+def test_rows_without_an_exact_one(O, soft):
+    """Rows scaled by their maximum are stored as the two entries beside the
+    exact 1.0 (16 bytes); a cohort with a called row that has no entry equal
+    to 1 keeps the three likelihoods (24 bytes).  Both match the oracle."""
+    c, gl, _ = soft
+    say("case: a called row without an exact 1.0, 64 x 9000")
+    g2 = gl.copy()
+    g2[100, 3] = (0.5, 0.25, 0.125)
+    g2[200, 9] = (1.0, 1.0, 0.25)  # two exact ones: the first is the one left out
+    with Cohort.gl(g2) as co:
+        af, kept = co.allele_freqs()
+        g = np.ascontiguousarray(g2.transpose(1, 0, 2)[:, kept])
+        pairs = np.array([[3, 9], [3, 40], [9, 9], [0, 63], [12, 3]])
+        out = co.fit_pairs(pairs)
+        seg = np.minimum(segment_bounds(co.n_kept), g.shape[1]).astype(np.uint32)
+        assert_bitexact(out, O.fit_pairs_gl(g, af[kept], pairs=pairs, order=O.ORDER_SEG, seg=seg))
+
+
 def test_soft_evaluate_bitexact(O, soft):
     c, gl, co = soft
     say("case: soft evaluate, 100 points")
