@@ -33,7 +33,7 @@ LDS_PEAK_TBS = 157.3  # gfx950 LDS reads: 256 B/clk/CU x 256 CUs x 2.4 GHz
 FLOP_PER_SITE_EVAL = 156  # 136 (gradient + Hessian) + 20 (objective, log counted as 1)
 # HBM bytes per sweep_fp64 launch on the default workload, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE scaled by a plain 256 MiB copy (profiles/sweep_fp64_c2.md)
-SWEEP_TRAFFIC_BYTES_C2 = 1.612e8
+SWEEP_TRAFFIC_BYTES_C2 = 1.567e8
 
 
 def log(*a):
@@ -76,7 +76,8 @@ def cpu_baseline(codes, af, budget_s: float, gl=None):
         y = np.where(y == x, (y + 1) % n, y)
         return np.stack([np.minimum(x, y), np.maximum(x, y)], 1)
 
-    probe = sample(2 * threads)
+    fit(sample(threads))  # warm-up: library load, first-touch of the cohort
+    probe = sample(8 * threads)
     t = time.perf_counter()
     fit(probe)
     dt = time.perf_counter() - t
