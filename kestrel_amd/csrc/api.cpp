@@ -86,6 +86,8 @@ int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, con
 // order.  The pairs in flight at any time then come from a few tiles, so few
 // sample rows are streamed per sweep (it pays where rows are likelihoods).
 void tile_order(const uint32_t* pairs, size_t n, uint32_t n_samples, uint32_t T, std::vector<uint64_t>& order) {
+    // at most 4,096 tiles a side (the bucket counters stay within 128 MB)
+    T = std::max<uint32_t>(T, (n_samples + 4095) / 4096);
     const uint64_t nt = (n_samples + T - 1) / T;
     std::vector<uint64_t> start(nt * nt + 1, 0);
     auto key = [&](size_t i) {
