@@ -257,6 +257,82 @@ def test_edge_cases(O):
         assert (out["status"] == 1).all()
 
 
+def test_minimum_cohort(O):
+    """Two samples, one segregating site."""
+    say("case: 2 samples x 1 site")
+    codes = np.array([[0], [2]], np.uint8)  # p = 1/2 (a site at exactly 1/(2N) is a threshold tie)
+    with Cohort(codes) as co:
+        af, kept = co.allele_freqs()
+        assert co.n_kept == 1 and np.array_equal(kept, O.keep_sites(O.allele_freqs(codes)))
+        out = co.fit_pairs()
+    exp = O.fit_pairs(codes, af, order=O.ORDER_SEG, seg=seg_for(1, 1))
+    assert len(out) == 3 and (out["n_sites"] == 1).all()
+    assert_bitexact(out, exp)
+
+
+@pytest.mark.parametrize("v", [255, 256, 257, 4095, 4096, 4097, 8193])
+def test_padding_and_segment_boundaries(O, v):
+    """Kept-site counts either side of the 256-site row padding and of the
+    4,096-site segment size."""
+    say(f"case: 24 samples x {v} kept sites")
+    c = synth.generate(24, v + v // 5 + 400, seed=40 + v, n_family_pairs=4, missing=0.03, maf=0.05)
+    seg_sites = np.flatnonzero(O.keep_sites(O.allele_freqs(c.codes)))[:v]
+    assert len(seg_sites) == v
+    codes = np.ascontiguousarray(c.codes[:, seg_sites])
+    with Cohort(codes) as co:
+        af, kept = co.allele_freqs()
+        assert kept.all() and co.n_kept == v
+        out = co.fit_pairs()
+    exp = O.fit_pairs(codes, af, order=O.ORDER_SEG, seg=seg_for(v, v))
+    assert_bitexact(out, exp)
+
+
+def test_many_small_segments(O):
+    say("case: segments of 32 sites over 1,000 sites")
+    c = synth.generate(20, 1000, seed=44, n_family_pairs=4, missing=0.02)
+    with Cohort(c.codes, segment_sites=32) as co:
+        af, kept = co.allele_freqs()
+        out = co.fit_pairs()
+        n = co.n_kept
+    kc = c.codes[:, kept]
+    seg = np.minimum(segment_bounds(n, 32), kc.shape[1]).astype(np.uint32)
+    assert len(seg) - 1 >= 20
+    assert_bitexact(out, O.fit_pairs(kc, af[kept], order=O.ORDER_SEG, seg=seg))
+
+
+def test_c3_full_size_properties(O):
+    """The full C3 job (4,000 x 50,000, per-sample missingness U(0, 0.3), 8.0 M
+    pairs): size-independent properties of every record, n_xy against counts
+    of co-called sites, designated relatives, and a seeded sample of pairs
+    bit-exact against the oracle."""
+    say("case: C3 4000 x 50000 all 8,002,000 pairs")
+    c = synth.config("c3")
+    with Cohort(c.codes) as co:
+        af, kept = co.allele_freqs()
+        out = co.fit_pairs()
+    n = c.codes.shape[0]
+    assert len(out) == n * (n + 1) // 2
+    assert (out["status"] == 0).all()
+    d = out["jacquard"]
+    assert (d >= 0).all() and np.abs(d.sum(1) - 1).max() < 1e-12
+    rng = np.random.default_rng(9)
+    idx = rng.integers(0, len(out), 200)
+    kc = c.codes[:, kept]
+    called = kc != 3
+    for i in idx[:40]:
+        x, y = int(out["x"][i]), int(out["y"][i])
+        assert out["n_sites"][i] == int((called[x] & called[y]).sum())
+    pairs = np.stack([out["x"][idx], out["y"][idx]], 1)
+    exp = O.fit_pairs(kc, af[kept], pairs=pairs, order=O.ORDER_SEG, seg=seg_for(int(kept.sum()), kc.shape[1]))
+    assert_bitexact(out[idx], exp)
+    expect = {"PO": 0.25, "FS": 0.25, "HS": 0.125, "FC": 0.0625}
+    pos = lambda x, y: x * n - x * (x - 1) // 2 + (y - x)  # noqa: E731  row-major upper triangle
+    for x, y, kind in c.pairs:
+        a, b = min(x, y), max(x, y)
+        k = kinship(out["jacquard"][pos(a, b):pos(a, b) + 1])[0]
+        assert abs(k - expect[kind]) < 0.035, (kind, k)
+
+
 def test_given_allele_frequencies(O):
     say("case: given allele frequencies")
     rng = np.random.default_rng(4)
