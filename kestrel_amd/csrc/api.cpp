@@ -1266,8 +1266,10 @@ int kestrel_fit_pairs_gt(kestrel_cohort* c, uint64_t iterations, const uint32_t*
             HIPCHK(klaunch::cls_multi(4, p4->geno.p, p4->row_words, p4->mtab.p, n4, dpairs.p, n_pairs, p3 ? 1 : 0,
                                       c34.p, st));
     }
+    DevBuf<double> csum;
+    HIPCHK(csum.alloc(n_pairs * 9));
     HIPCHK(klaunch::cls_pairs(c->geno.p, c->row_words, c->af.p, c->n_kept, c34.p, v_total, c->qmat.p, dpairs.p,
-                              n_pairs, to_dev(t), dout.p, st));
+                              n_pairs, to_dev(t), csum.p, dout.p, st));
     HIPCHK(hipEventRecord(e1, st));
     HIPCHK(hipMemcpyAsync(out, dout.p, n_pairs * sizeof(kestrel_output), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));

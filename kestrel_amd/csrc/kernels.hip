@@ -1666,21 +1666,19 @@ __global__ __launch_bounds__(256) void cls_multi_kernel(const uint32_t* __restri
     for (int t = 0; t < 9; t++) o[t] = A == 4 ? (add_prev ? o[t] : 0.0) + c[t] : c[t];
 }
 
-// coefficients.rs:216-276, one pair per lane: c = sum over all sites of
-// m[gx][gy] in site order (cls.rs:28-49), c / -V, the QP from 1/9 (a self pair
-// solves the 2-variable (delta1, delta7) QP from (1/2, 1/2)), brier, gradient
-// and condition.
-__global__ __launch_bounds__(256) void cls_pairs_kernel(const uint32_t* __restrict__ geno, uint32_t row_words,
-                                                         const double* __restrict__ af, uint32_t n_sites,
-                                                         const double* __restrict__ c34, uint32_t v_total,
-                                                         const double* __restrict__ qmat,
-                                                         const uint32_t* __restrict__ pairs, uint64_t npairs,
-                                                         Tune tune, kestrel_output* __restrict__ out) {
-    __shared__ __attribute__((aligned(16))) double lut[2 * LUT_BUF];
+// cls.rs:28-49 over the 2-allele sites, one pair per lane: c2 = sum of
+// m_s[gx][gy] in site order from zero.  The block stages a 32-site chunk's
+// 12-row table in LDS (row min(4 gx + gy, 11); rows 3, 7, 11 and padding sites
+// are zero: x + 0 = x) and each lane reads its 72-byte row per site.
+__global__ __launch_bounds__(256) void cls_sum_kernel(const uint32_t* __restrict__ geno, uint32_t row_words,
+                                                       const double* __restrict__ af, uint32_t n_sites,
+                                                       const uint32_t* __restrict__ pairs, uint64_t npairs,
+                                                       double* __restrict__ csum) {
+    __shared__ __attribute__((aligned(16))) double lut[LUT_BUF];
     const uint32_t tid = threadIdx.x;
     const uint64_t idx = (uint64_t)blockIdx.x * 256 + tid;
     const bool live = idx < npairs;
-    for (uint32_t i = tid; i < 2 * LUT_BUF; i += 256) lut[i] = 0.0;
+    for (uint32_t i = tid; i < LUT_BUF; i += 256) lut[i] = 0.0;
     uint32_t x = 0, y = 0;
     if (live) {
         x = pairs[2 * idx];
@@ -1689,58 +1687,67 @@ __global__ __launch_bounds__(256) void cls_pairs_kernel(const uint32_t* __restri
     const uint32_t* rowx = geno + (size_t)x * row_words;
     const uint32_t* rowy = geno + (size_t)y * row_words;
     const int lut_site = tid >> 3, lut_part = tid & 7;
-    auto build = [&](uint32_t cs, int b) {
-        const uint32_t s = cs + lut_site;
-        double* buf = lut + b * LUT_BUF;
-        const int parts[2] = {lut_part, 8};
-        for (int w = 0; w < (lut_part == 0 ? 2 : 1); w++) {
-            const int pr = parts[w];
-            const int gx = pr / 3, gy = pr % 3;
-            double* dst = buf + (gx * 4 + gy) * LUT_RS + lut_site * LUT_W;
-            if (s < n_sites) m_row(af[2 * s], af[2 * s + 1], gx, gy, dst);
-            else
-                for (int k = 0; k < 9; k++) dst[k] = 0.0;
-        }
-    };
     double c[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) c[k] = 0.0;
     const uint32_t v_pad = row_words * 16;
-    __syncthreads();
-    build(0, 0);
-    __syncthreads();
-    int b = 0;
-    for (uint32_t cs = 0; cs < v_pad; cs += CHUNK, b ^= 1) {
-        if (cs + CHUNK < v_pad) build(cs + CHUNK, b ^ 1);
-        const uint2 wx = *reinterpret_cast<const uint2*>(rowx + cs / 16);
-        const uint2 wy = *reinterpret_cast<const uint2*>(rowy + cs / 16);
-        const double* L0 = lut + b * LUT_BUF;
-        // padding sites past n_sites carry code 3 and select a zero row (x + 0 = x)
-#pragma unroll
-        for (int k = 0; k < CHUNK; k++) {
-            {
-                const uint32_t ax = (k < 16 ? wx.x : wx.y) >> (2 * (k & 15));
-                const uint32_t ay = (k < 16 ? wy.x : wy.y) >> (2 * (k & 15));
-                const uint32_t row = min(((ax & 3u) << 2) | (ay & 3u), 11u);
-                const double2* r2 = reinterpret_cast<const double2*>(L0 + row * LUT_RS + k * LUT_W);
-                double m[10];
-#pragma unroll
-                for (int j = 0; j < 5; j++) {
-                    const double2 v = r2[j];
-                    m[2 * j] = v.x;
-                    m[2 * j + 1] = v.y;
-                }
-#pragma unroll
-                for (int j = 0; j < 9; j++) c[j] = c[j] + m[j];
+    for (uint32_t cs = 0; cs < v_pad; cs += CHUNK) {
+        __syncthreads();  // the previous chunk's rows are read
+        {
+            const uint32_t s = cs + lut_site;
+            const int parts[2] = {lut_part, 8};
+            for (int w = 0; w < (lut_part == 0 ? 2 : 1); w++) {
+                const int pr = parts[w];
+                const int gx = pr / 3, gy = pr % 3;
+                double* dst = lut + (gx * 4 + gy) * LUT_RS + lut_site * LUT_W;
+                if (s < n_sites) m_row(af[2 * s], af[2 * s + 1], gx, gy, dst);
+                else
+                    for (int k = 0; k < 9; k++) dst[k] = 0.0;
             }
         }
+        const uint2 wx = *reinterpret_cast<const uint2*>(rowx + cs / 16);
+        const uint2 wy = *reinterpret_cast<const uint2*>(rowy + cs / 16);
         __syncthreads();
+#pragma unroll
+        for (int k = 0; k < CHUNK; k++) {
+            const uint32_t ax = (k < 16 ? wx.x : wx.y) >> (2 * (k & 15));
+            const uint32_t ay = (k < 16 ? wy.x : wy.y) >> (2 * (k & 15));
+            const uint32_t row = min(((ax & 3u) << 2) | (ay & 3u), 11u);
+            const double* r = lut + row * LUT_RS + k * LUT_W;
+            const double2* r2 = reinterpret_cast<const double2*>(r);
+            double m[9];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const double2 v = r2[j];
+                m[2 * j] = v.x;
+                m[2 * j + 1] = v.y;
+            }
+            m[8] = r[8];
+#pragma unroll
+            for (int j = 0; j < 9; j++) c[j] = c[j] + m[j];
+        }
     }
     if (!live) return;
-    // coefficients.rs:226-228: (c2 + (c3 + c4)) / -V over all V sites
-    const double nv = -(double)v_total;
 #pragma unroll
-    for (int k = 0; k < 9; k++) c[k] = (c[k] + (c34 ? c34[idx * 9 + k] : 0.0 + 0.0)) / nv;
+    for (int k = 0; k < 9; k++) csum[idx * 9 + k] = c[k];
+}
+
+// coefficients.rs:216-276 for one pair per lane: c = (c2 + (c3 + c4)) / -V
+// (coefficients.rs:226-228) from the per-bucket sums, the QP from 1/9 (a self
+// pair solves the 2-variable (delta1, delta7) QP from (1/2, 1/2)), brier,
+// gradient and condition.
+__global__ __launch_bounds__(64) void cls_pairs_kernel(const double* __restrict__ csum,
+                                                        const double* __restrict__ c34, uint32_t v_total,
+                                                        const double* __restrict__ qmat,
+                                                        const uint32_t* __restrict__ pairs, uint64_t npairs,
+                                                        Tune tune, kestrel_output* __restrict__ out) {
+    const uint64_t idx = (uint64_t)blockIdx.x * 64 + threadIdx.x;
+    if (idx >= npairs) return;
+    const uint32_t x = pairs[2 * idx], y = pairs[2 * idx + 1];
+    const double nv = -(double)v_total;
+    double c[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) c[k] = (csum[idx * 9 + k] + (c34 ? c34[idx * 9 + k] : 0.0 + 0.0)) / nv;
     double Q[9][9];
 #pragma unroll
     for (int i = 0; i < 9; i++)
@@ -1989,10 +1996,12 @@ hipError_t cls_multi(int alleles, const uint32_t* geno, uint32_t row_words, cons
 
 hipError_t cls_pairs(const uint32_t* geno, uint32_t row_words, const double* af, uint32_t n_sites, const double* c34,
                      uint32_t v_total, const double* q, const uint32_t* pairs, uint64_t npairs, const Tune& t,
-                     kestrel_output* out, hipStream_t st) {
+                     double* csum, kestrel_output* out, hipStream_t st) {
     if (npairs == 0) return hipSuccess;
-    hipLaunchKernelGGL(cls_pairs_kernel, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, st, geno, row_words, af,
-                       n_sites, c34, v_total, q, pairs, npairs, t, out);
+    hipLaunchKernelGGL(cls_sum_kernel, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, st, geno, row_words, af,
+                       n_sites, pairs, npairs, csum);
+    hipLaunchKernelGGL(cls_pairs_kernel, dim3((unsigned)((npairs + 63) / 64)), dim3(64), 0, st, csum, c34, v_total, q,
+                       pairs, npairs, t, out);
     return hipGetLastError();
 }
 

@@ -96,9 +96,10 @@ hipError_t gt_count_multi(int alleles, const uint32_t* geno, uint32_t row_words,
 hipError_t cls_m(int alleles, const double* af, uint32_t n_sites, double* m, hipStream_t st);
 hipError_t cls_multi(int alleles, const uint32_t* geno, uint32_t row_words, const double* mtab, uint32_t n_sites,
                      const uint32_t* pairs, uint64_t npairs, int add_prev, double* out, hipStream_t st);
+// csum: npairs x 9 doubles of scratch (the 2-allele sums)
 hipError_t cls_pairs(const uint32_t* geno, uint32_t row_words, const double* af, uint32_t n_sites, const double* c34,
                      uint32_t v_total, const double* q, const uint32_t* pairs, uint64_t npairs, const kdev::Tune& t,
-                     kestrel_output* out, hipStream_t st);
+                     double* csum, kestrel_output* out, hipStream_t st);
 hipError_t finalize(const double* partial, uint32_t nseg, uint32_t nslots, uint32_t npts, double* f, double* g,
                     double* h, uint32_t* nsites, hipStream_t st);
 }  // namespace klaunch
