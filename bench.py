@@ -363,6 +363,8 @@ def main():
                     help="serving feed order: shortest remaining first or round robin")
     ap.add_argument("--cli", action="store_true", help="end-to-end kestrel CLI run on the workload as a VCF")
     ap.add_argument("--cls", action="store_true", help="the CLS / Brier estimator of -f GT")
+    ap.add_argument("--multi", type=float, default=0.0, metavar="FRACTION",
+                    help="this fraction of the sites with 3 or 4 alleles (4:1), as one-hot likelihood rows")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -386,12 +388,30 @@ def main():
     c = synth.config(args.config)
     n, v = c.codes.shape
     gl = None
+    buckets = None
     if args.soft:
         gl = synth.soft_likelihoods(c.codes, seed=0x50F7, mean_depth=args.depth)
-    log(f"cohort {args.config}: {n} x {v}{' soft likelihoods' if args.soft else ''} generated in "
+    if args.multi > 0:
+        # the workload's sizes with a fraction of the sites tri/tetra-allelic
+        # (hard calls of the multi-allelic generator, 1 % missing) as one-hot
+        # likelihood rows: (V_A, N, A(A+1)/2), site-major as vcf.rs builds them
+        v3 = int(round(v * args.multi * 0.8))
+        v4 = int(round(v * args.multi * 0.2))
+        idx = synth.multi_genotypes(n, (v - v3 - v4, v3, v4), seed=0xA4, missing=0.01)
+        buckets = []
+        for A, ix in zip((2, 3, 4), idx):
+            G = A * (A + 1) // 2
+            rows = np.zeros((ix.shape[1], n, G))
+            called = ix.T != 255
+            s_i, n_i = np.nonzero(called)
+            rows[s_i, n_i, ix.T[called]] = 1.0
+            rows[~called] = np.nan
+            buckets.append(rows)
+    log(f"cohort {args.config}: {n} x {v}{' soft likelihoods' if args.soft else ''}"
+        f"{f' with {args.multi:.0%} tri/tetra-allelic sites' if buckets else ''} generated in "
         f"{time.perf_counter() - t:.1f}s")
     t = time.perf_counter()
-    co = Cohort.gl(gl) if args.soft else Cohort(c.codes)
+    co = Cohort.gl(gl) if args.soft else Cohort.gl_multi(*buckets) if buckets else Cohort(c.codes)
     if args.segment_sites:
         co.set_option("segment_sites", args.segment_sites)
     if args.slots:
@@ -404,8 +424,14 @@ def main():
         co.set_option("graph", 1)
     if args.pair_tile is not None:
         co.set_option("pair_tile", args.pair_tile)
-    af, kept = co.allele_freqs()
-    log(f"device cohort + allele frequencies in {time.perf_counter() - t:.2f}s, kept {co.n_kept} sites")
+    if buckets:
+        parts, _ = co.multi_info()
+        kept = None
+        log(f"device cohort + allele frequencies in {time.perf_counter() - t:.2f}s, sites with 2 / 3 / 4 alleles "
+            f"after zero-frequency removal: {parts}")
+    else:
+        af, kept = co.allele_freqs()
+        log(f"device cohort + allele frequencies in {time.perf_counter() - t:.2f}s, kept {co.n_kept} sites")
     tuned = None
     if args.autotune:
         t = time.perf_counter()
@@ -440,8 +466,9 @@ def main():
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_VALU_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_VALU_PEAK_TFLOPS, 4),
                 "traffic": (SWEEP_TRAFFIC_BYTES_C2 if args.config == "c2" and not args.segment_sites
-                            and not args.soft else None),
-                "kernel": "sweep_gl" if args.soft else "sweep_fp64", "avg_launch_ms": round(sweep_ms / sweeps, 4),
+                            and not args.soft and not buckets else None),
+                "kernel": "sweep_gl + sweep_glm" if buckets else "sweep_gl" if args.soft else "sweep_fp64",
+                "avg_launch_ms": round(sweep_ms / sweeps, 4),
                 "concurrent_launches": groups, "sweep_busy_ms_per_step": round(busy_ms / args.steps, 2),
                 "algorithmic_flop_per_launch": flops / sweeps}
     measured_peak = None
@@ -456,7 +483,10 @@ def main():
     except Exception as e:  # the measurement is informative only
         log(f"fp64 peak measurement failed: {e}")
     cpu = None
-    if args.cpu_budget > 0:
+    if buckets:
+        log("cpu baseline: not measured for tri/tetra-allelic cohorts (the oracle's multi-allelic path is "
+            "Python over sites)")
+    elif args.cpu_budget > 0:
         glk = None if gl is None else np.ascontiguousarray(gl.transpose(1, 0, 2)[:, kept])
         cpu = cpu_baseline(c.codes[:, kept], af[kept], args.cpu_budget, gl=glk)
         log(f"cpu baseline: {cpu}")
@@ -464,11 +494,14 @@ def main():
         "metric": "pair-fits/sec", "value": round(value, 1), "unit": "pair-fits/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(total / args.steps * 1e3, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{args.config}: {n} samples x {v} sites ({co.n_kept} segregating), "
+        "config": {"workload": f"{args.config}: {n} samples x {v} sites ({co.n_kept} segregating"
+                               + (", 2-allele part" if buckets else "") + f"), "
                                f"{offdiag} off-diagonal pairs (+{len(out) - offdiag} self pairs), FP64"
-                               + (f", soft likelihoods (mean depth {args.depth})" if args.soft else ""),
+                               + (f", soft likelihoods (mean depth {args.depth})" if args.soft else "")
+                               + (f", {args.multi:.0%} of sites tri/tetra-allelic" if buckets else ""),
                    "samples": n, "sites": v, "pairs": offdiag, "precision": "fp64",
-                   "input": "soft genotype likelihoods" if args.soft else "hard calls",
+                   "input": ("soft genotype likelihoods" if args.soft else
+                             "one-hot likelihoods, 2-4 alleles" if buckets else "hard calls"),
                    "launch_shape": ({k: tuned[k] for k in ("slots", "streams", "batch")} if tuned else "default"),
                    "hip_graph": bool(args.graph)},
         "roofline": roofline, "cpu_baseline": cpu,

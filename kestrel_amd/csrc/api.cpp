@@ -74,7 +74,7 @@ int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, con
                           c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride));
     size_t base = c->nseg;
     for (const auto& p : c->parts) {
-        HIPCHK(klaunch::sweep_multi(p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->af.p, p->seg.p,
+        HIPCHK(klaunch::sweep_multi(p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->table.p, p->seg.p,
                                     p->nseg, sx, sy, pt, active, ns, eps, partial + base * NPART * ns, st));
         base += p->nseg;
     }
@@ -419,6 +419,9 @@ int kestrel_cohort_create_gl_multi(kestrel_cohort** out, uint32_t n_samples, con
         p->row_words = p->v_pad / 16;
         rc = build_part(q, p->v_pad, p->rows, p->mask, p->af);
         if (rc) return rc;
+        HIPCHK(p->table.alloc((size_t)p->v_pad * klaunch::iis_table_size(p->alleles)));
+        HIPCHK(klaunch::iis_table(p->alleles, p->af.p, p->v_pad, p->table.p, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
         c->parts.push_back(std::move(p));
     }
     rc = c->set_segments();

@@ -63,6 +63,15 @@ inline std::vector<uint32_t> segment_bounds(uint32_t v_pad, int64_t segment_site
     return b;
 }
 
+// Segment size of a tri/tetra-allelic part: as many segments as the main part
+// (main_nseg) so that the part's sweep launches fill the device like the main
+// one, rounded up to whole chunks, at most the cohort's segment size.
+inline int64_t part_segment_sites(uint32_t part_v_pad, uint32_t main_nseg, int64_t segment_sites) {
+    const uint64_t per = ((uint64_t)part_v_pad + main_nseg - 1) / std::max<uint32_t>(1, main_nseg);
+    const int64_t rounded = (int64_t)((per + CHUNK - 1) / CHUNK * CHUNK);
+    return std::min<int64_t>(segment_sites, std::max<int64_t>(CHUNK, rounded));
+}
+
 // Sites with 3 or 4 alleles after zero-frequency removal (likelihood rows of
 // A(A+1)/2 genotypes); their segments follow the main part's in a sweep.
 struct SitePart {
@@ -71,6 +80,7 @@ struct SitePart {
     DevBuf<double> rows;    // n_samples x v_pad x G
     DevBuf<uint32_t> mask;  // n_samples x row_words 2-bit rows, 3 = missing
     DevBuf<double> af;      // v_pad x alleles
+    DevBuf<double> table;   // v_pad x the mode coefficients of every genotype pair (A >= 3)
     DevBuf<uint32_t> seg;
 };
 // The 3- or 4-allele sites of a -f GT cohort (kestrel_cohort_create_gt_multi):
@@ -136,7 +146,7 @@ struct kestrel_cohort {
         HIPCHK(seg.alloc(nseg + 1));
         HIPCHK(hipMemcpy(seg.p, b.data(), b.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         for (auto& p : parts) {
-            std::vector<uint32_t> pb = segment_bounds(p->v_pad, opt_segment_sites);
+            std::vector<uint32_t> pb = segment_bounds(p->v_pad, part_segment_sites(p->v_pad, nseg, opt_segment_sites));
             p->nseg = (uint32_t)pb.size() - 1;
             HIPCHK(p->seg.alloc(p->nseg + 1));
             HIPCHK(hipMemcpy(p->seg.p, pb.data(), pb.size() * sizeof(uint32_t), hipMemcpyHostToDevice));

@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 #include "kestrel_internal.hpp"
 #include "sqp_device.hpp"
 
@@ -880,20 +882,166 @@ __device__ __forceinline__ void mixture_generic(const double* lx, const double* 
         }
 }
 
+// accumulate_iis split in two: the coefficients of a mode depend only on the
+// site's frequencies (iis_coefs writes them once per site, in accumulate_iis's
+// expressions and order), and a lane applies them to its g = lx ly with the
+// same fma sequence (iis_target: the component each coefficient goes to).
+__host__ __device__ constexpr int iis_nnz(int mode) {
+    return mode == 1 ? 9 : (mode >= 2 && mode <= 4) || mode == 6 || mode == 7 ? 4 : mode == 5 || mode == 8 ? 2
+         : mode == 9 || mode == 10 ? 3 : mode >= 11 && mode <= 14 ? 2 : 1;
+}
+
+__host__ __device__ constexpr int iis_target(int mode, int t) {
+    return mode == 1 ? t
+         : mode == 2 ? (t == 0 ? 1 : t == 1 ? 3 : t == 2 ? 5 : 8)
+         : mode == 3 || mode == 4 ? (t == 0 ? 2 : t == 1 ? 3 : t == 2 ? 7 : 8)
+         : mode == 5 ? (t == 0 ? 3 : 8)
+         : mode == 6 || mode == 7 ? (t == 0 ? 4 : t == 1 ? 5 : t == 2 ? 7 : 8)
+         : mode == 8 ? (t == 0 ? 5 : 8)
+         : mode == 9 || mode == 10 ? 6 + t
+         : mode >= 11 && mode <= 14 ? 7 + t : 8;
+}
+
+__device__ __forceinline__ void iis_coefs(int mode, double pi, double pj, double pk, double pl, double* o) {
+    switch (mode) {
+    case 1: {
+        const double pi2 = pi * pi, pi3 = pi2 * pi;
+        o[0] = pi; o[1] = pi2; o[2] = pi2; o[3] = pi3; o[4] = pi2; o[5] = pi3; o[6] = pi2; o[7] = pi3;
+        o[8] = pi2 * pi2;
+    } break;
+    case 2: {
+        const double pipk = pi * pk;
+        o[0] = pipk; o[1] = pipk * pk; o[2] = pi * pipk; o[3] = pipk * pipk;
+    } break;
+    case 3: {
+        const double pipl = pi * pl, pi2pl = pi * pipl, two = pi2pl + pi2pl;
+        o[0] = pipl; o[1] = two; o[2] = pi2pl; o[3] = two * pi;
+    } break;
+    case 4: {
+        const double pipk = pi * pk, pi2pk = pi * pipk, two = pi2pk + pi2pk;
+        o[0] = pipk; o[1] = two; o[2] = pi2pk; o[3] = two * pi;
+    } break;
+    case 5: {
+        const double v = (pi * pk) * pl, two = v + v;
+        o[0] = two; o[1] = two * pi;
+    } break;
+    case 6: {
+        const double pipj = pi * pj, pi2pj = pi * pipj, two = pi2pj + pi2pj;
+        o[0] = pipj; o[1] = two; o[2] = pi2pj; o[3] = two * pi;
+    } break;
+    case 7: {
+        const double pipj = pi * pj, pj2pi = pj * pipj, two = pj2pi + pj2pi;
+        o[0] = pipj; o[1] = two; o[2] = pj2pi; o[3] = two * pj;
+    } break;
+    case 8: {
+        const double v = (pi * pj) * pk, two = v + v;
+        o[0] = two; o[1] = two * pk;
+    } break;
+    case 9:
+    case 10: {
+        const double pipj = pi * pj, two = pipj + pipj;
+        o[0] = two; o[1] = pipj * (pi + pj); o[2] = two * two;
+    } break;
+    case 11: {
+        const double v = (pi * pj) * pl;
+        o[0] = v; o[1] = (4.0 * v) * pi;
+    } break;
+    case 12: {
+        const double v = (pi * pj) * pk;
+        o[0] = v; o[1] = (4.0 * v) * pi;
+    } break;
+    case 13: {
+        const double v = (pi * pj) * pl;
+        o[0] = v; o[1] = (4.0 * v) * pj;
+    } break;
+    case 14: {
+        const double v = (pi * pj) * pk;
+        o[0] = v; o[1] = (4.0 * v) * pj;
+    } break;
+    default: {  // 15
+        const double v = (pi * pj) * (pk * pl);
+        o[0] = 4.0 * v;
+    } break;
+    }
+}
+
+// The genotype pairs of IIS_A in mixture.rs's order: their IBS modes and the
+// offsets of their coefficients in a site's table (NC coefficients a site).
+template <int A>
+struct IisTable {
+    static constexpr int G = A * (A + 1) / 2;
+    int mode[G * G];
+    int off[G * G + 1];
+    constexpr IisTable() : mode(), off() {
+        constexpr Genotypes<A> gt;
+        off[0] = 0;
+        for (int c = 0; c < G * G; c++) {
+            const int gx = c / G, gy = c % G;
+            mode[c] = iis_mode(gt.gi[gx], gt.gj[gx], gt.gi[gy], gt.gj[gy]);
+            off[c + 1] = off[c] + iis_nnz(mode[c]);
+        }
+    }
+};
+
+// One genotype pair's fmas with every index a template constant (a fold over
+// the mode's coefficients, left to right: accumulate_iis's order).
+template <int MODE, int OFF, int... Ts>
+__device__ __forceinline__ void iis_apply(double* cv, double g, const double* T, std::integer_sequence<int, Ts...>) {
+    ((cv[iis_target(MODE, Ts)] = fma(g, T[OFF + Ts], cv[iis_target(MODE, Ts)])), ...);
+}
+
+template <int A, int C>
+__device__ __forceinline__ void iis_pair(const double* __restrict__ px, const double* ly, const double* T,
+                                         double* cv) {
+    constexpr IisTable<A> tab{};
+    constexpr int G = IisTable<A>::G, MODE = tab.mode[C], OFF = tab.off[C];
+    const double g = px[C / G] * ly[C % G];
+    iis_apply<MODE, OFF>(cv, g, T, std::make_integer_sequence<int, iis_nnz(MODE)>{});
+}
+
+// mixture.rs:8-60 from a site's coefficient table: every (gx, gy) in genotype
+// order, g = lx[gx] ly[gy], then its mode's fmas.
+template <int A, int... Cs>
+__device__ __forceinline__ void mixture_table(const double* __restrict__ px, const double* ly, const double* T,
+                                              double* cv, std::integer_sequence<int, Cs...>) {
+    (iis_pair<A, Cs>(px, ly, T, cv), ...);
+}
+
+// Each site's coefficient table: thread (site, genotype pair).
+template <int A>
+__global__ __launch_bounds__(64) void iis_table_kernel(const double* __restrict__ af, uint32_t n_sites,
+                                                        double* __restrict__ table) {
+    constexpr IisTable<A> tab;
+    constexpr Genotypes<A> gt;
+    constexpr int G = IisTable<A>::G, NC = tab.off[G * G];
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (uint64_t)n_sites * G * G) return;
+    const uint64_t s = e / (G * G);
+    const int c = (int)(e % (G * G)), gx = c / G, gy = c % G;
+    const double* p = af + s * A;
+    double o[9];
+    iis_coefs(tab.mode[c], p[gt.gi[gx]], p[gt.gj[gx]], p[gt.gi[gy]], p[gt.gj[gy]], o);
+    double* dst = table + s * NC + tab.off[c];
+    for (int t = 0; t < iis_nnz(tab.mode[c]); t++) dst[t] = o[t];
+}
+
 // The sweep over sites with A = 3 or 4 alleles: as sweep_gl with rows of
 // G = A(A+1)/2 likelihoods per site and the generic mixture.
 template <int A>
 __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __restrict__ geno, uint32_t row_words,
                                                            const double* __restrict__ gl, uint32_t v_pad,
-                                                           const double* __restrict__ af,
+                                                           const double* __restrict__ table,
                                                            const uint32_t* __restrict__ seg_bounds,
                                                            const uint32_t* __restrict__ sx,
                                                            const uint32_t* __restrict__ sy,
                                                            const double* __restrict__ pt,
                                                            const int* __restrict__ active, uint32_t nslots,
                                                            double eps, double* __restrict__ partial) {
-    constexpr int G = A * (A + 1) / 2;
-    __shared__ __attribute__((aligned(16))) double kt[2 * CHUNK * 4];
+    constexpr IisTable<A> tab;
+    constexpr Genotypes<A> gt;
+    constexpr int G = A * (A + 1) / 2, NC = tab.off[G * G];
+    // the chunk's coefficient tables (NC doubles a site: 132 for A = 3, 300 for A = 4)
+    __shared__ __attribute__((aligned(16))) double kt[CHUNK * NC];
     const uint32_t tid = threadIdx.x;
     const uint32_t slot = blockIdx.x * SWEEP_TPB + tid;
     const uint32_t seg = blockIdx.y;
@@ -923,16 +1071,13 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __rest
     for (int k = 0; k < 45; k++) H[k] = 0.0;
 
     const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
-    auto build = [&](uint32_t cs, int b) {
-        if (tid < CHUNK)
-#pragma unroll
-            for (int a = 0; a < 4; a++) kt[b * CHUNK * 4 + tid * 4 + a] = a < A ? af[(size_t)A * (cs + tid) + a] : 0.0;
-    };
-    if (s0 < s1) build(s0, 0);
-    __syncthreads();
-    int b = 0;
-    for (uint32_t cs = s0; cs < s1; cs += CHUNK, b ^= 1) {
-        if (cs + CHUNK < s1) build(cs + CHUNK, b ^ 1);
+    for (uint32_t cs = s0; cs < s1; cs += CHUNK) {
+        __syncthreads();  // the previous chunk's tables are read
+        {
+            const double2* src = reinterpret_cast<const double2*>(table + (size_t)cs * NC);
+            double2* dst = reinterpret_cast<double2*>(kt);
+            for (uint32_t i = tid; i < CHUNK * NC / 2; i += SWEEP_TPB) dst[i] = src[i];
+        }
         const uint2 wx = *reinterpret_cast<const uint2*>(rowx + cs / 16);
         const uint2 wy = *reinterpret_cast<const uint2*>(rowy + cs / 16);
         {
@@ -940,29 +1085,32 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __rest
             const uint32_t uy = ~((wx.y & (wx.y >> 1)) | (wy.y & (wy.y >> 1))) & 0x55555555u;
             N += (double)(__popc(ux) + __popc(uy));
         }
-        const double* K0 = kt + b * CHUNK * 4;
-#pragma unroll 2
+        __syncthreads();
+#pragma unroll 1
         for (int k = 0; k < CHUNK; k++) {
             const uint32_t ax = (k < 16 ? wx.x : wx.y) >> (2 * (k & 15));
             const uint32_t ay = (k < 16 ? wy.x : wy.y) >> (2 * (k & 15));
             const bool missing = (ax & 3u) == 3u || (ay & 3u) == 3u;
-            double lx[G], ly[G];
-            const double2* px = reinterpret_cast<const double2*>(glx + (size_t)(cs + k) * G);
+            double ly[G];
+            const double* px = glx + (size_t)(cs + k) * G;
             const double2* py = reinterpret_cast<const double2*>(gly + (size_t)(cs + k) * G);
 #pragma unroll
             for (int q = 0; q < G / 2; q++) {
-                const double2 vx = px[q], vy = py[q];
-                lx[2 * q] = vx.x;
-                lx[2 * q + 1] = vx.y;
+                const double2 vy = py[q];
                 ly[2 * q] = vy.x;
                 ly[2 * q + 1] = vy.y;
             }
-            const double p[4] = {K0[k * 4], K0[k * 4 + 1], K0[k * 4 + 2], K0[k * 4 + 3]};
+            // mixture.rs:8-60: for every (gx, gy) in genotype order, g = lx ly and
+            // accumulate_iis with the site's coefficients of that pair's mode
+            // (x's likelihoods read per genotype: fewer registers)
+            const double* T = kt + k * NC;
             double cv[9];
-            mixture_generic<A>(lx, ly, p, cv);
+#pragma unroll
+            for (int j = 0; j < 9; j++) cv[j] = 0.0;
+            mixture_table<A>(px, ly, T, cv, std::make_integer_sequence<int, G * G>{});
+            (void)gt;
             site_update(cv, missing ? 1.0 : eps, D, k, P, F, Gr, H);
         }
-        __syncthreads();
     }
     if (live) {
         double* out = partial + (size_t)seg * NPART * nslots + slot;
@@ -1853,16 +2001,33 @@ hipError_t af_gl(const double* gl, uint32_t n_samples, uint32_t n_sites, const T
 }
 
 hipError_t sweep_multi(int alleles, const uint32_t* geno, uint32_t row_words, const double* gl, uint32_t v_pad,
-                       const double* af, const uint32_t* seg_bounds, uint32_t nseg, const uint32_t* sx,
+                       const double* table, const uint32_t* seg_bounds, uint32_t nseg, const uint32_t* sx,
                        const uint32_t* sy, const double* pt, const int* active, uint32_t nslots, double eps,
                        double* partial, hipStream_t st) {
     dim3 grid((nslots + SWEEP_TPB - 1) / SWEEP_TPB, nseg);
     if (alleles == 3)
-        hipLaunchKernelGGL(sweep_glm<3>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, af, seg_bounds, sx,
-                           sy, pt, active, nslots, eps, partial);
+        hipLaunchKernelGGL(sweep_glm<3>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, table, seg_bounds,
+                           sx, sy, pt, active, nslots, eps, partial);
     else if (alleles == 4)
-        hipLaunchKernelGGL(sweep_glm<4>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, af, seg_bounds, sx,
-                           sy, pt, active, nslots, eps, partial);
+        hipLaunchKernelGGL(sweep_glm<4>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, table, seg_bounds,
+                           sx, sy, pt, active, nslots, eps, partial);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+int iis_table_size(int alleles) {
+    return alleles == 3 ? IisTable<3>().off[36] : alleles == 4 ? IisTable<4>().off[100] : 0;
+}
+
+hipError_t iis_table(int alleles, const double* af, uint32_t n_sites, double* table, hipStream_t st) {
+    if (n_sites == 0) return hipSuccess;
+    const uint64_t gg = alleles == 3 ? 36 : 100, n = (uint64_t)n_sites * gg;
+    const dim3 grid((unsigned)((n + 63) / 64));
+    if (alleles == 3)
+        hipLaunchKernelGGL(iis_table_kernel<3>, grid, dim3(64), 0, st, af, n_sites, table);
+    else if (alleles == 4)
+        hipLaunchKernelGGL(iis_table_kernel<4>, grid, dim3(64), 0, st, af, n_sites, table);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

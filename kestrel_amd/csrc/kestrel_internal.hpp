@@ -60,9 +60,12 @@ hipError_t gather_gl(const double* gl_site, uint32_t n_samples, const uint32_t* 
 // tri/tetra-allelic parts: the sweep over a part of sites with 3 or 4 alleles
 // (rows of A(A+1)/2 likelihoods, af of A per site), and the re-bucketing gather
 hipError_t sweep_multi(int alleles, const uint32_t* geno, uint32_t row_words, const double* gl, uint32_t v_pad,
-                       const double* af, const uint32_t* seg_bounds, uint32_t nseg, const uint32_t* sx,
+                       const double* table, const uint32_t* seg_bounds, uint32_t nseg, const uint32_t* sx,
                        const uint32_t* sy, const double* pt, const int* active, uint32_t nslots, double eps,
                        double* partial, hipStream_t st);
+// per-site accumulate_iis coefficients of a part (iis_table_size doubles a site)
+int iis_table_size(int alleles);
+hipError_t iis_table(int alleles, const double* af, uint32_t n_sites, double* table, hipStream_t st);
 hipError_t gather_multi(const double* src2, const double* src3, const double* src4, uint32_t n_samples,
                         const uint32_t* map_site, const uint8_t* map_alleles, const uint8_t* gmap, uint32_t n_t,
                         int gp, double* rows, uint32_t v_pad, uint32_t* mask, uint32_t row_words, hipStream_t st);

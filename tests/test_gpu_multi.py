@@ -28,15 +28,23 @@ def assert_bitexact(out, exp):
         assert ok.all(), f"{k}: {int((~ok).sum())} of {len(out)} pairs differ"
 
 
-def device_order(parts):
+def device_order(parts, segment_sites=4096):
     """Pads and segment bounds of the device's parts (each part's sites
-    rounded up to 256, the 2-allele part at least 256)."""
-    pads, seg, base = {}, [0], 0
+    rounded up to 256, the 2-allele part at least 256; a 3- or 4-allele part
+    cut into as many segments as the 2-allele part, in whole 32-site chunks,
+    at most segment_sites each)."""
+    pads, seg, base, main_nseg = {}, [0], 0, 1
     for A in (2, 3, 4):
         n = parts[A][0].shape[0]
         if A > 2 and n == 0:
             continue
-        b = segment_bounds(n)
+        if A == 2:
+            b = segment_bounds(n, segment_sites)
+            main_nseg = len(b) - 1
+        else:
+            v_pad = (n + 255) // 256 * 256
+            per = -(-v_pad // main_nseg)
+            b = segment_bounds(n, min(segment_sites, max(32, -(-per // 32) * 32)))
         pads[A] = int(b[-1])
         seg += [int(v) + base for v in b[1:]]
         base += int(b[-1])
@@ -81,6 +89,22 @@ def test_multi_fit_bitexact_and_tolerance(O, multi):
         assert_bitexact(co.fit_pairs(), out)
     finally:
         co.set_option("graph", 0)
+
+
+def test_multi_parts_cut_like_the_main_part(O, multi):
+    """With several main-part segments the 3- and 4-allele parts are cut into
+    as many (whole 32-site chunks), and the records follow that order."""
+    co, parts, af_in, _ = multi
+    say("case: multi-allelic fit with 256-site segments")
+    co.set_option("segment_sites", 256)
+    try:
+        out = co.fit_pairs()
+    finally:
+        co.set_option("segment_sites", 4096)
+    pads, seg = device_order(parts, 256)
+    assert len(seg) - 1 >= 8
+    pairs = np.stack([out["x"], out["y"]], 1)
+    assert_bitexact(out, O.multi_fit_pairs(parts, pairs, order=O.ORDER_SEG, pads=pads, seg=seg))
 
 
 def test_multi_evaluate_bitexact(O, multi):
