@@ -379,8 +379,10 @@ int kestrel_cohort_create_gl_multi(kestrel_cohort** out, uint32_t n_samples, con
     }
     for (int q = 0; q < 3; q++) c->part_sites[q] = (uint32_t)pm[q].site.size();
 
-    auto build_part = [&](int q, uint32_t v_pad, DevBuf<double>& rows, DevBuf<uint32_t>& mask,
-                          DevBuf<double>& af) -> int {
+    // compact: 16-byte rows of the 2-allele part (as create_gl makes them);
+    // *lacks reports a called row with no exact 1.0
+    auto build_part = [&](int q, uint32_t v_pad, DevBuf<double>& rows, DevBuf<uint32_t>& mask, DevBuf<double>& af,
+                          int compact, int* lacks) -> int {
         const int A = q + 2, G = A * (A + 1) / 2;
         const uint32_t n_t = (uint32_t)pm[q].site.size(), row_words = v_pad / 16;
         DevBuf<uint32_t> dsite;
@@ -393,10 +395,14 @@ int kestrel_cohort_create_gl_multi(kestrel_cohort** out, uint32_t n_samples, con
             HIPCHK(hipMemcpyAsync(dal.p, pm[q].alleles.data(), n_t, hipMemcpyHostToDevice, c->stream));
             HIPCHK(hipMemcpyAsync(dgm.p, pm[q].gmap.data(), pm[q].gmap.size(), hipMemcpyHostToDevice, c->stream));
         }
-        HIPCHK(rows.alloc((size_t)n_samples * v_pad * G));
+        DevBuf<int> no_one;
+        HIPCHK(no_one.alloc(1));
+        HIPCHK(hipMemsetAsync(no_one.p, 0, sizeof(int), c->stream));
+        HIPCHK(rows.alloc((size_t)n_samples * v_pad * (compact ? 2 : G)));
         HIPCHK(mask.alloc((size_t)n_samples * row_words));
         HIPCHK(klaunch::gather_multi(dgl[0].p, dgl[1].p, dgl[2].p, n_samples, dsite.p, dal.p, dgm.p, n_t, G, rows.p,
-                                     v_pad, mask.p, row_words, c->stream));
+                                     v_pad, mask.p, row_words, c->stream, compact, no_one.p));
+        if (lacks) HIPCHK(hipMemcpyAsync(lacks, no_one.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         std::vector<double> afp((size_t)v_pad * A, 1.0 / (double)A);  // padding sites are missing for every sample
         std::copy(pm[q].af.begin(), pm[q].af.end(), afp.begin());
         HIPCHK(af.alloc(afp.size()));
@@ -408,8 +414,15 @@ int kestrel_cohort_create_gl_multi(kestrel_cohort** out, uint32_t n_samples, con
     c->n_kept = c->part_sites[0];
     c->v_pad = std::max<uint32_t>(SITE_PAD, (c->n_kept + SITE_PAD - 1) / SITE_PAD * SITE_PAD);
     c->row_words = c->v_pad / 16;
-    rc = build_part(0, c->v_pad, c->glrows, c->geno, c->af);
+    int lacks = 0;
+    rc = build_part(0, c->v_pad, c->glrows, c->geno, c->af, 1, &lacks);
     if (rc) return rc;
+    c->gl_stride = 2;
+    if (lacks) {
+        rc = build_part(0, c->v_pad, c->glrows, c->geno, c->af, 0, nullptr);
+        if (rc) return rc;
+        c->gl_stride = 3;
+    }
     for (int q = 1; q < 3; q++) {
         if (!c->part_sites[q]) continue;
         std::unique_ptr<SitePart> p(new SitePart());
@@ -417,7 +430,7 @@ int kestrel_cohort_create_gl_multi(kestrel_cohort** out, uint32_t n_samples, con
         p->n_sites = c->part_sites[q];
         p->v_pad = (p->n_sites + SITE_PAD - 1) / SITE_PAD * SITE_PAD;
         p->row_words = p->v_pad / 16;
-        rc = build_part(q, p->v_pad, p->rows, p->mask, p->af);
+        rc = build_part(q, p->v_pad, p->rows, p->mask, p->af, 0, nullptr);
         if (rc) return rc;
         HIPCHK(p->table.alloc((size_t)p->v_pad * klaunch::iis_table_size(p->alleles)));
         HIPCHK(klaunch::iis_table(p->alleles, p->af.p, p->v_pad, p->table.p, c->stream));

@@ -991,20 +991,20 @@ __device__ __forceinline__ void iis_apply(double* cv, double g, const double* T,
 }
 
 template <int A, int C>
-__device__ __forceinline__ void iis_pair(const double* __restrict__ px, const double* ly, const double* T,
+__device__ __forceinline__ void iis_pair(const double* lx, const double* ly, const double* T,
                                          double* cv) {
     constexpr IisTable<A> tab{};
     constexpr int G = IisTable<A>::G, MODE = tab.mode[C], OFF = tab.off[C];
-    const double g = px[C / G] * ly[C % G];
+    const double g = lx[C / G] * ly[C % G];
     iis_apply<MODE, OFF>(cv, g, T, std::make_integer_sequence<int, iis_nnz(MODE)>{});
 }
 
 // mixture.rs:8-60 from a site's coefficient table: every (gx, gy) in genotype
 // order, g = lx[gx] ly[gy], then its mode's fmas.
 template <int A, int... Cs>
-__device__ __forceinline__ void mixture_table(const double* __restrict__ px, const double* ly, const double* T,
+__device__ __forceinline__ void mixture_table(const double* lx, const double* ly, const double* T,
                                               double* cv, std::integer_sequence<int, Cs...>) {
-    (iis_pair<A, Cs>(px, ly, T, cv), ...);
+    (iis_pair<A, Cs>(lx, ly, T, cv), ...);
 }
 
 // Each site's coefficient table: thread (site, genotype pair).
@@ -1026,7 +1026,12 @@ __global__ __launch_bounds__(64) void iis_table_kernel(const double* __restrict_
 }
 
 // The sweep over sites with A = 3 or 4 alleles: as sweep_gl with rows of
-// G = A(A+1)/2 likelihoods per site and the generic mixture.
+// G = A(A+1)/2 likelihoods per site and the generic mixture.  A site's
+// coefficient table (NC doubles: 132 for A = 3, 300 for A = 4) is the same for
+// every lane; the block stages a chunk's tables in LDS.  Each lane reads its
+// own pair's two rows, 8G bytes a site: with A = 3 the next site's rows are
+// loaded while this site is computed (with A = 4 the extra 40 VGPRs would
+// spill; its rows are loaded at the site).
 template <int A>
 __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __restrict__ geno, uint32_t row_words,
                                                            const double* __restrict__ gl, uint32_t v_pad,
@@ -1038,9 +1043,8 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __rest
                                                            const int* __restrict__ active, uint32_t nslots,
                                                            double eps, double* __restrict__ partial) {
     constexpr IisTable<A> tab;
-    constexpr Genotypes<A> gt;
     constexpr int G = A * (A + 1) / 2, NC = tab.off[G * G];
-    // the chunk's coefficient tables (NC doubles a site: 132 for A = 3, 300 for A = 4)
+    constexpr bool PREFETCH = A == 3;
     __shared__ __attribute__((aligned(16))) double kt[CHUNK * NC];
     const uint32_t tid = threadIdx.x;
     const uint32_t slot = blockIdx.x * SWEEP_TPB + tid;
@@ -1070,47 +1074,62 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __rest
 #pragma unroll
     for (int k = 0; k < 45; k++) H[k] = 0.0;
 
-    const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
-    for (uint32_t cs = s0; cs < s1; cs += CHUNK) {
-        __syncthreads();  // the previous chunk's tables are read
-        {
-            const double2* src = reinterpret_cast<const double2*>(table + (size_t)cs * NC);
-            double2* dst = reinterpret_cast<double2*>(kt);
-            for (uint32_t i = tid; i < CHUNK * NC / 2; i += SWEEP_TPB) dst[i] = src[i];
+    // site t's two rows (G doubles each, 16-byte aligned: G is even)
+    auto load_rows = [&](uint32_t t, double* rx, double* ry) {
+        const double2* ax = reinterpret_cast<const double2*>(glx + (size_t)t * G);
+        const double2* ay = reinterpret_cast<const double2*>(gly + (size_t)t * G);
+#pragma unroll
+        for (int q = 0; q < G / 2; q++) {
+            const double2 u = ax[q], v = ay[q];
+            rx[2 * q] = u.x;
+            rx[2 * q + 1] = u.y;
+            ry[2 * q] = v.x;
+            ry[2 * q + 1] = v.y;
         }
-        const uint2 wx = *reinterpret_cast<const uint2*>(rowx + cs / 16);
-        const uint2 wy = *reinterpret_cast<const uint2*>(rowy + cs / 16);
-        {
+    };
+    // segment bounds are whole chunks: t - s0 counts chunks from 0
+    const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
+    double nx[G], ny[G];
+    if (PREFETCH && s0 < s1) load_rows(s0, nx, ny);
+    uint2 wx = make_uint2(0, 0), wy = make_uint2(0, 0);
+#pragma unroll 1
+    for (uint32_t t = s0; t < s1; t++) {
+        const int k = (int)((t - s0) & (CHUNK - 1));
+        if (k == 0) {
+            __syncthreads();  // the previous chunk's tables are read
+            {
+                const double2* src = reinterpret_cast<const double2*>(table + (size_t)t * NC);
+                double2* dst = reinterpret_cast<double2*>(kt);
+                for (uint32_t i = tid; i < CHUNK * NC / 2; i += SWEEP_TPB) dst[i] = src[i];
+            }
+            wx = *reinterpret_cast<const uint2*>(rowx + t / 16);
+            wy = *reinterpret_cast<const uint2*>(rowy + t / 16);
             const uint32_t ux = ~((wx.x & (wx.x >> 1)) | (wy.x & (wy.x >> 1))) & 0x55555555u;
             const uint32_t uy = ~((wx.y & (wx.y >> 1)) | (wy.y & (wy.y >> 1))) & 0x55555555u;
             N += (double)(__popc(ux) + __popc(uy));
+            __syncthreads();
         }
-        __syncthreads();
-#pragma unroll 1
-        for (int k = 0; k < CHUNK; k++) {
-            const uint32_t ax = (k < 16 ? wx.x : wx.y) >> (2 * (k & 15));
-            const uint32_t ay = (k < 16 ? wy.x : wy.y) >> (2 * (k & 15));
-            const bool missing = (ax & 3u) == 3u || (ay & 3u) == 3u;
-            double ly[G];
-            const double* px = glx + (size_t)(cs + k) * G;
-            const double2* py = reinterpret_cast<const double2*>(gly + (size_t)(cs + k) * G);
+        const uint32_t ax = (k < 16 ? wx.x : wx.y) >> (2 * (k & 15));
+        const uint32_t ay = (k < 16 ? wy.x : wy.y) >> (2 * (k & 15));
+        const bool missing = (ax & 3u) == 3u || (ay & 3u) == 3u;
+        double lx[G], ly[G];
+        if (PREFETCH) {
 #pragma unroll
-            for (int q = 0; q < G / 2; q++) {
-                const double2 vy = py[q];
-                ly[2 * q] = vy.x;
-                ly[2 * q + 1] = vy.y;
+            for (int q = 0; q < G; q++) {
+                lx[q] = nx[q];
+                ly[q] = ny[q];
             }
-            // mixture.rs:8-60: for every (gx, gy) in genotype order, g = lx ly and
-            // accumulate_iis with the site's coefficients of that pair's mode
-            // (x's likelihoods read per genotype: fewer registers)
-            const double* T = kt + k * NC;
-            double cv[9];
-#pragma unroll
-            for (int j = 0; j < 9; j++) cv[j] = 0.0;
-            mixture_table<A>(px, ly, T, cv, std::make_integer_sequence<int, G * G>{});
-            (void)gt;
-            site_update(cv, missing ? 1.0 : eps, D, k, P, F, Gr, H);
+            if (t + 1 < s1) load_rows(t + 1, nx, ny);
+        } else {
+            load_rows(t, lx, ly);
         }
+        // mixture.rs:8-60: for every (gx, gy) in genotype order, g = lx ly and
+        // accumulate_iis with the site's coefficients of that pair's mode
+        double cv[9];
+#pragma unroll
+        for (int j = 0; j < 9; j++) cv[j] = 0.0;
+        mixture_table<A>(lx, ly, kt + k * NC, cv, std::make_integer_sequence<int, G * G>{});
+        site_update(cv, missing ? 1.0 : eps, D, k, P, F, Gr, H);
     }
     if (live) {
         double* out = partial + (size_t)seg * NPART * nslots + slot;
@@ -1123,12 +1142,31 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __rest
     }
 }
 
+// A compact 16-byte row: the two entries other than the first exact 1.0, in
+// order; the 1.0's position is the row's 2-bit code (3 = missing: the stored
+// pair is 0, 0).  A called row with no exact 1.0 sets *lacks (the caller then
+// builds 24-byte rows instead).
+__device__ __forceinline__ uint32_t store_compact(double* dst, const double* l, uint32_t code, int* lacks) {
+    if (code != 3) {
+        code = l[0] == 1.0 ? 0u : l[1] == 1.0 ? 1u : l[2] == 1.0 ? 2u : 3u;
+        if (code == 3) {
+            *lacks = 1;
+            code = 0;
+        }
+    }
+    dst[0] = code == 0 ? l[1] : l[0];
+    dst[1] = code == 2 ? l[1] : l[2];
+    return code;
+}
+
 // Re-bucketed rows (wrapper.rs:237-410 remove_zero_frequencies): new site t of
 // a part with Gp genotypes takes, from site map_site[t] of the input bucket
 // with map_alleles[t] alleles, the likelihoods at gmap[t][0..Gp) (the
 // genotypes of the alleles with non-zero frequency); thread (sample i, word w)
 // writes sites 16w..16w+15 of row i (0 for a missing call, a NaN among the
 // selected entries) and the 2-bit mask word (3 = missing, padding = 3).
+// With compact (3 genotypes only) the rows are 16-byte ones as store_compact
+// makes them.
 __global__ __launch_bounds__(256) void gather_multi_kernel(const double* __restrict__ src2,
                                                             const double* __restrict__ src3,
                                                             const double* __restrict__ src4, uint32_t n_samples,
@@ -1136,15 +1174,34 @@ __global__ __launch_bounds__(256) void gather_multi_kernel(const double* __restr
                                                             const uint8_t* __restrict__ map_alleles,
                                                             const uint8_t* __restrict__ gmap, uint32_t n_t, int gp,
                                                             double* __restrict__ rows, uint32_t v_pad,
-                                                            uint32_t* __restrict__ mask, uint32_t row_words) {
+                                                            uint32_t* __restrict__ mask, uint32_t row_words,
+                                                            int compact, int* __restrict__ no_one) {
     const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= (size_t)n_samples * row_words) return;
     const uint32_t i = (uint32_t)(tid / row_words), w = (uint32_t)(tid % row_words);
     uint32_t word = 0;
+    int lacks_one = 0;
     for (int j = 0; j < 16; j++) {
         const uint32_t t = w * 16 + j;
-        double* dst = rows + ((size_t)i * v_pad + t) * gp;
         uint32_t code = 3;
+        if (compact) {  // gp == 3: the biallelic part
+            double l[3] = {0.0, 0.0, 0.0};
+            if (t < n_t) {
+                const int na = map_alleles[t];
+                const int gs = na * (na + 1) / 2;
+                const double* src =
+                    (na == 2 ? src2 : na == 3 ? src3 : src4) + ((size_t)map_site[t] * n_samples + i) * gs;
+                for (int k = 0; k < 3; k++) l[k] = src[gmap[(size_t)t * 3 + k]];
+                if (!(isnan(l[0]) || isnan(l[1]) || isnan(l[2])))
+                    code = 0;
+                else
+                    l[0] = l[1] = l[2] = 0.0;
+            }
+            code = store_compact(rows + ((size_t)i * v_pad + t) * 2, l, code, &lacks_one);
+            word |= code << (2 * j);
+            continue;
+        }
+        double* dst = rows + ((size_t)i * v_pad + t) * gp;
         if (t < n_t) {
             const int na = map_alleles[t];
             const int gs = na * (na + 1) / 2;
@@ -1161,6 +1218,7 @@ __global__ __launch_bounds__(256) void gather_multi_kernel(const double* __restr
         word |= code << (2 * j);
     }
     mask[(size_t)i * row_words + w] = word;
+    if (lacks_one) atomicOr(no_one, 1);
 }
 
 // Soft rows of the kept sites: thread (sample i, word w) moves sites
@@ -1192,19 +1250,7 @@ __global__ __launch_bounds__(256) void gather_gl_kernel(const double* __restrict
             }
         }
         if (compact) {
-            // 16-byte row: the two entries other than the first exact 1.0, in
-            // order; the 1.0's position is the row's 2-bit code (3 = missing:
-            // the stored pair is 0, 0)
-            double* dst = rows + ((size_t)i * v_pad + t) * 2;
-            if (code != 3) {
-                code = l[0] == 1.0 ? 0u : l[1] == 1.0 ? 1u : l[2] == 1.0 ? 2u : 3u;
-                if (code == 3) {  // no entry is exactly 1: the caller falls back to 24-byte rows
-                    lacks_one = 1;
-                    code = 0;
-                }
-            }
-            dst[0] = code == 0 ? l[1] : l[0];
-            dst[1] = code == 2 ? l[1] : l[2];
+            code = store_compact(rows + ((size_t)i * v_pad + t) * 2, l, code, &lacks_one);
         } else {
             double* dst = rows + ((size_t)i * v_pad + t) * 3;
             dst[0] = l[0];
@@ -2035,11 +2081,13 @@ hipError_t iis_table(int alleles, const double* af, uint32_t n_sites, double* ta
 
 hipError_t gather_multi(const double* src2, const double* src3, const double* src4, uint32_t n_samples,
                         const uint32_t* map_site, const uint8_t* map_alleles, const uint8_t* gmap, uint32_t n_t,
-                        int gp, double* rows, uint32_t v_pad, uint32_t* mask, uint32_t row_words, hipStream_t st) {
+                        int gp, double* rows, uint32_t v_pad, uint32_t* mask, uint32_t row_words, hipStream_t st,
+                        int compact, int* no_one) {
     const size_t n = (size_t)n_samples * row_words;
     if (n == 0) return hipSuccess;
+    if (compact && gp != 3) return hipErrorInvalidValue;
     hipLaunchKernelGGL(gather_multi_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src2, src3, src4,
-                       n_samples, map_site, map_alleles, gmap, n_t, gp, rows, v_pad, mask, row_words);
+                       n_samples, map_site, map_alleles, gmap, n_t, gp, rows, v_pad, mask, row_words, compact, no_one);
     return hipGetLastError();
 }
 

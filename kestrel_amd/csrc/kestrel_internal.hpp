@@ -68,7 +68,8 @@ int iis_table_size(int alleles);
 hipError_t iis_table(int alleles, const double* af, uint32_t n_sites, double* table, hipStream_t st);
 hipError_t gather_multi(const double* src2, const double* src3, const double* src4, uint32_t n_samples,
                         const uint32_t* map_site, const uint8_t* map_alleles, const uint8_t* gmap, uint32_t n_t,
-                        int gp, double* rows, uint32_t v_pad, uint32_t* mask, uint32_t row_words, hipStream_t st);
+                        int gp, double* rows, uint32_t v_pad, uint32_t* mask, uint32_t row_words, hipStream_t st,
+                        int compact = 0, int* no_one = nullptr);
 hipError_t gather(const uint8_t* src, size_t src_row_bytes, const uint32_t* kept, uint32_t n_kept,
                   uint32_t n_samples, uint32_t* dst, uint32_t dst_row_words, hipStream_t st,
                   const uint32_t* dst_rows = nullptr);

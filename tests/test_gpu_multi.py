@@ -107,6 +107,22 @@ def test_multi_parts_cut_like_the_main_part(O, multi):
     assert_bitexact(out, O.multi_fit_pairs(parts, pairs, order=O.ORDER_SEG, pads=pads, seg=seg))
 
 
+@pytest.mark.parametrize("scale", [1.0, 0.5])
+def test_multi_biallelic_rows_compact_and_full(O, scale):
+    """The 2-allele part is held as 16-byte rows when every called row has an
+    exact 1.0 (scale 1: rows scaled by their maximum) and as 24-byte rows when
+    some has none (scale 0.5); both give the oracle's records bit for bit."""
+    say(f"case: multi-allelic fit, 2-allele rows scaled by {scale}")
+    b2, b3, b4 = synth.multi_likelihoods(14, (300, 90, 40), seed=53, absent=0.0)
+    b2 = b2 * scale
+    with Cohort.gl_multi(b2, b3, b4) as co:
+        out = co.fit_pairs()
+    parts, _ = O.multi_prepare({2: b2, 3: b3, 4: b4})
+    pairs = np.stack([out["x"], out["y"]], 1)
+    pads, seg = device_order(parts)
+    assert_bitexact(out, O.multi_fit_pairs(parts, pairs, order=O.ORDER_SEG, pads=pads, seg=seg))
+
+
 def test_multi_evaluate_bitexact(O, multi):
     co, parts, af_in, _ = multi
     say("case: multi-allelic evaluate")
