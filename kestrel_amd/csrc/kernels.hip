@@ -1029,9 +1029,10 @@ __global__ __launch_bounds__(64) void iis_table_kernel(const double* __restrict_
 // G = A(A+1)/2 likelihoods per site and the generic mixture.  A site's
 // coefficient table (NC doubles: 132 for A = 3, 300 for A = 4) is the same for
 // every lane; the block stages a chunk's tables in LDS.  Each lane reads its
-// own pair's two rows, 8G bytes a site: with A = 3 the next site's rows are
-// loaded while this site is computed (with A = 4 the extra 40 VGPRs would
-// spill; its rows are loaded at the site).
+// own pair's two rows, 8G bytes a site, and loads the next site's rows while
+// this site is computed: both rows with A = 3; with A = 4 only y's (both would
+// take 40 more VGPRs and spill), as pairs are claimed in sample tiles, so a
+// block's lanes share a few x rows, which stay in cache, and read many y rows.
 template <int A>
 __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __restrict__ geno, uint32_t row_words,
                                                            const double* __restrict__ gl, uint32_t v_pad,
@@ -1044,7 +1045,7 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __rest
                                                            double eps, double* __restrict__ partial) {
     constexpr IisTable<A> tab;
     constexpr int G = A * (A + 1) / 2, NC = tab.off[G * G];
-    constexpr bool PREFETCH = A == 3;
+    constexpr bool PREFETCH_X = A == 3;
     __shared__ __attribute__((aligned(16))) double kt[CHUNK * NC];
     const uint32_t tid = threadIdx.x;
     const uint32_t slot = blockIdx.x * SWEEP_TPB + tid;
@@ -1074,23 +1075,23 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __rest
 #pragma unroll
     for (int k = 0; k < 45; k++) H[k] = 0.0;
 
-    // site t's two rows (G doubles each, 16-byte aligned: G is even)
-    auto load_rows = [&](uint32_t t, double* rx, double* ry) {
-        const double2* ax = reinterpret_cast<const double2*>(glx + (size_t)t * G);
-        const double2* ay = reinterpret_cast<const double2*>(gly + (size_t)t * G);
+    // site t's row of one sample (G doubles, 16-byte aligned: G is even)
+    auto load_row = [&](const double* base, uint32_t t, double* r) {
+        const double2* a = reinterpret_cast<const double2*>(base + (size_t)t * G);
 #pragma unroll
         for (int q = 0; q < G / 2; q++) {
-            const double2 u = ax[q], v = ay[q];
-            rx[2 * q] = u.x;
-            rx[2 * q + 1] = u.y;
-            ry[2 * q] = v.x;
-            ry[2 * q + 1] = v.y;
+            const double2 u = a[q];
+            r[2 * q] = u.x;
+            r[2 * q + 1] = u.y;
         }
     };
     // segment bounds are whole chunks: t - s0 counts chunks from 0
     const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
-    double nx[G], ny[G];
-    if (PREFETCH && s0 < s1) load_rows(s0, nx, ny);
+    double nx[PREFETCH_X ? G : 1], ny[G];
+    if (s0 < s1) {
+        if (PREFETCH_X) load_row(glx, s0, nx);
+        load_row(gly, s0, ny);
+    }
     uint2 wx = make_uint2(0, 0), wy = make_uint2(0, 0);
 #pragma unroll 1
     for (uint32_t t = s0; t < s1; t++) {
@@ -1113,15 +1114,15 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __rest
         const uint32_t ay = (k < 16 ? wy.x : wy.y) >> (2 * (k & 15));
         const bool missing = (ax & 3u) == 3u || (ay & 3u) == 3u;
         double lx[G], ly[G];
-        if (PREFETCH) {
 #pragma unroll
-            for (int q = 0; q < G; q++) {
-                lx[q] = nx[q];
-                ly[q] = ny[q];
-            }
-            if (t + 1 < s1) load_rows(t + 1, nx, ny);
-        } else {
-            load_rows(t, lx, ly);
+        for (int q = 0; q < G; q++) {
+            if (PREFETCH_X) lx[q] = nx[q];
+            ly[q] = ny[q];
+        }
+        if (!PREFETCH_X) load_row(glx, t, lx);
+        if (t + 1 < s1) {
+            if (PREFETCH_X) load_row(glx, t + 1, nx);
+            load_row(gly, t + 1, ny);
         }
         // mixture.rs:8-60: for every (gx, gy) in genotype order, g = lx ly and
         // accumulate_iis with the site's coefficients of that pair's mode
