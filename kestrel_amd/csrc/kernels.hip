@@ -767,125 +767,11 @@ __host__ __device__ constexpr int iis_mode(int i, int j, int k, int l) {
          : key == 0x08 ? 12 : key == 0x04 ? 13 : key == 0x02 ? 14 : key == 0x00 ? 15 : 0;
 }
 
-// iis.rs:106-277 accumulate_iis1..15, in their operation order; mode is a
-// constant wherever the genotype loops are unrolled.
-__device__ __forceinline__ void accumulate_iis(double* a, double g, double pi, double pj, double pk, double pl,
-                                               int mode) {
-    switch (mode) {
-    case 1: {
-        const double pi2 = pi * pi, pi3 = pi2 * pi;
-        a[0] = fma(g, pi, a[0]);
-        a[1] = fma(g, pi2, a[1]);
-        a[2] = fma(g, pi2, a[2]);
-        a[3] = fma(g, pi3, a[3]);
-        a[4] = fma(g, pi2, a[4]);
-        a[5] = fma(g, pi3, a[5]);
-        a[6] = fma(g, pi2, a[6]);
-        a[7] = fma(g, pi3, a[7]);
-        a[8] = fma(g, pi2 * pi2, a[8]);
-    } break;
-    case 2: {
-        const double pipk = pi * pk;
-        a[1] = fma(g, pipk, a[1]);
-        a[3] = fma(g, pipk * pk, a[3]);
-        a[5] = fma(g, pi * pipk, a[5]);
-        a[8] = fma(g, pipk * pipk, a[8]);
-    } break;
-    case 3: {
-        const double pipl = pi * pl, pi2pl = pi * pipl, two = pi2pl + pi2pl;
-        a[2] = fma(g, pipl, a[2]);
-        a[3] = fma(g, two, a[3]);
-        a[7] = fma(g, pi2pl, a[7]);
-        a[8] = fma(g, two * pi, a[8]);
-    } break;
-    case 4: {
-        const double pipk = pi * pk, pi2pk = pi * pipk, two = pi2pk + pi2pk;
-        a[2] = fma(g, pipk, a[2]);
-        a[3] = fma(g, two, a[3]);
-        a[7] = fma(g, pi2pk, a[7]);
-        a[8] = fma(g, two * pi, a[8]);
-    } break;
-    case 5: {
-        const double v = (pi * pk) * pl, two = v + v;
-        a[3] = fma(g, two, a[3]);
-        a[8] = fma(g, two * pi, a[8]);
-    } break;
-    case 6: {
-        const double pipj = pi * pj, pi2pj = pi * pipj, two = pi2pj + pi2pj;
-        a[4] = fma(g, pipj, a[4]);
-        a[5] = fma(g, two, a[5]);
-        a[7] = fma(g, pi2pj, a[7]);
-        a[8] = fma(g, two * pi, a[8]);
-    } break;
-    case 7: {
-        const double pipj = pi * pj, pj2pi = pj * pipj, two = pj2pi + pj2pi;
-        a[4] = fma(g, pipj, a[4]);
-        a[5] = fma(g, two, a[5]);
-        a[7] = fma(g, pj2pi, a[7]);
-        a[8] = fma(g, two * pj, a[8]);
-    } break;
-    case 8: {
-        const double v = (pi * pj) * pk, two = v + v;
-        a[5] = fma(g, two, a[5]);
-        a[8] = fma(g, two * pk, a[8]);
-    } break;
-    case 9:
-    case 10: {  // mode 10 is mode 9 (iis.rs:213-215)
-        const double pipj = pi * pj, two = pipj + pipj;
-        a[6] = fma(g, two, a[6]);
-        a[7] = fma(g, pipj * (pi + pj), a[7]);
-        a[8] = fma(g, two * two, a[8]);
-    } break;
-    case 11: {
-        const double v = (pi * pj) * pl;
-        a[7] = fma(g, v, a[7]);
-        a[8] = fma(g, (4.0 * v) * pi, a[8]);
-    } break;
-    case 12: {
-        const double v = (pi * pj) * pk;
-        a[7] = fma(g, v, a[7]);
-        a[8] = fma(g, (4.0 * v) * pi, a[8]);
-    } break;
-    case 13: {
-        const double v = (pi * pj) * pl;
-        a[7] = fma(g, v, a[7]);
-        a[8] = fma(g, (4.0 * v) * pj, a[8]);
-    } break;
-    case 14: {
-        const double v = (pi * pj) * pk;
-        a[7] = fma(g, v, a[7]);
-        a[8] = fma(g, (4.0 * v) * pj, a[8]);
-    } break;
-    case 15: {
-        const double v = (pi * pj) * (pk * pl);
-        a[8] = fma(g, 4.0 * v, a[8]);
-    } break;
-    default:
-        break;
-    }
-}
-
-// mixture.rs:8-60 over IIS_A: for every (gx, gy) in genotype order,
-// g = lx[gx] ly[gy] and accumulate_iis with the four alleles' frequencies.
-template <int A>
-__device__ __forceinline__ void mixture_generic(const double* lx, const double* ly, const double* p, double* a) {
-    constexpr Genotypes<A> gt;
-    constexpr int G = Genotypes<A>::G;
-#pragma unroll
-    for (int k = 0; k < 9; k++) a[k] = 0.0;
-#pragma unroll
-    for (int gx = 0; gx < G; gx++)
-#pragma unroll
-        for (int gy = 0; gy < G; gy++) {
-            const int i = gt.gi[gx], j = gt.gj[gx], k = gt.gi[gy], l = gt.gj[gy];
-            accumulate_iis(a, lx[gx] * ly[gy], p[i], p[j], p[k], p[l], iis_mode(i, j, k, l));
-        }
-}
-
-// accumulate_iis split in two: the coefficients of a mode depend only on the
-// site's frequencies (iis_coefs writes them once per site, in accumulate_iis's
-// expressions and order), and a lane applies them to its g = lx ly with the
-// same fma sequence (iis_target: the component each coefficient goes to).
+// iis.rs:106-277 accumulate_iis1..15 split in two: the coefficients of a mode
+// depend only on the site's frequencies (iis_coefs writes them once per site,
+// in accumulate_iis's expressions and order), and a lane applies them to its
+// g = lx ly with the same fma sequence (iis_nnz: how many a mode has;
+// iis_target: the component of the 9-vector each one goes to).
 __host__ __device__ constexpr int iis_nnz(int mode) {
     return mode == 1 ? 9 : (mode >= 2 && mode <= 4) || mode == 6 || mode == 7 ? 4 : mode == 5 || mode == 8 ? 2
          : mode == 9 || mode == 10 ? 3 : mode >= 11 && mode <= 14 ? 2 : 1;
@@ -938,7 +824,7 @@ __device__ __forceinline__ void iis_coefs(int mode, double pi, double pj, double
         o[0] = two; o[1] = two * pk;
     } break;
     case 9:
-    case 10: {
+    case 10: {  // mode 10 is mode 9 (iis.rs:213-215)
         const double pipj = pi * pj, two = pipj + pipj;
         o[0] = two; o[1] = pipj * (pi + pj); o[2] = two * two;
     } break;
@@ -984,7 +870,7 @@ struct IisTable {
 };
 
 // One genotype pair's fmas with every index a template constant (a fold over
-// the mode's coefficients, left to right: accumulate_iis's order).
+// the mode's coefficients, left to right: iis.rs's accumulate_iis order).
 template <int MODE, int OFF, int... Ts>
 __device__ __forceinline__ void iis_apply(double* cv, double g, const double* T, std::integer_sequence<int, Ts...>) {
     ((cv[iis_target(MODE, Ts)] = fma(g, T[OFF + Ts], cv[iis_target(MODE, Ts)])), ...);
