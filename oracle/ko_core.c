@@ -1,8 +1,8 @@
 /*
  * ko_core.c -- oracle (test infrastructure only; see kestrel_oracle.h):
  * identity-state tables, c_s construction, segmented-order reductions,
- * GMW81 / LDL^T, the active-set QP, the condition estimate, the SQP driver,
- * the allele-frequency fit and threaded drivers.  Compiled with
+ * GMW81 / LDL^T, the active-set QP, the condition estimate, the SQP loop,
+ * the allele-frequency fit and the threaded pair loops.  Compiled with
  * -ffp-contract=off: every fused multiply-add below is an explicit fma()
  * exactly where the reference writes mul_add / .fma().
  */
@@ -1128,7 +1128,7 @@ int ko_allele_freqs_gl(const double* gl, uint32_t N, uint32_t V, uint64_t iterat
     return allele_freqs(NULL, gl, N, V, iterations, maf, nthreads, af_out);
 }
 
-/* ------------------------------------------------------- pair driver */
+/* --------------------------------------------------------- pair loop */
 
 typedef struct {
     int order;
