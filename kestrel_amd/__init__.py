@@ -6,7 +6,8 @@ this package is its Python host binding plus the seeded synthetic cohort generat
 from . import synth
 from ._lib import OUTPUT_DTYPE, KestrelError, Tune
 from .api import (Coefficients, Cohort, build_arrays, kinship, relatedness_coefficients_gl,
-                  relatedness_coefficients_gt, segment_bounds, tune_default)
+                  relatedness_coefficients_gt, run_cli, segment_bounds, tune_default)
 
 __all__ = ["Coefficients", "Cohort", "KestrelError", "OUTPUT_DTYPE", "Tune", "build_arrays", "kinship",
-           "relatedness_coefficients_gl", "relatedness_coefficients_gt", "segment_bounds", "synth", "tune_default"]
+           "relatedness_coefficients_gl", "relatedness_coefficients_gt", "run_cli", "segment_bounds", "synth",
+           "tune_default"]

@@ -5,6 +5,7 @@ a cohort of hard calls in, one Output record per pair out (coefficients.rs:143-1
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -485,3 +486,33 @@ def relatedness_coefficients_gl(genotype_likelihoods, allele_frequencies=None, t
     finally:
         co.close()
     return build_arrays(out, n)
+
+
+def run_cli(argv=None) -> None:
+    """The kestrel command line from Python (lib.rs:111-117 `run_cli`, the
+    module's console-script entry point, pyproject.toml:17-18): the arguments
+    are `sys.argv` (argv[0] the program name) and the shipped `kestrel` binary
+    runs them as a child process, its log lines passed through to stderr.
+    As with clap's `Args::parse_from` in cli.rs:73, a usage error ends the
+    process with status 2 and `--help` / `--version` with status 0
+    (SystemExit); a failure of the run raises RuntimeError with the command's
+    error message (the reference's `PyRuntimeError(format!("{:#}", e))`)."""
+    import subprocess
+    import sys
+
+    args = [str(a) for a in (sys.argv if argv is None else argv)]
+    exe = os.path.join(_lib.HERE, "kestrel")
+    if not os.path.exists(exe):
+        raise ImportError(f"{exe} is not built; run __graft_entry__.build() (make -C kestrel_amd)")
+    err = []
+    with subprocess.Popen([exe] + args[1:], stderr=subprocess.PIPE, text=True) as p:
+        for line in p.stderr:
+            sys.stderr.write(line)
+            err.append(line)
+    if p.returncode == 2:  # usage error, reported by the argument parser
+        raise SystemExit(2)
+    if p.returncode != 0:
+        msg = [ln[len("Error: "):].rstrip("\n") for ln in err if ln.startswith("Error: ")]
+        raise RuntimeError(msg[-1] if msg else f"kestrel exited with status {p.returncode}")
+    if any(a in ("-h", "--help", "-V", "--version") for a in args[1:]):
+        raise SystemExit(0)

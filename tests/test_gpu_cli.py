@@ -185,3 +185,16 @@ def test_cli_multiallelic_matches_oracle(O, tmp_path, field):
     assert np.array_equal(obj, exp["obj"])
     assert np.array_equal(cond, log10(exp["cond"]), equal_nan=True)
     assert f"Identified {parts[3][0].shape[0]} tri-allelic sites" in log
+
+
+def test_run_cli_from_python_writes_the_same_tsv(tmp_path):
+    """kestrel_amd.run_cli (lib.rs:111-117) runs the same command line: the
+    TSV is byte-identical to the binary's."""
+    from kestrel_amd import run_cli
+    print("case: run_cli from Python vs the binary", flush=True)
+    c = synth.generate(24, 1500, seed=23, n_family_pairs=4, missing=0.01)
+    vcf = str(tmp_path / "c.vcf.gz")
+    synth.write_vcf(c, vcf, "GL")
+    run(vcf, str(tmp_path / "a.tsv"), "-f", "GL", "-i", "50")
+    assert run_cli(["kestrel", vcf, str(tmp_path / "b.tsv"), "-f", "GL", "-i", "50"]) is None
+    assert open(tmp_path / "a.tsv", "rb").read() == open(tmp_path / "b.tsv", "rb").read()

@@ -54,3 +54,20 @@ def test_build_arrays_matches_reference_assembly():
     assert kin[0, 1] == kin[1, 0] and abs(kin[0, 1] - k01) < 1e-15
     assert inb[1] == 1.0 and inb[0] == -1.0 and inb[2] == 0.0
     assert (jac[2] == 0).all() and (kin[2] == 0).all()
+
+
+def test_run_cli_exits_and_errors_like_the_reference(tmp_path, capfd):
+    """lib.rs:111-117 run_cli: clap's parse_from ends the process on a usage
+    error (2) and after --version (0); a failed run raises RuntimeError with
+    the error message.  None of these reaches the device."""
+    from kestrel_amd import run_cli
+    with pytest.raises(SystemExit) as e:
+        run_cli(["kestrel", "--version"])
+    assert e.value.code == 0
+    assert "kestrel" in capfd.readouterr().out
+    with pytest.raises(SystemExit) as e:
+        run_cli(["kestrel", str(tmp_path / "in.vcf"), str(tmp_path / "o.tsv"), "-f", "XX"])
+    assert e.value.code == 2
+    assert "invalid value" in capfd.readouterr().err
+    with pytest.raises(RuntimeError, match="No such file"):
+        run_cli(["kestrel", str(tmp_path / "missing.vcf"), str(tmp_path / "o.tsv")])
