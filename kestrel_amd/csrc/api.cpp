@@ -534,7 +534,7 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     if (!pairs) {
         if (n_select > c->n_samples)  // coefficients.rs:53-55
             return fail(KESTREL_EINVAL, "Requested samples %u exceeds total samples %u", n_select, c->n_samples);
-        n_pairs = (size_t)n_select * (n_select + 1) / 2;
+        n_pairs = (size_t)n_select * ((size_t)n_select + 1) / 2;
         tri.resize(2 * n_pairs);
         size_t i = 0;
         for (uint32_t x = 0; x < n_select; x++)
@@ -1227,7 +1227,7 @@ int kestrel_fit_pairs_gt(kestrel_cohort* c, uint64_t iterations, const uint32_t*
     if (!pairs) {
         if (n_select > c->n_samples)
             return fail(KESTREL_EINVAL, "Requested samples %u exceeds total samples %u", n_select, c->n_samples);
-        n_pairs = (size_t)n_select * (n_select + 1) / 2;
+        n_pairs = (size_t)n_select * ((size_t)n_select + 1) / 2;
         tri.resize(2 * n_pairs);
         size_t i = 0;
         for (uint32_t x = 0; x < n_select; x++)
