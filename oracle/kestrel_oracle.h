@@ -15,6 +15,19 @@
  *                  segment partials are folded left to right.  This is the
  *                  order the device kernels use, so device results can be
  *                  compared bit for bit.
+ *
+ * Pins (tests/test_oracle_*.py): the identity-state probabilities against an
+ * exact-rational brute force, the two reference encodings of c_s against each
+ * other, log against libm within 1 ulp, the QP against enumeration of all
+ * supports, GMW81 against its reconstruction, objective / gradient / Hessian
+ * against compensated sums and finite differences, allele frequencies against
+ * the closed-form root, fits against their KKT conditions, CLS Q and brier
+ * against numpy, and the golden fixture tests/golden/c1_small.npz.
+ * Parity unpinned against the running Rust reference (no Rust toolchain here,
+ * crates not vendored): exact iteration counts and last bits under the real
+ * _mm512_reduce_add_pd code generation, active-set tie-breaks on degenerate
+ * pairs, sites at exactly p = 1/(2N), and ryu's float text (restated as
+ * shortest round-trip digits).
  */
 #ifndef KESTREL_ORACLE_H
 #define KESTREL_ORACLE_H
