@@ -352,6 +352,8 @@ def main():
     ap.add_argument("--batch", type=int, default=None, help="iterations queued per host synchronisation")
     ap.add_argument("--graph", action="store_true", help="relaunch each batch from a captured HIP graph")
     ap.add_argument("--pair-tile", type=int, default=None, help="claim order in tiles of this many samples")
+    ap.add_argument("--skip-masked", type=int, choices=(-1, 0, 1), default=None,
+                    help="claims in runs + sweeps passing over masked sites (default: the library's automatic choice)")
     ap.add_argument("--autotune", type=int, default=0, metavar="PAIRS",
                     help="before the warmup, tune slots/streams/batch on this many random pairs (0 = off)")
     ap.add_argument("--serving", action="store_true", help="serving workload (default config c5)")
@@ -424,6 +426,8 @@ def main():
         co.set_option("graph", 1)
     if args.pair_tile is not None:
         co.set_option("pair_tile", args.pair_tile)
+    if args.skip_masked is not None:
+        co.set_option("skip_masked", args.skip_masked)
     if buckets:
         parts, _ = co.multi_info()
         kept = None
@@ -503,7 +507,8 @@ def main():
                    "input": ("soft genotype likelihoods" if args.soft else
                              "one-hot likelihoods, 2-4 alleles" if buckets else "hard calls"),
                    "launch_shape": ({k: tuned[k] for k in ("slots", "streams", "batch")} if tuned else "default"),
-                   "hip_graph": bool(args.graph)},
+                   "hip_graph": bool(args.graph),
+                   "skip_masked": "auto" if args.skip_masked is None else args.skip_masked},
         "roofline": roofline, "cpu_baseline": cpu,
         "iterations_mean": round(float(out["iters"].mean()), 3),
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,

@@ -190,7 +190,12 @@ int kestrel_cohort_create_gt_multi(kestrel_cohort** out, uint32_t n_samples, con
  * are taken in tiles of this many samples so that few sample rows are in
  * flight; 0 = list order, -1 = automatic: sqrt(pairs in flight), within 32
  * to 256, for likelihood-row cohorts, list order for 2-bit ones; records stay
- * in list order and do not change),
+ * in list order and do not change), "skip_masked" (2-bit cohorts: the
+ * 256-slot workgroups of a fit claim pairs in runs of consecutive claims that
+ * share their first sample, and a sweep passes over the sites where a
+ * wavefront's shared sample is missing, which add nothing; 1 = on, 0 = off, -1 = on when
+ * at least 2 % of the kept calls are missing (default); results do not
+ * change),
  * "serve_policy" (order in
  * which a serving engine started afterwards feeds admitted requests to the
  * device: 1 = shortest remaining first (default), 0 = round robin). */

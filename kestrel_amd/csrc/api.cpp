@@ -69,9 +69,9 @@ int repack_kept(kestrel_cohort* c, const uint8_t* dpacked, size_t row_stride_byt
 // One evaluation pass over every part of the cohort: the main part's segments,
 // then those of the tri/tetra-allelic parts, into consecutive partials.
 int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
-                  uint32_t ns, double eps, double* partial, hipStream_t st) {
+                  uint32_t ns, double eps, double* partial, hipStream_t st, bool skip_masked = false) {
     HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, sx, sy, pt, active, ns, eps, partial, st,
-                          c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride));
+                          c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride, skip_masked));
     size_t base = c->nseg;
     for (const auto& p : c->parts) {
         HIPCHK(klaunch::sweep_multi(p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->table.p, p->seg.p,
@@ -198,6 +198,16 @@ int kestrel_cohort_create(kestrel_cohort** out, const uint8_t* packed, uint32_t 
         }
     rc = repack_kept(c, dpacked.p, row_stride_bytes, kept);
     if (rc) return rc;
+    if (c->n_kept) {
+        DevBuf<unsigned long long> dmiss;
+        HIPCHK(dmiss.alloc(1));
+        HIPCHK(hipMemsetAsync(dmiss.p, 0, sizeof(unsigned long long), c->stream));
+        HIPCHK(klaunch::count_missing(c->geno.p, n_samples, c->row_words, c->n_kept, dmiss.p, c->stream));
+        unsigned long long miss = 0;
+        HIPCHK(hipMemcpyAsync(&miss, dmiss.p, sizeof(miss), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->missing_frac = (double)miss / ((double)n_samples * (double)c->n_kept);
+    }
     std::vector<double> afk((size_t)c->v_pad * 2, 0.5);
     for (uint32_t t = 0; t < c->n_kept; t++) {
         afk[2 * t] = c->af_all[2 * kept[t]];
@@ -491,6 +501,11 @@ int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value) {
         c->opt_pair_tile = value;
         return KESTREL_OK;
     }
+    if (!strcmp(key, "skip_masked")) {
+        if (value < -1 || value > 1) return fail(KESTREL_EINVAL, "skip_masked must be -1, 0 or 1");
+        c->opt_skip_masked = value;
+        return KESTREL_OK;
+    }
     if (!strcmp(key, "graph")) {
         if (value != 0 && value != 1) return fail(KESTREL_EINVAL, "graph must be 0 or 1");
         c->opt_graph = value;
@@ -608,6 +623,32 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
         HIPCHK(hipMemcpyAsync(dorder.p, order.data(), n_pairs * sizeof(uint64_t), hipMemcpyHostToDevice, st));
         dorder_p = dorder.p;
     }
+    // claims in runs and sweeps that pass over the sites where a wavefront's
+    // shared sample is missing (2-bit cohorts; option "skip_masked")
+    const bool use_runs = !c->soft && c->parts.empty() &&
+                          (c->opt_skip_masked == 1 || (c->opt_skip_masked < 0 && c->missing_frac >= 0.02));
+    DevBuf<uint64_t> druns;
+    RunClaim rclaim;
+    if (use_runs) {
+        // runs of the claim order, at least one workgroup (256 claims) long,
+        // cut where the first sample changes
+        std::vector<uint64_t> runs;
+        auto first_sample = [&](uint64_t idx) { return pairs[2 * (order.empty() ? idx : order[idx])]; };
+        uint64_t begin = 0;
+        for (uint64_t idx = 1; idx < n_pairs; idx++)
+            if (idx - begin >= SWEEP_TPB && first_sample(idx) != first_sample(idx - 1)) {
+                runs.push_back(begin);
+                runs.push_back(idx);
+                begin = idx;
+            }
+        runs.push_back(begin);
+        runs.push_back(n_pairs);
+        HIPCHK(druns.alloc(runs.size()));
+        HIPCHK(hipMemcpyAsync(druns.p, runs.data(), runs.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));  // runs goes out of scope
+        rclaim.runs = druns.p;
+        rclaim.nruns = runs.size() / 2;
+    }
     HIPCHK(hipMemsetAsync(dnext.p, 0, sizeof(unsigned long long), st));
     const int live0 = (int)ns * ngroups;
     HIPCHK(hipMemcpyAsync(dlive.p, &live0, sizeof(int), hipMemcpyHostToDevice, st));
@@ -623,6 +664,8 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
         DevBuf<double> pt, part;
         DevBuf<int> active, hist;
         DevBuf<SlotState> st;
+        DevBuf<unsigned long long> wrun;  // claims in runs: each workgroup's unclaimed run part
+        RunClaim rc;
         std::vector<hipEvent_t> ev;
         std::vector<int> hist_h;
         int live_before = 0;
@@ -662,6 +705,12 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
         HIPCHK(hipMemsetAsync(G.sx.p, 0, (size_t)ns * sizeof(uint32_t), G.stream));
         HIPCHK(hipMemsetAsync(G.sy.p, 0, (size_t)ns * sizeof(uint32_t), G.stream));
         HIPCHK(hipMemsetAsync(G.active.p, 0, (size_t)ns * sizeof(int), G.stream));
+        if (use_runs) {
+            HIPCHK(G.wrun.alloc((size_t)2 * (ns / SWEEP_TPB)));
+            HIPCHK(hipMemsetAsync(G.wrun.p, 0, (size_t)2 * (ns / SWEEP_TPB) * sizeof(unsigned long long), G.stream));
+            G.rc = rclaim;
+            G.rc.wrun = G.wrun.p;
+        }
         G.live_before = (int)ns;
     }
     HIPCHK(hipEventRecord(ref, st));
@@ -670,7 +719,7 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     for (int g = 0; g < ngroups; g++) {
         Group& G = grp[g];
         HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, c->nseg_total(), dpairs.p,
-                             n_pairs, dorder_p, dnext.p, dout.p, dt, dlive.p, PairQueue{}, nullptr, G.stream));
+                             n_pairs, dorder_p, dnext.p, dout.p, dt, dlive.p, PairQueue{}, nullptr, G.stream, G.rc));
     }
     // option "graph": each group's BATCH sweeps and steps are captured once
     // into a HIP graph (kernel nodes only; the histogram is cleared outside
@@ -682,12 +731,13 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
             HIPCHK(hipStreamBeginCapture(G.stream, hipStreamCaptureModeThreadLocal));
             int rs = KESTREL_OK;
             for (int i = 0; i < BATCH && rs == KESTREL_OK; i++) {
-                rs = launch_sweeps(c, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, tune->epsilon, G.part.p, G.stream);
+                rs = launch_sweeps(c, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, tune->epsilon, G.part.p, G.stream,
+                                   use_runs);
                 if (rs == KESTREL_OK) {
                     const hipError_t e = klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p,
                                                        c->nseg_total(), dpairs.p, n_pairs, dorder_p, dnext.p,
                                                        dout.p, dt,
-                                                       dlive.p, PairQueue{}, G.hist.p + i, G.stream);
+                                                       dlive.p, PairQueue{}, G.hist.p + i, G.stream, G.rc);
                     if (e != hipSuccess) rs = fail(KESTREL_EDEVICE, "HIP error %s", hipGetErrorString(e));
                 }
             }
@@ -726,11 +776,13 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
                 Group& G = grp[g];
                 if (G.live_before <= 0) continue;
                 HIPCHK(hipEventRecord(G.ev[2 * i], G.stream));
-                int rs = launch_sweeps(c, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, tune->epsilon, G.part.p, G.stream);
+                int rs = launch_sweeps(c, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, tune->epsilon, G.part.p, G.stream,
+                                       use_runs);
                 if (rs) return rs;
                 HIPCHK(hipEventRecord(G.ev[2 * i + 1], G.stream));
                 HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, c->nseg_total(), dpairs.p,
-                                     n_pairs, dorder_p, dnext.p, dout.p, dt, dlive.p, PairQueue{}, G.hist.p + i, G.stream));
+                                     n_pairs, dorder_p, dnext.p, dout.p, dt, dlive.p, PairQueue{}, G.hist.p + i, G.stream,
+                                     G.rc));
             }
         launched += BATCH;
         for (int g = 0; g < ngroups; g++) {

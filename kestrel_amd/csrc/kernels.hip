@@ -417,6 +417,13 @@ __device__ __forceinline__ void lut_rows(double p, double q, int part, double* b
 // One evaluation pass.  blockIdx.y = site segment, blockIdx.x * 256 +
 // threadIdx.x = slot (one pair per lane).  Partials (f, g[9], h[45], n) go to
 // partial[(seg * NPART + k) * nslots + slot].
+// SKIP: a wavefront whose live lanes all pair the same sample x passes over
+// the sites where x's call is missing.  Every lane's table row there is a
+// missing row (c = 0, L = 1): the site multiplies the log product by 1 and
+// adds fma(0, 1, s) = s to the gradient and Hessian sums (which start at +0
+// and only gain non-negative terms, so never hold -0), so passing over it
+// changes no bit; the log of a 16-site group is still taken at its last site.
+template <bool SKIP>
 __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __restrict__ geno, uint32_t row_words,
                                                             const double* __restrict__ af,
                                                             const uint32_t* __restrict__ seg_bounds,
@@ -454,6 +461,17 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
     }
     const uint32_t* rowx = geno + (size_t)x * row_words;
     const uint32_t* rowy = geno + (size_t)y * row_words;
+    // SKIP: the first live lane, and whether every live lane has its x
+    bool x_shared = false;
+    int x_lane = 0;
+    if (SKIP) {
+        const unsigned long long lm = __ballot(live);
+        if (lm) {
+            x_lane = __ffsll((long long)lm) - 1;
+            const uint32_t xf = (uint32_t)__builtin_amdgcn_readlane((int)x, x_lane);
+            x_shared = __all(!live || x == xf);
+        }
+    }
 
     double F = 0.0, N = 0.0, P = 1.0, G[9], H[45];
 #pragma unroll
@@ -503,8 +521,24 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
             const uint32_t uy = ~((wx.y & (wx.y >> 1)) | (wy.y & (wy.y >> 1))) & 0x55555555u;
             N += (double)(__popc(ux) + __popc(uy));
         }
+        // SKIP: the chunk's sites where the shared x is missing (bit 2k of
+        // the word of sites 16h..16h+15), uniform over the wavefront
+        uint32_t skip_lo = 0, skip_hi = 0;
+        if (SKIP && x_shared) {
+            const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)wx.x, x_lane);
+            const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)wx.y, x_lane);
+            skip_lo = a & (a >> 1) & 0x55555555u;
+            skip_hi = c & (c >> 1) & 0x55555555u;
+        }
 #pragma unroll
         for (int k = 0; k < CHUNK; k++) {
+            if (SKIP && (((k < 16 ? skip_lo : skip_hi) >> (2 * (k & 15))) & 1u)) {
+                if ((k & (LOG_GROUP - 1)) == LOG_GROUP - 1) {
+                    F = F + sleef_log_pos(P);
+                    P = 1.0;
+                }
+                continue;
+            }
             const uint32_t ax = (k < 16 ? wx.x : wx.y) >> (2 * (k & 15));
             const uint32_t ay = (k < 16 ? wy.x : wy.y) >> (2 * (k & 15));
             const uint32_t row = min(((ax & 3u) << 2) | (ay & 3u), 11u);
@@ -1108,6 +1142,27 @@ __global__ __launch_bounds__(256) void gather_multi_kernel(const double* __restr
     if (lacks_one) atomicOr(no_one, 1);
 }
 
+// Missing calls (code 3) among the first n_sites sites of every row: one
+// thread per row word (padding sites excluded), one atomic per wavefront.
+__global__ __launch_bounds__(256) void count_missing_kernel(const uint32_t* __restrict__ geno, uint32_t n_samples,
+                                                             uint32_t row_words, uint32_t n_sites,
+                                                             unsigned long long* __restrict__ count) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    uint32_t c = 0;
+    if (i < (size_t)n_samples * row_words) {
+        const uint32_t first = (uint32_t)(i % row_words) * 16;
+        const uint32_t v = geno[i];
+        uint32_t m = v & (v >> 1) & 0x55555555u;
+        if (first + 16 > n_sites) {
+            const uint32_t keep = n_sites > first ? n_sites - first : 0;
+            m &= (1u << (2 * keep)) - 1u;  // keep < 16
+        }
+        c = (uint32_t)__popc(m);
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if (__lane_id() == 0 && c) atomicAdd(count, (unsigned long long)c);
+}
+
 // Soft rows of the kept sites: thread (sample i, word w) moves sites
 // 16w..16w+15 from the site-major input (V x N x 3) into the sample-major
 // rows (0 for a missing call or a padding site) and writes the 2-bit mask word
@@ -1169,6 +1224,66 @@ __device__ __forceinline__ void write_point(double* pt, uint32_t nslots, uint32_
 
 }  // namespace
 
+// Claims of a workgroup's lanes from runs (the step kernel's workgroup is a
+// sweep workgroup's 256 slots): the lanes that want a pair take consecutive
+// claim indices, first from what is left of the workgroup's run, then from
+// the next unclaimed run (next_run counts runs handed out; a run is at least
+// 256 claims long unless it is the last, so one new run covers the rest).
+// Every thread of the workgroup calls it.  idx: the lane's claim index, or
+// -1; exhausted: no run was left for it.
+__device__ __forceinline__ void group_claim(const RunClaim& rc, unsigned long long* __restrict__ next_run, bool want,
+                                            int64_t& idx, bool& exhausted) {
+    __shared__ unsigned int wcnt[SWEEP_TPB / 64];
+    __shared__ unsigned long long grant[5];  // b1, n1, b2, n2, exhausted
+    const unsigned long long m = __ballot(want);
+    const int lane = (int)__lane_id(), wave = (int)(threadIdx.x / 64), nw = (int)(blockDim.x / 64);
+    if (lane == 0) wcnt[wave] = (unsigned int)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long cnt = 0;
+        for (int w = 0; w < nw; w++) cnt += wcnt[w];
+        unsigned long long* g = rc.wrun + 2 * blockIdx.x;
+        unsigned long long cur = g[0], end = g[1], b2 = 0, n2 = 0, ex = 0;
+        const unsigned long long n1 = min(cnt, end - cur), b1 = cur;
+        cur += n1;
+        if (n1 < cnt) {
+            const unsigned long long r = atomicAdd(next_run, 1ull);
+            if (r < rc.nruns) {
+                cur = rc.runs[2 * r];
+                end = rc.runs[2 * r + 1];
+                b2 = cur;
+                n2 = min(cnt - n1, end - cur);
+                cur += n2;
+            } else {
+                ex = 1;
+            }
+        }
+        if (cnt) {
+            g[0] = cur;
+            g[1] = end;
+        }
+        grant[0] = b1;
+        grant[1] = n1;
+        grant[2] = b2;
+        grant[3] = n2;
+        grant[4] = ex;
+    }
+    __syncthreads();
+    if (want) {
+        unsigned long long rank = (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
+        for (int w = 0; w < wave; w++) rank += wcnt[w];
+        const unsigned long long b1 = grant[0], n1 = grant[1], b2 = grant[2], n2 = grant[3];
+        if (rank < n1)
+            idx = (int64_t)(b1 + rank);
+        else if (rank - n1 < n2)
+            idx = (int64_t)(b2 + (rank - n1));
+        else
+            exhausted = grant[4] != 0;
+    }
+    __syncthreads();  // wcnt and grant are free for the next call
+}
+
+template <bool RUNS>
 __device__ __forceinline__ void step_slot(SlotState* __restrict__ st, uint32_t* __restrict__ sx,
                                           uint32_t* __restrict__ sy, double* __restrict__ pt,
                                           int* __restrict__ active, uint32_t nslots,
@@ -1178,9 +1293,12 @@ __device__ __forceinline__ void step_slot(SlotState* __restrict__ st, uint32_t* 
                                           unsigned long long* __restrict__ next_pair,
                                           kestrel_output* __restrict__ out, const Tune& tune,
                                           int* __restrict__ n_live, const PairQueue& q, uint32_t slot,
-                                          SlotState& s) {
-    bool claim = s.phase == PH_IDLE || s.phase == PH_WAIT;
-    if (!claim) {
+                                          SlotState& s, const RunClaim& rc, bool alive) {
+    // RUNS: batch claims in runs, by the whole workgroup (step_runs_kernel);
+    // alive: false for a retired slot (or one past nslots) there, which only
+    // takes part in the claim
+    bool claim = alive && (s.phase == PH_IDLE || s.phase == PH_WAIT);
+    if (alive && !claim) {
         // segment sums, left in segment 0 by fold_kernel
         double acc[NPART];
         for (int k = 0; k < NPART; k++) acc[k] = partial[(size_t)k * nslots + slot];
@@ -1293,10 +1411,31 @@ __device__ __forceinline__ void step_slot(SlotState* __restrict__ st, uint32_t* 
             }
         }
     }
+    // batch claims in runs: every thread of the workgroup reaches this point
+    int64_t run_idx = -1;
+    bool run_out = false;
+    if (RUNS) group_claim(rc, next_pair, claim && !q.cap, run_idx, run_out);
+    if (!alive) return;
     if (claim) {
         bool got = false;
         uint32_t px = 0, py = 0;
-        if (q.cap) {
+        if (RUNS && !q.cap) {
+            if (run_idx >= 0) {
+                const uint64_t p = order ? order[run_idx] : (uint64_t)run_idx;
+                s.pair = (int64_t)p;
+                px = pairs[2 * p];
+                py = pairs[2 * p + 1];
+                got = true;
+            } else if (run_out) {
+                s.phase = PH_RETIRED;
+                s.pair = -1;
+                atomicSub(n_live, 1);
+            } else {
+                // the new run was the short last one: claim at the next step
+                s.phase = PH_IDLE;
+                s.pair = -1;
+            }
+        } else if (q.cap) {
             const unsigned long long idx =
                 s.phase == PH_WAIT ? (unsigned long long)s.pair : atomicAdd(next_pair, 1ull);
             const unsigned long long tail = __atomic_load_n(q.tail, __ATOMIC_RELAXED);
@@ -1376,14 +1515,35 @@ __global__ __launch_bounds__(64) void step_kernel(SlotState* __restrict__ st, ui
     if (slot >= nslots) return;
     SlotState s = st[slot];
     if (s.phase == PH_RETIRED) return;
-    step_slot(st, sx, sy, pt, active, nslots, partial, nseg, pairs, npairs, order, next_pair, out, tune, n_live, q, slot,
-              s);
+    step_slot<false>(st, sx, sy, pt, active, nslots, partial, nseg, pairs, npairs, order, next_pair, out, tune, n_live,
+                     q, slot, s, RunClaim{}, true);
     if (live_out) {
         // slots still holding or claiming pairs after this step, one atomic
         // per wave (retired lanes have returned above)
         const unsigned long long act = __ballot(1);
         const unsigned long long m = __ballot(s.phase != PH_RETIRED);
         if (__lane_id() == __ffsll(act) - 1 && m) atomicAdd(live_out, (int)__popcll(m));
+    }
+}
+
+// The step of a batch fit that claims in runs: a workgroup is a sweep
+// workgroup's 256 slots, and no thread leaves before the workgroup's claim.
+__global__ __launch_bounds__(SWEEP_TPB) void step_runs_kernel(
+    SlotState* __restrict__ st, uint32_t* __restrict__ sx, uint32_t* __restrict__ sy, double* __restrict__ pt,
+    int* __restrict__ active, uint32_t nslots, const double* __restrict__ partial, uint32_t nseg,
+    const uint32_t* __restrict__ pairs, uint64_t npairs, const uint64_t* __restrict__ order,
+    unsigned long long* __restrict__ next_run, kestrel_output* __restrict__ out, Tune tune, int* __restrict__ n_live,
+    int* __restrict__ live_out, RunClaim rc) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    SlotState s{};
+    const bool in = slot < nslots;
+    if (in) s = st[slot];
+    const bool alive = in && s.phase != PH_RETIRED;
+    step_slot<true>(st, sx, sy, pt, active, nslots, partial, nseg, pairs, npairs, order, next_run, out, tune, n_live,
+                    PairQueue{}, slot, s, rc, alive);
+    if (live_out) {
+        const unsigned long long m = __ballot(alive && s.phase != PH_RETIRED);
+        if (__lane_id() == 0 && m) atomicAdd(live_out, (int)__popcll(m));
     }
 }
 
@@ -1904,7 +2064,7 @@ hipError_t gather(const uint8_t* src, size_t src_row_bytes, const uint32_t* kept
 hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, const uint32_t* seg_bounds,
                  uint32_t nseg, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
                  uint32_t nslots, double eps, double* partial, hipStream_t st, const double* gl, uint32_t v_pad,
-                 int gl_stride) {
+                 int gl_stride, bool skip_masked) {
     dim3 grid((nslots + SWEEP_TPB - 1) / SWEEP_TPB, nseg);
     if (gl && gl_stride == 2)
         hipLaunchKernelGGL(sweep_gl<2>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, af, seg_bounds, sx,
@@ -1912,9 +2072,12 @@ hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, con
     else if (gl)
         hipLaunchKernelGGL(sweep_gl<3>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, af, seg_bounds, sx,
                            sy, pt, active, nslots, eps, partial);
+    else if (skip_masked)
+        hipLaunchKernelGGL(sweep_fp64<true>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, af, seg_bounds, sx, sy,
+                           pt, active, nslots, eps, partial);
     else
-        hipLaunchKernelGGL(sweep_fp64, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, af, seg_bounds, sx, sy, pt,
-                           active, nslots, eps, partial);
+        hipLaunchKernelGGL(sweep_fp64<false>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, af, seg_bounds, sx, sy,
+                           pt, active, nslots, eps, partial);
     return hipGetLastError();
 }
 
@@ -1978,6 +2141,15 @@ hipError_t gather_multi(const double* src2, const double* src3, const double* sr
     return hipGetLastError();
 }
 
+hipError_t count_missing(const uint32_t* geno, uint32_t n_samples, uint32_t row_words, uint32_t n_sites,
+                         unsigned long long* count, hipStream_t st) {
+    const size_t n = (size_t)n_samples * row_words;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(count_missing_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, geno, n_samples,
+                       row_words, n_sites, count);
+    return hipGetLastError();
+}
+
 hipError_t gather_gl(const double* gl_site, uint32_t n_samples, const uint32_t* kept, uint32_t n_kept, double* rows,
                      uint32_t v_pad, uint32_t* mask, uint32_t row_words, int compact, int* no_one, hipStream_t st) {
     const size_t n = (size_t)n_samples * row_words;
@@ -1990,14 +2162,19 @@ hipError_t gather_gl(const double* gl_site, uint32_t n_samples, const uint32_t* 
 hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* active, uint32_t nslots,
                 double* partial, uint32_t nseg, const uint32_t* pairs, uint64_t npairs, const uint64_t* order,
                 unsigned long long* next_pair, kestrel_output* out, const Tune& t, int* n_live, const PairQueue& q,
-                int* live_out, hipStream_t st) {
+                int* live_out, hipStream_t st, const RunClaim& rc) {
     if (nseg > 1) {
         const size_t n = (size_t)NPART * nslots;
         hipLaunchKernelGGL(fold_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, partial, nseg, nslots,
                            active);
     }
-    hipLaunchKernelGGL(step_kernel, dim3((nslots + 63) / 64), dim3(64), 0, st, stt, sx, sy, pt, active, nslots,
-                       partial, nseg, pairs, npairs, order, next_pair, out, t, n_live, q, live_out);
+    if (rc.runs && !q.cap)
+        hipLaunchKernelGGL(step_runs_kernel, dim3((nslots + SWEEP_TPB - 1) / SWEEP_TPB), dim3(SWEEP_TPB), 0, st, stt,
+                           sx, sy, pt, active, nslots, partial, nseg, pairs, npairs, order, next_pair, out, t, n_live,
+                           live_out, rc);
+    else
+        hipLaunchKernelGGL(step_kernel, dim3((nslots + 63) / 64), dim3(64), 0, st, stt, sx, sy, pt, active, nslots,
+                           partial, nseg, pairs, npairs, order, next_pair, out, t, n_live, q, live_out);
     return hipGetLastError();
 }
 

@@ -46,6 +46,18 @@ struct PairQueue {
     int* busy;           // slots holding a pair
 };
 
+// Claims in runs (batch fits): the claim order is cut into runs of
+// consecutive claims, each at least 256 long (the last may be shorter) and cut
+// where the first sample changes, so that the 256 lanes of a sweep workgroup,
+// which claim together from their workgroup's run, mostly fit pairs of one
+// sample x.  runs[2r], runs[2r + 1]: claim indices [begin, end) of run r;
+// wrun[2w], wrun[2w + 1]: the unclaimed part of workgroup w's run.
+struct RunClaim {
+    const uint64_t* runs = nullptr;
+    uint64_t nruns = 0;
+    unsigned long long* wrun = nullptr;
+};
+
 namespace klaunch {
 hipError_t af(const uint8_t* packed, size_t row_bytes, uint32_t n_samples, uint32_t n_sites, const kdev::Tune& t,
               double maf, double* af_out, int* status, hipStream_t st);
@@ -76,11 +88,14 @@ hipError_t gather(const uint8_t* src, size_t src_row_bytes, const uint32_t* kept
 hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, const uint32_t* seg_bounds,
                  uint32_t nseg, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
                  uint32_t nslots, double eps, double* partial, hipStream_t st,
-                 const double* gl = nullptr, uint32_t v_pad = 0, int gl_stride = 3);
+                 const double* gl = nullptr, uint32_t v_pad = 0, int gl_stride = 3, bool skip_masked = false);
 hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* active, uint32_t nslots,
                 double* partial, uint32_t nseg, const uint32_t* pairs, uint64_t npairs, const uint64_t* order,
                 unsigned long long* next_pair, kestrel_output* out, const kdev::Tune& t, int* n_live,
-                const PairQueue& q, int* live_out, hipStream_t st);
+                const PairQueue& q, int* live_out, hipStream_t st, const RunClaim& rc = RunClaim{});
+// code-3 calls of the first n_sites sites of every row (missing-call count)
+hipError_t count_missing(const uint32_t* geno, uint32_t n_samples, uint32_t row_words, uint32_t n_sites,
+                         unsigned long long* count, hipStream_t st);
 hipError_t selftest(uint64_t n, uint64_t seed, unsigned long long* bad, hipStream_t st);
 hipError_t calib_fp64(uint32_t blocks, uint32_t iters, double* sink, hipStream_t st);
 hipError_t calib_fp32(uint32_t blocks, uint32_t iters, float* sink, hipStream_t st);

@@ -333,6 +333,40 @@ def test_c3_full_size_properties(O):
         assert abs(k - expect[kind]) < 0.035, (kind, k)
 
 
+def test_claims_in_runs_and_skipped_masked_sites(O):
+    """Cohorts with many missing calls fit with claims in runs (a 256-slot
+    workgroup's lanes claim consecutive pairs, mostly of one first sample) and
+    sweeps that pass over the sites where that shared sample is missing.  The
+    records are bit-identical to per-lane claims with full sweeps, for any
+    number of pairs in flight and for a pair list whose runs mix samples, and
+    to the oracle."""
+    say("case: claims in runs, masked sites skipped (missing U(0, 0.5) per sample)")
+    c = synth.generate(260, 5000, seed=77, missing_per_sample_max=0.5, n_family_pairs=8)
+    rng = np.random.default_rng(5)
+    pairs = rng.integers(0, 260, (2500, 2))
+    with Cohort(c.codes) as co:
+        af, kept = co.allele_freqs()
+        auto = co.fit_pairs()  # missing calls >= 2 %: on
+        co.set_option("skip_masked", 0)
+        off = co.fit_pairs()
+        off_list = co.fit_pairs(pairs)
+        co.set_option("skip_masked", 1)
+        on_list = co.fit_pairs(pairs)
+        co.set_option("slots", 256)
+        few = co.fit_pairs()
+        co.set_option("streams", 1)
+        one = co.fit_pairs()
+    assert_bitexact(auto, off)
+    assert_bitexact(few, off)
+    assert_bitexact(one, off)
+    assert_bitexact(on_list, off_list)
+    kc = c.codes[:, kept]
+    idx = rng.integers(0, len(auto), 300)
+    sample = np.stack([auto["x"][idx], auto["y"][idx]], 1)
+    exp = O.fit_pairs(kc, af[kept], pairs=sample, order=O.ORDER_SEG, seg=seg_for(int(kept.sum()), kc.shape[1]))
+    assert_bitexact(auto[idx], exp)
+
+
 def test_given_allele_frequencies(O):
     say("case: given allele frequencies")
     rng = np.random.default_rng(4)
