@@ -583,9 +583,14 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     }
     const auto t0 = std::chrono::steady_clock::now();
 
-    // pairs in flight per group: fill the device unless set, never more than
-    // the pairs available
-    uint64_t nslots = c->opt_slots > 0 ? (uint64_t)c->opt_slots : default_slots(c);
+    // claims in runs and sweeps that pass over the sites where a wavefront's
+    // shared sample is missing (2-bit cohorts; option "skip_masked")
+    const bool use_runs = !c->soft && c->parts.empty() &&
+                          (c->opt_skip_masked == 1 || (c->opt_skip_masked < 0 && c->missing_frac >= 0.02));
+    // pairs in flight per group: fill the device unless set (twice with runs:
+    // workgroups then end at different times, and a launch of one round would
+    // last as long as its slowest), never more than the pairs available
+    uint64_t nslots = c->opt_slots > 0 ? (uint64_t)c->opt_slots : default_slots(c) * (use_runs ? 2 : 1);
     nslots = std::min<uint64_t>(nslots, ((uint64_t)n_pairs + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB);
     nslots = (nslots + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB;
     const uint32_t ns = (uint32_t)nslots;
@@ -623,10 +628,6 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
         HIPCHK(hipMemcpyAsync(dorder.p, order.data(), n_pairs * sizeof(uint64_t), hipMemcpyHostToDevice, st));
         dorder_p = dorder.p;
     }
-    // claims in runs and sweeps that pass over the sites where a wavefront's
-    // shared sample is missing (2-bit cohorts; option "skip_masked")
-    const bool use_runs = !c->soft && c->parts.empty() &&
-                          (c->opt_skip_masked == 1 || (c->opt_skip_masked < 0 && c->missing_frac >= 0.02));
     DevBuf<uint64_t> druns;
     RunClaim rclaim;
     if (use_runs) {
