@@ -28,6 +28,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+FP32_VALU_PEAK_TFLOPS = 157.3  # MI355X vendor FP32 vector (packed)
 FP64_VALU_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
 LDS_PEAK_TBS = 157.3  # gfx950 LDS reads: 256 B/clk/CU x 256 CUs x 2.4 GHz
 FLOP_PER_SITE_EVAL = 156  # 136 (gradient + Hessian) + 20 (objective, log counted as 1)
@@ -339,6 +340,38 @@ def cls_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def mixed_errors(codes, co_mixed, config, n_pairs=20000, seed=0xE44):
+    """Errors of the mixed form against the FP64 form on a seeded sample of
+    off-diagonal pairs, by the bar of the FP32-mixed form: max |dDelta_k|,
+    |d kinship|, relative objective and log10 condition on the pairs whose
+    FP64 log10 condition is below 12 (and max |dDelta_k| below 6)."""
+    from kestrel_amd import Cohort, kinship
+    rng = np.random.default_rng(seed)
+    n = codes.shape[0]
+    x = rng.integers(0, n, n_pairs)
+    y = rng.integers(0, n, n_pairs)
+    keep = x != y
+    pairs = np.stack([np.minimum(x, y), np.maximum(x, y)], 1)[keep]
+    a = co_mixed.fit_pairs(pairs)
+    with Cohort(codes) as co64:
+        b = co64.fit_pairs(pairs)
+    lc = np.log10(b["cond"])
+    lca = np.log10(a["cond"])
+    capped = ~np.isfinite(lca)  # the mixed form reached the iteration cap (condition NaN)
+    well = np.isfinite(lc) & (lc < 12) & ~capped
+    good = well & (lc < 6)
+    d = np.abs(a["jacquard"] - b["jacquard"]).max(1)
+    return {"pairs": int(len(pairs)), "well_conditioned": int(well.sum()),
+            "mixed_at_iteration_cap": int(capped.sum()), "fp64_at_iteration_cap": int((~np.isfinite(lc)).sum()),
+            "max_abs_delta": float(d[well].max()), "max_abs_delta_cond_below_1e6": float(d[good].max()),
+            "pairs_over_1e-6": int((d[well] > 1e-6).sum()),
+            "max_abs_kinship": float(np.abs(kinship(a["jacquard"]) - kinship(b["jacquard"]))[well].max()),
+            "max_rel_objective": float((np.abs(a["obj"] - b["obj"]) / np.abs(b["obj"]))[well].max()),
+            "max_abs_log10_cond": float(np.abs(lca[well] - lc[well]).max()),
+            "iterations_mean": [round(float(a["iters"].mean()), 3), round(float(b["iters"].mean()), 3)],
+            "reference": "the FP64 form of this library (bit-exact with the oracle in its summation order)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -352,6 +385,8 @@ def main():
     ap.add_argument("--batch", type=int, default=None, help="iterations queued per host synchronisation")
     ap.add_argument("--graph", action="store_true", help="relaunch each batch from a captured HIP graph")
     ap.add_argument("--pair-tile", type=int, default=None, help="claim order in tiles of this many samples")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+                    help="mixed: per-site terms in FP32 (KESTREL_MIXED), with its errors against the FP64 form")
     ap.add_argument("--skip-masked", type=int, choices=(-1, 0, 1), default=None,
                     help="claims in runs + sweeps passing over masked sites (default: the library's automatic choice)")
     ap.add_argument("--autotune", type=int, default=0, metavar="PAIRS",
@@ -413,7 +448,10 @@ def main():
         f"{f' with {args.multi:.0%} tri/tetra-allelic sites' if buckets else ''} generated in "
         f"{time.perf_counter() - t:.1f}s")
     t = time.perf_counter()
-    co = Cohort.gl(gl) if args.soft else Cohort.gl_multi(*buckets) if buckets else Cohort(c.codes)
+    if args.precision == "mixed" and (args.soft or buckets):
+        sys.exit("--precision mixed applies to 2-bit cohorts (hard calls)")
+    co = (Cohort.gl(gl) if args.soft else Cohort.gl_multi(*buckets) if buckets
+          else Cohort(c.codes, precision=args.precision))
     if args.segment_sites:
         co.set_option("segment_sites", args.segment_sites)
     if args.slots:
@@ -471,21 +509,31 @@ def main():
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_VALU_PEAK_TFLOPS, 4),
                 "traffic": (SWEEP_TRAFFIC_BYTES_C2 if args.config == "c2" and not args.segment_sites
                             and not args.soft and not buckets else None),
-                "kernel": "sweep_gl + sweep_glm" if buckets else "sweep_gl" if args.soft else "sweep_fp64",
+                "kernel": ("sweep_gl + sweep_glm" if buckets else "sweep_gl" if args.soft
+                           else "sweep_mixed" if args.precision == "mixed" else "sweep_fp64"),
                 "avg_launch_ms": round(sweep_ms / sweeps, 4),
                 "concurrent_launches": groups, "sweep_busy_ms_per_step": round(busy_ms / args.steps, 2),
                 "algorithmic_flop_per_launch": flops / sweeps}
+    mixed = args.precision == "mixed"
+    if mixed:  # the mixed form is priced against FP32 (vendor figure: packed FP32)
+        roofline["peak"] = FP32_VALU_PEAK_TFLOPS
+        roofline["frac"] = round(achieved / FP32_VALU_PEAK_TFLOPS, 4)
     measured_peak = None
     try:
         import ctypes
         from kestrel_amd import _lib
         pk = ctypes.c_double()
-        _lib.check(_lib.load().kestrel_measure_fp64_peak(ctypes.byref(pk)))
+        _lib.check((_lib.load().kestrel_measure_fp32_peak if mixed else _lib.load().kestrel_measure_fp64_peak)(
+            ctypes.byref(pk)))
         measured_peak = round(pk.value, 2)
         roofline["measured_peak"] = measured_peak
         roofline["frac_of_measured"] = round(achieved / pk.value, 4)
     except Exception as e:  # the measurement is informative only
         log(f"fp64 peak measurement failed: {e}")
+    precision_error = None
+    if mixed:
+        precision_error = mixed_errors(c.codes, co, args.config)
+        log(f"mixed vs FP64 form: {precision_error}")
     cpu = None
     if buckets:
         log("cpu baseline: not measured for tri/tetra-allelic cohorts (the oracle's multi-allelic path is "
@@ -497,19 +545,22 @@ def main():
     line = {
         "metric": "pair-fits/sec", "value": round(value, 1), "unit": "pair-fits/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(total / args.steps * 1e3, 2),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if mixed else "f64",
+        "data": "synthetic",
         "config": {"workload": f"{args.config}: {n} samples x {v} sites ({co.n_kept} segregating"
                                + (", 2-allele part" if buckets else "") + f"), "
-                               f"{offdiag} off-diagonal pairs (+{len(out) - offdiag} self pairs), FP64"
+                               f"{offdiag} off-diagonal pairs (+{len(out) - offdiag} self pairs), "
+                               + ("FP32-mixed" if mixed else "FP64")
                                + (f", soft likelihoods (mean depth {args.depth})" if args.soft else "")
                                + (f", {args.multi:.0%} of sites tri/tetra-allelic" if buckets else ""),
-                   "samples": n, "sites": v, "pairs": offdiag, "precision": "fp64",
+                   "samples": n, "sites": v, "pairs": offdiag, "precision": args.precision,
                    "input": ("soft genotype likelihoods" if args.soft else
                              "one-hot likelihoods, 2-4 alleles" if buckets else "hard calls"),
                    "launch_shape": ({k: tuned[k] for k in ("slots", "streams", "batch")} if tuned else "default"),
                    "hip_graph": bool(args.graph),
                    "skip_masked": "auto" if args.skip_masked is None else args.skip_masked},
         "roofline": roofline, "cpu_baseline": cpu,
+        **({"precision_error": precision_error} if mixed else {}),
         "iterations_mean": round(float(out["iters"].mean()), 3),
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
     }

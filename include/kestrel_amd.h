@@ -44,8 +44,13 @@ extern "C" {
 #define KESTREL_PAIR_OK 0u
 #define KESTREL_PAIR_NO_SITES 1u /* no usable site: the reference panics (lanevector.rs:133-138) */
 
-/* precision forms */
+/* precision forms: KESTREL_FP64 is the reference's arithmetic (bit-exact in
+ * the device's summation order); KESTREL_MIXED computes the per-site terms in
+ * FP32 (32-site FP32 sums added to FP64 totals; log, objective, QP and
+ * condition in FP64), 2-bit cohorts only, its errors against FP64 measured,
+ * not bounded */
 #define KESTREL_FP64 0
+#define KESTREL_MIXED 1
 
 /* sqp.rs:16-29 Tuneables */
 typedef struct {

@@ -22,6 +22,7 @@ PAIR_OK = 0
 PAIR_NO_SITES = 1
 
 FP64 = 0
+MIXED = 1
 
 
 class KestrelError(RuntimeError):

@@ -28,8 +28,12 @@ class Cohort:
 
     def __init__(self, codes: np.ndarray | None = None, *, packed: np.ndarray | None = None,
                  n_sites: int | None = None, af: np.ndarray | None = None, maf: float | None = None,
-                 af_iterations: int = 100, segment_sites: int | None = None):
+                 af_iterations: int = 100, segment_sites: int | None = None, precision: str = "fp64"):
+        """precision: "fp64" (the reference's arithmetic) or "mixed" (per-site
+        terms in FP32, KESTREL_MIXED)."""
         L = _lib.load()
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
         if packed is None:
             codes = np.ascontiguousarray(codes, np.uint8)
             n_sites = codes.shape[1]
@@ -42,7 +46,8 @@ class Cohort:
             afp = np.ascontiguousarray(af, np.float64).reshape(self.n_sites, 2)
         h = ctypes.c_void_p()
         check(L.kestrel_cohort_create(ctypes.byref(h), ptr(packed), self.n_samples, self.n_sites, packed.shape[1],
-                                      ptr(afp), -1.0 if maf is None else float(maf), af_iterations, _lib.FP64))
+                                      ptr(afp), -1.0 if maf is None else float(maf), af_iterations,
+                                      _lib.FP64 if precision == "fp64" else _lib.MIXED))
         self._h = h
         if segment_sites is not None:
             self.set_option("segment_sites", segment_sites)

@@ -71,7 +71,7 @@ int repack_kept(kestrel_cohort* c, const uint8_t* dpacked, size_t row_stride_byt
 int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
                   uint32_t ns, double eps, double* partial, hipStream_t st, bool skip_masked = false) {
     HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, sx, sy, pt, active, ns, eps, partial, st,
-                          c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride, skip_masked));
+                          c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride, skip_masked, c->mixed));
     size_t base = c->nseg;
     for (const auto& p : c->parts) {
         HIPCHK(klaunch::sweep_multi(p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->table.p, p->seg.p,
@@ -148,7 +148,8 @@ int kestrel_cohort_create(kestrel_cohort** out, const uint8_t* packed, uint32_t 
     *out = nullptr;
     if (n_samples == 0 || n_sites == 0) return fail(KESTREL_EINVAL, "need at least one sample and one site");
     if (row_stride_bytes < (n_sites + 3) / 4) return fail(KESTREL_EINVAL, "row stride too small");
-    if (precision != KESTREL_FP64) return fail(KESTREL_EINVAL, "unknown precision %d", precision);
+    if (precision != KESTREL_FP64 && precision != KESTREL_MIXED)
+        return fail(KESTREL_EINVAL, "unknown precision %d", precision);
     if (maf < 0) maf = 1.0 / (2.0 * (double)n_samples);  // wrapper.rs:101
     if (!(maf >= 0.0 && maf <= 1.0)) return fail(KESTREL_EINVAL, "Minimum allele frequency must be within 0 to 1");
     if (!af && af_iterations == 0) return fail(KESTREL_EINVAL, "iterations must be non-zero");
@@ -159,6 +160,7 @@ int kestrel_cohort_create(kestrel_cohort** out, const uint8_t* packed, uint32_t 
     if (rc) return rc;
     c->n_samples = n_samples;
     c->n_sites = n_sites;
+    c->mixed = precision == KESTREL_MIXED;
 
     DevBuf<uint8_t> dpacked;
     HIPCHK(dpacked.alloc((size_t)n_samples * row_stride_bytes));
@@ -585,7 +587,7 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
 
     // claims in runs and sweeps that pass over the sites where a wavefront's
     // shared sample is missing (2-bit cohorts; option "skip_masked")
-    const bool use_runs = !c->soft && c->parts.empty() &&
+    const bool use_runs = !c->soft && !c->mixed && c->parts.empty() &&
                           (c->opt_skip_masked == 1 || (c->opt_skip_masked < 0 && c->missing_frac >= 0.02));
     // pairs in flight per group: fill the device unless set (twice with runs:
     // workgroups then end at different times, and a launch of one round would

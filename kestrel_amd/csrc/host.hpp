@@ -122,6 +122,7 @@ struct kestrel_cohort {
     int64_t opt_skip_masked = -1;  // claims in runs + sweeps passing over sites where a wavefront's shared
                                    // sample is missing: 1 on, 0 off, -1 when missing_frac >= 0.02
     double missing_frac = 0.0;     // missing calls among the kept sites (2-bit cohorts)
+    bool mixed = false;            // precision KESTREL_MIXED: sweeps by sweep_mixed
     int64_t opt_pair_tile = -1;    // claim order in tiles of this many samples (0 list order, -1 automatic)
     hipStream_t stream2 = nullptr; // second stream of a fit
     double last_sweep_ms = 0, last_wall_ms = 0, last_busy_ms = 0;

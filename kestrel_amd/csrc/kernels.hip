@@ -361,12 +361,15 @@ constexpr int LUT_BUF = LUT_ROWS * LUT_RS;  // doubles per buffer
 // Rows of the c_s table for site (p, q): the non-zero block of mixture.rs
 // mixture_row2 for one-hot likelihoods of genotype pair (gx, gy), written in
 // the same expression order, so each row equals what mixture_row2 returns.
-__device__ __forceinline__ void lut_rows(double p, double q, int part, double* buf, int site, double eps) {
+// The block of genotype pair `part` (row-major over (gx, gy) in 0..2) into
+// r[9]; returns its table row.
+__device__ __forceinline__ int lut_block(double p, double q, int part, double* r) {
     const double p2 = p * p, p3 = p2 * p, q2 = q * q, q3 = q2 * q;
     const double pq = p * q, p2q = p2 * q, pq2 = p * q2, p2q2 = p2 * q2;
     const double p3q = p3 * q, pq3 = p * q3;
     const double twopq = pq + pq, twop2q = p2q + p2q, twop3q = p3q + p3q, twopq2 = pq2 + pq2, twopq3 = pq3 + pq3;
-    double r[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 9; k++) r[k] = 0.0;
     int row;
     switch (part) {
     case 0:  // (0,0)
@@ -406,10 +409,32 @@ __device__ __forceinline__ void lut_rows(double p, double q, int part, double* b
         r[0] = q; r[1] = q2; r[2] = q2; r[3] = q3; r[4] = q2; r[5] = q3; r[6] = q2; r[7] = q3; r[8] = q2 * q2;
         break;
     }
+    return row;
+}
+
+__device__ __forceinline__ void lut_rows(double p, double q, int part, double* buf, int site, double eps) {
+    double r[9];
+    const int row = lut_block(p, q, part, r);
     double* dst = buf + row * LUT_RS + site * LUT_W;
 #pragma unroll
     for (int k = 0; k < 9; k++) dst[k] = r[k];
     dst[9] = eps;  // start of the L = eps + c.D chain
+}
+
+// The mixed-precision form's table: the same rows rounded to FP32, 12 floats
+// an entry (c[9], the start of the L chain, 2 of padding: three 16-byte
+// reads), row stride padded by 16 bytes as above.
+constexpr int LUTF_W = 12;
+constexpr int LUTF_RS = CHUNK * LUTF_W + 4;
+constexpr int LUTF_BUF = LUT_ROWS * LUTF_RS;  // floats per buffer
+
+__device__ __forceinline__ void lut_rows_f32(double p, double q, int part, float* buf, int site, double eps) {
+    double r[9];
+    const int row = lut_block(p, q, part, r);
+    float* dst = buf + row * LUTF_RS + site * LUTF_W;
+#pragma unroll
+    for (int k = 0; k < 9; k++) dst[k] = (float)r[k];
+    dst[9] = (float)eps;
 }
 
 }  // namespace
@@ -553,6 +578,146 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
             // column 9 holds eps, or 1.0 on a missing row
             site_update(cv, cv[9], D, k, P, F, G, H);
         }
+        wx = nwx;
+        wy = nwy;
+        pn = pnn;
+        qn = qnn;
+        __syncthreads();
+    }
+    if (live) {
+        double* out = partial + (size_t)seg * NPART * nslots + slot;
+        out[0] = F;
+#pragma unroll
+        for (int k = 0; k < 9; k++) out[(size_t)(1 + k) * nslots] = G[k];
+#pragma unroll
+        for (int k = 0; k < 45; k++) out[(size_t)(10 + k) * nslots] = H[k];
+        out[(size_t)55 * nslots] = N;
+    }
+}
+
+// The mixed-precision form (precision KESTREL_MIXED): the pass of sweep_fp64
+// with the per-site terms in FP32 -- L = eps + c.D from the FP32 table, 1/L by
+// v_rcp_f32, the gradient terms c/L, the scaled c' = c/L and the 45 Hessian
+// terms -- summed in FP32 over each 32-site chunk and added to FP64 totals at
+// the chunk's end.  The log product, the objective and everything the step
+// does (QP, line search, convergence, condition) stay FP64.  Results are not
+// the FP64 form's bits: the errors against it are measured, not bounded.
+__global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_mixed(const uint32_t* __restrict__ geno, uint32_t row_words,
+                                                             const double* __restrict__ af,
+                                                             const uint32_t* __restrict__ seg_bounds,
+                                                             const uint32_t* __restrict__ sx,
+                                                             const uint32_t* __restrict__ sy,
+                                                             const double* __restrict__ pt,
+                                                             const int* __restrict__ active, uint32_t nslots,
+                                                             double eps, double* __restrict__ partial) {
+    __shared__ __attribute__((aligned(16))) float lut[2 * LUTF_BUF];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t slot = blockIdx.x * SWEEP_TPB + tid;
+    const uint32_t seg = blockIdx.y;
+    const bool live = slot < nslots && active[slot] != 0;
+    if (!__syncthreads_or(live)) return;
+
+    // missing rows (3, 7, 11): c = 0 and 1.0 as the start of the L chain
+    for (uint32_t i = tid; i < 2 * LUTF_BUF; i += SWEEP_TPB) {
+        const uint32_t off = i % LUTF_BUF, row = off / LUTF_RS, w = off % LUTF_RS;
+        lut[i] = (row & 3u) == 3u && w < CHUNK * LUTF_W && w % LUTF_W == 9 ? 1.0f : 0.0f;
+    }
+    __syncthreads();
+
+    float D[9];
+    uint32_t x = 0, y = 0;
+    if (live) {
+        x = sx[slot];
+        y = sy[slot];
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k] = (float)pt[(size_t)k * nslots + slot];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k] = 0.0f;
+    }
+    const uint32_t* rowx = geno + (size_t)x * row_words;
+    const uint32_t* rowy = geno + (size_t)y * row_words;
+
+    double F = 0.0, N = 0.0, P = 1.0, G[9], H[45];
+#pragma unroll
+    for (int k = 0; k < 9; k++) G[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 45; k++) H[k] = 0.0;
+
+    const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
+    const int lut_site = tid >> 3, lut_part = tid & 7;
+    auto build = [&](double p, double q, int b) {
+        float* buf = lut + b * LUTF_BUF;
+        lut_rows_f32(p, q, lut_part, buf, lut_site, eps);
+        if (lut_part == 0) lut_rows_f32(p, q, 8, buf, lut_site, eps);
+    };
+    uint2 wx = make_uint2(0u, 0u), wy = make_uint2(0u, 0u);
+    double pn = 0.5, qn = 0.5;
+    if (s0 < s1) {
+        build(af[2 * (s0 + lut_site)], af[2 * (s0 + lut_site) + 1], 0);
+        wx = *reinterpret_cast<const uint2*>(rowx + s0 / 16);
+        wy = *reinterpret_cast<const uint2*>(rowy + s0 / 16);
+    }
+    if (s0 + CHUNK < s1) {
+        pn = af[2 * (s0 + CHUNK + lut_site)];
+        qn = af[2 * (s0 + CHUNK + lut_site) + 1];
+    }
+    __syncthreads();
+    int b = 0;
+    for (uint32_t cs = s0; cs < s1; cs += CHUNK, b ^= 1) {
+        uint2 nwx = wx, nwy = wy;
+        double pnn = 0.5, qnn = 0.5;
+        if (cs + CHUNK < s1) {
+            nwx = *reinterpret_cast<const uint2*>(rowx + (cs + CHUNK) / 16);
+            nwy = *reinterpret_cast<const uint2*>(rowy + (cs + CHUNK) / 16);
+            build(pn, qn, b ^ 1);
+        }
+        if (cs + 2 * CHUNK < s1) {
+            pnn = af[2 * (cs + 2 * CHUNK + lut_site)];
+            qnn = af[2 * (cs + 2 * CHUNK + lut_site) + 1];
+        }
+        const float* L0 = lut + b * LUTF_BUF;
+        {
+            const uint32_t ux = ~((wx.x & (wx.x >> 1)) | (wy.x & (wy.x >> 1))) & 0x55555555u;
+            const uint32_t uy = ~((wx.y & (wx.y >> 1)) | (wy.y & (wy.y >> 1))) & 0x55555555u;
+            N += (double)(__popc(ux) + __popc(uy));
+        }
+        float g[9], h[45];
+#pragma unroll
+        for (int k = 0; k < 9; k++) g[k] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 45; k++) h[k] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < CHUNK; k++) {
+            const uint32_t ax = (k < 16 ? wx.x : wx.y) >> (2 * (k & 15));
+            const uint32_t ay = (k < 16 ? wy.x : wy.y) >> (2 * (k & 15));
+            const uint32_t row = min(((ax & 3u) << 2) | (ay & 3u), 11u);
+            const float4* c4 = reinterpret_cast<const float4*>(L0 + row * LUTF_RS + k * LUTF_W);
+            const float4 u = c4[0], v = c4[1], w = c4[2];
+            float cv[9] = {u.x, u.y, u.z, u.w, v.x, v.y, v.z, v.w, w.x};
+            float L = w.y;
+#pragma unroll
+            for (int j = 0; j < 9; j++) L = fmaf(D[j], cv[j], L);
+            P = P * (double)L;
+            if ((k & (LOG_GROUP - 1)) == LOG_GROUP - 1) {
+                F = F + sleef_log_pos(P);
+                P = 1.0;
+            }
+            const float r = __builtin_amdgcn_rcpf(L);
+#pragma unroll
+            for (int j = 0; j < 9; j++) g[j] = fmaf(cv[j], r, g[j]);
+#pragma unroll
+            for (int j = 0; j < 9; j++) cv[j] = cv[j] * r;
+            int id = 0;
+#pragma unroll
+            for (int a2 = 0; a2 < 9; a2++)
+#pragma unroll
+                for (int bb = a2; bb < 9; bb++, id++) h[id] = fmaf(cv[a2], cv[bb], h[id]);
+        }
+#pragma unroll
+        for (int k = 0; k < 9; k++) G[k] = G[k] + (double)g[k];
+#pragma unroll
+        for (int k = 0; k < 45; k++) H[k] = H[k] + (double)h[k];
         wx = nwx;
         wy = nwy;
         pn = pnn;
@@ -2064,7 +2229,7 @@ hipError_t gather(const uint8_t* src, size_t src_row_bytes, const uint32_t* kept
 hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, const uint32_t* seg_bounds,
                  uint32_t nseg, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
                  uint32_t nslots, double eps, double* partial, hipStream_t st, const double* gl, uint32_t v_pad,
-                 int gl_stride, bool skip_masked) {
+                 int gl_stride, bool skip_masked, bool mixed) {
     dim3 grid((nslots + SWEEP_TPB - 1) / SWEEP_TPB, nseg);
     if (gl && gl_stride == 2)
         hipLaunchKernelGGL(sweep_gl<2>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, af, seg_bounds, sx,
@@ -2072,6 +2237,9 @@ hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, con
     else if (gl)
         hipLaunchKernelGGL(sweep_gl<3>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, af, seg_bounds, sx,
                            sy, pt, active, nslots, eps, partial);
+    else if (mixed)
+        hipLaunchKernelGGL(sweep_mixed, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, af, seg_bounds, sx, sy, pt,
+                           active, nslots, eps, partial);
     else if (skip_masked)
         hipLaunchKernelGGL(sweep_fp64<true>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, af, seg_bounds, sx, sy,
                            pt, active, nslots, eps, partial);

@@ -294,7 +294,8 @@ int engine_round(ServeState* S, bool* idle) {
             ServeState::Group& G = S->grp[g];
             if (sweep)
                 HIPCHK(klaunch::sweep(S->geno.p, S->row_words, S->c->af.p, S->seg.p, S->nseg, G.sx.p, G.sy.p, G.pt.p,
-                                      G.active.p, S->ns, S->tune.epsilon, G.partial.p, G.stream));
+                                      G.active.p, S->ns, S->tune.epsilon, G.partial.p, G.stream, nullptr, 0, 3,
+                                      false, S->c->mixed));
             HIPCHK(klaunch::step(G.slots.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, S->ns, G.partial.p, S->nseg, nullptr,
                                  0, nullptr, ctrl_next(S), S->outpool.p, S->dt, nullptr, q, nullptr, G.stream));
         }

@@ -367,6 +367,32 @@ def test_claims_in_runs_and_skipped_masked_sites(O):
     assert_bitexact(auto[idx], exp)
 
 
+def test_mixed_precision_form():
+    """KESTREL_MIXED (per-site terms in FP32, sums and solver in FP64): the
+    frequencies, kept sites and n_xy are the FP64 form's; Delta is within the
+    form's bar (1e-6) of the FP64 form on the pairs both forms bring to
+    convergence with log10 condition below 12.  Its iteration counts are not
+    the FP64 form's (the 1e-8 gradient test meets FP32 rounding)."""
+    say("case: mixed-precision form vs FP64 form, 160 x 20000")
+    c = synth.generate(160, 20000, seed=41, n_family_pairs=8)
+    with Cohort(c.codes, precision="mixed") as a, Cohort(c.codes) as b:
+        afa, ka = a.allele_freqs()
+        afb, kb = b.allele_freqs()
+        assert np.array_equal(afa, afb) and np.array_equal(ka, kb)
+        oa, ob = a.fit_pairs(), b.fit_pairs()
+    assert np.array_equal(oa["x"], ob["x"]) and np.array_equal(oa["n_sites"], ob["n_sites"])
+    lb, la = np.log10(ob["cond"]), np.log10(oa["cond"])
+    both = np.isfinite(lb) & np.isfinite(la) & (lb < 12)
+    assert both.sum() > 0.9 * len(ob)
+    d = np.abs(oa["jacquard"] - ob["jacquard"]).max(1)[both]
+    say(f"mixed vs fp64: max |dDelta| {d.max():.3g} over {both.sum()} pairs; iterations "
+        f"{oa['iters'].mean():.2f} vs {ob['iters'].mean():.2f}")
+    assert d.max() <= 1e-6
+    assert (np.abs(oa["obj"] - ob["obj"]) / np.abs(ob["obj"]))[both].max() <= 1e-6
+    with pytest.raises(ValueError):
+        Cohort(c.codes, precision="fp16")
+
+
 def test_given_allele_frequencies(O):
     say("case: given allele frequencies")
     rng = np.random.default_rng(4)
