@@ -596,12 +596,16 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
 }
 
 // The mixed-precision form (precision KESTREL_MIXED): the pass of sweep_fp64
-// with the per-site terms in FP32 -- L = eps + c.D from the FP32 table, 1/L by
-// v_rcp_f32, the gradient terms c/L, the scaled c' = c/L and the 45 Hessian
-// terms -- summed in FP32 over each 32-site chunk and added to FP64 totals at
-// the chunk's end.  The log product, the objective and everything the step
-// does (QP, line search, convergence, condition) stay FP64.  Results are not
-// the FP64 form's bits: the errors against it are measured, not bounded.
+// with the c_s table in FP32 and the Hessian in FP32 -- 1/L by v_rcp_f32, the
+// scaled c' = c/L and the 45 Hessian terms, summed in FP32 over each 32-site
+// chunk and added to FP64 totals at the chunk's end.  L = eps + c.D and the
+// gradient terms c * (1/L) are FP64 (from the FP32 c and 1/L, whose products
+// are exact in FP64): the line search compares objectives, and the
+// convergence test |g_i - lambda| <= 1e-8 reads the gradient, at resolutions
+// FP32 sums do not reach (with L or the gradient in FP32 the fits took 1.35x
+// the iterations and 3.4x the evaluations).  The log product, the objective
+// and everything the step does stay FP64.  Results are not the FP64 form's
+// bits: the errors against it are measured, not bounded.
 __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_mixed(const uint32_t* __restrict__ geno, uint32_t row_words,
                                                              const double* __restrict__ af,
                                                              const uint32_t* __restrict__ seg_bounds,
@@ -624,16 +628,16 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_mixed(const uint32_t* __re
     }
     __syncthreads();
 
-    float D[9];
+    double D[9];
     uint32_t x = 0, y = 0;
     if (live) {
         x = sx[slot];
         y = sy[slot];
 #pragma unroll
-        for (int k = 0; k < 9; k++) D[k] = (float)pt[(size_t)k * nslots + slot];
+        for (int k = 0; k < 9; k++) D[k] = pt[(size_t)k * nslots + slot];
     } else {
 #pragma unroll
-        for (int k = 0; k < 9; k++) D[k] = 0.0f;
+        for (int k = 0; k < 9; k++) D[k] = 0.0;
     }
     const uint32_t* rowx = geno + (size_t)x * row_words;
     const uint32_t* rowy = geno + (size_t)y * row_words;
@@ -682,9 +686,7 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_mixed(const uint32_t* __re
             const uint32_t uy = ~((wx.y & (wx.y >> 1)) | (wy.y & (wy.y >> 1))) & 0x55555555u;
             N += (double)(__popc(ux) + __popc(uy));
         }
-        float g[9], h[45];
-#pragma unroll
-        for (int k = 0; k < 9; k++) g[k] = 0.0f;
+        float h[45];
 #pragma unroll
         for (int k = 0; k < 45; k++) h[k] = 0.0f;
 #pragma unroll
@@ -695,17 +697,21 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_mixed(const uint32_t* __re
             const float4* c4 = reinterpret_cast<const float4*>(L0 + row * LUTF_RS + k * LUTF_W);
             const float4 u = c4[0], v = c4[1], w = c4[2];
             float cv[9] = {u.x, u.y, u.z, u.w, v.x, v.y, v.z, v.w, w.x};
-            float L = w.y;
+            double cd[9];
 #pragma unroll
-            for (int j = 0; j < 9; j++) L = fmaf(D[j], cv[j], L);
-            P = P * (double)L;
+            for (int j = 0; j < 9; j++) cd[j] = (double)cv[j];
+            double L = w.y == 1.0f ? 1.0 : eps;  // 1.0 on a missing row
+#pragma unroll
+            for (int j = 0; j < 9; j++) L = fma(D[j], cd[j], L);
+            P = P * L;
             if ((k & (LOG_GROUP - 1)) == LOG_GROUP - 1) {
                 F = F + sleef_log_pos(P);
                 P = 1.0;
             }
-            const float r = __builtin_amdgcn_rcpf(L);
+            const float r = __builtin_amdgcn_rcpf((float)L);
+            const double rd = (double)r;
 #pragma unroll
-            for (int j = 0; j < 9; j++) g[j] = fmaf(cv[j], r, g[j]);
+            for (int j = 0; j < 9; j++) G[j] = fma(cd[j], rd, G[j]);
 #pragma unroll
             for (int j = 0; j < 9; j++) cv[j] = cv[j] * r;
             int id = 0;
@@ -714,8 +720,6 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_mixed(const uint32_t* __re
 #pragma unroll
                 for (int bb = a2; bb < 9; bb++, id++) h[id] = fmaf(cv[a2], cv[bb], h[id]);
         }
-#pragma unroll
-        for (int k = 0; k < 9; k++) G[k] = G[k] + (double)g[k];
 #pragma unroll
         for (int k = 0; k < 45; k++) H[k] = H[k] + (double)h[k];
         wx = nwx;

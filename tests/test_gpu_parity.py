@@ -368,11 +368,11 @@ def test_claims_in_runs_and_skipped_masked_sites(O):
 
 
 def test_mixed_precision_form():
-    """KESTREL_MIXED (per-site terms in FP32, sums and solver in FP64): the
-    frequencies, kept sites and n_xy are the FP64 form's; Delta is within the
-    form's bar (1e-6) of the FP64 form on the pairs both forms bring to
-    convergence with log10 condition below 12.  Its iteration counts are not
-    the FP64 form's (the 1e-8 gradient test meets FP32 rounding)."""
+    """KESTREL_MIXED (FP32 table and Hessian terms; L, gradient, objective and
+    solver FP64): the frequencies, kept sites and n_xy are the FP64 form's;
+    Delta is within the form's bar (1e-6) of the FP64 form on the pairs both
+    forms bring to convergence with log10 condition below 12, and so is the
+    relative objective."""
     say("case: mixed-precision form vs FP64 form, 160 x 20000")
     c = synth.generate(160, 20000, seed=41, n_family_pairs=8)
     with Cohort(c.codes, precision="mixed") as a, Cohort(c.codes) as b:
