@@ -118,7 +118,7 @@ def serving(args):
     del c
     log(f"cohort {args.config}: {n} x {v} (+{n_query_pool} query samples) generated in {time.perf_counter() - t:.1f}s")
     t = time.perf_counter()
-    co = Cohort(packed=cohort_packed, n_sites=v)
+    co = Cohort(packed=cohort_packed, n_sites=v, precision=args.precision)
     if args.slots:
         co.set_option("slots", args.slots)
     co.set_option("serve_policy", {"rr": 0, "srpt": 1}[args.policy])
@@ -188,10 +188,11 @@ def serving(args):
     line = {
         "metric": "pair-fits/sec (serving)", "value": round(capacity, 1), "unit": "pair-fits/s", "n_gpus": 1,
         "steps": 2 * k, "warmup": 1, "ms_per_step": round(dt / k * 1e3, 2), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if args.precision == "mixed" else "f64",
+        "data": "synthetic",
         "config": {"workload": f"{args.config} serving: cohort {n} samples x {v} sites ({n_kept} segregating) "
                                f"device-resident; requests of 1-256 queries vs all or 500-5000 samples",
-                   "samples": n, "sites": v, "requests": k, "precision": "fp64", "policy": args.policy},
+                   "samples": n, "sites": v, "requests": k, "precision": args.precision, "policy": args.policy},
         "saturated": {"requests": k, "pairs": int(pairs), "seconds": round(dt, 3),
                       "pair_fits_per_s": round(capacity, 1),
                       "latency_ms_p50": round(percentile(lat_sat, 50), 1),

@@ -208,3 +208,19 @@ def test_round_robin_policy_same_results(served):
         co.serve_stop()
         co.set_option("serve_policy", 1)
         co.serve_start(max_query_rows=48, max_pairs=40_000)
+
+
+def test_mixed_precision_cohort_serves_its_own_form():
+    """A cohort made with the FP32-mixed form serves with that form: a query
+    that repeats a cohort row gives kestrel_fit_pairs' records on the same
+    cohort, bit for bit."""
+    say("case: serving a KESTREL_MIXED cohort, query = cohort sample 5")
+    c = synth.generate(96, 6000, seed=43, n_family_pairs=6, missing=0.02)
+    with Cohort(c.codes, precision="mixed") as co:
+        co.allele_freqs()
+        co.serve_start(max_query_rows=8, max_pairs=4_000)
+        sub = np.array([0, 1, 2, 17, 40, 95])
+        out, _ = co.wait(co.submit(c.codes[5:6], sub))
+        exp = co.fit_pairs(np.stack([np.full(len(sub), 5), sub], 1))
+        exp["x"] = 0
+        assert_bitexact(out, exp)
