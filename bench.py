@@ -341,7 +341,7 @@ def cls_bench(args):
     print(json.dumps(line), flush=True)
 
 
-def mixed_errors(codes, co_mixed, config, n_pairs=20000, seed=0xE44):
+def mixed_errors(codes, co_mixed, config, n_pairs=20000, seed=0xE44, gl=None):
     """Errors of the mixed form against the FP64 form on a seeded sample of
     off-diagonal pairs, by the bar of the FP32-mixed form: max |dDelta_k|,
     |d kinship|, relative objective and log10 condition on the pairs whose
@@ -354,7 +354,7 @@ def mixed_errors(codes, co_mixed, config, n_pairs=20000, seed=0xE44):
     keep = x != y
     pairs = np.stack([np.minimum(x, y), np.maximum(x, y)], 1)[keep]
     a = co_mixed.fit_pairs(pairs)
-    with Cohort(codes) as co64:
+    with (Cohort(codes) if gl is None else Cohort.gl(gl)) as co64:
         b = co64.fit_pairs(pairs)
     lc = np.log10(b["cond"])
     lca = np.log10(a["cond"])
@@ -449,9 +449,9 @@ def main():
         f"{f' with {args.multi:.0%} tri/tetra-allelic sites' if buckets else ''} generated in "
         f"{time.perf_counter() - t:.1f}s")
     t = time.perf_counter()
-    if args.precision == "mixed" and (args.soft or buckets):
-        sys.exit("--precision mixed applies to 2-bit cohorts (hard calls)")
-    co = (Cohort.gl(gl) if args.soft else Cohort.gl_multi(*buckets) if buckets
+    if args.precision == "mixed" and buckets:
+        sys.exit("--precision mixed applies to 2-bit and biallelic likelihood cohorts")
+    co = (Cohort.gl(gl, precision=args.precision) if args.soft else Cohort.gl_multi(*buckets) if buckets
           else Cohort(c.codes, precision=args.precision))
     if args.segment_sites:
         co.set_option("segment_sites", args.segment_sites)
@@ -510,7 +510,8 @@ def main():
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_VALU_PEAK_TFLOPS, 4),
                 "traffic": (SWEEP_TRAFFIC_BYTES_C2 if args.config == "c2" and not args.segment_sites
                             and not args.soft and not buckets else None),
-                "kernel": ("sweep_gl + sweep_glm" if buckets else "sweep_gl" if args.soft
+                "kernel": ("sweep_gl + sweep_glm" if buckets
+                           else ("sweep_gl_mixed" if args.precision == "mixed" else "sweep_gl") if args.soft
                            else "sweep_mixed" if args.precision == "mixed" else "sweep_fp64"),
                 "avg_launch_ms": round(sweep_ms / sweeps, 4),
                 "concurrent_launches": groups, "sweep_busy_ms_per_step": round(busy_ms / args.steps, 2),
@@ -533,7 +534,7 @@ def main():
         log(f"fp64 peak measurement failed: {e}")
     precision_error = None
     if mixed:
-        precision_error = mixed_errors(c.codes, co, args.config)
+        precision_error = mixed_errors(c.codes, co, args.config, gl=gl)
         log(f"mixed vs FP64 form: {precision_error}")
     cpu = None
     if buckets:

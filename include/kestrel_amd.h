@@ -45,10 +45,12 @@ extern "C" {
 #define KESTREL_PAIR_NO_SITES 1u /* no usable site: the reference panics (lanevector.rs:133-138) */
 
 /* precision forms: KESTREL_FP64 is the reference's arithmetic (bit-exact in
- * the device's summation order); KESTREL_MIXED computes the per-site terms in
- * FP32 (32-site FP32 sums added to FP64 totals; log, objective, QP and
- * condition in FP64), 2-bit cohorts only, its errors against FP64 measured,
- * not bounded */
+ * the device's summation order); KESTREL_MIXED keeps the c_s table (or, for
+ * likelihood rows, the c_s construction), the reciprocal for the Hessian and
+ * the Hessian terms in FP32 and L, the gradient, the log, the objective, the
+ * QP and the condition in FP64; its errors against FP64 are measured, not
+ * bounded.  2-bit cohorts take it at kestrel_cohort_create, others through
+ * the option "precision". */
 #define KESTREL_FP64 0
 #define KESTREL_MIXED 1
 
@@ -200,7 +202,9 @@ int kestrel_cohort_create_gt_multi(kestrel_cohort** out, uint32_t n_samples, con
  * share their first sample, and a sweep passes over the sites where a
  * wavefront's shared sample is missing, which add nothing; 1 = on, 0 = off, -1 = on when
  * at least 2 % of the kept calls are missing (default); results do not
- * change),
+ * change), "precision" (KESTREL_FP64 or KESTREL_MIXED, for 2-bit and
+ * biallelic likelihood-row cohorts: the form of the next fits; this one
+ * changes records),
  * "serve_policy" (order in
  * which a serving engine started afterwards feeds admitted requests to the
  * device: 1 = shortest remaining first (default), 0 = round robin). */

@@ -119,10 +119,13 @@ class Cohort:
 
     @classmethod
     def gl(cls, gl: np.ndarray, *, af: np.ndarray | None = None, maf: float | None = None,
-           af_iterations: int = 100, segment_sites: int | None = None) -> "Cohort":
+           af_iterations: int = 100, segment_sites: int | None = None, precision: str = "fp64") -> "Cohort":
         """Cohort of soft genotype likelihoods (kestrel_cohort_create_gl): gl is
         (V, N, 3) site-major linear-scale likelihoods of 0/0, 0/1, 1/1, a NaN
-        marking a missing call."""
+        marking a missing call.  precision "mixed": option "precision"
+        KESTREL_MIXED."""
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
         L = _lib.load()
         self = cls.__new__(cls)
         g = np.ascontiguousarray(gl, np.float64)
@@ -136,6 +139,8 @@ class Cohort:
         self._h = h
         if segment_sites is not None:
             self.set_option("segment_sites", segment_sites)
+        if precision == "mixed":
+            self.set_option("precision", _lib.MIXED)
         return self
 
     @classmethod

@@ -503,6 +503,14 @@ int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value) {
         c->opt_pair_tile = value;
         return KESTREL_OK;
     }
+    if (!strcmp(key, "precision")) {
+        if (value != KESTREL_FP64 && value != KESTREL_MIXED)
+            return fail(KESTREL_EINVAL, "precision must be KESTREL_FP64 or KESTREL_MIXED");
+        if (value == KESTREL_MIXED && (c->gt || !c->parts.empty()))
+            return fail(KESTREL_EINVAL, "the mixed-precision form covers 2-bit and biallelic likelihood cohorts");
+        c->mixed = value == KESTREL_MIXED;
+        return KESTREL_OK;
+    }
     if (!strcmp(key, "skip_masked")) {
         if (value < -1 || value > 1) return fail(KESTREL_EINVAL, "skip_masked must be -1, 0 or 1");
         c->opt_skip_masked = value;

@@ -776,11 +776,24 @@ __device__ __forceinline__ void site_coefficients(double p, double q, double* K)
     K[K_2PQ3] = twopq3;
 }
 
-// mixture.rs:138-233 block by block
-__device__ __forceinline__ void mixture_soft(const double* lx, const double* ly, const double* K, double* a) {
+// The mixed-precision form's coefficients: the FP64 ones rounded to FP32,
+// KF_W floats a site (K_N and 2 of padding: 16-byte reads).
+constexpr int KF_W = 20;
+constexpr int KF_BUF = CHUNK * KF_W;  // floats per coefficient buffer
+
+__device__ __forceinline__ void site_coefficients_f32(double p, double q, float* K) {
+    double Kd[K_N];
+    site_coefficients(p, q, Kd);
 #pragma unroll
-    for (int k = 0; k < 9; k++) a[k] = 0.0;
-    double g = lx[0] * ly[0];  // (0,0) (0,0)
+    for (int k = 0; k < K_N; k++) K[k] = (float)Kd[k];
+}
+
+// mixture.rs:138-233 block by block (T = float in the mixed-precision form)
+template <typename T>
+__device__ __forceinline__ void mixture_soft(const T* lx, const T* ly, const T* K, T* a) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) a[k] = T(0);
+    T g = lx[0] * ly[0];  // (0,0) (0,0)
     a[0] = fma(g, K[K_P], a[0]);
     a[1] = fma(g, K[K_P2], a[1]);
     a[2] = fma(g, K[K_P2], a[2]);
@@ -956,6 +969,161 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_gl(const uint32_t* __restr
         for (int k = 0; k < 9; k++) out[(size_t)(1 + k) * nslots] = G[k];
 #pragma unroll
         for (int k = 0; k < 45; k++) out[(size_t)(10 + k) * nslots] = H[k];
+        out[(size_t)55 * nslots] = N;
+    }
+}
+
+// The mixed-precision form of sweep_gl (precision KESTREL_MIXED): the site
+// coefficients, the likelihood products and c_s (mixture_row2's order), 1/L,
+// the scaled c' and the 45 Hessian terms in FP32 (the Hessian summed in FP32
+// over the segment); L = eps + c.D, the gradient terms (exact FP64 products
+// of the FP32 c and 1/L), the log product and the objective in FP64, as in
+// sweep_mixed.
+template <int STRIDE>
+__global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_gl_mixed(const uint32_t* __restrict__ geno, uint32_t row_words,
+                                                                const double* __restrict__ gl, uint32_t v_pad,
+                                                                const double* __restrict__ af,
+                                                                const uint32_t* __restrict__ seg_bounds,
+                                                                const uint32_t* __restrict__ sx,
+                                                                const uint32_t* __restrict__ sy,
+                                                                const double* __restrict__ pt,
+                                                                const int* __restrict__ active, uint32_t nslots,
+                                                                double eps, double* __restrict__ partial) {
+    __shared__ __attribute__((aligned(16))) float kt[2 * KF_BUF];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t slot = blockIdx.x * SWEEP_TPB + tid;
+    const uint32_t seg = blockIdx.y;
+    const bool live = slot < nslots && active[slot] != 0;
+    if (!__syncthreads_or(live)) return;
+
+    double D[9];
+    uint32_t x = 0, y = 0;
+    if (live) {
+        x = sx[slot];
+        y = sy[slot];
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k] = pt[(size_t)k * nslots + slot];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k] = 0.0;
+    }
+    const uint32_t* rowx = geno + (size_t)x * row_words;
+    const uint32_t* rowy = geno + (size_t)y * row_words;
+    const double* glx = gl + (size_t)x * v_pad * STRIDE;
+    const double* gly = gl + (size_t)y * v_pad * STRIDE;
+
+    // the Hessian is summed in FP32 over the whole segment (its FP64 totals
+    // beside FP32 chunk sums would not fit the registers with the rows)
+    double F = 0.0, N = 0.0, P = 1.0, G[9];
+    float h[45];
+#pragma unroll
+    for (int k = 0; k < 9; k++) G[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 45; k++) h[k] = 0.0f;
+
+    const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
+    auto build = [&](uint32_t cs, int b) {
+        if (tid < CHUNK)
+            site_coefficients_f32(af[2 * (cs + tid)], af[2 * (cs + tid) + 1], kt + b * KF_BUF + tid * KF_W);
+    };
+    if (s0 < s1) build(s0, 0);
+    __syncthreads();
+    int b = 0;
+    for (uint32_t cs = s0; cs < s1; cs += CHUNK, b ^= 1) {
+        if (cs + CHUNK < s1) build(cs + CHUNK, b ^ 1);
+        const uint2 wx = *reinterpret_cast<const uint2*>(rowx + cs / 16);
+        const uint2 wy = *reinterpret_cast<const uint2*>(rowy + cs / 16);
+        {
+            const uint32_t ux = ~((wx.x & (wx.x >> 1)) | (wy.x & (wy.x >> 1))) & 0x55555555u;
+            const uint32_t uy = ~((wx.y & (wx.y >> 1)) | (wy.y & (wy.y >> 1))) & 0x55555555u;
+            N += (double)(__popc(ux) + __popc(uy));
+        }
+        const float* K0 = kt + b * KF_BUF;
+#pragma unroll
+        for (int k = 0; k < CHUNK; k += 2) {
+            double lx[2][3], ly[2][3];
+            if (STRIDE == 3) {
+                const double2* px = reinterpret_cast<const double2*>(glx + (size_t)(cs + k) * 3);
+                const double2* py = reinterpret_cast<const double2*>(gly + (size_t)(cs + k) * 3);
+                const double2 x0 = px[0], x1 = px[1], x2 = px[2];
+                const double2 y0 = py[0], y1 = py[1], y2 = py[2];
+                lx[0][0] = x0.x; lx[0][1] = x0.y; lx[0][2] = x1.x;
+                lx[1][0] = x1.y; lx[1][1] = x2.x; lx[1][2] = x2.y;
+                ly[0][0] = y0.x; ly[0][1] = y0.y; ly[0][2] = y1.x;
+                ly[1][0] = y1.y; ly[1][1] = y2.x; ly[1][2] = y2.y;
+            } else {
+                const double2* px = reinterpret_cast<const double2*>(glx + (size_t)(cs + k) * 2);
+                const double2* py = reinterpret_cast<const double2*>(gly + (size_t)(cs + k) * 2);
+#pragma unroll
+                for (int hh = 0; hh < 2; hh++) {
+                    const int kk = k + hh;
+                    const uint32_t cx = ((kk < 16 ? wx.x : wx.y) >> (2 * (kk & 15))) & 3u;
+                    const uint32_t cy = ((kk < 16 ? wy.x : wy.y) >> (2 * (kk & 15))) & 3u;
+                    const double2 vx = px[hh], vy = py[hh];
+                    lx[hh][0] = cx == 0 ? 1.0 : vx.x;
+                    lx[hh][1] = cx == 0 ? vx.x : cx == 1 ? 1.0 : vx.y;
+                    lx[hh][2] = cx == 2 ? 1.0 : vx.y;
+                    ly[hh][0] = cy == 0 ? 1.0 : vy.x;
+                    ly[hh][1] = cy == 0 ? vy.x : cy == 1 ? 1.0 : vy.y;
+                    ly[hh][2] = cy == 2 ? 1.0 : vy.y;
+                }
+            }
+#pragma unroll
+            for (int hh = 0; hh < 2; hh++) {
+                const int kk = k + hh;
+                const uint32_t ax = (kk < 16 ? wx.x : wx.y) >> (2 * (kk & 15));
+                const uint32_t ay = (kk < 16 ? wy.x : wy.y) >> (2 * (kk & 15));
+                const bool missing = (ax & 3u) == 3u || (ay & 3u) == 3u;
+                float K[KF_W];
+                const float4* k4 = reinterpret_cast<const float4*>(K0 + kk * KF_W);
+#pragma unroll
+                for (int j = 0; j < KF_W / 4; j++) {
+                    const float4 v = k4[j];
+                    K[4 * j] = v.x;
+                    K[4 * j + 1] = v.y;
+                    K[4 * j + 2] = v.z;
+                    K[4 * j + 3] = v.w;
+                }
+                float lxf[3], lyf[3], cv[9];
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    lxf[j] = (float)lx[hh][j];
+                    lyf[j] = (float)ly[hh][j];
+                }
+                mixture_soft<float>(lxf, lyf, K, cv);
+                double cd[9];
+#pragma unroll
+                for (int j = 0; j < 9; j++) cd[j] = (double)cv[j];
+                double L = missing ? 1.0 : eps;
+#pragma unroll
+                for (int j = 0; j < 9; j++) L = fma(D[j], cd[j], L);
+                P = P * L;
+                if ((kk & (LOG_GROUP - 1)) == LOG_GROUP - 1) {
+                    F = F + sleef_log_pos(P);
+                    P = 1.0;
+                }
+                const float r = __builtin_amdgcn_rcpf((float)L);
+                const double rd = (double)r;
+#pragma unroll
+                for (int j = 0; j < 9; j++) G[j] = fma(cd[j], rd, G[j]);
+#pragma unroll
+                for (int j = 0; j < 9; j++) cv[j] = cv[j] * r;
+                int id = 0;
+#pragma unroll
+                for (int a2 = 0; a2 < 9; a2++)
+#pragma unroll
+                    for (int bb = a2; bb < 9; bb++, id++) h[id] = fmaf(cv[a2], cv[bb], h[id]);
+            }
+        }
+        __syncthreads();
+    }
+    if (live) {
+        double* out = partial + (size_t)seg * NPART * nslots + slot;
+        out[0] = F;
+#pragma unroll
+        for (int k = 0; k < 9; k++) out[(size_t)(1 + k) * nslots] = G[k];
+#pragma unroll
+        for (int k = 0; k < 45; k++) out[(size_t)(10 + k) * nslots] = (double)h[k];
         out[(size_t)55 * nslots] = N;
     }
 }
@@ -2235,7 +2403,13 @@ hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, con
                  uint32_t nslots, double eps, double* partial, hipStream_t st, const double* gl, uint32_t v_pad,
                  int gl_stride, bool skip_masked, bool mixed) {
     dim3 grid((nslots + SWEEP_TPB - 1) / SWEEP_TPB, nseg);
-    if (gl && gl_stride == 2)
+    if (gl && mixed && gl_stride == 2)
+        hipLaunchKernelGGL(sweep_gl_mixed<2>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, af,
+                           seg_bounds, sx, sy, pt, active, nslots, eps, partial);
+    else if (gl && mixed)
+        hipLaunchKernelGGL(sweep_gl_mixed<3>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, af,
+                           seg_bounds, sx, sy, pt, active, nslots, eps, partial);
+    else if (gl && gl_stride == 2)
         hipLaunchKernelGGL(sweep_gl<2>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, af, seg_bounds, sx,
                            sy, pt, active, nslots, eps, partial);
     else if (gl)

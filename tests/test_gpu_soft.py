@@ -97,6 +97,26 @@ def test_soft_fit_bitexact_and_tolerance(O, soft):
     assert (np.abs(out["obj"] - ref["obj"]) / np.abs(ref["obj"]))[well].max() <= 1e-9
 
 
+def test_soft_mixed_precision_form(soft):
+    """The mixed-precision form on likelihood rows (option "precision"): Delta
+    and objective within 1e-6 of the FP64 form on the pairs both bring to
+    convergence with log10 condition below 12; n_xy equal."""
+    c, gl, co = soft
+    say("case: soft fit, mixed-precision form vs FP64 form")
+    b = co.fit_pairs()
+    with Cohort.gl(gl, precision="mixed") as m:
+        a = m.fit_pairs()
+    assert np.array_equal(a["n_sites"], b["n_sites"])
+    lb, la = np.log10(b["cond"]), np.log10(a["cond"])
+    both = np.isfinite(lb) & np.isfinite(la) & (lb < 12)
+    assert both.sum() > 0.9 * len(b)
+    d = np.abs(a["jacquard"] - b["jacquard"]).max(1)[both]
+    say(f"soft mixed vs fp64: max |dDelta| {d.max():.3g} over {both.sum()} pairs; iterations "
+        f"{a['iters'].mean():.2f} vs {b['iters'].mean():.2f}")
+    assert d.max() <= 1e-6
+    assert (np.abs(a["obj"] - b["obj"]) / np.abs(b["obj"]))[both].max() <= 1e-6
+
+
 def test_rows_without_an_exact_one(O, soft):
     """Rows scaled by their maximum are stored as the two entries beside the
     exact 1.0 (16 bytes); a cohort with a called row that has no entry equal
