@@ -599,8 +599,11 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
                           (c->opt_skip_masked == 1 || (c->opt_skip_masked < 0 && c->missing_frac >= 0.02));
     // pairs in flight per group: fill the device unless set (twice with runs:
     // workgroups then end at different times, and a launch of one round would
-    // last as long as its slowest), never more than the pairs available
-    uint64_t nslots = c->opt_slots > 0 ? (uint64_t)c->opt_slots : default_slots(c) * (use_runs ? 2 : 1);
+    // last as long as its slowest; three times for the 2-bit mixed form, whose
+    // sweep fits three wavefronts per SIMD: measured 571,872 -> 593,813 on C2
+    // and 596,779 -> 657,444 on C4), never more than the pairs available
+    const uint64_t fills = use_runs ? 2 : (c->mixed && !c->soft) ? 3 : 1;
+    uint64_t nslots = c->opt_slots > 0 ? (uint64_t)c->opt_slots : default_slots(c) * fills;
     nslots = std::min<uint64_t>(nslots, ((uint64_t)n_pairs + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB);
     nslots = (nslots + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB;
     const uint32_t ns = (uint32_t)nslots;

@@ -597,15 +597,17 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
 
 // The mixed-precision form (precision KESTREL_MIXED): the pass of sweep_fp64
 // with the c_s table in FP32 and the Hessian in FP32 -- 1/L by v_rcp_f32, the
-// scaled c' = c/L and the 45 Hessian terms, summed in FP32 over each 32-site
-// chunk and added to FP64 totals at the chunk's end.  L = eps + c.D and the
-// gradient terms c * (1/L) are FP64 (from the FP32 c and 1/L, whose products
-// are exact in FP64): the line search compares objectives, and the
-// convergence test |g_i - lambda| <= 1e-8 reads the gradient, at resolutions
-// FP32 sums do not reach (with L or the gradient in FP32 the fits took 1.35x
-// the iterations and 3.4x the evaluations).  The log product, the objective
-// and everything the step does stay FP64.  Results are not the FP64 form's
-// bits: the errors against it are measured, not bounded.
+// scaled c' = c/L and the 45 Hessian terms, summed in FP32 over the segment
+// (without FP64 Hessian totals the sweep needs 168 VGPRs: three wavefronts
+// per SIMD; the Hessian only steers the Newton step, and fits take the FP64
+// form's iterations).  L = eps + c.D and the gradient terms c * (1/L) are
+// FP64 (from the FP32 c and 1/L, whose products are exact in FP64): the line
+// search compares objectives, and the convergence test |g_i - lambda| <= 1e-8
+// reads the gradient, at resolutions FP32 sums do not reach (with L or the
+// gradient in FP32 the fits took 1.35x the iterations and 3.4x the
+// evaluations).  The log product, the objective and everything the step does
+// stay FP64.  Results are not the FP64 form's bits: the errors against it are
+// measured, not bounded.
 __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_mixed(const uint32_t* __restrict__ geno, uint32_t row_words,
                                                              const double* __restrict__ af,
                                                              const uint32_t* __restrict__ seg_bounds,
@@ -642,11 +644,12 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_mixed(const uint32_t* __re
     const uint32_t* rowx = geno + (size_t)x * row_words;
     const uint32_t* rowy = geno + (size_t)y * row_words;
 
-    double F = 0.0, N = 0.0, P = 1.0, G[9], H[45];
+    double F = 0.0, N = 0.0, P = 1.0, G[9];
+    float h[45];
 #pragma unroll
     for (int k = 0; k < 9; k++) G[k] = 0.0;
 #pragma unroll
-    for (int k = 0; k < 45; k++) H[k] = 0.0;
+    for (int k = 0; k < 45; k++) h[k] = 0.0f;
 
     const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
     const int lut_site = tid >> 3, lut_part = tid & 7;
@@ -686,9 +689,6 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_mixed(const uint32_t* __re
             const uint32_t uy = ~((wx.y & (wx.y >> 1)) | (wy.y & (wy.y >> 1))) & 0x55555555u;
             N += (double)(__popc(ux) + __popc(uy));
         }
-        float h[45];
-#pragma unroll
-        for (int k = 0; k < 45; k++) h[k] = 0.0f;
 #pragma unroll
         for (int k = 0; k < CHUNK; k++) {
             const uint32_t ax = (k < 16 ? wx.x : wx.y) >> (2 * (k & 15));
@@ -720,8 +720,6 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_mixed(const uint32_t* __re
 #pragma unroll
                 for (int bb = a2; bb < 9; bb++, id++) h[id] = fmaf(cv[a2], cv[bb], h[id]);
         }
-#pragma unroll
-        for (int k = 0; k < 45; k++) H[k] = H[k] + (double)h[k];
         wx = nwx;
         wy = nwy;
         pn = pnn;
@@ -734,7 +732,7 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_mixed(const uint32_t* __re
 #pragma unroll
         for (int k = 0; k < 9; k++) out[(size_t)(1 + k) * nslots] = G[k];
 #pragma unroll
-        for (int k = 0; k < 45; k++) out[(size_t)(10 + k) * nslots] = H[k];
+        for (int k = 0; k < 45; k++) out[(size_t)(10 + k) * nslots] = (double)h[k];
         out[(size_t)55 * nslots] = N;
     }
 }
