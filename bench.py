@@ -55,17 +55,26 @@ def cpu_threads() -> int:
     return max(1, n)
 
 
-def cpu_baseline(codes, af, budget_s: float, gl=None):
+def cpu_baseline(codes, af, budget_s: float, gl=None, buckets=None):
     """Oracle in the reference's order (8-lane AVX-512 loops, per-pair c_s buffer,
     work stealing over pairs) on a seeded random sample of off-diagonal pairs;
-    gl (N, V, 3) likelihood rows instead of codes when given."""
+    gl (N, V, 3) likelihood rows instead of codes when given; buckets (the
+    2-, 3-, 4-allele likelihood rows, site-major) for a multi-allelic cohort,
+    whose frequencies and re-bucketing the oracle computes first, untimed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     threads = cpu_threads()
-    n = codes.shape[0] if gl is None else gl.shape[0]
+    parts = None
+    if buckets is not None:
+        parts, _ = O.multi_prepare({A: b for A, b in zip((2, 3, 4), buckets)})
+        n = buckets[0].shape[1]
+    else:
+        n = codes.shape[0] if gl is None else gl.shape[0]
 
     def fit(pairs):
+        if parts is not None:
+            return O.multi_fit_pairs_threads(parts, pairs, nthreads=threads)
         if gl is None:
             return O.fit_pairs(codes, af, pairs=pairs, order=O.ORDER_REF, nthreads=threads)
         return O.fit_pairs_gl(gl, af, pairs=pairs, order=O.ORDER_REF, nthreads=threads)
@@ -537,9 +546,9 @@ def main():
         precision_error = mixed_errors(c.codes, co, args.config, gl=gl)
         log(f"mixed vs FP64 form: {precision_error}")
     cpu = None
-    if buckets:
-        log("cpu baseline: not measured for tri/tetra-allelic cohorts (the oracle's multi-allelic path is "
-            "Python over sites)")
+    if buckets and args.cpu_budget > 0:
+        cpu = cpu_baseline(None, None, args.cpu_budget, buckets=buckets)
+        log(f"cpu baseline: {cpu}")
     elif args.cpu_budget > 0:
         glk = None if gl is None else np.ascontiguousarray(gl.transpose(1, 0, 2)[:, kept])
         cpu = cpu_baseline(c.codes[:, kept], af[kept], args.cpu_budget, gl=glk)

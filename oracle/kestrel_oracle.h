@@ -144,6 +144,13 @@ int ko_allele_freqA_rows(int A, const double* gl, size_t n, uint64_t iterations,
 size_t ko_pair_rows_generic(int A, const double* lx, const double* ly, const double* af, size_t V, double* rows,
                             uint8_t* valid);
 
+/* Pair fits over re-bucketed 2-, 3- and 4-allele sites in the reference's
+ * order (coefficients.rs:97-107), gl_A sample-major N x V_A x G_A, af_A
+ * V_A x A; -1 if a pair names a sample >= N. */
+int ko_fit_pairs_multi(uint32_t N, const double* gl2, const double* af2, uint32_t V2, const double* gl3,
+                       const double* af3, uint32_t V3, const double* gl4, const double* af4, uint32_t V4,
+                       const uint32_t* pairs, size_t npairs, uint64_t iterations, int nthreads, ko_output* out);
+
 /* Soft likelihoods (vcf.rs:338-410 rows: 10^(gl - max) or 10^(-pl/10), NaN =
  * missing), sample-major N x V x 3.  Pair rows of one pair (V x 3 each), the
  * allele-frequency fit (allele/mod.rs:37-134) and the pair fits

@@ -1259,6 +1259,118 @@ int ko_fit_pairs_gl(int order, const double* gl, uint32_t N, uint32_t V, const d
     return fit_pairs(order, NULL, gl, N, V, af, pairs, npairs, nsel, seg, nseg, iterations, nthreads, out);
 }
 
+/* Pair fits over re-bucketed 2-, 3- and 4-allele sites in the reference's
+ * order (coefficients.rs:97-107: the 8-lane parts of the 2-, 3- and 4-allele
+ * rows, then their remainders, each compacted to the usable rows by
+ * LaneVectorBuilder, lanevector.rs:99-139).  gl_A is sample-major N x V_A x
+ * G_A, af_A is V_A x A; the 2-allele rows come from mixture.rs:96-236, the
+ * others from mixture.rs:8-60.  Threads take pairs with an atomic counter. */
+typedef struct {
+    const double* gl[3];
+    const double* af[3];
+    uint32_t V[3];
+    const uint32_t* pairs;
+    size_t npairs;
+    ko_tune tune;
+    ko_output* out;
+    atomic_size_t next;
+} multi_job;
+
+static void* multi_worker(void* vj) {
+    multi_job* j = (multi_job*)vj;
+    const size_t Vt = (size_t)j->V[0] + j->V[1] + j->V[2];
+    double* rows[3];
+    uint8_t* valid[3];
+    for (int b = 0; b < 3; b++) {
+        rows[b] = (double*)malloc(((size_t)j->V[b] + 1) * 9 * sizeof(double));
+        valid[b] = (uint8_t*)malloc((size_t)j->V[b] + 1);
+    }
+    double* comp = (double*)malloc((Vt + 1) * 9 * sizeof(double));
+    pair_ctx ctx;
+    memset(&ctx, 0, sizeof ctx);
+    ctx.order = KO_ORDER_REF;
+    ctx.lv.blocks = (double*)malloc(((Vt / 8) + 1) * 72 * sizeof(double));
+    for (;;) {
+        size_t i = atomic_fetch_add(&j->next, 1);
+        if (i >= j->npairs) break;
+        const uint32_t x = j->pairs[2 * i], y = j->pairs[2 * i + 1];
+        ko_output* o = j->out + i;
+        memset(o, 0, sizeof *o);
+        o->x = x;
+        o->y = y;
+        for (int b = 0; b < 3; b++) {
+            const int A = b + 2, G = A * (A + 1) / 2;
+            const size_t V = j->V[b];
+            if (V == 0) continue;
+            const double* lx = j->gl[b] + (size_t)x * V * G;
+            const double* ly = j->gl[b] + (size_t)y * V * G;
+            if (A == 2)
+                ko_pair_rows_gl(lx, ly, j->af[b], V, rows[b], valid[b]);
+            else
+                ko_pair_rows_generic(A, lx, ly, j->af[b], V, rows[b], valid[b]);
+        }
+        size_t n = 0;
+        for (int half = 0; half < 2; half++)
+            for (int b = 0; b < 3; b++) {
+                const size_t m = (j->V[b] / 8) * 8;
+                const size_t lo = half ? m : 0, hi = half ? j->V[b] : m;
+                for (size_t s = lo; s < hi; s++)
+                    if (valid[b][s]) memcpy(comp + 9 * n++, rows[b] + 9 * s, 72);
+            }
+        o->n_sites = (uint32_t)n;
+        if (n == 0) { /* lanevector.rs:133-138 would panic; report NaN */
+            for (int k = 0; k < 9; k++) o->jacquard[k] = NAN;
+            o->obj = NAN;
+            o->cond = NAN;
+            continue;
+        }
+        ctx.lv.nblocks = n / 8;
+        ctx.lv.nrem = n % 8;
+        for (size_t bl = 0; bl < ctx.lv.nblocks; bl++)
+            for (int l = 0; l < 8; l++)
+                for (int c = 0; c < 9; c++) ctx.lv.blocks[bl * 72 + c * 8 + l] = comp[(bl * 8 + l) * 9 + c];
+        for (size_t r = 0; r < ctx.lv.nrem; r++)
+            for (int c = 0; c < 9; c++) ctx.lv.rem[r * 9 + c] = comp[(ctx.lv.nblocks * 8 + r) * 9 + c];
+        fit_ctx(&ctx, x == y, &j->tune, o);
+    }
+    for (int b = 0; b < 3; b++) {
+        free(rows[b]);
+        free(valid[b]);
+    }
+    free(comp);
+    free(ctx.lv.blocks);
+    return NULL;
+}
+
+int ko_fit_pairs_multi(uint32_t N, const double* gl2, const double* af2, uint32_t V2, const double* gl3,
+                       const double* af3, uint32_t V3, const double* gl4, const double* af4, uint32_t V4,
+                       const uint32_t* pairs, size_t npairs, uint64_t iterations, int nthreads, ko_output* out) {
+    for (size_t i = 0; i < 2 * npairs; i++)
+        if (pairs[i] >= N) return -1;
+    multi_job j;
+    memset(&j, 0, sizeof j);
+    j.gl[0] = gl2;
+    j.gl[1] = gl3;
+    j.gl[2] = gl4;
+    j.af[0] = af2;
+    j.af[1] = af3;
+    j.af[2] = af4;
+    j.V[0] = V2;
+    j.V[1] = V3;
+    j.V[2] = V4;
+    j.pairs = pairs;
+    j.npairs = npairs;
+    ko_tune_new(&j.tune, iterations, 10); /* coefficients.rs:73 */
+    j.out = out;
+    atomic_init(&j.next, 0);
+    int nt = nthreads_default(nthreads);
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nt);
+    for (int i = 0; i < nt; i++) pthread_create(&th[i], NULL, multi_worker, &j);
+    for (int i = 0; i < nt; i++) pthread_join(th[i], NULL);
+    free(th);
+    return 0;
+}
+
 /* ------------------------------------------------- CLS (Brier) path, -f GT */
 
 /* wrapper.rs:412-428 calculate_allele_frequencies::<A>: frequencies by

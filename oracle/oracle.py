@@ -103,6 +103,8 @@ def lib():
         L.ko_allele_freqA_rows.argtypes = [I, P, SZ, U64, D, P, P]
         L.ko_pair_rows_generic.restype = SZ
         L.ko_pair_rows_generic.argtypes = [I, P, P, P, SZ, P, P]
+        L.ko_fit_pairs_multi.restype = I
+        L.ko_fit_pairs_multi.argtypes = [U32, P, P, U32, P, P, U32, P, P, U32, P, SZ, U64, I, P]
         _lib = L
     return _lib
 
@@ -467,6 +469,25 @@ def multi_fit_pairs(parts, pairs, order=ORDER_REF, pads=None, seg=None, iteratio
             o = fit_rows(rows, self_pair=x == y, order=ORDER_SEG, valid=valid, seg=seg, iterations=iterations)
         out[i] = o
         out[i]["x"], out[i]["y"] = x, y
+    return out
+
+
+def multi_fit_pairs_threads(parts, pairs, iterations=100, nthreads=0):
+    """multi_fit_pairs in the reference's order, in C on nthreads threads
+    (ko_fit_pairs_multi); parts as multi_prepare returns them."""
+    pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+    n = next(parts[A][1].shape[0] for A in (2, 3, 4) if parts[A][1].size)
+    args = []
+    for A in (2, 3, 4):
+        af, gl = parts[A]
+        af, gl = f64(af), f64(gl)
+        args += [gl, af, af.shape[0]]
+    out = np.zeros(len(pairs), OUTPUT_DTYPE)
+    rc = lib().ko_fit_pairs_multi(n, _p(args[0]), _p(args[1]), args[2], _p(args[3]), _p(args[4]), args[5],
+                                  _p(args[6]), _p(args[7]), args[8], _p(pairs), len(pairs), iterations, nthreads,
+                                  _p(out))
+    if rc != 0:
+        raise ValueError("a pair names a sample outside the cohort")
     return out
 
 

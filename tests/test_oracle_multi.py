@@ -165,3 +165,29 @@ def test_multi_fit_kkt(O):
             for xi, gi in zip(o["jacquard"], g):
                 assert (gi >= lam - 1e-8 - 1e-10) if xi == 0.0 else abs(gi - lam) <= 1e-8 + 1e-10
         assert abs(-np.log(L).mean() - o["obj"]) <= 1e-12 * abs(o["obj"])
+
+
+def test_threaded_multi_fit_equals_the_pair_loop(O):
+    """ko_fit_pairs_multi (the CPU baseline of the multi-allelic bench) gives
+    the records of the per-pair loop, bit for bit, with the 2- and 4-allele
+    buckets of ragged sizes (8-lane parts and remainders) and an empty one."""
+    rng = np.random.default_rng(33)
+    n = 12
+    b2 = rng.random((203, n, 3))
+    b3 = rng.random((29, n, 6))
+    b4 = rng.random((11, n, 10))
+    for b in (b2, b3, b4):
+        b /= b.max(axis=2, keepdims=True)
+    b2[7, 4] = np.nan
+    b3[3, 0] = np.nan
+    pairs = np.array([(0, 1), (2, 2), (3, 9), (7, 11), (4, 10), (5, 6)], np.uint32)
+    for buckets in ({2: b2, 3: b3, 4: b4}, {2: b2, 3: None, 4: b4}):
+        parts, _ = O.multi_prepare(buckets)
+        want = O.multi_fit_pairs(parts, pairs)
+        got = O.multi_fit_pairs_threads(parts, pairs, nthreads=3)
+        for k in ("x", "y", "iters", "n_sites"):
+            np.testing.assert_array_equal(got[k], want[k])
+        for k in ("jacquard", "obj", "cond"):
+            np.testing.assert_array_equal(got[k].view(np.uint64), want[k].view(np.uint64))
+    with pytest.raises(ValueError):
+        O.multi_fit_pairs_threads(parts, np.array([(0, n)], np.uint32))
