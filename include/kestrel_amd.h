@@ -201,8 +201,8 @@ int kestrel_cohort_create_gt_multi(kestrel_cohort** out, uint32_t n_samples, con
  * 256-slot workgroups of a fit claim pairs in runs of consecutive claims that
  * share their first sample, and a sweep passes over the sites where a
  * wavefront's shared sample is missing, which add nothing; 1 = on, 0 = off, -1 = on when
- * at least 2 % of the kept calls are missing (default); results do not
- * change), "precision" (KESTREL_FP64 or KESTREL_MIXED, for 2-bit and
+ * at least 2 % of the kept calls are missing and the form is FP64 (default);
+ * results do not change), "precision" (KESTREL_FP64 or KESTREL_MIXED, for 2-bit and
  * biallelic likelihood-row cohorts: the form of the next fits; this one
  * changes records),
  * "serve_policy" (order in

@@ -595,13 +595,18 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
 
     // claims in runs and sweeps that pass over the sites where a wavefront's
     // shared sample is missing (2-bit cohorts; option "skip_masked")
-    const bool use_runs = !c->soft && !c->mixed && c->parts.empty() &&
-                          (c->opt_skip_masked == 1 || (c->opt_skip_masked < 0 && c->missing_frac >= 0.02));
+    // (automatic for the FP64 form only: the mixed form's pass holds two
+    // wavefronts per SIMD instead of three, and measured 1,241,070 against
+    // 1,266,335 pair-fits/s without it on C3)
+    const bool use_runs = !c->soft && c->parts.empty() &&
+                          (c->opt_skip_masked == 1 ||
+                           (c->opt_skip_masked < 0 && !c->mixed && c->missing_frac >= 0.02));
     // pairs in flight per group: fill the device unless set (twice with runs:
     // workgroups then end at different times, and a launch of one round would
-    // last as long as its slowest; three times for the 2-bit mixed form, whose
-    // sweep fits three wavefronts per SIMD: measured 571,872 -> 593,813 on C2
-    // and 596,779 -> 657,444 on C4), never more than the pairs available
+    // last as long as its slowest; three times for the 2-bit mixed form
+    // without runs, whose sweep fits three wavefronts per SIMD: measured
+    // 571,872 -> 593,813 on C2 and 596,779 -> 657,444 on C4; its pass over
+    // masked sites fits two), never more than the pairs available
     const uint64_t fills = use_runs ? 2 : (c->mixed && !c->soft) ? 3 : 1;
     uint64_t nslots = c->opt_slots > 0 ? (uint64_t)c->opt_slots : default_slots(c) * fills;
     nslots = std::min<uint64_t>(nslots, ((uint64_t)n_pairs + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB);

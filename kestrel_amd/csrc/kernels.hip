@@ -607,7 +607,10 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
 // gradient in FP32 the fits took 1.35x the iterations and 3.4x the
 // evaluations).  The log product, the objective and everything the step does
 // stay FP64.  Results are not the FP64 form's bits: the errors against it are
-// measured, not bounded.
+// measured, not bounded.  SKIP: sites where the wavefront's shared sample x is
+// missing are passed over as in sweep_fp64<true> (c = 0 and L = 1 there, so
+// no FP32 or FP64 sum changes: the same bits as without).
+template <bool SKIP>
 __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_mixed(const uint32_t* __restrict__ geno, uint32_t row_words,
                                                              const double* __restrict__ af,
                                                              const uint32_t* __restrict__ seg_bounds,
@@ -643,6 +646,16 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_mixed(const uint32_t* __re
     }
     const uint32_t* rowx = geno + (size_t)x * row_words;
     const uint32_t* rowy = geno + (size_t)y * row_words;
+    bool x_shared = false;
+    int x_lane = 0;
+    if (SKIP) {
+        const unsigned long long lm = __ballot(live);
+        if (lm) {
+            x_lane = __ffsll((long long)lm) - 1;
+            const uint32_t xf = (uint32_t)__builtin_amdgcn_readlane((int)x, x_lane);
+            x_shared = __all(!live || x == xf);
+        }
+    }
 
     double F = 0.0, N = 0.0, P = 1.0, G[9];
     float h[45];
@@ -689,8 +702,22 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_mixed(const uint32_t* __re
             const uint32_t uy = ~((wx.y & (wx.y >> 1)) | (wy.y & (wy.y >> 1))) & 0x55555555u;
             N += (double)(__popc(ux) + __popc(uy));
         }
+        uint32_t skip_lo = 0, skip_hi = 0;
+        if (SKIP && x_shared) {
+            const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)wx.x, x_lane);
+            const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)wx.y, x_lane);
+            skip_lo = a & (a >> 1) & 0x55555555u;
+            skip_hi = c & (c >> 1) & 0x55555555u;
+        }
 #pragma unroll
         for (int k = 0; k < CHUNK; k++) {
+            if (SKIP && (((k < 16 ? skip_lo : skip_hi) >> (2 * (k & 15))) & 1u)) {
+                if ((k & (LOG_GROUP - 1)) == LOG_GROUP - 1) {
+                    F = F + sleef_log_pos(P);
+                    P = 1.0;
+                }
+                continue;
+            }
             const uint32_t ax = (k < 16 ? wx.x : wx.y) >> (2 * (k & 15));
             const uint32_t ay = (k < 16 ? wy.x : wy.y) >> (2 * (k & 15));
             const uint32_t row = min(((ax & 3u) << 2) | (ay & 3u), 11u);
@@ -2413,9 +2440,12 @@ hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, con
     else if (gl)
         hipLaunchKernelGGL(sweep_gl<3>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, af, seg_bounds, sx,
                            sy, pt, active, nslots, eps, partial);
+    else if (mixed && skip_masked)
+        hipLaunchKernelGGL(sweep_mixed<true>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, af, seg_bounds, sx, sy,
+                           pt, active, nslots, eps, partial);
     else if (mixed)
-        hipLaunchKernelGGL(sweep_mixed, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, af, seg_bounds, sx, sy, pt,
-                           active, nslots, eps, partial);
+        hipLaunchKernelGGL(sweep_mixed<false>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, af, seg_bounds, sx, sy,
+                           pt, active, nslots, eps, partial);
     else if (skip_masked)
         hipLaunchKernelGGL(sweep_fp64<true>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, af, seg_bounds, sx, sy,
                            pt, active, nslots, eps, partial);

@@ -393,6 +393,29 @@ def test_mixed_precision_form():
         Cohort(c.codes, precision="fp16")
 
 
+def test_mixed_form_with_claims_in_runs_and_skipped_masked_sites():
+    """The mixed-precision form on a cohort with many missing calls: claims in
+    runs and the pass over the wavefront's masked sites (option skip_masked 1;
+    off by default for this form) give the records of per-lane claims with
+    full sweeps bit for bit, for the whole triangle and for a pair list."""
+    say("case: mixed form, claims in runs, masked sites skipped (missing U(0, 0.5))")
+    c = synth.generate(260, 5000, seed=78, missing_per_sample_max=0.5, n_family_pairs=8)
+    rng = np.random.default_rng(6)
+    pairs = rng.integers(0, 260, (2500, 2))
+    with Cohort(c.codes, precision="mixed") as co:
+        auto = co.fit_pairs()
+        co.set_option("skip_masked", 1)
+        on = co.fit_pairs()
+        on_list = co.fit_pairs(pairs)
+        co.set_option("skip_masked", 0)
+        off = co.fit_pairs()
+        off_list = co.fit_pairs(pairs)
+    assert_bitexact(on, off)
+    assert_bitexact(auto, off)
+    assert_bitexact(on_list, off_list)
+    assert np.isfinite(on["jacquard"]).all()
+
+
 def test_given_allele_frequencies(O):
     say("case: given allele frequencies")
     rng = np.random.default_rng(4)
