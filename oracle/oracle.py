@@ -476,7 +476,7 @@ def multi_fit_pairs_threads(parts, pairs, iterations=100, nthreads=0):
     """multi_fit_pairs in the reference's order, in C on nthreads threads
     (ko_fit_pairs_multi); parts as multi_prepare returns them."""
     pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
-    n = next(parts[A][1].shape[0] for A in (2, 3, 4) if parts[A][1].size)
+    n = parts[2][1].shape[0]  # every part's rows are (N, V_A, G_A), V_A possibly 0
     args = []
     for A in (2, 3, 4):
         af, gl = parts[A]
