@@ -45,14 +45,25 @@ extern "C" {
 #define KESTREL_PAIR_NO_SITES 1u /* no usable site: the reference panics (lanevector.rs:133-138) */
 
 /* precision forms: KESTREL_FP64 is the reference's arithmetic (bit-exact in
- * the device's summation order); KESTREL_MIXED keeps the c_s table (or, for
- * likelihood rows, the c_s construction), the reciprocal for the Hessian and
- * the Hessian terms in FP32 and L, the gradient, the log, the objective, the
- * QP and the condition in FP64; its errors against FP64 are measured, not
- * bounded.  2-bit cohorts take it at kestrel_cohort_create, others through
- * the option "precision". */
+ * the device's summation order).  KESTREL_MIXED keeps in FP32 the c_s table
+ * (or, for likelihood rows, the c_s construction), 1/L (v_rcp_f32, about one
+ * FP32 ulp), the scaled terms c/L and the Hessian sums over a segment of
+ * sites; L = eps + c.D, the log and the objective are FP64; each gradient
+ * term is the exact FP64 product of the FP32 c and the FP32 1/L, accumulated
+ * in FP64, so the KKT residual of the convergence test carries FP32 per-site
+ * error.  The QP, the line search and the step are FP64.  The condition of a
+ * converged pair whose estimate from the FP32-summed Hessian reaches 100 is
+ * computed again from an FP64 evaluation at its final point, so a
+ * non-identifiable pair keeps its large (or infinite) condition.  Errors
+ * against the FP64 form are measured, not bounded.  2-bit cohorts take it at
+ * kestrel_cohort_create, others through the option "precision". */
 #define KESTREL_FP64 0
 #define KESTREL_MIXED 1
+
+/* maf argument of the cohort constructors: exactly this value selects the
+ * reference's default 1/(2 n_samples) (wrapper.rs:101); any other value must
+ * lie within 0 to 1 (wrapper.rs:101-106 asserts it), else KESTREL_EINVAL. */
+#define KESTREL_MAF_DEFAULT (-1.0)
 
 /* sqp.rs:16-29 Tuneables */
 typedef struct {
@@ -91,7 +102,7 @@ int kestrel_tune_default(kestrel_tune* tune, uint64_t iterations);
  *   af          NULL: estimate allele frequencies on the device as the
  *               reference does for -f GL/PL (allele/mod.rs:37-134) with
  *               af_iterations SQP iterations and minimum allele frequency maf
- *               (maf < 0: 1/(2 n_samples), wrapper.rs:101).  Non-NULL: n_sites
+ *               (KESTREL_MAF_DEFAULT: 1/(2 n_samples), wrapper.rs:101).  Non-NULL: n_sites
  *               x 2 frequencies (REF, ALT) used as given.
  * Sites where either frequency is 0 are dropped (wrapper.rs:352-410).
  * Returns KESTREL_EDATA if some site has no called sample. */
@@ -99,14 +110,13 @@ int kestrel_cohort_create(kestrel_cohort** out, const uint8_t* packed, uint32_t 
                           size_t row_stride_bytes, const double* af, double maf, uint64_t af_iterations,
                           int precision);
 
-/* Frequencies of all input sites (n_sites x 2) and the kept-site mask. */
 /* A cohort of soft genotype likelihoods for the maximum-likelihood fit.
  * gl: n_sites x n_samples x 3 doubles, site-major as the reference's
  * GenotypeLikelihoods (vcf.rs:321-323), each triple the linear-scale
  * likelihoods of 0/0, 0/1, 1/1 (10^(gl - max gl) or 10^(-pl/10), vcf.rs:368,
  * 407); a triple with a NaN is a missing call.  Replaces wrapper.rs:89-207 up
  * to the pair loop: allele frequencies by the per-site SQP on these rows
- * (allele/mod.rs:37-134, maf < 0 = 1/(2N)) unless af (n_sites x 2) is given,
+ * (allele/mod.rs:37-134, KESTREL_MAF_DEFAULT = 1/(2N)) unless af (n_sites x 2) is given,
  * kept sites as kestrel_cohort_create.  kestrel_fit_pairs / kestrel_evaluate
  * then build c_s per pair and site as mixture_row2 (mixture.rs:96-236).
  * Biallelic sites only. */
@@ -119,7 +129,7 @@ int kestrel_cohort_create_gl(kestrel_cohort** out, const double* gl, uint32_t n_
  * order), site-major, a NaN marking a missing call; NULL when nA = 0.
  * Replaces wrapper.rs:89-130 + split_gl_af (wrapper.rs:237-410): allele
  * frequencies per site by the A-allele SQP (allele/mod.rs:37-134, QP cap
- * A + 1, maf < 0 = 1/(2N)), alleles of frequency 0 removed and every site
+ * A + 1, KESTREL_MAF_DEFAULT = 1/(2N)), alleles of frequency 0 removed and every site
  * re-bucketed by its remaining allele count (sites left with one allele are
  * dropped).  Pair fits then add the 3- and 4-allele sites' c_s from the generic
  * mixture over IIS_3 / IIS_4 (mixture.rs:8-60) after the 2-allele sites. */
@@ -134,6 +144,7 @@ int kestrel_cohort_create_gl_multi(kestrel_cohort** out, uint32_t n_samples, con
 int kestrel_cohort_multi_info(const kestrel_cohort* c, uint32_t part_sites[3], double* af2, double* af3,
                               double* af4);
 
+/* Frequencies of all input sites (n_sites x 2) and the kept-site mask. */
 int kestrel_cohort_allele_freqs(const kestrel_cohort* c, double* af_out, uint8_t* kept_out);
 /* Number of kept (segregating) sites. */
 int kestrel_cohort_n_kept(const kestrel_cohort* c, uint32_t* n_kept);
@@ -238,6 +249,11 @@ int kestrel_last_fit_stats(const kestrel_cohort* c, double* sweep_ms, uint64_t* 
  * running; the launches of the two slot groups overlap) and its number of
  * slot groups (option "streams"). */
 int kestrel_last_fit_busy(const kestrel_cohort* c, double* busy_ms, uint32_t* groups);
+
+/* The last kestrel_fit_pairs of a KESTREL_MIXED cohort: host milliseconds of
+ * the FP64 condition pass (included in the fit's wall time) and the records
+ * whose condition it estimated again (no counterpart in the reference). */
+int kestrel_last_fit_recondition(const kestrel_cohort* c, double* ms, uint64_t* pairs);
 
 /* Serving: the cohort stays device-resident and answers requests of query
  * samples (new 2-bit rows over the cohort's n_sites, fitted with the cohort's

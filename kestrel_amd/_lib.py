@@ -23,6 +23,7 @@ PAIR_NO_SITES = 1
 
 FP64 = 0
 MIXED = 1
+MAF_DEFAULT = -1.0  # KESTREL_MAF_DEFAULT: the reference default 1/(2N)
 
 
 class KestrelError(RuntimeError):
@@ -62,7 +63,7 @@ EXPORTS = (
     "kestrel_serve_start", "kestrel_submit_queries", "kestrel_poll", "kestrel_wait", "kestrel_serve_get_stats",
     "kestrel_serve_stop", "kestrel_last_fit_busy", "kestrel_cohort_create_gl", "kestrel_cohort_create_gl_multi",
     "kestrel_cohort_multi_info", "kestrel_cohort_create_gt_multi", "kestrel_measure_fp32_peak", "kestrel_autotune",
-    "kestrel_measure_lds",
+    "kestrel_measure_lds", "kestrel_last_fit_recondition",
 )
 
 
@@ -115,6 +116,7 @@ def load():
     L.kestrel_set_option.argtypes = [P, ctypes.c_char_p, ctypes.c_int64]
     L.kestrel_last_fit_stats.argtypes = [P, P, P, P]
     L.kestrel_last_fit_busy.argtypes = [P, P, P]
+    L.kestrel_last_fit_recondition.argtypes = [P, P, P]
     L.kestrel_selftest.argtypes = [ctypes.c_uint64, ctypes.c_uint64, P]
     L.kestrel_cohort_create_gt.argtypes = [P, P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_size_t, P]
     L.kestrel_cohort_create_gl.argtypes = [P, P, ctypes.c_uint32, ctypes.c_uint32, P, ctypes.c_double,

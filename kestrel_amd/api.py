@@ -23,6 +23,17 @@ def tune_default(iterations: int = 100) -> Tune:
     return t
 
 
+def _maf_arg(maf: float | None) -> float:
+    """None selects the reference's default 1/(2N) (KESTREL_MAF_DEFAULT);
+    a given value must lie within 0 to 1 (wrapper.rs:101-106)."""
+    if maf is None:
+        return _lib.MAF_DEFAULT
+    maf = float(maf)
+    if not 0.0 <= maf <= 1.0:
+        raise ValueError("Minimum allele frequency must be within 0 to 1")
+    return maf
+
+
 class Cohort:
     """Device-resident cohort of 2-bit hard calls (kestrel_cohort_create)."""
 
@@ -46,7 +57,7 @@ class Cohort:
             afp = np.ascontiguousarray(af, np.float64).reshape(self.n_sites, 2)
         h = ctypes.c_void_p()
         check(L.kestrel_cohort_create(ctypes.byref(h), ptr(packed), self.n_samples, self.n_sites, packed.shape[1],
-                                      ptr(afp), -1.0 if maf is None else float(maf), af_iterations,
+                                      ptr(afp), _maf_arg(maf), af_iterations,
                                       _lib.FP64 if precision == "fp64" else _lib.MIXED))
         self._h = h
         if segment_sites is not None:
@@ -135,7 +146,7 @@ class Cohort:
         afp = None if af is None else np.ascontiguousarray(af, np.float64).reshape(self.n_sites, 2)
         h = ctypes.c_void_p()
         check(L.kestrel_cohort_create_gl(ctypes.byref(h), ptr(g), self.n_samples, self.n_sites, ptr(afp),
-                                         -1.0 if maf is None else float(maf), af_iterations))
+                                         _maf_arg(maf), af_iterations))
         self._h = h
         if segment_sites is not None:
             self.set_option("segment_sites", segment_sites)
@@ -173,7 +184,7 @@ class Cohort:
         h = ctypes.c_void_p()
         check(L.kestrel_cohort_create_gl_multi(ctypes.byref(h), self.n_samples, ptr(arrs[0]), counts[0],
                                                ptr(arrs[1]), counts[1], ptr(arrs[2]), counts[2],
-                                               -1.0 if maf is None else float(maf), af_iterations))
+                                               _maf_arg(maf), af_iterations))
         self._h = h
         if segment_sites is not None:
             self.set_option("segment_sites", segment_sites)
@@ -351,8 +362,10 @@ class Cohort:
         check(_lib.load().kestrel_last_fit_stats(self._h, ctypes.byref(ms), ctypes.byref(sw), ctypes.byref(wall)))
         busy, groups = ctypes.c_double(), ctypes.c_uint32()
         check(_lib.load().kestrel_last_fit_busy(self._h, ctypes.byref(busy), ctypes.byref(groups)))
+        rms, rpairs = ctypes.c_double(), ctypes.c_uint64()
+        check(_lib.load().kestrel_last_fit_recondition(self._h, ctypes.byref(rms), ctypes.byref(rpairs)))
         return {"sweep_ms": ms.value, "sweeps": sw.value, "wall_ms": wall.value, "busy_ms": busy.value,
-                "groups": groups.value}
+                "groups": groups.value, "recondition_ms": rms.value, "recondition_pairs": rpairs.value}
 
 
 def segment_bounds(n_kept: int, segment_sites: int = 4096, chunk: int = 32, pad: int = 256) -> np.ndarray:

@@ -43,6 +43,7 @@ kdev::Tune to_dev(const kestrel_tune& t) {
     d.bls_sufficient_decrease = t.bls_sufficient_decrease;
     d.bls_step_size_reduce = t.bls_step_size_reduce;
     d.epsilon = t.epsilon;
+    d.recond_margin = 0.0;
     return d;
 }
 
@@ -107,6 +108,62 @@ uint64_t default_slots(const kestrel_cohort* c) {
     return std::max<uint64_t>(1, resident / std::max<uint32_t>(1, c->nseg)) * SWEEP_TPB;
 }
 
+}  // namespace
+
+int recondition_records(kestrel_cohort* c, const uint32_t* geno, uint32_t row_words, const uint32_t* seg,
+                        uint32_t nseg, const kestrel_tune& tune, kestrel_output* recs, size_t n,
+                        const std::function<void(size_t, uint32_t&, uint32_t&)>& rows, hipStream_t st,
+                        uint64_t* n_done) {
+    std::vector<size_t> idx;
+    for (size_t i = 0; i < n; i++)
+        if (recs[i].status == KESTREL_PAIR_OK && recs[i].iters < tune.sqp_max_iter &&
+            !(recs[i].cond >= 0.0 && recs[i].cond < RECOND_MIN))
+            idx.push_back(i);
+    if (n_done) *n_done = idx.size();
+    if (idx.empty()) return KESTREL_OK;
+    // up to eight device fills of pairs per launch
+    const size_t batch = std::min<size_t>(idx.size(), std::max<size_t>(8 * default_slots(c), SWEEP_TPB));
+    const uint32_t ns = (uint32_t)((batch + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB);
+    DevBuf<uint32_t> dsx, dsy;
+    DevBuf<int> dact;
+    DevBuf<double> dpt, dpart, dcond;
+    HIPCHK(dsx.alloc(ns));
+    HIPCHK(dsy.alloc(ns));
+    HIPCHK(dact.alloc(ns));
+    HIPCHK(dpt.alloc((size_t)9 * ns));
+    HIPCHK(dpart.alloc((size_t)nseg * NPART * ns));
+    HIPCHK(dcond.alloc(ns));
+    std::vector<uint32_t> hx(ns), hy(ns);
+    std::vector<int> hact(ns);
+    std::vector<double> hpt((size_t)9 * ns), hcond(ns);
+    for (size_t b0 = 0; b0 < idx.size(); b0 += batch) {
+        const size_t m = std::min(batch, idx.size() - b0);
+        std::fill(hx.begin(), hx.end(), 0u);
+        std::fill(hy.begin(), hy.end(), 0u);
+        std::fill(hact.begin(), hact.end(), 0);
+        std::fill(hpt.begin(), hpt.end(), 0.0);
+        for (size_t j = 0; j < m; j++) {
+            const kestrel_output& o = recs[idx[b0 + j]];
+            rows(idx[b0 + j], hx[j], hy[j]);
+            hact[j] = 1;
+            for (int k = 0; k < 9; k++) hpt[(size_t)k * ns + j] = o.jacquard[k];
+        }
+        HIPCHK(hipMemcpyAsync(dsx.p, hx.data(), ns * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dsy.p, hy.data(), ns * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dact.p, hact.data(), ns * sizeof(int), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dpt.p, hpt.data(), hpt.size() * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(klaunch::sweep(geno, row_words, c->af.p, seg, nseg, dsx.p, dsy.p, dpt.p, dact.p, ns, tune.epsilon,
+                              dpart.p, st, c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride, false, false));
+        HIPCHK(klaunch::recondition(dpart.p, nseg, ns, (uint32_t)m, dpt.p, tune.sqp_conv_tol, dcond.p, st));
+        HIPCHK(hipMemcpyAsync(hcond.data(), dcond.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (size_t j = 0; j < m; j++) recs[idx[b0 + j]].cond = hcond[j];
+    }
+    return KESTREL_OK;
+}
+
+namespace {
+
 int open_device(kestrel_cohort* c) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -150,7 +207,7 @@ int kestrel_cohort_create(kestrel_cohort** out, const uint8_t* packed, uint32_t 
     if (row_stride_bytes < (n_sites + 3) / 4) return fail(KESTREL_EINVAL, "row stride too small");
     if (precision != KESTREL_FP64 && precision != KESTREL_MIXED)
         return fail(KESTREL_EINVAL, "unknown precision %d", precision);
-    if (maf < 0) maf = 1.0 / (2.0 * (double)n_samples);  // wrapper.rs:101
+    if (maf == KESTREL_MAF_DEFAULT) maf = 1.0 / (2.0 * (double)n_samples);  // wrapper.rs:101
     if (!(maf >= 0.0 && maf <= 1.0)) return fail(KESTREL_EINVAL, "Minimum allele frequency must be within 0 to 1");
     if (!af && af_iterations == 0) return fail(KESTREL_EINVAL, "iterations must be non-zero");
 
@@ -229,7 +286,7 @@ int kestrel_cohort_create_gl(kestrel_cohort** out, const double* gl, uint32_t n_
     if (!out || !gl) return fail(KESTREL_EINVAL, "null argument");
     *out = nullptr;
     if (n_samples == 0 || n_sites == 0) return fail(KESTREL_EINVAL, "need at least one sample and one site");
-    if (maf < 0) maf = 1.0 / (2.0 * (double)n_samples);  // wrapper.rs:101
+    if (maf == KESTREL_MAF_DEFAULT) maf = 1.0 / (2.0 * (double)n_samples);  // wrapper.rs:101
     if (!(maf >= 0.0 && maf <= 1.0)) return fail(KESTREL_EINVAL, "Minimum allele frequency must be within 0 to 1");
     if (!af && af_iterations == 0) return fail(KESTREL_EINVAL, "iterations must be non-zero");
 
@@ -327,7 +384,7 @@ int kestrel_cohort_create_gl_multi(kestrel_cohort** out, uint32_t n_samples, con
         return fail(KESTREL_EINVAL, "need at least one sample and one site");
     for (int b = 0; b < 3; b++)
         if (nin[b] && !gin[b]) return fail(KESTREL_EINVAL, "null likelihoods for %u sites", nin[b]);
-    if (maf < 0) maf = 1.0 / (2.0 * (double)n_samples);  // wrapper.rs:101
+    if (maf == KESTREL_MAF_DEFAULT) maf = 1.0 / (2.0 * (double)n_samples);  // wrapper.rs:101
     if (!(maf >= 0.0 && maf <= 1.0)) return fail(KESTREL_EINVAL, "Minimum allele frequency must be within 0 to 1");
     if (af_iterations == 0) return fail(KESTREL_EINVAL, "iterations must be non-zero");
 
@@ -549,6 +606,13 @@ int kestrel_last_fit_busy(const kestrel_cohort* c, double* busy_ms, uint32_t* gr
     return KESTREL_OK;
 }
 
+int kestrel_last_fit_recondition(const kestrel_cohort* c, double* ms, uint64_t* pairs) {
+    if (!c) return fail(KESTREL_EINVAL, "null cohort");
+    if (ms) *ms = c->last_recond_ms;
+    if (pairs) *pairs = c->last_recond_pairs;
+    return KESTREL_OK;
+}
+
 int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_t* pairs, size_t n_pairs,
                       uint32_t n_select, kestrel_output* out) {
     if (!c || !tune || !out) return fail(KESTREL_EINVAL, "null argument");
@@ -671,7 +735,8 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     HIPCHK(hipMemsetAsync(dnext.p, 0, sizeof(unsigned long long), st));
     const int live0 = (int)ns * ngroups;
     HIPCHK(hipMemcpyAsync(dlive.p, &live0, sizeof(int), hipMemcpyHostToDevice, st));
-    const kdev::Tune dt = to_dev(*tune);
+    kdev::Tune dt = to_dev(*tune);
+    if (c->mixed) dt.recond_margin = RECOND_MARGIN;
 
     // sweeps and steps are queued BATCH at a time between host synchronisations;
     // each step adds the slots still live after it to hist[i], so the host
@@ -870,6 +935,20 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     c->last_end_ms = end_ms;
     HIPCHK(hipMemcpyAsync(out, dout.p, n_pairs * sizeof(kestrel_output), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    c->last_recond_ms = 0;
+    c->last_recond_pairs = 0;
+    if (c->mixed) {
+        const auto r0 = std::chrono::steady_clock::now();
+        const int rs = recondition_records(
+            c, c->geno.p, c->row_words, c->seg.p, c->nseg, *tune, out, n_pairs,
+            [&](size_t i, uint32_t& x, uint32_t& y) {
+                x = out[i].x;
+                y = out[i].y;
+            },
+            st, &c->last_recond_pairs);
+        if (rs) return rs;
+        c->last_recond_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - r0).count();
+    }
     c->last_sweep_ms = sweep_ms;
     c->last_sweeps = sweeps;
     c->last_wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -335,7 +335,7 @@ int main(int argc, char** argv) {
                 warn("No convergence for Jacquard coefficients, max iterations " + std::to_string(a.iterations) +
                      " exceeded");
     } else {
-        const double maf = a.have_maf ? a.maf : -1.0;
+        const double maf = a.have_maf ? a.maf : KESTREL_MAF_DEFAULT;
         if (a.have_maf && !(maf >= 0.0 && maf <= 1.0)) {
             fprintf(stderr, "Minimum allele frequency must be within 0 to 1\n");
             return 1;

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -93,6 +94,23 @@ struct GtPart {
     DevBuf<double> af;
     DevBuf<double> mtab;
 };
+// The mixed-precision form's condition estimates: a converged record whose
+// estimate from the FP32-summed Hessian is not below this is estimated again
+// from an FP64 evaluation at its final point.
+constexpr double RECOND_MIN = 100.0;
+// ... and so is one whose strict-complementarity decision is marginal (the
+// step reports -condition): within this distance of the test's threshold.
+constexpr double RECOND_MARGIN = 1e-6;
+
+struct kestrel_cohort;
+// Recomputes the condition of the records that need it (RECOND_MIN) from FP64
+// sweeps over geno (row_words per row, segments seg[0..nseg]); rows(i, x, y)
+// gives the geno rows of record i.  Ordered on st, blocks until done.
+int recondition_records(kestrel_cohort* c, const uint32_t* geno, uint32_t row_words, const uint32_t* seg,
+                        uint32_t nseg, const kestrel_tune& tune, kestrel_output* recs, size_t n,
+                        const std::function<void(size_t, uint32_t&, uint32_t&)>& rows, hipStream_t st,
+                        uint64_t* n_done = nullptr);
+
 // Stops the serving engine of a cohort (if any) and frees its buffers.
 void serve_release(ServeState* s);
 
@@ -130,6 +148,8 @@ struct kestrel_cohort {
     int last_groups = 1;
     uint32_t last_slots = 0;                        // pairs in flight per group in the last fit
     double last_exhaust_ms = 0, last_end_ms = 0;    // device ms until the first slot went idle / the last sweep
+    double last_recond_ms = 0;                      // mixed form: host ms of the FP64 condition pass
+    uint64_t last_recond_pairs = 0;                 // mixed form: records whose condition it recomputed
 
     // tri/tetra-allelic cohorts (kestrel_cohort_create_gl_multi)
     std::vector<std::unique_ptr<SitePart>> parts;

@@ -1732,6 +1732,9 @@ __device__ __forceinline__ void step_slot(SlotState* __restrict__ st, uint32_t* 
                 } else if (converged<9>(xr, g, tune.sqp_conv_tol)) {
                     o.iters = s.k;
                     o.cond = condition_reg<9>(xr, g, h, tune.sqp_conv_tol);
+                    if (tune.recond_margin > 0.0 &&
+                        complementarity_marginal<9>(xr, g, tune.sqp_conv_tol, tune.recond_margin))
+                        o.cond = -o.cond;
                     done = true;
                 } else {
                     double c[9], yq[9];
@@ -1930,6 +1933,37 @@ __global__ __launch_bounds__(64) void finalize_eval(const double* __restrict__ p
             h[(size_t)i * 81 + b * 9 + a] = v;
         }
     nsites[i] = (uint32_t)n;
+}
+
+// Condition estimates from FP64 sweep partials at given final points (the
+// mixed-precision form's converged pairs whose estimate from the FP32-summed
+// Hessian reached its resolution): fold, normalise as the step does
+// (grad_hess.rs:20-38), then estimate_condition_number (sqp.rs:312-391) at x.
+__global__ __launch_bounds__(64) void recondition_kernel(const double* __restrict__ partial, uint32_t nseg,
+                                                          uint32_t nslots, uint32_t npts,
+                                                          const double* __restrict__ pt, double tol,
+                                                          double* __restrict__ cond) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    double acc[NPART];
+    fold(partial, nseg, nslots, i, acc);
+    const double n = acc[55];
+    double x[9], g[9], h[9][9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        x[k] = pt[(size_t)k * nslots + i];
+        g[k] = -acc[1 + k] / n;
+    }
+    int id = 0;
+#pragma unroll
+    for (int a = 0; a < 9; a++)
+#pragma unroll
+        for (int b = a; b < 9; b++, id++) {
+            const double v = acc[10 + id] / n;
+            h[a][b] = v;
+            h[b][a] = v;
+        }
+    cond[i] = condition_reg<9>(x, g, h, tol);
 }
 
 // Self-test of the sweep's arithmetic shortcuts: recip_rn against IEEE
@@ -2660,6 +2694,13 @@ hipError_t finalize(const double* partial, uint32_t nseg, uint32_t nslots, uint3
                     double* h, uint32_t* nsites, hipStream_t st) {
     hipLaunchKernelGGL(finalize_eval, dim3((npts + 63) / 64), dim3(64), 0, st, partial, nseg, nslots, npts, f, g, h,
                        nsites);
+    return hipGetLastError();
+}
+
+hipError_t recondition(const double* partial, uint32_t nseg, uint32_t nslots, uint32_t npts, const double* pt,
+                       double tol, double* cond, hipStream_t st) {
+    hipLaunchKernelGGL(recondition_kernel, dim3((npts + 63) / 64), dim3(64), 0, st, partial, nseg, nslots, npts, pt,
+                       tol, cond);
     return hipGetLastError();
 }
 

@@ -122,4 +122,7 @@ hipError_t cls_pairs(const uint32_t* geno, uint32_t row_words, const double* af,
                      double* csum, kestrel_output* out, hipStream_t st);
 hipError_t finalize(const double* partial, uint32_t nseg, uint32_t nslots, uint32_t npts, double* f, double* g,
                     double* h, uint32_t* nsites, hipStream_t st);
+// condition estimates at the points pt (SoA, nslots apart) from FP64 partials
+hipError_t recondition(const double* partial, uint32_t nseg, uint32_t nslots, uint32_t npts, const double* pt,
+                       double tol, double* cond, hipStream_t st);
 }  // namespace klaunch

@@ -254,6 +254,17 @@ int finish(ServeState* S, Request* r) {
                               hipMemcpyDeviceToHost, S->stream));
         HIPCHK(hipStreamSynchronize(S->stream));
     }
+    if (S->c->mixed) {  // condition estimates from FP64 where the FP32-summed Hessian cannot resolve them
+        const int rc = recondition_records(
+            S->c, S->geno.p, S->row_words, S->seg.p, S->nseg, S->tune, r->out, r->npairs,
+            [&](size_t i, uint32_t& x, uint32_t& y) {
+                const uint64_t j = i % r->n_sub;
+                x = r->rows[i / r->n_sub];
+                y = r->subset.empty() ? (uint32_t)j : r->subset[j];
+            },
+            S->stream);
+        if (rc) return rc;
+    }
     for (uint64_t i = 0; i < r->npairs; i++) r->out[i].x = (uint32_t)(i / r->n_sub);
     return KESTREL_OK;
 }
@@ -434,6 +445,7 @@ int kestrel_serve_start(kestrel_cohort* c, const kestrel_tune* tune, uint32_t ma
     S->c = c;
     S->tune = *tune;
     S->dt = to_dev(*tune);
+    if (c->mixed) S->dt.recond_margin = RECOND_MARGIN;
     S->qmax = max_query_rows;
     S->n_cohort = c->n_samples;
     S->row_words = c->row_words;

@@ -24,6 +24,11 @@ struct Tune {
     double bls_sufficient_decrease;
     double bls_step_size_reduce;
     double epsilon;
+    // > 0 (mixed-precision form): a converged pair with a held variable whose
+    // strict-complementarity test g_i >= lambda + tol is decided within this
+    // margin reports -condition, so that the host estimates it again from an
+    // FP64 evaluation (the FP32 reciprocal moves g by up to ~1e-7)
+    double recond_margin;
 };
 
 // log/scalar.rs:17-70 (identical to log/avx512.rs:15-79 on positive normals):
@@ -584,6 +589,20 @@ __device__ inline uint64_t solve_qp_reg(const double (&Q)[N][N], const double (&
         iter++;
     }
     return iter;
+}
+
+// Whether the strict-complementarity decision of the condition estimate
+// (sqp.rs:324-331) is within `margin` of flipping for some held variable.
+template <int N>
+__device__ inline bool complementarity_marginal(const double (&x)[N], const double (&g)[N], double tol,
+                                                double margin) {
+    double lambda = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; i++) lambda = fma(x[i], g[i], lambda);
+    bool m = false;
+#pragma unroll
+    for (int i = 0; i < N; i++) m = m || (x[i] == 0.0 && fabs(g[i] - (lambda + tol)) < margin);
+    return m;
 }
 
 // sqp.rs:312-391, register-resident (see above).

@@ -1,0 +1,222 @@
+"""The workloads C1, C4 and C5 at their full sizes on the GPU, through the C ABI,
+against the oracle.
+
+  C1  64 samples x 10,000 sites, every pair: bit-exact against the oracle in
+      the device's segmented order, within the tolerance below of the
+      reference's 8-lane order.
+  C4  2,000 x 100,000 sites (MAF >= 0.005), all 2,001,000 records in the FP64
+      form and in the FP32-mixed form: the mixed form against the FP64 form on
+      every pair both bring to convergence with log10 condition below 12
+      (max |dDelta_k|, kinship, relative objective, log10 condition); pairs the
+      FP64 form puts at or above log10 condition 12 keep a large condition
+      unless the mixed form's point has a lower objective and a small FP64
+      estimate (checked through kestrel_evaluate and the oracle); a seeded
+      sample of FP64 records bit-exact against the oracle and of mixed
+      records within the tolerance of the oracle's reference order.
+  C5  a device-resident 10,000 x 100,000 cohort serving a 256-query request
+      against the whole cohort and a subset request submitted while it runs:
+      the subset request's records are bit-identical alone and in the mix, a
+      seeded sample of both requests' records is bit-exact against the oracle;
+      a mixed-precision engine on the same cohort answers the subset request
+      within the tolerance of the FP64 records.
+
+Tolerances (reference's order, and mixed form vs FP64): max |dDelta_k| <= 1e-6,
+|d kinship| <= 1e-6, log10 condition within 0.1 on pairs with log10
+condition < 12; relative objective <= 1e-9 (orders) and <= 1e-7 (mixed form,
+whose per-site terms are FP32).
+"""
+import numpy as np
+import pytest
+
+from kestrel_amd import Cohort, kinship, segment_bounds, synth
+
+pytestmark = pytest.mark.gpu
+
+TOL_DELTA = 1e-6
+TOL_KIN = 1e-6
+TOL_OBJ_REL = 1e-9
+TOL_OBJ_REL_MIXED = 1e-7
+TOL_LOG10_COND = 0.1
+EXPECT_KIN = {"PO": 0.25, "FS": 0.25, "HS": 0.125, "FC": 0.0625}
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def same(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return (a == b) | (np.isnan(a) & np.isnan(b))
+
+
+def assert_bitexact(out, exp, keys=("jacquard", "obj", "iters", "n_sites", "cond")):
+    for k in keys:
+        ok = same(out[k], exp[k]).reshape(len(out), -1).all(1)
+        assert ok.all(), f"{k}: {int((~ok).sum())} of {len(out)} records differ"
+
+
+def seg_for(n_kept, v):
+    return np.minimum(segment_bounds(n_kept), v).astype(np.uint32)
+
+
+def compare(out, ref, tol_obj):
+    """Errors of out against ref on the pairs ref puts below log10 condition 12
+    and both bring to convergence; returns the worst of each and the count."""
+    lc, lo = np.log10(ref["cond"]), np.log10(out["cond"])
+    well = np.isfinite(lc) & (lc < 12) & np.isfinite(lo)
+    lc, lo = np.where(well, lc, 0.0), np.where(well, lo, 0.0)
+    d = np.abs(out["jacquard"] - ref["jacquard"]).max(1)[well]
+    k = np.abs(kinship(out["jacquard"]) - kinship(ref["jacquard"]))[well]
+    r = (np.abs(out["obj"] - ref["obj"]) / np.abs(ref["obj"]))[well]
+    c = np.abs(lo - lc)[well]
+    worst = dict(pairs=int(well.sum()), delta=float(d.max()), kinship=float(k.max()), obj_rel=float(r.max()),
+                 log10_cond=float(c.max()))
+    assert worst["delta"] <= TOL_DELTA, worst
+    assert worst["kinship"] <= TOL_KIN, worst
+    assert worst["obj_rel"] <= tol_obj, worst
+    assert worst["log10_cond"] <= TOL_LOG10_COND, worst
+    return worst
+
+
+def tri_pos(n, x, y):
+    """Record index of pair (x <= y) in the row-major upper triangle of n samples."""
+    return x * n - x * (x - 1) // 2 + (y - x)
+
+
+def test_c1_all_pairs(O):
+    say("case: C1 64 x 10000, all 2,080 records")
+    c = synth.config("c1")
+    with Cohort(c.codes) as co:
+        af, kept = co.allele_freqs()
+        assert np.array_equal(af, O.allele_freqs(c.codes))
+        out = co.fit_pairs()
+    n = c.codes.shape[0]
+    assert len(out) == n * (n + 1) // 2 and (out["status"] == 0).all()
+    kc, ka = c.codes[:, kept], af[kept]
+    exp = O.fit_pairs(kc, ka, order=O.ORDER_SEG, seg=seg_for(int(kept.sum()), kc.shape[1]))
+    assert_bitexact(out, exp, keys=("x", "y", "jacquard", "obj", "iters", "n_sites", "cond"))
+    worst = compare(out, O.fit_pairs(kc, ka, order=O.ORDER_REF), TOL_OBJ_REL)
+    say(f"C1 vs reference order: {worst}")
+    for x, y, kind in c.pairs:
+        a, b = min(x, y), max(x, y)
+        k = kinship(out["jacquard"][tri_pos(n, a, b):tri_pos(n, a, b) + 1])[0]
+        assert abs(k - EXPECT_KIN[kind]) < 0.06, (kind, k)
+
+
+def test_c4_fp64_and_mixed_forms(O):
+    say("case: C4 2000 x 100000 (MAF >= 0.005), all 2,001,000 records, FP64 and mixed forms")
+    c = synth.config("c4")
+    n = c.codes.shape[0]
+    co64 = Cohort(c.codes)
+    af, kept = co64.allele_freqs()
+    out64 = co64.fit_pairs()
+    t64 = co64.last_fit_stats()["wall_ms"]
+    with Cohort(c.codes, precision="mixed") as co:
+        afm, keptm = co.allele_freqs()
+        assert np.array_equal(af, afm) and np.array_equal(kept, keptm)
+        outm = co.fit_pairs()
+        tm = co.last_fit_stats()["wall_ms"]
+    say(f"C4 fits: FP64 {t64:.0f} ms, mixed {tm:.0f} ms")
+    assert len(out64) == n * (n + 1) // 2
+    for o in (out64, outm):
+        assert (o["status"] == 0).all()
+        d = o["jacquard"]
+        assert (d >= 0).all() and np.abs(d.sum(1) - 1).max() < 1e-12
+    assert np.array_equal(out64["n_sites"], outm["n_sites"])
+    worst = compare(outm, out64, TOL_OBJ_REL_MIXED)
+    say(f"C4 mixed vs FP64 form: {worst}; iterations {outm['iters'].mean():.3f} vs {out64['iters'].mean():.3f}")
+    # Pairs the FP64 form puts at or above log10 condition 12 keep a large
+    # condition in the mixed form, except where the mixed form stops at a
+    # point (within ~1e-7 in Delta) with a lower objective at which the FP64
+    # estimate itself is small: the strict-complementarity test of the
+    # estimate (sqp.rs:324-331, tol 1e-8) decides differently there.  The
+    # mixed form's condition of such a pair is the FP64 estimate at its point.
+    lc, lm = np.log10(out64["cond"]), np.log10(outm["cond"])
+    conv = out64["iters"] < 100
+    high = conv & ~(lc < 12)
+    flip = np.nonzero(high & (lm < 10))[0]
+    say(f"C4 pairs at or above log10 condition 12 (FP64 form): {int(high.sum())}, "
+        f"below 10 in the mixed form: {len(flip)}")
+    assert len(flip) <= 0.01 * high.sum()
+    if len(flip):
+        assert ((outm["obj"][flip] - out64["obj"][flip]) / np.abs(out64["obj"][flip]) <= 1e-9).all()
+        sel = flip[:24]
+        pairs = np.stack([outm["x"][sel], outm["y"][sel]], 1)
+        f, g, h, _ = co64.evaluate(pairs, outm["jacquard"][sel])
+        for j, i in enumerate(sel):
+            k = O.condition(outm["jacquard"][i], g[j], h[j])
+            assert abs(k - outm["cond"][i]) <= 1e-9 * k, (i, k, outm["cond"][i])
+    co64.close()
+    # seeded samples against the oracle
+    rng = np.random.default_rng(0xC4)
+    idx = np.unique(rng.integers(0, len(out64), 150))
+    pairs = np.stack([out64["x"][idx], out64["y"][idx]], 1)
+    kc, ka = c.codes[:, kept], af[kept]
+    exp = O.fit_pairs(kc, ka, pairs=pairs, order=O.ORDER_SEG, seg=seg_for(int(kept.sum()), kc.shape[1]))
+    assert_bitexact(out64[idx], exp)
+    ref = O.fit_pairs(kc, ka, pairs=pairs, order=O.ORDER_REF)
+    say(f"C4 FP64 sample vs reference order: {compare(out64[idx], ref, TOL_OBJ_REL)}")
+    say(f"C4 mixed sample vs reference order: {compare(outm[idx], ref, TOL_OBJ_REL_MIXED)}")
+    for x, y, kind in c.pairs:
+        a, b = min(x, y), max(x, y)
+        for o in (out64, outm):
+            k = kinship(o["jacquard"][tri_pos(n, a, b):tri_pos(n, a, b) + 1])[0]
+            assert abs(k - EXPECT_KIN[kind]) < 0.05, (kind, k)  # inbred founders (F = 0.125) raise some
+
+
+@pytest.fixture(scope="module")
+def c5():
+    n_pool = 256
+    kw = dict(synth.CONFIGS["c5"])
+    kw["n_samples"] += n_pool
+    c = synth.generate(**kw)
+    n = c.codes.shape[0] - n_pool
+    return c.codes[:n], c.codes[n:]
+
+
+def oracle_records(O, cohort, queries, kept, af, n_kept, recs, qrow, crow):
+    """Oracle (segmented order) of request records: record i pairs query
+    qrow[i] with cohort sample crow[i]."""
+    qs, cs = np.unique(qrow), np.unique(crow)
+    codes = np.vstack([queries[qs][:, kept], cohort[cs][:, kept]])
+    pairs = np.stack([np.searchsorted(qs, qrow), len(qs) + np.searchsorted(cs, crow)], 1)
+    return O.fit_pairs(codes, af[kept], pairs=pairs, order=O.ORDER_SEG, seg=seg_for(n_kept, codes.shape[1]))
+
+
+def test_c5_serving_full_cohort(O, c5):
+    cohort, queries = c5
+    n, v = cohort.shape
+    say(f"case: C5 serving, cohort {n} x {v}, 256 queries vs all + 48 queries vs 3,000 samples")
+    packed = synth.pack2(cohort)
+    rng = np.random.default_rng(0xC5C5)
+    sub = np.sort(rng.choice(n, 3000, replace=False))
+    with Cohort(packed=packed, n_sites=v) as co:
+        af, kept = co.allele_freqs()
+        n_kept = co.n_kept
+        co.serve_start(max_query_rows=512, max_pairs=1 << 22)
+        alone, _ = co.wait(co.submit(queries[200:248], sub))
+        big = co.submit(queries)
+        mixed_in = co.submit(queries[200:248], sub)
+        mix, _ = co.wait(mixed_in)
+        full, lat = co.wait(big)
+        st = co.serve_stats()
+    say(f"C5 full request: {len(full)} records in {lat / 1e3:.1f} s; engine {st}")
+    assert len(full) == 256 * n and len(alone) == 48 * 3000
+    assert (full["status"] == 0).all() and (alone["status"] == 0).all()
+    assert_bitexact(mix, alone, keys=("x", "y", "jacquard", "obj", "iters", "n_sites", "cond"))
+    d = full["jacquard"]
+    assert (d >= 0).all() and np.abs(d.sum(1) - 1).max() < 1e-12
+    # a seeded sample of each request against the oracle
+    i_full = np.unique(rng.integers(0, len(full), 60))
+    exp = oracle_records(O, cohort, queries, kept, af, n_kept, full[i_full], full["x"][i_full], full["y"][i_full])
+    assert_bitexact(full[i_full], exp)
+    i_sub = np.unique(rng.integers(0, len(alone), 40))
+    exp = oracle_records(O, cohort, queries, kept, af, n_kept, alone[i_sub], 200 + alone["x"][i_sub],
+                         alone["y"][i_sub])
+    assert_bitexact(alone[i_sub], exp)
+    # the mixed-precision engine on the same cohort (the workload's form)
+    with Cohort(packed=packed, n_sites=v, precision="mixed") as cm:
+        cm.serve_start(max_query_rows=64, max_pairs=1 << 18)
+        outm, _ = cm.wait(cm.submit(queries[200:248], sub))
+    assert np.array_equal(outm["n_sites"], alone["n_sites"])
+    say(f"C5 mixed engine vs FP64 records: {compare(outm, alone, TOL_OBJ_REL_MIXED)}")
