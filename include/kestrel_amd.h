@@ -161,6 +161,22 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
 int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* points, size_t n, double eps,
                      double* f, double* g, double* h, uint32_t* n_sites);
 
+/* kestrel_evaluate for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc);
+ * no counterpart in the reference.  pairs (n x 2), points (n x 9), f (n),
+ * g (n x 9), h (n x 81) and n_sites (n) are DEVICE pointers; results are
+ * kestrel_evaluate's bit for bit.  The sweep's slot arrays and segment
+ * partials live in a fixed 64 MiB array of device memory that each GPU the
+ * library runs on holds until the process ends, so a call takes at most
+ * 64 MiB / (84 + 448 x segments) bytes' worth of points (segments: one per
+ * 4,096 kept sites by default; about 5,900 points at 100,000 sites) and
+ * returns KESTREL_EINVAL beyond that.  The call allocates nothing, runs on
+ * the cohort's stream and returns when the results are written.  Every call
+ * on one GPU shares the array: calls that overlap on one GPU (from several
+ * threads or cohorts) race. */
+int kestrel_evaluate_noalloc(kestrel_cohort* c, const uint32_t* pairs, const double* points, size_t n, double eps,
+                             double* f, double* g, double* h, uint32_t* n_sites);
+
 /* The CLS / Brier estimator of the CLI default -f GT.
  *   kestrel_cohort_create_gt  replaces wrapper.rs:22-87 up to the pair loop:
  *                             sites with a missing call in any sample are

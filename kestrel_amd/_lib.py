@@ -63,7 +63,7 @@ EXPORTS = (
     "kestrel_serve_start", "kestrel_submit_queries", "kestrel_poll", "kestrel_wait", "kestrel_serve_get_stats",
     "kestrel_serve_stop", "kestrel_last_fit_busy", "kestrel_cohort_create_gl", "kestrel_cohort_create_gl_multi",
     "kestrel_cohort_multi_info", "kestrel_cohort_create_gt_multi", "kestrel_measure_fp32_peak", "kestrel_autotune",
-    "kestrel_measure_lds", "kestrel_last_fit_recondition",
+    "kestrel_measure_lds", "kestrel_last_fit_recondition", "kestrel_evaluate_noalloc",
 )
 
 
@@ -113,6 +113,7 @@ def load():
     L.kestrel_cohort_n_kept.argtypes = [P, P]
     L.kestrel_fit_pairs.argtypes = [P, P, P, ctypes.c_size_t, ctypes.c_uint32, P]
     L.kestrel_evaluate.argtypes = [P, P, P, ctypes.c_size_t, ctypes.c_double, P, P, P, P]
+    L.kestrel_evaluate_noalloc.argtypes = [P, P, P, ctypes.c_size_t, ctypes.c_double, P, P, P, P]
     L.kestrel_set_option.argtypes = [P, ctypes.c_char_p, ctypes.c_int64]
     L.kestrel_last_fit_stats.argtypes = [P, P, P, P]
     L.kestrel_last_fit_busy.argtypes = [P, P, P]
