@@ -8,9 +8,15 @@ off-diagonal ones).  One step = one full fit from the device-resident 2-bit
 cohort + allele frequencies to Output records in host memory.
 
 Prints one JSON line with the throughput, the roofline of the dominant kernel
-(the sweep: fused objective + gradient + Hessian passes, VALU FP64 bound) and a
-CPU baseline (the oracle's restatement of the reference CPU path, timed on a
-bounded sample of the same workload on this host's cores).  Other modes:
+(the sweep: fused objective + gradient + Hessian passes, VALU FP64 bound), a
+CPU baseline (the oracle's restatement of the reference CPU path in the
+reference's loop shape, timed on a bounded sample of the same workload on this
+host's cores), every output's error against that reference-order fit
+("errors"), the FP32-mixed form on the same workload ("mixed": throughput,
+ratio to the FP64 form, errors) and the serving workload ("serving": a
+device-resident 10,000 x 100,000 cohort, saturated pair-fits/s, p50/p99
+request latency at 0.7 load over 100 Poisson requests, 1-query latency).
+Other modes:
 --serving (device-resident cohort answering query requests, p50/p99),
 --soft (likelihood input), --cli (the kestrel binary end to end on the cohort
 written as a VCF), --autotune / --graph / --batch (launch shape).
@@ -55,12 +61,26 @@ def cpu_threads() -> int:
     return max(1, n)
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (as lscpu reports it)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(codes, af, budget_s: float, gl=None, buckets=None):
-    """Oracle in the reference's order (8-lane AVX-512 loops, per-pair c_s buffer,
-    work stealing over pairs) on a seeded random sample of off-diagonal pairs;
-    gl (N, V, 3) likelihood rows instead of codes when given; buckets (the
-    2-, 3-, 4-allele likelihood rows, site-major) for a multi-allelic cohort,
-    whose frequencies and re-bucketing the oracle computes first, untimed."""
+    """The oracle in the reference's order on a seeded random sample of
+    off-diagonal pairs, work stealing over pairs on this host's threads.  Hard
+    calls run in the reference's own AVX-512 loop shape (ORDER_FAST: c_s rows
+    per 8-site block, getexp/getmant log, three 15-accumulator Hessian passes
+    per tile; bit-identical to the 8-lane loops); likelihood rows and
+    multi-allelic cohorts in the 8-lane loops (buckets: the 2-, 3-, 4-allele
+    likelihood rows, site-major; frequencies and re-bucketing untimed).
+    Returns (baseline dict, the sample's pairs, its records)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
@@ -71,12 +91,14 @@ def cpu_baseline(codes, af, budget_s: float, gl=None, buckets=None):
         n = buckets[0].shape[1]
     else:
         n = codes.shape[0] if gl is None else gl.shape[0]
+    fast = parts is None and gl is None and O.lib().ko_has_avx512()
+    order = O.ORDER_FAST if fast else O.ORDER_REF
 
     def fit(pairs):
         if parts is not None:
             return O.multi_fit_pairs_threads(parts, pairs, nthreads=threads)
         if gl is None:
-            return O.fit_pairs(codes, af, pairs=pairs, order=O.ORDER_REF, nthreads=threads)
+            return O.fit_pairs(codes, af, pairs=pairs, order=order, nthreads=threads)
         return O.fit_pairs_gl(gl, af, pairs=pairs, order=O.ORDER_REF, nthreads=threads)
     rng = np.random.default_rng(12345)
 
@@ -91,33 +113,67 @@ def cpu_baseline(codes, af, budget_s: float, gl=None, buckets=None):
     t = time.perf_counter()
     fit(probe)
     dt = time.perf_counter() - t
-    m = int(max(2 * threads, min(200_000, budget_s / max(dt, 1e-3) * len(probe))))
+    m = int(max(2 * threads, min(400_000, budget_s / max(dt, 1e-3) * len(probe))))
     pairs = sample(m)
     t = time.perf_counter()
-    fit(pairs)
+    recs = fit(pairs)
     dt = time.perf_counter() - t
-    return {"value": m / dt, "unit": "pair-fits/s", "cores": threads, "kind": "port",
-            "isa": O.isa(), "sample": f"{m} random off-diagonal pairs of the same cohort, {dt:.1f} s"}
+    shape = ("the reference's AVX-512 loop shape (c_s rows per 8-site block, getexp/getmant log, three "
+             "15-accumulator Hessian passes per 32-block tile), bit-identical to the oracle's 8-lane loops"
+             if fast else "the oracle's 8-lane loops (one accumulator chain per Hessian entry)")
+    return ({"value": m / dt, "unit": "pair-fits/s", "cores": threads, "kind": "port", "isa": O.isa(),
+             "cpu_model": cpu_model(), "loop_shape": shape,
+             "sample": f"{m} random off-diagonal pairs of the same cohort, {dt:.1f} s"}, pairs, recs)
+
+
+def record_errors(out, ref, tol_log10_cond=12.0):
+    """Per-output errors of records `out` against `ref` on the pairs ref puts
+    below log10 condition 12 (both converged): max |dDelta_k|, |d kinship|,
+    relative objective, |d log10 condition|."""
+    from kestrel_amd import kinship
+    lc, lo = np.log10(ref["cond"]), np.log10(out["cond"])
+    well = np.isfinite(lc) & (lc < tol_log10_cond) & np.isfinite(lo)
+    if not well.any():
+        return {"pairs": int(len(out)), "well_conditioned": 0}
+    d = np.abs(out["jacquard"] - ref["jacquard"]).max(1)[well]
+    return {"pairs": int(len(out)), "well_conditioned": int(well.sum()),
+            "max_abs_delta": float(d.max()),
+            "max_abs_kinship": float(np.abs(kinship(out["jacquard"]) - kinship(ref["jacquard"]))[well].max()),
+            "max_rel_objective": float((np.abs(out["obj"] - ref["obj"]) / np.abs(ref["obj"]))[well].max()),
+            "max_abs_log10_cond": float(np.abs(lo[well] - lc[well]).max()),
+            "same_iterations": float((out["iters"] == ref["iters"]).mean())}
+
+
+def tri_index(n, pairs):
+    """Record positions of pairs (x <= y) in the row-major upper triangle with the diagonal."""
+    x = pairs[:, 0].astype(np.int64)
+    y = pairs[:, 1].astype(np.int64)
+    return x * n - x * (x - 1) // 2 + (y - x)
 
 
 def percentile(a, q):
     return float(np.percentile(np.asarray(a, np.float64), q)) if len(a) else None
 
 
-def serving(args):
-    """Serving workload: a device-resident cohort (default 10,000 x 100,000)
-    answers requests of 1-256 new query samples against the whole cohort or a
-    random subset of 500-5,000 samples (even odds).  Phase 1 submits every
-    request at once (saturated: pair-fits/s capacity); phase 2 submits the same
-    number of fresh requests at Poisson times with a mean rate of `load` x the
-    phase-1 capacity (request latency p50/p99)."""
+def serving_run(config="c5", precision="mixed", policy="srpt", slots=None, sat_requests=12, requests=100,
+                load=0.7, single_probes=20):
+    """Serving workload: a device-resident cohort (default 10,000 x 100,000,
+    the FP32-mixed form) answers requests of 1-256 new query samples against
+    the whole cohort or a random subset of 500-5,000 samples (even odds).
+    Phase 1 submits sat_requests requests at once (saturated: pair-fits/s
+    capacity); phase 2 submits `requests` fresh requests at Poisson times with
+    a mean rate of `load` x the phase-1 capacity (request latency p50/p99),
+    with `single_probes` 1-query requests against the whole cohort at uniform
+    times in the same window (their latency is reported apart).  Returns the
+    summary dict."""
+    import queue
     import threading
 
     from kestrel_amd import Cohort, synth
 
     t = time.perf_counter()
     n_query_pool = 256
-    kw = dict(synth.CONFIGS[args.config])
+    kw = dict(synth.CONFIGS[config])
     kw["n_samples"] += n_query_pool
     c = synth.generate(**kw)
     n = c.codes.shape[0] - n_query_pool
@@ -125,12 +181,12 @@ def serving(args):
     packed = synth.pack2(c.codes)
     cohort_packed, pool = packed[:n], packed[n:]
     del c
-    log(f"cohort {args.config}: {n} x {v} (+{n_query_pool} query samples) generated in {time.perf_counter() - t:.1f}s")
+    log(f"cohort {config}: {n} x {v} (+{n_query_pool} query samples) generated in {time.perf_counter() - t:.1f}s")
     t = time.perf_counter()
-    co = Cohort(packed=cohort_packed, n_sites=v, precision=args.precision)
-    if args.slots:
-        co.set_option("slots", args.slots)
-    co.set_option("serve_policy", {"rr": 0, "srpt": 1}[args.policy])
+    co = Cohort(packed=cohort_packed, n_sites=v, precision=precision)
+    if slots:
+        co.set_option("slots", slots)
+    co.set_option("serve_policy", {"rr": 0, "srpt": 1}[policy])
     co.serve_start(max_query_rows=1024, max_pairs=1 << 24)
     st0 = co.serve_stats()
     log(f"device cohort, allele frequencies and engine in {time.perf_counter() - t:.2f}s, kept {co.n_kept} sites, "
@@ -143,74 +199,101 @@ def serving(args):
         sub = None if rng.random() < 0.5 else np.sort(rng.choice(n, int(rng.integers(500, 5001)), replace=False))
         return pool[q0:q0 + nq], sub
 
-    def run(reqs, arrivals):
-        """Submit reqs at arrival offsets (s); returns (wall s, pairs, latencies ms)."""
-        lat = []
-        done = threading.Event()
+    def run(reqs, arrivals, kinds):
+        """Submit reqs at arrival offsets (s), in arrival order; returns (wall s,
+        pairs, latencies ms by kind)."""
+        lat = {}
+        ids_q = queue.Queue()
 
         def waiter():
-            for i in iter(ids_q.get, None):
+            done = 0
+            for i, kind in iter(ids_q.get, None):
                 out, ms = co.wait(i)
-                lat.append(ms)
+                lat.setdefault(kind, []).append(ms)
                 del out
-            done.set()
+                done += 1
+                if done % 10 == 0:  # progress of a phase that runs for minutes
+                    log(f"  {done} of {len(reqs)} requests done at {time.perf_counter() - t0:.1f}s")
 
-        import queue
-        ids_q = queue.Queue()
+        t0 = time.perf_counter()
         th = threading.Thread(target=waiter)
         th.start()
-        t0 = time.perf_counter()
         pairs = 0
-        for (q, sub), at in zip(reqs, arrivals):
+        for j in np.argsort(arrivals, kind="stable"):
+            (q, sub), at = reqs[j], arrivals[j]
             d = t0 + at - time.perf_counter()
             if d > 0:
                 time.sleep(d)
-            ids_q.put(co.submit(query_packed=q, subset=sub))
+            ids_q.put((co.submit(query_packed=q, subset=sub), kinds[j]))
             pairs += q.shape[0] * (n if sub is None else len(sub))
         ids_q.put(None)
-        done.wait()
         th.join()
         return time.perf_counter() - t0, pairs, lat
 
-    # warm up: one mid-size request
+    # warm up: one mid-size request, then one 1-query request on the idle engine
     t = time.perf_counter()
-    w = co.wait(co.submit(query_packed=pool[:16]))[0]
+    co.wait(co.submit(query_packed=pool[:16]))
     log(f"warmup request 16 x {n}: {time.perf_counter() - t:.2f}s")
-    del w
-    k = args.requests
+    idle = [co.wait(co.submit(query_packed=pool[i:i + 1]))[1] for i in range(5)]
+    k = sat_requests
     sat = [make_request() for _ in range(k)]
-    dt, pairs, lat_sat = run(sat, [0.0] * k)
+    dt, pairs, lat_sat = run(sat, [0.0] * k, ["request"] * k)
     capacity = pairs / dt
     log(f"saturated: {k} requests, {pairs} pairs in {dt:.2f}s = {capacity:.0f} pair-fits/s")
-    reqs = [make_request() for _ in range(k)]
+    reqs = [make_request() for _ in range(requests)]
     mean_pairs = np.mean([q.shape[0] * (n if s is None else len(s)) for q, s in reqs])
-    rate = args.load * capacity / mean_pairs
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, k))
+    rate = load * capacity / mean_pairs
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, requests))
     arrivals -= arrivals[0]
-    dt2, pairs2, lat = run(reqs, arrivals)
+    window = float(arrivals[-1]) if requests > 1 else 1.0
+    probes = [(pool[int(i):int(i) + 1], None) for i in rng.integers(0, n_query_pool, single_probes)]
+    probe_at = list(np.sort(rng.uniform(0.0, window, single_probes)))
+    dt2, pairs2, lat = run(reqs + probes, list(arrivals) + probe_at,
+                           ["request"] * requests + ["single_query"] * single_probes)
     st = co.serve_stats()
     occupancy = st["busy_slot_sweeps"] / max(1, st["sweeps"]) / st["slots"]
     n_kept = co.n_kept
-    log(f"poisson: {k} requests at {rate:.3f}/s, {pairs2} pairs in {dt2:.2f}s, "
-        f"p50 {percentile(lat, 50):.0f} ms p99 {percentile(lat, 99):.0f} ms")
+    lr, l1 = lat.get("request", []), lat.get("single_query", [])
+    log(f"poisson: {requests} requests at {rate:.3f}/s (+{single_probes} 1-query), {pairs2} pairs in {dt2:.2f}s, "
+        f"p50 {percentile(lr, 50):.0f} ms p99 {percentile(lr, 99):.0f} ms, 1-query p50 {percentile(l1, 50):.0f} ms")
     co.close()
-    line = {
-        "metric": "pair-fits/sec (serving)", "value": round(capacity, 1), "unit": "pair-fits/s", "n_gpus": 1,
-        "steps": 2 * k, "warmup": 1, "ms_per_step": round(dt / k * 1e3, 2), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if args.precision == "mixed" else "f64",
-        "data": "synthetic",
-        "config": {"workload": f"{args.config} serving: cohort {n} samples x {v} sites ({n_kept} segregating) "
-                               f"device-resident; requests of 1-256 queries vs all or 500-5000 samples",
-                   "samples": n, "sites": v, "requests": k, "precision": args.precision, "policy": args.policy},
+
+    def r1(x):
+        return None if x is None else round(x, 1)
+    return {
+        "workload": f"{config} serving: cohort {n} samples x {v} sites ({n_kept} segregating) device-resident, "
+                    f"{precision}; requests of 1-256 queries vs all or 500-5000 samples",
+        "samples": n, "sites": v, "precision": precision, "policy": policy, "slots": st0["slots"],
         "saturated": {"requests": k, "pairs": int(pairs), "seconds": round(dt, 3),
                       "pair_fits_per_s": round(capacity, 1),
-                      "latency_ms_p50": round(percentile(lat_sat, 50), 1),
-                      "latency_ms_p99": round(percentile(lat_sat, 99), 1)},
-        "poisson": {"requests": k, "load": args.load, "rate_per_s": round(rate, 4), "pairs": int(pairs2),
+                      "latency_ms_p50": r1(percentile(lat_sat.get("request", []), 50)),
+                      "latency_ms_p99": r1(percentile(lat_sat.get("request", []), 99))},
+        "poisson": {"requests": requests, "load": load, "rate_per_s": round(rate, 4), "pairs": int(pairs2),
                     "seconds": round(dt2, 3), "pair_fits_per_s": round(pairs2 / dt2, 1),
-                    "latency_ms_p50": round(percentile(lat, 50), 1),
-                    "latency_ms_p99": round(percentile(lat, 99), 1)},
+                    "latency_ms_p50": r1(percentile(lr, 50)), "latency_ms_p99": r1(percentile(lr, 99)),
+                    "latency_ms_max": r1(max(lr) if lr else None),
+                    "p99_is_max": len(lr) < 100},
+        "single_query": {"requests_under_load": single_probes, "latency_ms_p50": r1(percentile(l1, 50)),
+                         "latency_ms_p99": r1(percentile(l1, 99)),
+                         "idle_latency_ms_p50": r1(percentile(idle, 50)),
+                         "pairs_each": n},
         "slot_occupancy": round(occupancy, 4),
+    }
+
+
+def serving(args):
+    """The serving workload alone as the JSON line (--serving)."""
+    res = serving_run(args.config, args.precision, args.policy, args.slots, args.requests, args.poisson_requests,
+                      args.load, args.single_probes)
+    k = res["saturated"]["requests"]
+    line = {
+        "metric": "pair-fits/sec (serving)", "value": res["saturated"]["pair_fits_per_s"], "unit": "pair-fits/s",
+        "n_gpus": 1, "steps": k + res["poisson"]["requests"], "warmup": 1,
+        "ms_per_step": round(res["saturated"]["seconds"] / k * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if args.precision == "mixed" else "f64",
+        "data": "synthetic", "config": {"workload": res["workload"], "samples": res["samples"],
+                                        "sites": res["sites"], "precision": args.precision, "policy": args.policy},
+        **{k2: res[k2] for k2 in ("saturated", "poisson", "single_query", "slot_occupancy")},
     }
     print(json.dumps(line), flush=True)
 
@@ -382,6 +465,59 @@ def mixed_errors(codes, co_mixed, config, n_pairs=20000, seed=0xE44, gl=None):
             "reference": "the FP64 form of this library (bit-exact with the oracle in its summation order)"}
 
 
+def timed_fits(co, steps, warmup):
+    """warmup untimed fits, then `steps` timed fits of every pair: the
+    throughput, the algorithmic rate over the time with a sweep running, and
+    the sweep launch statistics."""
+    for i in range(warmup):
+        t = time.perf_counter()
+        co.fit_pairs()
+        log(f"warmup {i}: {time.perf_counter() - t:.3f}s")
+    times, flops, sweep_ms, sweeps, busy_ms, groups, rec_ms, rec_pairs = [], 0.0, 0.0, 0, 0.0, 1, 0.0, 0
+    out = None
+    for i in range(steps):
+        t = time.perf_counter()
+        out = co.fit_pairs()
+        dt = time.perf_counter() - t
+        st = co.last_fit_stats()
+        times.append(dt)
+        flops += algorithmic_flops(out)
+        sweep_ms += st["sweep_ms"]
+        sweeps += st["sweeps"]
+        busy_ms += st["busy_ms"]
+        groups = st["groups"]
+        rec_ms += st["recondition_ms"]
+        rec_pairs += st["recondition_pairs"]
+        log(f"step {i}: {dt:.3f}s, {st['sweeps']} sweeps in {st['groups']} slot groups, sweep launches "
+            f"{st['sweep_ms']:.1f} ms summed, {st['busy_ms']:.1f} ms with a sweep running"
+            + (f", FP64 condition pass {st['recondition_ms']:.1f} ms for {st['recondition_pairs']} pairs"
+               if st["recondition_pairs"] else ""))
+    total = float(np.sum(times))
+    offdiag = int((out["x"] != out["y"]).sum())
+    # the slot groups' launches overlap: the device rate is the algorithmic
+    # work over the time with at least one sweep running
+    return {"out": out, "seconds": total, "value": offdiag * steps / total,
+            "achieved": flops / (busy_ms / 1e3) / 1e12,
+            "recondition": {"ms_per_step": round(rec_ms / steps, 2), "pairs_per_step": rec_pairs // max(1, steps)},
+            "launch": {"avg_launch_ms": round(sweep_ms / sweeps, 4), "concurrent_launches": groups,
+                       "sweep_busy_ms_per_step": round(busy_ms / steps, 2),
+                       "algorithmic_flop_per_launch": flops / sweeps}}
+
+
+def measured_peak(mixed):
+    """FP32 (mixed) or FP64 FMA-chain rate of this device in TFLOP/s, or None."""
+    try:
+        import ctypes
+        from kestrel_amd import _lib
+        pk = ctypes.c_double()
+        _lib.check((_lib.load().kestrel_measure_fp32_peak if mixed else _lib.load().kestrel_measure_fp64_peak)(
+            ctypes.byref(pk)))
+        return pk.value
+    except Exception as e:  # the measurement is informative only
+        log(f"peak measurement failed: {e}")
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -404,7 +540,16 @@ def main():
     ap.add_argument("--serving", action="store_true", help="serving workload (default config c5)")
     ap.add_argument("--soft", action="store_true", help="soft genotype likelihoods instead of hard calls")
     ap.add_argument("--depth", type=float, default=4.0, help="mean read depth of --soft likelihoods")
-    ap.add_argument("--requests", type=int, default=12, help="requests per serving phase")
+    ap.add_argument("--requests", type=int, default=12, help="serving: requests of the saturated phase")
+    ap.add_argument("--poisson-requests", type=int, default=100, help="serving: requests of the Poisson phase")
+    ap.add_argument("--single-probes", type=int, default=20,
+                    help="serving: 1-query requests at uniform times in the Poisson phase")
+    ap.add_argument("--serving-precision", choices=("fp64", "mixed"), default="mixed",
+                    help="form of the serving workload (its configuration names the FP32-mixed kernels)")
+    ap.add_argument("--no-mixed-leg", action="store_true",
+                    help="skip the FP32-mixed fit of the same workload in the default run")
+    ap.add_argument("--no-serving-leg", action="store_true",
+                    help="skip the serving workload (C5) in the default run")
     ap.add_argument("--load", type=float, default=0.7, help="Poisson offered load / saturated capacity")
     ap.add_argument("--policy", choices=("srpt", "rr"), default="srpt",
                     help="serving feed order: shortest remaining first or round robin")
@@ -420,6 +565,7 @@ def main():
     if args.serving:
         if args.config == "c2":
             args.config = "c5"
+        args.precision = args.serving_precision
         serving(args)
         return
     if args.cli:
@@ -490,31 +636,9 @@ def main():
         tuned = co.autotune(sample=args.autotune)
         log(f"autotune ({time.perf_counter() - t:.1f}s, untimed): {tuned}")
 
-    for i in range(args.warmup):
-        t = time.perf_counter()
-        co.fit_pairs()
-        log(f"warmup {i}: {time.perf_counter() - t:.3f}s")
-    times, flops, sweep_ms, sweeps, busy_ms, groups = [], 0.0, 0.0, 0, 0.0, 1
-    out = None
-    for i in range(args.steps):
-        t = time.perf_counter()
-        out = co.fit_pairs()
-        dt = time.perf_counter() - t
-        st = co.last_fit_stats()
-        times.append(dt)
-        flops += algorithmic_flops(out)
-        sweep_ms += st["sweep_ms"]
-        sweeps += st["sweeps"]
-        busy_ms += st["busy_ms"]
-        groups = st["groups"]
-        log(f"step {i}: {dt:.3f}s, {st['sweeps']} sweeps in {st['groups']} slot groups, sweep launches "
-            f"{st['sweep_ms']:.1f} ms summed, {st['busy_ms']:.1f} ms with a sweep running")
-    total = float(np.sum(times))
+    fit64 = timed_fits(co, args.steps, args.warmup)
+    out, value, total, achieved = fit64["out"], fit64["value"], fit64["seconds"], fit64["achieved"]
     offdiag = int((out["x"] != out["y"]).sum())
-    value = offdiag * args.steps / total
-    # the slot groups' launches overlap: the device rate is the algorithmic
-    # work over the time with at least one sweep running
-    achieved = flops / (busy_ms / 1e3) / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_VALU_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_VALU_PEAK_TFLOPS, 4),
                 "traffic": (SWEEP_TRAFFIC_BYTES_C2 if args.config == "c2" and not args.segment_sites
@@ -522,37 +646,60 @@ def main():
                 "kernel": ("sweep_gl + sweep_glm" if buckets
                            else ("sweep_gl_mixed" if args.precision == "mixed" else "sweep_gl") if args.soft
                            else "sweep_mixed" if args.precision == "mixed" else "sweep_fp64"),
-                "avg_launch_ms": round(sweep_ms / sweeps, 4),
-                "concurrent_launches": groups, "sweep_busy_ms_per_step": round(busy_ms / args.steps, 2),
-                "algorithmic_flop_per_launch": flops / sweeps}
+                **fit64["launch"]}
     mixed = args.precision == "mixed"
     if mixed:  # the mixed form is priced against FP32 (vendor figure: packed FP32)
         roofline["peak"] = FP32_VALU_PEAK_TFLOPS
         roofline["frac"] = round(achieved / FP32_VALU_PEAK_TFLOPS, 4)
-    measured_peak = None
-    try:
-        import ctypes
-        from kestrel_amd import _lib
-        pk = ctypes.c_double()
-        _lib.check((_lib.load().kestrel_measure_fp32_peak if mixed else _lib.load().kestrel_measure_fp64_peak)(
-            ctypes.byref(pk)))
-        measured_peak = round(pk.value, 2)
-        roofline["measured_peak"] = measured_peak
-        roofline["frac_of_measured"] = round(achieved / pk.value, 4)
-    except Exception as e:  # the measurement is informative only
-        log(f"fp64 peak measurement failed: {e}")
+    pk = measured_peak(mixed)
+    if pk:
+        roofline["measured_peak"] = round(pk, 2)
+        roofline["frac_of_measured"] = round(achieved / pk, 4)
     precision_error = None
     if mixed:
         precision_error = mixed_errors(c.codes, co, args.config, gl=gl)
         log(f"mixed vs FP64 form: {precision_error}")
-    cpu = None
+    # the FP32-mixed form on the same workload, beside the FP64 form
+    mixed_leg = None
+    if not mixed and not args.soft and not buckets and not args.no_mixed_leg:
+        from kestrel_amd import _lib
+        co.set_option("precision", _lib.MIXED)
+        fm = timed_fits(co, args.steps, min(args.warmup, 1))
+        co.set_option("precision", _lib.FP64)
+        pkm = measured_peak(True)
+        mixed_leg = {
+            "value": round(fm["value"], 1), "unit": "pair-fits/s", "ms_per_step": round(fm["seconds"] / args.steps * 1e3, 2),
+            "vs_fp64_form": round(fm["value"] / value, 4),
+            "roofline": {"bound": "valu", "achieved": round(fm["achieved"], 3), "peak": FP32_VALU_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(fm["achieved"] / FP32_VALU_PEAK_TFLOPS, 4),
+                         "kernel": "sweep_mixed", **fm["launch"],
+                         **({"measured_peak": round(pkm, 2), "frac_of_measured": round(fm["achieved"] / pkm, 4)}
+                            if pkm else {})},
+            "fp64_condition_pass": fm["recondition"],
+            "iterations_mean": round(float(fm["out"]["iters"].mean()), 3),
+            "errors_vs_fp64_form": record_errors(fm["out"], out),
+        }
+        log(f"mixed form: {mixed_leg}")
+    cpu, errors = None, None
     if buckets and args.cpu_budget > 0:
-        cpu = cpu_baseline(None, None, args.cpu_budget, buckets=buckets)
+        cpu, _, _ = cpu_baseline(None, None, args.cpu_budget, buckets=buckets)
         log(f"cpu baseline: {cpu}")
     elif args.cpu_budget > 0:
         glk = None if gl is None else np.ascontiguousarray(gl.transpose(1, 0, 2)[:, kept])
-        cpu = cpu_baseline(c.codes[:, kept], af[kept], args.cpu_budget, gl=glk)
+        cpu, cpairs, crecs = cpu_baseline(c.codes[:, kept], af[kept], args.cpu_budget, gl=glk)
         log(f"cpu baseline: {cpu}")
+        # every output's error against the oracle in the reference's order on
+        # the pairs the CPU leg fitted
+        idx = tri_index(n, cpairs)
+        errors = {"reference": "the oracle in the reference's 8-lane order (" + cpu["sample"] + ")",
+                  **record_errors(out[idx], crecs)}
+        log(f"errors vs reference order: {errors}")
+        if mixed_leg:
+            mixed_leg["errors_vs_reference_order"] = record_errors(fm["out"][idx], crecs)
+    serving_leg = None
+    if not args.no_serving_leg and args.config == "c2" and not args.soft and not buckets and not mixed:
+        serving_leg = serving_run("c5", args.serving_precision, args.policy, None, args.requests,
+                                  args.poisson_requests, args.load, args.single_probes)
     line = {
         "metric": "pair-fits/sec", "value": round(value, 1), "unit": "pair-fits/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(total / args.steps * 1e3, 2),
@@ -572,6 +719,9 @@ def main():
                    "skip_masked": "auto" if args.skip_masked is None else args.skip_masked},
         "roofline": roofline, "cpu_baseline": cpu,
         **({"precision_error": precision_error} if mixed else {}),
+        **({"errors": errors} if errors else {}),
+        **({"mixed": mixed_leg} if mixed_leg else {}),
+        **({"serving": serving_leg} if serving_leg else {}),
         "iterations_mean": round(float(out["iters"].mean()), 3),
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
     }

@@ -234,7 +234,9 @@ int kestrel_cohort_create_gt_multi(kestrel_cohort** out, uint32_t n_samples, con
  * changes records),
  * "serve_policy" (order in
  * which a serving engine started afterwards feeds admitted requests to the
- * device: 1 = shortest remaining first (default), 0 = round robin). */
+ * device: 1 = shortest remaining first, aged by the device's pair rate times
+ * the time since submission so that large requests are not starved (default),
+ * 0 = round robin). */
 int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value);
 
 /* Launch-shape autotuner (no counterpart in the reference; the reference's

@@ -15,6 +15,8 @@
  *                  segment partials are folded left to right.  This is the
  *                  order the device kernels use, so device results can be
  *                  compared bit for bit.
+ *   KO_ORDER_FAST: KO_ORDER_REF in the reference's AVX-512 loop shape (the
+ *                  CPU baseline's timing form; same bits).
  *
  * Pins (tests/test_oracle_*.py): the identity-state probabilities against an
  * exact-rational brute force, the two reference encodings of c_s against each
@@ -39,7 +41,12 @@
 extern "C" {
 #endif
 
-enum { KO_ORDER_REF = 0, KO_ORDER_SEG = 1 };
+/* KO_ORDER_FAST: KO_ORDER_REF computed in the reference's own AVX-512 loop
+ * shape (c_s rows built per 8-site block, objective with the getexp/getmant
+ * log, Hessian in three 15-accumulator passes per tile; ko_lanes_fast.c):
+ * the same bits as KO_ORDER_REF, for timing the CPU baseline.  Hard-call
+ * cohorts on AVX-512 hosts; elsewhere it runs KO_ORDER_REF. */
+enum { KO_ORDER_REF = 0, KO_ORDER_SEG = 1, KO_ORDER_FAST = 2 };
 
 /* sqp.rs:16-47 Tuneables */
 typedef struct {

@@ -49,6 +49,8 @@ const ko_lane_ops* ko_ops(void) {
 
 const char* ko_isa_name(void) { return ko_ops()->name; }
 
+int ko_has_avx512(void) { return ko_ops() == &OPS_AVX512; }
+
 /* ------------------------------------------------------------------ tuning */
 
 /* sqp.rs:31-46 */
@@ -760,6 +762,7 @@ typedef struct {
 
 static double pair_obj(void* vc, const double* x, double eps) {
     pair_ctx* c = (pair_ctx*)vc;
+    if (c->order == KO_ORDER_FAST) return ko_jacq_obj_lv_fast(&c->lv, x, eps);
     if (c->order == KO_ORDER_REF) return ko_ops()->jacq_obj(&c->lv, x, eps);
     return ko_jacq_obj_seg(c->rows, c->valid, 0, c->seg, c->nseg, x, eps);
 }
@@ -767,7 +770,8 @@ static double pair_obj(void* vc, const double* x, double eps) {
 static void pair_gh(void* vc, const double* x, double eps, double* g, double* h) {
     pair_ctx* c = (pair_ctx*)vc;
     double hh[81];
-    if (c->order == KO_ORDER_REF) ko_ops()->jacq_gh(&c->lv, x, eps, g, hh);
+    if (c->order == KO_ORDER_FAST) ko_jacq_gh_lv_fast(&c->lv, x, eps, g, hh);
+    else if (c->order == KO_ORDER_REF) ko_ops()->jacq_gh(&c->lv, x, eps, g, hh);
     else ko_jacq_gh_seg(c->rows, c->valid, 0, c->seg, c->nseg, x, eps, g, hh);
     memcpy(h, hh, sizeof hh);
 }
@@ -1144,6 +1148,7 @@ typedef struct {
     ko_tune tune;
     ko_output* out;
     atomic_size_t next;
+    const double *p, *q; /* KO_ORDER_FAST: the frequencies split into two arrays */
 } fit_job;
 
 /* row-major upper triangle incl. diagonal: index -> (x, y) */
@@ -1181,6 +1186,20 @@ static void* fit_worker(void* vj) {
         memset(o, 0, sizeof *o);
         o->x = x;
         o->y = y;
+        if (j->order == KO_ORDER_FAST) {
+            /* the lane vector filled directly (LaneVectorBuilder) */
+            const size_t n = ko_pair_lanes_codes_fast(j->codes + (size_t)x * V, j->codes + (size_t)y * V, j->p, j->q,
+                                                      V, &ctx.lv);
+            o->n_sites = (uint32_t)n;
+            if (n == 0) {
+                for (int k = 0; k < 9; k++) o->jacquard[k] = NAN;
+                o->obj = NAN;
+                o->cond = NAN;
+                continue;
+            }
+            fit_ctx(&ctx, x == y, &j->tune, o);
+            continue;
+        }
         size_t n = j->gl ? ko_pair_rows_gl(j->gl + (size_t)x * V * 3, j->gl + (size_t)y * V * 3, j->af, V, rows, valid)
                          : ko_pair_rows_codes(j->codes + (size_t)x * V, j->codes + (size_t)y * V, j->af, V, rows,
                                               valid);
@@ -1239,11 +1258,26 @@ static int fit_pairs(int order, const uint8_t* codes, const double* glrows, uint
     ko_tune_new(&j.tune, iterations, 10); /* coefficients.rs:73 */
     j.out = out;
     atomic_init(&j.next, 0);
+    double* pq = NULL;
+    if (order == KO_ORDER_FAST) {
+        if (glrows || !ko_has_avx512()) {
+            j.order = KO_ORDER_REF;  /* the fast loops cover hard calls on AVX-512 hosts */
+        } else {
+            pq = (double*)malloc(2 * (size_t)V * sizeof(double) + 64);
+            for (uint32_t s = 0; s < V; s++) {
+                pq[s] = af[2 * s];
+                pq[V + s] = af[2 * s + 1];
+            }
+            j.p = pq;
+            j.q = pq + V;
+        }
+    }
     int nt = nthreads_default(nthreads);
     pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nt);
     for (int i = 0; i < nt; i++) pthread_create(&th[i], NULL, fit_worker, &j);
     for (int i = 0; i < nt; i++) pthread_join(th[i], NULL);
     free(th);
+    free(pq);
     return 0;
 }
 

@@ -27,6 +27,15 @@ KO_DECL_VARIANT(avx512)
 KO_DECL_VARIANT(avx2)
 KO_DECL_VARIANT(base)
 
+/* ko_lanes_fast.c: the REF loops in the reference's AVX-512 loop shape
+ * (bit-identical to the variants above; AVX-512 hosts only). */
+double ko_lane_log8_fast(double d, int lane);
+double ko_jacq_obj_lv_fast(const ko_lanevec* lv, const double x[9], double eps);
+void ko_jacq_gh_lv_fast(const ko_lanevec* lv, const double x[9], double eps, double g[9], double h[81]);
+size_t ko_pair_lanes_codes_fast(const uint8_t* gx, const uint8_t* gy, const double* p, const double* q, size_t V,
+                                ko_lanevec* lv);
+int ko_has_avx512(void);
+
 typedef struct {
     double (*log)(double);
     double (*jacq_obj)(const ko_lanevec*, const double*, double);

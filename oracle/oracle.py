@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "_build", "libkestrel_oracle.so")
 
 ORDER_REF = 0
 ORDER_SEG = 1
+ORDER_FAST = 2  # ORDER_REF in the reference's AVX-512 loop shape (same bits; timing form)
 
 
 class Tune(ctypes.Structure):
@@ -62,6 +63,9 @@ def lib():
         L.ko_log.argtypes = [D]
         L.ko_log_getmant.restype = D
         L.ko_log_getmant.argtypes = [D]
+        L.ko_lane_log8_fast.restype = D
+        L.ko_lane_log8_fast.argtypes = [D, I]
+        L.ko_has_avx512.restype = I
         L.ko_isa_name.restype = ctypes.c_char_p
         L.ko_tune_new.argtypes = [P, U64, U64]
         L.ko_mixture_row2.argtypes = [P, P, P, P]
