@@ -157,7 +157,17 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
                       uint32_t n_select, kestrel_output* out);
 
 /* Objective, gradient and Hessian of the Jacquard likelihood at n points:
- * pairs n x 2, points n x 9; f n, g n x 9, h n x 81 (row-major), n_sites n. */
+ * pairs n x 2, points n x 9; f n, g n x 9, h n x 81 (row-major), n_sites n.
+ * Any point and any eps, as the reference's Objective / GradHess accept: a
+ * point with finite, non-negative coefficients, eps in [1e-18, 1e14] and
+ * eps + sum(D) x (largest likelihood)^2 <= 1e15 runs on the sweeps (every
+ * L = eps + c.D is then a positive normal number, as are 1/L and the products
+ * of 16 L the objective takes logs of); any other point is evaluated with
+ * IEEE division and per-site logs wherever a 16-site group holds an L that is
+ * not a positive normal number, so L = 0 gives f = +inf, a negative L gives
+ * NaN and 1/L is the IEEE quotient (log/scalar.rs:49-57, grad_hess.rs:122),
+ * in FP64 whatever the cohort's precision form.  kestrel_fit_pairs requires
+ * eps in [1e-18, 1e14] (its points lie on the simplex). */
 int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* points, size_t n, double eps,
                      double* f, double* g, double* h, uint32_t* n_sites);
 
@@ -168,7 +178,7 @@ int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* poi
  * kestrel_evaluate's bit for bit.  The sweep's slot arrays and segment
  * partials live in a fixed 64 MiB array of device memory that each GPU the
  * library runs on holds until the process ends, so a call takes at most
- * 64 MiB / (84 + 448 x segments) bytes' worth of points (segments: one per
+ * 64 MiB / (88 + 448 x segments) bytes' worth of points (segments: one per
  * 4,096 kept sites by default; about 5,900 points at 100,000 sites) and
  * returns KESTREL_EINVAL beyond that.  The call allocates nothing, runs on
  * the cohort's stream and returns when the results are written.  Every call

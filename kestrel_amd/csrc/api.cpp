@@ -82,6 +82,39 @@ int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, con
     return KESTREL_OK;
 }
 
+// Largest likelihood of n values (NaN marks a missing call); a negative or
+// infinite likelihood is rejected (vcf.rs:368, 407 make them 10^x >= 0).
+int scan_likelihoods(const double* v, size_t n, double* lmax) {
+    double m = 0.0;
+    bool bad = false;
+    for (size_t i = 0; i < n; i++) {
+        const double x = v[i];
+        bad |= x < 0.0 || x == INFINITY;
+        m = x > m ? x : m;  // NaN compares false
+    }
+    if (bad) return fail(KESTREL_EINVAL, "likelihoods must be non-negative and finite (NaN marks a missing call)");
+    *lmax = std::max(*lmax, m);
+    return KESTREL_OK;
+}
+
+}  // namespace
+
+int launch_exact(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* exact,
+                 uint32_t ns, double eps, double* partial, hipStream_t st) {
+    const int kind = c->soft ? (c->gl_stride == 2 ? 2 : 3) : 0;
+    HIPCHK(klaunch::sweep_exact(kind, c->geno.p, c->row_words, c->soft ? c->glrows.p : nullptr, c->v_pad, c->af.p,
+                                c->seg.p, c->nseg, sx, sy, pt, exact, ns, eps, partial, st));
+    size_t base = c->nseg;
+    for (const auto& p : c->parts) {
+        HIPCHK(klaunch::sweep_exact(10 + p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->table.p,
+                                    p->seg.p, p->nseg, sx, sy, pt, exact, ns, eps, partial + base * NPART * ns, st));
+        base += p->nseg;
+    }
+    return KESTREL_OK;
+}
+
+namespace {
+
 // Claim order of a pair list in sample tiles: a stable counting sort by the
 // tiles (min(x, y) / T, max(x, y) / T) of its samples, tile pairs in row-major
 // order.  The pairs in flight at any time then come from a few tiles, so few
@@ -298,6 +331,9 @@ int kestrel_cohort_create_gl(kestrel_cohort** out, const double* gl, uint32_t n_
     c->n_samples = n_samples;
     c->n_sites = n_sites;
     const size_t n_vals = (size_t)n_sites * n_samples * 3;
+    c->lmax = 0.0;
+    rc = scan_likelihoods(gl, n_vals, &c->lmax);
+    if (rc) return rc;
     DevBuf<double> dgl;
     HIPCHK(dgl.alloc(n_vals));
     HIPCHK(hipMemcpyAsync(dgl.p, gl, n_vals * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -396,6 +432,11 @@ int kestrel_cohort_create_gl_multi(kestrel_cohort** out, uint32_t n_samples, con
     c->multi = true;
     c->n_samples = n_samples;
     c->n_sites = n2 + n3 + n4;
+    c->lmax = 0.0;
+    for (int b = 0; b < 3; b++) {
+        rc = scan_likelihoods(gin[b], (size_t)nin[b] * n_samples * ((b + 2) * (b + 3) / 2), &c->lmax);
+        if (rc) return rc;
+    }
 
     // allele frequencies per input bucket (allele/mod.rs:37-134)
     DevBuf<double> dgl[3];
@@ -619,6 +660,12 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     if (c->gt) return fail(KESTREL_EINVAL, "a GT cohort is fitted with kestrel_fit_pairs_gt");
     if (tune->sqp_max_iter == 0 || tune->qp_max_iter == 0 || tune->bls_max_iter == 0)
         return fail(KESTREL_EINVAL, "iteration limits must be non-zero");
+    // the fit's points lie on the simplex: with eps in [1e-18, 1e14] every
+    // L = eps + c.D is a positive normal number, as are 1/L and the product of
+    // 16 of them, the sweeps' domain
+    if (!eps_in_domain(tune->epsilon) || c->lmax * c->lmax > 1e14)
+        return fail(KESTREL_EINVAL, "epsilon %g outside [1e-18, 1e14] (or likelihoods above 1e7) for a pair fit",
+                    tune->epsilon);
     std::vector<uint32_t> tri;
     if (!pairs) {
         if (n_select > c->n_samples)  // coefficients.rs:53-55
@@ -1053,20 +1100,26 @@ int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* poi
         if (pairs[i] >= c->n_samples) return fail(KESTREL_EINVAL, "pair names sample %u of %u", pairs[i], c->n_samples);
     const uint32_t ns = (uint32_t)((n + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB);
     std::vector<uint32_t> hx(ns, 0), hy(ns, 0);
-    std::vector<int> hact(ns, 0);
+    std::vector<int> hact(ns, 0), hex(ns, 0);
     std::vector<double> hpt((size_t)9 * ns, 0.0);
+    bool any_exact = false;
     for (size_t i = 0; i < n; i++) {
         hx[i] = pairs[2 * i];
         hy[i] = pairs[2 * i + 1];
-        hact[i] = 1;
+        // points outside the sweeps' domain go to the exact kernel
+        const bool in = point_in_domain(points + i * 9, eps, c->lmax);
+        hact[i] = in ? 1 : 0;
+        hex[i] = in ? 0 : 1;
+        any_exact |= !in;
         for (int k = 0; k < 9; k++) hpt[(size_t)k * ns + i] = points[i * 9 + k];
     }
     DevBuf<uint32_t> dsx, dsy, dn;
     DevBuf<double> dpt, dpart, df, dg, dh;
-    DevBuf<int> dact;
+    DevBuf<int> dact, dex;
     HIPCHK(dsx.alloc(ns));
     HIPCHK(dsy.alloc(ns));
     HIPCHK(dact.alloc(ns));
+    HIPCHK(dex.alloc(ns));
     HIPCHK(dpt.alloc((size_t)9 * ns));
     HIPCHK(dpart.alloc((size_t)c->nseg_total() * NPART * ns));
     HIPCHK(df.alloc(n));
@@ -1078,8 +1131,13 @@ int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* poi
     HIPCHK(hipMemcpyAsync(dsy.p, hy.data(), ns * sizeof(uint32_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dact.p, hact.data(), ns * sizeof(int), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dpt.p, hpt.data(), hpt.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dex.p, hex.data(), ns * sizeof(int), hipMemcpyHostToDevice, st));
     int rs = launch_sweeps(c, dsx.p, dsy.p, dpt.p, dact.p, ns, eps, dpart.p, st);
     if (rs) return rs;
+    if (any_exact) {
+        rs = launch_exact(c, dsx.p, dsy.p, dpt.p, dex.p, ns, eps, dpart.p, st);
+        if (rs) return rs;
+    }
     HIPCHK(klaunch::finalize(dpart.p, c->nseg_total(), ns, (uint32_t)n, df.p, dg.p, dh.p, dn.p, st));
     HIPCHK(hipMemcpyAsync(f, df.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(g, dg.p, 9 * n * sizeof(double), hipMemcpyDeviceToHost, st));

@@ -111,6 +111,27 @@ int recondition_records(kestrel_cohort* c, const uint32_t* geno, uint32_t row_wo
                         const std::function<void(size_t, uint32_t&, uint32_t&)>& rows, hipStream_t st,
                         uint64_t* n_done = nullptr);
 
+// The sweeps' domain: eps in [1e-18, 1e14] and a point of finite,
+// non-negative coefficients with eps + lmax^2 sum(D) <= 1e15.  There every
+// L = eps + c.D (c >= 0, c_k <= lmax^2) lies in [1e-18, 1e15], so 1/L and
+// the product of 16 L are positive normal numbers; points outside it are
+// evaluated by the exact kernel.
+__host__ __device__ inline bool eps_in_domain(double eps) { return eps >= 1e-18 && eps <= 1e14; }
+__host__ __device__ inline bool point_in_domain(const double* d, double eps, double lmax) {
+    if (!eps_in_domain(eps)) return false;
+    double s = 0.0;
+    for (int k = 0; k < 9; k++) {
+        if (!(d[k] >= 0.0 && d[k] <= 1e15)) return false;
+        s += d[k];
+    }
+    return eps + lmax * lmax * s <= 1e15;
+}
+
+// Exact-arithmetic evaluation of the slots with exact[slot] != 0 over every
+// part of the cohort into the sweeps' partial layout.
+int launch_exact(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* exact,
+                 uint32_t ns, double eps, double* partial, hipStream_t st);
+
 // Stops the serving engine of a cohort (if any) and frees its buffers.
 void serve_release(ServeState* s);
 
@@ -140,6 +161,7 @@ struct kestrel_cohort {
     int64_t opt_skip_masked = -1;  // claims in runs + sweeps passing over sites where a wavefront's shared
                                    // sample is missing: 1 on, 0 off, -1 when missing_frac >= 0.02
     double missing_frac = 0.0;     // missing calls among the kept sites (2-bit cohorts)
+    double lmax = 1.0;             // largest likelihood of the cohort's rows (1 for hard calls)
     bool mixed = false;            // precision KESTREL_MIXED: sweeps by sweep_mixed
     int64_t opt_pair_tile = -1;    // claim order in tiles of this many samples (0 list order, -1 automatic)
     hipStream_t stream2 = nullptr; // second stream of a fit

@@ -1425,6 +1425,112 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __rest
     }
 }
 
+// Evaluation at points outside the sweeps' domain (any point, any eps:
+// L = eps + c.D may be zero, negative, subnormal or not finite), in the
+// sweeps' segmented order with the reference's arithmetic at every site:
+// 1/L by IEEE division and, where a 16-site group holds an L that is not a
+// positive normal number or its product is not one, the group's logs taken
+// one site at a time with the full log (0 -> -inf, negative or NaN -> NaN,
+// log/scalar.rs:49-57), added in site order; otherwise the log of the
+// group's product as in the sweeps.  On the sweeps' domain it returns their
+// FP64 bits.  One thread per (slot, segment); slots with exact[slot] == 0
+// are left to the sweeps.  KIND: 0 = 2-bit rows (c from the site's table
+// block), 2 / 3 = likelihood rows of stride 2 / 3 (mixture_row2), 13 / 14 =
+// a part of 3- / 4-allele sites (the generic mixture over its tables).
+template <int KIND>
+__global__ __launch_bounds__(64) void sweep_exact_kernel(const uint32_t* __restrict__ geno, uint32_t row_words,
+                                                   const double* __restrict__ gl, uint32_t v_pad,
+                                                   const double* __restrict__ af_or_table,
+                                                   const uint32_t* __restrict__ seg_bounds,
+                                                   const uint32_t* __restrict__ sx, const uint32_t* __restrict__ sy,
+                                                   const double* __restrict__ pt, const int* __restrict__ exact,
+                                                   uint32_t nslots, double eps, double* __restrict__ partial) {
+    constexpr int A = KIND == 13 ? 3 : KIND == 14 ? 4 : 2;
+    constexpr int G = A * (A + 1) / 2;
+    const uint32_t slot = blockIdx.x * 64 + threadIdx.x;
+    const uint32_t seg = blockIdx.y;
+    if (slot >= nslots || !exact[slot]) return;
+    const uint32_t x = sx[slot], y = sy[slot];
+    double D[9];
+    for (int k = 0; k < 9; k++) D[k] = pt[(size_t)k * nslots + slot];
+    const uint32_t* rowx = geno + (size_t)x * row_words;
+    const uint32_t* rowy = geno + (size_t)y * row_words;
+    double F = 0.0, N = 0.0, P = 1.0, Gr[9], H[45], Lg[LOG_GROUP];
+    bool normal = true;
+    for (int k = 0; k < 9; k++) Gr[k] = 0.0;
+    for (int k = 0; k < 45; k++) H[k] = 0.0;
+    const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
+    for (uint32_t t = s0; t < s1; t++) {
+        const uint32_t cx = (rowx[t / 16] >> (2 * (t % 16))) & 3u, cy = (rowy[t / 16] >> (2 * (t % 16))) & 3u;
+        const bool missing = cx == 3u || cy == 3u;
+        double cv[9];
+        for (int j = 0; j < 9; j++) cv[j] = 0.0;
+        if (KIND == 0) {
+            if (!missing) lut_block(af_or_table[2 * t], af_or_table[2 * t + 1], (int)(cx * 3 + cy), cv);
+        } else {
+            constexpr int ST = KIND == 2 ? 2 : G;
+            const double* px = gl + ((size_t)x * v_pad + t) * ST;
+            const double* py = gl + ((size_t)y * v_pad + t) * ST;
+            double lx[G], ly[G];
+            if (KIND == 2) {  // the exact 1.0 back at the position of the 2-bit code
+                lx[0] = cx == 0 ? 1.0 : px[0];
+                lx[1] = cx == 0 ? px[0] : cx == 1 ? 1.0 : px[1];
+                lx[2] = cx == 2 ? 1.0 : px[1];
+                ly[0] = cy == 0 ? 1.0 : py[0];
+                ly[1] = cy == 0 ? py[0] : cy == 1 ? 1.0 : py[1];
+                ly[2] = cy == 2 ? 1.0 : py[1];
+            } else {
+                for (int q = 0; q < G; q++) {
+                    lx[q] = px[q];
+                    ly[q] = py[q];
+                }
+            }
+            if (KIND == 2 || KIND == 3) {
+                double K[K_N];
+                site_coefficients(af_or_table[2 * t], af_or_table[2 * t + 1], K);
+                mixture_soft(lx, ly, K, cv);
+            } else {
+                constexpr IisTable<A> tab{};
+                mixture_table<A>(lx, ly, af_or_table + (size_t)t * tab.off[G * G], cv,
+                                 std::make_integer_sequence<int, G * G>{});
+            }
+        }
+        // a site with a missing call is not a row (lanevector.rs:100-104):
+        // it adds nothing (the sweeps' L = 1, c = 0 there, which is the same
+        // on their domain)
+        const int k = (int)((t - s0) & (LOG_GROUP - 1));
+        Lg[k] = 1.0;
+        if (!missing) {
+            N += 1.0;
+            double L = eps;
+            for (int j = 0; j < 9; j++) L = fma(D[j], cv[j], L);
+            P = P * L;
+            Lg[k] = L;
+            normal = normal && L >= 2.2250738585072014e-308 && L <= 1.7976931348623157e308;
+            const double r = 1.0 / L;
+            for (int j = 0; j < 9; j++) Gr[j] = fma(cv[j], r, Gr[j]);
+            for (int j = 0; j < 9; j++) cv[j] = cv[j] * r;
+            int id = 0;
+            for (int a = 0; a < 9; a++)
+                for (int b = a; b < 9; b++, id++) H[id] = fma(cv[a], cv[b], H[id]);
+        }
+        if (k == LOG_GROUP - 1) {
+            if (normal && P >= 2.2250738585072014e-308 && P <= 1.7976931348623157e308) {
+                F = F + sleef_log_pos(P);
+            } else {
+                for (int q = 0; q < LOG_GROUP; q++) F = F + sleef_log(Lg[q]);
+            }
+            P = 1.0;
+            normal = true;
+        }
+    }
+    double* out = partial + (size_t)seg * NPART * nslots + slot;
+    out[0] = F;
+    for (int k = 0; k < 9; k++) out[(size_t)(1 + k) * nslots] = Gr[k];
+    for (int k = 0; k < 45; k++) out[(size_t)(10 + k) * nslots] = H[k];
+    out[(size_t)55 * nslots] = N;
+}
+
 // A compact 16-byte row: the two entries other than the first exact 1.0, in
 // order; the 1.0's position is the row's 2-bit code (3 = missing: the stored
 // pair is 0, 0).  A called row with no exact 1.0 sets *lacks (the caller then
@@ -2517,6 +2623,38 @@ hipError_t sweep_multi(int alleles, const uint32_t* geno, uint32_t row_words, co
                            sx, sy, pt, active, nslots, eps, partial);
     else
         return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t sweep_exact(int kind, const uint32_t* geno, uint32_t row_words, const double* gl, uint32_t v_pad,
+                       const double* af_or_table, const uint32_t* seg_bounds, uint32_t nseg, const uint32_t* sx,
+                       const uint32_t* sy, const double* pt, const int* exact, uint32_t nslots, double eps,
+                       double* partial, hipStream_t st) {
+    const dim3 grid((nslots + 63) / 64, nseg), block(64);
+    switch (kind) {
+    case 0:
+        hipLaunchKernelGGL(sweep_exact_kernel<0>, grid, block, 0, st, geno, row_words, gl, v_pad, af_or_table, seg_bounds,
+                           sx, sy, pt, exact, nslots, eps, partial);
+        break;
+    case 2:
+        hipLaunchKernelGGL(sweep_exact_kernel<2>, grid, block, 0, st, geno, row_words, gl, v_pad, af_or_table, seg_bounds,
+                           sx, sy, pt, exact, nslots, eps, partial);
+        break;
+    case 3:
+        hipLaunchKernelGGL(sweep_exact_kernel<3>, grid, block, 0, st, geno, row_words, gl, v_pad, af_or_table, seg_bounds,
+                           sx, sy, pt, exact, nslots, eps, partial);
+        break;
+    case 13:
+        hipLaunchKernelGGL(sweep_exact_kernel<13>, grid, block, 0, st, geno, row_words, gl, v_pad, af_or_table, seg_bounds,
+                           sx, sy, pt, exact, nslots, eps, partial);
+        break;
+    case 14:
+        hipLaunchKernelGGL(sweep_exact_kernel<14>, grid, block, 0, st, geno, row_words, gl, v_pad, af_or_table, seg_bounds,
+                           sx, sy, pt, exact, nslots, eps, partial);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -75,6 +75,13 @@ hipError_t sweep_multi(int alleles, const uint32_t* geno, uint32_t row_words, co
                        const double* table, const uint32_t* seg_bounds, uint32_t nseg, const uint32_t* sx,
                        const uint32_t* sy, const double* pt, const int* active, uint32_t nslots, double eps,
                        double* partial, hipStream_t st);
+// evaluation at points outside the sweeps' domain, the reference's arithmetic
+// at every site (kind 0: 2-bit rows, 2 / 3: likelihood rows of that stride,
+// 13 / 14: a 3- / 4-allele part); only slots with exact[slot] != 0
+hipError_t sweep_exact(int kind, const uint32_t* geno, uint32_t row_words, const double* gl, uint32_t v_pad,
+                       const double* af_or_table, const uint32_t* seg_bounds, uint32_t nseg, const uint32_t* sx,
+                       const uint32_t* sy, const double* pt, const int* exact, uint32_t nslots, double eps,
+                       double* partial, hipStream_t st);
 // per-site accumulate_iis coefficients of a part (iis_table_size doubles a site)
 int iis_table_size(int alleles);
 hipError_t iis_table(int alleles, const double* af, uint32_t n_sites, double* table, hipStream_t st);

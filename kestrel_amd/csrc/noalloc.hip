@@ -40,30 +40,37 @@ int scratch_ptr(int device, unsigned char** out) {
 }
 
 // Layout of one call in scratch (after the 256-byte head): sx, sy (ns u32),
-// active (ns int), points (9 x ns doubles, SoA), partials (nseg x 56 x ns).
-size_t slot_bytes(uint32_t nseg) { return 3 * sizeof(uint32_t) + 9 * sizeof(double) + (size_t)nseg * NPART * 8; }
+// active, exact (ns int), points (9 x ns doubles, SoA), partials
+// (nseg x 56 x ns).
+size_t slot_bytes(uint32_t nseg) { return 4 * sizeof(uint32_t) + 9 * sizeof(double) + (size_t)nseg * NPART * 8; }
 
 __global__ __launch_bounds__(64) void clear_head_kernel(unsigned int* __restrict__ err) {
     if (threadIdx.x == 0) *err = 0u;
 }
 
 // The caller's pairs (n x 2) and points (n x 9) into the sweep's slot arrays;
-// slots past n are inactive.  A pair naming a sample beyond the cohort is
-// counted in *err and left inactive.
+// slots past n are inactive.  A point inside the sweeps' domain goes to the
+// sweeps (active), one outside it to the exact kernel (exact).  A pair naming
+// a sample beyond the cohort is counted in *err and left to neither.
 __global__ __launch_bounds__(256) void pack_kernel(const uint32_t* __restrict__ pairs,
                                                    const double* __restrict__ points, uint32_t n, uint32_t ns,
-                                                   uint32_t n_samples, uint32_t* __restrict__ sx,
-                                                   uint32_t* __restrict__ sy, int* __restrict__ active,
+                                                   uint32_t n_samples, double eps, double lmax,
+                                                   uint32_t* __restrict__ sx, uint32_t* __restrict__ sy,
+                                                   int* __restrict__ active, int* __restrict__ exact,
                                                    double* __restrict__ pt, unsigned int* __restrict__ err) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= ns) return;
     uint32_t x = 0, y = 0;
-    int act = 0;
+    int act = 0, ex = 0;
+    double d[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) d[k] = i < n ? points[(size_t)i * 9 + k] : 0.0;
     if (i < n) {
         x = pairs[2 * (size_t)i];
         y = pairs[2 * (size_t)i + 1];
         if (x < n_samples && y < n_samples) {
-            act = 1;
+            act = point_in_domain(d, eps, lmax) ? 1 : 0;
+            ex = 1 - act;
         } else {
             atomicAdd(err, 1u);
             x = 0;
@@ -73,8 +80,9 @@ __global__ __launch_bounds__(256) void pack_kernel(const uint32_t* __restrict__ 
     sx[i] = x;
     sy[i] = y;
     active[i] = act;
+    exact[i] = ex;
 #pragma unroll
-    for (int k = 0; k < 9; k++) pt[(size_t)k * ns + i] = i < n ? points[(size_t)i * 9 + k] : 0.0;
+    for (int k = 0; k < 9; k++) pt[(size_t)k * ns + i] = d[k];
 }
 
 }  // namespace
@@ -101,14 +109,15 @@ extern "C" int kestrel_evaluate_noalloc(kestrel_cohort* c, const uint32_t* pairs
     uint32_t* sx = reinterpret_cast<uint32_t*>(base + SCRATCH_HEAD);
     uint32_t* sy = sx + ns;
     int* act = reinterpret_cast<int*>(sy + ns);
-    // 8-byte alignment for the doubles: 3 ns words, ns a multiple of 256
-    double* pt = reinterpret_cast<double*>(act + ns);
+    int* ex = act + ns;
+    // 8-byte alignment for the doubles: 4 ns words, ns a multiple of 256
+    double* pt = reinterpret_cast<double*>(ex + ns);
     double* part = pt + 9 * ns;
     hipStream_t st = c->stream;
     hipLaunchKernelGGL(clear_head_kernel, dim3(1), dim3(64), 0, st, err);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(pack_kernel, dim3((unsigned)(ns / 256)), dim3(256), 0, st, pairs, points, (uint32_t)n,
-                       (uint32_t)ns, c->n_samples, sx, sy, act, pt, err);
+                       (uint32_t)ns, c->n_samples, eps, c->lmax, sx, sy, act, ex, pt, err);
     HIPCHK(hipGetLastError());
     HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, sx, sy, pt, act, (uint32_t)ns, eps,
                           part, st, c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride, false, c->mixed));
@@ -118,6 +127,9 @@ extern "C" int kestrel_evaluate_noalloc(kestrel_cohort* c, const uint32_t* pairs
                                     p->nseg, sx, sy, pt, act, (uint32_t)ns, eps, part + off * NPART * ns, st));
         off += p->nseg;
     }
+    // points outside the sweeps' domain (the kernel leaves every other slot)
+    rc = launch_exact(c, sx, sy, pt, ex, (uint32_t)ns, eps, part, st);
+    if (rc) return rc;
     HIPCHK(klaunch::finalize(part, nseg, (uint32_t)ns, (uint32_t)n, f, g, h, n_sites, st));
     unsigned int bad = 0;
     HIPCHK(hipMemcpyAsync(&bad, err, sizeof bad, hipMemcpyDeviceToHost, st));

@@ -484,6 +484,8 @@ int kestrel_serve_start(kestrel_cohort* c, const kestrel_tune* tune, uint32_t ma
     if ((uint64_t)c->n_samples + max_query_rows > 0xffffffffull) return fail(KESTREL_EINVAL, "too many rows");
     if (tune->sqp_max_iter == 0 || tune->qp_max_iter == 0 || tune->bls_max_iter == 0)
         return fail(KESTREL_EINVAL, "iteration limits must be non-zero");
+    if (!eps_in_domain(tune->epsilon))
+        return fail(KESTREL_EINVAL, "epsilon %g outside [1e-18, 1e14] for a pair fit", tune->epsilon);
     HIPCHK(hipSetDevice(c->device));
     std::unique_ptr<ServeState> S(new ServeState());
     S->c = c;

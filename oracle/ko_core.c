@@ -360,7 +360,10 @@ void ko_jacq_gh_ref(const double* rows, size_t n, const double x[9], double eps,
  * partials are folded left to right.  The objective sum of log L_s is taken
  * as a sum over aligned 16-site groups of log(prod L_s) (usable sites only,
  * product from 1.0 in site order): the same quantity with no more rounding
- * error than 16 separate logs, and one log per group. */
+ * error than 16 separate logs, and one log per group.  A group holding an L
+ * that is not a positive normal number, or whose product is not one, adds
+ * its sites' logs one at a time in site order instead, so zero, negative,
+ * subnormal and infinite L keep the meaning of log/scalar.rs:49-57. */
 #define KO_LOG_GROUP 16
 static void seg_sums(const double* rows, const uint8_t* valid, const uint32_t* seg, size_t nseg, const double x[9],
                      double eps, int want_obj, int want_gh, double* fs, double gs[9], double hs[45], size_t* n_out) {
@@ -368,14 +371,19 @@ static void seg_sums(const double* rows, const uint8_t* valid, const uint32_t* s
     size_t n = 0;
     for (size_t k = 0; k < nseg; k++) {
         double zf = 0.0, zg[9] = {0}, zh[45] = {0};
-        double prod = 1.0;
+        double prod = 1.0, gl[KO_LOG_GROUP];
+        int ng = 0, normal = 1;
         for (uint32_t s = seg[k]; s < seg[k + 1]; s++) {
             if (valid[s]) {
             n++;
             const double* c = rows + (size_t)s * 9;
             double d = eps;
             for (int col = 0; col < 9; col++) d = fma(x[col], c[col], d);
-            if (want_obj) prod = prod * d;
+            if (want_obj) {
+                prod = prod * d;
+                gl[ng++] = d;
+                normal = normal && d >= DBL_MIN && d <= DBL_MAX;
+            }
             if (want_gh) {
                 const double r = 1.0 / d;
                 double cs[9];
@@ -387,8 +395,15 @@ static void seg_sums(const double* rows, const uint8_t* valid, const uint32_t* s
             }
             }
             if ((s + 1) % KO_LOG_GROUP == 0 || s + 1 == seg[k + 1]) {
-                if (want_obj) zf = zf + ko_log(prod);
+                if (want_obj) {
+                    if (normal && prod >= DBL_MIN && prod <= DBL_MAX)
+                        zf = zf + ko_log(prod);
+                    else
+                        for (int q = 0; q < ng; q++) zf = zf + ko_log(gl[q]);
+                }
                 prod = 1.0;
+                ng = 0;
+                normal = 1;
             }
         }
         if (k == 0) {

@@ -101,3 +101,45 @@ def test_grouped_log_objective_accuracy(O):
         f, _, _ = exact_fgh(rows, x)
         fs = O.obj_seg(rows, np.ones(n, np.uint8), np.array([0, n // 2 // 32 * 32, n], np.uint32), x)
         assert abs(fs - f) <= 2e-15 * abs(f) + 1e-16
+
+
+def seg_of(n):
+    return np.array([0, n], np.uint32)
+
+
+def test_seg_objective_outside_the_positive_normal_range(O):
+    """The segmented order's 16-site groups: where a group's product leaves
+    the positive normal range, its logs are taken one site at a time, so the
+    objective is the per-site sum (compensated reference within 1e-13) and
+    not the log of an underflowed or overflowed product."""
+    rng = np.random.default_rng(70)
+    rows = pair_rows(O, rng, 200)
+    valid = np.ones(200, np.uint8)
+    for x, eps in ((np.eye(9)[8] * 1e-30, 1e-300), (np.full(9, 1e200), 1e-8), (np.full(9, 1e-200), 1e-300)):
+        L = [math.fsum([float(x[k] * r[k]) for k in range(9)] + [eps]) for r in rows]
+        exact = -math.fsum(math.log(v) for v in L) / len(rows)
+        got = O.obj_seg(rows, valid, seg_of(200), x, eps)
+        assert abs(got - exact) <= 1e-13 * abs(exact), (x[0], got, exact)
+        ref = O.obj_ref(rows, x, eps)
+        assert abs(ref - exact) <= 1e-13 * abs(exact)
+
+
+def test_seg_objective_special_values(O):
+    """L = 0 gives +inf, a negative L NaN (log/scalar.rs:49-57) in both
+    orders, also with two negative L in one 16-site group (whose product is
+    positive)."""
+    rng = np.random.default_rng(71)
+    rows = pair_rows(O, rng, 64)
+    valid = np.ones(64, np.uint8)
+    zero = int(np.nonzero(rows[:, 0] == 0)[0][0])  # a site where c_0 = 0
+    x = np.eye(9)[0]
+    assert np.isposinf(O.obj_seg(rows, valid, seg_of(64), x, 0.0))
+    assert np.isposinf(O.obj_ref(rows, x, 0.0))
+    assert rows[zero, 0] == 0
+    neg = rows.copy()
+    neg[:, :] = np.abs(neg)
+    x = np.full(9, 1 / 9)
+    neg[3] = -neg[3]   # two negative L in the first group: product positive
+    neg[5] = -neg[5]
+    assert np.isnan(O.obj_seg(neg, valid, seg_of(64), x, 0.0))
+    assert np.isnan(O.obj_ref(neg, x, 0.0))
