@@ -504,6 +504,56 @@ def timed_fits(co, steps, warmup):
                        "algorithmic_flop_per_launch": flops / sweeps}}
 
 
+def hessian_forms_bench(args):
+    """The per-site Hessian accumulation in its vector and matrix-core forms
+    (--hessian-forms): time per pair and site of each form on the same
+    synthetic FP32 rows, the Hessian's share (each form against the same form
+    without it) and each form's error against FP64 sums of the same rows."""
+    import ctypes
+    from kestrel_amd import _lib
+    L = _lib.load()
+    n_pairs, n_sites = 30720, args.hessian_sites
+    rng = np.random.default_rng(0x4E55)
+    table = np.zeros((256, 12), np.float32)
+    table[:, :9] = rng.uniform(0.0, 1.0, (256, 9)).astype(np.float32)
+    table[:, 9] = (1.0 / rng.uniform(0.01, 1.0, 256)).astype(np.float32)  # 1/L of a site
+    names = {0: "lane_fp64", 1: "lane_fp32", 2: "mfma_bf16x2", 3: "mfma_bf16x3", 4: "lane_fp32_no_hessian",
+             5: "sites_across_lanes_no_hessian", 6: "sites_across_lanes_fp32"}
+    # FP64 sums of the FP32 rows of the first 256 pairs (every per-pair scale)
+    w = (np.float32(1.0) + (np.arange(256) % 64).astype(np.float32) / np.float32(64.0)).astype(np.float32)
+    r = (table[None, :, 9] * w[:, None]).astype(np.float32)            # 256 pairs x 256 table sites
+    cp = (table[None, :, :9] * r[:, :, None]).astype(np.float32)      # c' rows
+    reps = n_sites // 256
+    href = np.einsum("psi,psj->pij", cp.astype(np.float64), cp.astype(np.float64)) * reps
+    gref = cp.astype(np.float64).sum(1) * reps
+    il, jl = np.tril_indices(9)
+    order = np.lexsort((jl, il))  # row by row: (0,0), (1,0), (1,1), ...
+    il, jl = il[order], jl[order]
+    scale = np.sqrt(href[:, il, il] * href[:, jl, jl])
+    res = {}
+    for form in range(7):
+        nseg = n_sites // 4096
+        out = np.zeros((nseg, n_pairs, 54), np.float64 if form == 0 else np.float32)
+        ms = ctypes.c_double()
+        _lib.check(L.kestrel_measure_hessian_form(form, n_pairs, n_sites, table.ctypes.data_as(ctypes.c_void_p),
+                                                  out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(ms)))
+        e = {"ms": round(ms.value, 4), "ns_per_pair_site": round(ms.value * 1e6 / (n_pairs * n_sites), 5)}
+        tot = out[:, :256].astype(np.float64).sum(0)  # segments folded in FP64, as the sweeps' fold
+        if form not in (4, 5):
+            e["max_rel_error_hessian"] = float(np.max(np.abs(tot[:, :45] - href[:, il, jl]) / scale))
+        e["max_rel_error_gradient"] = float(np.max(np.abs(tot[:, 45:] - gref) / gref))
+        res[names[form]] = e
+        log(f"form {form} {names[form]}: {e}")
+    lane_h = res["lane_fp32"]["ns_per_pair_site"] - res["lane_fp32_no_hessian"]["ns_per_pair_site"]
+    line = {"metric": "hessian accumulation ns per pair and site", "n_pairs": n_pairs, "n_sites": n_sites,
+            "forms": res,
+            "hessian_share_ns": {"lane_fp32": round(lane_h, 5)}}
+    base = res["sites_across_lanes_no_hessian"]["ns_per_pair_site"]
+    for k in ("mfma_bf16x2", "mfma_bf16x3", "sites_across_lanes_fp32"):
+        line["hessian_share_ns"][k] = round(res[k]["ns_per_pair_site"] - base, 5)
+    print(json.dumps(line), flush=True)
+
+
 def measured_peak(mixed):
     """FP32 (mixed) or FP64 FMA-chain rate of this device in TFLOP/s, or None."""
     try:
@@ -557,6 +607,9 @@ def main():
     ap.add_argument("--cls", action="store_true", help="the CLS / Brier estimator of -f GT")
     ap.add_argument("--multi", type=float, default=0.0, metavar="FRACTION",
                     help="this fraction of the sites with 3 or 4 alleles (4:1), as one-hot likelihood rows")
+    ap.add_argument("--hessian-forms", action="store_true",
+                    help="time the per-site Hessian in its vector and matrix-core forms (synthetic rows)")
+    ap.add_argument("--hessian-sites", type=int, default=32768, help="sites per pair of --hessian-forms")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -573,6 +626,9 @@ def main():
         return
     if args.cls:
         cls_bench(args)
+        return
+    if args.hessian_forms:
+        hessian_forms_bench(args)
         return
 
     from kestrel_amd import Cohort, synth

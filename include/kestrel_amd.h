@@ -351,6 +351,24 @@ int kestrel_measure_fp64_peak(double* tflops);
  * (no counterpart in the reference). */
 int kestrel_measure_fp32_peak(double* tflops);
 
+/* Timing of the per-site Hessian accumulation H = sum_s c'_s c'_s^T (the 45
+ * terms of jacquard/grad_hess.rs:149-448) in one of the forms a sweep could
+ * give it, on synthetic scaled rows c'_k = table[s % 256][k] * (table[s %
+ * 256][9] * (1 + (pair % 64) / 64)) in FP32 (no counterpart in the reference:
+ * it measures the matrix-core form against the vector forms).  form: 0 one
+ * pair per lane in FP64, 1 one pair per lane in FP32, 2 one pair per
+ * wavefront with its sites across the lanes and H by bf16 hi/lo matrix-core
+ * products accumulated in FP32, 3 the same with a hi/mid/lo split, 4 form 1
+ * and 5 form 2 without the Hessian (rows and gradient sums only), 6 form 2's
+ * layout with the Hessian by FP32 FMA in each lane.  table:
+ * 256 x 12 floats on the host; n_pairs a multiple of 256, n_sites of 4096
+ * (the sites are summed in segments of 4096, as in the sweeps).  out (host):
+ * per segment, n_pairs rows of 54 values (45 Hessian terms, lower triangle by
+ * rows, then the 9 sums of c'), doubles for form 0 and floats otherwise;
+ * *ms: the second of two launches. */
+int kestrel_measure_hessian_form(int form, uint32_t n_pairs, uint32_t n_sites, const float* table, void* out,
+                                 double* ms);
+
 /* Achievable LDS read bandwidth of this device in TB/s (no counterpart in the
  * reference): conflict-free 16-byte reads by every lane, the bound of the CLS
  * per-pair kernel. */

@@ -64,6 +64,7 @@ EXPORTS = (
     "kestrel_serve_stop", "kestrel_last_fit_busy", "kestrel_cohort_create_gl", "kestrel_cohort_create_gl_multi",
     "kestrel_cohort_multi_info", "kestrel_cohort_create_gt_multi", "kestrel_measure_fp32_peak", "kestrel_autotune",
     "kestrel_measure_lds", "kestrel_last_fit_recondition", "kestrel_evaluate_noalloc",
+    "kestrel_measure_hessian_form",
 )
 
 
@@ -131,6 +132,7 @@ def load():
     L.kestrel_measure_fp64_peak.argtypes = [P]
     L.kestrel_measure_fp32_peak.argtypes = [P]
     L.kestrel_measure_lds.argtypes = [P]
+    L.kestrel_measure_hessian_form.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, P, P, P]
     L.kestrel_autotune.argtypes = [P, P, P, ctypes.c_size_t, ctypes.c_uint64, P]
     L.kestrel_measure_copy.argtypes = [ctypes.c_uint64, P]
     L.kestrel_serve_start.argtypes = [P, P, ctypes.c_uint32, ctypes.c_uint64]
