@@ -554,6 +554,42 @@ def hessian_forms_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def small_jobs(co, n, reps=5):
+    """Fits of few pairs (one query against part or all of the cohort, a few
+    queries against all) in both layouts: one pair per lane, and one pair per
+    wavefront with its sites across the lanes (option "pair_lanes" 64).  Wall
+    milliseconds per fit (median of reps, after one untimed fit) and the
+    wavefront form's errors against the per-lane form."""
+    rng = np.random.default_rng(0x5A11)
+    jobs = []
+    for q, m in ((1, 100), (1, n - 1), (4, n - 1), (32, n - 1)):
+        queries = rng.choice(n, q, replace=False)
+        pairs = []
+        for x in queries:
+            others = rng.choice(np.setdiff1d(np.arange(n), [x]), m, replace=False)
+            pairs.append(np.stack([np.full(m, x), others], 1))
+        jobs.append((f"{q} x {m}", np.concatenate(pairs).astype(np.uint32)))
+    res = []
+    for name, pairs in jobs:
+        row = {"job": name, "pairs": int(len(pairs))}
+        outs = {}
+        for lanes in (1, 64):
+            co.set_option("pair_lanes", lanes)
+            co.fit_pairs(pairs=pairs)
+            ts = []
+            for _ in range(reps):
+                t = time.perf_counter()
+                outs[lanes] = co.fit_pairs(pairs=pairs)
+                ts.append((time.perf_counter() - t) * 1e3)
+            row[f"ms_pair_lanes_{lanes}"] = round(float(np.median(ts)), 2)
+        co.set_option("pair_lanes", 1)
+        row["speedup"] = round(row["ms_pair_lanes_1"] / row["ms_pair_lanes_64"], 2)
+        row["errors_vs_per_lane"] = record_errors(outs[64], outs[1])
+        log(f"small job {row}")
+        res.append(row)
+    return res
+
+
 def measured_peak(mixed):
     """FP32 (mixed) or FP64 FMA-chain rate of this device in TFLOP/s, or None."""
     try:
@@ -598,6 +634,8 @@ def main():
                     help="form of the serving workload (its configuration names the FP32-mixed kernels)")
     ap.add_argument("--no-mixed-leg", action="store_true",
                     help="skip the FP32-mixed fit of the same workload in the default run")
+    ap.add_argument("--no-small-jobs", action="store_true",
+                    help="skip the few-pair fits in both layouts (one pair per lane / per wavefront)")
     ap.add_argument("--no-serving-leg", action="store_true",
                     help="skip the serving workload (C5) in the default run")
     ap.add_argument("--load", type=float, default=0.7, help="Poisson offered load / saturated capacity")
@@ -736,6 +774,9 @@ def main():
             "errors_vs_fp64_form": record_errors(fm["out"], out),
         }
         log(f"mixed form: {mixed_leg}")
+    small = None
+    if not mixed and not args.soft and not buckets and not args.no_small_jobs:
+        small = small_jobs(co, n)
     cpu, errors = None, None
     if buckets and args.cpu_budget > 0:
         cpu, _, _ = cpu_baseline(None, None, args.cpu_budget, buckets=buckets)
@@ -778,6 +819,7 @@ def main():
         **({"errors": errors} if errors else {}),
         **({"mixed": mixed_leg} if mixed_leg else {}),
         **({"serving": serving_leg} if serving_leg else {}),
+        **({"small_jobs": small} if small else {}),
         "iterations_mean": round(float(out["iters"].mean()), 3),
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
     }

@@ -70,7 +70,13 @@ int repack_kept(kestrel_cohort* c, const uint8_t* dpacked, size_t row_stride_byt
 // One evaluation pass over every part of the cohort: the main part's segments,
 // then those of the tri/tetra-allelic parts, into consecutive partials.
 int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
-                  uint32_t ns, double eps, double* partial, hipStream_t st, bool skip_masked = false) {
+                  uint32_t ns, double eps, double* partial, hipStream_t st, bool skip_masked = false,
+                  bool lanes = false) {
+    if (lanes) {  // 2-bit FP64 cohorts only (checked by the caller)
+        HIPCHK(klaunch::sweep_lanes(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, sx, sy, pt, active, ns, eps,
+                                    partial, st));
+        return KESTREL_OK;
+    }
     HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, sx, sy, pt, active, ns, eps, partial, st,
                           c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride, skip_masked, c->mixed));
     size_t base = c->nseg;
@@ -609,6 +615,11 @@ int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value) {
         c->mixed = value == KESTREL_MIXED;
         return KESTREL_OK;
     }
+    if (!strcmp(key, "pair_lanes")) {
+        if (value != 1 && value != 64 && value != -1) return fail(KESTREL_EINVAL, "pair_lanes must be 1, 64 or -1");
+        c->opt_pair_lanes = value;
+        return KESTREL_OK;
+    }
     if (!strcmp(key, "skip_masked")) {
         if (value < -1 || value > 1) return fail(KESTREL_EINVAL, "skip_masked must be -1, 0 or 1");
         c->opt_skip_masked = value;
@@ -709,7 +720,16 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     // (automatic for the FP64 form only: the mixed form's pass holds two
     // wavefronts per SIMD instead of three, and measured 1,241,070 against
     // 1,266,335 pair-fits/s without it on C3)
-    const bool use_runs = !c->soft && c->parts.empty() &&
+    // a pair's sites across a wavefront (option "pair_lanes" 64): 2-bit FP64
+    // cohorts; claims in list order (the pass has no shared sample to skip on)
+    // (-1: automatic, for fits of at most LANES_AUTO_PAIRS pairs: measured on
+    // 1,000 x 100,000 sites, 100 / 999 / 3,996 / 31,968 pairs fit in 1.9 /
+    // 8.4 / 24.6 / 162 ms against 18.5 / 21.5 / 38.0 / 110 ms one pair a lane)
+    const bool lanes_form = !c->soft && c->parts.empty() && !c->mixed;
+    if (c->opt_pair_lanes == 64 && !lanes_form)
+        return fail(KESTREL_EINVAL, "pair_lanes 64 covers FP64 fits of 2-bit (hard-call) cohorts");
+    const bool lanes = c->opt_pair_lanes == 64 || (c->opt_pair_lanes < 0 && lanes_form && n_pairs <= LANES_AUTO_PAIRS);
+    const bool use_runs = !lanes && !c->soft && c->parts.empty() &&
                           (c->opt_skip_masked == 1 ||
                            (c->opt_skip_masked < 0 && !c->mixed && c->missing_frac >= 0.02));
     // pairs in flight per group: fill the device unless set (twice with runs:
@@ -863,7 +883,7 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
             int rs = KESTREL_OK;
             for (int i = 0; i < BATCH && rs == KESTREL_OK; i++) {
                 rs = launch_sweeps(c, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, tune->epsilon, G.part.p, G.stream,
-                                   use_runs);
+                                   use_runs, lanes);
                 if (rs == KESTREL_OK) {
                     const hipError_t e = klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p,
                                                        c->nseg_total(), dpairs.p, n_pairs, dorder_p, dnext.p,
@@ -908,7 +928,7 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
                 if (G.live_before <= 0) continue;
                 HIPCHK(hipEventRecord(G.ev[2 * i], G.stream));
                 int rs = launch_sweeps(c, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, tune->epsilon, G.part.p, G.stream,
-                                       use_runs);
+                                       use_runs, lanes);
                 if (rs) return rs;
                 HIPCHK(hipEventRecord(G.ev[2 * i + 1], G.stream));
                 HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, c->nseg_total(), dpairs.p,

@@ -94,6 +94,9 @@ struct GtPart {
     DevBuf<double> af;
     DevBuf<double> mtab;
 };
+// option "pair_lanes" -1: fits of at most this many pairs take the
+// one-pair-per-wavefront sweep
+constexpr size_t LANES_AUTO_PAIRS = 4096;
 // The mixed-precision form's condition estimates: a converged record whose
 // estimate from the FP32-summed Hessian is not below this is estimated again
 // from an FP64 evaluation at its final point.
@@ -164,6 +167,7 @@ struct kestrel_cohort {
     double lmax = 1.0;             // largest likelihood of the cohort's rows (1 for hard calls)
     bool mixed = false;            // precision KESTREL_MIXED: sweeps by sweep_mixed
     int64_t opt_pair_tile = -1;    // claim order in tiles of this many samples (0 list order, -1 automatic)
+    int64_t opt_pair_lanes = 1;    // lanes a pair: 1, 64 (its sites across a wavefront; 2-bit FP64 fits), -1 auto
     hipStream_t stream2 = nullptr; // second stream of a fit
     double last_sweep_ms = 0, last_wall_ms = 0, last_busy_ms = 0;
     uint64_t last_sweeps = 0;

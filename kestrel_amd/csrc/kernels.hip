@@ -595,6 +595,128 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
     }
 }
 
+// c_s of one site for the 2-bit codes (ax, ay) (both called), built in
+// registers from the site's frequencies: the value lut_block writes into
+// table row 4*ax + ay, bit for bit.  Swapping p and q maps genotype pair
+// (gx, gy) to (2 - gx, 2 - gy), and every product of lut_block to the same
+// two operands in the other order (p2q of (q, p) is q2*p = pq2 of (p, q), and
+// so on), so the five pairs with 3*ax + ay <= 4 cover the nine; each
+// component is then a select among values computed once.
+__device__ __forceinline__ void cs_from_codes(double p, double q, uint32_t ax, uint32_t ay, double* r) {
+    int part = (int)(3 * ax + ay);
+    const bool sw = part > 4;
+    const double a = sw ? q : p, b = sw ? p : q;
+    part = sw ? 8 - part : part;
+    const double a2 = a * a, a3 = a2 * a, b2 = b * b, ab = a * b;
+    const double a2b = a2 * b, ab2 = a * b2, a2b2 = a2 * b2, a3b = a3 * b;
+    const double tab = ab + ab, ta2b = a2b + a2b, ta3b = a3b + a3b;
+    const double a4 = a2 * a2, tab2 = tab * tab;
+    const bool p0 = part == 0, p1 = part == 1, p2 = part == 2, p3 = part == 3, p4 = part == 4;
+    r[0] = p0 ? a : 0.0;
+    r[1] = p0 ? a2 : p2 ? ab : 0.0;
+    r[2] = p0 ? a2 : p1 ? ab : 0.0;
+    r[3] = p0 ? a3 : p1 ? ta2b : p2 ? ab2 : 0.0;
+    r[4] = p0 ? a2 : p3 ? ab : 0.0;
+    r[5] = p0 ? a3 : p2 ? a2b : p3 ? ta2b : 0.0;
+    r[6] = p0 ? a2 : p4 ? tab : 0.0;
+    r[7] = p0 ? a3 : (p1 || p3) ? a2b : p4 ? ab : 0.0;
+    r[8] = p0 ? a4 : (p1 || p3) ? ta3b : p2 ? a2b2 : tab2;
+}
+
+// The sweep with a pair's sites across the lanes of a wavefront (option
+// "pair_lanes" 64), for fits with fewer pairs than the device has lanes: one
+// pair per wavefront, four a workgroup (wavefront w of block b takes slot
+// 4b + w), blockIdx.y = site segment.  Lane l takes the sites s0 + l + 64 t
+// of its segment, t = 0, 1, ..., builds c_s in registers from the 2-bit codes
+// and the site's frequencies (cs_from_codes), and keeps its own sums in site
+// order: the log product over its usable sites with a log after every 16 of
+// its steps (and after its last step), the gradient and the Hessian as in
+// site_update.  A site with a missing call is passed over (c = 0, L = 1 adds
+// nothing: the argument of sweep_fp64<true>).  At the segment's end the 64
+// lanes' sums are added in a butterfly (xor 32, 16, ..., 1: lane i adds lane
+// i ^ o's value, and both partners hold the same sum since IEEE addition
+// commutes), and lane 0 writes the partials in sweep_fp64's layout, so the
+// fold and the step are unchanged.  The summation order differs from
+// sweep_fp64's, so records differ from the one-pair-per-lane form's in the
+// last bits; the oracle restates this order.
+__global__ __launch_bounds__(SWEEP_TPB) void sweep_lanes_fp64(const uint32_t* __restrict__ geno, uint32_t row_words,
+                                                               const double* __restrict__ af,
+                                                               const uint32_t* __restrict__ seg_bounds,
+                                                               const uint32_t* __restrict__ sx,
+                                                               const uint32_t* __restrict__ sy,
+                                                               const double* __restrict__ pt,
+                                                               const int* __restrict__ active, uint32_t nslots,
+                                                               double eps, double* __restrict__ partial) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t slot = blockIdx.x * (SWEEP_TPB / 64) +
+                          (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (slot >= nslots || active[slot] == 0) return;  // uniform over the wavefront
+    const uint32_t seg = blockIdx.y;
+    const uint32_t x = sx[slot], y = sy[slot];
+    double D[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) D[k] = pt[(size_t)k * nslots + slot];
+    const uint32_t* rowx = geno + (size_t)x * row_words;
+    const uint32_t* rowy = geno + (size_t)y * row_words;
+    const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
+
+    double F = 0.0, N = 0.0, P = 1.0, G[9], H[45];
+#pragma unroll
+    for (int k = 0; k < 9; k++) G[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 45; k++) H[k] = 0.0;
+    const uint32_t steps = (s1 - s0 + 63) / 64;
+    for (uint32_t t = 0; t < steps; t++) {
+        const uint32_t s = s0 + 64 * t + lane;
+        if (s < s1) {
+            const uint32_t ax = (rowx[s >> 4] >> (2 * (s & 15u))) & 3u;
+            const uint32_t ay = (rowy[s >> 4] >> (2 * (s & 15u))) & 3u;
+            if (ax != 3u && ay != 3u) {
+                const double2 f = *reinterpret_cast<const double2*>(af + 2 * (size_t)s);
+                double cv[9];
+                cs_from_codes(f.x, f.y, ax, ay, cv);
+                double L = eps;
+#pragma unroll
+                for (int j = 0; j < 9; j++) L = fma(D[j], cv[j], L);
+                P = P * L;
+                const double r = recip_rn(L);
+#pragma unroll
+                for (int j = 0; j < 9; j++) G[j] = fma(cv[j], r, G[j]);
+#pragma unroll
+                for (int j = 0; j < 9; j++) cv[j] = cv[j] * r;
+                int id = 0;
+#pragma unroll
+                for (int a = 0; a < 9; a++)
+#pragma unroll
+                    for (int bb = a; bb < 9; bb++, id++) H[id] = fma(cv[a], cv[bb], H[id]);
+                N += 1.0;
+            }
+        }
+        if ((t & (LOG_GROUP - 1)) == LOG_GROUP - 1 || t + 1 == steps) {
+            F = F + sleef_log_pos(P);
+            P = 1.0;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        F = F + __shfl_xor(F, o, 64);
+        N = N + __shfl_xor(N, o, 64);
+#pragma unroll
+        for (int k = 0; k < 9; k++) G[k] = G[k] + __shfl_xor(G[k], o, 64);
+#pragma unroll
+        for (int k = 0; k < 45; k++) H[k] = H[k] + __shfl_xor(H[k], o, 64);
+    }
+    if (lane == 0) {
+        double* out = partial + (size_t)seg * NPART * nslots + slot;
+        out[0] = F;
+#pragma unroll
+        for (int k = 0; k < 9; k++) out[(size_t)(1 + k) * nslots] = G[k];
+#pragma unroll
+        for (int k = 0; k < 45; k++) out[(size_t)(10 + k) * nslots] = H[k];
+        out[(size_t)55 * nslots] = N;
+    }
+}
+
 // The mixed-precision form (precision KESTREL_MIXED): the pass of sweep_fp64
 // with the c_s table in FP32 and the Hessian in FP32 -- 1/L by v_rcp_f32, the
 // scaled c' = c/L and the 45 Hessian terms, summed in FP32 over the segment
@@ -2592,6 +2714,16 @@ hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, con
     else
         hipLaunchKernelGGL(sweep_fp64<false>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, af, seg_bounds, sx, sy,
                            pt, active, nslots, eps, partial);
+    return hipGetLastError();
+}
+
+hipError_t sweep_lanes(const uint32_t* geno, uint32_t row_words, const double* af, const uint32_t* seg_bounds,
+                       uint32_t nseg, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
+                       uint32_t nslots, double eps, double* partial, hipStream_t st) {
+    constexpr uint32_t per_block = SWEEP_TPB / 64;
+    dim3 grid((nslots + per_block - 1) / per_block, nseg);
+    hipLaunchKernelGGL(sweep_lanes_fp64, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, af, seg_bounds, sx, sy, pt,
+                       active, nslots, eps, partial);
     return hipGetLastError();
 }
 

@@ -17,6 +17,8 @@
  *                  compared bit for bit.
  *   KO_ORDER_FAST: KO_ORDER_REF in the reference's AVX-512 loop shape (the
  *                  CPU baseline's timing form; same bits).
+ *   KO_ORDER_LANES: the segments of KO_ORDER_SEG, each summed by 64 strided
+ *                  lanes and a butterfly (the device's wavefront form).
  *
  * Pins (tests/test_oracle_*.py): the identity-state probabilities against an
  * exact-rational brute force, the two reference encodings of c_s against each
@@ -46,7 +48,12 @@ extern "C" {
  * log, Hessian in three 15-accumulator passes per tile; ko_lanes_fast.c):
  * the same bits as KO_ORDER_REF, for timing the CPU baseline.  Hard-call
  * cohorts on AVX-512 hosts; elsewhere it runs KO_ORDER_REF. */
-enum { KO_ORDER_REF = 0, KO_ORDER_SEG = 1, KO_ORDER_FAST = 2 };
+/* KO_ORDER_LANES: the wavefront order of the device's one-pair-per-wavefront
+ * sweep (option "pair_lanes" 64): per segment, 64 lanes each sum the sites
+ * s0 + l + 64 t in order (a log per 16 steps and at the last), then a
+ * butterfly adds the lanes (o = 32 .. 1: v[i] + v[i ^ o]); rows/valid/seg as
+ * KO_ORDER_SEG. */
+enum { KO_ORDER_REF = 0, KO_ORDER_SEG = 1, KO_ORDER_FAST = 2, KO_ORDER_LANES = 3 };
 
 /* sqp.rs:16-47 Tuneables */
 typedef struct {
@@ -98,6 +105,12 @@ double ko_jacq_obj_seg(const double* rows, const uint8_t* valid, size_t V, const
                        size_t nseg, const double x[9], double eps);
 void ko_jacq_gh_seg(const double* rows, const uint8_t* valid, size_t V, const uint32_t* seg, size_t nseg,
                     const double x[9], double eps, double g[9], double h[81]);
+
+/* wavefront order (KO_ORDER_LANES), arguments as the segmented order's */
+double ko_jacq_obj_lanes(const double* rows, const uint8_t* valid, size_t V, const uint32_t* seg,
+                         size_t nseg, const double x[9], double eps);
+void ko_jacq_gh_lanes(const double* rows, const uint8_t* valid, size_t V, const uint32_t* seg, size_t nseg,
+                      const double x[9], double eps, double g[9], double h[81]);
 
 /* cholesky.rs:6-65, 72-102, 105-140 (matrices are 9x9 row-major, first n used) */
 void ko_modify_gmw(int n, const double* a, double* l, double* d);

@@ -448,6 +448,135 @@ void ko_jacq_gh_seg(const double* rows, const uint8_t* valid, size_t V, const ui
         }
 }
 
+/* Wavefront order (the device's sweep_lanes_fp64, fits of option
+ * "pair_lanes" 64): per segment, "lane" l of 64 takes the sites s0 + l + 64 t
+ * (t = 0, 1, ...) and sums its usable sites in that order from zero with the
+ * per-site operations of seg_sums; its log product is logged after every 16
+ * of its steps and after its last (a step whose site is past the segment or
+ * not usable multiplies by nothing, so a group may log 1.0), with seg_sums'
+ * per-site fallback where a group is not a positive normal product.  The 64
+ * lanes' sums are then added in a butterfly: for o = 32, 16, ..., 1 every
+ * lane i takes v[i] + v[i ^ o]; lane 0's sums are the segment's, folded left
+ * to right as in seg_sums. */
+#define KO_WAVE 64
+static void lanes_sums(const double* rows, const uint8_t* valid, const uint32_t* seg, size_t nseg,
+                       const double x[9], double eps, int want_obj, int want_gh, double* fs, double gs[9],
+                       double hs[45], size_t* n_out) {
+    double F = 0.0, G[9] = {0}, H[45] = {0};
+    size_t n = 0;
+    double (*lf)[1] = malloc(sizeof(double[KO_WAVE][1]));
+    double (*lg)[9] = malloc(sizeof(double[KO_WAVE][9]));
+    double (*lh)[45] = malloc(sizeof(double[KO_WAVE][45]));
+    double (*tf)[1] = malloc(sizeof(double[KO_WAVE][1]));
+    double (*tg)[9] = malloc(sizeof(double[KO_WAVE][9]));
+    double (*th)[45] = malloc(sizeof(double[KO_WAVE][45]));
+    for (size_t k = 0; k < nseg; k++) {
+        const uint32_t s0 = seg[k], s1 = seg[k + 1];
+        const uint32_t steps = (s1 - s0 + KO_WAVE - 1) / KO_WAVE;
+        for (int l = 0; l < KO_WAVE; l++) {
+            double zf = 0.0, zg[9] = {0}, zh[45] = {0};
+            double prod = 1.0, gl[KO_LOG_GROUP];
+            int ng = 0, normal = 1;
+            for (uint32_t t = 0; t < steps; t++) {
+                const uint32_t s = s0 + KO_WAVE * t + (uint32_t)l;
+                if (s < s1 && valid[s]) {
+                    n++;
+                    const double* c = rows + (size_t)s * 9;
+                    double d = eps;
+                    for (int col = 0; col < 9; col++) d = fma(x[col], c[col], d);
+                    if (want_obj) {
+                        prod = prod * d;
+                        gl[ng++] = d;
+                        normal = normal && d >= DBL_MIN && d <= DBL_MAX;
+                    }
+                    if (want_gh) {
+                        const double r = 1.0 / d;
+                        double cs[9];
+                        for (int col = 0; col < 9; col++) zg[col] = fma(c[col], r, zg[col]);
+                        for (int col = 0; col < 9; col++) cs[col] = c[col] * r;
+                        int id = 0;
+                        for (int a = 0; a < 9; a++)
+                            for (int b = a; b < 9; b++, id++) zh[id] = fma(cs[a], cs[b], zh[id]);
+                    }
+                }
+                if (t % KO_LOG_GROUP == KO_LOG_GROUP - 1 || t + 1 == steps) {
+                    if (want_obj) {
+                        if (normal && prod >= DBL_MIN && prod <= DBL_MAX)
+                            zf = zf + ko_log(prod);
+                        else
+                            for (int q = 0; q < ng; q++) zf = zf + ko_log(gl[q]);
+                    }
+                    prod = 1.0;
+                    ng = 0;
+                    normal = 1;
+                }
+            }
+            lf[l][0] = zf;
+            memcpy(lg[l], zg, sizeof zg);
+            memcpy(lh[l], zh, sizeof zh);
+        }
+        for (int o = KO_WAVE / 2; o > 0; o >>= 1) {
+            for (int l = 0; l < KO_WAVE; l++) {
+                tf[l][0] = lf[l][0] + lf[l ^ o][0];
+                for (int i = 0; i < 9; i++) tg[l][i] = lg[l][i] + lg[l ^ o][i];
+                for (int i = 0; i < 45; i++) th[l][i] = lh[l][i] + lh[l ^ o][i];
+            }
+            memcpy(lf, tf, sizeof(double[KO_WAVE][1]));
+            memcpy(lg, tg, sizeof(double[KO_WAVE][9]));
+            memcpy(lh, th, sizeof(double[KO_WAVE][45]));
+        }
+        if (k == 0) {
+            F = lf[0][0];
+            memcpy(G, lg[0], sizeof G);
+            memcpy(H, lh[0], sizeof H);
+        } else {
+            F = F + lf[0][0];
+            for (int i = 0; i < 9; i++) G[i] = G[i] + lg[0][i];
+            for (int i = 0; i < 45; i++) H[i] = H[i] + lh[0][i];
+        }
+    }
+    free(lf);
+    free(lg);
+    free(lh);
+    free(tf);
+    free(tg);
+    free(th);
+    *fs = F;
+    memcpy(gs, G, sizeof G);
+    memcpy(hs, H, sizeof H);
+    *n_out = n;
+}
+
+static void gh_from_sums(const double G[9], const double H[45], size_t n, double g[9], double h[81]) {
+    const double nn = (double)n;
+    for (int i = 0; i < 9; i++) g[i] = -G[i] / nn;
+    int id = 0;
+    for (int a = 0; a < 9; a++)
+        for (int b = a; b < 9; b++, id++) {
+            double v = H[id] / nn;
+            h[a * 9 + b] = v;
+            h[b * 9 + a] = v;
+        }
+}
+
+double ko_jacq_obj_lanes(const double* rows, const uint8_t* valid, size_t V, const uint32_t* seg, size_t nseg,
+                         const double x[9], double eps) {
+    (void)V;
+    double F, G[9], H[45];
+    size_t n;
+    lanes_sums(rows, valid, seg, nseg, x, eps, 1, 0, &F, G, H, &n);
+    return -F / (double)n;
+}
+
+void ko_jacq_gh_lanes(const double* rows, const uint8_t* valid, size_t V, const uint32_t* seg, size_t nseg,
+                      const double x[9], double eps, double g[9], double h[81]) {
+    (void)V;
+    double F, G[9], H[45];
+    size_t n;
+    lanes_sums(rows, valid, seg, nseg, x, eps, 0, 1, &F, G, H, &n);
+    gh_from_sums(G, H, n, g, h);
+}
+
 /* ------------------------------------------------------ linear algebra */
 
 #define MX 9
@@ -779,6 +908,7 @@ static double pair_obj(void* vc, const double* x, double eps) {
     pair_ctx* c = (pair_ctx*)vc;
     if (c->order == KO_ORDER_FAST) return ko_jacq_obj_lv_fast(&c->lv, x, eps);
     if (c->order == KO_ORDER_REF) return ko_ops()->jacq_obj(&c->lv, x, eps);
+    if (c->order == KO_ORDER_LANES) return ko_jacq_obj_lanes(c->rows, c->valid, 0, c->seg, c->nseg, x, eps);
     return ko_jacq_obj_seg(c->rows, c->valid, 0, c->seg, c->nseg, x, eps);
 }
 
@@ -787,6 +917,7 @@ static void pair_gh(void* vc, const double* x, double eps, double* g, double* h)
     double hh[81];
     if (c->order == KO_ORDER_FAST) ko_jacq_gh_lv_fast(&c->lv, x, eps, g, hh);
     else if (c->order == KO_ORDER_REF) ko_ops()->jacq_gh(&c->lv, x, eps, g, hh);
+    else if (c->order == KO_ORDER_LANES) ko_jacq_gh_lanes(c->rows, c->valid, 0, c->seg, c->nseg, x, eps, g, hh);
     else ko_jacq_gh_seg(c->rows, c->valid, 0, c->seg, c->nseg, x, eps, g, hh);
     memcpy(h, hh, sizeof hh);
 }

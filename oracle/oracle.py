@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "_build", "libkestrel_oracle.so")
 ORDER_REF = 0
 ORDER_SEG = 1
 ORDER_FAST = 2  # ORDER_REF in the reference's AVX-512 loop shape (same bits; timing form)
+ORDER_LANES = 3  # the segments of ORDER_SEG summed by 64 strided lanes and a butterfly
 
 
 class Tune(ctypes.Structure):
@@ -79,6 +80,9 @@ def lib():
         L.ko_jacq_obj_seg.restype = D
         L.ko_jacq_obj_seg.argtypes = [P, P, SZ, P, SZ, P, D]
         L.ko_jacq_gh_seg.argtypes = [P, P, SZ, P, SZ, P, D, P, P]
+        L.ko_jacq_obj_lanes.restype = D
+        L.ko_jacq_obj_lanes.argtypes = [P, P, SZ, P, SZ, P, D]
+        L.ko_jacq_gh_lanes.argtypes = [P, P, SZ, P, SZ, P, D, P, P]
         L.ko_modify_gmw.argtypes = [I, P, P, P]
         L.ko_unmodified.argtypes = [I, P, P, P]
         L.ko_solve_ldl.argtypes = [I, P, P, P]
@@ -191,6 +195,18 @@ def gh_seg(rows, valid, seg, x, eps=1e-8):
     return g, h
 
 
+def obj_lanes(rows, valid, seg, x, eps=1e-8) -> float:
+    rows, x, valid, seg = f64(rows), f64(x), np.ascontiguousarray(valid, np.uint8), _seg(seg)
+    return lib().ko_jacq_obj_lanes(_p(rows), _p(valid), rows.shape[0], _p(seg), len(seg) - 1, _p(x), eps)
+
+
+def gh_lanes(rows, valid, seg, x, eps=1e-8):
+    rows, x, valid, seg = f64(rows), f64(x), np.ascontiguousarray(valid, np.uint8), _seg(seg)
+    g, h = np.zeros(9), np.zeros((9, 9))
+    lib().ko_jacq_gh_lanes(_p(rows), _p(valid), rows.shape[0], _p(seg), len(seg) - 1, _p(x), eps, _p(g), _p(h))
+    return g, h
+
+
 def modify_gmw(a):
     a = f64(a)
     n = a.shape[0]
@@ -228,7 +244,7 @@ def condition(x, g, h, tol=1e-8) -> float:
 
 
 def fit_rows(rows, self_pair=False, order=ORDER_REF, valid=None, seg=None, iterations=100):
-    """Fit one pair from its c rows: compacted (REF) or per-site with valid (SEG)."""
+    """Fit one pair from its c rows: compacted (REF) or per-site with valid (SEG, LANES)."""
     rows = f64(rows)
     out = np.zeros(1, OUTPUT_DTYPE)
     t = tune(iterations, 10)
@@ -238,7 +254,7 @@ def fit_rows(rows, self_pair=False, order=ORDER_REF, valid=None, seg=None, itera
     else:
         valid = np.ascontiguousarray(valid, np.uint8)
         seg = _seg(seg)
-        lib().ko_fit_pair_rows(ORDER_SEG, _p(rows), rows.shape[0], _p(valid), _p(seg), len(seg) - 1,
+        lib().ko_fit_pair_rows(order, _p(rows), rows.shape[0], _p(valid), _p(seg), len(seg) - 1,
                                int(self_pair), ctypes.byref(t), _p(out))
     return out[0]
 
