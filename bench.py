@@ -39,8 +39,12 @@ FP64_VALU_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
 LDS_PEAK_TBS = 157.3  # gfx950 LDS reads: 256 B/clk/CU x 256 CUs x 2.4 GHz
 FLOP_PER_SITE_EVAL = 156  # 136 (gradient + Hessian) + 20 (objective, log counted as 1)
 # HBM bytes per sweep_fp64 launch on the default workload, from rocprofv3
-# FETCH_SIZE / WRITE_SIZE scaled by a plain 256 MiB copy (profiles/sweep_fp64_c2.md)
-SWEEP_TRAFFIC_BYTES_C2 = 1.567e8
+# FETCH_SIZE / WRITE_SIZE scaled by a plain 256 MiB copy (profiles/sweep_fp64_c2.md);
+# counters cannot be read inside this process, so the line carries the
+# profiled figure and says so
+SWEEP_TRAFFIC_BYTES_C2 = 1.571e8
+SWEEP_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of sweep_fp64 on this workload, scaled by a "
+                        "256 MiB copy_kernel (profiles/sweep_fp64_c2.md); profiled, not measured in this run")
 
 
 def log(*a):
@@ -737,6 +741,8 @@ def main():
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_VALU_PEAK_TFLOPS, 4),
                 "traffic": (SWEEP_TRAFFIC_BYTES_C2 if args.config == "c2" and not args.segment_sites
                             and not args.soft and not buckets and args.precision == "fp64" else None),
+                "traffic_source": (SWEEP_TRAFFIC_SOURCE if args.config == "c2" and not args.segment_sites
+                                   and not args.soft and not buckets and args.precision == "fp64" else None),
                 "kernel": ("sweep_gl + sweep_glm" if buckets
                            else ("sweep_gl_mixed" if args.precision == "mixed" else "sweep_gl") if args.soft
                            else "sweep_mixed" if args.precision == "mixed" else "sweep_fp64"),
