@@ -1030,6 +1030,10 @@ int kestrel_autotune(kestrel_cohort* c, const kestrel_tune* tune, const uint32_t
     if (n_pairs < SWEEP_TPB) return fail(KESTREL_EINVAL, "need at least %d sample pairs", SWEEP_TPB);
     if (job_pairs == 0) job_pairs = n_pairs;
     const int64_t keep_slots = c->opt_slots, keep_streams = c->opt_streams, keep_batch = c->opt_batch;
+    // the shapes tuned are those of the one-pair-per-lane sweep, whatever the
+    // sample's size
+    const int64_t keep_lanes = c->opt_pair_lanes;
+    c->opt_pair_lanes = 1;
     std::vector<kestrel_output> out(n_pairs);
     uint32_t trials = 0;
     // Predicted milliseconds of a job of job_pairs pairs with one shape, from
@@ -1060,6 +1064,7 @@ int kestrel_autotune(kestrel_cohort* c, const kestrel_tune* tune, const uint32_t
         return KESTREL_OK;
     };
     auto restore = [&] {
+        c->opt_pair_lanes = keep_lanes;
         c->opt_slots = keep_slots;
         c->opt_streams = keep_streams;
         c->opt_batch = keep_batch;
@@ -1098,6 +1103,7 @@ int kestrel_autotune(kestrel_cohort* c, const kestrel_tune* tune, const uint32_t
             }
         }
     }
+    c->opt_pair_lanes = keep_lanes;
     c->opt_slots = best[0];
     c->opt_streams = best[1];
     c->opt_batch = best[2];

@@ -70,6 +70,26 @@ def test_lanes_c1_and_pair_list(O):
     close(out, O.fit_pairs(kc, ka, order=O.ORDER_REF))
 
 
+def test_lanes_graph_two_streams_and_autotune(O):
+    say("case: pair_lanes 64 with HIP-graph relaunch, two streams and small slot groups; autotune keeps the option")
+    c = synth.generate(36, 4000, seed=91, missing=0.03, n_family_pairs=2)
+    with Cohort(c.codes) as co:
+        af, kept = co.allele_freqs()
+        co.set_option("pair_lanes", 64)
+        eager = co.fit_pairs()
+        for opts in ({"graph": 1}, {"graph": 0, "streams": 2, "slots": 256}, {"batch": 3, "slots": 512}):
+            for k, v in opts.items():
+                co.set_option(k, v)
+            assert_bitexact(co.fit_pairs(), eager)
+            assert_bitexact(co.fit_pairs(), eager)  # a graph replayed again
+        co.set_option("graph", 0)
+        co.autotune(sample=600)
+        assert_bitexact(co.fit_pairs(), eager)  # still the wavefront form after tuning
+    kc, ka = np.ascontiguousarray(c.codes[:, kept]), af[kept]
+    seg = np.minimum(segment_bounds(int(kept.sum())), kc.shape[1]).astype(np.uint32)
+    assert_bitexact(eager, O.fit_pairs(kc, ka, order=O.ORDER_LANES, seg=seg))
+
+
 def test_lanes_refused_where_not_covered():
     say("case: pair_lanes 64 on soft and mixed cohorts, bad values")
     c = synth.generate(12, 800, seed=3)
