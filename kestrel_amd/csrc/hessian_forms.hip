@@ -279,18 +279,22 @@ extern "C" int kestrel_measure_hessian_form(int form, uint32_t n_pairs, uint32_t
         }
         return hipGetLastError();
     };
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    struct Events {  // destroyed on every return path
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events() {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    HIPCHK(hipEventCreate(&ev.e[0]));
+    HIPCHK(hipEventCreate(&ev.e[1]));
     HIPCHK(launch());  // warm up
-    HIPCHK(hipEventRecord(e0, nullptr));
+    HIPCHK(hipEventRecord(ev.e[0], nullptr));
     HIPCHK(launch());
-    HIPCHK(hipEventRecord(e1, nullptr));
-    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventRecord(ev.e[1], nullptr));
+    HIPCHK(hipEventSynchronize(ev.e[1]));
     float t = 0;
-    HIPCHK(hipEventElapsedTime(&t, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIPCHK(hipEventElapsedTime(&t, ev.e[0], ev.e[1]));
     *ms = t;
     HIPCHK(hipMemcpy(out, dout.p, (size_t)nseg * n_pairs * NOUT * esz, hipMemcpyDeviceToHost));
     return KESTREL_OK;
