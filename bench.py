@@ -437,7 +437,7 @@ def cls_bench(args):
     print(json.dumps(line), flush=True)
 
 
-def mixed_errors(codes, co_mixed, config, n_pairs=20000, seed=0xE44, gl=None):
+def mixed_errors(codes, co_mixed, config, n_pairs=20000, seed=0xE44, gl=None, buckets=None):
     """Errors of the mixed form against the FP64 form on a seeded sample of
     off-diagonal pairs, by the bar of the FP32-mixed form: max |dDelta_k|,
     |d kinship|, relative objective and log10 condition on the pairs whose
@@ -450,7 +450,7 @@ def mixed_errors(codes, co_mixed, config, n_pairs=20000, seed=0xE44, gl=None):
     keep = x != y
     pairs = np.stack([np.minimum(x, y), np.maximum(x, y)], 1)[keep]
     a = co_mixed.fit_pairs(pairs)
-    with (Cohort(codes) if gl is None else Cohort.gl(gl)) as co64:
+    with (Cohort.gl_multi(*buckets) if buckets else Cohort(codes) if gl is None else Cohort.gl(gl)) as co64:
         b = co64.fit_pairs(pairs)
     lc = np.log10(b["cond"])
     lca = np.log10(a["cond"])
@@ -702,9 +702,8 @@ def main():
         f"{f' with {args.multi:.0%} tri/tetra-allelic sites' if buckets else ''} generated in "
         f"{time.perf_counter() - t:.1f}s")
     t = time.perf_counter()
-    if args.precision == "mixed" and buckets:
-        sys.exit("--precision mixed applies to 2-bit and biallelic likelihood cohorts")
-    co = (Cohort.gl(gl, precision=args.precision) if args.soft else Cohort.gl_multi(*buckets) if buckets
+    co = (Cohort.gl(gl, precision=args.precision) if args.soft
+          else Cohort.gl_multi(*buckets, precision=args.precision) if buckets
           else Cohort(c.codes, precision=args.precision))
     if args.segment_sites:
         co.set_option("segment_sites", args.segment_sites)
@@ -743,7 +742,8 @@ def main():
                             and not args.soft and not buckets and args.precision == "fp64" else None),
                 "traffic_source": (SWEEP_TRAFFIC_SOURCE if args.config == "c2" and not args.segment_sites
                                    and not args.soft and not buckets and args.precision == "fp64" else None),
-                "kernel": ("sweep_gl + sweep_glm" if buckets
+                "kernel": (("sweep_gl_mixed + sweep_glm_mixed" if args.precision == "mixed" else "sweep_gl + sweep_glm")
+                           if buckets
                            else ("sweep_gl_mixed" if args.precision == "mixed" else "sweep_gl") if args.soft
                            else "sweep_mixed" if args.precision == "mixed" else "sweep_fp64"),
                 **fit64["launch"]}
@@ -757,7 +757,7 @@ def main():
         roofline["frac_of_measured"] = round(achieved / pk, 4)
     precision_error = None
     if mixed:
-        precision_error = mixed_errors(c.codes, co, args.config, gl=gl)
+        precision_error = mixed_errors(c.codes, co, args.config, gl=gl, buckets=buckets)
         log(f"mixed vs FP64 form: {precision_error}")
     # the FP32-mixed form on the same workload, beside the FP64 form
     mixed_leg = None

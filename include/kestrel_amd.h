@@ -250,9 +250,9 @@ int kestrel_cohort_create_gt_multi(kestrel_cohort** out, uint32_t n_samples, con
  * end, for fits of fewer pairs than the device has lanes; -1 = 64 for fits
  * of at most 4,096 pairs, 1 otherwise; records are those of another
  * summation order, within the reference tolerances of the default form's),
- * "precision" (KESTREL_FP64 or KESTREL_MIXED, for 2-bit and
- * biallelic likelihood-row cohorts: the form of the next fits; this one
- * changes records),
+ * "precision" (KESTREL_FP64 or KESTREL_MIXED, for 2-bit and likelihood-row
+ * cohorts, tri/tetra-allelic parts included: the form of the next fits; this
+ * one changes records),
  * "serve_policy" (order in
  * which a serving engine started afterwards feeds admitted requests to the
  * device: 1 = shortest remaining first, aged by the device's pair rate times

@@ -156,10 +156,14 @@ class Cohort:
 
     @classmethod
     def gl_multi(cls, gl2: np.ndarray | None = None, gl3: np.ndarray | None = None, gl4: np.ndarray | None = None,
-                 *, maf: float | None = None, af_iterations: int = 100, segment_sites: int | None = None) -> "Cohort":
+                 *, maf: float | None = None, af_iterations: int = 100, segment_sites: int | None = None,
+                 precision: str = "fp64") -> "Cohort":
         """Soft likelihoods of sites with 2, 3 and 4 alleles
         (kestrel_cohort_create_gl_multi): glA is (V_A, N, G) site-major,
-        G = 3, 6, 10 in VCF genotype order."""
+        G = 3, 6, 10 in VCF genotype order.  precision "mixed": option
+        "precision" KESTREL_MIXED for every part."""
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
         L = _lib.load()
         self = cls.__new__(cls)
         arrs, counts, n = [], [], None
@@ -188,6 +192,8 @@ class Cohort:
         self._h = h
         if segment_sites is not None:
             self.set_option("segment_sites", segment_sites)
+        if precision == "mixed":
+            self.set_option("precision", _lib.MIXED)
         return self
 
     def multi_info(self):

@@ -82,7 +82,8 @@ int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, con
     size_t base = c->nseg;
     for (const auto& p : c->parts) {
         HIPCHK(klaunch::sweep_multi(p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->table.p, p->seg.p,
-                                    p->nseg, sx, sy, pt, active, ns, eps, partial + base * NPART * ns, st));
+                                    p->nseg, sx, sy, pt, active, ns, eps, partial + base * NPART * ns, st,
+                                    c->mixed ? p->table32.p : nullptr));
         base += p->nseg;
     }
     return KESTREL_OK;
@@ -163,6 +164,9 @@ int recondition_records(kestrel_cohort* c, const uint32_t* geno, uint32_t row_wo
     // up to eight device fills of pairs per launch
     const size_t batch = std::min<size_t>(idx.size(), std::max<size_t>(8 * default_slots(c), SWEEP_TPB));
     const uint32_t ns = (uint32_t)((batch + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB);
+    // the tri/tetra-allelic parts' segments follow the main part's
+    uint32_t nall = nseg;
+    for (const auto& p : c->parts) nall += p->nseg;
     DevBuf<uint32_t> dsx, dsy;
     DevBuf<int> dact;
     DevBuf<double> dpt, dpart, dcond;
@@ -170,7 +174,7 @@ int recondition_records(kestrel_cohort* c, const uint32_t* geno, uint32_t row_wo
     HIPCHK(dsy.alloc(ns));
     HIPCHK(dact.alloc(ns));
     HIPCHK(dpt.alloc((size_t)9 * ns));
-    HIPCHK(dpart.alloc((size_t)nseg * NPART * ns));
+    HIPCHK(dpart.alloc((size_t)nall * NPART * ns));
     HIPCHK(dcond.alloc(ns));
     std::vector<uint32_t> hx(ns), hy(ns);
     std::vector<int> hact(ns);
@@ -193,7 +197,14 @@ int recondition_records(kestrel_cohort* c, const uint32_t* geno, uint32_t row_wo
         HIPCHK(hipMemcpyAsync(dpt.p, hpt.data(), hpt.size() * sizeof(double), hipMemcpyHostToDevice, st));
         HIPCHK(klaunch::sweep(geno, row_words, c->af.p, seg, nseg, dsx.p, dsy.p, dpt.p, dact.p, ns, tune.epsilon,
                               dpart.p, st, c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride, false, false));
-        HIPCHK(klaunch::recondition(dpart.p, nseg, ns, (uint32_t)m, dpt.p, tune.sqp_conv_tol, dcond.p, st));
+        size_t off = nseg;
+        for (const auto& p : c->parts) {  // FP64 sweeps of the tri/tetra-allelic parts
+            HIPCHK(klaunch::sweep_multi(p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->table.p,
+                                        p->seg.p, p->nseg, dsx.p, dsy.p, dpt.p, dact.p, ns, tune.epsilon,
+                                        dpart.p + off * NPART * ns, st));
+            off += p->nseg;
+        }
+        HIPCHK(klaunch::recondition(dpart.p, nall, ns, (uint32_t)m, dpt.p, tune.sqp_conv_tol, dcond.p, st));
         HIPCHK(hipMemcpyAsync(hcond.data(), dcond.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         for (size_t j = 0; j < m; j++) recs[idx[b0 + j]].cond = hcond[j];
@@ -550,6 +561,8 @@ int kestrel_cohort_create_gl_multi(kestrel_cohort** out, uint32_t n_samples, con
         if (rc) return rc;
         HIPCHK(p->table.alloc((size_t)p->v_pad * klaunch::iis_table_size(p->alleles)));
         HIPCHK(klaunch::iis_table(p->alleles, p->af.p, p->v_pad, p->table.p, c->stream));
+        HIPCHK(p->table32.alloc(p->table.n));
+        HIPCHK(klaunch::round_f32(p->table.p, p->table32.p, p->table.n, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         c->parts.push_back(std::move(p));
     }
@@ -610,8 +623,8 @@ int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value) {
     if (!strcmp(key, "precision")) {
         if (value != KESTREL_FP64 && value != KESTREL_MIXED)
             return fail(KESTREL_EINVAL, "precision must be KESTREL_FP64 or KESTREL_MIXED");
-        if (value == KESTREL_MIXED && (c->gt || !c->parts.empty()))
-            return fail(KESTREL_EINVAL, "the mixed-precision form covers 2-bit and biallelic likelihood cohorts");
+        if (value == KESTREL_MIXED && c->gt)
+            return fail(KESTREL_EINVAL, "the mixed-precision form covers the likelihood fit, not -f GT cohorts");
         c->mixed = value == KESTREL_MIXED;
         return KESTREL_OK;
     }

@@ -82,6 +82,7 @@ struct SitePart {
     DevBuf<uint32_t> mask;  // n_samples x row_words 2-bit rows, 3 = missing
     DevBuf<double> af;      // v_pad x alleles
     DevBuf<double> table;   // v_pad x the mode coefficients of every genotype pair (A >= 3)
+    DevBuf<float> table32;  // the same rounded to FP32 (precision KESTREL_MIXED)
     DevBuf<uint32_t> seg;
 };
 // The 3- or 4-allele sites of a -f GT cohort (kestrel_cohort_create_gt_multi):

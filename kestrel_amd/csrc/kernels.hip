@@ -1394,25 +1394,25 @@ struct IisTable {
 
 // One genotype pair's fmas with every index a template constant (a fold over
 // the mode's coefficients, left to right: iis.rs's accumulate_iis order).
-template <int MODE, int OFF, int... Ts>
-__device__ __forceinline__ void iis_apply(double* cv, double g, const double* T, std::integer_sequence<int, Ts...>) {
+// (V = double; V = float in the mixed-precision form)
+template <int MODE, int OFF, typename V, int... Ts>
+__device__ __forceinline__ void iis_apply(V* cv, V g, const V* T, std::integer_sequence<int, Ts...>) {
     ((cv[iis_target(MODE, Ts)] = fma(g, T[OFF + Ts], cv[iis_target(MODE, Ts)])), ...);
 }
 
-template <int A, int C>
-__device__ __forceinline__ void iis_pair(const double* lx, const double* ly, const double* T,
-                                         double* cv) {
+template <int A, int C, typename V>
+__device__ __forceinline__ void iis_pair(const V* lx, const V* ly, const V* T, V* cv) {
     constexpr IisTable<A> tab{};
     constexpr int G = IisTable<A>::G, MODE = tab.mode[C], OFF = tab.off[C];
-    const double g = lx[C / G] * ly[C % G];
+    const V g = lx[C / G] * ly[C % G];
     iis_apply<MODE, OFF>(cv, g, T, std::make_integer_sequence<int, iis_nnz(MODE)>{});
 }
 
 // mixture.rs:8-60 from a site's coefficient table: every (gx, gy) in genotype
 // order, g = lx[gx] ly[gy], then its mode's fmas.
-template <int A, int... Cs>
-__device__ __forceinline__ void mixture_table(const double* lx, const double* ly, const double* T,
-                                              double* cv, std::integer_sequence<int, Cs...>) {
+template <int A, typename V, int... Cs>
+__device__ __forceinline__ void mixture_table(const V* lx, const V* ly, const V* T, V* cv,
+                                              std::integer_sequence<int, Cs...>) {
     (iis_pair<A, Cs>(lx, ly, T, cv), ...);
 }
 
@@ -1550,6 +1550,148 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __rest
         for (int k = 0; k < 45; k++) out[(size_t)(10 + k) * nslots] = H[k];
         out[(size_t)55 * nslots] = N;
     }
+}
+
+// The mixed-precision form of sweep_glm (precision KESTREL_MIXED): the
+// coefficient table rounded to FP32 (half the LDS bytes a site: 528 for A =
+// 3, 1,200 for A = 4), the likelihoods rounded to FP32 and the generic
+// mixture in FP32; then the mixed site update of sweep_gl_mixed (L, the
+// gradient terms, the log product and the objective in FP64; 1/L, c' and the
+// Hessian summed over the segment in FP32).  Both rows of the next site are
+// loaded while this one is computed (FP32 rows leave the registers for it).
+template <int A>
+__global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm_mixed(const uint32_t* __restrict__ geno,
+                                                                 uint32_t row_words, const double* __restrict__ gl,
+                                                                 uint32_t v_pad, const float* __restrict__ table,
+                                                                 const uint32_t* __restrict__ seg_bounds,
+                                                                 const uint32_t* __restrict__ sx,
+                                                                 const uint32_t* __restrict__ sy,
+                                                                 const double* __restrict__ pt,
+                                                                 const int* __restrict__ active, uint32_t nslots,
+                                                                 double eps, double* __restrict__ partial) {
+    constexpr IisTable<A> tab;
+    constexpr int G = A * (A + 1) / 2, NC = tab.off[G * G];
+    static_assert(NC % 4 == 0, "16-byte table rows");
+    __shared__ __attribute__((aligned(16))) float kt[CHUNK * NC];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t slot = blockIdx.x * SWEEP_TPB + tid;
+    const uint32_t seg = blockIdx.y;
+    const bool live = slot < nslots && active[slot] != 0;
+    if (!__syncthreads_or(live)) return;
+
+    double D[9];
+    uint32_t x = 0, y = 0;
+    if (live) {
+        x = sx[slot];
+        y = sy[slot];
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k] = pt[(size_t)k * nslots + slot];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k] = 0.0;
+    }
+    const uint32_t* rowx = geno + (size_t)x * row_words;
+    const uint32_t* rowy = geno + (size_t)y * row_words;
+    const double* glx = gl + (size_t)x * v_pad * G;
+    const double* gly = gl + (size_t)y * v_pad * G;
+
+    double F = 0.0, N = 0.0, P = 1.0, Gr[9];
+    float h[45];
+#pragma unroll
+    for (int k = 0; k < 9; k++) Gr[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 45; k++) h[k] = 0.0f;
+
+    // site t's row of one sample, rounded to FP32
+    auto load_row = [&](const double* base, uint32_t t, float* r) {
+        const double2* a = reinterpret_cast<const double2*>(base + (size_t)t * G);
+#pragma unroll
+        for (int q = 0; q < G / 2; q++) {
+            const double2 u = a[q];
+            r[2 * q] = (float)u.x;
+            r[2 * q + 1] = (float)u.y;
+        }
+    };
+    const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
+    float nx[G], ny[G];
+    if (s0 < s1) {
+        load_row(glx, s0, nx);
+        load_row(gly, s0, ny);
+    }
+    uint2 wx = make_uint2(0, 0), wy = make_uint2(0, 0);
+#pragma unroll 1
+    for (uint32_t t = s0; t < s1; t++) {
+        const int k = (int)((t - s0) & (CHUNK - 1));
+        if (k == 0) {
+            __syncthreads();  // the previous chunk's tables are read
+            {
+                const float4* src = reinterpret_cast<const float4*>(table + (size_t)t * NC);
+                float4* dst = reinterpret_cast<float4*>(kt);
+                for (uint32_t i = tid; i < CHUNK * NC / 4; i += SWEEP_TPB) dst[i] = src[i];
+            }
+            wx = *reinterpret_cast<const uint2*>(rowx + t / 16);
+            wy = *reinterpret_cast<const uint2*>(rowy + t / 16);
+            const uint32_t ux = ~((wx.x & (wx.x >> 1)) | (wy.x & (wy.x >> 1))) & 0x55555555u;
+            const uint32_t uy = ~((wx.y & (wx.y >> 1)) | (wy.y & (wy.y >> 1))) & 0x55555555u;
+            N += (double)(__popc(ux) + __popc(uy));
+            __syncthreads();
+        }
+        const uint32_t ax = (k < 16 ? wx.x : wx.y) >> (2 * (k & 15));
+        const uint32_t ay = (k < 16 ? wy.x : wy.y) >> (2 * (k & 15));
+        const bool missing = (ax & 3u) == 3u || (ay & 3u) == 3u;
+        float lx[G], ly[G];
+#pragma unroll
+        for (int q = 0; q < G; q++) {
+            lx[q] = nx[q];
+            ly[q] = ny[q];
+        }
+        if (t + 1 < s1) {
+            load_row(glx, t + 1, nx);
+            load_row(gly, t + 1, ny);
+        }
+        float cv[9];
+#pragma unroll
+        for (int j = 0; j < 9; j++) cv[j] = 0.0f;
+        mixture_table<A>(lx, ly, kt + k * NC, cv, std::make_integer_sequence<int, G * G>{});
+        double cd[9];
+#pragma unroll
+        for (int j = 0; j < 9; j++) cd[j] = (double)cv[j];
+        double L = missing ? 1.0 : eps;
+#pragma unroll
+        for (int j = 0; j < 9; j++) L = fma(D[j], cd[j], L);
+        P = P * L;
+        if ((k & (LOG_GROUP - 1)) == LOG_GROUP - 1) {
+            F = F + sleef_log_pos(P);
+            P = 1.0;
+        }
+        const float r = __builtin_amdgcn_rcpf((float)L);
+        const double rd = (double)r;
+#pragma unroll
+        for (int j = 0; j < 9; j++) Gr[j] = fma(cd[j], rd, Gr[j]);
+#pragma unroll
+        for (int j = 0; j < 9; j++) cv[j] = cv[j] * r;
+        int id = 0;
+#pragma unroll
+        for (int a2 = 0; a2 < 9; a2++)
+#pragma unroll
+            for (int bb = a2; bb < 9; bb++, id++) h[id] = fmaf(cv[a2], cv[bb], h[id]);
+    }
+    if (live) {
+        double* out = partial + (size_t)seg * NPART * nslots + slot;
+        out[0] = F;
+#pragma unroll
+        for (int k = 0; k < 9; k++) out[(size_t)(1 + k) * nslots] = Gr[k];
+#pragma unroll
+        for (int k = 0; k < 45; k++) out[(size_t)(10 + k) * nslots] = (double)h[k];
+        out[(size_t)55 * nslots] = N;
+    }
+}
+
+// FP32 copy of a table (the mixed form's coefficients)
+__global__ __launch_bounds__(256) void round_f32_kernel(const double* __restrict__ src, float* __restrict__ dst,
+                                                        uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = (float)src[i];
 }
 
 // Evaluation at points outside the sweeps' domain (any point, any eps:
@@ -2750,9 +2892,17 @@ hipError_t af_gl(const double* gl, uint32_t n_samples, uint32_t n_sites, const T
 hipError_t sweep_multi(int alleles, const uint32_t* geno, uint32_t row_words, const double* gl, uint32_t v_pad,
                        const double* table, const uint32_t* seg_bounds, uint32_t nseg, const uint32_t* sx,
                        const uint32_t* sy, const double* pt, const int* active, uint32_t nslots, double eps,
-                       double* partial, hipStream_t st) {
+                       double* partial, hipStream_t st, const float* table32) {
     dim3 grid((nslots + SWEEP_TPB - 1) / SWEEP_TPB, nseg);
-    if (alleles == 3)
+    if (table32 && alleles == 3)
+        hipLaunchKernelGGL(sweep_glm_mixed<3>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, table32,
+                           seg_bounds, sx, sy, pt, active, nslots, eps, partial);
+    else if (table32 && alleles == 4)
+        hipLaunchKernelGGL(sweep_glm_mixed<4>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, table32,
+                           seg_bounds, sx, sy, pt, active, nslots, eps, partial);
+    else if (table32)
+        return hipErrorInvalidValue;
+    else if (alleles == 3)
         hipLaunchKernelGGL(sweep_glm<3>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, table, seg_bounds,
                            sx, sy, pt, active, nslots, eps, partial);
     else if (alleles == 4)
@@ -2792,6 +2942,12 @@ hipError_t sweep_exact(int kind, const uint32_t* geno, uint32_t row_words, const
     default:
         return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t round_f32(const double* src, float* dst, uint64_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(round_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
     return hipGetLastError();
 }
 

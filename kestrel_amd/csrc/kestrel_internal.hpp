@@ -74,7 +74,10 @@ hipError_t gather_gl(const double* gl_site, uint32_t n_samples, const uint32_t* 
 hipError_t sweep_multi(int alleles, const uint32_t* geno, uint32_t row_words, const double* gl, uint32_t v_pad,
                        const double* table, const uint32_t* seg_bounds, uint32_t nseg, const uint32_t* sx,
                        const uint32_t* sy, const double* pt, const int* active, uint32_t nslots, double eps,
-                       double* partial, hipStream_t st);
+                       double* partial, hipStream_t st, const float* table32 = nullptr);
+// dst[i] = (float)src[i] (the mixed form's FP32 coefficient tables; table32
+// above selects that form of sweep_multi)
+hipError_t round_f32(const double* src, float* dst, uint64_t n, hipStream_t st);
 // evaluation at points outside the sweeps' domain, the reference's arithmetic
 // at every site (kind 0: 2-bit rows, 2 / 3: likelihood rows of that stride,
 // 13 / 14: a 3- / 4-allele part); only slots with exact[slot] != 0

@@ -157,3 +157,34 @@ def test_relatedness_coefficients_gl_tetra_only(O):
     from kestrel_amd import build_arrays
     ej, ek, ei = build_arrays(exp, 12)
     assert np.array_equal(jac, ej) and np.array_equal(kin, ek) and np.array_equal(inb, ei)
+
+
+def test_multi_mixed_precision_form(O):
+    """The mixed-precision form on 2/3/4-allele likelihood rows (option
+    "precision"; sweep_gl_mixed + sweep_glm_mixed with FP32 coefficient
+    tables): Delta, kinship and objective within 1e-6 of the FP64 form on the
+    pairs both bring to convergence with log10 condition below 12, n_xy equal,
+    log10 condition within 0.1 below 100; the option switches back to
+    bit-exact FP64 records."""
+    from kestrel_amd import _lib, kinship
+    say("case: multi-allelic fit, mixed-precision form vs FP64 form, 40 x (1500, 400, 160)")
+    b2, b3, b4 = synth.multi_likelihoods(40, (1500, 400, 160), seed=57)
+    with Cohort.gl_multi(b2, b3, b4) as co:
+        b = co.fit_pairs()
+        co.set_option("precision", _lib.MIXED)
+        a = co.fit_pairs()
+        co.set_option("precision", _lib.FP64)
+        again = co.fit_pairs()
+    assert_bitexact(again, b)
+    assert np.array_equal(a["n_sites"], b["n_sites"])
+    lb, la = np.log10(b["cond"]), np.log10(a["cond"])
+    both = np.isfinite(lb) & np.isfinite(la) & (lb < 12)
+    assert both.sum() > 0.9 * len(b)
+    d = np.abs(a["jacquard"] - b["jacquard"]).max(1)[both]
+    k = np.abs(kinship(a["jacquard"]) - kinship(b["jacquard"]))[both]
+    r = (np.abs(a["obj"] - b["obj"]) / np.abs(b["obj"]))[both]
+    say(f"multi mixed vs fp64: max |dDelta| {d.max():.3g}, kinship {k.max():.3g}, objective {r.max():.3g} over "
+        f"{both.sum()} pairs; iterations {a['iters'].mean():.2f} vs {b['iters'].mean():.2f}")
+    assert d.max() <= 1e-6 and k.max() <= 1e-6 and r.max() <= 1e-6
+    c = both & (lb < 2.0)
+    assert np.abs(la - lb)[c].max() <= 0.1
