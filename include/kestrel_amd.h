@@ -244,7 +244,8 @@ int kestrel_cohort_create_gt_multi(kestrel_cohort** out, uint32_t n_samples, con
  * share their first sample, and a sweep passes over the sites where a
  * wavefront's shared sample is missing, which add nothing; 1 = on, 0 = off, -1 = on when
  * at least 2 % of the kept calls are missing and the form is FP64 (default);
- * results do not change), "pair_lanes" (FP64 fits of 2-bit cohorts: 1 = one
+ * results do not change), "pair_lanes" (FP64 fits of 2-bit and biallelic
+ * likelihood-row cohorts: 1 = one
  * pair per lane (default), 64 = one pair per wavefront with its sites across
  * the 64 lanes and the lanes' sums added in a butterfly at each segment's
  * end, for fits of fewer pairs than the device has lanes; -1 = 64 for fits

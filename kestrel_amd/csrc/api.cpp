@@ -72,9 +72,9 @@ int repack_kept(kestrel_cohort* c, const uint8_t* dpacked, size_t row_stride_byt
 int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
                   uint32_t ns, double eps, double* partial, hipStream_t st, bool skip_masked = false,
                   bool lanes = false) {
-    if (lanes) {  // 2-bit FP64 cohorts only (checked by the caller)
+    if (lanes) {  // FP64 fits of 2-bit and biallelic likelihood-row cohorts (checked by the caller)
         HIPCHK(klaunch::sweep_lanes(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, sx, sy, pt, active, ns, eps,
-                                    partial, st));
+                                    partial, st, c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride));
         return KESTREL_OK;
     }
     HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, sx, sy, pt, active, ns, eps, partial, st,
@@ -733,14 +733,15 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     // (automatic for the FP64 form only: the mixed form's pass holds two
     // wavefronts per SIMD instead of three, and measured 1,241,070 against
     // 1,266,335 pair-fits/s without it on C3)
-    // a pair's sites across a wavefront (option "pair_lanes" 64): 2-bit FP64
-    // cohorts; claims in list order (the pass has no shared sample to skip on)
+    // a pair's sites across a wavefront (option "pair_lanes" 64): FP64 fits of
+    // 2-bit and biallelic likelihood-row cohorts; claims in list order (the
+    // pass has no shared sample to skip on)
     // (-1: automatic, for fits of at most LANES_AUTO_PAIRS pairs: measured on
     // 1,000 x 100,000 sites, 100 / 999 / 3,996 / 31,968 pairs fit in 1.9 /
     // 8.4 / 24.6 / 162 ms against 18.5 / 21.5 / 38.0 / 110 ms one pair a lane)
-    const bool lanes_form = !c->soft && c->parts.empty() && !c->mixed;
+    const bool lanes_form = c->parts.empty() && !c->mixed;
     if (c->opt_pair_lanes == 64 && !lanes_form)
-        return fail(KESTREL_EINVAL, "pair_lanes 64 covers FP64 fits of 2-bit (hard-call) cohorts");
+        return fail(KESTREL_EINVAL, "pair_lanes 64 covers FP64 fits of 2-bit and biallelic likelihood-row cohorts");
     const bool lanes = c->opt_pair_lanes == 64 || (c->opt_pair_lanes < 0 && lanes_form && n_pairs <= LANES_AUTO_PAIRS);
     const bool use_runs = !lanes && !c->soft && c->parts.empty() &&
                           (c->opt_skip_masked == 1 ||

@@ -168,7 +168,7 @@ struct kestrel_cohort {
     double lmax = 1.0;             // largest likelihood of the cohort's rows (1 for hard calls)
     bool mixed = false;            // precision KESTREL_MIXED: sweeps by sweep_mixed
     int64_t opt_pair_tile = -1;    // claim order in tiles of this many samples (0 list order, -1 automatic)
-    int64_t opt_pair_lanes = 1;    // lanes a pair: 1, 64 (its sites across a wavefront; 2-bit FP64 fits), -1 auto
+    int64_t opt_pair_lanes = 1;    // lanes a pair: 1, 64 (its sites across a wavefront; FP64 fits), -1 auto
     hipStream_t stream2 = nullptr; // second stream of a fit
     double last_sweep_ms = 0, last_wall_ms = 0, last_busy_ms = 0;
     uint64_t last_sweeps = 0;

@@ -623,6 +623,52 @@ __device__ __forceinline__ void cs_from_codes(double p, double q, uint32_t ax, u
     r[8] = p0 ? a4 : (p1 || p3) ? ta3b : p2 ? a2b2 : tab2;
 }
 
+// One usable site of the wavefront form in a lane: L = eps + c.D, the log
+// product's factor, r = 1/L, g += c r, c' = c r, H += c' c'^T (site_update's
+// operations).
+__device__ __forceinline__ void lanes_site(double* cv, double eps, const double* D, double& P, double* G,
+                                           double* H) {
+    double L = eps;
+#pragma unroll
+    for (int j = 0; j < 9; j++) L = fma(D[j], cv[j], L);
+    P = P * L;
+    const double r = recip_rn(L);
+#pragma unroll
+    for (int j = 0; j < 9; j++) G[j] = fma(cv[j], r, G[j]);
+#pragma unroll
+    for (int j = 0; j < 9; j++) cv[j] = cv[j] * r;
+    int id = 0;
+#pragma unroll
+    for (int a = 0; a < 9; a++)
+#pragma unroll
+        for (int bb = a; bb < 9; bb++, id++) H[id] = fma(cv[a], cv[bb], H[id]);
+}
+
+// The 64 lanes' sums added in a butterfly (xor 32, 16, ..., 1: lane i adds
+// lane i ^ o's value; both partners hold the same sum since IEEE addition
+// commutes); lane 0 writes the segment's partials (out = the slot's entry of
+// segment 0 of them, stride nslots).
+__device__ __forceinline__ void lanes_finish(double F, double N, double* G, double* H, uint32_t lane, double* out,
+                                             uint32_t nslots) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        F = F + __shfl_xor(F, o, 64);
+        N = N + __shfl_xor(N, o, 64);
+#pragma unroll
+        for (int k = 0; k < 9; k++) G[k] = G[k] + __shfl_xor(G[k], o, 64);
+#pragma unroll
+        for (int k = 0; k < 45; k++) H[k] = H[k] + __shfl_xor(H[k], o, 64);
+    }
+    if (lane == 0) {
+        out[0] = F;
+#pragma unroll
+        for (int k = 0; k < 9; k++) out[(size_t)(1 + k) * nslots] = G[k];
+#pragma unroll
+        for (int k = 0; k < 45; k++) out[(size_t)(10 + k) * nslots] = H[k];
+        out[(size_t)55 * nslots] = N;
+    }
+}
+
 // The sweep with a pair's sites across the lanes of a wavefront (option
 // "pair_lanes" 64), for fits with fewer pairs than the device has lanes: one
 // pair per wavefront, four a workgroup (wavefront w of block b takes slot
@@ -675,20 +721,7 @@ __global__ __launch_bounds__(SWEEP_TPB) void sweep_lanes_fp64(const uint32_t* __
                 const double2 f = *reinterpret_cast<const double2*>(af + 2 * (size_t)s);
                 double cv[9];
                 cs_from_codes(f.x, f.y, ax, ay, cv);
-                double L = eps;
-#pragma unroll
-                for (int j = 0; j < 9; j++) L = fma(D[j], cv[j], L);
-                P = P * L;
-                const double r = recip_rn(L);
-#pragma unroll
-                for (int j = 0; j < 9; j++) G[j] = fma(cv[j], r, G[j]);
-#pragma unroll
-                for (int j = 0; j < 9; j++) cv[j] = cv[j] * r;
-                int id = 0;
-#pragma unroll
-                for (int a = 0; a < 9; a++)
-#pragma unroll
-                    for (int bb = a; bb < 9; bb++, id++) H[id] = fma(cv[a], cv[bb], H[id]);
+                lanes_site(cv, eps, D, P, G, H);
                 N += 1.0;
             }
         }
@@ -697,24 +730,7 @@ __global__ __launch_bounds__(SWEEP_TPB) void sweep_lanes_fp64(const uint32_t* __
             P = 1.0;
         }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        F = F + __shfl_xor(F, o, 64);
-        N = N + __shfl_xor(N, o, 64);
-#pragma unroll
-        for (int k = 0; k < 9; k++) G[k] = G[k] + __shfl_xor(G[k], o, 64);
-#pragma unroll
-        for (int k = 0; k < 45; k++) H[k] = H[k] + __shfl_xor(H[k], o, 64);
-    }
-    if (lane == 0) {
-        double* out = partial + (size_t)seg * NPART * nslots + slot;
-        out[0] = F;
-#pragma unroll
-        for (int k = 0; k < 9; k++) out[(size_t)(1 + k) * nslots] = G[k];
-#pragma unroll
-        for (int k = 0; k < 45; k++) out[(size_t)(10 + k) * nslots] = H[k];
-        out[(size_t)55 * nslots] = N;
-    }
+    lanes_finish(F, N, G, H, lane, partial + (size_t)seg * NPART * nslots + slot, nslots);
 }
 
 // The mixed-precision form (precision KESTREL_MIXED): the pass of sweep_fp64
@@ -1123,6 +1139,80 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_gl(const uint32_t* __restr
         for (int k = 0; k < 45; k++) out[(size_t)(10 + k) * nslots] = H[k];
         out[(size_t)55 * nslots] = N;
     }
+}
+
+// The wavefront form of sweep_gl (option "pair_lanes" 64): as
+// sweep_lanes_fp64 with likelihood rows -- lane l takes the sites s0 + l +
+// 64 t of the segment, builds the site's coefficients from its (p, q) and c_s
+// by mixture_soft (the bits of sweep_gl's c_s) from the pair's two rows
+// (STRIDE 2: the exact 1.0 put back at the position of the 2-bit code), and
+// passes over sites where either call is missing (c = 0 there, L = 1).
+template <int STRIDE>
+__global__ __launch_bounds__(SWEEP_TPB) void sweep_lanes_gl(const uint32_t* __restrict__ geno, uint32_t row_words,
+                                                             const double* __restrict__ gl, uint32_t v_pad,
+                                                             const double* __restrict__ af,
+                                                             const uint32_t* __restrict__ seg_bounds,
+                                                             const uint32_t* __restrict__ sx,
+                                                             const uint32_t* __restrict__ sy,
+                                                             const double* __restrict__ pt,
+                                                             const int* __restrict__ active, uint32_t nslots,
+                                                             double eps, double* __restrict__ partial) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t slot = blockIdx.x * (SWEEP_TPB / 64) +
+                          (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (slot >= nslots || active[slot] == 0) return;  // uniform over the wavefront
+    const uint32_t seg = blockIdx.y;
+    const uint32_t x = sx[slot], y = sy[slot];
+    double D[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) D[k] = pt[(size_t)k * nslots + slot];
+    const uint32_t* rowx = geno + (size_t)x * row_words;
+    const uint32_t* rowy = geno + (size_t)y * row_words;
+    const double* glx = gl + (size_t)x * v_pad * STRIDE;
+    const double* gly = gl + (size_t)y * v_pad * STRIDE;
+    const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
+
+    auto load_row = [&](const double* base, uint32_t t, uint32_t code, double* l) {
+        if (STRIDE == 3) {
+            const double* r = base + (size_t)t * 3;
+            l[0] = r[0];
+            l[1] = r[1];
+            l[2] = r[2];
+        } else {
+            const double2 v = *reinterpret_cast<const double2*>(base + (size_t)t * 2);
+            l[0] = code == 0 ? 1.0 : v.x;
+            l[1] = code == 0 ? v.x : code == 1 ? 1.0 : v.y;
+            l[2] = code == 2 ? 1.0 : v.y;
+        }
+    };
+    double F = 0.0, N = 0.0, P = 1.0, G[9], H[45];
+#pragma unroll
+    for (int k = 0; k < 9; k++) G[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 45; k++) H[k] = 0.0;
+    const uint32_t steps = (s1 - s0 + 63) / 64;
+    for (uint32_t t = 0; t < steps; t++) {
+        const uint32_t s = s0 + 64 * t + lane;
+        if (s < s1) {
+            const uint32_t ax = (rowx[s >> 4] >> (2 * (s & 15u))) & 3u;
+            const uint32_t ay = (rowy[s >> 4] >> (2 * (s & 15u))) & 3u;
+            if (ax != 3u && ay != 3u) {
+                const double2 f = *reinterpret_cast<const double2*>(af + 2 * (size_t)s);
+                double K[K_N], lx[3], ly[3], cv[9];
+                site_coefficients(f.x, f.y, K);
+                load_row(glx, s, ax, lx);
+                load_row(gly, s, ay, ly);
+                mixture_soft(lx, ly, K, cv);
+                lanes_site(cv, eps, D, P, G, H);
+                N += 1.0;
+            }
+        }
+        if ((t & (LOG_GROUP - 1)) == LOG_GROUP - 1 || t + 1 == steps) {
+            F = F + sleef_log_pos(P);
+            P = 1.0;
+        }
+    }
+    lanes_finish(F, N, G, H, lane, partial + (size_t)seg * NPART * nslots + slot, nslots);
 }
 
 // The mixed-precision form of sweep_gl (precision KESTREL_MIXED): the site
@@ -2866,11 +2956,19 @@ hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, con
 
 hipError_t sweep_lanes(const uint32_t* geno, uint32_t row_words, const double* af, const uint32_t* seg_bounds,
                        uint32_t nseg, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
-                       uint32_t nslots, double eps, double* partial, hipStream_t st) {
+                       uint32_t nslots, double eps, double* partial, hipStream_t st, const double* gl,
+                       uint32_t v_pad, int gl_stride) {
     constexpr uint32_t per_block = SWEEP_TPB / 64;
     dim3 grid((nslots + per_block - 1) / per_block, nseg);
-    hipLaunchKernelGGL(sweep_lanes_fp64, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, af, seg_bounds, sx, sy, pt,
-                       active, nslots, eps, partial);
+    if (gl && gl_stride == 2)
+        hipLaunchKernelGGL(sweep_lanes_gl<2>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, af,
+                           seg_bounds, sx, sy, pt, active, nslots, eps, partial);
+    else if (gl)
+        hipLaunchKernelGGL(sweep_lanes_gl<3>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, af,
+                           seg_bounds, sx, sy, pt, active, nslots, eps, partial);
+    else
+        hipLaunchKernelGGL(sweep_lanes_fp64, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, af, seg_bounds, sx, sy,
+                           pt, active, nslots, eps, partial);
     return hipGetLastError();
 }
 

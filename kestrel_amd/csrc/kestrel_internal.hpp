@@ -100,11 +100,13 @@ hipError_t sweep(const uint32_t* geno, uint32_t row_words, const double* af, con
                  uint32_t nslots, double eps, double* partial, hipStream_t st,
                  const double* gl = nullptr, uint32_t v_pad = 0, int gl_stride = 3, bool skip_masked = false,
                  bool mixed = false);
-// 2-bit FP64 cohorts, one pair per wavefront with its sites across the lanes
-// (option "pair_lanes" 64): partials in sweep's layout, another summation order
+// FP64 fits of 2-bit or biallelic likelihood-row cohorts (gl != nullptr), one
+// pair per wavefront with its sites across the lanes (option "pair_lanes"
+// 64): partials in sweep's layout, another summation order
 hipError_t sweep_lanes(const uint32_t* geno, uint32_t row_words, const double* af, const uint32_t* seg_bounds,
                        uint32_t nseg, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
-                       uint32_t nslots, double eps, double* partial, hipStream_t st);
+                       uint32_t nslots, double eps, double* partial, hipStream_t st, const double* gl = nullptr,
+                       uint32_t v_pad = 0, int gl_stride = 3);
 hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* active, uint32_t nslots,
                 double* partial, uint32_t nseg, const uint32_t* pairs, uint64_t npairs, const uint64_t* order,
                 unsigned long long* next_pair, kestrel_output* out, const kdev::Tune& t, int* n_live,

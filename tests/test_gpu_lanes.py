@@ -1,9 +1,10 @@
 """Fits with a pair's sites across a wavefront (option "pair_lanes" 64,
-sweep_lanes_fp64) through the C ABI: records bit-exact against the oracle in
-the wavefront order (ORDER_LANES) -- whole cohorts with missing calls and self
-pairs, segments that are not multiples of 64 sites, C1 in full -- and within
-the reference tolerances of the default one-pair-per-lane form.  The option
-is refused for the forms it does not cover."""
+sweep_lanes_fp64 and sweep_lanes_gl) through the C ABI: records bit-exact
+against the oracle in the wavefront order (ORDER_LANES) -- whole cohorts with
+missing calls and self pairs, segments that are not multiples of 64 sites, C1
+in full, soft likelihood rows of both strides -- and within the reference
+tolerances of the default one-pair-per-lane form.  The option is refused for
+the forms it does not cover."""
 import numpy as np
 import pytest
 
@@ -90,11 +91,30 @@ def test_lanes_graph_two_streams_and_autotune(O):
     assert_bitexact(eager, O.fit_pairs(kc, ka, order=O.ORDER_LANES, seg=seg))
 
 
-def test_lanes_refused_where_not_covered():
-    say("case: pair_lanes 64 on soft and mixed cohorts, bad values")
-    c = synth.generate(12, 800, seed=3)
-    gl = synth.soft_likelihoods(c.codes, seed=4, mean_depth=3.0)
+@pytest.mark.parametrize("stride", [2, 3])
+def test_lanes_soft_bitexact_against_oracle(O, stride):
+    say(f"case: pair_lanes 64, soft likelihoods 28 x 3000, rows of stride {stride}")
+    c = synth.generate(28, 3000, seed=93, missing=0.03, n_family_pairs=2)
+    gl = synth.soft_likelihoods(c.codes, seed=94, mean_depth=3.0)
+    if stride == 3:
+        gl = gl.copy()
+        gl[100, 3] = (0.5, 0.25, 0.125)  # a called row without an exact 1.0: 24-byte rows
     with Cohort.gl(gl) as co:
+        af, kept = co.allele_freqs()
+        base = co.fit_pairs()
+        co.set_option("pair_lanes", 64)
+        out = co.fit_pairs()
+    g = np.ascontiguousarray(gl.transpose(1, 0, 2)[:, kept])
+    seg = np.minimum(segment_bounds(int(kept.sum())), g.shape[1]).astype(np.uint32)
+    assert_bitexact(out, O.fit_pairs_gl(g, af[kept], order=O.ORDER_LANES, seg=seg))
+    close(out, base)
+
+
+def test_lanes_refused_where_not_covered():
+    say("case: pair_lanes 64 on tri/tetra-allelic and mixed cohorts, bad values")
+    c = synth.generate(12, 800, seed=3)
+    b2, b3, b4 = synth.multi_likelihoods(12, (300, 80, 40), seed=5)
+    with Cohort.gl_multi(b2, b3, b4) as co:
         co.set_option("pair_lanes", 64)
         with pytest.raises(KestrelError):
             co.fit_pairs()
