@@ -113,11 +113,17 @@ def cpu_baseline(codes, af, budget_s: float, gl=None, buckets=None):
         return np.stack([np.minimum(x, y), np.maximum(x, y)], 1)
 
     fit(sample(threads))  # warm-up: library load, first-touch of the cohort
+    # the sample's size from two probes: a small one sizes a second of about
+    # one second, whose rate (thread start-up and uneven pairs amortised) sizes
+    # the sample to the budget
     probe = sample(8 * threads)
-    t = time.perf_counter()
-    fit(probe)
-    dt = time.perf_counter() - t
-    m = int(max(2 * threads, min(400_000, budget_s / max(dt, 1e-3) * len(probe))))
+    for target in (1.0, budget_s):
+        t = time.perf_counter()
+        fit(probe)
+        dt = time.perf_counter() - t
+        m = int(max(2 * threads, min(400_000, min(target, budget_s) / max(dt, 1e-3) * len(probe))))
+        if target < budget_s:
+            probe = sample(m)
     pairs = sample(m)
     t = time.perf_counter()
     recs = fit(pairs)
