@@ -81,8 +81,8 @@ int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, con
                           c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride, skip_masked, c->mixed));
     size_t base = c->nseg;
     for (const auto& p : c->parts) {
-        HIPCHK(klaunch::sweep_multi(p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->table.p, p->seg.p,
-                                    p->nseg, sx, sy, pt, active, ns, eps, partial + base * NPART * ns, st,
+        HIPCHK(klaunch::sweep_multi(p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->table.p, p->af.p,
+                                    p->seg.p, p->nseg, sx, sy, pt, active, ns, eps, partial + base * NPART * ns, st,
                                     c->mixed ? p->table32.p : nullptr));
         base += p->nseg;
     }
@@ -200,7 +200,7 @@ int recondition_records(kestrel_cohort* c, const uint32_t* geno, uint32_t row_wo
         size_t off = nseg;
         for (const auto& p : c->parts) {  // FP64 sweeps of the tri/tetra-allelic parts
             HIPCHK(klaunch::sweep_multi(p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->table.p,
-                                        p->seg.p, p->nseg, dsx.p, dsy.p, dpt.p, dact.p, ns, tune.epsilon,
+                                        p->af.p, p->seg.p, p->nseg, dsx.p, dsy.p, dpt.p, dact.p, ns, tune.epsilon,
                                         dpart.p + off * NPART * ns, st));
             off += p->nseg;
         }

@@ -1506,6 +1506,27 @@ __device__ __forceinline__ void mixture_table(const V* lx, const V* ly, const V*
     (iis_pair<A, Cs>(lx, ly, T, cv), ...);
 }
 
+// mixture.rs:8-60 with each genotype pair's coefficients computed in
+// registers from the site's allele frequencies p[A] by iis_coefs -- the same
+// operations as iis_table_kernel, so the same bits as the table's -- instead
+// of read from a table (products shared between pairs are computed once).
+template <int A, int C>
+__device__ __forceinline__ void iis_pair_reg(const double* lx, const double* ly, const double* p, double* cv) {
+    constexpr IisTable<A> tab{};
+    constexpr Genotypes<A> gt{};
+    constexpr int G = IisTable<A>::G, MODE = tab.mode[C], GX = C / G, GY = C % G;
+    double o[9];
+    iis_coefs(MODE, p[gt.gi[GX]], p[gt.gj[GX]], p[gt.gi[GY]], p[gt.gj[GY]], o);
+    const double g = lx[GX] * ly[GY];
+    iis_apply<MODE, 0>(cv, g, static_cast<const double*>(o), std::make_integer_sequence<int, iis_nnz(MODE)>{});
+}
+
+template <int A, int... Cs>
+__device__ __forceinline__ void mixture_reg(const double* lx, const double* ly, const double* p, double* cv,
+                                            std::integer_sequence<int, Cs...>) {
+    (iis_pair_reg<A, Cs>(lx, ly, p, cv), ...);
+}
+
 // Each site's coefficient table: thread (site, genotype pair).
 template <int A>
 __global__ __launch_bounds__(64) void iis_table_kernel(const double* __restrict__ af, uint32_t n_sites,
@@ -1532,7 +1553,10 @@ __global__ __launch_bounds__(64) void iis_table_kernel(const double* __restrict_
 // this site is computed: both rows with A = 3; with A = 4 only y's (both would
 // take 40 more VGPRs and spill), as pairs are claimed in sample tiles, so a
 // block's lanes share a few x rows, which stay in cache, and read many y rows.
-template <int A>
+// REG: each genotype pair's coefficients computed in registers from the
+// site's allele frequencies (mixture_reg; `table` is then the part's
+// frequencies, A a site), the chunk's frequencies staged in LDS.
+template <int A, bool REG = false>
 __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __restrict__ geno, uint32_t row_words,
                                                            const double* __restrict__ gl, uint32_t v_pad,
                                                            const double* __restrict__ table,
@@ -1544,8 +1568,10 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __rest
                                                            double eps, double* __restrict__ partial) {
     constexpr IisTable<A> tab;
     constexpr int G = A * (A + 1) / 2, NC = tab.off[G * G];
-    constexpr bool PREFETCH_X = A == 3;
-    __shared__ __attribute__((aligned(16))) double kt[CHUNK * NC];
+    constexpr bool PREFETCH_X = A == 3 && !REG;
+    constexpr bool PREFETCH_Y = !REG;
+    constexpr int KW = REG ? A : NC;  // doubles a site in LDS
+    __shared__ __attribute__((aligned(16))) double kt[CHUNK * KW];
     const uint32_t tid = threadIdx.x;
     const uint32_t slot = blockIdx.x * SWEEP_TPB + tid;
     const uint32_t seg = blockIdx.y;
@@ -1586,10 +1612,10 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __rest
     };
     // segment bounds are whole chunks: t - s0 counts chunks from 0
     const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
-    double nx[PREFETCH_X ? G : 1], ny[G];
+    double nx[PREFETCH_X ? G : 1], ny[PREFETCH_Y ? G : 1];
     if (s0 < s1) {
         if (PREFETCH_X) load_row(glx, s0, nx);
-        load_row(gly, s0, ny);
+        if (PREFETCH_Y) load_row(gly, s0, ny);
     }
     uint2 wx = make_uint2(0, 0), wy = make_uint2(0, 0);
 #pragma unroll 1
@@ -1598,9 +1624,9 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __rest
         if (k == 0) {
             __syncthreads();  // the previous chunk's tables are read
             {
-                const double2* src = reinterpret_cast<const double2*>(table + (size_t)t * NC);
+                const double2* src = reinterpret_cast<const double2*>(table + (size_t)t * KW);
                 double2* dst = reinterpret_cast<double2*>(kt);
-                for (uint32_t i = tid; i < CHUNK * NC / 2; i += SWEEP_TPB) dst[i] = src[i];
+                for (uint32_t i = tid; i < CHUNK * KW / 2; i += SWEEP_TPB) dst[i] = src[i];
             }
             wx = *reinterpret_cast<const uint2*>(rowx + t / 16);
             wy = *reinterpret_cast<const uint2*>(rowy + t / 16);
@@ -1616,19 +1642,23 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __rest
 #pragma unroll
         for (int q = 0; q < G; q++) {
             if (PREFETCH_X) lx[q] = nx[q];
-            ly[q] = ny[q];
+            if (PREFETCH_Y) ly[q] = ny[q];
         }
         if (!PREFETCH_X) load_row(glx, t, lx);
+        if (!PREFETCH_Y) load_row(gly, t, ly);
         if (t + 1 < s1) {
             if (PREFETCH_X) load_row(glx, t + 1, nx);
-            load_row(gly, t + 1, ny);
+            if (PREFETCH_Y) load_row(gly, t + 1, ny);
         }
         // mixture.rs:8-60: for every (gx, gy) in genotype order, g = lx ly and
         // accumulate_iis with the site's coefficients of that pair's mode
         double cv[9];
 #pragma unroll
         for (int j = 0; j < 9; j++) cv[j] = 0.0;
-        mixture_table<A>(lx, ly, kt + k * NC, cv, std::make_integer_sequence<int, G * G>{});
+        if (REG)
+            mixture_reg<A>(lx, ly, kt + k * KW, cv, std::make_integer_sequence<int, G * G>{});
+        else
+            mixture_table<A>(lx, ly, kt + k * KW, cv, std::make_integer_sequence<int, G * G>{});
         site_update(cv, missing ? 1.0 : eps, D, k, P, F, Gr, H);
     }
     if (live) {
@@ -2988,9 +3018,9 @@ hipError_t af_gl(const double* gl, uint32_t n_samples, uint32_t n_sites, const T
 }
 
 hipError_t sweep_multi(int alleles, const uint32_t* geno, uint32_t row_words, const double* gl, uint32_t v_pad,
-                       const double* table, const uint32_t* seg_bounds, uint32_t nseg, const uint32_t* sx,
-                       const uint32_t* sy, const double* pt, const int* active, uint32_t nslots, double eps,
-                       double* partial, hipStream_t st, const float* table32) {
+                       const double* table, const double* af, const uint32_t* seg_bounds, uint32_t nseg,
+                       const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active, uint32_t nslots,
+                       double eps, double* partial, hipStream_t st, const float* table32) {
     dim3 grid((nslots + SWEEP_TPB - 1) / SWEEP_TPB, nseg);
     if (table32 && alleles == 3)
         hipLaunchKernelGGL(sweep_glm_mixed<3>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, table32,
@@ -3000,10 +3030,10 @@ hipError_t sweep_multi(int alleles, const uint32_t* geno, uint32_t row_words, co
                            seg_bounds, sx, sy, pt, active, nslots, eps, partial);
     else if (table32)
         return hipErrorInvalidValue;
-    else if (alleles == 3)
-        hipLaunchKernelGGL(sweep_glm<3>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, table, seg_bounds,
-                           sx, sy, pt, active, nslots, eps, partial);
-    else if (alleles == 4)
+    else if (alleles == 3)  // coefficients in registers: 1.1-1.2x the table form (LDS-bound), the same bits
+        hipLaunchKernelGGL((sweep_glm<3, true>), grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, af,
+                           seg_bounds, sx, sy, pt, active, nslots, eps, partial);
+    else if (alleles == 4)  // in registers the products shared between pairs spill (46 VGPRs)
         hipLaunchKernelGGL(sweep_glm<4>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, table, seg_bounds,
                            sx, sy, pt, active, nslots, eps, partial);
     else

@@ -72,9 +72,9 @@ hipError_t gather_gl(const double* gl_site, uint32_t n_samples, const uint32_t* 
 // tri/tetra-allelic parts: the sweep over a part of sites with 3 or 4 alleles
 // (rows of A(A+1)/2 likelihoods, af of A per site), and the re-bucketing gather
 hipError_t sweep_multi(int alleles, const uint32_t* geno, uint32_t row_words, const double* gl, uint32_t v_pad,
-                       const double* table, const uint32_t* seg_bounds, uint32_t nseg, const uint32_t* sx,
-                       const uint32_t* sy, const double* pt, const int* active, uint32_t nslots, double eps,
-                       double* partial, hipStream_t st, const float* table32 = nullptr);
+                       const double* table, const double* af, const uint32_t* seg_bounds, uint32_t nseg,
+                       const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active, uint32_t nslots,
+                       double eps, double* partial, hipStream_t st, const float* table32 = nullptr);
 // dst[i] = (float)src[i] (the mixed form's FP32 coefficient tables; table32
 // above selects that form of sweep_multi)
 hipError_t round_f32(const double* src, float* dst, uint64_t n, hipStream_t st);

@@ -123,8 +123,8 @@ extern "C" int kestrel_evaluate_noalloc(kestrel_cohort* c, const uint32_t* pairs
                           part, st, c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride, false, c->mixed));
     size_t off = c->nseg;
     for (const auto& p : c->parts) {
-        HIPCHK(klaunch::sweep_multi(p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->table.p, p->seg.p,
-                                    p->nseg, sx, sy, pt, act, (uint32_t)ns, eps, part + off * NPART * ns, st,
+        HIPCHK(klaunch::sweep_multi(p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->table.p, p->af.p,
+                                    p->seg.p, p->nseg, sx, sy, pt, act, (uint32_t)ns, eps, part + off * NPART * ns, st,
                                     c->mixed ? p->table32.p : nullptr));
         off += p->nseg;
     }
