@@ -261,7 +261,7 @@ int main(int argc, char** argv) {
         bool ok = f != nullptr;
         d.join_probs();
         for (const std::vector<double>* v : {&d.probs, &d.probs3, &d.probs4})
-            ok = ok && fwrite(v->data(), sizeof(double), v->size(), f) == v->size();
+            ok = ok && (v->empty() || fwrite(v->data(), sizeof(double), v->size(), f) == v->size());
         if (f) fclose(f);
         if (!ok) return die("cannot write " + a.dump_probs);
         return 0;

@@ -192,12 +192,20 @@ class BgzfSource {
     std::deque<std::future<Chunk>> pending_;
     bool eof_ = false;
     size_t ahead_;
+    // a block: 18 header bytes (with the 'BC' field), deflate data, CRC32 and
+    // ISIZE (8 bytes); BSIZE + 1 is its length, ISIZE at most 65,536
+    static constexpr size_t BLOCK_MIN = 18 + 8;
+    static constexpr size_t BLOCK_MAX = 65536;
 
     static Chunk inflate_group(std::vector<std::string> blocks) {
         Chunk c;
         for (const std::string& b : blocks) {
             const size_t isize = (size_t)(uint8_t)b[b.size() - 4] | (size_t)(uint8_t)b[b.size() - 3] << 8 |
                                  (size_t)(uint8_t)b[b.size() - 2] << 16 | (size_t)(uint8_t)b[b.size() - 1] << 24;
+            if (isize > BLOCK_MAX) {  // a block inflates to at most 64 KiB
+                c.err = "bgzf: corrupt block";
+                return c;
+            }
             const size_t off = c.text.size();
             c.text.resize(off + isize);
             z_stream s;
@@ -245,6 +253,10 @@ public:
                     return true;
                 }
                 const size_t bsize = (size_t)h[16] | (size_t)h[17] << 8;
+                if (bsize + 1 < BLOCK_MIN) {  // shorter than its own header and footer
+                    out.err = "bgzf: bad block size";
+                    return true;
+                }
                 std::string blk(bsize + 1, '\0');
                 memcpy(&blk[0], h, 18);
                 if (fread(&blk[18], 1, bsize + 1 - 18, f_) != bsize + 1 - 18) {

@@ -278,3 +278,32 @@ def test_float_fields_parse_as_f32(tmp_path):
     err = dump(write(tmp_path, "1\t1\t.\tA\tG\t.\tPASS\tAF=0.1x\tGT\t0/0\t0/1\t1/1\n", "b.vcf"), af_info=True,
                expect_ok=False)
     assert "Error reading AF" in err
+
+
+def _bgzf_block(payload: bytes, isize=None, bsize=None) -> bytes:
+    import struct
+    import zlib
+    co = zlib.compressobj(6, zlib.DEFLATED, -15)
+    comp = co.compress(payload) + co.flush()
+    bs = 18 + len(comp) + 8 - 1 if bsize is None else bsize
+    head = b"\x1f\x8b\x08\x04" + b"\x00" * 4 + b"\x00\xff" + struct.pack("<H", 6) + b"BC" + struct.pack("<HH", 2, bs)
+    return head + comp + struct.pack("<II", zlib.crc32(payload) & 0xFFFFFFFF,
+                                     len(payload) if isize is None else isize)
+
+
+def test_bgzf_corrupt_block_sizes(tmp_path):
+    """A BGZF block whose BSIZE is shorter than its own header and footer, or
+    whose ISIZE exceeds the 64 KiB a block may hold, is an error with a
+    message (not a read past the block or a huge allocation)."""
+    body = (HEADER + "1\t1\t.\tA\tC\t.\t.\t.\tGT\t0/1\t1/1\t0/0\n").encode()
+    eof = bytes.fromhex("1f8b08040000000000ff0600424302001b0003000000000000000000")
+    good = tmp_path / "good.vcf.gz"
+    good.write_bytes(_bgzf_block(body) + eof)
+    samples, sites, _ = dump(str(good))
+    assert samples == ["A", "B", "C"] and len(sites) == 1
+    for name, blk in (("short.vcf.gz", _bgzf_block(body, bsize=5)),
+                      ("huge.vcf.gz", _bgzf_block(body, isize=1 << 31))):
+        p = tmp_path / name
+        p.write_bytes(blk + eof)
+        err = dump(str(p), expect_ok=False)
+        assert "bgzf" in err, err
