@@ -1240,6 +1240,7 @@ int kestrel_measure_fp32_peak(double* tflops) {
 }
 
 int kestrel_measure_lds(double* tbs) {
+    if (!tbs) return fail(KESTREL_EINVAL, "null argument");
     // the FMA timer's launch shape; 8 loads of 16 bytes per lane and step
     double per = 0;
     const int rc = measure_fma<double>(

@@ -42,3 +42,31 @@ def test_host_only_calls_without_gpu():
     rc = L.kestrel_cohort_create(ctypes.byref(h), _lib.ptr(packed), 2, 8, 4, None, 2.0, 100, 0)
     assert rc == _lib.EINVAL and b"Minimum allele frequency" in L.kestrel_last_error()
     assert L.kestrel_tune_default(ctypes.byref(t), 0) == _lib.EINVAL
+
+
+def test_every_entry_point_rejects_null_arguments():
+    """Every int-returning entry point, called with null pointers and zero
+    sizes (a null cohort where it takes one), returns an error code and
+    leaves a message; none dereferences a null pointer (the checks come
+    before any device call, so this runs without a GPU)."""
+    from kestrel_amd import _lib
+    L = _lib.load()
+    skip = {"kestrel_version", "kestrel_last_error"}
+    for name in declared_functions():
+        if name in skip:
+            continue
+        fn = getattr(L, name)
+        assert fn.argtypes is not None, name  # the binding declares every argument
+        args = []
+        for t in fn.argtypes or []:
+            if t in (ctypes.c_double, ctypes.c_float):
+                args.append(0.0)
+            elif t is ctypes.c_char_p:
+                args.append(None)
+            elif isinstance(t, type) and issubclass(t, (ctypes._SimpleCData,)) and t not in (ctypes.c_void_p,):
+                args.append(0)
+            else:
+                args.append(None)
+        rc = fn(*args)
+        assert rc != 0, name
+        assert L.kestrel_last_error(), name
