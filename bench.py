@@ -42,7 +42,7 @@ FLOP_PER_SITE_EVAL = 156  # 136 (gradient + Hessian) + 20 (objective, log counte
 # FETCH_SIZE / WRITE_SIZE scaled by a plain 256 MiB copy (profiles/sweep_fp64_c2.md);
 # counters cannot be read inside this process, so the line carries the
 # profiled figure and says so
-SWEEP_TRAFFIC_BYTES_C2 = 1.571e8
+SWEEP_TRAFFIC_BYTES_C2 = 1.563e8
 SWEEP_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of sweep_fp64 on this workload, scaled by a "
                         "256 MiB copy_kernel (profiles/sweep_fp64_c2.md); profiled, not measured in this run")
 

@@ -505,7 +505,11 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_fp64(const uint32_t* __res
     for (int k = 0; k < 45; k++) H[k] = 0.0;
 
     const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
-    const int lut_site = tid >> 3, lut_part = tid & 7;
+    // (site, row) of the table rows this thread builds: the 32 sites of a chunk
+    // across the lanes and the rows across the wavefronts, so that a
+    // wavefront's lanes take two of lut_block's cases (three for wavefront 0,
+    // which adds row (2,2)) instead of all nine
+    const int lut_site = tid & (CHUNK - 1), lut_part = tid / CHUNK;
     auto build = [&](double p, double q, int b) {
         double* buf = lut + b * LUT_BUF;
         lut_rows(p, q, lut_part, buf, lut_site, eps);
@@ -808,7 +812,11 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_mixed(const uint32_t* __re
     for (int k = 0; k < 45; k++) h[k] = 0.0f;
 
     const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
-    const int lut_site = tid >> 3, lut_part = tid & 7;
+    // (site, row) of the table rows this thread builds: the 32 sites of a chunk
+    // across the lanes and the rows across the wavefronts, so that a
+    // wavefront's lanes take two of lut_block's cases (three for wavefront 0,
+    // which adds row (2,2)) instead of all nine
+    const int lut_site = tid & (CHUNK - 1), lut_part = tid / CHUNK;
     auto build = [&](double p, double q, int b) {
         float* buf = lut + b * LUTF_BUF;
         lut_rows_f32(p, q, lut_part, buf, lut_site, eps);
@@ -2818,7 +2826,11 @@ __global__ __launch_bounds__(256) void cls_sum_kernel(const uint32_t* __restrict
     }
     const uint32_t* rowx = geno + (size_t)x * row_words;
     const uint32_t* rowy = geno + (size_t)y * row_words;
-    const int lut_site = tid >> 3, lut_part = tid & 7;
+    // (site, row) of the table rows this thread builds: the 32 sites of a chunk
+    // across the lanes and the rows across the wavefronts, so that a
+    // wavefront's lanes take two of lut_block's cases (three for wavefront 0,
+    // which adds row (2,2)) instead of all nine
+    const int lut_site = tid & (CHUNK - 1), lut_part = tid / CHUNK;
     double c[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) c[k] = 0.0;
