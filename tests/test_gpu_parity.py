@@ -457,6 +457,26 @@ def test_c2_sample_of_pairs(O):
         assert abs(k - expect[kind]) < 0.03, (kind, k)
 
 
+def test_c2_whole_job_sample(O):
+    """The whole C2 job (500,500 records in the default schedule: two slot
+    groups, claims as slots free up): a seeded sample of 2,000 records
+    bit-exact against the oracle, every record on the simplex."""
+    say("case: C2 1000 x 100000, all 500,500 records, 2,000 checked against the oracle")
+    c = synth.config("c2")
+    with Cohort(c.codes) as co:
+        af, kept = co.allele_freqs()
+        out = co.fit_pairs()
+    n = c.codes.shape[0]
+    assert len(out) == n * (n + 1) // 2 and (out["status"] == 0).all()
+    s = out["jacquard"].sum(1)
+    assert np.abs(s - 1).max() < 1e-12 and (out["jacquard"] >= 0).all()
+    idx = np.random.default_rng(11).choice(len(out), 2000, replace=False)
+    pairs = np.stack([out["x"][idx], out["y"][idx]], 1)
+    codes, afk = np.ascontiguousarray(c.codes[:, kept]), af[kept]
+    exp = O.fit_pairs(codes, afk, pairs=pairs, order=O.ORDER_SEG, seg=seg_for(int(kept.sum()), codes.shape[1]))
+    assert_bitexact(out[idx], exp)
+
+
 def test_device_arithmetic_selftest():
     """recip_rn == IEEE 1/x and the positive-domain log == full log, bit for bit."""
     import ctypes
