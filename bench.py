@@ -748,7 +748,8 @@ def main():
                             and not args.soft and not buckets and args.precision == "fp64" else None),
                 "traffic_source": (SWEEP_TRAFFIC_SOURCE if args.config == "c2" and not args.segment_sites
                                    and not args.soft and not buckets and args.precision == "fp64" else None),
-                "kernel": (("sweep_gl_mixed + sweep_glm_mixed" if args.precision == "mixed" else "sweep_gl + sweep_glm")
+                # (the --multi rows are one-hot: the 2-allele part is held as 2-bit codes)
+                "kernel": (("sweep_mixed + sweep_glm_mixed" if args.precision == "mixed" else "sweep_fp64 + sweep_glm")
                            if buckets
                            else ("sweep_gl_mixed" if args.precision == "mixed" else "sweep_gl") if args.soft
                            else "sweep_mixed" if args.precision == "mixed" else "sweep_fp64"),

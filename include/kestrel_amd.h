@@ -124,7 +124,12 @@ int kestrel_cohort_create(kestrel_cohort** out, const uint8_t* packed, uint32_t 
  * (allele/mod.rs:37-134, KESTREL_MAF_DEFAULT = 1/(2N)) unless af (n_sites x 2) is given,
  * kept sites as kestrel_cohort_create.  kestrel_fit_pairs / kestrel_evaluate
  * then build c_s per pair and site as mixture_row2 (mixture.rs:96-236).
- * Biallelic sites only. */
+ * Biallelic sites only.  When every called row is one-hot (an exact 1.0
+ * beside two zeros: hard calls fed as likelihoods) the cohort is held as its
+ * 2-bit codes and fitted as a kestrel_cohort_create cohort (c_s of a one-hot
+ * row is m(gx, gy) bit for bit, so the records are the same; it can then
+ * serve).  kestrel_cohort_create_gl_multi does the same for its 2-allele
+ * part. */
 int kestrel_cohort_create_gl(kestrel_cohort** out, const double* gl, uint32_t n_samples, uint32_t n_sites,
                              const double* af, double maf, uint64_t af_iterations);
 

@@ -89,6 +89,34 @@ int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, con
     return KESTREL_OK;
 }
 
+// A likelihood-row cohort whose every called row is one-hot (an exact 1.0
+// beside two zeros: hard calls fed as likelihoods) is held as its 2-bit codes
+// alone -- the mask rows already are those codes, 3 for a missing call -- and
+// fitted by the 2-bit sweeps: c_s of a one-hot row is m(gx, gy) bit for bit
+// (mixture.rs:96-236 adds exact zeros), so every record is the same.
+int codes_if_one_hot(kestrel_cohort* c) {
+    if (!c->soft || c->gl_stride != 2 || c->n_kept == 0) return KESTREL_OK;
+    DevBuf<int> nz;
+    HIPCHK(nz.alloc(1));
+    HIPCHK(hipMemsetAsync(nz.p, 0, sizeof(int), c->stream));
+    HIPCHK(klaunch::any_nonzero(c->glrows.p, c->glrows.n, nz.p, c->stream));
+    int any = 0;
+    HIPCHK(hipMemcpyAsync(&any, nz.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (any) return KESTREL_OK;
+    c->soft = false;
+    c->glrows.release();
+    DevBuf<unsigned long long> dmiss;
+    HIPCHK(dmiss.alloc(1));
+    HIPCHK(hipMemsetAsync(dmiss.p, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(klaunch::count_missing(c->geno.p, c->n_samples, c->row_words, c->n_kept, dmiss.p, c->stream));
+    unsigned long long miss = 0;
+    HIPCHK(hipMemcpyAsync(&miss, dmiss.p, sizeof(miss), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->missing_frac = (double)miss / ((double)c->n_samples * (double)c->n_kept);
+    return KESTREL_OK;
+}
+
 // Largest likelihood of n values (NaN marks a missing call); a negative or
 // infinite likelihood is rejected (vcf.rs:368, 407 make them 10^x >= 0).
 int scan_likelihoods(const double* v, size_t n, double* lmax) {
@@ -420,6 +448,8 @@ int kestrel_cohort_create_gl(kestrel_cohort** out, const double* gl, uint32_t n_
     HIPCHK(c->af.alloc(afk.size()));
     HIPCHK(hipMemcpyAsync(c->af.p, afk.data(), afk.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    rc = codes_if_one_hot(c);
+    if (rc) return rc;
     rc = c->set_segments();
     if (rc) return rc;
     *out = guard.release();
@@ -550,6 +580,8 @@ int kestrel_cohort_create_gl_multi(kestrel_cohort** out, uint32_t n_samples, con
         if (rc) return rc;
         c->gl_stride = 3;
     }
+    rc = codes_if_one_hot(c);  // the 2-allele part (its rows are the main part's)
+    if (rc) return rc;
     for (int q = 1; q < 3; q++) {
         if (!c->part_sites[q]) continue;
         std::unique_ptr<SitePart> p(new SitePart());

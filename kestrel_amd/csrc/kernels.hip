@@ -2070,6 +2070,16 @@ __global__ __launch_bounds__(256) void gather_gl_kernel(const double* __restrict
     if (lacks_one) atomicOr(no_one, 1);
 }
 
+// *nonzero = 1 where any of rows[0..n) is not zero (the 16-byte rows of a
+// likelihood cohort hold the two entries beside the exact 1.0: all zero means
+// every called row is one-hot)
+__global__ __launch_bounds__(256) void any_nonzero_kernel(const double* __restrict__ rows, uint64_t n,
+                                                          int* __restrict__ nonzero) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool nz = i < n && rows[i] != 0.0;
+    if (__ballot(nz) && __lane_id() == 0) atomicOr(nonzero, 1);
+}
+
 // ------------------------------------------------------------ SQP step
 
 namespace {
@@ -3082,6 +3092,12 @@ hipError_t sweep_exact(int kind, const uint32_t* geno, uint32_t row_words, const
     default:
         return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t any_nonzero(const double* rows, uint64_t n, int* nonzero, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(any_nonzero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, rows, n, nonzero);
     return hipGetLastError();
 }
 

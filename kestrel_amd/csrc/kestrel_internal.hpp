@@ -78,6 +78,8 @@ hipError_t sweep_multi(int alleles, const uint32_t* geno, uint32_t row_words, co
 // dst[i] = (float)src[i] (the mixed form's FP32 coefficient tables; table32
 // above selects that form of sweep_multi)
 hipError_t round_f32(const double* src, float* dst, uint64_t n, hipStream_t st);
+// *nonzero |= 1 where rows[0..n) holds a value other than zero
+hipError_t any_nonzero(const double* rows, uint64_t n, int* nonzero, hipStream_t st);
 // evaluation at points outside the sweeps' domain, the reference's arithmetic
 // at every site (kind 0: 2-bit rows, 2 / 3: likelihood rows of that stride,
 // 13 / 14: a 3- / 4-allele part); only slots with exact[slot] != 0

@@ -477,6 +477,7 @@ int kestrel_serve_start(kestrel_cohort* c, const kestrel_tune* tune, uint32_t ma
     if (!c || !tune) return fail(KESTREL_EINVAL, "null argument");
     if (c->gt) return fail(KESTREL_EINVAL, "serving fits the ML estimator; this cohort was created for -f GT");
     if (c->soft) return fail(KESTREL_EINVAL, "serving takes 2-bit hard calls; this cohort holds likelihood rows");
+    if (!c->parts.empty()) return fail(KESTREL_EINVAL, "serving takes biallelic sites; this cohort has 3- or 4-allele sites");
     if (c->serve) return fail(KESTREL_EINVAL, "serving is already started on this cohort");
     if (c->n_kept == 0) return fail(KESTREL_EINVAL, "the cohort has no kept site");
     if (max_query_rows == 0 || max_pairs == 0) return fail(KESTREL_EINVAL, "max_query_rows and max_pairs must be > 0");

@@ -6,7 +6,7 @@ rows through the likelihood path equal the 2-bit path bit for bit."""
 import numpy as np
 import pytest
 
-from kestrel_amd import Cohort, build_arrays, relatedness_coefficients_gl, segment_bounds, synth
+from kestrel_amd import Cohort, KestrelError, build_arrays, relatedness_coefficients_gl, segment_bounds, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -39,15 +39,31 @@ def soft():
 
 
 def test_one_hot_rows_equal_the_2bit_path():
+    """A cohort of one-hot likelihood rows is held as its 2-bit codes (it
+    serves, which likelihood-row cohorts do not) with the codes cohort's
+    records; and one-hot rows through the likelihood sweep (a cohort made
+    soft by one extra sample, frequencies given) give the same records for
+    the pairs among the one-hot samples."""
     say("case: one-hot likelihood rows vs 2-bit codes, 40 x 4000")
     c = synth.generate(40, 4000, seed=33, missing=0.03)
     gl = np.eye(3)[np.minimum(c.codes, 2)]
     gl[c.codes == 3] = np.nan
-    with Cohort.gl(np.ascontiguousarray(gl.transpose(1, 0, 2))) as a, Cohort(c.codes) as b:
+    g = np.ascontiguousarray(gl.transpose(1, 0, 2))
+    with Cohort.gl(g) as a, Cohort(c.codes) as b:
         afa, ka = a.allele_freqs()
         afb, kb = b.allele_freqs()
         assert np.array_equal(afa, afb) and np.array_equal(ka, kb)
-        assert_bitexact(a.fit_pairs(), b.fit_pairs())
+        base = b.fit_pairs()
+        assert_bitexact(a.fit_pairs(), base)
+        a.serve_start(max_query_rows=4, max_pairs=1000)  # held as 2-bit codes
+        a.serve_stop()
+    soft_sample = np.full((4000, 1, 3), 0.5)
+    soft_sample[:, 0, 0] = 1.0
+    with Cohort.gl(np.concatenate([g, soft_sample], 1), af=afb) as m, Cohort(c.codes, af=afb) as b2:
+        with pytest.raises(KestrelError):
+            m.serve_start(max_query_rows=4, max_pairs=1000)  # likelihood rows
+        pairs = np.array([[i, j] for i in range(40) for j in range(i, 40)], np.uint32)
+        assert_bitexact(m.fit_pairs(pairs=pairs), b2.fit_pairs(pairs=pairs))
 
 
 def test_claim_order_tiles_keep_records(soft):
