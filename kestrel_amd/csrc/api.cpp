@@ -71,7 +71,14 @@ int repack_kept(kestrel_cohort* c, const uint8_t* dpacked, size_t row_stride_byt
 // then those of the tri/tetra-allelic parts, into consecutive partials.
 int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
                   uint32_t ns, double eps, double* partial, hipStream_t st, bool skip_masked = false,
-                  bool lanes = false) {
+                  bool lanes = false, ClassLists* cls = nullptr) {
+    if (cls) {  // the class-sorted sweep (c->class_form(); partials in c->nseg_cls segments)
+        HIPCHK(klaunch::class_build(c->geno.p, c->row_words, sx, sy, active, ns, cls->nblk, cls->lists.p,
+                                    cls->counts.p, cls->built.p, st));
+        HIPCHK(klaunch::sweep_class(c->af.p, c->v_pad, c->seg_cls.p, c->nseg_cls, cls->lists.p, cls->counts.p,
+                                    cls->nblk, pt, active, ns, eps, partial, st, (int)c->opt_class_waves));
+        return KESTREL_OK;
+    }
     if (lanes) {  // FP64 fits of 2-bit and biallelic likelihood-row cohorts (checked by the caller)
         HIPCHK(klaunch::sweep_lanes(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, sx, sy, pt, active, ns, eps,
                                     partial, st, c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride));
@@ -174,6 +181,13 @@ void tile_order(const uint32_t* pairs, size_t n, uint32_t n_samples, uint32_t T,
 uint64_t default_slots(const kestrel_cohort* c) {
     const uint64_t resident = (uint64_t)std::max(1, c->n_cu) * 2;
     return std::max<uint64_t>(1, resident / std::max<uint32_t>(1, c->nseg)) * SWEEP_TPB;
+}
+
+// ... of the class-sorted sweep: twelve pairs (three wavefronts per SIMD) per
+// CU, two device fills
+uint64_t class_slots(const kestrel_cohort* c) {
+    const uint64_t resident = (uint64_t)std::max(1, c->n_cu) * 12;
+    return std::max<uint64_t>(1, resident / std::max<uint32_t>(1, c->nseg_cls)) * 2;
 }
 
 }  // namespace
@@ -665,6 +679,16 @@ int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value) {
         c->opt_pair_lanes = value;
         return KESTREL_OK;
     }
+    if (!strcmp(key, "class_sweep")) {
+        if (value != 0 && value != 1) return fail(KESTREL_EINVAL, "class_sweep must be 0 or 1");
+        c->opt_class_sweep = value;
+        return KESTREL_OK;
+    }
+    if (!strcmp(key, "class_waves")) {
+        if (value != 4 && value != 8 && value != 12) return fail(KESTREL_EINVAL, "class_waves must be 4, 8 or 12");
+        c->opt_class_waves = value;
+        return KESTREL_OK;
+    }
     if (!strcmp(key, "skip_masked")) {
         if (value < -1 || value > 1) return fail(KESTREL_EINVAL, "skip_masked must be -1, 0 or 1");
         c->opt_skip_masked = value;
@@ -775,7 +799,11 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     if (c->opt_pair_lanes == 64 && !lanes_form)
         return fail(KESTREL_EINVAL, "pair_lanes 64 covers FP64 fits of 2-bit and biallelic likelihood-row cohorts");
     const bool lanes = c->opt_pair_lanes == 64 || (c->opt_pair_lanes < 0 && lanes_form && n_pairs <= LANES_AUTO_PAIRS);
-    const bool use_runs = !lanes && !c->soft && c->parts.empty() &&
+    // the class-sorted sweep (option "class_sweep"): one pair a wavefront over
+    // its sites sorted by genotype-pair pattern; the sweep passes over missing
+    // calls by construction, so claims in runs add nothing
+    const bool use_class = !lanes && c->class_form();
+    const bool use_runs = !lanes && !use_class && !c->soft && c->parts.empty() &&
                           (c->opt_skip_masked == 1 ||
                            (c->opt_skip_masked < 0 && !c->mixed && c->missing_frac >= 0.02));
     // pairs in flight per group: fill the device unless set (twice with runs:
@@ -785,7 +813,9 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     // 571,872 -> 593,813 on C2 and 596,779 -> 657,444 on C4; its pass over
     // masked sites fits two), never more than the pairs available
     const uint64_t fills = use_runs ? 2 : (c->mixed && !c->soft) ? 3 : 1;
-    uint64_t nslots = c->opt_slots > 0 ? (uint64_t)c->opt_slots : default_slots(c) * fills;
+    uint64_t nslots = c->opt_slots > 0 ? (uint64_t)c->opt_slots
+                      : use_class        ? class_slots(c)
+                                         : default_slots(c) * fills;
     nslots = std::min<uint64_t>(nslots, ((uint64_t)n_pairs + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB);
     nslots = (nslots + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB;
     const uint32_t ns = (uint32_t)nslots;
@@ -850,6 +880,7 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     HIPCHK(hipMemcpyAsync(dlive.p, &live0, sizeof(int), hipMemcpyHostToDevice, st));
     kdev::Tune dt = to_dev(*tune);
     if (c->mixed) dt.recond_margin = RECOND_MARGIN;
+    const uint32_t nseg_fit = use_class ? c->nseg_cls : c->nseg_total();
 
     // sweeps and steps are queued BATCH at a time between host synchronisations;
     // each step adds the slots still live after it to hist[i], so the host
@@ -863,6 +894,7 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
         DevBuf<SlotState> st;
         DevBuf<unsigned long long> wrun;  // claims in runs: each workgroup's unclaimed run part
         RunClaim rc;
+        ClassLists cls;                   // the class-sorted sweep's pattern lists
         std::vector<hipEvent_t> ev;
         std::vector<int> hist_h;
         int live_before = 0;
@@ -891,7 +923,11 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
         HIPCHK(G.sx.alloc(ns));
         HIPCHK(G.sy.alloc(ns));
         HIPCHK(G.pt.alloc((size_t)9 * ns));
-        HIPCHK(G.part.alloc((size_t)c->nseg_total() * NPART * ns));
+        HIPCHK(G.part.alloc((size_t)nseg_fit * NPART * ns));
+        if (use_class) {
+            const int rc = G.cls.alloc(ns, c->class_blocks(), G.stream);
+            if (rc) return rc;
+        }
         HIPCHK(G.active.alloc(ns));
         HIPCHK(G.hist.alloc(BATCH));
         HIPCHK(G.st.alloc(ns));
@@ -915,7 +951,7 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     // claim the first pairs
     for (int g = 0; g < ngroups; g++) {
         Group& G = grp[g];
-        HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, c->nseg_total(), dpairs.p,
+        HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, nseg_fit, dpairs.p,
                              n_pairs, dorder_p, dnext.p, dout.p, dt, dlive.p, PairQueue{}, nullptr, G.stream, G.rc));
     }
     // option "graph": each group's BATCH sweeps and steps are captured once
@@ -929,10 +965,10 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
             int rs = KESTREL_OK;
             for (int i = 0; i < BATCH && rs == KESTREL_OK; i++) {
                 rs = launch_sweeps(c, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, tune->epsilon, G.part.p, G.stream,
-                                   use_runs, lanes);
+                                   use_runs, lanes, use_class ? &G.cls : nullptr);
                 if (rs == KESTREL_OK) {
                     const hipError_t e = klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p,
-                                                       c->nseg_total(), dpairs.p, n_pairs, dorder_p, dnext.p,
+                                                       nseg_fit, dpairs.p, n_pairs, dorder_p, dnext.p,
                                                        dout.p, dt,
                                                        dlive.p, PairQueue{}, G.hist.p + i, G.stream, G.rc);
                     if (e != hipSuccess) rs = fail(KESTREL_EDEVICE, "HIP error %s", hipGetErrorString(e));
@@ -974,10 +1010,10 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
                 if (G.live_before <= 0) continue;
                 HIPCHK(hipEventRecord(G.ev[2 * i], G.stream));
                 int rs = launch_sweeps(c, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, tune->epsilon, G.part.p, G.stream,
-                                       use_runs, lanes);
+                                       use_runs, lanes, use_class ? &G.cls : nullptr);
                 if (rs) return rs;
                 HIPCHK(hipEventRecord(G.ev[2 * i + 1], G.stream));
-                HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, c->nseg_total(), dpairs.p,
+                HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, nseg_fit, dpairs.p,
                                      n_pairs, dorder_p, dnext.p, dout.p, dt, dlive.p, PairQueue{}, G.hist.p + i, G.stream,
                                      G.rc));
             }
@@ -1193,7 +1229,11 @@ int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* poi
     HIPCHK(dact.alloc(ns));
     HIPCHK(dex.alloc(ns));
     HIPCHK(dpt.alloc((size_t)9 * ns));
-    HIPCHK(dpart.alloc((size_t)c->nseg_total() * NPART * ns));
+    // the class-sorted sweep writes its one segment, the exact kernel the
+    // cohort's segments: partials start at zero and are folded over both
+    const bool use_class = c->class_form();
+    const uint32_t nfold = std::max(c->nseg_total(), use_class ? c->nseg_cls : 0u);
+    HIPCHK(dpart.alloc((size_t)nfold * NPART * ns));
     HIPCHK(df.alloc(n));
     HIPCHK(dg.alloc(9 * n));
     HIPCHK(dh.alloc(81 * n));
@@ -1204,13 +1244,20 @@ int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* poi
     HIPCHK(hipMemcpyAsync(dact.p, hact.data(), ns * sizeof(int), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dpt.p, hpt.data(), hpt.size() * sizeof(double), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dex.p, hex.data(), ns * sizeof(int), hipMemcpyHostToDevice, st));
-    int rs = launch_sweeps(c, dsx.p, dsy.p, dpt.p, dact.p, ns, eps, dpart.p, st);
+    ClassLists cls;
+    if (use_class) {
+        HIPCHK(hipMemsetAsync(dpart.p, 0, dpart.n * sizeof(double), st));
+        const int rc = cls.alloc(ns, c->class_blocks(), st);
+        if (rc) return rc;
+    }
+    int rs = launch_sweeps(c, dsx.p, dsy.p, dpt.p, dact.p, ns, eps, dpart.p, st, false, false,
+                           use_class ? &cls : nullptr);
     if (rs) return rs;
     if (any_exact) {
         rs = launch_exact(c, dsx.p, dsy.p, dpt.p, dex.p, ns, eps, dpart.p, st);
         if (rs) return rs;
     }
-    HIPCHK(klaunch::finalize(dpart.p, c->nseg_total(), ns, (uint32_t)n, df.p, dg.p, dh.p, dn.p, st));
+    HIPCHK(klaunch::finalize(dpart.p, nfold, ns, (uint32_t)n, df.p, dg.p, dh.p, dn.p, st));
     HIPCHK(hipMemcpyAsync(f, df.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(g, dg.p, 9 * n * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(h, dh.p, 81 * n * sizeof(double), hipMemcpyDeviceToHost, st));

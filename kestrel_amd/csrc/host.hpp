@@ -139,6 +139,22 @@ int launch_exact(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, cons
 // Stops the serving engine of a cohort (if any) and frees its buffers.
 void serve_release(ServeState* s);
 
+// Per-slot pattern lists of the class-sorted sweep (class_sweep.hip).
+struct ClassLists {
+    DevBuf<uint16_t> lists, counts;
+    DevBuf<uint2> built;
+    uint32_t nblk = 0;
+    int alloc(uint32_t nslots, uint32_t nblocks, hipStream_t st) {
+        nblk = nblocks;
+        HIPCHK(lists.alloc((size_t)nslots * nblk * CLASS_BLOCK));
+        HIPCHK(counts.alloc((size_t)nslots * nblk * 8));
+        HIPCHK(built.alloc(nslots));
+        // no pair's lists are built yet (0xffffffff names no sample)
+        HIPCHK(hipMemsetAsync(built.p, 0xff, (size_t)nslots * sizeof(uint2), st));
+        return 0;
+    }
+};
+
 struct kestrel_cohort {
     int device = 0;
     int n_cu = 0;
@@ -169,6 +185,10 @@ struct kestrel_cohort {
     bool mixed = false;            // precision KESTREL_MIXED: sweeps by sweep_mixed
     int64_t opt_pair_tile = -1;    // claim order in tiles of this many samples (0 list order, -1 automatic)
     int64_t opt_pair_lanes = 1;    // lanes a pair: 1, 64 (its sites across a wavefront; FP64 fits), -1 auto
+    int64_t opt_class_sweep = 0;   // 1: FP64 fits of 2-bit cohorts by the class-sorted sweep
+    int64_t opt_class_waves = CLASS_WAVES;  // its wavefronts (pairs) per workgroup: 4, 8 or 12
+    DevBuf<uint32_t> seg_cls;      // its segment bounds (multiples of CLASS_BLOCK; one segment)
+    uint32_t nseg_cls = 0;
     hipStream_t stream2 = nullptr; // second stream of a fit
     double last_sweep_ms = 0, last_wall_ms = 0, last_busy_ms = 0;
     uint64_t last_sweeps = 0;
@@ -196,6 +216,10 @@ struct kestrel_cohort {
         nseg = (uint32_t)b.size() - 1;
         HIPCHK(seg.alloc(nseg + 1));
         HIPCHK(hipMemcpy(seg.p, b.data(), b.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        const uint32_t bc[2] = {0, v_pad};
+        nseg_cls = 1;
+        HIPCHK(seg_cls.alloc(2));
+        HIPCHK(hipMemcpy(seg_cls.p, bc, sizeof bc, hipMemcpyHostToDevice));
         for (auto& p : parts) {
             std::vector<uint32_t> pb = segment_bounds(p->v_pad, part_segment_sites(p->v_pad, nseg, opt_segment_sites));
             p->nseg = (uint32_t)pb.size() - 1;
@@ -206,6 +230,10 @@ struct kestrel_cohort {
     }
 
     ServeState* serve = nullptr;  // serving engine (kestrel_serve_start)
+
+    // the class-sorted sweep covers FP64 evaluations of 2-bit cohorts
+    bool class_form() const { return opt_class_sweep == 1 && !soft && !mixed && !gt && parts.empty(); }
+    uint32_t class_blocks() const { return (v_pad + CLASS_BLOCK - 1) / CLASS_BLOCK; }
 
     ~kestrel_cohort() {
         serve_release(serve);

@@ -12,6 +12,8 @@ constexpr int LOG_GROUP = 16;   // sites per product in the objective (one log p
 constexpr int SWEEP_TPB = 256;  // threads (pairs) per sweep block
 constexpr int NPART = 56;       // f, g[9], h[45], n per segment partial
 constexpr uint32_t SITE_PAD = 256;  // rows padded to a multiple of this many sites
+constexpr int CLASS_BLOCK = 4096;   // sites per block of the class-sorted sweep's lists
+constexpr int CLASS_WAVES = 12;     // wavefronts (pairs) per class-sorted sweep workgroup (default)
 
 // PH_WAIT: the slot holds a reservation (SlotState::pair) of a serving-queue
 // entry that is not published yet
@@ -109,6 +111,17 @@ hipError_t sweep_lanes(const uint32_t* geno, uint32_t row_words, const double* a
                        uint32_t nseg, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
                        uint32_t nslots, double eps, double* partial, hipStream_t st, const double* gl = nullptr,
                        uint32_t v_pad = 0, int gl_stride = 3);
+// class-sorted sweep of 2-bit FP64 fits: per slot the pattern lists of its
+// pair (built where built[slot] differs from (sx, sy); nblk blocks of
+// CLASS_BLOCK entries, counts 8 uint16 a block), then the sweep over segments
+// whose bounds are multiples of CLASS_BLOCK; partials in sweep's layout
+hipError_t class_build(const uint32_t* geno, uint32_t row_words, const uint32_t* sx, const uint32_t* sy,
+                       const int* active, uint32_t nslots, uint32_t nblk, uint16_t* lists, uint16_t* counts,
+                       uint2* built, hipStream_t st);
+hipError_t sweep_class(const double* af, uint32_t v_pad, const uint32_t* seg_bounds, uint32_t nseg,
+                       const uint16_t* lists, const uint16_t* counts, uint32_t nblk, const double* pt,
+                       const int* active, uint32_t nslots, double eps, double* partial, hipStream_t st,
+                       int waves = CLASS_WAVES);
 hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* active, uint32_t nslots,
                 double* partial, uint32_t nseg, const uint32_t* pairs, uint64_t npairs, const uint64_t* order,
                 unsigned long long* next_pair, kestrel_output* out, const kdev::Tune& t, int* n_live,

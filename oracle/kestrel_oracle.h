@@ -53,7 +53,11 @@ extern "C" {
  * s0 + l + 64 t in order (a log per 16 steps and at the last), then a
  * butterfly adds the lanes (o = 32 .. 1: v[i] + v[i ^ o]); rows/valid/seg as
  * KO_ORDER_SEG. */
-enum { KO_ORDER_REF = 0, KO_ORDER_SEG = 1, KO_ORDER_FAST = 2, KO_ORDER_LANES = 3 };
+/* KO_ORDER_CLASS: the order of the device's class-sorted sweep (ko_class.c):
+ * per segment, blocks of 4096 sites, each genotype-pair pattern's sites in
+ * turn across 64 lanes, sums in the pattern's small basis, a butterfly over
+ * the lanes, then the expansion into g and h; hard-call cohorts only. */
+enum { KO_ORDER_REF = 0, KO_ORDER_SEG = 1, KO_ORDER_FAST = 2, KO_ORDER_LANES = 3, KO_ORDER_CLASS = 4 };
 
 /* sqp.rs:16-47 Tuneables */
 typedef struct {
@@ -111,6 +115,14 @@ double ko_jacq_obj_lanes(const double* rows, const uint8_t* valid, size_t V, con
                          size_t nseg, const double x[9], double eps);
 void ko_jacq_gh_lanes(const double* rows, const uint8_t* valid, size_t V, const uint32_t* seg, size_t nseg,
                       const double x[9], double eps, double g[9], double h[81]);
+
+/* class order (ko_class.c): one pair's sums F = sum log L, G = sum c/L,
+ * H = sum c c^T / L^2 (upper 45) and the usable-site count, from its codes
+ * gx, gy (V each, 3 = missing), af (V x 2) and segment bounds; and the
+ * expansion of a segment's 47 pattern sums into F, G, H. */
+void ko_class_sums(const uint8_t* gx, const uint8_t* gy, const double* af, const uint32_t* seg, size_t nseg,
+                   const double x[9], double eps, double* F, double G[9], double H[45], size_t* n);
+void ko_class_expand(const double* R, double* F, double G[9], double H[45]);
 
 /* cholesky.rs:6-65, 72-102, 105-140 (matrices are 9x9 row-major, first n used) */
 void ko_modify_gmw(int n, const double* a, double* l, double* d);

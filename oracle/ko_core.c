@@ -902,13 +902,28 @@ typedef struct {
     const uint8_t* valid;
     const uint32_t* seg;
     size_t nseg;
+    const uint8_t *gx, *gy; /* CLASS: the pair's codes and the frequencies */
+    const double* af;
 } pair_ctx;
+
+static void class_gh(const pair_ctx* c, const double* x, double eps, double* f, double* g, double* h) {
+    double F, G[9], H[45];
+    size_t n;
+    ko_class_sums(c->gx, c->gy, c->af, c->seg, c->nseg, x, eps, &F, G, H, &n);
+    if (f) *f = -F / (double)n;
+    if (g) gh_from_sums(G, H, n, g, h);
+}
 
 static double pair_obj(void* vc, const double* x, double eps) {
     pair_ctx* c = (pair_ctx*)vc;
     if (c->order == KO_ORDER_FAST) return ko_jacq_obj_lv_fast(&c->lv, x, eps);
     if (c->order == KO_ORDER_REF) return ko_ops()->jacq_obj(&c->lv, x, eps);
     if (c->order == KO_ORDER_LANES) return ko_jacq_obj_lanes(c->rows, c->valid, 0, c->seg, c->nseg, x, eps);
+    if (c->order == KO_ORDER_CLASS) {
+        double f;
+        class_gh(c, x, eps, &f, NULL, NULL);
+        return f;
+    }
     return ko_jacq_obj_seg(c->rows, c->valid, 0, c->seg, c->nseg, x, eps);
 }
 
@@ -918,6 +933,7 @@ static void pair_gh(void* vc, const double* x, double eps, double* g, double* h)
     if (c->order == KO_ORDER_FAST) ko_jacq_gh_lv_fast(&c->lv, x, eps, g, hh);
     else if (c->order == KO_ORDER_REF) ko_ops()->jacq_gh(&c->lv, x, eps, g, hh);
     else if (c->order == KO_ORDER_LANES) ko_jacq_gh_lanes(c->rows, c->valid, 0, c->seg, c->nseg, x, eps, g, hh);
+    else if (c->order == KO_ORDER_CLASS) class_gh(c, x, eps, NULL, g, hh);
     else ko_jacq_gh_seg(c->rows, c->valid, 0, c->seg, c->nseg, x, eps, g, hh);
     memcpy(h, hh, sizeof hh);
 }
@@ -1337,6 +1353,24 @@ static void* fit_worker(void* vj) {
             const size_t n = ko_pair_lanes_codes_fast(j->codes + (size_t)x * V, j->codes + (size_t)y * V, j->p, j->q,
                                                       V, &ctx.lv);
             o->n_sites = (uint32_t)n;
+            if (n == 0) {
+                for (int k = 0; k < 9; k++) o->jacquard[k] = NAN;
+                o->obj = NAN;
+                o->cond = NAN;
+                continue;
+            }
+            fit_ctx(&ctx, x == y, &j->tune, o);
+            continue;
+        }
+        if (j->order == KO_ORDER_CLASS) {
+            ctx.gx = j->codes + (size_t)x * V;
+            ctx.gy = j->codes + (size_t)y * V;
+            ctx.af = j->af;
+            ctx.seg = j->seg;
+            ctx.nseg = j->nseg;
+            uint32_t n = 0;
+            for (size_t s = 0; s < V; s++) n += ctx.gx[s] < 3 && ctx.gy[s] < 3;
+            o->n_sites = n;
             if (n == 0) {
                 for (int k = 0; k < 9; k++) o->jacquard[k] = NAN;
                 o->obj = NAN;

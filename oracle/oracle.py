@@ -18,6 +18,8 @@ ORDER_REF = 0
 ORDER_SEG = 1
 ORDER_FAST = 2  # ORDER_REF in the reference's AVX-512 loop shape (same bits; timing form)
 ORDER_LANES = 3  # the segments of ORDER_SEG summed by 64 strided lanes and a butterfly
+ORDER_CLASS = 4  # the class-sorted sweep's order: genotype-pair patterns per 4096-site block (ko_class.c)
+CLASS_BLOCK = 4096
 
 
 class Tune(ctypes.Structure):
@@ -83,6 +85,8 @@ def lib():
         L.ko_jacq_obj_lanes.restype = D
         L.ko_jacq_obj_lanes.argtypes = [P, P, SZ, P, SZ, P, D]
         L.ko_jacq_gh_lanes.argtypes = [P, P, SZ, P, SZ, P, D, P, P]
+        L.ko_class_sums.argtypes = [P, P, P, P, SZ, P, D, P, P, P, P]
+        L.ko_class_expand.argtypes = [P, P, P, P]
         L.ko_modify_gmw.argtypes = [I, P, P, P]
         L.ko_unmodified.argtypes = [I, P, P, P]
         L.ko_solve_ldl.argtypes = [I, P, P, P]
@@ -257,6 +261,32 @@ def fit_rows(rows, self_pair=False, order=ORDER_REF, valid=None, seg=None, itera
         lib().ko_fit_pair_rows(order, _p(rows), rows.shape[0], _p(valid), _p(seg), len(seg) - 1,
                                int(self_pair), ctypes.byref(t), _p(out))
     return out[0]
+
+
+def class_sums(gx, gy, af, x, eps=1e-8, seg=None):
+    """One pair's sums in the class order: (F, G[9], H[45], n) with F = sum log L,
+    G = sum c/L, H = sum c c^T / L^2 (upper triangle, row-major)."""
+    gx, gy = np.ascontiguousarray(gx, np.uint8), np.ascontiguousarray(gy, np.uint8)
+    af, x = f64(af), f64(x)
+    if seg is None:
+        seg = np.array([0, gx.shape[0]], np.uint32)
+    seg = _seg(seg)
+    F = np.zeros(1)
+    G = np.zeros(9)
+    H = np.zeros(45)
+    n = np.zeros(1, np.uint64)
+    lib().ko_class_sums(_p(gx), _p(gy), _p(af), _p(seg), len(seg) - 1, _p(x), eps, _p(F), _p(G), _p(H), _p(n))
+    return float(F[0]), G, H, int(n[0])
+
+
+def class_expand(R):
+    """The expansion of a segment's 47 pattern sums into (F, G[9], H[45])."""
+    R = f64(R)
+    F = np.zeros(1)
+    G = np.zeros(9)
+    H = np.zeros(45)
+    lib().ko_class_expand(_p(R), _p(F), _p(G), _p(H))
+    return float(F[0]), G, H
 
 
 def pair_rows_codes(gx, gy, af):
