@@ -183,10 +183,10 @@ uint64_t default_slots(const kestrel_cohort* c) {
     return std::max<uint64_t>(1, resident / std::max<uint32_t>(1, c->nseg)) * SWEEP_TPB;
 }
 
-// ... of the class-sorted sweep: twelve pairs (three wavefronts per SIMD) per
-// CU, two device fills
+// ... of the class-sorted sweep: one workgroup of sixteen pairs (four
+// wavefronts per SIMD) per CU, two device fills
 uint64_t class_slots(const kestrel_cohort* c) {
-    const uint64_t resident = (uint64_t)std::max(1, c->n_cu) * 12;
+    const uint64_t resident = (uint64_t)std::max(1, c->n_cu) * 16;
     return std::max<uint64_t>(1, resident / std::max<uint32_t>(1, c->nseg_cls)) * 2;
 }
 
@@ -685,7 +685,8 @@ int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value) {
         return KESTREL_OK;
     }
     if (!strcmp(key, "class_waves")) {
-        if (value != 4 && value != 8 && value != 12) return fail(KESTREL_EINVAL, "class_waves must be 4, 8 or 12");
+        if (value != 8 && value != 12 && value != 14 && value != 16)
+            return fail(KESTREL_EINVAL, "class_waves must be 8, 12, 14 or 16");
         c->opt_class_waves = value;
         return KESTREL_OK;
     }

@@ -19,9 +19,13 @@
 // class_build_kernel sorts a pair's sites once, when its slot takes it: per
 // block of CLASS_BLOCK sites, the list of each pattern's sites in ascending
 // order (entries (site in block << 1) | flag, flag = the pattern's second
-// genotype pair) and the five list lengths.  sweep_class_fp64 then stages the
-// block's frequencies in LDS for the workgroup's pairs and walks the lists:
-// entry j of a pattern's list is lane j % 64's site in chunk j / 64.
+// genotype pair) and the five list lengths.  Entry j of a pattern's list is
+// lane j % 64's site in chunk j / 64; in memory each pattern starts on a group
+// of 256 entries, and a group holds lane l's entries of its four chunks side
+// by side (entry j at (j / 256) 256 + 4 (j % 64) + (j / 64) % 4), so one
+// 8-byte load gives a lane four chunks.  sweep_class_fp64 stages each block's
+// frequencies in LDS for the workgroup's pairs (asynchronously, a block ahead)
+// and walks the groups of the block's lists, loaded two groups ahead.
 //
 // The summation order (each lane's sums in list order, a log of the lane's
 // product after every 16th chunk of the segment and at its end, a butterfly
@@ -63,21 +67,25 @@ __device__ __forceinline__ uint32_t wave_scan(uint32_t v, uint32_t lane) {
 }  // namespace
 
 // One wavefront per slot whose pair (sx, sy) differs from the one its lists
-// were built for (built[slot]); four a workgroup.  Lane l takes the sites
-// 64 l .. 64 l + 63 of each block (four words of each row), counts its sites
-// of each pattern, a wavefront scan gives each lane its positions, and the
-// lane writes its entries in site order through per-lane LDS counters.
-__global__ __launch_bounds__(256) void class_build_kernel(const uint32_t* __restrict__ geno, uint32_t row_words,
-                                                          const uint32_t* __restrict__ sx,
-                                                          const uint32_t* __restrict__ sy,
-                                                          const int* __restrict__ active, uint32_t nslots,
-                                                          uint32_t nblk, uint16_t* __restrict__ lists,
-                                                          uint16_t* __restrict__ counts, uint2* __restrict__ built) {
-    __shared__ uint32_t ctr[4][5 * 64];
-    __shared__ __attribute__((aligned(16))) uint16_t stage[4][CLASS_BLOCK];
+// were built for (built[slot]); two a workgroup.  Lane l takes the sites
+// 128 l .. 128 l + 127 of each block (eight words of each row), counts its
+// sites of each pattern, and a wavefront scan gives each lane the position of
+// its first site in its pattern's list; the lane then writes its entries in
+// site order into the LDS copy of the block's lists through per-lane LDS
+// counters (pattern k's list starting at the group-aligned pb[k]).  The copy
+// is written out in the group layout: lane l's entries j, j + 64, j + 128,
+// j + 192 of a group side by side.
+constexpr int BUILD_WAVES = 2;
+
+__global__ __launch_bounds__(BUILD_WAVES * 64) void class_build_kernel(
+    const uint32_t* __restrict__ geno, uint32_t row_words, const uint32_t* __restrict__ sx,
+    const uint32_t* __restrict__ sy, const int* __restrict__ active, uint32_t nslots, uint32_t nblk,
+    uint16_t* __restrict__ lists, uint16_t* __restrict__ counts, uint2* __restrict__ built) {
+    __shared__ uint32_t ctr[BUILD_WAVES][5 * 64];
+    __shared__ __attribute__((aligned(16))) uint16_t stage[BUILD_WAVES][CLASS_STRIDE];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const uint32_t slot = blockIdx.x * 4 + w;
+    const uint32_t slot = blockIdx.x * BUILD_WAVES + w;
     if (slot >= nslots || active[slot] == 0) return;  // uniform over the wavefront
     const uint32_t x = sx[slot], y = sy[slot];
     const uint2 b0 = built[slot];
@@ -86,17 +94,23 @@ __global__ __launch_bounds__(256) void class_build_kernel(const uint32_t* __rest
     const uint32_t* rowy = geno + (size_t)y * row_words;
     uint32_t* c = ctr[w];
     uint16_t* sg = stage[w];
+    constexpr int WL = CLASS_BLOCK / 16 / 64;  // words of a row a lane takes in a block
     for (uint32_t b = 0; b < nblk; b++) {
-        const uint32_t w0 = b * (CLASS_BLOCK / 16) + 4 * lane;
-        uint4 wx = make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu), wy = wx;
-        if (w0 < row_words) {  // row_words is a multiple of 16: the four words are in the row
-            wx = *reinterpret_cast<const uint4*>(rowx + w0);
-            wy = *reinterpret_cast<const uint4*>(rowy + w0);
+        const uint32_t w0 = b * (CLASS_BLOCK / 16) + WL * lane;
+        uint32_t ax[WL], ay[WL];
+#pragma unroll
+        for (int h = 0; h < WL; h += 4) {
+            uint4 wx = make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu), wy = wx;
+            if (w0 + h < row_words) {  // row_words is a multiple of 16: the four words are in the row
+                wx = *reinterpret_cast<const uint4*>(rowx + w0 + h);
+                wy = *reinterpret_cast<const uint4*>(rowy + w0 + h);
+            }
+            ax[h] = wx.x, ax[h + 1] = wx.y, ax[h + 2] = wx.z, ax[h + 3] = wx.w;
+            ay[h] = wy.x, ay[h + 1] = wy.y, ay[h + 2] = wy.z, ay[h + 3] = wy.w;
         }
-        const uint32_t ax[4] = {wx.x, wx.y, wx.z, wx.w}, ay[4] = {wy.x, wy.y, wy.z, wy.w};
         uint32_t cnt[5] = {0, 0, 0, 0, 0};
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < WL; q++) {
             const uint32_t m = 0x55555555u;
             const uint32_t xl = ax[q] & m, xh = (ax[q] >> 1) & m, yl = ay[q] & m, yh = (ay[q] >> 1) & m;
             const uint32_t x0 = ~(xl | xh) & m, x1 = xl & ~xh, x2 = xh & ~xl;
@@ -107,7 +121,6 @@ __global__ __launch_bounds__(256) void class_build_kernel(const uint32_t* __rest
             cnt[3] += __popc((x0 & y2) | (x2 & y0));
             cnt[4] += __popc(x1 & y1);
         }
-        // positions: pattern k's list starts after the lists of patterns < k
         const uint32_t s01 = wave_scan(cnt[0] | (cnt[1] << 16), lane);
         const uint32_t s23 = wave_scan(cnt[2] | (cnt[3] << 16), lane);
         const uint32_t s4 = wave_scan(cnt[4], lane);
@@ -116,15 +129,16 @@ __global__ __launch_bounds__(256) void class_build_kernel(const uint32_t* __rest
         const uint32_t t4 = (uint32_t)__builtin_amdgcn_readlane((int)s4, 63);
         const uint32_t tot[5] = {t01 & 0xffffu, t01 >> 16, t23 & 0xffffu, t23 >> 16, t4};
         const uint32_t inc[5] = {s01 & 0xffffu, s01 >> 16, s23 & 0xffffu, s23 >> 16, s4};
-        uint32_t base = 0;
+        uint32_t pb[6];
+        pb[0] = 0;
 #pragma unroll
         for (int k = 0; k < 5; k++) {
-            c[k * 64 + lane] = base + inc[k] - cnt[k];
-            base += tot[k];
+            c[k * 64 + lane] = pb[k] + inc[k] - cnt[k];
+            pb[k + 1] = pb[k] + (tot[k] + CLASS_GROUP - 1) / CLASS_GROUP * CLASS_GROUP;
         }
         wave_sync();
 #pragma unroll
-        for (int q = 0; q < 4; q++)
+        for (int q = 0; q < WL; q++)
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 const uint32_t idx = (((ax[q] >> (2 * i)) & 3u) << 2) | ((ay[q] >> (2 * i)) & 3u);
@@ -132,15 +146,17 @@ __global__ __launch_bounds__(256) void class_build_kernel(const uint32_t* __rest
                 const uint32_t k = nib & 7u;
                 if (k != 7u) {
                     const uint32_t pos = atomicAdd(&c[k * 64 + lane], 1u);
-                    sg[pos] = (uint16_t)(((64u * lane + 16u * q + (uint32_t)i) << 1) | (nib >> 3));
+                    sg[pos] = (uint16_t)(((CLASS_BLOCK / 64 * lane + 16u * q + (uint32_t)i) << 1) | (nib >> 3));
                 }
             }
         wave_sync();
-        // the block's entries (base of them) to the slot's lists, 8 a lane-step
-        uint16_t* dst = lists + ((size_t)slot * nblk + b) * CLASS_BLOCK;
-        const uint32_t units = (base + 7) / 8;
-        for (uint32_t u = lane; u < units; u += 64)
-            reinterpret_cast<uint4*>(dst)[u] = reinterpret_cast<const uint4*>(sg)[u];
+        // out in the group layout: a lane's four entries of a group side by side
+        uint2* dst = reinterpret_cast<uint2*>(lists + ((size_t)slot * nblk + b) * CLASS_STRIDE);
+        for (uint32_t g = 0; g < pb[5] / CLASS_GROUP; g++) {
+            const uint16_t* src = sg + g * CLASS_GROUP + lane;
+            dst[(size_t)g * 64 + lane] =
+                make_uint2((uint32_t)src[0] | ((uint32_t)src[64] << 16), (uint32_t)src[128] | ((uint32_t)src[192] << 16));
+        }
         if (lane == 0) {
             uint4 cv;
             cv.x = tot[0] | (tot[1] << 16);
@@ -308,47 +324,150 @@ __device__ __forceinline__ void class_expand_store(const double* R, double* out,
         }
 }
 
-// the chunks of one pattern's list in a block (entries blk[off .. off + n)):
-// each lane's entry is loaded three chunks ahead and its frequencies read from
-// LDS one chunk ahead (reads past the list stay inside the block's region and
-// are masked to a site of the block; their lanes compute nothing); t counts
-// the segment's chunks, and every 16th ends each lane's log group
+// One group of pattern K's list: the four chunks j0 = jg, jg + 64, ... whose
+// entries are q (lane l's four entries side by side); a chunk past the list's
+// end stops the group.  t counts the segment's chunks, and every 16th ends
+// each lane's log group.
 template <int K>
-__device__ __forceinline__ void class_pattern(const uint16_t* __restrict__ blk, uint32_t off, uint32_t n,
-                                              uint32_t lane, const double2* afb, double* A, double& P, uint32_t& t,
-                                              const ClassCoef& c, double eps) {
-    if (n == 0) return;
-    auto entry = [&](uint32_t j) { return (uint32_t)blk[min(off + j, (uint32_t)CLASS_BLOCK - 1)]; };
-    uint32_t e = entry(lane), e1 = entry(64 + lane), e2 = entry(128 + lane);
-    double2 f = afb[(e >> 1) & (CLASS_BLOCK - 1)];
-    for (uint32_t j0 = 0; j0 < n; j0 += 64) {
-        const uint32_t e3 = entry(j0 + 192 + lane);
-        const double2 f1 = afb[(e1 >> 1) & (CLASS_BLOCK - 1)];
-        if (j0 + lane < n) class_site<K>(A, P, e & 1u, f.x, f.y, c, eps);
-        if ((++t & (LOG_GROUP - 1)) == 0) {
-            A[0] = A[0] + sleef_log_pos(P);
-            P = 1.0;
-        }
-        e = e1;
-        e1 = e2;
-        e2 = e3;
-        f = f1;
+__device__ __forceinline__ void class_chunk(uint32_t e, bool on, const double2* afb, double* A, double& P,
+                                            uint32_t& t, const ClassCoef& c, double eps) {
+    const double2 f = afb[(e >> 1) & (CLASS_BLOCK - 1)];
+    if (on) class_site<K>(A, P, e & 1u, f.x, f.y, c, eps);
+    if ((++t & (LOG_GROUP - 1)) == 0) {
+        A[0] = A[0] + sleef_log_pos(P);
+        P = 1.0;
     }
+}
+
+template <int K>
+__device__ __forceinline__ void class_group(uint2 q, uint32_t jg, uint32_t n, uint32_t lane, const double2* afb,
+                                            double* A, double& P, uint32_t& t, const ClassCoef& c, double eps) {
+    if (jg + CLASS_GROUP <= n) {  // four whole chunks
+        class_chunk<K>(q.x & 0xffffu, true, afb, A, P, t, c, eps);
+        class_chunk<K>(q.x >> 16, true, afb, A, P, t, c, eps);
+        class_chunk<K>(q.y & 0xffffu, true, afb, A, P, t, c, eps);
+        class_chunk<K>(q.y >> 16, true, afb, A, P, t, c, eps);
+        return;
+    }
+    const uint32_t m = (n - jg + 63) / 64;  // chunks of the group within the list
+    for (uint32_t sub = 0; sub < m; sub++) {
+        const uint32_t e = ((sub < 2 ? q.x : q.y) >> (16 * (sub & 1))) & 0xffffu;
+        class_chunk<K>(e, jg + 64 * sub + lane < n, afb, A, P, t, c, eps);
+    }
+}
+
+// the (p, q) of the block's sites into LDS without registers: each 64-lane
+// global_load_lds_dwordx4 moves 64 consecutive sites (a block holds a multiple
+// of 64 sites: rows are padded to 256)
+template <int W>
+__device__ __forceinline__ void stage_af(const double* __restrict__ af, uint32_t site0, uint32_t nsite, double2* buf,
+                                         uint32_t w, uint32_t lane) {
+    for (uint32_t u = w; 64u * u < nsite; u += W)
+        __builtin_amdgcn_global_load_lds((const void*)(af + 2 * ((size_t)site0 + 64u * u + lane)),
+                                         (__attribute__((address_space(3))) void*)(buf + 64u * u), 16, 0, 0);
 }
 
 }  // namespace
 
-// W wavefronts a workgroup, one slot each; blockIdx.y = segment, whose bounds
-// are multiples of CLASS_BLOCK (the last ends the rows).  Per block the
-// workgroup stages the sites' (p, q) in LDS, then each wavefront walks its
-// pair's five lists.  Partials (f, g[9], h[45], n) as sweep_fp64 writes them.
+// The groups [g0, g1) of one block's lists for pass PASS (0: the HS list;
+// 1: the B, C, D and E lists), each lane's 8-byte share of a group loaded two
+// groups ahead.
+// group g (of the block's groups [g0, g1) for patterns K0..K1) of the pattern
+// it belongs to; gend[k] = the groups of patterns 0..k
+template <int K0, int K1>
+__device__ __forceinline__ void class_group_of(uint2 q, uint32_t g, const uint32_t* nk, const uint32_t* gend,
+                                               uint32_t lane, const double2* afb, double* A, double& P, uint32_t& t,
+                                               const ClassCoef& c, double eps) {
+    if constexpr (K0 == K1) {
+        class_group<K0>(q, (g - (K0 ? gend[K0 - 1] : 0)) * CLASS_GROUP, nk[K0], lane, afb, A, P, t, c, eps);
+    } else {
+        if (g < gend[K0])
+            class_group<K0>(q, (g - gend[K0 - 1]) * CLASS_GROUP, nk[K0], lane, afb, A, P, t, c, eps);
+        else
+            class_group<K1>(q, (g - gend[K0]) * CLASS_GROUP, nk[K1], lane, afb, A, P, t, c, eps);
+    }
+}
+
+// One pass over the lists of patterns K0..K1: per block of the segment the
+// workgroup stages the block's (p, q) in LDS, then each live wavefront walks
+// its pair's lists of those patterns in the block, a group (four chunks) at a
+// time, two groups in flight (alternate registers, so that no loaded register
+// is copied before its use).  The patterns' sums are then added over the
+// lanes and kept in LDS.
+template <int K0, int K1, int W>
+__device__ __forceinline__ void class_pass(const double* __restrict__ af, uint32_t v_pad, uint32_t bb0, uint32_t nb,
+                                           const uint16_t* __restrict__ lists, const uint16_t* __restrict__ counts,
+                                           uint32_t nblk, uint32_t slot, bool live, uint32_t w, uint32_t lane,
+                                           double2* afb, double* sums, double& F, double& P, uint32_t& t,
+                                           uint32_t& n, const ClassCoef& c, double eps) {
+    constexpr int FIRST[6] = {1, 12, 20, 28, 42, 47};  // the sums of pattern k: A[FIRST[k] .. FIRST[k + 1])
+    constexpr int O = FIRST[K0], E = FIRST[K1 + 1];
+    double A[NACC];
+#pragma unroll
+    for (int k = 0; k < NACC; k++) A[k] = 0.0;
+    A[0] = F;
+    for (uint32_t i = 0; i < nb; i++) {
+        const uint32_t b = bb0 + i;
+        const uint32_t site0 = b * CLASS_BLOCK;
+        __syncthreads();  // the previous block's frequencies are no longer read
+        stage_af<W>(af, site0, min((uint32_t)CLASS_BLOCK, v_pad - site0), afb, w, lane);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (!live) continue;
+        const uint4 cv = reinterpret_cast<const uint4*>(counts)[(size_t)slot * nblk + b];
+        const uint32_t nk[5] = {cv.x & 0xffffu, cv.x >> 16, cv.y & 0xffffu, cv.y >> 16, cv.z};
+        uint32_t gend[5], gt = 0;
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            gt += (nk[k] + CLASS_GROUP - 1) / CLASS_GROUP;
+            gend[k] = gt;
+        }
+        if (K0 == 0) n += nk[0] + nk[1] + nk[2] + nk[3] + nk[4];
+        const uint32_t g0 = K0 ? gend[K0 - 1] : 0, g1 = gend[K1];
+        const uint2* gp = reinterpret_cast<const uint2*>(lists + ((size_t)slot * nblk + b) * CLASS_STRIDE) + lane;
+        uint2 qa = make_uint2(0, 0), qb = make_uint2(0, 0);
+        if (g0 < g1) qa = gp[(size_t)g0 * 64];
+        if (g0 + 1 < g1) qb = gp[(size_t)(g0 + 1) * 64];
+        for (uint32_t g = g0; g < g1; g += 2) {
+            class_group_of<K0, K1>(qa, g, nk, gend, lane, afb, A, P, t, c, eps);
+            if (g + 2 < g1) qa = gp[(size_t)(g + 2) * 64];
+            if (g + 1 >= g1) break;
+            class_group_of<K0, K1>(qb, g + 1, nk, gend, lane, afb, A, P, t, c, eps);
+            if (g + 3 < g1) qb = gp[(size_t)(g + 3) * 64];
+        }
+    }
+    F = A[0];
+    if (!live) return;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int k = O; k < E; k++) A[k] = A[k] + __shfl_xor(A[k], o, 64);
+    if (lane == 0)
+#pragma unroll
+        for (int k = O; k < E; k++) sums[k] = A[k];
+}
+
+// wavefronts per SIMD the register budget is set for (one workgroup per CU:
+// the block's frequencies fill the LDS)
 template <int W>
-__global__ __launch_bounds__(W * 64, 3) void sweep_class_fp64(
+constexpr int class_waves_per_simd() {
+    return W / 4;
+}
+
+// W wavefronts a workgroup, one slot each; blockIdx.y = segment, whose bounds
+// are multiples of CLASS_BLOCK (the last ends the rows).  Three passes over
+// the segment's blocks -- the HS lists, the B and C lists, the D and E lists
+// -- so that a pass holds only its patterns' sums (and the objective) in
+// registers; the lane sums of each pattern are kept in LDS for the expansion.
+// Partials (f, g[9], h[45], n) as sweep_fp64 writes them.
+template <int W>
+__global__ __launch_bounds__(W * 64, class_waves_per_simd<W>()) void sweep_class_fp64(
     const double* __restrict__ af, uint32_t v_pad, const uint32_t* __restrict__ seg_bounds,
     const uint16_t* __restrict__ lists, const uint16_t* __restrict__ counts, uint32_t nblk,
     const double* __restrict__ pt, const int* __restrict__ active, uint32_t nslots, double eps,
     double* __restrict__ partial) {
     __shared__ double2 afb[CLASS_BLOCK];
+    __shared__ double sums[W][NACC + 1];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const uint32_t slot = blockIdx.x * W + w;
@@ -356,50 +475,30 @@ __global__ __launch_bounds__(W * 64, 3) void sweep_class_fp64(
     if (!__syncthreads_or(live)) return;
     const uint32_t seg = blockIdx.y;
     const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
-    const uint32_t bb0 = s0 / CLASS_BLOCK, bb1 = (s1 + CLASS_BLOCK - 1) / CLASS_BLOCK;
+    const uint32_t bb0 = s0 / CLASS_BLOCK, nb = (s1 + CLASS_BLOCK - 1) / CLASS_BLOCK - bb0;
 
     double D[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) D[k] = live ? pt[(size_t)k * nslots + slot] : 0.0;
     const ClassCoef c = class_coef(D);
-    double A[NACC];
-#pragma unroll
-    for (int k = 0; k < NACC; k++) A[k] = 0.0;
-    double P = 1.0;
+    double F = 0.0, P = 1.0;
     uint32_t t = 0, n = 0;
-    for (uint32_t b = bb0; b < bb1; b++) {
-        const uint32_t site0 = b * CLASS_BLOCK;
-        const uint32_t nsite = min((uint32_t)CLASS_BLOCK, v_pad - site0);
-        __syncthreads();  // the previous block's frequencies are no longer read
-        for (uint32_t i = threadIdx.x; i < nsite; i += W * 64)
-            afb[i] = *reinterpret_cast<const double2*>(af + 2 * ((size_t)site0 + i));
-        __syncthreads();
-        if (!live) continue;
-        const uint4 cv = reinterpret_cast<const uint4*>(counts)[(size_t)slot * nblk + b];
-        const uint32_t nk[5] = {cv.x & 0xffffu, cv.x >> 16, cv.y & 0xffffu, cv.y >> 16, cv.z};
-        const uint16_t* blk = lists + ((size_t)slot * nblk + b) * CLASS_BLOCK;
-        uint32_t off = 0;
-        class_pattern<0>(blk, off, nk[0], lane, afb, A, P, t, c, eps);
-        off += nk[0];
-        class_pattern<1>(blk, off, nk[1], lane, afb, A, P, t, c, eps);
-        off += nk[1];
-        class_pattern<2>(blk, off, nk[2], lane, afb, A, P, t, c, eps);
-        off += nk[2];
-        class_pattern<3>(blk, off, nk[3], lane, afb, A, P, t, c, eps);
-        off += nk[3];
-        class_pattern<4>(blk, off, nk[4], lane, afb, A, P, t, c, eps);
-        n += nk[0] + nk[1] + nk[2] + nk[3] + nk[4];
-    }
+    double* my = sums[w];
+    class_pass<0, 0, W>(af, v_pad, bb0, nb, lists, counts, nblk, slot, live, w, lane, afb, my, F, P, t, n, c, eps);
+    class_pass<1, 2, W>(af, v_pad, bb0, nb, lists, counts, nblk, slot, live, w, lane, afb, my, F, P, t, n, c, eps);
+    class_pass<3, 4, W>(af, v_pad, bb0, nb, lists, counts, nblk, slot, live, w, lane, afb, my, F, P, t, n, c, eps);
     if (!live) return;
-    if (t & (LOG_GROUP - 1)) A[0] = A[0] + sleef_log_pos(P);
+    if (t & (LOG_GROUP - 1)) F = F + sleef_log_pos(P);
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-#pragma unroll
-        for (int k = 0; k < NACC; k++) A[k] = A[k] + __shfl_xor(A[k], o, 64);
+    for (int o = 32; o > 0; o >>= 1) F = F + __shfl_xor(F, o, 64);
     if (lane == 0) {
+        double R[NACC];
+        R[0] = F;
+#pragma unroll
+        for (int k = 1; k < NACC; k++) R[k] = my[k];
         double* out = partial + (size_t)seg * NPART * nslots + slot;
-        out[0] = A[0];
-        class_expand_store(A, out, nslots);
+        out[0] = F;
+        class_expand_store(R, out, nslots);
         out[(size_t)55 * nslots] = (double)n;
     }
 }
@@ -409,8 +508,8 @@ namespace klaunch {
 hipError_t class_build(const uint32_t* geno, uint32_t row_words, const uint32_t* sx, const uint32_t* sy,
                        const int* active, uint32_t nslots, uint32_t nblk, uint16_t* lists, uint16_t* counts,
                        uint2* built, hipStream_t st) {
-    hipLaunchKernelGGL(class_build_kernel, dim3((nslots + 3) / 4), dim3(256), 0, st, geno, row_words, sx, sy, active,
-                       nslots, nblk, lists, counts, built);
+    hipLaunchKernelGGL(class_build_kernel, dim3((nslots + BUILD_WAVES - 1) / BUILD_WAVES), dim3(BUILD_WAVES * 64), 0,
+                       st, geno, row_words, sx, sy, active, nslots, nblk, lists, counts, built);
     return hipGetLastError();
 }
 
@@ -421,12 +520,14 @@ hipError_t sweep_class(const double* af, uint32_t v_pad, const uint32_t* seg_bou
         hipLaunchKernelGGL(kern, dim3((nslots + W - 1) / W, nseg), dim3(W * 64), 0, st, af, v_pad, seg_bounds, lists,
                            counts, nblk, pt, active, nslots, eps, partial);
     };
-    if (waves == 4)
-        go(sweep_class_fp64<4>, 4);
-    else if (waves == 8)
+    if (waves == 8)
         go(sweep_class_fp64<8>, 8);
     else if (waves == 12)
         go(sweep_class_fp64<12>, 12);
+    else if (waves == 14)
+        go(sweep_class_fp64<14>, 14);
+    else if (waves == 16)
+        go(sweep_class_fp64<16>, 16);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

@@ -146,7 +146,7 @@ struct ClassLists {
     uint32_t nblk = 0;
     int alloc(uint32_t nslots, uint32_t nblocks, hipStream_t st) {
         nblk = nblocks;
-        HIPCHK(lists.alloc((size_t)nslots * nblk * CLASS_BLOCK));
+        HIPCHK(lists.alloc((size_t)nslots * nblk * CLASS_STRIDE));
         HIPCHK(counts.alloc((size_t)nslots * nblk * 8));
         HIPCHK(built.alloc(nslots));
         // no pair's lists are built yet (0xffffffff names no sample)
@@ -186,7 +186,7 @@ struct kestrel_cohort {
     int64_t opt_pair_tile = -1;    // claim order in tiles of this many samples (0 list order, -1 automatic)
     int64_t opt_pair_lanes = 1;    // lanes a pair: 1, 64 (its sites across a wavefront; FP64 fits), -1 auto
     int64_t opt_class_sweep = 0;   // 1: FP64 fits of 2-bit cohorts by the class-sorted sweep
-    int64_t opt_class_waves = CLASS_WAVES;  // its wavefronts (pairs) per workgroup: 4, 8 or 12
+    int64_t opt_class_waves = CLASS_WAVES;  // its wavefronts (pairs) per workgroup: 8, 12, 14 or 16
     DevBuf<uint32_t> seg_cls;      // its segment bounds (multiples of CLASS_BLOCK; one segment)
     uint32_t nseg_cls = 0;
     hipStream_t stream2 = nullptr; // second stream of a fit

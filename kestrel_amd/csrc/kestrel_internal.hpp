@@ -12,8 +12,10 @@ constexpr int LOG_GROUP = 16;   // sites per product in the objective (one log p
 constexpr int SWEEP_TPB = 256;  // threads (pairs) per sweep block
 constexpr int NPART = 56;       // f, g[9], h[45], n per segment partial
 constexpr uint32_t SITE_PAD = 256;  // rows padded to a multiple of this many sites
-constexpr int CLASS_BLOCK = 4096;   // sites per block of the class-sorted sweep's lists
-constexpr int CLASS_WAVES = 12;     // wavefronts (pairs) per class-sorted sweep workgroup (default)
+constexpr int CLASS_BLOCK = 8192;   // sites per block of the class-sorted sweep's lists
+constexpr int CLASS_GROUP = 256;    // list entries per group (four chunks of 64)
+constexpr int CLASS_STRIDE = CLASS_BLOCK + 5 * CLASS_GROUP;  // list entries per block (patterns group-aligned)
+constexpr int CLASS_WAVES = 16;     // wavefronts (pairs) per class-sorted sweep workgroup (default)
 
 // PH_WAIT: the slot holds a reservation (SlotState::pair) of a serving-queue
 // entry that is not published yet
@@ -113,7 +115,7 @@ hipError_t sweep_lanes(const uint32_t* geno, uint32_t row_words, const double* a
                        uint32_t v_pad = 0, int gl_stride = 3);
 // class-sorted sweep of 2-bit FP64 fits: per slot the pattern lists of its
 // pair (built where built[slot] differs from (sx, sy); nblk blocks of
-// CLASS_BLOCK entries, counts 8 uint16 a block), then the sweep over segments
+// CLASS_STRIDE entries, counts 8 uint16 a block), then the sweep over segments
 // whose bounds are multiples of CLASS_BLOCK; partials in sweep's layout
 hipError_t class_build(const uint32_t* geno, uint32_t row_words, const uint32_t* sx, const uint32_t* sy,
                        const int* active, uint32_t nslots, uint32_t nblk, uint16_t* lists, uint16_t* counts,

@@ -54,7 +54,7 @@ extern "C" {
  * butterfly adds the lanes (o = 32 .. 1: v[i] + v[i ^ o]); rows/valid/seg as
  * KO_ORDER_SEG. */
 /* KO_ORDER_CLASS: the order of the device's class-sorted sweep (ko_class.c):
- * per segment, blocks of 4096 sites, each genotype-pair pattern's sites in
+ * per segment, blocks of 8192 sites, each genotype-pair pattern's sites in
  * turn across 64 lanes, sums in the pattern's small basis, a butterfly over
  * the lanes, then the expansion into g and h; hard-call cohorts only. */
 enum { KO_ORDER_REF = 0, KO_ORDER_SEG = 1, KO_ORDER_FAST = 2, KO_ORDER_LANES = 3, KO_ORDER_CLASS = 4 };

@@ -24,10 +24,10 @@
  * reference's in rounding only: tests/test_oracle_class.py holds them to the
  * reference order within 1e-13 relative).
  *
- * Order: per segment [s0, s1) of the pair's sites, blocks of 4096 sites from
- * s0; in each block the patterns HS, B, C, D, E in turn, each over the list of
- * its sites in ascending order; entry j of a list goes to lane j mod 64 in
- * chunk j / 64.  Each of the 64 lanes keeps its own sums from zero (plain
+ * Order: per segment [s0, s1) of the pair's sites, blocks of 8192 sites from
+ * s0, taken in three passes -- HS; B and C; D and E -- each walking every
+ * block's lists of its patterns in turn, each list in ascending site order;
+ * entry j of a list goes to lane j mod 64 in chunk j / 64.  Each of the 64 lanes keeps its own sums from zero (plain
  * additions for the gradient terms, fma for the Hessian terms) and its own log
  * product; after every 16th chunk of the segment (counted across blocks and
  * patterns) and at the segment's end each lane adds log(product) to its
@@ -44,7 +44,7 @@
 #include "kestrel_oracle.h"
 #include "ko_internal.h"
 
-#define CL_BLOCK 4096
+#define CL_BLOCK 8192
 #define CL_WAVE 64
 #define CL_LOG 16
 #define CL_NACC 47
@@ -236,9 +236,11 @@ void ko_class_sums(const uint8_t* gx, const uint8_t* gy, const double* af, const
             ln[l].normal = 1;
         }
         uint64_t t = 0;
+        static const int PASS_FIRST[3] = {PAT_HS, PAT_B, PAT_D}, PASS_LAST[3] = {PAT_HS, PAT_C, PAT_E};
+        for (int pass = 0; pass < 3; pass++)
         for (uint32_t b0 = s0; b0 < s1; b0 += CL_BLOCK) {
             const uint32_t b1 = b0 + CL_BLOCK < s1 ? b0 + CL_BLOCK : s1;
-            for (int pat = PAT_HS; pat <= PAT_E; pat++) {
+            for (int pat = PASS_FIRST[pass]; pat <= PASS_LAST[pass]; pat++) {
                 uint32_t m = 0;
                 for (uint32_t s = b0; s < b1; s++) {
                     int f;

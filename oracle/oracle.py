@@ -18,8 +18,8 @@ ORDER_REF = 0
 ORDER_SEG = 1
 ORDER_FAST = 2  # ORDER_REF in the reference's AVX-512 loop shape (same bits; timing form)
 ORDER_LANES = 3  # the segments of ORDER_SEG summed by 64 strided lanes and a butterfly
-ORDER_CLASS = 4  # the class-sorted sweep's order: genotype-pair patterns per 4096-site block (ko_class.c)
-CLASS_BLOCK = 4096
+ORDER_CLASS = 4  # the class-sorted sweep's order: genotype-pair patterns per 8192-site block (ko_class.c)
+CLASS_BLOCK = 8192
 
 
 class Tune(ctypes.Structure):

@@ -52,9 +52,9 @@ def expected_eval(O, codes, afk, pair, pt):
     return -F / n, -G / n, h, n
 
 
-# site counts around the 4,096-site blocks of the lists (one block, a partial
+# site counts around the 8,192-site blocks of the lists (one block, a partial
 # last block, several) and missing fractions that empty whole pattern lists
-@pytest.mark.parametrize("n,v,seed,missing", [(40, 3000, 11, 0.01), (48, 9500, 12, 0.05), (24, 13000, 13, 0.3)])
+@pytest.mark.parametrize("n,v,seed,missing", [(40, 3000, 11, 0.01), (48, 9500, 12, 0.05), (24, 17500, 13, 0.3)])
 def test_class_evaluate_bitexact(O, n, v, seed, missing):
     say(f"case: class-sorted evaluate {n} x {v} missing {missing}")
     c, co, codes, afk = class_cohort(n, v, seed, missing)
