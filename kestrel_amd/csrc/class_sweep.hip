@@ -28,8 +28,9 @@
 // and walks the groups of the block's lists, loaded two groups ahead.
 //
 // The summation order (each lane's sums in list order, a log of the lane's
-// product after every 16th chunk of the segment and at its end, a butterfly
-// over the 64 lanes, the expansion) is restated by the oracle's class order;
+// product after the first group that ends at least 13 chunks after the last
+// log and at the segment's end, a butterfly over the 64 lanes, the expansion)
+// is restated by the oracle's class order;
 // records are bit-identical to it, and within rounding of the reference's
 // order (different from sweep_fp64's in the last bits).
 #include "kestrel_internal.hpp"
@@ -328,32 +329,45 @@ __device__ __forceinline__ void class_expand_store(const double* R, double* out,
 // entries are q (lane l's four entries side by side); a chunk past the list's
 // end stops the group.  t counts the segment's chunks, and every 16th ends
 // each lane's log group.
-template <int K>
-__device__ __forceinline__ void class_chunk(uint32_t e, bool on, const double2* afb, double* A, double& P,
-                                            uint32_t& t, const ClassCoef& c, double eps) {
-    const double2 f = afb[(e >> 1) & (CLASS_BLOCK - 1)];
-    if (on) class_site<K>(A, P, e & 1u, f.x, f.y, c, eps);
-    if ((++t & (LOG_GROUP - 1)) == 0) {
+// the log of each lane's product once a group ends at least LOG_MIN chunks
+// after the last log (a group adds at most four: a product never has more
+// than 16 factors)
+constexpr uint32_t LOG_MIN = LOG_GROUP - 3;
+
+__device__ __forceinline__ void class_log(double* A, double& P, uint32_t& since) {
+    if (since >= LOG_MIN) {
         A[0] = A[0] + sleef_log_pos(P);
         P = 1.0;
+        since = 0;
     }
 }
 
+// One group of pattern K's list: the chunks j0 = jg, jg + 64, jg + 128,
+// jg + 192 below n, whose entries are q (lane l's four entries side by side).
 template <int K>
 __device__ __forceinline__ void class_group(uint2 q, uint32_t jg, uint32_t n, uint32_t lane, const double2* afb,
-                                            double* A, double& P, uint32_t& t, const ClassCoef& c, double eps) {
-    if (jg + CLASS_GROUP <= n) {  // four whole chunks
-        class_chunk<K>(q.x & 0xffffu, true, afb, A, P, t, c, eps);
-        class_chunk<K>(q.x >> 16, true, afb, A, P, t, c, eps);
-        class_chunk<K>(q.y & 0xffffu, true, afb, A, P, t, c, eps);
-        class_chunk<K>(q.y >> 16, true, afb, A, P, t, c, eps);
-        return;
+                                            double* A, double& P, uint32_t& since, const ClassCoef& c, double eps) {
+    const uint32_t e0 = q.x & 0xffffu, e1 = q.x >> 16, e2 = q.y & 0xffffu, e3 = q.y >> 16;
+    if (jg + CLASS_GROUP <= n) {  // four whole chunks: their frequencies read first
+        const double2 f0 = afb[(e0 >> 1) & (CLASS_BLOCK - 1)];
+        const double2 f1 = afb[(e1 >> 1) & (CLASS_BLOCK - 1)];
+        const double2 f2 = afb[(e2 >> 1) & (CLASS_BLOCK - 1)];
+        const double2 f3 = afb[(e3 >> 1) & (CLASS_BLOCK - 1)];
+        class_site<K>(A, P, e0 & 1u, f0.x, f0.y, c, eps);
+        class_site<K>(A, P, e1 & 1u, f1.x, f1.y, c, eps);
+        class_site<K>(A, P, e2 & 1u, f2.x, f2.y, c, eps);
+        class_site<K>(A, P, e3 & 1u, f3.x, f3.y, c, eps);
+        since += 4;
+    } else {
+        const uint32_t m = (n - jg + 63) / 64;  // chunks of the group within the list (1 to 4)
+        for (uint32_t sub = 0; sub < m; sub++) {
+            const uint32_t e = sub == 0 ? e0 : sub == 1 ? e1 : sub == 2 ? e2 : e3;
+            const double2 f = afb[(e >> 1) & (CLASS_BLOCK - 1)];
+            if (jg + 64 * sub + lane < n) class_site<K>(A, P, e & 1u, f.x, f.y, c, eps);
+        }
+        since += m;
     }
-    const uint32_t m = (n - jg + 63) / 64;  // chunks of the group within the list
-    for (uint32_t sub = 0; sub < m; sub++) {
-        const uint32_t e = ((sub < 2 ? q.x : q.y) >> (16 * (sub & 1))) & 0xffffu;
-        class_chunk<K>(e, jg + 64 * sub + lane < n, afb, A, P, t, c, eps);
-    }
+    class_log(A, P, since);
 }
 
 // the (p, q) of the block's sites into LDS without registers: each 64-lane
@@ -376,15 +390,15 @@ __device__ __forceinline__ void stage_af(const double* __restrict__ af, uint32_t
 // it belongs to; gend[k] = the groups of patterns 0..k
 template <int K0, int K1>
 __device__ __forceinline__ void class_group_of(uint2 q, uint32_t g, const uint32_t* nk, const uint32_t* gend,
-                                               uint32_t lane, const double2* afb, double* A, double& P, uint32_t& t,
+                                               uint32_t lane, const double2* afb, double* A, double& P, uint32_t& since,
                                                const ClassCoef& c, double eps) {
     if constexpr (K0 == K1) {
-        class_group<K0>(q, (g - (K0 ? gend[K0 - 1] : 0)) * CLASS_GROUP, nk[K0], lane, afb, A, P, t, c, eps);
+        class_group<K0>(q, (g - (K0 ? gend[K0 - 1] : 0)) * CLASS_GROUP, nk[K0], lane, afb, A, P, since, c, eps);
     } else {
         if (g < gend[K0])
-            class_group<K0>(q, (g - gend[K0 - 1]) * CLASS_GROUP, nk[K0], lane, afb, A, P, t, c, eps);
+            class_group<K0>(q, (g - gend[K0 - 1]) * CLASS_GROUP, nk[K0], lane, afb, A, P, since, c, eps);
         else
-            class_group<K1>(q, (g - gend[K0]) * CLASS_GROUP, nk[K1], lane, afb, A, P, t, c, eps);
+            class_group<K1>(q, (g - gend[K0]) * CLASS_GROUP, nk[K1], lane, afb, A, P, since, c, eps);
     }
 }
 
@@ -398,7 +412,7 @@ template <int K0, int K1, int W>
 __device__ __forceinline__ void class_pass(const double* __restrict__ af, uint32_t v_pad, uint32_t bb0, uint32_t nb,
                                            const uint16_t* __restrict__ lists, const uint16_t* __restrict__ counts,
                                            uint32_t nblk, uint32_t slot, bool live, uint32_t w, uint32_t lane,
-                                           double2* afb, double* sums, double& F, double& P, uint32_t& t,
+                                           double2* afb, double* sums, double& F, double& P, uint32_t& since,
                                            uint32_t& n, const ClassCoef& c, double eps) {
     constexpr int FIRST[6] = {1, 12, 20, 28, 42, 47};  // the sums of pattern k: A[FIRST[k] .. FIRST[k + 1])
     constexpr int O = FIRST[K0], E = FIRST[K1 + 1];
@@ -429,10 +443,10 @@ __device__ __forceinline__ void class_pass(const double* __restrict__ af, uint32
         if (g0 < g1) qa = gp[(size_t)g0 * 64];
         if (g0 + 1 < g1) qb = gp[(size_t)(g0 + 1) * 64];
         for (uint32_t g = g0; g < g1; g += 2) {
-            class_group_of<K0, K1>(qa, g, nk, gend, lane, afb, A, P, t, c, eps);
+            class_group_of<K0, K1>(qa, g, nk, gend, lane, afb, A, P, since, c, eps);
             if (g + 2 < g1) qa = gp[(size_t)(g + 2) * 64];
             if (g + 1 >= g1) break;
-            class_group_of<K0, K1>(qb, g + 1, nk, gend, lane, afb, A, P, t, c, eps);
+            class_group_of<K0, K1>(qb, g + 1, nk, gend, lane, afb, A, P, since, c, eps);
             if (g + 3 < g1) qb = gp[(size_t)(g + 3) * 64];
         }
     }
@@ -482,13 +496,13 @@ __global__ __launch_bounds__(W * 64, class_waves_per_simd<W>()) void sweep_class
     for (int k = 0; k < 9; k++) D[k] = live ? pt[(size_t)k * nslots + slot] : 0.0;
     const ClassCoef c = class_coef(D);
     double F = 0.0, P = 1.0;
-    uint32_t t = 0, n = 0;
+    uint32_t since = 0, n = 0;
     double* my = sums[w];
-    class_pass<0, 0, W>(af, v_pad, bb0, nb, lists, counts, nblk, slot, live, w, lane, afb, my, F, P, t, n, c, eps);
-    class_pass<1, 2, W>(af, v_pad, bb0, nb, lists, counts, nblk, slot, live, w, lane, afb, my, F, P, t, n, c, eps);
-    class_pass<3, 4, W>(af, v_pad, bb0, nb, lists, counts, nblk, slot, live, w, lane, afb, my, F, P, t, n, c, eps);
+    class_pass<0, 0, W>(af, v_pad, bb0, nb, lists, counts, nblk, slot, live, w, lane, afb, my, F, P, since, n, c, eps);
+    class_pass<1, 2, W>(af, v_pad, bb0, nb, lists, counts, nblk, slot, live, w, lane, afb, my, F, P, since, n, c, eps);
+    class_pass<3, 4, W>(af, v_pad, bb0, nb, lists, counts, nblk, slot, live, w, lane, afb, my, F, P, since, n, c, eps);
     if (!live) return;
-    if (t & (LOG_GROUP - 1)) F = F + sleef_log_pos(P);
+    if (since) F = F + sleef_log_pos(P);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) F = F + __shfl_xor(F, o, 64);
     if (lane == 0) {

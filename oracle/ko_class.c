@@ -27,11 +27,12 @@
  * Order: per segment [s0, s1) of the pair's sites, blocks of 8192 sites from
  * s0, taken in three passes -- HS; B and C; D and E -- each walking every
  * block's lists of its patterns in turn, each list in ascending site order;
- * entry j of a list goes to lane j mod 64 in chunk j / 64.  Each of the 64 lanes keeps its own sums from zero (plain
- * additions for the gradient terms, fma for the Hessian terms) and its own log
- * product; after every 16th chunk of the segment (counted across blocks and
- * patterns) and at the segment's end each lane adds log(product) to its
- * objective sum (a group whose product is not a positive normal number adds
+ * entry j of a list goes to lane j mod 64 in chunk j / 64, and chunks 4g ..
+ * 4g + 3 of a list form its group g.  Each of the 64 lanes keeps its own sums
+ * from zero (plain additions for the gradient terms, fma for the Hessian
+ * terms) and its own log product; after the first group that ends at least 13
+ * chunks after the last log (so at most 16 factors) and at the segment's end
+ * each lane adds log(product) to its objective sum (a group whose product is not a positive normal number adds
  * its factors' logs one at a time, as the segmented order does).  The 64
  * lanes' 47 sums are added in a butterfly (o = 32 .. 1: v[i] + v[i ^ o]),
  * expanded into g[9] and h[45], and the segments folded left to right.
@@ -235,7 +236,7 @@ void ko_class_sums(const uint8_t* gx, const uint8_t* gy, const double* af, const
             ln[l].prod = 1.0;
             ln[l].normal = 1;
         }
-        uint64_t t = 0;
+        uint64_t since = 0;
         static const int PASS_FIRST[3] = {PAT_HS, PAT_B, PAT_D}, PASS_LAST[3] = {PAT_HS, PAT_C, PAT_E};
         for (int pass = 0; pass < 3; pass++)
         for (uint32_t b0 = s0; b0 < s1; b0 += CL_BLOCK) {
@@ -259,13 +260,21 @@ void ko_class_sums(const uint8_t* gx, const uint8_t* gy, const double* af, const
                             cl_site(&ln[l], pat, flag[j], af[2 * (size_t)s], af[2 * (size_t)s + 1], &c, eps);
                         }
                     }
-                    t++;
-                    if (t % CL_LOG == 0)
-                        for (int l = 0; l < CL_WAVE; l++) cl_flush(&ln[l]);
+                    /* a group is four chunks (the list's last may have fewer); a
+                     * group that ends at least CL_LOG - 3 chunks after the
+                     * last log ends the lanes' products */
+                    const uint32_t ch = j0 / CL_WAVE;
+                    if (ch % 4 == 3 || j0 + CL_WAVE >= m) {
+                        since += ch % 4 + 1;
+                        if (since >= CL_LOG - 3) {
+                            for (int l = 0; l < CL_WAVE; l++) cl_flush(&ln[l]);
+                            since = 0;
+                        }
+                    }
                 }
             }
         }
-        if (t % CL_LOG != 0)
+        if (since)
             for (int l = 0; l < CL_WAVE; l++) cl_flush(&ln[l]);
         for (int o = CL_WAVE / 2; o > 0; o >>= 1) {
             for (int l = 0; l < CL_WAVE; l++)
