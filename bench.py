@@ -631,6 +631,8 @@ def main():
                     help="mixed: per-site terms in FP32 (KESTREL_MIXED), with its errors against the FP64 form")
     ap.add_argument("--skip-masked", type=int, choices=(-1, 0, 1), default=None,
                     help="claims in runs + sweeps passing over masked sites (default: the library's automatic choice)")
+    ap.add_argument("--class-sweep", type=int, choices=(0, 1), default=None,
+                    help="FP64 fits of hard calls by the class-sorted sweep (1) or the site-table sweep (0)")
     ap.add_argument("--autotune", type=int, default=0, metavar="PAIRS",
                     help="before the warmup, tune slots/streams/batch on this many random pairs (0 = off)")
     ap.add_argument("--serving", action="store_true", help="serving workload (default config c5)")
@@ -725,6 +727,8 @@ def main():
         co.set_option("pair_tile", args.pair_tile)
     if args.skip_masked is not None:
         co.set_option("skip_masked", args.skip_masked)
+    if args.class_sweep is not None:
+        co.set_option("class_sweep", args.class_sweep)
     if buckets:
         parts, _ = co.multi_info()
         kept = None

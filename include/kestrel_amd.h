@@ -185,12 +185,15 @@ int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* poi
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc);
  * no counterpart in the reference.  pairs (n x 2), points (n x 9), f (n),
  * g (n x 9), h (n x 81) and n_sites (n) are DEVICE pointers; results are
- * kestrel_evaluate's bit for bit.  The sweep's slot arrays and segment
- * partials live in a fixed 64 MiB array of device memory that each GPU the
- * library runs on holds until the process ends, so a call takes at most
- * 64 MiB / (88 + 448 x segments) bytes' worth of points (segments: one per
- * 4,096 kept sites by default; about 5,900 points at 100,000 sites) and
- * returns KESTREL_EINVAL beyond that.  The call allocates nothing, runs on
+ * kestrel_evaluate's bit for bit.  The sweep's slot arrays, segment
+ * partials and (class-sorted sweep, the default for FP64 2-bit cohorts) each
+ * point's pattern lists live in a fixed 64 MiB array of device memory that
+ * each GPU the library runs on holds until the process ends, so a call takes
+ * at most 64 MiB / (88 + 448 x segments + lists) bytes' worth of points
+ * (segments: one per 4,096 kept sites by default; lists: 8 + 18,960 bytes
+ * per 8,192 kept sites with the class-sorted sweep; about 5,900 points at
+ * 100,000 sites with the table sweep, about 260 with the class-sorted one)
+ * and returns KESTREL_EINVAL beyond that.  The call allocates nothing, runs on
  * the cohort's stream and returns when the results are written.  Every call
  * on one GPU shares the array: calls that overlap on one GPU (from several
  * threads or cohorts) race. */
@@ -248,14 +251,23 @@ int kestrel_cohort_create_gt_multi(kestrel_cohort** out, uint32_t n_samples, con
  * 256-slot workgroups of a fit claim pairs in runs of consecutive claims that
  * share their first sample, and a sweep passes over the sites where a
  * wavefront's shared sample is missing, which add nothing; 1 = on, 0 = off, -1 = on when
- * at least 2 % of the kept calls are missing and the form is FP64 (default);
- * results do not change), "pair_lanes" (FP64 fits of 2-bit and biallelic
+ * at least 2 % of the kept calls are missing and the form is FP64 (default;
+ * the table sweep only: the class-sorted sweep's lists hold no missing
+ * site); results do not change), "pair_lanes" (FP64 fits of 2-bit and biallelic
  * likelihood-row cohorts: 1 = one
  * pair per lane (default), 64 = one pair per wavefront with its sites across
  * the 64 lanes and the lanes' sums added in a butterfly at each segment's
  * end, for fits of fewer pairs than the device has lanes; -1 = 64 for fits
  * of at most 4,096 pairs, 1 otherwise; records are those of another
  * summation order, within the reference tolerances of the default form's),
+ * "class_sweep" (FP64 fits, evaluations and serving of 2-bit cohorts: 1 =
+ * the class-sorted sweep (default): one pair per wavefront, its usable sites
+ * sorted once per pair into lists by genotype-pair pattern and summed in each
+ * pattern's basis of at most four values (mixture.rs:137-227), a third of
+ * the table sweep's per-site arithmetic; 0 = the table sweep of one pair per
+ * lane; the two give the reference's sums in different summation orders, so
+ * records differ in the last bits), "class_waves" (pairs per workgroup of
+ * the class-sorted sweep: 8, 12, 14 or 16 (default); results do not change),
  * "precision" (KESTREL_FP64 or KESTREL_MIXED, for 2-bit and likelihood-row
  * cohorts, tri/tetra-allelic parts included: the form of the next fits; this
  * one changes records),

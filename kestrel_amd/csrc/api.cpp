@@ -71,10 +71,11 @@ int repack_kept(kestrel_cohort* c, const uint8_t* dpacked, size_t row_stride_byt
 // then those of the tri/tetra-allelic parts, into consecutive partials.
 int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
                   uint32_t ns, double eps, double* partial, hipStream_t st, bool skip_masked = false,
-                  bool lanes = false, ClassLists* cls = nullptr) {
+                  bool lanes = false, ClassLists* cls = nullptr, bool build = true) {
     if (cls) {  // the class-sorted sweep (c->class_form(); partials in c->nseg_cls segments)
-        HIPCHK(klaunch::class_build(c->geno.p, c->row_words, sx, sy, active, ns, cls->nblk, cls->lists.p,
-                                    cls->counts.p, cls->built.p, st));
+        if (build)
+            HIPCHK(klaunch::class_build(c->geno.p, c->row_words, sx, sy, active, ns, cls->nblk, cls->lists.p,
+                                        cls->counts.p, cls->built.p, st));
         HIPCHK(klaunch::sweep_class(c->af.p, c->v_pad, c->seg_cls.p, c->nseg_cls, cls->lists.p, cls->counts.p,
                                     cls->nblk, pt, active, ns, eps, partial, st, (int)c->opt_class_waves));
         return KESTREL_OK;
@@ -1009,9 +1010,14 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
             for (int g = 0; g < ngroups; g++) {
                 Group& G = grp[g];
                 if (G.live_before <= 0) continue;
+                // (the class lists of newly claimed pairs are built outside the
+                // timed interval: it times the sweep kernel alone)
+                if (use_class)
+                    HIPCHK(klaunch::class_build(c->geno.p, c->row_words, G.sx.p, G.sy.p, G.active.p, ns, G.cls.nblk,
+                                                G.cls.lists.p, G.cls.counts.p, G.cls.built.p, G.stream));
                 HIPCHK(hipEventRecord(G.ev[2 * i], G.stream));
                 int rs = launch_sweeps(c, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, tune->epsilon, G.part.p, G.stream,
-                                       use_runs, lanes, use_class ? &G.cls : nullptr);
+                                       use_runs, lanes, use_class ? &G.cls : nullptr, false);
                 if (rs) return rs;
                 HIPCHK(hipEventRecord(G.ev[2 * i + 1], G.stream));
                 HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, nseg_fit, dpairs.p,
@@ -1162,7 +1168,7 @@ int kestrel_autotune(kestrel_cohort* c, const kestrel_tune* tune, const uint32_t
     // candidates: 1/8 to 3 device fills per group, one or two groups, 4 / 8 /
     // 16 iterations per synchronisation; a shape is tried only when the
     // sample holds at least 4 times its pairs in flight
-    const uint64_t base = default_slots(c);
+    const uint64_t base = c->class_form() ? class_slots(c) / 2 : default_slots(c);
     double best_ms = ms0;
     int64_t best[3] = {keep_slots, keep_streams, keep_batch};
     for (const double fill : {0.125, 0.25, 0.5, 1.0, 2.0, 3.0}) {

@@ -185,7 +185,7 @@ struct kestrel_cohort {
     bool mixed = false;            // precision KESTREL_MIXED: sweeps by sweep_mixed
     int64_t opt_pair_tile = -1;    // claim order in tiles of this many samples (0 list order, -1 automatic)
     int64_t opt_pair_lanes = 1;    // lanes a pair: 1, 64 (its sites across a wavefront; FP64 fits), -1 auto
-    int64_t opt_class_sweep = 0;   // 1: FP64 fits of 2-bit cohorts by the class-sorted sweep
+    int64_t opt_class_sweep = 1;   // FP64 fits of 2-bit cohorts by the class-sorted sweep (0: the table sweep)
     int64_t opt_class_waves = CLASS_WAVES;  // its wavefronts (pairs) per workgroup: 8, 12, 14 or 16
     DevBuf<uint32_t> seg_cls;      // its segment bounds (multiples of CLASS_BLOCK; one segment)
     uint32_t nseg_cls = 0;

@@ -1,8 +1,9 @@
 // noalloc.hip -- kestrel_evaluate_noalloc: the objective / gradient / Hessian
 // evaluation of kestrel_evaluate for callers that cannot allocate device
 // memory.  Inputs and outputs are the caller's device buffers; the sweep's
-// slot arrays and segment partials live in `scratch`, a fixed array in device
-// memory that each GPU holds until the process ends.
+// slot arrays, segment partials and (class-sorted sweep) pattern lists live in
+// `scratch`, a fixed array in device memory that each GPU holds until the
+// process ends.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -41,11 +42,22 @@ int scratch_ptr(int device, unsigned char** out) {
 
 // Layout of one call in scratch (after the 256-byte head): sx, sy (ns u32),
 // active, exact (ns int), points (9 x ns doubles, SoA), partials
-// (nseg x 56 x ns).
-size_t slot_bytes(uint32_t nseg) { return 4 * sizeof(uint32_t) + 9 * sizeof(double) + (size_t)nseg * NPART * 8; }
+// (nseg x 56 x ns); with the class-sorted sweep then the lists' built marks
+// (ns x 8 bytes), counts (ns x nblk x 16 bytes) and lists (ns x nblk x
+// CLASS_STRIDE x 2 bytes).
+size_t slot_bytes(uint32_t nseg, uint32_t nblk) {
+    return 4 * sizeof(uint32_t) + 9 * sizeof(double) + (size_t)nseg * NPART * 8 +
+           (nblk ? 8 + (size_t)nblk * (16 + 2 * CLASS_STRIDE) : 0);
+}
 
 __global__ __launch_bounds__(64) void clear_head_kernel(unsigned int* __restrict__ err) {
     if (threadIdx.x == 0) *err = 0u;
+}
+
+// scratch words set to a value (a kernel rather than a memset, so that the
+// call can be captured into a graph)
+__global__ __launch_bounds__(256) void fill_kernel(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = v;
 }
 
 // The caller's pairs (n x 2) and points (n x 9) into the sweep's slot arrays;
@@ -94,13 +106,15 @@ extern "C" int kestrel_evaluate_noalloc(kestrel_cohort* c, const uint32_t* pairs
         return fail(KESTREL_EINVAL, "a -f GT cohort with 3- or 4-allele sites has no likelihood evaluation");
     if (n == 0) return KESTREL_OK;
     const uint32_t nseg = c->nseg_total();
+    const bool cls = c->class_form();  // points inside the domain by the class-sorted sweep, as kestrel_evaluate
+    const uint32_t nblk = cls ? c->class_blocks() : 0;
     const uint64_t ns = ((uint64_t)n + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB;
-    const uint64_t cap = (SCRATCH_BYTES - SCRATCH_HEAD) / slot_bytes(nseg) / SWEEP_TPB * SWEEP_TPB;
+    const uint64_t cap = (SCRATCH_BYTES - SCRATCH_HEAD) / slot_bytes(nseg, nblk) / SWEEP_TPB * SWEEP_TPB;
     if (ns > cap)
         return fail(KESTREL_EINVAL,
                     "kestrel_evaluate_noalloc: %zu points exceed its %llu-point limit for this cohort (%u segments, "
-                    "%zu MiB of scratch)",
-                    n, (unsigned long long)cap, nseg, SCRATCH_BYTES >> 20);
+                    "%u list blocks, %zu MiB of scratch)",
+                    n, (unsigned long long)cap, nseg, nblk, SCRATCH_BYTES >> 20);
     unsigned char* base = nullptr;
     int rc = scratch_ptr(c->device, &base);
     if (rc) return rc;
@@ -113,14 +127,34 @@ extern "C" int kestrel_evaluate_noalloc(kestrel_cohort* c, const uint32_t* pairs
     // 8-byte alignment for the doubles: 4 ns words, ns a multiple of 256
     double* pt = reinterpret_cast<double*>(ex + ns);
     double* part = pt + 9 * ns;
+    // class lists: 8-byte built marks, then 16-byte counts (8-byte aligned
+    // after the partials; ns a multiple of 256 keeps 16-byte alignment)
+    uint2* built = reinterpret_cast<uint2*>(part + (size_t)nseg * NPART * ns);
+    uint16_t* counts = reinterpret_cast<uint16_t*>(built + ns);
+    uint16_t* lists = counts + ns * nblk * 8;
     hipStream_t st = c->stream;
     hipLaunchKernelGGL(clear_head_kernel, dim3(1), dim3(64), 0, st, err);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(pack_kernel, dim3((unsigned)(ns / 256)), dim3(256), 0, st, pairs, points, (uint32_t)n,
                        (uint32_t)ns, c->n_samples, eps, c->lmax, sx, sy, act, ex, pt, err);
     HIPCHK(hipGetLastError());
-    HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, sx, sy, pt, act, (uint32_t)ns, eps,
-                          part, st, c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride, false, c->mixed));
+    if (cls) {
+        // every slot's lists are built anew; the sweep writes its one segment,
+        // the exact kernel the cohort's: the others start at zero
+        hipLaunchKernelGGL(fill_kernel, dim3(256), dim3(256), 0, st, reinterpret_cast<uint32_t*>(built), 2 * ns,
+                           0xffffffffu);
+        hipLaunchKernelGGL(fill_kernel, dim3(1024), dim3(256), 0, st, reinterpret_cast<uint32_t*>(part),
+                           (size_t)nseg * NPART * ns * 2, 0u);
+        HIPCHK(hipGetLastError());
+        HIPCHK(klaunch::class_build(c->geno.p, c->row_words, sx, sy, act, (uint32_t)ns, nblk, lists, counts, built,
+                                    st));
+        HIPCHK(klaunch::sweep_class(c->af.p, c->v_pad, c->seg_cls.p, c->nseg_cls, lists, counts, nblk, pt, act,
+                                    (uint32_t)ns, eps, part, st, (int)c->opt_class_waves));
+    } else {
+        HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, sx, sy, pt, act, (uint32_t)ns,
+                              eps, part, st, c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride, false,
+                              c->mixed));
+    }
     size_t off = c->nseg;
     for (const auto& p : c->parts) {
         HIPCHK(klaunch::sweep_multi(p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->table.p, p->af.p,

@@ -96,6 +96,7 @@ struct ServeState {
     kdev::Tune dt{};
     hipStream_t stream = nullptr;
     uint32_t qmax = 0, n_cohort = 0, row_words = 0, nseg = 0, ns = 0;
+    bool cls = false;  // sweeps by the class-sorted form (partials in one segment)
     uint64_t out_cap = 0, ring_cap = 0, backlog = 0;
     int policy = 1;  // 0 round robin, 1 shortest remaining first (with aging)
     double rate = 0;                 // pairs per second the device consumes (moving average)
@@ -113,6 +114,7 @@ struct ServeState {
         DevBuf<SlotState> slots;
         DevBuf<double> pt, partial;
         DevBuf<int> active;
+        ClassLists cls;  // the class-sorted sweep's pattern lists (FP64 cohorts)
     };
     Group grp[2];
     int ngroups = 1;
@@ -188,6 +190,11 @@ int upload(ServeState* S, Request* r) {
     HIPCHK(klaunch::gather(S->qstage.p, S->rb, S->kept.p, S->c->n_kept, r->n_query, S->geno.p, S->row_words,
                            S->stream, S->qrows.p));
     HIPCHK(hipMemsetAsync(ctrl_done(S) + r->slot, 0, sizeof(uint32_t), S->stream));
+    // query rows may have been rewritten: every slot's pattern lists are built
+    // again (both streams are idle between engine rounds)
+    if (S->cls)
+        for (int g = 0; g < S->ngroups; g++)
+            HIPCHK(hipMemsetAsync(S->grp[g].cls.built.p, 0xff, (size_t)S->ns * sizeof(uint2), S->stream));
     // the staging buffers are reused by the next upload only after this one
     // has run
     HIPCHK(hipStreamSynchronize(S->stream));
@@ -336,11 +343,19 @@ int engine_round(ServeState* S, bool* idle) {
     for (int it = 0; it < S->iters; it++)
         for (int g = 0; g < S->ngroups; g++) {
             ServeState::Group& G = S->grp[g];
-            if (sweep)
+            if (sweep && S->cls) {
+                HIPCHK(klaunch::class_build(S->geno.p, S->row_words, G.sx.p, G.sy.p, G.active.p, S->ns, G.cls.nblk,
+                                            G.cls.lists.p, G.cls.counts.p, G.cls.built.p, G.stream));
+                HIPCHK(klaunch::sweep_class(S->c->af.p, S->c->v_pad, S->c->seg_cls.p, S->c->nseg_cls, G.cls.lists.p,
+                                            G.cls.counts.p, G.cls.nblk, G.pt.p, G.active.p, S->ns, S->tune.epsilon,
+                                            G.partial.p, G.stream, (int)S->c->opt_class_waves));
+            } else if (sweep) {
                 HIPCHK(klaunch::sweep(S->geno.p, S->row_words, S->c->af.p, S->seg.p, S->nseg, G.sx.p, G.sy.p, G.pt.p,
                                       G.active.p, S->ns, S->tune.epsilon, G.partial.p, G.stream, nullptr, 0, 3,
                                       false, S->c->mixed));
-            HIPCHK(klaunch::step(G.slots.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, S->ns, G.partial.p, S->nseg, nullptr,
+            }
+            HIPCHK(klaunch::step(G.slots.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, S->ns, G.partial.p,
+                                 S->cls ? S->c->nseg_cls : S->nseg, nullptr,
                                  0, nullptr, ctrl_next(S), S->outpool.p, S->dt, nullptr, q, nullptr, G.stream));
         }
     if (S->ngroups == 2) {
@@ -503,10 +518,14 @@ int kestrel_serve_start(kestrel_cohort* c, const kestrel_tune* tune, uint32_t ma
 
     // slots as kestrel_fit_pairs: the resident sweep blocks over the segments,
     // per group; two groups on two streams unless option "streams" is 1
+    // (the class-sorted form: one pair a wavefront, sixteen a CU)
     const uint64_t resident = (uint64_t)std::max(1, c->n_cu) * 2;
     S->nseg = c->nseg;
+    S->cls = c->class_form();
     const uint64_t slot_blocks = std::max<uint64_t>(1, resident / S->nseg);
-    uint64_t ns = c->opt_slots > 0 ? (uint64_t)c->opt_slots : slot_blocks * SWEEP_TPB;
+    uint64_t ns = c->opt_slots > 0 ? (uint64_t)c->opt_slots
+                  : S->cls        ? (uint64_t)std::max(1, c->n_cu) * 16
+                                  : slot_blocks * SWEEP_TPB;
     ns = (ns + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB;
     S->ns = (uint32_t)ns;
     S->ngroups = c->opt_streams >= 2 ? 2 : 1;
@@ -551,13 +570,18 @@ int kestrel_serve_start(kestrel_cohort* c, const kestrel_tune* tune, uint32_t ma
         HIPCHK(hipMemsetAsync(G.sx.p, 0, ns * sizeof(uint32_t), S->stream));
         HIPCHK(hipMemsetAsync(G.sy.p, 0, ns * sizeof(uint32_t), S->stream));
         HIPCHK(hipMemsetAsync(G.active.p, 0, ns * sizeof(int), S->stream));
+        if (S->cls) {
+            const int rc = G.cls.alloc((uint32_t)ns, c->class_blocks(), S->stream);
+            if (rc) return rc;
+        }
     }
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&S->hctrl), S->ctrl.n, hipHostMallocDefault));
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&S->hent), S->backlog * 16, hipHostMallocDefault));
     HIPCHK(hipStreamSynchronize(S->stream));
     S->device_bytes = rows * S->row_words * 4 + kept.size() * 4 + (S->nseg + 1) * 4 + S->qstage.n + 4 * S->qrows.n +
                       16 * S->ring_cap + max_pairs * sizeof(kestrel_output) + S->ctrl.n +
-                      S->ngroups * (ns * sizeof(SlotState) + 8 * ns + 72 * ns + (size_t)S->nseg * NPART * ns * 8 + 4 * ns);
+                      S->ngroups * (ns * sizeof(SlotState) + 8 * ns + 72 * ns + (size_t)S->nseg * NPART * ns * 8 + 4 * ns +
+                                    (S->cls ? ns * ((size_t)c->class_blocks() * (CLASS_STRIDE + 8) * 2 + 8) : 0));
 
     for (uint32_t i = 0; i < MAX_REQ; i++) S->free_slots.push_back(MAX_REQ - 1 - i);
     for (uint32_t i = 0; i < max_query_rows; i++) S->free_rows.push_back(c->n_samples + max_query_rows - 1 - i);

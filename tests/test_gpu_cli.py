@@ -103,9 +103,8 @@ def test_cli_ml_matches_oracle(O, tmp_path, field):
     assert len(rows) == 20 * 21 // 2 and x.max() < 20
     af = O.allele_freqs(c.codes)  # frequencies use every sample (cli.rs:41-45)
     kept = O.keep_sites(af)
-    codes, afk = c.codes[:, kept], af[kept]
-    seg = np.minimum(segment_bounds(int(kept.sum())), codes.shape[1]).astype(np.uint32)
-    exp = O.fit_pairs(codes, afk, nsel=20, order=O.ORDER_SEG, seg=seg)
+    codes, afk = np.ascontiguousarray(c.codes[:, kept]), af[kept]
+    exp = O.fit_pairs(codes, afk, nsel=20, order=O.ORDER_CLASS)  # hard calls: the class-sorted sweep
     assert np.array_equal(jac, exp["jacquard"]) and np.array_equal(it, exp["iters"])
     assert np.array_equal(obj, exp["obj"])
     assert np.array_equal(cond, log10(exp["cond"]), equal_nan=True)

@@ -92,8 +92,8 @@ def test_c1_all_pairs(O):
         out = co.fit_pairs()
     n = c.codes.shape[0]
     assert len(out) == n * (n + 1) // 2 and (out["status"] == 0).all()
-    kc, ka = c.codes[:, kept], af[kept]
-    exp = O.fit_pairs(kc, ka, order=O.ORDER_SEG, seg=seg_for(int(kept.sum()), kc.shape[1]))
+    kc, ka = np.ascontiguousarray(c.codes[:, kept]), af[kept]
+    exp = O.fit_pairs(kc, ka, order=O.ORDER_CLASS)
     assert_bitexact(out, exp, keys=("x", "y", "jacquard", "obj", "iters", "n_sites", "cond"))
     worst = compare(out, O.fit_pairs(kc, ka, order=O.ORDER_REF), TOL_OBJ_REL)
     say(f"C1 vs reference order: {worst}")
@@ -142,6 +142,7 @@ def test_c4_fp64_and_mixed_forms(O):
         assert ((outm["obj"][flip] - out64["obj"][flip]) / np.abs(out64["obj"][flip]) <= 1e-9).all()
         sel = flip[:24]
         pairs = np.stack([outm["x"][sel], outm["y"][sel]], 1)
+        co64.set_option("class_sweep", 0)  # the mixed form's FP64 condition pass is the table sweep's
         f, g, h, _ = co64.evaluate(pairs, outm["jacquard"][sel])
         for j, i in enumerate(sel):
             k = O.condition(outm["jacquard"][i], g[j], h[j])
@@ -151,8 +152,8 @@ def test_c4_fp64_and_mixed_forms(O):
     rng = np.random.default_rng(0xC4)
     idx = np.unique(rng.integers(0, len(out64), 150))
     pairs = np.stack([out64["x"][idx], out64["y"][idx]], 1)
-    kc, ka = c.codes[:, kept], af[kept]
-    exp = O.fit_pairs(kc, ka, pairs=pairs, order=O.ORDER_SEG, seg=seg_for(int(kept.sum()), kc.shape[1]))
+    kc, ka = np.ascontiguousarray(c.codes[:, kept]), af[kept]
+    exp = O.fit_pairs(kc, ka, pairs=pairs, order=O.ORDER_CLASS, nthreads=16)
     assert_bitexact(out64[idx], exp)
     ref = O.fit_pairs(kc, ka, pairs=pairs, order=O.ORDER_REF)
     say(f"C4 FP64 sample vs reference order: {compare(out64[idx], ref, TOL_OBJ_REL)}")
@@ -175,12 +176,13 @@ def c5():
 
 
 def oracle_records(O, cohort, queries, kept, af, n_kept, recs, qrow, crow):
-    """Oracle (segmented order) of request records: record i pairs query
-    qrow[i] with cohort sample crow[i]."""
+    """Oracle (the class-sorted sweep's order, the serving form of FP64
+    cohorts) of request records: record i pairs query qrow[i] with cohort
+    sample crow[i]."""
     qs, cs = np.unique(qrow), np.unique(crow)
-    codes = np.vstack([queries[qs][:, kept], cohort[cs][:, kept]])
+    codes = np.ascontiguousarray(np.vstack([queries[qs][:, kept], cohort[cs][:, kept]]))
     pairs = np.stack([np.searchsorted(qs, qrow), len(qs) + np.searchsorted(cs, crow)], 1)
-    return O.fit_pairs(codes, af[kept], pairs=pairs, order=O.ORDER_SEG, seg=seg_for(n_kept, codes.shape[1]))
+    return O.fit_pairs(codes, af[kept], pairs=pairs, order=O.ORDER_CLASS, nthreads=16)
 
 
 def test_c5_serving_full_cohort(O, c5):

@@ -5,8 +5,10 @@ coefficients, L = 0, negative or subnormal L) are evaluated with IEEE
 division and per-site logs where a 16-site group's L are not all positive
 normal numbers; f, g and H are bit-exact against the oracle in the device's
 segmented order (special values included), and the objective's special values
-are those of the reference's own order (L = 0 -> +inf, L < 0 -> NaN).  The
-pair fit rejects an eps outside [1e-18, 1e14]."""
+are those of the reference's own order (L = 0 -> +inf, L < 0 -> NaN).  With
+the class-sorted sweep (the default for 2-bit cohorts) points inside the
+domain take its order and the others the same exact evaluation.  The pair fit
+rejects an eps outside [1e-18, 1e14]."""
 import numpy as np
 import pytest
 
@@ -73,16 +75,39 @@ def test_domain_2bit(O):
     c = synth.generate(30, 5000, seed=61, n_family_pairs=4, missing=0.05)
     with Cohort(c.codes) as co:
         af, kept = co.allele_freqs()
-        codes, afk = c.codes[:, kept], af[kept]
+        codes, afk = np.ascontiguousarray(c.codes[:, kept]), af[kept]
         seg = np.minimum(segment_bounds(co.n_kept), codes.shape[1]).astype(np.uint32)
 
         def rows_of(p):
             rows, valid, _ = O.pair_rows_codes(codes[p[0]], codes[p[1]], afk)
             return rows, valid
+        co.set_option("class_sweep", 0)  # the table sweep: every point in the segmented order
         n = check_against_oracle(O, co, rows_of, seg, (3, 17))
         check_against_oracle(O, co, rows_of, seg, (5, 5))
+        table = [co.evaluate(np.array([[3, 17]]), pt[None, :], eps=eps) for pt, eps in domain_points()]
+        co.set_option("class_sweep", 1)  # the class-sorted sweep for the points inside the domain
+        for i, (pt, eps) in enumerate(domain_points()):
+            f, g, h, _ = co.evaluate(np.array([[3, 17]]), pt[None, :], eps=eps)
+            inside = i == len(domain_points()) - 1
+            if inside:
+                F, G, H, m = O.class_sums(codes[3], codes[17], afk, pt, eps)
+                hh = np.zeros((9, 9))
+                iu = np.triu_indices(9)
+                hh[iu] = H / m
+                hh[(iu[1], iu[0])] = H / m
+                exp = (np.array([-F / m]), (-G / m)[None, :], hh[None, :, :])
+            else:
+                exp = table[i][:3]
+            assert same(f, exp[0]) and same(g, exp[1]) and same(h, exp[2]), (i, eps)
         # a batch mixing both kinds of points gives each point's own result
+        # (both forms)
+        co.set_option("class_sweep", 0)
         pts = np.array([p for p, _ in domain_points()])
+        for i, pt in enumerate(pts):
+            ft, gt, ht, _ = co.evaluate(np.tile([3, 17], (len(pts), 1)), pts, eps=1e-8)
+            fi, gi, hi, _ = co.evaluate(np.array([[3, 17]]), pt[None, :], eps=1e-8)
+            assert same(ft[i], fi[0]) and same(gt[i], gi[0]) and same(ht[i], hi[0])
+        co.set_option("class_sweep", 1)
         f, g, h, _ = co.evaluate(np.tile([3, 17], (len(pts), 1)), pts, eps=1e-8)
         for i, pt in enumerate(pts):
             fi, gi, hi, _ = co.evaluate(np.array([[3, 17]]), pt[None, :], eps=1e-8)

@@ -130,7 +130,7 @@ def test_lanes_refused_where_not_covered():
 
 
 def test_lanes_automatic_by_job_size(O):
-    say("case: pair_lanes -1, 30 x 2000: a 435-pair job in the wavefront order, 4,097 pairs one pair a lane")
+    say("case: pair_lanes -1, 30 x 2000: a 435-pair job in the wavefront order, 4,097 pairs in the default form")
     c = synth.generate(30, 2000, seed=77, missing=0.02)
     with Cohort(c.codes) as co:
         af, kept = co.allele_freqs()
@@ -144,4 +144,4 @@ def test_lanes_automatic_by_job_size(O):
     sp = np.stack([small["x"], small["y"]], 1)
     assert_bitexact(small, O.fit_pairs(kc, ka, pairs=sp, order=O.ORDER_LANES, seg=seg))
     idx = np.arange(0, 4097, 37)
-    assert_bitexact(big[idx], O.fit_pairs(kc, ka, pairs=big_pairs[idx], order=O.ORDER_SEG, seg=seg))
+    assert_bitexact(big[idx], O.fit_pairs(kc, ka, pairs=big_pairs[idx], order=O.ORDER_CLASS))  # the default form

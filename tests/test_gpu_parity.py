@@ -4,7 +4,9 @@ against the oracle.
 Bars:
   * allele frequencies, kept-site masks, n_xy, iterations: bit-exact;
   * objective / gradient / Hessian and the whole fit: bit-exact against the
-    oracle evaluated in the device's segmented summation order;
+    oracle evaluated in the device's summation order -- the class order of
+    the class-sorted sweep (the default for FP64 fits of hard calls), the
+    segmented order of the table sweep (option class_sweep 0);
   * against the reference's own 8-lane order: max |dDelta_k| <= 1e-6,
     |d kinship| <= 1e-6, relative objective <= 1e-9, log10 condition within
     0.1, on pairs with log10(condition) < 12 (the reference's order differs
@@ -36,6 +38,21 @@ def seg_for(n_kept, V, segment_sites=4096):
     return np.minimum(segment_bounds(n_kept, segment_sites), V).astype(np.uint32)
 
 
+def class_fit(O, codes, af, **kw):
+    """The oracle's records in the class-sorted sweep's order (the default
+    form of FP64 fits of hard calls)."""
+    return O.fit_pairs(np.ascontiguousarray(codes), af, order=O.ORDER_CLASS, **kw)
+
+
+def class_eval(O, gx, gy, af, pt):
+    F, G, H, n = O.class_sums(gx, gy, af, pt)
+    h = np.zeros((9, 9))
+    iu = np.triu_indices(9)
+    h[iu] = H / n
+    h[(iu[1], iu[0])] = H / n
+    return -F / n, -G / n, h, n
+
+
 def assert_bitexact(out, exp):
     for k in ("x", "y", "jacquard", "obj", "iters", "n_sites", "cond"):
         ok = same(out[k], exp[k]).reshape(len(out), -1).all(1)
@@ -60,7 +77,7 @@ def golden():
     return {k: z[k] for k in z.files}
 
 
-def test_golden_allele_freqs_and_fit(golden):
+def test_golden_allele_freqs_and_fit(O, golden):
     say("case: golden c1_small (40 x 3000)")
     with Cohort(golden["codes"], segment_sites=int(golden["segment_sites"])) as co:
         af, kept = co.allele_freqs()
@@ -68,7 +85,11 @@ def test_golden_allele_freqs_and_fit(golden):
         assert np.array_equal(kept, golden["kept"])
         assert co.n_kept == int(golden["kept"].sum())
         out = co.fit_pairs()
-    assert_bitexact(out, golden["out_seg"])
+        co.set_option("class_sweep", 0)  # the table sweep: the fixture's segmented order
+        table = co.fit_pairs()
+    assert_bitexact(table, golden["out_seg"])
+    assert_within_tolerance(table, golden["out_ref"])
+    assert_bitexact(out, class_fit(O, golden["codes"][:, golden["kept"]], golden["af"][golden["kept"]]))
     assert_within_tolerance(out, golden["out_ref"])
 
 
@@ -100,13 +121,18 @@ def test_evaluate_matches_oracle(O, medium):
     pts = rng.dirichlet(np.ones(9), 300)
     pts[:20] = np.eye(9)[rng.integers(0, 9, 20)] * 0.5 + 0.5 / 9  # points with large 1/L spread
     f, g, h, ns = co.evaluate(pairs, pts)
+    co.set_option("class_sweep", 0)
+    ft, gt, ht, nt = co.evaluate(pairs, pts)
+    co.set_option("class_sweep", 1)
     seg = seg_for(co.n_kept, codes.shape[1])
     for i in range(300):
         rows, valid, n = O.pair_rows_codes(codes[pairs[i, 0]], codes[pairs[i, 1]], afk)
-        assert ns[i] == n
+        assert ns[i] == n and nt[i] == n
+        fe, ge, he, _ = class_eval(O, codes[pairs[i, 0]], codes[pairs[i, 1]], afk, pts[i])
+        assert f[i] == fe and np.array_equal(g[i], ge) and np.array_equal(h[i], he), i
         fe = O.obj_seg(rows, valid, seg, pts[i])
         ge, he = O.gh_seg(rows, valid, seg, pts[i])
-        assert f[i] == fe and np.array_equal(g[i], ge) and np.array_equal(h[i], he), i
+        assert ft[i] == fe and np.array_equal(gt[i], ge) and np.array_equal(ht[i], he), i
         if i < 40:
             comp = rows[valid == 1]
             fr = O.obj_ref(comp, pts[i])
@@ -120,10 +146,16 @@ def test_fit_all_pairs_bitexact_and_tolerance(O, medium):
     c, co, codes, afk = medium
     say("case: fit 96 x 15000 all 4656 pairs")
     out = co.fit_pairs()
-    assert (out["status"] == 0).all()
+    co.set_option("class_sweep", 0)
+    table = co.fit_pairs()
+    co.set_option("class_sweep", 1)
+    assert (out["status"] == 0).all() and (table["status"] == 0).all()
+    ref = O.fit_pairs(codes, afk, order=O.ORDER_REF)
+    assert_bitexact(out, class_fit(O, codes, afk))
+    assert_within_tolerance(out, ref)
     seg = seg_for(co.n_kept, codes.shape[1])
-    assert_bitexact(out, O.fit_pairs(codes, afk, order=O.ORDER_SEG, seg=seg))
-    assert_within_tolerance(out, O.fit_pairs(codes, afk, order=O.ORDER_REF))
+    assert_bitexact(table, O.fit_pairs(codes, afk, order=O.ORDER_SEG, seg=seg))
+    assert_within_tolerance(table, ref)
 
 
 def test_schedule_independence(O, medium):
@@ -217,8 +249,7 @@ def test_iteration_cap(O, medium):
     say("case: iteration cap 1 and 2")
     for it in (1, 2):
         out = co.fit_pairs(np.array([[0, 1], [2, 9], [4, 4]]), iterations=it)
-        exp = O.fit_pairs(codes, afk, pairs=np.array([[0, 1], [2, 9], [4, 4]]), order=O.ORDER_SEG,
-                          seg=seg_for(co.n_kept, codes.shape[1]), iterations=it)
+        exp = class_fit(O, codes, afk, pairs=np.array([[0, 1], [2, 9], [4, 4]]), iterations=it)
         assert_bitexact(out, exp)
         assert (out["iters"] <= it).all()
 
@@ -242,8 +273,7 @@ def test_edge_cases(O):
         assert (out["status"][~bad] == 0).all()
         good = out[~bad]
         kc, ka = codes[:, kept], af[kept]
-        exp = O.fit_pairs(kc, ka, pairs=np.stack([good["x"], good["y"]], 1), order=O.ORDER_SEG,
-                          seg=seg_for(co.n_kept, kc.shape[1]))
+        exp = class_fit(O, kc, ka, pairs=np.stack([good["x"], good["y"]], 1))
         assert_bitexact(good, exp)
     # all samples missing at one site: the reference panics; the library reports it
     codes2 = codes.copy()
@@ -265,15 +295,16 @@ def test_minimum_cohort(O):
         af, kept = co.allele_freqs()
         assert co.n_kept == 1 and np.array_equal(kept, O.keep_sites(O.allele_freqs(codes)))
         out = co.fit_pairs()
-    exp = O.fit_pairs(codes, af, order=O.ORDER_SEG, seg=seg_for(1, 1))
+    exp = class_fit(O, codes, af)
     assert len(out) == 3 and (out["n_sites"] == 1).all()
     assert_bitexact(out, exp)
 
 
-@pytest.mark.parametrize("v", [255, 256, 257, 4095, 4096, 4097, 8193])
+@pytest.mark.parametrize("v", [255, 256, 257, 4095, 4096, 4097, 8191, 8192, 8193, 16385])
 def test_padding_and_segment_boundaries(O, v):
-    """Kept-site counts either side of the 256-site row padding and of the
-    4,096-site segment size."""
+    """Kept-site counts either side of the 256-site row padding, the table
+    sweep's 4,096-site segments and the class-sorted sweep's 8,192-site
+    blocks, in both forms."""
     say(f"case: 24 samples x {v} kept sites")
     c = synth.generate(24, v + v // 5 + 400, seed=40 + v, n_family_pairs=4, missing=0.03, maf=0.05)
     seg_sites = np.flatnonzero(O.keep_sites(O.allele_freqs(c.codes)))[:v]
@@ -283,14 +314,17 @@ def test_padding_and_segment_boundaries(O, v):
         af, kept = co.allele_freqs()
         assert kept.all() and co.n_kept == v
         out = co.fit_pairs()
-    exp = O.fit_pairs(codes, af, order=O.ORDER_SEG, seg=seg_for(v, v))
-    assert_bitexact(out, exp)
+        co.set_option("class_sweep", 0)
+        table = co.fit_pairs()
+    assert_bitexact(out, class_fit(O, codes, af))
+    assert_bitexact(table, O.fit_pairs(codes, af, order=O.ORDER_SEG, seg=seg_for(v, v)))
 
 
 def test_many_small_segments(O):
     say("case: segments of 32 sites over 1,000 sites")
     c = synth.generate(20, 1000, seed=44, n_family_pairs=4, missing=0.02)
     with Cohort(c.codes, segment_sites=32) as co:
+        co.set_option("class_sweep", 0)  # segments are the table sweep's
         af, kept = co.allele_freqs()
         out = co.fit_pairs()
         n = co.n_kept
@@ -323,7 +357,7 @@ def test_c3_full_size_properties(O):
         x, y = int(out["x"][i]), int(out["y"][i])
         assert out["n_sites"][i] == int((called[x] & called[y]).sum())
     pairs = np.stack([out["x"][idx], out["y"][idx]], 1)
-    exp = O.fit_pairs(kc, af[kept], pairs=pairs, order=O.ORDER_SEG, seg=seg_for(int(kept.sum()), kc.shape[1]))
+    exp = class_fit(O, kc, af[kept], pairs=pairs, nthreads=16)
     assert_bitexact(out[idx], exp)
     expect = {"PO": 0.25, "FS": 0.25, "HS": 0.125, "FC": 0.0625}
     pos = lambda x, y: x * n - x * (x - 1) // 2 + (y - x)  # noqa: E731  row-major upper triangle
@@ -345,6 +379,7 @@ def test_claims_in_runs_and_skipped_masked_sites(O):
     rng = np.random.default_rng(5)
     pairs = rng.integers(0, 260, (2500, 2))
     with Cohort(c.codes) as co:
+        co.set_option("class_sweep", 0)  # claims in runs are the table sweep's
         af, kept = co.allele_freqs()
         auto = co.fit_pairs()  # missing calls >= 2 %: on
         co.set_option("skip_masked", 0)
@@ -429,8 +464,7 @@ def test_given_allele_frequencies(O):
         assert np.array_equal(got, af) and not kept[:7].any() and kept[7:].all()
         out = co.fit_pairs()
     kc, ka = codes[:, kept], af[kept]
-    exp = O.fit_pairs(kc, ka, order=O.ORDER_SEG, seg=seg_for(int(kept.sum()), kc.shape[1]))
-    assert_bitexact(out, exp)
+    assert_bitexact(out, class_fit(O, kc, ka))
 
 
 def test_c2_sample_of_pairs(O):
@@ -446,9 +480,8 @@ def test_c2_sample_of_pairs(O):
         pairs = np.concatenate([rng.integers(0, 1000, (120, 2)), np.array([(x, y) for x, y, _ in c.pairs])])
         out = co.fit_pairs(pairs)
     codes, afk = c.codes[:, kept], af[kept]
-    exp = O.fit_pairs(codes, afk, pairs=pairs, order=O.ORDER_SEG, seg=seg_for(int(kept.sum()), codes.shape[1]))
-    assert_bitexact(out, exp)
-    assert_within_tolerance(out, O.fit_pairs(codes, afk, pairs=pairs, order=O.ORDER_REF))
+    assert_bitexact(out, class_fit(O, codes, afk, pairs=pairs, nthreads=16))
+    assert_within_tolerance(out, O.fit_pairs(codes, afk, pairs=pairs, order=O.ORDER_REF, nthreads=16))
     s = out["jacquard"].sum(1)
     assert np.abs(s - 1).max() < 1e-12 and (out["jacquard"] >= 0).all()
     expect = {"PO": 0.25, "FS": 0.25, "HS": 0.125, "FC": 0.0625}
@@ -473,8 +506,7 @@ def test_c2_whole_job_sample(O):
     idx = np.random.default_rng(11).choice(len(out), 2000, replace=False)
     pairs = np.stack([out["x"][idx], out["y"][idx]], 1)
     codes, afk = np.ascontiguousarray(c.codes[:, kept]), af[kept]
-    exp = O.fit_pairs(codes, afk, pairs=pairs, order=O.ORDER_SEG, seg=seg_for(int(kept.sum()), codes.shape[1]))
-    assert_bitexact(out[idx], exp)
+    assert_bitexact(out[idx], class_fit(O, codes, afk, pairs=pairs, nthreads=16))
 
 
 def test_device_arithmetic_selftest():

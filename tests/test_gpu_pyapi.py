@@ -58,8 +58,7 @@ def test_relatedness_coefficients_gl(O):
     jac, kin, inb = relatedness_coefficients_gl(gl.transpose(1, 0, 2))
     af = O.allele_freqs(c.codes)
     kept = O.keep_sites(af)
-    kc = c.codes[:, kept]
-    seg = np.minimum(segment_bounds(int(kept.sum())), kc.shape[1]).astype(np.uint32)
-    exp = O.fit_pairs(kc, af[kept], order=O.ORDER_SEG, seg=seg)
+    kc = np.ascontiguousarray(c.codes[:, kept])
+    exp = O.fit_pairs(kc, af[kept], order=O.ORDER_CLASS)  # one-hot rows: the 2-bit path's class-sorted sweep
     ej, ek, ei = build_arrays(exp, 24)
     assert np.array_equal(jac, ej) and np.array_equal(kin, ek) and np.array_equal(inb, ei)

@@ -2,7 +2,8 @@
 through the C ABI.
 
 Bars: every request's records are bit-identical to the oracle fit of the same
-(query, cohort sample) pairs in the device's segmented order, bit-identical
+(query, cohort sample) pairs in the device's summation order (the class-sorted
+sweep's for FP64 cohorts, the default), bit-identical
 alone and in a mix of concurrently admitted requests, and bit-identical to
 kestrel_fit_pairs when a query repeats a cohort sample's row.
 """
@@ -11,7 +12,7 @@ import threading
 import numpy as np
 import pytest
 
-from kestrel_amd import Cohort, KestrelError, segment_bounds, synth
+from kestrel_amd import Cohort, KestrelError, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -55,8 +56,7 @@ def oracle_request(O, co, cohort, kept, af, qcodes, subset):
     sub = np.arange(N_COHORT) if subset is None else np.asarray(subset)
     q = np.repeat(np.arange(qcodes.shape[0]), len(sub))
     pairs = np.stack([N_COHORT + q, np.tile(sub, qcodes.shape[0])], 1)
-    seg = np.minimum(segment_bounds(co.n_kept), codes.shape[1]).astype(np.uint32)
-    exp = O.fit_pairs(codes, af[kept], pairs=pairs, order=O.ORDER_SEG, seg=seg)
+    exp = O.fit_pairs(np.ascontiguousarray(codes), af[kept], pairs=pairs, order=O.ORDER_CLASS)
     exp["x"] = q
     return exp
 

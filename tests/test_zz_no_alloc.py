@@ -81,6 +81,12 @@ def test_noalloc_matches_evaluate(precision):
             assert rc == 0, _lib.load().kestrel_last_error()
             exp = co.evaluate(pairs, pts)
             check_same(got, exp)
+            if precision == "fp64":  # the table sweep (option class_sweep 0) as well
+                co.set_option("class_sweep", 0)
+                rc, got0 = noalloc(co, pairs, pts)
+                assert rc == 0
+                check_same(got0, co.evaluate(pairs, pts))
+                co.set_option("class_sweep", 1)
         # points outside the sweeps' domain (eps = 0, negative coefficients)
         pairs = rng.integers(0, 120, (40, 2))
         pts = rng.dirichlet(np.ones(9), 40)
@@ -94,7 +100,8 @@ def test_noalloc_matches_evaluate(precision):
         bad = np.array([[0, 5], [3, 120]])
         rc, _ = noalloc(co, bad, np.full((2, 9), 1 / 9))
         assert rc == _lib.EINVAL
-        # more points than the scratch array holds
+        # more points than the scratch array holds (the table sweep's slots
+        # are the smallest: beyond them every form rejects)
         big = 64 * 2**20 // (88 + 448 * 3) + 512
         rc, _ = noalloc(co, np.zeros((big, 2), np.uint32), np.full((big, 9), 1 / 9))
         assert rc == _lib.EINVAL
