@@ -38,13 +38,16 @@ FP32_VALU_PEAK_TFLOPS = 157.3  # MI355X vendor FP32 vector (packed)
 FP64_VALU_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
 LDS_PEAK_TBS = 157.3  # gfx950 LDS reads: 256 B/clk/CU x 256 CUs x 2.4 GHz
 FLOP_PER_SITE_EVAL = 156  # 136 (gradient + Hessian) + 20 (objective, log counted as 1)
-# HBM bytes per sweep_fp64 launch on the default workload, from rocprofv3
-# FETCH_SIZE / WRITE_SIZE scaled by a plain 256 MiB copy (profiles/sweep_fp64_c2.md);
-# counters cannot be read inside this process, so the line carries the
-# profiled figure and says so
-SWEEP_TRAFFIC_BYTES_C2 = 1.563e8
-SWEEP_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of sweep_fp64 on this workload, scaled by a "
-                        "256 MiB copy_kernel (profiles/sweep_fp64_c2.md); profiled, not measured in this run")
+# HBM bytes per sweep launch on the default workload, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE scaled by a plain 256 MiB copy; counters cannot be
+# read inside this process, so the line carries the profiled figure and says so
+SWEEP_TRAFFIC_BYTES_C2 = {"sweep_class_fp64": 1.662e9, "sweep_fp64": 1.563e8}
+SWEEP_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of {k} on this workload, scaled by a 256 MiB "
+                        "copy_kernel (profiles/{k}_c2.md); profiled, not measured in this run")
+CLASS_NOTE = ("achieved counts 156 FLOP per site evaluation, the direct per-site form's work (9 coefficients, 9 "
+              "gradient and 45 Hessian terms); the class-sorted sweep obtains the same f, g, H from 2-11 sums a site "
+              "in each genotype-pair pattern's basis (about 35 VALU instructions a site, VALU issue 0.57 of SIMD "
+              "cycles in profiles/sweep_class_fp64_c2.md), so frac can exceed 1")
 
 
 def log(*a):
@@ -746,18 +749,21 @@ def main():
     fit64 = timed_fits(co, args.steps, args.warmup)
     out, value, total, achieved = fit64["out"], fit64["value"], fit64["seconds"], fit64["achieved"]
     offdiag = int((out["x"] != out["y"]).sum())
+    # FP64 fits of hard calls run the class-sorted sweep unless --class-sweep 0
+    class_form = args.precision == "fp64" and not args.soft and not buckets and args.class_sweep != 0
+    kernel = (("sweep_mixed + sweep_glm_mixed" if args.precision == "mixed" else "sweep_fp64 + sweep_glm")
+              if buckets  # (the --multi rows are one-hot: the 2-allele part is held as 2-bit codes)
+              else ("sweep_gl_mixed" if args.precision == "mixed" else "sweep_gl") if args.soft
+              else "sweep_mixed" if args.precision == "mixed"
+              else "sweep_class_fp64" if class_form else "sweep_fp64")
+    profiled = args.config == "c2" and not args.segment_sites and kernel in SWEEP_TRAFFIC_BYTES_C2
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_VALU_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_VALU_PEAK_TFLOPS, 4),
-                "traffic": (SWEEP_TRAFFIC_BYTES_C2 if args.config == "c2" and not args.segment_sites
-                            and not args.soft and not buckets and args.precision == "fp64" else None),
-                "traffic_source": (SWEEP_TRAFFIC_SOURCE if args.config == "c2" and not args.segment_sites
-                                   and not args.soft and not buckets and args.precision == "fp64" else None),
-                # (the --multi rows are one-hot: the 2-allele part is held as 2-bit codes)
-                "kernel": (("sweep_mixed + sweep_glm_mixed" if args.precision == "mixed" else "sweep_fp64 + sweep_glm")
-                           if buckets
-                           else ("sweep_gl_mixed" if args.precision == "mixed" else "sweep_gl") if args.soft
-                           else "sweep_mixed" if args.precision == "mixed" else "sweep_fp64"),
-                **fit64["launch"]}
+                "traffic": SWEEP_TRAFFIC_BYTES_C2[kernel] if profiled else None,
+                "traffic_source": SWEEP_TRAFFIC_SOURCE.format(k=kernel) if profiled else None,
+                "kernel": kernel, **fit64["launch"]}
+    if class_form:
+        roofline["note"] = CLASS_NOTE
     mixed = args.precision == "mixed"
     if mixed:  # the mixed form is priced against FP32 (vendor figure: packed FP32)
         roofline["peak"] = FP32_VALU_PEAK_TFLOPS
