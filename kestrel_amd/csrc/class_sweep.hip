@@ -24,8 +24,9 @@
 // of 256 entries, and a group holds lane l's entries of its four chunks side
 // by side (entry j at (j / 256) 256 + 4 (j % 64) + (j / 64) % 4), so one
 // 8-byte load gives a lane four chunks.  sweep_class_fp64 stages each block's
-// frequencies in LDS for the workgroup's pairs (asynchronously, a block ahead)
-// and walks the groups of the block's lists, loaded two groups ahead.
+// frequencies in LDS for the workgroup's pairs (global_load_lds, no registers)
+// and walks the groups of the block's lists, loaded two groups ahead; an
+// entry addresses the frequency its flag selects directly.
 //
 // The summation order (each lane's sums in list order, a log of the lane's
 // product after the first group that ends at least 13 chunks after the last
@@ -221,12 +222,13 @@ __device__ __forceinline__ ClassCoef class_coef(const double* D) {
 
 // One site of pattern K in a lane: L into the log product, then the
 // pattern's gradient and Hessian sums (A[1..46]; the layout of the oracle's
-// class order).
+// class order).  (v0, v1) are the site's frequencies as class_freqs reads
+// them: v0 = the one the entry's flag selects, v1 = the other (E: p, q).
 template <int K>
-__device__ __forceinline__ void class_site(double* A, double& P, uint32_t f, double p, double q, const ClassCoef& c,
+__device__ __forceinline__ void class_site(double* A, double& P, double v0, double v1, const ClassCoef& c,
                                            double eps) {
     if constexpr (K == 0) {
-        const double a = f ? q : p;
+        const double a = v0;
         const double L = fma(a, fma(a, fma(a, fma(a, c.x8, c.T3), c.T1), c.x0), eps);
         P = P * L;
         const double r = recip_rn(L);
@@ -244,7 +246,7 @@ __device__ __forceinline__ void class_site(double* A, double& P, uint32_t f, dou
         A[11] = fma(v3, v3, A[11]);
     } else if constexpr (K == 1 || K == 2) {
         constexpr int o = K == 1 ? 12 : 20;
-        const double u = p * q, w = f ? q : p;
+        const double u = v0 * v1, w = v0;  // (p q = q p bit for bit)
         const double lin = K == 1 ? c.B37 : c.C57, con = K == 1 ? c.x2 : c.x4;
         const double L = fma(u, fma(w, fma(w, c.B8, lin), con), eps);
         P = P * L;
@@ -259,7 +261,7 @@ __device__ __forceinline__ void class_site(double* A, double& P, uint32_t f, dou
         A[o + 6] = fma(b1, b2, A[o + 6]);
         A[o + 7] = fma(b2, b2, A[o + 7]);
     } else if constexpr (K == 3) {
-        const double u = p * q, w1 = f ? p : q, w2 = f ? q : p;
+        const double u = v1 * v0, w1 = v1, w2 = v0;
         const double L = fma(u, fma(w1, c.x3, fma(w2, fma(w1, c.x8, c.x5), c.x1)), eps);
         P = P * L;
         const double r = recip_rn(L);
@@ -276,7 +278,7 @@ __device__ __forceinline__ void class_site(double* A, double& P, uint32_t f, dou
 #pragma unroll
             for (int j = i; j < 4; j++, id++) A[id] = fma(d[i], d[j], A[id]);
     } else {
-        const double u = p * q;
+        const double u = v0 * v1;
         const double L = fma(u, fma(u, c.E8, c.E67), eps);
         P = P * L;
         const double r = recip_rn(L);
@@ -342,28 +344,51 @@ __device__ __forceinline__ void class_log(double* A, double& P, uint32_t& since)
     }
 }
 
+// The frequencies of list entry e = (site << 1) | flag from the block's
+// (p, q) pairs in LDS, read as doubles: afd[e] is p for flag 0 and q for
+// flag 1, afd[e ^ 1] the other, so the entry selects without a compare.
+// E sites (flag 0, e even) read the pair whole.  Entries within a list are below
+// 2 CLASS_BLOCK; MASK keeps the lanes past a list's end (whose entries are
+// not written) inside the block's frequencies.
+template <int K, bool MASK>
+__device__ __forceinline__ void class_freqs(const double* afd, uint32_t e, double& v0, double& v1) {
+    const uint32_t i = MASK ? e & (2 * CLASS_BLOCK - 1) : e;
+    const uint32_t o = i << 3;  // byte offsets: the partner is 8 bytes away
+    const char* base = reinterpret_cast<const char*>(afd);
+    if constexpr (K == 4) {
+        const double2 f = *reinterpret_cast<const double2*>(base + o);
+        v0 = f.x;
+        v1 = f.y;
+    } else {
+        v0 = *reinterpret_cast<const double*>(base + o);
+        v1 = K == 0 ? 0.0 : *reinterpret_cast<const double*>(base + (o ^ 8u));
+    }
+}
+
 // One group of pattern K's list: the chunks j0 = jg, jg + 64, jg + 128,
 // jg + 192 below n, whose entries are q (lane l's four entries side by side).
 template <int K>
-__device__ __forceinline__ void class_group(uint2 q, uint32_t jg, uint32_t n, uint32_t lane, const double2* afb,
+__device__ __forceinline__ void class_group(uint2 q, uint32_t jg, uint32_t n, uint32_t lane, const double* afd,
                                             double* A, double& P, uint32_t& since, const ClassCoef& c, double eps) {
     const uint32_t e0 = q.x & 0xffffu, e1 = q.x >> 16, e2 = q.y & 0xffffu, e3 = q.y >> 16;
     if (jg + CLASS_GROUP <= n) {  // four whole chunks: their frequencies read first
-        const double2 f0 = afb[(e0 >> 1) & (CLASS_BLOCK - 1)];
-        const double2 f1 = afb[(e1 >> 1) & (CLASS_BLOCK - 1)];
-        const double2 f2 = afb[(e2 >> 1) & (CLASS_BLOCK - 1)];
-        const double2 f3 = afb[(e3 >> 1) & (CLASS_BLOCK - 1)];
-        class_site<K>(A, P, e0 & 1u, f0.x, f0.y, c, eps);
-        class_site<K>(A, P, e1 & 1u, f1.x, f1.y, c, eps);
-        class_site<K>(A, P, e2 & 1u, f2.x, f2.y, c, eps);
-        class_site<K>(A, P, e3 & 1u, f3.x, f3.y, c, eps);
+        double a0, b0, a1, b1, a2, b2, a3, b3;
+        class_freqs<K, false>(afd, e0, a0, b0);
+        class_freqs<K, false>(afd, e1, a1, b1);
+        class_freqs<K, false>(afd, e2, a2, b2);
+        class_freqs<K, false>(afd, e3, a3, b3);
+        class_site<K>(A, P, a0, b0, c, eps);
+        class_site<K>(A, P, a1, b1, c, eps);
+        class_site<K>(A, P, a2, b2, c, eps);
+        class_site<K>(A, P, a3, b3, c, eps);
         since += 4;
     } else {
         const uint32_t m = (n - jg + 63) / 64;  // chunks of the group within the list (1 to 4)
         for (uint32_t sub = 0; sub < m; sub++) {
             const uint32_t e = sub == 0 ? e0 : sub == 1 ? e1 : sub == 2 ? e2 : e3;
-            const double2 f = afb[(e >> 1) & (CLASS_BLOCK - 1)];
-            if (jg + 64 * sub + lane < n) class_site<K>(A, P, e & 1u, f.x, f.y, c, eps);
+            double a, b;
+            class_freqs<K, true>(afd, e, a, b);
+            if (jg + 64 * sub + lane < n) class_site<K>(A, P, a, b, c, eps);
         }
         since += m;
     }
@@ -390,15 +415,15 @@ __device__ __forceinline__ void stage_af(const double* __restrict__ af, uint32_t
 // it belongs to; gend[k] = the groups of patterns 0..k
 template <int K0, int K1>
 __device__ __forceinline__ void class_group_of(uint2 q, uint32_t g, const uint32_t* nk, const uint32_t* gend,
-                                               uint32_t lane, const double2* afb, double* A, double& P, uint32_t& since,
+                                               uint32_t lane, const double* afd, double* A, double& P, uint32_t& since,
                                                const ClassCoef& c, double eps) {
     if constexpr (K0 == K1) {
-        class_group<K0>(q, (g - (K0 ? gend[K0 - 1] : 0)) * CLASS_GROUP, nk[K0], lane, afb, A, P, since, c, eps);
+        class_group<K0>(q, (g - (K0 ? gend[K0 - 1] : 0)) * CLASS_GROUP, nk[K0], lane, afd, A, P, since, c, eps);
     } else {
         if (g < gend[K0])
-            class_group<K0>(q, (g - gend[K0 - 1]) * CLASS_GROUP, nk[K0], lane, afb, A, P, since, c, eps);
+            class_group<K0>(q, (g - gend[K0 - 1]) * CLASS_GROUP, nk[K0], lane, afd, A, P, since, c, eps);
         else
-            class_group<K1>(q, (g - gend[K0]) * CLASS_GROUP, nk[K1], lane, afb, A, P, since, c, eps);
+            class_group<K1>(q, (g - gend[K0]) * CLASS_GROUP, nk[K1], lane, afd, A, P, since, c, eps);
     }
 }
 
@@ -416,6 +441,7 @@ __device__ __forceinline__ void class_pass(const double* __restrict__ af, uint32
                                            uint32_t& n, const ClassCoef& c, double eps) {
     constexpr int FIRST[6] = {1, 12, 20, 28, 42, 47};  // the sums of pattern k: A[FIRST[k] .. FIRST[k + 1])
     constexpr int O = FIRST[K0], E = FIRST[K1 + 1];
+    const double* afd = reinterpret_cast<const double*>(afb);
     double A[NACC];
 #pragma unroll
     for (int k = 0; k < NACC; k++) A[k] = 0.0;
@@ -443,10 +469,10 @@ __device__ __forceinline__ void class_pass(const double* __restrict__ af, uint32
         if (g0 < g1) qa = gp[(size_t)g0 * 64];
         if (g0 + 1 < g1) qb = gp[(size_t)(g0 + 1) * 64];
         for (uint32_t g = g0; g < g1; g += 2) {
-            class_group_of<K0, K1>(qa, g, nk, gend, lane, afb, A, P, since, c, eps);
+            class_group_of<K0, K1>(qa, g, nk, gend, lane, afd, A, P, since, c, eps);
             if (g + 2 < g1) qa = gp[(size_t)(g + 2) * 64];
             if (g + 1 >= g1) break;
-            class_group_of<K0, K1>(qb, g + 1, nk, gend, lane, afb, A, P, since, c, eps);
+            class_group_of<K0, K1>(qb, g + 1, nk, gend, lane, afd, A, P, since, c, eps);
             if (g + 3 < g1) qb = gp[(size_t)(g + 3) * 64];
         }
     }
