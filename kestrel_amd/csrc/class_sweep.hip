@@ -78,6 +78,9 @@ __device__ __forceinline__ uint32_t wave_scan(uint32_t v, uint32_t lane) {
 // is written out in the group layout: lane l's entries j, j + 64, j + 128,
 // j + 192 of a group side by side.
 constexpr int BUILD_WAVES = 2;
+constexpr uint32_t MISS = 0xffffu;  // above every entry ((site << 1) | flag < 2 CLASS_BLOCK)
+// a block's lists fill at most CLASS_BLOCK entries + 4 partial groups
+static_assert(2 * CLASS_BLOCK <= (int)MISS && CLASS_BLOCK + 4 * CLASS_GROUP < CLASS_STRIDE, "stage bounds");
 
 __global__ __launch_bounds__(BUILD_WAVES * 64) void class_build_kernel(
     const uint32_t* __restrict__ geno, uint32_t row_words, const uint32_t* __restrict__ sx,
@@ -139,18 +142,26 @@ __global__ __launch_bounds__(BUILD_WAVES * 64) void class_build_kernel(
             pb[k + 1] = pb[k] + (tot[k] + CLASS_GROUP - 1) / CLASS_GROUP * CLASS_GROUP;
         }
         wave_sync();
+        // a word's 16 sites: the 16 counter increments issued back to back
+        // (a lane's increments of one counter return in issue order, so its
+        // entries keep site order), then the 16 entries written; a missing
+        // site (entry MISS) adds 0 and writes the stage's last entry, which
+        // no list reaches
 #pragma unroll
-        for (int q = 0; q < WL; q++)
+        for (int q = 0; q < WL; q++) {
+            uint32_t pos[16], ent[16];
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 const uint32_t idx = (((ax[q] >> (2 * i)) & 3u) << 2) | ((ay[q] >> (2 * i)) & 3u);
                 const uint32_t nib = (uint32_t)(PATTERN_LUT >> (4 * idx)) & 15u;
                 const uint32_t k = nib & 7u;
-                if (k != 7u) {
-                    const uint32_t pos = atomicAdd(&c[k * 64 + lane], 1u);
-                    sg[pos] = (uint16_t)(((CLASS_BLOCK / 64 * lane + 16u * q + (uint32_t)i) << 1) | (nib >> 3));
-                }
+                const bool miss = k == 7u;
+                pos[i] = atomicAdd(&c[(miss ? 0u : k) * 64 + lane], miss ? 0u : 1u);
+                ent[i] = miss ? MISS : ((CLASS_BLOCK / 64 * lane + 16u * q + (uint32_t)i) << 1) | (nib >> 3);
             }
+#pragma unroll
+            for (int i = 0; i < 16; i++) sg[ent[i] == MISS ? CLASS_STRIDE - 1 : pos[i]] = (uint16_t)ent[i];
+        }
         wave_sync();
         // out in the group layout: a lane's four entries of a group side by side
         uint2* dst = reinterpret_cast<uint2*>(lists + ((size_t)slot * nblk + b) * CLASS_STRIDE);
