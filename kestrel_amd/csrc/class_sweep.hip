@@ -453,19 +453,43 @@ __device__ __forceinline__ void class_pass(const double* __restrict__ af, uint32
     constexpr int FIRST[6] = {1, 12, 20, 28, 42, 47};  // the sums of pattern k: A[FIRST[k] .. FIRST[k + 1])
     constexpr int O = FIRST[K0], E = FIRST[K1 + 1];
     const double* afd = reinterpret_cast<const double*>(afb);
+    const uint4* counts4 = reinterpret_cast<const uint4*>(counts);
     double A[NACC];
 #pragma unroll
     for (int k = 0; k < NACC; k++) A[k] = 0.0;
     A[0] = F;
+    // block b's first two groups of the pass's lists (counts cb), loaded
+    // before the block's frequencies are staged, so that their latency
+    // overlaps the staging and the barrier
+    uint4 cv = make_uint4(0, 0, 0, 0);
+    uint2 qa = make_uint2(0, 0), qb = make_uint2(0, 0);
+    auto first_groups = [&](uint32_t b, uint4 cb) {
+        uint32_t g0 = 0, g1 = 0, gt = 0;
+        const uint32_t nk[5] = {cb.x & 0xffffu, cb.x >> 16, cb.y & 0xffffu, cb.y >> 16, cb.z};
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            if (k == K0) g0 = gt;
+            gt += (nk[k] + CLASS_GROUP - 1) / CLASS_GROUP;
+            if (k == K1) g1 = gt;
+        }
+        const uint2* gp = reinterpret_cast<const uint2*>(lists + ((size_t)slot * nblk + b) * CLASS_STRIDE) + lane;
+        if (g0 < g1) qa = gp[(size_t)g0 * 64];
+        if (g0 + 1 < g1) qb = gp[(size_t)(g0 + 1) * 64];
+    };
+    if (live && nb) {
+        cv = counts4[(size_t)slot * nblk + bb0];
+        first_groups(bb0, cv);
+    }
     for (uint32_t i = 0; i < nb; i++) {
         const uint32_t b = bb0 + i;
         const uint32_t site0 = b * CLASS_BLOCK;
         __syncthreads();  // the previous block's frequencies are no longer read
         stage_af<W>(af, site0, min((uint32_t)CLASS_BLOCK, v_pad - site0), afb, w, lane);
+        uint4 cvn = cv;  // the next block's counts
+        if (live && i + 1 < nb) cvn = counts4[(size_t)slot * nblk + b + 1];
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         if (!live) continue;
-        const uint4 cv = reinterpret_cast<const uint4*>(counts)[(size_t)slot * nblk + b];
         const uint32_t nk[5] = {cv.x & 0xffffu, cv.x >> 16, cv.y & 0xffffu, cv.y >> 16, cv.z};
         uint32_t gend[5], gt = 0;
 #pragma unroll
@@ -476,15 +500,16 @@ __device__ __forceinline__ void class_pass(const double* __restrict__ af, uint32
         if (K0 == 0) n += nk[0] + nk[1] + nk[2] + nk[3] + nk[4];
         const uint32_t g0 = K0 ? gend[K0 - 1] : 0, g1 = gend[K1];
         const uint2* gp = reinterpret_cast<const uint2*>(lists + ((size_t)slot * nblk + b) * CLASS_STRIDE) + lane;
-        uint2 qa = make_uint2(0, 0), qb = make_uint2(0, 0);
-        if (g0 < g1) qa = gp[(size_t)g0 * 64];
-        if (g0 + 1 < g1) qb = gp[(size_t)(g0 + 1) * 64];
         for (uint32_t g = g0; g < g1; g += 2) {
             class_group_of<K0, K1>(qa, g, nk, gend, lane, afd, A, P, since, c, eps);
             if (g + 2 < g1) qa = gp[(size_t)(g + 2) * 64];
             if (g + 1 >= g1) break;
             class_group_of<K0, K1>(qb, g + 1, nk, gend, lane, afd, A, P, since, c, eps);
             if (g + 3 < g1) qb = gp[(size_t)(g + 3) * 64];
+        }
+        if (i + 1 < nb) {
+            cv = cvn;
+            first_groups(b + 1, cv);
         }
     }
     F = A[0];
