@@ -70,7 +70,7 @@ __device__ __forceinline__ uint32_t wave_scan(uint32_t v, uint32_t lane) {
 
 // One wavefront per slot whose pair (sx, sy) differs from the one its lists
 // were built for (built[slot]); two a workgroup.  Lane l takes the sites
-// 128 l .. 128 l + 127 of each block (eight words of each row), counts its
+// CLASS_BLOCK / 64 l .. CLASS_BLOCK / 64 (l + 1) - 1 of each block, counts its
 // sites of each pattern, and a wavefront scan gives each lane the position of
 // its first site in its pattern's list; the lane then writes its entries in
 // site order into the LDS copy of the block's lists through per-lane LDS
@@ -473,8 +473,8 @@ __device__ __forceinline__ void class_pass(const double* __restrict__ af, uint32
             if (k == K1) g1 = gt;
         }
         const uint2* gp = reinterpret_cast<const uint2*>(lists + ((size_t)slot * nblk + b) * CLASS_STRIDE) + lane;
-        if (g0 < g1) qa = gp[(size_t)g0 * 64];
-        if (g0 + 1 < g1) qb = gp[(size_t)(g0 + 1) * 64];
+        qa = gp[(size_t)(g0 < g1 ? g0 : 0) * 64];  // (group 0 of the block: a load within the lists)
+        qb = gp[(size_t)(g0 + 1 < g1 ? g0 + 1 : 0) * 64];
     };
     if (live && nb) {
         cv = counts4[(size_t)slot * nblk + bb0];
@@ -500,12 +500,13 @@ __device__ __forceinline__ void class_pass(const double* __restrict__ af, uint32
         if (K0 == 0) n += nk[0] + nk[1] + nk[2] + nk[3] + nk[4];
         const uint32_t g0 = K0 ? gend[K0 - 1] : 0, g1 = gend[K1];
         const uint2* gp = reinterpret_cast<const uint2*>(lists + ((size_t)slot * nblk + b) * CLASS_STRIDE) + lane;
+        // (the loads are unconditional, clamped to the last group, so that
+        // each waits only for its own group's load)
         for (uint32_t g = g0; g < g1; g += 2) {
             class_group_of<K0, K1>(qa, g, nk, gend, lane, afd, A, P, since, c, eps);
-            if (g + 2 < g1) qa = gp[(size_t)(g + 2) * 64];
-            if (g + 1 >= g1) break;
-            class_group_of<K0, K1>(qb, g + 1, nk, gend, lane, afd, A, P, since, c, eps);
-            if (g + 3 < g1) qb = gp[(size_t)(g + 3) * 64];
+            qa = gp[(size_t)min(g + 2, g1 - 1) * 64];
+            if (g + 1 < g1) class_group_of<K0, K1>(qb, g + 1, nk, gend, lane, afd, A, P, since, c, eps);
+            qb = gp[(size_t)min(g + 3, g1 - 1) * 64];
         }
         if (i + 1 < nb) {
             cv = cvn;
@@ -523,11 +524,14 @@ __device__ __forceinline__ void class_pass(const double* __restrict__ af, uint32
         for (int k = O; k < E; k++) sums[k] = A[k];
 }
 
-// wavefronts per SIMD the register budget is set for (one workgroup per CU:
-// the block's frequencies fill the LDS)
+// wavefronts per SIMD the register budget is set for: the workgroups whose
+// LDS (the block's frequencies and the lane sums) fits a CU's 160 KiB, at
+// most four wavefronts a SIMD (128 VGPRs each)
 template <int W>
 constexpr int class_waves_per_simd() {
-    return W / 4;
+    constexpr int lds = CLASS_BLOCK * 16 + W * (NACC + 1) * 8;
+    constexpr int v = W * ((160 * 1024) / lds) / 4;
+    return v < 4 ? v : 4;
 }
 
 // W wavefronts a workgroup, one slot each; blockIdx.y = segment, whose bounds
