@@ -50,12 +50,6 @@ constexpr uint64_t PATTERN_LUT =
     (0xBull << 32) | (0x9ull << 36) | (0x8ull << 40) | (0x7ull << 44) |  // (2,0) (2,1) (2,2) (2,3)
     (0x7ull << 48) | (0x7ull << 52) | (0x7ull << 56) | (0x7ull << 60);   // (3,*)
 
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // inclusive sum over lanes 0..lane
 __device__ __forceinline__ uint32_t wave_scan(uint32_t v, uint32_t lane) {
 #pragma unroll
@@ -68,54 +62,60 @@ __device__ __forceinline__ uint32_t wave_scan(uint32_t v, uint32_t lane) {
 
 }  // namespace
 
-// One wavefront per slot whose pair (sx, sy) differs from the one its lists
-// were built for (built[slot]); two a workgroup.  Lane l takes the sites
-// CLASS_BLOCK / 64 l .. CLASS_BLOCK / 64 (l + 1) - 1 of each block, counts its
-// sites of each pattern, and a wavefront scan gives each lane the position of
-// its first site in its pattern's list; the lane then writes its entries in
-// site order into the LDS copy of the block's lists through per-lane LDS
+// One workgroup per slot whose pair (sx, sy) differs from the one its lists
+// were built for (built[slot]); its BUILD_WAVES wavefronts sort one block at
+// a time.  Thread t takes the sites BUILD_SITES t .. BUILD_SITES (t + 1) - 1
+// of the block (BUILD_WORDS words of each row, loaded a block ahead), counts
+// its sites of each pattern, and a workgroup scan (wavefront scans, then the
+// wavefront totals through LDS) gives each thread the position of its first
+// site in its pattern's list; the thread then writes its entries in site
+// order into the LDS copy of the block's lists through per-thread LDS
 // counters (pattern k's list starting at the group-aligned pb[k]).  The copy
 // is written out in the group layout: lane l's entries j, j + 64, j + 128,
 // j + 192 of a group side by side.
-constexpr int BUILD_WAVES = 2;
+constexpr int BUILD_WAVES = 4;
+constexpr int BUILD_THREADS = BUILD_WAVES * 64;
+constexpr int BUILD_SITES = CLASS_BLOCK / BUILD_THREADS;  // sites of a block a thread takes
+constexpr int BUILD_WORDS = BUILD_SITES / 16;             // 2-bit words of a row a thread takes
 constexpr uint32_t MISS = 0xffffu;  // above every entry ((site << 1) | flag < 2 CLASS_BLOCK)
 // a block's lists fill at most CLASS_BLOCK entries + 4 partial groups
 static_assert(2 * CLASS_BLOCK <= (int)MISS && CLASS_BLOCK + 4 * CLASS_GROUP < CLASS_STRIDE, "stage bounds");
+static_assert(BUILD_WORDS == 2, "a thread loads its words of a row as one uint2");
 
-__global__ __launch_bounds__(BUILD_WAVES * 64) void class_build_kernel(
+__global__ __launch_bounds__(BUILD_THREADS) void class_build_kernel(
     const uint32_t* __restrict__ geno, uint32_t row_words, const uint32_t* __restrict__ sx,
     const uint32_t* __restrict__ sy, const int* __restrict__ active, uint32_t nslots, uint32_t nblk,
     uint16_t* __restrict__ lists, uint16_t* __restrict__ counts, uint2* __restrict__ built) {
-    __shared__ uint32_t ctr[BUILD_WAVES][5 * 64];
-    __shared__ __attribute__((aligned(16))) uint16_t stage[BUILD_WAVES][CLASS_STRIDE];
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const uint32_t slot = blockIdx.x * BUILD_WAVES + w;
-    if (slot >= nslots || active[slot] == 0) return;  // uniform over the wavefront
+    __shared__ uint32_t ctr[5 * BUILD_THREADS];     // pattern k's counter of thread t at k BUILD_THREADS + t
+    __shared__ uint32_t wtot[BUILD_WAVES][4];       // each wavefront's packed pattern counts
+    __shared__ __attribute__((aligned(16))) uint16_t stage[CLASS_STRIDE];
+    const uint32_t t = threadIdx.x, lane = t & 63u;
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(t >> 6));
+    const uint32_t slot = blockIdx.x;
+    if (slot >= nslots || active[slot] == 0) return;  // uniform over the workgroup
     const uint32_t x = sx[slot], y = sy[slot];
     const uint2 b0 = built[slot];
     if (b0.x == x && b0.y == y) return;
     const uint32_t* rowx = geno + (size_t)x * row_words;
     const uint32_t* rowy = geno + (size_t)y * row_words;
-    uint32_t* c = ctr[w];
-    uint16_t* sg = stage[w];
-    constexpr int WL = CLASS_BLOCK / 16 / 64;  // words of a row a lane takes in a block
-    for (uint32_t b = 0; b < nblk; b++) {
-        const uint32_t w0 = b * (CLASS_BLOCK / 16) + WL * lane;
-        uint32_t ax[WL], ay[WL];
-#pragma unroll
-        for (int h = 0; h < WL; h += 4) {
-            uint4 wx = make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu), wy = wx;
-            if (w0 + h < row_words) {  // row_words is a multiple of 16: the four words are in the row
-                wx = *reinterpret_cast<const uint4*>(rowx + w0 + h);
-                wy = *reinterpret_cast<const uint4*>(rowy + w0 + h);
-            }
-            ax[h] = wx.x, ax[h + 1] = wx.y, ax[h + 2] = wx.z, ax[h + 3] = wx.w;
-            ay[h] = wy.x, ay[h + 1] = wy.y, ay[h + 2] = wy.z, ay[h + 3] = wy.w;
+    // thread t's words of block b (missing calls past the row's end; row_words
+    // is a multiple of 16, so both words are in the row or neither)
+    auto words = [&](uint32_t b, uint2& wx, uint2& wy) {
+        const uint32_t w0 = b * (CLASS_BLOCK / 16) + BUILD_WORDS * t;
+        wx = wy = make_uint2(0xffffffffu, 0xffffffffu);
+        if (w0 < row_words) {
+            wx = *reinterpret_cast<const uint2*>(rowx + w0);
+            wy = *reinterpret_cast<const uint2*>(rowy + w0);
         }
+    };
+    uint2 nx, ny;
+    words(0, nx, ny);
+    for (uint32_t b = 0; b < nblk; b++) {
+        const uint32_t ax[BUILD_WORDS] = {nx.x, nx.y}, ay[BUILD_WORDS] = {ny.x, ny.y};
+        if (b + 1 < nblk) words(b + 1, nx, ny);
         uint32_t cnt[5] = {0, 0, 0, 0, 0};
 #pragma unroll
-        for (int q = 0; q < WL; q++) {
+        for (int q = 0; q < BUILD_WORDS; q++) {
             const uint32_t m = 0x55555555u;
             const uint32_t xl = ax[q] & m, xh = (ax[q] >> 1) & m, yl = ay[q] & m, yh = (ay[q] >> 1) & m;
             const uint32_t x0 = ~(xl | xh) & m, x1 = xl & ~xh, x2 = xh & ~xl;
@@ -126,29 +126,46 @@ __global__ __launch_bounds__(BUILD_WAVES * 64) void class_build_kernel(
             cnt[3] += __popc((x0 & y2) | (x2 & y0));
             cnt[4] += __popc(x1 & y1);
         }
+        // counts packed two to a word (each at most CLASS_BLOCK < 2^16)
         const uint32_t s01 = wave_scan(cnt[0] | (cnt[1] << 16), lane);
         const uint32_t s23 = wave_scan(cnt[2] | (cnt[3] << 16), lane);
         const uint32_t s4 = wave_scan(cnt[4], lane);
-        const uint32_t t01 = (uint32_t)__builtin_amdgcn_readlane((int)s01, 63);
-        const uint32_t t23 = (uint32_t)__builtin_amdgcn_readlane((int)s23, 63);
-        const uint32_t t4 = (uint32_t)__builtin_amdgcn_readlane((int)s4, 63);
+        if (lane == 63) {
+            wtot[w][0] = s01;
+            wtot[w][1] = s23;
+            wtot[w][2] = s4;
+        }
+        __syncthreads();
+        uint32_t o01 = 0, o23 = 0, o4 = 0, t01 = 0, t23 = 0, t4 = 0;
+#pragma unroll
+        for (int v = 0; v < BUILD_WAVES; v++) {
+            const uint32_t a = wtot[v][0], c = wtot[v][1], e = wtot[v][2];
+            if ((uint32_t)v < w) {
+                o01 += a;
+                o23 += c;
+                o4 += e;
+            }
+            t01 += a;
+            t23 += c;
+            t4 += e;
+        }
         const uint32_t tot[5] = {t01 & 0xffffu, t01 >> 16, t23 & 0xffffu, t23 >> 16, t4};
-        const uint32_t inc[5] = {s01 & 0xffffu, s01 >> 16, s23 & 0xffffu, s23 >> 16, s4};
+        const uint32_t i01 = s01 + o01, i23 = s23 + o23, i4 = s4 + o4;  // inclusive over the workgroup
+        const uint32_t inc[5] = {i01 & 0xffffu, i01 >> 16, i23 & 0xffffu, i23 >> 16, i4};
         uint32_t pb[6];
         pb[0] = 0;
 #pragma unroll
         for (int k = 0; k < 5; k++) {
-            c[k * 64 + lane] = pb[k] + inc[k] - cnt[k];
+            ctr[k * BUILD_THREADS + t] = pb[k] + inc[k] - cnt[k];
             pb[k + 1] = pb[k] + (tot[k] + CLASS_GROUP - 1) / CLASS_GROUP * CLASS_GROUP;
         }
-        wave_sync();
         // a word's 16 sites: the 16 counter increments issued back to back
-        // (a lane's increments of one counter return in issue order, so its
+        // (a thread's increments of one counter return in issue order, so its
         // entries keep site order), then the 16 entries written; a missing
         // site (entry MISS) adds 0 and writes the stage's last entry, which
         // no list reaches
 #pragma unroll
-        for (int q = 0; q < WL; q++) {
+        for (int q = 0; q < BUILD_WORDS; q++) {
             uint32_t pos[16], ent[16];
 #pragma unroll
             for (int i = 0; i < 16; i++) {
@@ -156,21 +173,21 @@ __global__ __launch_bounds__(BUILD_WAVES * 64) void class_build_kernel(
                 const uint32_t nib = (uint32_t)(PATTERN_LUT >> (4 * idx)) & 15u;
                 const uint32_t k = nib & 7u;
                 const bool miss = k == 7u;
-                pos[i] = atomicAdd(&c[(miss ? 0u : k) * 64 + lane], miss ? 0u : 1u);
-                ent[i] = miss ? MISS : ((CLASS_BLOCK / 64 * lane + 16u * q + (uint32_t)i) << 1) | (nib >> 3);
+                pos[i] = atomicAdd(&ctr[(miss ? 0u : k) * BUILD_THREADS + t], miss ? 0u : 1u);
+                ent[i] = miss ? MISS : ((BUILD_SITES * t + 16u * q + (uint32_t)i) << 1) | (nib >> 3);
             }
 #pragma unroll
-            for (int i = 0; i < 16; i++) sg[ent[i] == MISS ? CLASS_STRIDE - 1 : pos[i]] = (uint16_t)ent[i];
+            for (int i = 0; i < 16; i++) stage[ent[i] == MISS ? CLASS_STRIDE - 1 : pos[i]] = (uint16_t)ent[i];
         }
-        wave_sync();
+        __syncthreads();  // the block's lists complete in LDS
         // out in the group layout: a lane's four entries of a group side by side
         uint2* dst = reinterpret_cast<uint2*>(lists + ((size_t)slot * nblk + b) * CLASS_STRIDE);
-        for (uint32_t g = 0; g < pb[5] / CLASS_GROUP; g++) {
-            const uint16_t* src = sg + g * CLASS_GROUP + lane;
+        for (uint32_t g = w; g < pb[5] / CLASS_GROUP; g += BUILD_WAVES) {
+            const uint16_t* src = stage + g * CLASS_GROUP + lane;
             dst[(size_t)g * 64 + lane] =
                 make_uint2((uint32_t)src[0] | ((uint32_t)src[64] << 16), (uint32_t)src[128] | ((uint32_t)src[192] << 16));
         }
-        if (lane == 0) {
+        if (t == 0) {
             uint4 cv;
             cv.x = tot[0] | (tot[1] << 16);
             cv.y = tot[2] | (tot[3] << 16);
@@ -178,9 +195,9 @@ __global__ __launch_bounds__(BUILD_WAVES * 64) void class_build_kernel(
             cv.w = 0;
             reinterpret_cast<uint4*>(counts)[(size_t)slot * nblk + b] = cv;
         }
-        wave_sync();
+        __syncthreads();  // the stage and the wavefront totals are free for the next block
     }
-    if (lane == 0) built[slot] = make_uint2(x, y);
+    if (t == 0) built[slot] = make_uint2(x, y);
 }
 
 namespace {
@@ -588,8 +605,8 @@ namespace klaunch {
 hipError_t class_build(const uint32_t* geno, uint32_t row_words, const uint32_t* sx, const uint32_t* sy,
                        const int* active, uint32_t nslots, uint32_t nblk, uint16_t* lists, uint16_t* counts,
                        uint2* built, hipStream_t st) {
-    hipLaunchKernelGGL(class_build_kernel, dim3((nslots + BUILD_WAVES - 1) / BUILD_WAVES), dim3(BUILD_WAVES * 64), 0,
-                       st, geno, row_words, sx, sy, active, nslots, nblk, lists, counts, built);
+    hipLaunchKernelGGL(class_build_kernel, dim3(nslots), dim3(BUILD_THREADS), 0, st, geno, row_words, sx, sy, active,
+                       nslots, nblk, lists, counts, built);
     return hipGetLastError();
 }
 
