@@ -410,14 +410,16 @@ __device__ __forceinline__ void class_group(uint2 q, uint32_t jg, uint32_t n, ui
         class_site<K>(A, P, a2, b2, c, eps);
         class_site<K>(A, P, a3, b3, c, eps);
         since += 4;
-    } else {
-        const uint32_t m = (n - jg + 63) / 64;  // chunks of the group within the list (1 to 4)
-        for (uint32_t sub = 0; sub < m; sub++) {
-            const uint32_t e = sub == 0 ? e0 : sub == 1 ? e1 : sub == 2 ? e2 : e3;
-            double a, b;
-            class_freqs<K, true>(afd, e, a, b);
-            if (jg + 64 * sub + lane < n) class_site<K>(A, P, a, b, c, eps);
-        }
+    } else {  // the list's last group: its 1 to 4 chunks, the last one partial
+        const uint32_t m = (n - jg + 63) / 64;
+        double a[4], b[4];
+        class_freqs<K, true>(afd, e0, a[0], b[0]);
+        class_freqs<K, true>(afd, e1, a[1], b[1]);
+        class_freqs<K, true>(afd, e2, a[2], b[2]);
+        class_freqs<K, true>(afd, e3, a[3], b[3]);
+#pragma unroll
+        for (uint32_t sub = 0; sub < 4; sub++)
+            if (jg + 64 * sub + lane < n) class_site<K>(A, P, a[sub], b[sub], c, eps);
         since += m;
     }
     class_log(A, P, since);
