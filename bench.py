@@ -14,8 +14,9 @@ reference's loop shape, timed on a bounded sample of the same workload on this
 host's cores), every output's error against that reference-order fit
 ("errors"), the FP32-mixed form on the same workload ("mixed": throughput,
 ratio to the FP64 form, errors) and the serving workload ("serving": a
-device-resident 10,000 x 100,000 cohort, saturated pair-fits/s, p50/p99
-request latency at 0.7 load over 100 Poisson requests, 1-query latency).
+device-resident 10,000 x 100,000 cohort in the FP64 form, saturated
+pair-fits/s, p50/p99 request latency at 0.7 load over 100 Poisson requests,
+1-query latency).
 Other modes:
 --serving (device-resident cohort answering query requests, p50/p99),
 --soft (likelihood input), --cli (the kestrel binary end to end on the cohort
@@ -168,7 +169,7 @@ def percentile(a, q):
     return float(np.percentile(np.asarray(a, np.float64), q)) if len(a) else None
 
 
-def serving_run(config="c5", precision="mixed", policy="srpt", slots=None, sat_requests=12, requests=100,
+def serving_run(config="c5", precision="fp64", policy="srpt", slots=None, sat_requests=12, requests=100,
                 load=0.7, single_probes=20):
     """Serving workload: a device-resident cohort (default 10,000 x 100,000,
     the FP32-mixed form) answers requests of 1-256 new query samples against
@@ -645,8 +646,9 @@ def main():
     ap.add_argument("--poisson-requests", type=int, default=100, help="serving: requests of the Poisson phase")
     ap.add_argument("--single-probes", type=int, default=20,
                     help="serving: 1-query requests at uniform times in the Poisson phase")
-    ap.add_argument("--serving-precision", choices=("fp64", "mixed"), default="mixed",
-                    help="form of the serving workload (its configuration names the FP32-mixed kernels)")
+    ap.add_argument("--serving-precision", choices=("fp64", "mixed"), default="fp64",
+                    help="form of the serving workload: FP64 (the class-sorted sweep, faster on this device than "
+                         "the FP32-mixed kernels the configuration names) or mixed")
     ap.add_argument("--no-mixed-leg", action="store_true",
                     help="skip the FP32-mixed fit of the same workload in the default run")
     ap.add_argument("--no-small-jobs", action="store_true",
@@ -787,6 +789,7 @@ def main():
         mixed_leg = {
             "value": round(fm["value"], 1), "unit": "pair-fits/s", "ms_per_step": round(fm["seconds"] / args.steps * 1e3, 2),
             "vs_fp64_form": round(fm["value"] / value, 4),
+            "forms": {"mixed": "sweep_mixed (site-table sweep, FP32 Hessian terms)", "fp64": kernel},
             "roofline": {"bound": "valu", "achieved": round(fm["achieved"], 3), "peak": FP32_VALU_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(fm["achieved"] / FP32_VALU_PEAK_TFLOPS, 4),
                          "kernel": "sweep_mixed", **fm["launch"],
