@@ -42,12 +42,12 @@ FLOP_PER_SITE_EVAL = 156  # 136 (gradient + Hessian) + 20 (objective, log counte
 # HBM bytes per sweep launch on the default workload, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE scaled by a plain 256 MiB copy; counters cannot be
 # read inside this process, so the line carries the profiled figure and says so
-SWEEP_TRAFFIC_BYTES_C2 = {"sweep_class_fp64": 1.662e9, "sweep_fp64": 1.563e8}
+SWEEP_TRAFFIC_BYTES_C2 = {"sweep_class_fp64": 1.657e9, "sweep_fp64": 1.563e8}
 SWEEP_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of {k} on this workload, scaled by a 256 MiB "
                         "copy_kernel (profiles/{k}_c2.md); profiled, not measured in this run")
 CLASS_NOTE = ("achieved counts 156 FLOP per site evaluation, the direct per-site form's work (9 coefficients, 9 "
               "gradient and 45 Hessian terms); the class-sorted sweep obtains the same f, g, H from 2-11 sums a site "
-              "in each genotype-pair pattern's basis (about 35 VALU instructions a site, VALU issue 0.57 of SIMD "
+              "in each genotype-pair pattern's basis (about 31 VALU instructions a site, VALU issue 0.62 of SIMD "
               "cycles in profiles/sweep_class_fp64_c2.md), so frac can exceed 1")
 
 

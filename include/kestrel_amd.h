@@ -192,7 +192,7 @@ int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* poi
  * at most 64 MiB / (88 + 448 x segments + lists) bytes' worth of points
  * (segments: one per 4,096 kept sites by default; lists: 8 + 18,960 bytes
  * per 8,192 kept sites with the class-sorted sweep; about 5,900 points at
- * 100,000 sites with the table sweep, about 260 with the class-sorted one)
+ * 100,000 sites with the table sweep, 256 with the class-sorted one)
  * and returns KESTREL_EINVAL beyond that.  The call allocates nothing, runs on
  * the cohort's stream and returns when the results are written.  Every call
  * on one GPU shares the array: calls that overlap on one GPU (from several
