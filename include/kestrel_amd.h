@@ -268,6 +268,12 @@ int kestrel_cohort_create_gt_multi(kestrel_cohort** out, uint32_t n_samples, con
  * lane; the two give the reference's sums in different summation orders, so
  * records differ in the last bits), "class_waves" (pairs per workgroup of
  * the class-sorted sweep: 8, 12, 14 or 16 (default); results do not change),
+ * "class_list_mib" (device memory the class-sorted sweep's pattern lists,
+ * about 2 bytes per kept site and slot, may take in one fit, evaluation or
+ * serving engine, in MiB; 0 = a quarter of the free memory, at most 16 GiB
+ * (default); fewer slots fit, evaluations go in chunks of the slots that
+ * fit, results do not change; a call returns KESTREL_EDEVICE when 256 slots
+ * a group do not fit),
  * "precision" (KESTREL_FP64 or KESTREL_MIXED, for 2-bit and likelihood-row
  * cohorts, tri/tetra-allelic parts included: the form of the next fits; this
  * one changes records),

@@ -73,11 +73,16 @@ int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, con
                   uint32_t ns, double eps, double* partial, hipStream_t st, bool skip_masked = false,
                   bool lanes = false, ClassLists* cls = nullptr, bool build = true) {
     if (cls) {  // the class-sorted sweep (c->class_form(); partials in c->nseg_cls segments)
-        if (build)
-            HIPCHK(klaunch::class_build(c->geno.p, c->row_words, sx, sy, active, ns, cls->nblk, cls->lists.p,
-                                        cls->counts.p, cls->built.p, st));
-        HIPCHK(klaunch::sweep_class(c->af.p, c->v_pad, c->seg_cls.p, c->nseg_cls, cls->lists.p, cls->counts.p,
-                                    cls->nblk, pt, active, ns, eps, partial, st, (int)c->opt_class_waves));
+        // (without build the lists hold every slot: cls->cap >= ns)
+        for (uint32_t s0 = 0; s0 < ns; s0 += cls->cap) {
+            const uint32_t nc = std::min(cls->cap, ns - s0);
+            if (build)
+                HIPCHK(klaunch::class_build(c->geno.p, c->row_words, sx, sy, active, ns, s0, nc, cls->nblk,
+                                            cls->lists.p, cls->counts.p, cls->built.p, st));
+            HIPCHK(klaunch::sweep_class(c->af.p, c->v_pad, c->seg_cls.p, c->nseg_cls, cls->lists.p, cls->counts.p,
+                                        cls->nblk, pt, active, ns, s0, nc, eps, partial, st,
+                                        (int)c->opt_class_waves));
+        }
         return KESTREL_OK;
     }
     if (lanes) {  // FP64 fits of 2-bit and biallelic likelihood-row cohorts (checked by the caller)
@@ -691,6 +696,11 @@ int kestrel_set_option(kestrel_cohort* c, const char* key, int64_t value) {
         c->opt_class_waves = value;
         return KESTREL_OK;
     }
+    if (!strcmp(key, "class_list_mib")) {
+        if (value < 0) return fail(KESTREL_EINVAL, "class_list_mib must be >= 0");
+        c->opt_class_list_mib = value;
+        return KESTREL_OK;
+    }
     if (!strcmp(key, "skip_masked")) {
         if (value < -1 || value > 1) return fail(KESTREL_EINVAL, "skip_masked must be -1, 0 or 1");
         c->opt_skip_masked = value;
@@ -804,7 +814,14 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     // the class-sorted sweep (option "class_sweep"): one pair a wavefront over
     // its sites sorted by genotype-pair pattern; the sweep passes over missing
     // calls by construction, so claims in runs add nothing
+    // its lists take ~2 bytes a site a slot: the slots are capped by the
+    // device memory
     const bool use_class = !lanes && c->class_form();
+    const uint64_t class_cap = use_class ? class_slot_cap(c->class_blocks(), 2, c->opt_class_list_mib) : 0;
+    if (use_class && class_cap == 0)
+        return fail(KESTREL_EDEVICE, "device memory for the class-sorted sweep's lists of 2 x 256 slots (%llu bytes) "
+                    "is not free (option class_sweep 0 selects the site-table sweep)",
+                    (unsigned long long)(512 * ClassLists::slot_bytes(c->class_blocks())));
     const bool use_runs = !lanes && !use_class && !c->soft && c->parts.empty() &&
                           (c->opt_skip_masked == 1 ||
                            (c->opt_skip_masked < 0 && !c->mixed && c->missing_frac >= 0.02));
@@ -820,6 +837,7 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
                                          : default_slots(c) * fills;
     nslots = std::min<uint64_t>(nslots, ((uint64_t)n_pairs + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB);
     nslots = (nslots + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB;
+    if (use_class) nslots = std::min(nslots, class_cap);
     const uint32_t ns = (uint32_t)nslots;
     // Two slot groups on two streams when there is work for both: while one
     // group folds and steps, the other's sweep keeps the device busy.  Both
@@ -1013,8 +1031,8 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
                 // (the class lists of newly claimed pairs are built outside the
                 // timed interval: it times the sweep kernel alone)
                 if (use_class)
-                    HIPCHK(klaunch::class_build(c->geno.p, c->row_words, G.sx.p, G.sy.p, G.active.p, ns, G.cls.nblk,
-                                                G.cls.lists.p, G.cls.counts.p, G.cls.built.p, G.stream));
+                    HIPCHK(klaunch::class_build(c->geno.p, c->row_words, G.sx.p, G.sy.p, G.active.p, ns, 0, ns,
+                                                G.cls.nblk, G.cls.lists.p, G.cls.counts.p, G.cls.built.p, G.stream));
                 HIPCHK(hipEventRecord(G.ev[2 * i], G.stream));
                 int rs = launch_sweeps(c, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, tune->epsilon, G.part.p, G.stream,
                                        use_runs, lanes, use_class ? &G.cls : nullptr, false);
@@ -1238,6 +1256,11 @@ int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* poi
     HIPCHK(dpt.alloc((size_t)9 * ns));
     // the class-sorted sweep writes its one segment, the exact kernel the
     // cohort's segments: partials start at zero and are folded over both
+    const uint64_t class_cap = c->class_form() ? class_slot_cap(c->class_blocks(), 1, c->opt_class_list_mib) : 0;
+    if (c->class_form() && class_cap == 0)
+        return fail(KESTREL_EDEVICE, "device memory for the class-sorted sweep's lists of 256 points (%llu bytes) is "
+                    "not free (option class_sweep 0 selects the site-table sweep)",
+                    (unsigned long long)(256 * ClassLists::slot_bytes(c->class_blocks())));
     const bool use_class = c->class_form();
     const uint32_t nfold = std::max(c->nseg_total(), use_class ? c->nseg_cls : 0u);
     HIPCHK(dpart.alloc((size_t)nfold * NPART * ns));
@@ -1254,7 +1277,8 @@ int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* poi
     ClassLists cls;
     if (use_class) {
         HIPCHK(hipMemsetAsync(dpart.p, 0, dpart.n * sizeof(double), st));
-        const int rc = cls.alloc(ns, c->class_blocks(), st);
+        // the lists of at most class_cap points at a time
+        const int rc = cls.alloc((uint32_t)std::min<uint64_t>(ns, class_cap), c->class_blocks(), st);
         if (rc) return rc;
     }
     int rs = launch_sweeps(c, dsx.p, dsy.p, dpt.p, dact.p, ns, eps, dpart.p, st, false, false,

@@ -84,17 +84,17 @@ static_assert(BUILD_WORDS == 2, "a thread loads its words of a row as one uint2"
 
 __global__ __launch_bounds__(BUILD_THREADS) void class_build_kernel(
     const uint32_t* __restrict__ geno, uint32_t row_words, const uint32_t* __restrict__ sx,
-    const uint32_t* __restrict__ sy, const int* __restrict__ active, uint32_t nslots, uint32_t nblk,
-    uint16_t* __restrict__ lists, uint16_t* __restrict__ counts, uint2* __restrict__ built) {
+    const uint32_t* __restrict__ sy, const int* __restrict__ active, uint32_t nslots, uint32_t slot0, uint32_t nchunk,
+    uint32_t nblk, uint16_t* __restrict__ lists, uint16_t* __restrict__ counts, uint2* __restrict__ built) {
     __shared__ uint32_t ctr[5 * BUILD_THREADS];     // pattern k's counter of thread t at k BUILD_THREADS + t
     __shared__ uint32_t wtot[BUILD_WAVES][4];       // each wavefront's packed pattern counts
     __shared__ __attribute__((aligned(16))) uint16_t stage[CLASS_STRIDE];
     const uint32_t t = threadIdx.x, lane = t & 63u;
     const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(t >> 6));
-    const uint32_t slot = blockIdx.x;
-    if (slot >= nslots || active[slot] == 0) return;  // uniform over the workgroup
+    const uint32_t ls = blockIdx.x, slot = slot0 + ls;  // the lists' slot (in the chunk), the slot
+    if (ls >= nchunk || slot >= nslots || active[slot] == 0) return;  // uniform over the workgroup
     const uint32_t x = sx[slot], y = sy[slot];
-    const uint2 b0 = built[slot];
+    const uint2 b0 = built[ls];
     if (b0.x == x && b0.y == y) return;
     const uint32_t* rowx = geno + (size_t)x * row_words;
     const uint32_t* rowy = geno + (size_t)y * row_words;
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(BUILD_THREADS) void class_build_kernel(
         }
         __syncthreads();  // the block's lists complete in LDS
         // out in the group layout: a lane's four entries of a group side by side
-        uint2* dst = reinterpret_cast<uint2*>(lists + ((size_t)slot * nblk + b) * CLASS_STRIDE);
+        uint2* dst = reinterpret_cast<uint2*>(lists + ((size_t)ls * nblk + b) * CLASS_STRIDE);
         for (uint32_t g = w; g < pb[5] / CLASS_GROUP; g += BUILD_WAVES) {
             const uint16_t* src = stage + g * CLASS_GROUP + lane;
             dst[(size_t)g * 64 + lane] =
@@ -193,11 +193,11 @@ __global__ __launch_bounds__(BUILD_THREADS) void class_build_kernel(
             cv.y = tot[2] | (tot[3] << 16);
             cv.z = tot[4];
             cv.w = 0;
-            reinterpret_cast<uint4*>(counts)[(size_t)slot * nblk + b] = cv;
+            reinterpret_cast<uint4*>(counts)[(size_t)ls * nblk + b] = cv;
         }
         __syncthreads();  // the stage and the wavefront totals are free for the next block
     }
-    if (t == 0) built[slot] = make_uint2(x, y);
+    if (t == 0) built[ls] = make_uint2(x, y);
 }
 
 namespace {
@@ -563,14 +563,14 @@ template <int W>
 __global__ __launch_bounds__(W * 64, class_waves_per_simd<W>()) void sweep_class_fp64(
     const double* __restrict__ af, uint32_t v_pad, const uint32_t* __restrict__ seg_bounds,
     const uint16_t* __restrict__ lists, const uint16_t* __restrict__ counts, uint32_t nblk,
-    const double* __restrict__ pt, const int* __restrict__ active, uint32_t nslots, double eps,
-    double* __restrict__ partial) {
+    const double* __restrict__ pt, const int* __restrict__ active, uint32_t nslots, uint32_t slot0, uint32_t nchunk,
+    double eps, double* __restrict__ partial) {
     __shared__ double2 afb[CLASS_BLOCK];
     __shared__ double sums[W][NACC + 1];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const uint32_t slot = blockIdx.x * W + w;
-    const bool live = slot < nslots && active[slot] != 0;
+    const uint32_t ls = blockIdx.x * W + w, slot = slot0 + ls;  // the lists' slot (in the chunk), the slot
+    const bool live = ls < nchunk && slot < nslots && active[slot] != 0;
     if (!__syncthreads_or(live)) return;
     const uint32_t seg = blockIdx.y;
     const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
@@ -583,9 +583,9 @@ __global__ __launch_bounds__(W * 64, class_waves_per_simd<W>()) void sweep_class
     double F = 0.0, P = 1.0;
     uint32_t since = 0, n = 0;
     double* my = sums[w];
-    class_pass<0, 0, W>(af, v_pad, bb0, nb, lists, counts, nblk, slot, live, w, lane, afb, my, F, P, since, n, c, eps);
-    class_pass<1, 2, W>(af, v_pad, bb0, nb, lists, counts, nblk, slot, live, w, lane, afb, my, F, P, since, n, c, eps);
-    class_pass<3, 4, W>(af, v_pad, bb0, nb, lists, counts, nblk, slot, live, w, lane, afb, my, F, P, since, n, c, eps);
+    class_pass<0, 0, W>(af, v_pad, bb0, nb, lists, counts, nblk, ls, live, w, lane, afb, my, F, P, since, n, c, eps);
+    class_pass<1, 2, W>(af, v_pad, bb0, nb, lists, counts, nblk, ls, live, w, lane, afb, my, F, P, since, n, c, eps);
+    class_pass<3, 4, W>(af, v_pad, bb0, nb, lists, counts, nblk, ls, live, w, lane, afb, my, F, P, since, n, c, eps);
     if (!live) return;
     if (since) F = F + sleef_log_pos(P);
 #pragma unroll
@@ -605,19 +605,22 @@ __global__ __launch_bounds__(W * 64, class_waves_per_simd<W>()) void sweep_class
 namespace klaunch {
 
 hipError_t class_build(const uint32_t* geno, uint32_t row_words, const uint32_t* sx, const uint32_t* sy,
-                       const int* active, uint32_t nslots, uint32_t nblk, uint16_t* lists, uint16_t* counts,
-                       uint2* built, hipStream_t st) {
-    hipLaunchKernelGGL(class_build_kernel, dim3(nslots), dim3(BUILD_THREADS), 0, st, geno, row_words, sx, sy, active,
-                       nslots, nblk, lists, counts, built);
+                       const int* active, uint32_t nslots, uint32_t slot0, uint32_t nchunk, uint32_t nblk,
+                       uint16_t* lists, uint16_t* counts, uint2* built, hipStream_t st) {
+    if (nchunk == 0) return hipSuccess;
+    hipLaunchKernelGGL(class_build_kernel, dim3(nchunk), dim3(BUILD_THREADS), 0, st, geno, row_words, sx, sy, active,
+                       nslots, slot0, nchunk, nblk, lists, counts, built);
     return hipGetLastError();
 }
 
 hipError_t sweep_class(const double* af, uint32_t v_pad, const uint32_t* seg_bounds, uint32_t nseg,
                        const uint16_t* lists, const uint16_t* counts, uint32_t nblk, const double* pt,
-                       const int* active, uint32_t nslots, double eps, double* partial, hipStream_t st, int waves) {
+                       const int* active, uint32_t nslots, uint32_t slot0, uint32_t nchunk, double eps, double* partial,
+                       hipStream_t st, int waves) {
+    if (nchunk == 0) return hipSuccess;
     auto go = [&](auto kern, int W) {
-        hipLaunchKernelGGL(kern, dim3((nslots + W - 1) / W, nseg), dim3(W * 64), 0, st, af, v_pad, seg_bounds, lists,
-                           counts, nblk, pt, active, nslots, eps, partial);
+        hipLaunchKernelGGL(kern, dim3((nchunk + W - 1) / W, nseg), dim3(W * 64), 0, st, af, v_pad, seg_bounds, lists,
+                           counts, nblk, pt, active, nslots, slot0, nchunk, eps, partial);
     };
     if (waves == 8)
         go(sweep_class_fp64<8>, 8);

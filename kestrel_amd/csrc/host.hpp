@@ -139,13 +139,17 @@ int launch_exact(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, cons
 // Stops the serving engine of a cohort (if any) and frees its buffers.
 void serve_release(ServeState* s);
 
-// Per-slot pattern lists of the class-sorted sweep (class_sweep.hip).
+// Per-slot pattern lists of the class-sorted sweep (class_sweep.hip), for
+// cap slots (launch_sweeps sweeps more slots than that in chunks of cap).
 struct ClassLists {
     DevBuf<uint16_t> lists, counts;
     DevBuf<uint2> built;
-    uint32_t nblk = 0;
+    uint32_t nblk = 0, cap = 0;
+    // device bytes of one slot's lists, counts and built mark
+    static uint64_t slot_bytes(uint32_t nblocks) { return (uint64_t)nblocks * (2 * CLASS_STRIDE + 16) + 8; }
     int alloc(uint32_t nslots, uint32_t nblocks, hipStream_t st) {
         nblk = nblocks;
+        cap = nslots;
         HIPCHK(lists.alloc((size_t)nslots * nblk * CLASS_STRIDE));
         HIPCHK(counts.alloc((size_t)nslots * nblk * 8));
         HIPCHK(built.alloc(nslots));
@@ -154,6 +158,20 @@ struct ClassLists {
         return 0;
     }
 };
+
+// Slots whose class lists fit the device memory that one fit, evaluation or
+// serving engine gives them (limit_mib MiB if set, else a quarter of the free
+// memory, at most 16 GiB; shared by `groups` slot groups), a multiple of 256;
+// 0 when 256 do not fit.
+inline uint64_t class_slot_cap(uint32_t nblocks, int groups, int64_t limit_mib) {
+    uint64_t budget = (uint64_t)limit_mib << 20;
+    if (limit_mib <= 0) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 0;
+        budget = std::min<uint64_t>(free_b / 4, 16ull << 30);
+    }
+    return budget / (uint64_t)std::max(1, groups) / ClassLists::slot_bytes(nblocks) / 256 * 256;
+}
 
 struct kestrel_cohort {
     int device = 0;
@@ -187,6 +205,7 @@ struct kestrel_cohort {
     int64_t opt_pair_lanes = 1;    // lanes a pair: 1, 64 (its sites across a wavefront; FP64 fits), -1 auto
     int64_t opt_class_sweep = 1;   // FP64 fits of 2-bit cohorts by the class-sorted sweep (0: the table sweep)
     int64_t opt_class_waves = CLASS_WAVES;  // its wavefronts (pairs) per workgroup: 8, 12, 14 or 16
+    int64_t opt_class_list_mib = 0;  // device memory its lists may take (0: automatic)
     DevBuf<uint32_t> seg_cls;      // its segment bounds (multiples of CLASS_BLOCK; one segment)
     uint32_t nseg_cls = 0;
     hipStream_t stream2 = nullptr; // second stream of a fit

@@ -116,14 +116,17 @@ hipError_t sweep_lanes(const uint32_t* geno, uint32_t row_words, const double* a
 // class-sorted sweep of 2-bit FP64 fits: per slot the pattern lists of its
 // pair (built where built[slot] differs from (sx, sy); nblk blocks of
 // CLASS_STRIDE entries, counts 8 uint16 a block), then the sweep over segments
-// whose bounds are multiples of CLASS_BLOCK; partials in sweep's layout
+// whose bounds are multiples of CLASS_BLOCK; partials in sweep's layout.
+// Both cover the slots slot0 .. slot0 + nchunk - 1, whose lists (and built
+// marks) sit at 0 .. nchunk - 1 of the arrays; the other arrays are indexed
+// by slot (points and partials with stride nslots)
 hipError_t class_build(const uint32_t* geno, uint32_t row_words, const uint32_t* sx, const uint32_t* sy,
-                       const int* active, uint32_t nslots, uint32_t nblk, uint16_t* lists, uint16_t* counts,
-                       uint2* built, hipStream_t st);
+                       const int* active, uint32_t nslots, uint32_t slot0, uint32_t nchunk, uint32_t nblk,
+                       uint16_t* lists, uint16_t* counts, uint2* built, hipStream_t st);
 hipError_t sweep_class(const double* af, uint32_t v_pad, const uint32_t* seg_bounds, uint32_t nseg,
                        const uint16_t* lists, const uint16_t* counts, uint32_t nblk, const double* pt,
-                       const int* active, uint32_t nslots, double eps, double* partial, hipStream_t st,
-                       int waves = CLASS_WAVES);
+                       const int* active, uint32_t nslots, uint32_t slot0, uint32_t nchunk, double eps,
+                       double* partial, hipStream_t st, int waves = CLASS_WAVES);
 hipError_t step(SlotState* stt, uint32_t* sx, uint32_t* sy, double* pt, int* active, uint32_t nslots,
                 double* partial, uint32_t nseg, const uint32_t* pairs, uint64_t npairs, const uint64_t* order,
                 unsigned long long* next_pair, kestrel_output* out, const kdev::Tune& t, int* n_live,

@@ -146,10 +146,10 @@ extern "C" int kestrel_evaluate_noalloc(kestrel_cohort* c, const uint32_t* pairs
         hipLaunchKernelGGL(fill_kernel, dim3(1024), dim3(256), 0, st, reinterpret_cast<uint32_t*>(part),
                            (size_t)nseg * NPART * ns * 2, 0u);
         HIPCHK(hipGetLastError());
-        HIPCHK(klaunch::class_build(c->geno.p, c->row_words, sx, sy, act, (uint32_t)ns, nblk, lists, counts, built,
-                                    st));
+        HIPCHK(klaunch::class_build(c->geno.p, c->row_words, sx, sy, act, (uint32_t)ns, 0, (uint32_t)ns, nblk, lists,
+                                    counts, built, st));
         HIPCHK(klaunch::sweep_class(c->af.p, c->v_pad, c->seg_cls.p, c->nseg_cls, lists, counts, nblk, pt, act,
-                                    (uint32_t)ns, eps, part, st, (int)c->opt_class_waves));
+                                    (uint32_t)ns, 0, (uint32_t)ns, eps, part, st, (int)c->opt_class_waves));
     } else {
         HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, sx, sy, pt, act, (uint32_t)ns,
                               eps, part, st, c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride, false,

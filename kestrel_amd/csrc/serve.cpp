@@ -344,11 +344,11 @@ int engine_round(ServeState* S, bool* idle) {
         for (int g = 0; g < S->ngroups; g++) {
             ServeState::Group& G = S->grp[g];
             if (sweep && S->cls) {
-                HIPCHK(klaunch::class_build(S->geno.p, S->row_words, G.sx.p, G.sy.p, G.active.p, S->ns, G.cls.nblk,
-                                            G.cls.lists.p, G.cls.counts.p, G.cls.built.p, G.stream));
+                HIPCHK(klaunch::class_build(S->geno.p, S->row_words, G.sx.p, G.sy.p, G.active.p, S->ns, 0, S->ns,
+                                            G.cls.nblk, G.cls.lists.p, G.cls.counts.p, G.cls.built.p, G.stream));
                 HIPCHK(klaunch::sweep_class(S->c->af.p, S->c->v_pad, S->c->seg_cls.p, S->c->nseg_cls, G.cls.lists.p,
-                                            G.cls.counts.p, G.cls.nblk, G.pt.p, G.active.p, S->ns, S->tune.epsilon,
-                                            G.partial.p, G.stream, (int)S->c->opt_class_waves));
+                                            G.cls.counts.p, G.cls.nblk, G.pt.p, G.active.p, S->ns, 0, S->ns,
+                                            S->tune.epsilon, G.partial.p, G.stream, (int)S->c->opt_class_waves));
             } else if (sweep) {
                 HIPCHK(klaunch::sweep(S->geno.p, S->row_words, S->c->af.p, S->seg.p, S->nseg, G.sx.p, G.sy.p, G.pt.p,
                                       G.active.p, S->ns, S->tune.epsilon, G.partial.p, G.stream, nullptr, 0, 3,
@@ -527,8 +527,15 @@ int kestrel_serve_start(kestrel_cohort* c, const kestrel_tune* tune, uint32_t ma
                   : S->cls        ? (uint64_t)std::max(1, c->n_cu) * 16
                                   : slot_blocks * SWEEP_TPB;
     ns = (ns + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB;
-    S->ns = (uint32_t)ns;
     S->ngroups = c->opt_streams >= 2 ? 2 : 1;
+    if (S->cls) {  // the slots' lists within the device memory they may take
+        const uint64_t cap = class_slot_cap(c->class_blocks(), S->ngroups, c->opt_class_list_mib);
+        if (cap == 0)
+            return fail(KESTREL_EDEVICE, "device memory for the class-sorted sweep's lists of %d x 256 slots is not "
+                        "free (option class_sweep 0 selects the site-table sweep)", S->ngroups);
+        ns = std::min(ns, cap);
+    }
+    S->ns = (uint32_t)ns;
     // a backlog of a quarter of the slots per sweep of a round refills
     // finished slots every step while keeping a newly admitted request close
     // to the front

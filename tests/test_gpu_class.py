@@ -132,3 +132,35 @@ def test_class_fit_c2_sample(O):
         out = co.fit_pairs(pairs)
     assert_bitexact(out, O.fit_pairs(codes, afk, pairs=pairs, order=O.ORDER_CLASS, nthreads=16))
     assert_within_tolerance(out, O.fit_pairs(codes, afk, pairs=pairs, order=O.ORDER_REF, nthreads=16))
+
+
+def test_class_list_limit_keeps_records():
+    """Option "class_list_mib" caps the lists' device memory: evaluations run
+    in chunks of the points that fit and fits with fewer slots, bit-identical
+    to the defaults; a cap below 256 slots a group is an error."""
+    say("case: class-sorted lists under a memory cap, 48 x 9500")
+    from kestrel_amd import _lib
+    c, co, codes, afk = class_cohort(48, 9500, 31, 0.05)
+    with co:
+        rng = np.random.default_rng(7)
+        pairs = rng.integers(0, 48, (700, 2))
+        pts = rng.dirichlet(np.ones(9), 700)
+        e0 = co.evaluate(pairs, pts)
+        f0 = co.fit_pairs()
+        sweeps0 = co.last_fit_stats()["sweeps"]
+        # 9,500 sites: two blocks, 37,928 bytes of lists a slot
+        co.set_option("class_list_mib", 10)  # 256 points at a time: three chunks
+        e1 = co.evaluate(pairs, pts)
+        for a, b in zip(e1, e0):
+            assert same(a, b).all()
+        co.set_option("class_list_mib", 20)  # 256 slots a group
+        f1 = co.fit_pairs()
+        assert co.last_fit_stats()["sweeps"] > sweeps0  # 1,176 pairs through 2 x 256 slots instead of 1,280
+        assert_bitexact(f1, f0)
+        co.set_option("class_list_mib", 1)  # not even 256 slots
+        with pytest.raises(_lib.KestrelError):
+            co.fit_pairs()
+        with pytest.raises(_lib.KestrelError):
+            co.evaluate(pairs, pts)
+        co.set_option("class_list_mib", 0)
+        assert_bitexact(co.fit_pairs(), f0)
