@@ -488,7 +488,7 @@ def timed_fits(co, steps, warmup):
         co.fit_pairs()
         log(f"warmup {i}: {time.perf_counter() - t:.3f}s")
     times, flops, sweep_ms, sweeps, busy_ms, groups, rec_ms, rec_pairs = [], 0.0, 0.0, 0, 0.0, 1, 0.0, 0
-    out = None
+    out, form = None, None
     for i in range(steps):
         t = time.perf_counter()
         out = co.fit_pairs()
@@ -502,6 +502,7 @@ def timed_fits(co, steps, warmup):
         groups = st["groups"]
         rec_ms += st["recondition_ms"]
         rec_pairs += st["recondition_pairs"]
+        form = st["form"]
         log(f"step {i}: {dt:.3f}s, {st['sweeps']} sweeps in {st['groups']} slot groups, sweep launches "
             f"{st['sweep_ms']:.1f} ms summed, {st['busy_ms']:.1f} ms with a sweep running"
             + (f", FP64 condition pass {st['recondition_ms']:.1f} ms for {st['recondition_pairs']} pairs"
@@ -510,7 +511,7 @@ def timed_fits(co, steps, warmup):
     offdiag = int((out["x"] != out["y"]).sum())
     # the slot groups' launches overlap: the device rate is the algorithmic
     # work over the time with at least one sweep running
-    return {"out": out, "seconds": total, "value": offdiag * steps / total,
+    return {"out": out, "seconds": total, "value": offdiag * steps / total, "form": form,
             "achieved": flops / (busy_ms / 1e3) / 1e12,
             "recondition": {"ms_per_step": round(rec_ms / steps, 2), "pairs_per_step": rec_pairs // max(1, steps)},
             "launch": {"avg_launch_ms": round(sweep_ms / sweeps, 4), "concurrent_launches": groups,
@@ -702,15 +703,7 @@ def main():
         v3 = int(round(v * args.multi * 0.8))
         v4 = int(round(v * args.multi * 0.2))
         idx = synth.multi_genotypes(n, (v - v3 - v4, v3, v4), seed=0xA4, missing=0.01)
-        buckets = []
-        for A, ix in zip((2, 3, 4), idx):
-            G = A * (A + 1) // 2
-            rows = np.zeros((ix.shape[1], n, G))
-            called = ix.T != 255
-            s_i, n_i = np.nonzero(called)
-            rows[s_i, n_i, ix.T[called]] = 1.0
-            rows[~called] = np.nan
-            buckets.append(rows)
+        buckets = list(synth.multi_onehot(idx))
     log(f"cohort {args.config}: {n} x {v}{' soft likelihoods' if args.soft else ''}"
         f"{f' with {args.multi:.0%} tri/tetra-allelic sites' if buckets else ''} generated in "
         f"{time.perf_counter() - t:.1f}s")
@@ -751,13 +744,15 @@ def main():
     fit64 = timed_fits(co, args.steps, args.warmup)
     out, value, total, achieved = fit64["out"], fit64["value"], fit64["seconds"], fit64["achieved"]
     offdiag = int((out["x"] != out["y"]).sum())
-    # FP64 fits of hard calls run the class-sorted sweep unless --class-sweep 0
-    class_form = args.precision == "fp64" and not args.soft and not buckets and args.class_sweep != 0
-    kernel = (("sweep_mixed + sweep_glm_mixed" if args.precision == "mixed" else "sweep_fp64 + sweep_glm")
-              if buckets  # (the --multi rows are one-hot: the 2-allele part is held as 2-bit codes)
+    # FP64 fits of hard calls run the class-sorted sweep unless --class-sweep
+    # 0 (the --multi rows are one-hot: their 2-allele part is held as 2-bit
+    # codes and takes it too when no called row lost its 1.0 to an allele
+    # below the minimum frequency); the library reports the form it ran
+    class_form = fit64["form"] == "class"
+    main = ("sweep_mixed" if args.precision == "mixed" else "sweep_class_fp64" if class_form else "sweep_fp64")
+    kernel = ((main + (" + sweep_glm_mixed" if args.precision == "mixed" else " + sweep_glm")) if buckets
               else ("sweep_gl_mixed" if args.precision == "mixed" else "sweep_gl") if args.soft
-              else "sweep_mixed" if args.precision == "mixed"
-              else "sweep_class_fp64" if class_form else "sweep_fp64")
+              else main)
     profiled = args.config == "c2" and not args.segment_sites and kernel in SWEEP_TRAFFIC_BYTES_C2
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_VALU_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_VALU_PEAK_TFLOPS, 4),

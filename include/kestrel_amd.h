@@ -318,6 +318,16 @@ int kestrel_last_fit_busy(const kestrel_cohort* c, double* busy_ms, uint32_t* gr
  * whose condition it estimated again (no counterpart in the reference). */
 int kestrel_last_fit_recondition(const kestrel_cohort* c, double* ms, uint64_t* pairs);
 
+/* The sweep form of the last kestrel_fit_pairs, which fixes its summation
+ * order (records of different forms differ in the last bits): the site-table
+ * sweep (one pair a lane; the mixed, soft-likelihood and tri/tetra-allelic
+ * sweeps), the class-sorted sweep of 2-bit rows (FP64, the 2-allele part of a
+ * tri/tetra-allelic cohort included; its other parts keep their sweeps), or a
+ * pair's sites across a wavefront (option "pair_lanes").  No counterpart in
+ * the reference (one summation order). */
+enum { KESTREL_FORM_TABLE = 0, KESTREL_FORM_CLASS = 1, KESTREL_FORM_LANES = 2 };
+int kestrel_last_fit_form(const kestrel_cohort* c, int* form);
+
 /* Serving: the cohort stays device-resident and answers requests of query
  * samples (new 2-bit rows over the cohort's n_sites, fitted with the cohort's
  * allele frequencies and kept sites) against the whole cohort or a subset of

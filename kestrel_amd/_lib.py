@@ -64,7 +64,7 @@ EXPORTS = (
     "kestrel_serve_stop", "kestrel_last_fit_busy", "kestrel_cohort_create_gl", "kestrel_cohort_create_gl_multi",
     "kestrel_cohort_multi_info", "kestrel_cohort_create_gt_multi", "kestrel_measure_fp32_peak", "kestrel_autotune",
     "kestrel_measure_lds", "kestrel_last_fit_recondition", "kestrel_evaluate_noalloc",
-    "kestrel_measure_hessian_form",
+    "kestrel_measure_hessian_form", "kestrel_last_fit_form",
 )
 
 
@@ -119,6 +119,7 @@ def load():
     L.kestrel_last_fit_stats.argtypes = [P, P, P, P]
     L.kestrel_last_fit_busy.argtypes = [P, P, P]
     L.kestrel_last_fit_recondition.argtypes = [P, P, P]
+    L.kestrel_last_fit_form.argtypes = [P, P]
     L.kestrel_selftest.argtypes = [ctypes.c_uint64, ctypes.c_uint64, P]
     L.kestrel_cohort_create_gt.argtypes = [P, P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_size_t, P]
     L.kestrel_cohort_create_gl.argtypes = [P, P, ctypes.c_uint32, ctypes.c_uint32, P, ctypes.c_double,

@@ -370,8 +370,11 @@ class Cohort:
         check(_lib.load().kestrel_last_fit_busy(self._h, ctypes.byref(busy), ctypes.byref(groups)))
         rms, rpairs = ctypes.c_double(), ctypes.c_uint64()
         check(_lib.load().kestrel_last_fit_recondition(self._h, ctypes.byref(rms), ctypes.byref(rpairs)))
+        form = ctypes.c_int()
+        check(_lib.load().kestrel_last_fit_form(self._h, ctypes.byref(form)))
         return {"sweep_ms": ms.value, "sweeps": sw.value, "wall_ms": wall.value, "busy_ms": busy.value,
-                "groups": groups.value, "recondition_ms": rms.value, "recondition_pairs": rpairs.value}
+                "groups": groups.value, "recondition_ms": rms.value, "recondition_pairs": rpairs.value,
+                "form": {0: "table", 1: "class", 2: "lanes"}[form.value]}
 
 
 def segment_bounds(n_kept: int, segment_sites: int = 4096, chunk: int = 32, pad: int = 256) -> np.ndarray:

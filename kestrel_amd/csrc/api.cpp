@@ -72,7 +72,8 @@ int repack_kept(kestrel_cohort* c, const uint8_t* dpacked, size_t row_stride_byt
 int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active,
                   uint32_t ns, double eps, double* partial, hipStream_t st, bool skip_masked = false,
                   bool lanes = false, ClassLists* cls = nullptr, bool build = true) {
-    if (cls) {  // the class-sorted sweep (c->class_form(); partials in c->nseg_cls segments)
+    size_t base;  // the tri/tetra-allelic parts' first partial segment
+    if (cls) {    // the class-sorted sweep (c->class_form(); partials in c->nseg_cls segments)
         // (without build the lists hold every slot: cls->cap >= ns)
         for (uint32_t s0 = 0; s0 < ns; s0 += cls->cap) {
             const uint32_t nc = std::min(cls->cap, ns - s0);
@@ -83,16 +84,17 @@ int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, con
                                         cls->nblk, pt, active, ns, s0, nc, eps, partial, st,
                                         (int)c->opt_class_waves));
         }
-        return KESTREL_OK;
-    }
-    if (lanes) {  // FP64 fits of 2-bit and biallelic likelihood-row cohorts (checked by the caller)
+        base = c->nseg_cls;
+    } else if (lanes) {  // FP64 fits of 2-bit and biallelic likelihood-row cohorts (checked by the caller)
         HIPCHK(klaunch::sweep_lanes(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, sx, sy, pt, active, ns, eps,
                                     partial, st, c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride));
         return KESTREL_OK;
+    } else {
+        HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, sx, sy, pt, active, ns, eps,
+                              partial, st, c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride, skip_masked,
+                              c->mixed));
+        base = c->nseg;
     }
-    HIPCHK(klaunch::sweep(c->geno.p, c->row_words, c->af.p, c->seg.p, c->nseg, sx, sy, pt, active, ns, eps, partial, st,
-                          c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride, skip_masked, c->mixed));
-    size_t base = c->nseg;
     for (const auto& p : c->parts) {
         HIPCHK(klaunch::sweep_multi(p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->table.p, p->af.p,
                                     p->seg.p, p->nseg, sx, sy, pt, active, ns, eps, partial + base * NPART * ns, st,
@@ -746,6 +748,12 @@ int kestrel_last_fit_recondition(const kestrel_cohort* c, double* ms, uint64_t* 
     return KESTREL_OK;
 }
 
+int kestrel_last_fit_form(const kestrel_cohort* c, int* form) {
+    if (!c || !form) return fail(KESTREL_EINVAL, "null argument");
+    *form = c->last_form;
+    return KESTREL_OK;
+}
+
 int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_t* pairs, size_t n_pairs,
                       uint32_t n_select, kestrel_output* out) {
     if (!c || !tune || !out) return fail(KESTREL_EINVAL, "null argument");
@@ -900,7 +908,7 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     HIPCHK(hipMemcpyAsync(dlive.p, &live0, sizeof(int), hipMemcpyHostToDevice, st));
     kdev::Tune dt = to_dev(*tune);
     if (c->mixed) dt.recond_margin = RECOND_MARGIN;
-    const uint32_t nseg_fit = use_class ? c->nseg_cls : c->nseg_total();
+    const uint32_t nseg_fit = use_class ? c->nseg_class_total() : c->nseg_total();
 
     // sweeps and steps are queued BATCH at a time between host synchronisations;
     // each step adds the slots still live after it to hist[i], so the host
@@ -1104,6 +1112,7 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
     if (cur_e > cur_s) busy += cur_e - cur_s;
     c->last_busy_ms = busy;
     c->last_groups = ngroups;
+    c->last_form = lanes ? KESTREL_FORM_LANES : use_class ? KESTREL_FORM_CLASS : KESTREL_FORM_TABLE;
     c->last_slots = ns;
     c->last_exhaust_ms = std::isfinite(exhaust_ms) ? exhaust_ms : end_ms;
     c->last_end_ms = end_ms;
@@ -1262,7 +1271,7 @@ int kestrel_evaluate(kestrel_cohort* c, const uint32_t* pairs, const double* poi
                     "not free (option class_sweep 0 selects the site-table sweep)",
                     (unsigned long long)(256 * ClassLists::slot_bytes(c->class_blocks())));
     const bool use_class = c->class_form();
-    const uint32_t nfold = std::max(c->nseg_total(), use_class ? c->nseg_cls : 0u);
+    const uint32_t nfold = std::max(c->nseg_total(), use_class ? c->nseg_class_total() : 0u);
     HIPCHK(dpart.alloc((size_t)nfold * NPART * ns));
     HIPCHK(df.alloc(n));
     HIPCHK(dg.alloc(9 * n));

@@ -213,6 +213,7 @@ struct kestrel_cohort {
     uint64_t last_sweeps = 0;
     int last_groups = 1;
     uint32_t last_slots = 0;                        // pairs in flight per group in the last fit
+    int last_form = KESTREL_FORM_TABLE;             // the last fit's sweep form
     double last_exhaust_ms = 0, last_end_ms = 0;    // device ms until the first slot went idle / the last sweep
     double last_recond_ms = 0;                      // mixed form: host ms of the FP64 condition pass
     uint64_t last_recond_pairs = 0;                 // mixed form: records whose condition it recomputed
@@ -226,6 +227,13 @@ struct kestrel_cohort {
 
     uint32_t nseg_total() const {
         uint32_t n = nseg;
+        for (const auto& p : parts) n += p->nseg;
+        return n;
+    }
+    // partial segments of the class-sorted form: its one segment of the
+    // 2-allele sites, then those of the tri/tetra-allelic parts
+    uint32_t nseg_class_total() const {
+        uint32_t n = nseg_cls;
         for (const auto& p : parts) n += p->nseg;
         return n;
     }
@@ -251,7 +259,9 @@ struct kestrel_cohort {
     ServeState* serve = nullptr;  // serving engine (kestrel_serve_start)
 
     // the class-sorted sweep covers FP64 evaluations of 2-bit cohorts
-    bool class_form() const { return opt_class_sweep == 1 && !soft && !mixed && !gt && parts.empty(); }
+    // FP64 fits of 2-bit rows (the 2-allele part of a tri/tetra-allelic
+    // cohort too, whose other parts keep their sweeps)
+    bool class_form() const { return opt_class_sweep == 1 && !soft && !mixed && !gt; }
     uint32_t class_blocks() const { return (v_pad + CLASS_BLOCK - 1) / CLASS_BLOCK; }
 
     ~kestrel_cohort() {

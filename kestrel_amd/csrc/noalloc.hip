@@ -105,8 +105,10 @@ extern "C" int kestrel_evaluate_noalloc(kestrel_cohort* c, const uint32_t* pairs
     if (c->gt_parts[0] || c->gt_parts[1])
         return fail(KESTREL_EINVAL, "a -f GT cohort with 3- or 4-allele sites has no likelihood evaluation");
     if (n == 0) return KESTREL_OK;
-    const uint32_t nseg = c->nseg_total();
     const bool cls = c->class_form();  // points inside the domain by the class-sorted sweep, as kestrel_evaluate
+    // partial segments: the exact kernel's layout (the cohort's segments) or
+    // the class-sorted form's (its one segment, then the parts'), whichever is longer
+    const uint32_t nseg = std::max(c->nseg_total(), cls ? c->nseg_class_total() : 0u);
     const uint32_t nblk = cls ? c->class_blocks() : 0;
     const uint64_t ns = ((uint64_t)n + SWEEP_TPB - 1) / SWEEP_TPB * SWEEP_TPB;
     const uint64_t cap = (SCRATCH_BYTES - SCRATCH_HEAD) / slot_bytes(nseg, nblk) / SWEEP_TPB * SWEEP_TPB;
@@ -155,7 +157,7 @@ extern "C" int kestrel_evaluate_noalloc(kestrel_cohort* c, const uint32_t* pairs
                               eps, part, st, c->soft ? c->glrows.p : nullptr, c->v_pad, c->gl_stride, false,
                               c->mixed));
     }
-    size_t off = c->nseg;
+    size_t off = cls ? c->nseg_cls : c->nseg;  // the tri/tetra-allelic parts' first partial segment
     for (const auto& p : c->parts) {
         HIPCHK(klaunch::sweep_multi(p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->table.p, p->af.p,
                                     p->seg.p, p->nseg, sx, sy, pt, act, (uint32_t)ns, eps, part + off * NPART * ns, st,

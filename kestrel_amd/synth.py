@@ -277,6 +277,22 @@ def multi_genotypes(n_samples: int, n_sites: tuple, seed: int, family_fraction: 
     return tuple(out)
 
 
+def multi_onehot(idx_buckets):
+    """One-hot likelihood rows of multi_genotypes' hard calls: per bucket a
+    (V_A, N, A(A+1)/2) array, site-major as vcf.rs builds them, a missing
+    call's row all NaN."""
+    out = []
+    for A, ix in zip((2, 3, 4), idx_buckets):
+        n = ix.shape[0]
+        rows = np.zeros((ix.shape[1], n, A * (A + 1) // 2))
+        called = ix.T != 255
+        s_i, n_i = np.nonzero(called)
+        rows[s_i, n_i, ix.T[called]] = 1.0
+        rows[~called] = np.nan
+        out.append(rows)
+    return tuple(out)
+
+
 def write_vcf_gt_multi(path: str, buckets, names=None, af=None) -> None:
     """A GT VCF of genotype-index buckets (idx2, idx3, idx4) as multi_genotypes
     makes them: records interleaved 2-, 3-, 4-allele site by site (each bucket

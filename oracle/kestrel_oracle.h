@@ -140,6 +140,17 @@ double ko_condition(int N, const double* x, const double* g, const double* h, do
  * order = KO_ORDER_SEG : rows V x 9 with valid / seg */
 void ko_fit_pair_rows(int order, const double* rows, size_t n_or_V, const uint8_t* valid, const uint32_t* seg,
                       size_t nseg, int self_pair, const ko_tune* t, ko_output* out);
+/* The class order with tri/tetra-allelic parts: the 2-allele part's sums in
+ * the class order (codes gx, gy and frequencies af2 of its V2 sites, one
+ * segment), then the parts' rows (V x 9 with valid) per segment of pseg,
+ * folded left to right after it. */
+void ko_fit_pair_class_parts(const uint8_t* gx, const uint8_t* gy, const double* af2, uint32_t V2,
+                             const double* prows, const uint8_t* pvalid, const uint32_t* pseg, size_t npseg,
+                             int self_pair, const ko_tune* t, ko_output* out);
+/* f, g, h (9 x 9) of one point in that order */
+void ko_class_parts_eval(const uint8_t* gx, const uint8_t* gy, const double* af2, uint32_t V2, const double* prows,
+                         const uint8_t* pvalid, const uint32_t* pseg, size_t npseg, const double x[9], double eps,
+                         double* f, double g[9], double h[81]);
 
 /* Build per-site c rows (V x 9) for a pair from 2-bit genotype codes (0,1,2 =
  * index of the unphased genotype, coefficients.rs:18-38; 3 = missing) using

@@ -904,12 +904,26 @@ typedef struct {
     size_t nseg;
     const uint8_t *gx, *gy; /* CLASS: the pair's codes and the frequencies */
     const double* af;
+    const double* prows;    /* CLASS with tri/tetra-allelic parts: their rows, */
+    const uint8_t* pvalid;  /* usable flags and segment bounds, after the */
+    const uint32_t* pseg;   /* 2-allele part's class-order sums */
+    size_t npseg;
 } pair_ctx;
 
 static void class_gh(const pair_ctx* c, const double* x, double eps, double* f, double* g, double* h) {
     double F, G[9], H[45];
     size_t n;
     ko_class_sums(c->gx, c->gy, c->af, c->seg, c->nseg, x, eps, &F, G, H, &n);
+    /* the parts' segment partials, folded after the 2-allele part's */
+    for (size_t k = 0; k < c->npseg; k++) {
+        double zf, zg[9], zh[45];
+        size_t zn;
+        seg_sums(c->prows, c->pvalid, c->pseg + k, 1, x, eps, 1, 1, &zf, zg, zh, &zn);
+        F = F + zf;
+        for (int i = 0; i < 9; i++) G[i] = G[i] + zg[i];
+        for (int i = 0; i < 45; i++) H[i] = H[i] + zh[i];
+        n += zn;
+    }
     if (f) *f = -F / (double)n;
     if (g) gh_from_sums(G, H, n, g, h);
 }
@@ -969,6 +983,45 @@ void ko_fit_pair_rows(int order, const double* rows, size_t n_or_V, const uint8_
     }
     fit_ctx(&ctx, self_pair, t, out);
     if (order == KO_ORDER_REF) free(ctx.lv.blocks);
+}
+
+static void class_parts_ctx(pair_ctx* ctx, const uint32_t* seg2, const uint8_t* gx, const uint8_t* gy,
+                            const double* af2, const double* prows, const uint8_t* pvalid, const uint32_t* pseg,
+                            size_t npseg) {
+    memset(ctx, 0, sizeof *ctx);
+    ctx->order = KO_ORDER_CLASS;
+    ctx->gx = gx;
+    ctx->gy = gy;
+    ctx->af = af2;
+    ctx->seg = seg2;
+    ctx->nseg = 1;
+    ctx->prows = prows;
+    ctx->pvalid = pvalid;
+    ctx->pseg = pseg;
+    ctx->npseg = npseg;
+}
+
+void ko_class_parts_eval(const uint8_t* gx, const uint8_t* gy, const double* af2, uint32_t V2, const double* prows,
+                         const uint8_t* pvalid, const uint32_t* pseg, size_t npseg, const double x[9], double eps,
+                         double* f, double g[9], double h[81]) {
+    pair_ctx ctx;
+    const uint32_t seg2[2] = {0, V2};
+    class_parts_ctx(&ctx, seg2, gx, gy, af2, prows, pvalid, pseg, npseg);
+    class_gh(&ctx, x, eps, f, g, h);
+}
+
+void ko_fit_pair_class_parts(const uint8_t* gx, const uint8_t* gy, const double* af2, uint32_t V2,
+                             const double* prows, const uint8_t* pvalid, const uint32_t* pseg, size_t npseg,
+                             int self_pair, const ko_tune* t, ko_output* out) {
+    pair_ctx ctx;
+    const uint32_t seg2[2] = {0, V2};
+    class_parts_ctx(&ctx, seg2, gx, gy, af2, prows, pvalid, pseg, npseg);
+    uint32_t n = 0;
+    for (uint32_t s = 0; s < V2; s++) n += gx[s] < 3 && gy[s] < 3;
+    for (size_t k = 0; k < npseg; k++)
+        for (uint32_t s = pseg[k]; s < pseg[k + 1]; s++) n += pvalid[s] != 0;
+    out->n_sites = n;
+    fit_ctx(&ctx, self_pair, t, out);
 }
 
 static const double ONEHOT[4][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {NAN, NAN, NAN}};

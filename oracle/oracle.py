@@ -115,6 +115,10 @@ def lib():
         L.ko_allele_freqA_rows.argtypes = [I, P, SZ, U64, D, P, P]
         L.ko_pair_rows_generic.restype = SZ
         L.ko_pair_rows_generic.argtypes = [I, P, P, P, SZ, P, P]
+        L.ko_fit_pair_class_parts.restype = None
+        L.ko_fit_pair_class_parts.argtypes = [P, P, P, U32, P, P, P, SZ, I, P, P]
+        L.ko_class_parts_eval.restype = None
+        L.ko_class_parts_eval.argtypes = [P, P, P, U32, P, P, P, SZ, P, D, P, P, P]
         L.ko_fit_pairs_multi.restype = I
         L.ko_fit_pairs_multi.argtypes = [U32, P, P, U32, P, P, U32, P, P, U32, P, SZ, U64, I, P]
         _lib = L
@@ -518,6 +522,55 @@ def multi_fit_pairs(parts, pairs, order=ORDER_REF, pads=None, seg=None, iteratio
         else:
             o = fit_rows(rows, self_pair=x == y, order=ORDER_SEG, valid=valid, seg=seg, iterations=iterations)
         out[i] = o
+        out[i]["x"], out[i]["y"] = x, y
+    return out
+
+
+def _class_parts_inputs(parts, pads, seg):
+    af2, gl2 = parts[2]
+    called = ~np.isnan(gl2).any(2)
+    one_hot = (np.nan_to_num(gl2) == 1.0).sum(2) == 1
+    if not (one_hot | ~called).all():
+        raise ValueError("the class form takes 2-allele rows that are one-hot where called")
+    codes = np.where(called, np.argmax(np.nan_to_num(gl2), 2), 3).astype(np.uint8)
+    p2 = int(pads[2])
+    seg = np.asarray(seg, np.int64)
+    i0 = int(np.nonzero(seg == p2)[0][0])
+    return codes, f64(af2), p2, _seg(seg[i0:] - p2)
+
+
+def multi_class_eval(parts, x, y, pads, seg, pt, eps=1e-8):
+    """f, g, h of pair (x, y) at pt in multi_class_fit_pairs' order."""
+    codes, af2, p2, pseg = _class_parts_inputs(parts, pads, seg)
+    rows, valid = multi_pair_rows(parts, int(x), int(y), ORDER_SEG, pads)
+    prows, pvalid = f64(rows[p2:]), np.ascontiguousarray(valid[p2:], np.uint8)
+    gx, gy = np.ascontiguousarray(codes[int(x)]), np.ascontiguousarray(codes[int(y)])
+    f, g, h = np.zeros(1), np.zeros(9), np.zeros(81)
+    lib().ko_class_parts_eval(_p(gx), _p(gy), _p(af2), af2.shape[0], _p(prows), _p(pvalid), _p(pseg), len(pseg) - 1,
+                              _p(f64(pt)), eps, _p(f), _p(g), _p(h))
+    return float(f[0]), g, h.reshape(9, 9)
+
+
+def multi_class_fit_pairs(parts, pairs, pads, seg, iterations=100):
+    """Pair fits over re-bucketed multi-allelic sites of hard calls (one-hot
+    rows) in the device's class form: the 2-allele part as codes (the one-hot
+    row's genotype index, 3 = missing) in the class order (ORDER_CLASS, one
+    segment), then the 3- and 4-allele parts' segments as ORDER_SEG takes them
+    (pads and seg of the whole SEG layout; the 2-allele part's padded rows
+    come first), folded left to right after it."""
+    codes, af2, p2, pseg = _class_parts_inputs(parts, pads, seg)
+    t = tune(iterations, 10)
+    out = np.zeros(len(pairs), OUTPUT_DTYPE)
+    for i, (x, y) in enumerate(pairs):
+        rows, valid = multi_pair_rows(parts, int(x), int(y), ORDER_SEG, pads)
+        prows, pvalid = f64(rows[p2:]), np.ascontiguousarray(valid[p2:], np.uint8)
+        gx, gy = np.ascontiguousarray(codes[int(x)]), np.ascontiguousarray(codes[int(y)])
+        o = np.zeros(1, OUTPUT_DTYPE)
+        lib().ko_fit_pair_class_parts(_p(gx), _p(gy), _p(af2), af2.shape[0], _p(prows), _p(pvalid), _p(pseg),
+                                      len(pseg) - 1, int(x == y), ctypes.byref(t), _p(o))
+        if o[0]["n_sites"] == 0:
+            o[0]["jacquard"], o[0]["obj"], o[0]["cond"] = np.nan, np.nan, np.nan
+        out[i] = o[0]
         out[i]["x"], out[i]["y"] = x, y
     return out
 

@@ -188,3 +188,43 @@ def test_multi_mixed_precision_form(O):
     assert d.max() <= 1e-6 and k.max() <= 1e-6 and r.max() <= 1e-6
     c = both & (lb < 2.0)
     assert np.abs(la - lb)[c].max() <= 0.1
+
+
+def test_multi_onehot_class_form(O):
+    """Hard calls at 2-, 3- and 4-allele sites as one-hot rows (minimum allele
+    frequency 0, so the re-bucketed 2-allele rows stay one-hot): the 2-allele
+    part is held as 2-bit codes and swept in the class-sorted order, the 3- and
+    4-allele parts in their segments after it.  Records and evaluations
+    bit-exact against the oracle's class order with parts; the site-table
+    form (option "class_sweep" 0) against the segmented order; both within
+    the stated tolerance of the reference's order."""
+    say("case: one-hot multi-allelic cohort in the class form, 48 x (1500, 300, 120)")
+    b2, b3, b4 = synth.multi_onehot(synth.multi_genotypes(48, (1500, 300, 120), seed=71, missing=0.02))
+    parts, _ = O.multi_prepare({2: b2, 3: b3, 4: b4}, maf=0.0)
+    rng = np.random.default_rng(4)
+    with Cohort.gl_multi(b2, b3, b4, maf=0.0) as co:
+        out = co.fit_pairs()
+        assert co.last_fit_stats()["form"] == "class"
+        pairs = np.stack([out["x"], out["y"]], 1)
+        pads, seg = device_order(parts)
+        sel = np.unique(np.concatenate([rng.choice(len(out), 240, replace=False),
+                                        np.nonzero(out["x"] == out["y"])[0][:8]]))
+        assert_bitexact(out[sel], O.multi_class_fit_pairs(parts, pairs[sel], pads, seg))
+        ref = O.multi_fit_pairs(parts, pairs[sel], order=O.ORDER_REF)
+        lc = np.log10(ref["cond"])
+        well = np.isfinite(lc) & (lc < 12)
+        assert well.sum() > 150
+        assert np.abs(out["jacquard"][sel] - ref["jacquard"]).max(1)[well].max() <= 1e-6
+        assert (np.abs(out["obj"][sel] - ref["obj"]) / np.abs(ref["obj"]))[well].max() <= 1e-9
+        # evaluations in the same order
+        ep = rng.integers(0, 48, (24, 2))
+        pts = rng.dirichlet(np.ones(9), 24)
+        f, g, h, ns = co.evaluate(ep, pts)
+        for i in range(24):
+            fe, ge, he = O.multi_class_eval(parts, ep[i, 0], ep[i, 1], pads, seg, pts[i])
+            assert f[i] == fe and np.array_equal(g[i], ge) and np.array_equal(h[i], he), i
+        # the site-table form of the same cohort
+        co.set_option("class_sweep", 0)
+        t = co.fit_pairs()
+        assert co.last_fit_stats()["form"] == "table"
+        assert_bitexact(t[sel], O.multi_fit_pairs(parts, pairs[sel], order=O.ORDER_SEG, pads=pads, seg=seg))

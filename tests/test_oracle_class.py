@@ -83,3 +83,63 @@ def test_class_fits_against_reference_order(O):
     assert np.abs(a["jacquard"] - b["jacquard"]).max(1)[well].max() <= 1e-6
     assert (np.abs(a["obj"] - b["obj"]) / np.abs(b["obj"]))[well].max() <= 1e-9
     assert np.array_equal(a["n_sites"], b["n_sites"])
+
+
+def _onehot_parts(O, n, sizes, seed):
+    from kestrel_amd import synth
+    b2, b3, b4 = synth.multi_onehot(synth.multi_genotypes(n, sizes, seed=seed, missing=0.02))
+    # (minimum allele frequency 0: only alleles no sample carries are removed,
+    # so the re-bucketed 2-allele rows stay one-hot)
+    parts, _ = O.multi_prepare({2: b2, 3: b3, 4: b4}, maf=0.0)
+    return parts
+
+
+def _layout(parts, per=96):
+    """A SEG layout: each part padded to 256 sites, cut every `per` sites."""
+    pads, seg, base = {}, [0], 0
+    for A in (2, 3, 4):
+        v = parts[A][0].shape[0]
+        if A > 2 and v == 0:
+            continue
+        pad = max(256, (v + 255) // 256 * 256)
+        pads[A] = pad
+        seg += [base + b for b in range(per, pad, per)] + [base + pad]
+        base += pad
+    return pads, np.array(seg, np.uint32)
+
+
+def test_class_parts_order_pins(O):
+    """The class order with tri/tetra-allelic parts (the device's class form
+    of one-hot multi-allelic cohorts): without 3- and 4-allele sites it is the
+    plain class order bit for bit; without 2-allele sites it is the segmented
+    order of the parts bit for bit (the class sums are zero and fold first);
+    with both, within the stated tolerance of the reference's order."""
+    pairs = np.array([(x, y) for x in range(14) for y in range(x, 14)], np.uint32)
+    parts = _onehot_parts(O, 14, (900, 0, 0), 61)
+    assert parts[3][0].shape[0] == 0 and parts[4][0].shape[0] == 0
+    pads, seg = _layout(parts)
+    a = O.multi_class_fit_pairs(parts, pairs, pads, seg)
+    af2, gl2 = parts[2]
+    codes = np.where(np.isnan(gl2).any(2), 3, np.argmax(np.nan_to_num(gl2), 2)).astype(np.uint8)
+    b = O.fit_pairs(codes, af2, pairs=pairs, order=O.ORDER_CLASS)
+    for k in ("jacquard", "obj", "iters", "n_sites", "cond"):
+        assert np.array_equal(a[k], b[k], equal_nan=True), k
+    parts = _onehot_parts(O, 14, (0, 300, 140), 62)
+    assert parts[2][0].shape[0] < 300  # (3-allele sites with an absent allele move down)
+    parts[2] = (np.zeros((0, 2)), np.zeros((14, 0, 3)))  # no 2-allele sites
+    pads, seg = _layout(parts)
+    a = O.multi_class_fit_pairs(parts, pairs, pads, seg)
+    b = O.multi_fit_pairs(parts, pairs, order=O.ORDER_SEG, pads=pads, seg=seg)
+    for k in ("jacquard", "obj", "iters", "n_sites", "cond"):
+        assert np.array_equal(a[k], b[k], equal_nan=True), k
+    parts = _onehot_parts(O, 20, (1500, 300, 120), 63)
+    pairs = np.array([(x, y) for x in range(20) for y in range(x, 20)], np.uint32)
+    pads, seg = _layout(parts)
+    a = O.multi_class_fit_pairs(parts, pairs, pads, seg)
+    r = O.multi_fit_pairs(parts, pairs, order=O.ORDER_REF)
+    lc = np.log10(r["cond"])
+    well = np.isfinite(lc) & (lc < 12)
+    assert well.sum() > 150
+    assert np.array_equal(a["n_sites"], r["n_sites"])
+    assert np.abs(a["jacquard"] - r["jacquard"]).max(1)[well].max() <= 1e-6
+    assert (np.abs(a["obj"] - r["obj"]) / np.abs(r["obj"]))[well].max() <= 1e-9

@@ -114,10 +114,14 @@ def test_noalloc_matches_evaluate(precision):
         check_same(got, co.evaluate(pairs, pts))
 
 
-def test_noalloc_multi_allelic_cohort():
-    say("case: kestrel_evaluate_noalloc on a 2/3/4-allele likelihood cohort")
-    gl = synth.multi_likelihoods(40, (3000, 600, 200), seed=12)
-    with Cohort.gl_multi(*gl) as co:
+@pytest.mark.parametrize("rows", ["soft", "one-hot"])
+def test_noalloc_multi_allelic_cohort(rows):
+    say(f"case: kestrel_evaluate_noalloc on a 2/3/4-allele likelihood cohort, {rows} rows")
+    if rows == "soft":
+        gl, maf = synth.multi_likelihoods(40, (3000, 600, 200), seed=12), None
+    else:  # hard calls: the 2-allele part in the class-sorted form
+        gl, maf = synth.multi_onehot(synth.multi_genotypes(40, (3000, 600, 200), seed=12, missing=0.02)), 0.0
+    with Cohort.gl_multi(*gl, maf=maf) as co:
         rng = np.random.default_rng(8)
         for n in (300, 17):
             pairs = rng.integers(0, 40, (n, 2))
