@@ -98,7 +98,7 @@ int launch_sweeps(kestrel_cohort* c, const uint32_t* sx, const uint32_t* sy, con
     for (const auto& p : c->parts) {
         HIPCHK(klaunch::sweep_multi(p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->table.p, p->af.p,
                                     p->seg.p, p->nseg, sx, sy, pt, active, ns, eps, partial + base * NPART * ns, st,
-                                    c->mixed ? p->table32.p : nullptr));
+                                    c->mixed ? p->table32.p : nullptr, p->codes.n ? p->codes.p : nullptr, p->lut.p));
         base += p->nseg;
     }
     return KESTREL_OK;
@@ -251,7 +251,7 @@ int recondition_records(kestrel_cohort* c, const uint32_t* geno, uint32_t row_wo
         for (const auto& p : c->parts) {  // FP64 sweeps of the tri/tetra-allelic parts
             HIPCHK(klaunch::sweep_multi(p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->table.p,
                                         p->af.p, p->seg.p, p->nseg, dsx.p, dsy.p, dpt.p, dact.p, ns, tune.epsilon,
-                                        dpart.p + off * NPART * ns, st));
+                                        dpart.p + off * NPART * ns, st, nullptr, p->codes.n ? p->codes.p : nullptr, p->lut.p));
             off += p->nseg;
         }
         HIPCHK(klaunch::recondition(dpart.p, nall, ns, (uint32_t)m, dpt.p, tune.sqp_conv_tol, dcond.p, st));
@@ -617,7 +617,25 @@ int kestrel_cohort_create_gl_multi(kestrel_cohort** out, uint32_t n_samples, con
         HIPCHK(klaunch::iis_table(p->alleles, p->af.p, p->v_pad, p->table.p, c->stream));
         HIPCHK(p->table32.alloc(p->table.n));
         HIPCHK(klaunch::round_f32(p->table.p, p->table32.p, p->table.n, c->stream));
+        // hard calls fed as likelihoods: the FP64 sweeps read each genotype
+        // pair's coefficients by the 4-bit indices (the rows stay for the
+        // mixed form and evaluations outside the sweeps' domain)
+        DevBuf<int> bad;
+        HIPCHK(bad.alloc(1));
+        HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(int), c->stream));
+        HIPCHK(p->codes.alloc((size_t)c->n_samples * (p->v_pad / 8)));
+        HIPCHK(klaunch::part_codes(p->alleles, p->rows.p, p->mask.p, p->row_words, p->v_pad, c->n_samples,
+                                   p->codes.p, bad.p, c->stream));
+        int hbad = 0;
+        HIPCHK(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+        if (hbad) {
+            p->codes.release();
+        } else {
+            HIPCHK(p->lut.alloc((size_t)p->v_pad * klaunch::part_lut_size(p->alleles)));
+            HIPCHK(klaunch::part_lut(p->alleles, p->table.p, p->v_pad, p->lut.p, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
         c->parts.push_back(std::move(p));
     }
     rc = c->set_segments();

@@ -84,6 +84,8 @@ struct SitePart {
     DevBuf<double> table;   // v_pad x the mode coefficients of every genotype pair (A >= 3)
     DevBuf<float> table32;  // the same rounded to FP32 (precision KESTREL_MIXED)
     DevBuf<uint32_t> seg;
+    DevBuf<uint32_t> codes;  // hard calls (every called row one-hot): 4-bit genotype indices, else empty
+    DevBuf<double> lut;      // with codes: each site's c_s rows of every genotype pair (part_lut)
 };
 // The 3- or 4-allele sites of a -f GT cohort (kestrel_cohort_create_gt_multi):
 // genotype indices in 4-bit rows (8 sites per word, sample-major, padding 15),

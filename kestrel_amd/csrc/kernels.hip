@@ -1680,6 +1680,195 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm(const uint32_t* __rest
     }
 }
 
+// A tri/tetra-allelic part whose every called row is one-hot (hard calls
+// fed as likelihoods: one exact 1.0, the other entries 0) is also held as
+// 4-bit genotype indices (8 sites a word, sample-major; 15 = missing): thread
+// (sample, word) writes one word and sets *bad when a called row is not
+// one-hot.
+template <int A>
+__global__ __launch_bounds__(256) void part_codes_kernel(const double* __restrict__ rows,
+                                                         const uint32_t* __restrict__ mask, uint32_t row_words,
+                                                         uint32_t v_pad, uint32_t n_samples,
+                                                         uint32_t* __restrict__ codes, int* __restrict__ bad) {
+    constexpr int G = A * (A + 1) / 2;
+    const uint32_t words = v_pad / 8;
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (uint64_t)n_samples * words) return;
+    const uint32_t smp = (uint32_t)(e / words), w = (uint32_t)(e % words);
+    uint32_t out = 0;
+    bool ok = true;
+    for (int i = 0; i < 8; i++) {
+        const uint32_t t = w * 8 + i;
+        const uint32_t m = (mask[(size_t)smp * row_words + t / 16] >> (2 * (t % 16))) & 3u;
+        uint32_t code = 15u;
+        if (m != 3u) {
+            const double* r = rows + ((size_t)smp * v_pad + t) * G;
+            int ones = 0, zeros = 0, at = 0;
+            for (int g = 0; g < G; g++) {
+                ones += r[g] == 1.0;
+                zeros += r[g] == 0.0;
+                if (r[g] == 1.0) at = g;
+            }
+            ok = ok && ones == 1 && zeros == G - 1;
+            code = (uint32_t)at;
+        }
+        out |= code << (4 * i);
+    }
+    codes[(size_t)smp * words + w] = out;
+    if (!ok) atomicOr(bad, 1);
+}
+
+// A hard-call part's rows of c_s for every genotype pair: c_s of a one-hot
+// row pair is its genotype pair's coefficients alone -- in mixture.rs:8-60
+// every other pair adds g = 0 times its coefficients, exact zeros, and the
+// pair's own g is 1.0 -- and each coefficient of a mode goes to a different
+// component, so c_s[iis_target(mode, e)] = fma(1, T_e, 0).  Per site G^2 rows
+// in genotype order, then a zero row for a missing call; GLC_RW doubles a row
+// (9 used, 16-byte aligned).  Thread (site, row).
+constexpr int GLC_RW = 10;
+template <int A>
+__global__ __launch_bounds__(256) void part_lut_kernel(const double* __restrict__ table, uint32_t v_pad,
+                                                       double* __restrict__ lut) {
+    constexpr IisTable<A> tab;
+    constexpr int G = A * (A + 1) / 2, NC = tab.off[G * G], R = G * G + 1;
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (uint64_t)v_pad * R) return;
+    const uint64_t site = e / R;
+    const int c = (int)(e % R);
+    double cv[GLC_RW];
+#pragma unroll
+    for (int q = 0; q < GLC_RW; q++) cv[q] = 0.0;
+    if (c < G * G) {
+        const int mode = tab.mode[c];
+        const double* T = table + site * NC + tab.off[c];
+        for (int q = 0; q < iis_nnz(mode); q++) cv[iis_target(mode, q)] = fma(1.0, T[q], 0.0);
+    }
+    double* dst = lut + e * GLC_RW;
+#pragma unroll
+    for (int q = 0; q < GLC_RW; q++) dst[q] = cv[q];
+}
+
+// sweep_glm for a part held as 4-bit genotype indices (part_codes_kernel):
+// a lane reads its genotype pair's row of c_s (part_lut_kernel) instead of
+// 2 G likelihoods and the mixture; the site update and the summation order
+// are sweep_glm's, so the records are the same bit for bit.  The block stages
+// the rows of SUB sites at a time in LDS, the next SUB sites' rows loaded into
+// registers while these sites are computed (two LDS buffers).
+template <int A>
+__global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm_codes(
+    const uint32_t* __restrict__ mask, uint32_t row_words, const uint32_t* __restrict__ codes,
+    const double* __restrict__ lutg, const uint32_t* __restrict__ seg_bounds, const uint32_t* __restrict__ sx,
+    const uint32_t* __restrict__ sy, const double* __restrict__ pt, const int* __restrict__ active,
+    uint32_t nslots, uint32_t v_pad, double eps, double* __restrict__ partial) {
+    constexpr int G = A * (A + 1) / 2, R = G * G + 1;  // rows a site
+    constexpr int SUB = A == 3 ? 8 : 4;                // sites a stage
+    constexpr int N2 = SUB * R * GLC_RW / 2;           // double2 a stage
+    constexpr int PER = (N2 + SWEEP_TPB - 1) / SWEEP_TPB;
+    static_assert(CHUNK % SUB == 0 && (SUB * R * GLC_RW) % 2 == 0, "stages inside a chunk, whole double2");
+    __shared__ __attribute__((aligned(16))) double2 lut[2][N2];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t slot = blockIdx.x * SWEEP_TPB + tid;
+    const uint32_t seg = blockIdx.y;
+    const bool live = slot < nslots && active[slot] != 0;
+    if (!__syncthreads_or(live)) return;
+
+    double D[9];
+    uint32_t x = 0, y = 0;
+    if (live) {
+        x = sx[slot];
+        y = sy[slot];
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k] = pt[(size_t)k * nslots + slot];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k] = 0.0;
+    }
+    const uint32_t* rowx = mask + (size_t)x * row_words;
+    const uint32_t* rowy = mask + (size_t)y * row_words;
+    const uint32_t* cx = codes + (size_t)x * (v_pad / 8);
+    const uint32_t* cy = codes + (size_t)y * (v_pad / 8);
+
+    double F = 0.0, N = 0.0, P = 1.0, Gr[9], H[45];
+#pragma unroll
+    for (int k = 0; k < 9; k++) Gr[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 45; k++) H[k] = 0.0;
+
+    // segment bounds are whole chunks (so whole stages): t - s0 counts chunks from 0
+    const uint32_t s0 = seg_bounds[seg], s1 = seg_bounds[seg + 1];
+    double2 nx[PER];
+    auto fetch = [&](uint32_t sb) {  // the rows of sites sb .. sb + SUB - 1
+        const double2* src = reinterpret_cast<const double2*>(lutg + (size_t)sb * R * GLC_RW);
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const uint32_t e = tid + (uint32_t)i * SWEEP_TPB;
+            if (e < (uint32_t)N2) nx[i] = src[e];
+        }
+    };
+    auto store = [&](int b) {
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const uint32_t e = tid + (uint32_t)i * SWEEP_TPB;
+            if (e < (uint32_t)N2) lut[b][e] = nx[i];
+        }
+    };
+    if (s0 < s1) {
+        fetch(s0);
+        store(0);
+    }
+    __syncthreads();
+    uint2 wx = make_uint2(0, 0), wy = make_uint2(0, 0);
+    uint32_t qx = 0, qy = 0;
+    int b = 0;
+#pragma unroll 1
+    for (uint32_t sb = s0; sb < s1; sb += SUB) {
+        if (sb + SUB < s1) fetch(sb + SUB);
+        const double* lb = reinterpret_cast<const double*>(lut[b]);
+#pragma unroll 1
+        for (uint32_t t = sb; t < sb + SUB; t++) {
+            const int k = (int)((t - s0) & (CHUNK - 1));
+            if (k == 0) {
+                wx = *reinterpret_cast<const uint2*>(rowx + t / 16);
+                wy = *reinterpret_cast<const uint2*>(rowy + t / 16);
+                const uint32_t ux = ~((wx.x & (wx.x >> 1)) | (wy.x & (wy.x >> 1))) & 0x55555555u;
+                const uint32_t uy = ~((wx.y & (wx.y >> 1)) | (wy.y & (wy.y >> 1))) & 0x55555555u;
+                N += (double)(__popc(ux) + __popc(uy));
+            }
+            if (k % 8 == 0) {
+                qx = cx[t / 8];
+                qy = cy[t / 8];
+            }
+            const uint32_t ax = (k < 16 ? wx.x : wx.y) >> (2 * (k & 15));
+            const uint32_t ay = (k < 16 ? wy.x : wy.y) >> (2 * (k & 15));
+            const uint32_t gx = (qx >> (4 * (k & 7))) & 15u, gy = (qy >> (4 * (k & 7))) & 15u;
+            const bool missing = (ax & 3u) == 3u || (ay & 3u) == 3u || gx >= (uint32_t)G || gy >= (uint32_t)G;
+            const uint32_t row = missing ? (uint32_t)(G * G) : gx * G + gy;
+            const double2* src = reinterpret_cast<const double2*>(lb + ((size_t)(t - sb) * R + row) * GLC_RW);
+            double cv[9];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const double2 u = src[q];
+                cv[2 * q] = u.x;
+                cv[2 * q + 1] = u.y;
+            }
+            cv[8] = src[4].x;
+            site_update(cv, missing ? 1.0 : eps, D, k, P, F, Gr, H);
+        }
+        if (sb + SUB < s1) store(b ^ 1);  // its readers finished at the previous barrier
+        __syncthreads();
+        b ^= 1;
+    }
+    if (live) {
+        double* out = partial + (size_t)seg * NPART * nslots + slot;
+        out[0] = F;
+#pragma unroll
+        for (int k = 0; k < 9; k++) out[(size_t)(1 + k) * nslots] = Gr[k];
+#pragma unroll
+        for (int k = 0; k < 45; k++) out[(size_t)(10 + k) * nslots] = H[k];
+        out[(size_t)55 * nslots] = N;
+    }
+}
+
 // The mixed-precision form of sweep_glm (precision KESTREL_MIXED): the
 // coefficient table rounded to FP32 (half the LDS bytes a site: 528 for A =
 // 3, 1,200 for A = 4), the likelihoods rounded to FP32 and the generic
@@ -3042,9 +3231,16 @@ hipError_t af_gl(const double* gl, uint32_t n_samples, uint32_t n_sites, const T
 hipError_t sweep_multi(int alleles, const uint32_t* geno, uint32_t row_words, const double* gl, uint32_t v_pad,
                        const double* table, const double* af, const uint32_t* seg_bounds, uint32_t nseg,
                        const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active, uint32_t nslots,
-                       double eps, double* partial, hipStream_t st, const float* table32) {
+                       double eps, double* partial, hipStream_t st, const float* table32, const uint32_t* codes,
+                       const double* lut) {
     dim3 grid((nslots + SWEEP_TPB - 1) / SWEEP_TPB, nseg);
-    if (table32 && alleles == 3)
+    if (codes && !table32 && alleles == 3)  // hard calls: the genotype pairs' rows (the same bits)
+        hipLaunchKernelGGL(sweep_glm_codes<3>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, codes, lut,
+                           seg_bounds, sx, sy, pt, active, nslots, v_pad, eps, partial);
+    else if (codes && !table32 && alleles == 4)
+        hipLaunchKernelGGL(sweep_glm_codes<4>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, codes, lut,
+                           seg_bounds, sx, sy, pt, active, nslots, v_pad, eps, partial);
+    else if (table32 && alleles == 3)
         hipLaunchKernelGGL(sweep_glm_mixed<3>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, table32,
                            seg_bounds, sx, sy, pt, active, nslots, eps, partial);
     else if (table32 && alleles == 4)
@@ -3092,6 +3288,38 @@ hipError_t sweep_exact(int kind, const uint32_t* geno, uint32_t row_words, const
     default:
         return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t part_codes(int alleles, const double* rows, const uint32_t* mask, uint32_t row_words, uint32_t v_pad,
+                      uint32_t n_samples, uint32_t* codes, int* bad, hipStream_t st) {
+    const uint64_t n = (uint64_t)n_samples * (v_pad / 8);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (alleles == 3)
+        hipLaunchKernelGGL(part_codes_kernel<3>, grid, dim3(256), 0, st, rows, mask, row_words, v_pad, n_samples, codes,
+                           bad);
+    else if (alleles == 4)
+        hipLaunchKernelGGL(part_codes_kernel<4>, grid, dim3(256), 0, st, rows, mask, row_words, v_pad, n_samples, codes,
+                           bad);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+size_t part_lut_size(int alleles) {
+    const size_t G = (size_t)alleles * (alleles + 1) / 2;
+    return (G * G + 1) * GLC_RW;
+}
+
+hipError_t part_lut(int alleles, const double* table, uint32_t v_pad, double* lut, hipStream_t st) {
+    const uint64_t n = (uint64_t)v_pad * (part_lut_size(alleles) / GLC_RW);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (alleles == 3)
+        hipLaunchKernelGGL(part_lut_kernel<3>, grid, dim3(256), 0, st, table, v_pad, lut);
+    else if (alleles == 4)
+        hipLaunchKernelGGL(part_lut_kernel<4>, grid, dim3(256), 0, st, table, v_pad, lut);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -78,7 +78,16 @@ hipError_t gather_gl(const double* gl_site, uint32_t n_samples, const uint32_t* 
 hipError_t sweep_multi(int alleles, const uint32_t* geno, uint32_t row_words, const double* gl, uint32_t v_pad,
                        const double* table, const double* af, const uint32_t* seg_bounds, uint32_t nseg,
                        const uint32_t* sx, const uint32_t* sy, const double* pt, const int* active, uint32_t nslots,
-                       double eps, double* partial, hipStream_t st, const float* table32 = nullptr);
+                       double eps, double* partial, hipStream_t st, const float* table32 = nullptr,
+                       const uint32_t* codes = nullptr, const double* lut = nullptr);
+// a hard-call part's rows of c_s per site and genotype pair (sweep_multi's
+// lut with codes): part_lut_size doubles a site
+size_t part_lut_size(int alleles);
+hipError_t part_lut(int alleles, const double* table, uint32_t v_pad, double* lut, hipStream_t st);
+// a part's 4-bit genotype indices (v_pad / 8 words a sample; 15 = missing)
+// from its likelihood rows; *bad set when a called row is not one-hot
+hipError_t part_codes(int alleles, const double* rows, const uint32_t* mask, uint32_t row_words, uint32_t v_pad,
+                      uint32_t n_samples, uint32_t* codes, int* bad, hipStream_t st);
 // dst[i] = (float)src[i] (the mixed form's FP32 coefficient tables; table32
 // above selects that form of sweep_multi)
 hipError_t round_f32(const double* src, float* dst, uint64_t n, hipStream_t st);

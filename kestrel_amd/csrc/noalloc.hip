@@ -161,7 +161,7 @@ extern "C" int kestrel_evaluate_noalloc(kestrel_cohort* c, const uint32_t* pairs
     for (const auto& p : c->parts) {
         HIPCHK(klaunch::sweep_multi(p->alleles, p->mask.p, p->row_words, p->rows.p, p->v_pad, p->table.p, p->af.p,
                                     p->seg.p, p->nseg, sx, sy, pt, act, (uint32_t)ns, eps, part + off * NPART * ns, st,
-                                    c->mixed ? p->table32.p : nullptr));
+                                    c->mixed ? p->table32.p : nullptr, p->codes.n ? p->codes.p : nullptr, p->lut.p));
         off += p->nseg;
     }
     // points outside the sweeps' domain (the kernel leaves every other slot)
