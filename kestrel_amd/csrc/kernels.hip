@@ -1817,8 +1817,16 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm_codes(
         store(0);
     }
     __syncthreads();
-    uint2 wx = make_uint2(0, 0), wy = make_uint2(0, 0);
-    uint32_t qx = 0, qy = 0;
+    // the mask words of a chunk and the code words of 8 sites, each loaded
+    // one chunk / one word ahead of its use (n*: the next)
+    uint2 wx = make_uint2(0, 0), wy = make_uint2(0, 0), nwx = wx, nwy = wx;
+    uint32_t qx = 0, qy = 0, nqx = 0, nqy = 0;
+    if (s0 < s1) {
+        nwx = *reinterpret_cast<const uint2*>(rowx + s0 / 16);
+        nwy = *reinterpret_cast<const uint2*>(rowy + s0 / 16);
+        nqx = cx[s0 / 8];
+        nqy = cy[s0 / 8];
+    }
     int b = 0;
 #pragma unroll 1
     for (uint32_t sb = s0; sb < s1; sb += SUB) {
@@ -1828,15 +1836,23 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm_codes(
         for (uint32_t t = sb; t < sb + SUB; t++) {
             const int k = (int)((t - s0) & (CHUNK - 1));
             if (k == 0) {
-                wx = *reinterpret_cast<const uint2*>(rowx + t / 16);
-                wy = *reinterpret_cast<const uint2*>(rowy + t / 16);
+                wx = nwx;
+                wy = nwy;
+                if (t + CHUNK < s1) {
+                    nwx = *reinterpret_cast<const uint2*>(rowx + (t + CHUNK) / 16);
+                    nwy = *reinterpret_cast<const uint2*>(rowy + (t + CHUNK) / 16);
+                }
                 const uint32_t ux = ~((wx.x & (wx.x >> 1)) | (wy.x & (wy.x >> 1))) & 0x55555555u;
                 const uint32_t uy = ~((wx.y & (wx.y >> 1)) | (wy.y & (wy.y >> 1))) & 0x55555555u;
                 N += (double)(__popc(ux) + __popc(uy));
             }
             if (k % 8 == 0) {
-                qx = cx[t / 8];
-                qy = cy[t / 8];
+                qx = nqx;
+                qy = nqy;
+                if (t + 8 < s1) {
+                    nqx = cx[(t + 8) / 8];
+                    nqy = cy[(t + 8) / 8];
+                }
             }
             const uint32_t ax = (k < 16 ? wx.x : wx.y) >> (2 * (k & 15));
             const uint32_t ay = (k < 16 ? wy.x : wy.y) >> (2 * (k & 15));
