@@ -1753,9 +1753,11 @@ __global__ __launch_bounds__(256) void part_lut_kernel(const double* __restrict_
 // 2 G likelihoods and the mixture; the site update and the summation order
 // are sweep_glm's, so the records are the same bit for bit.  The block stages
 // the rows of SUB sites at a time in LDS, the next SUB sites' rows loaded into
-// registers while these sites are computed (two LDS buffers).
+// registers while these sites are computed (two LDS buffers); GLC_TPB pairs a
+// block, one block a CU, so that a CU stages each site's rows once.
+constexpr int GLC_TPB = 512;
 template <int A>
-__global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm_codes(
+__global__ __launch_bounds__(GLC_TPB, 2) void sweep_glm_codes(
     const uint32_t* __restrict__ mask, uint32_t row_words, const uint32_t* __restrict__ codes,
     const double* __restrict__ lutg, const uint32_t* __restrict__ seg_bounds, const uint32_t* __restrict__ sx,
     const uint32_t* __restrict__ sy, const double* __restrict__ pt, const int* __restrict__ active,
@@ -1763,11 +1765,11 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm_codes(
     constexpr int G = A * (A + 1) / 2, R = G * G + 1;  // rows a site
     constexpr int SUB = A == 3 ? 8 : 4;                // sites a stage
     constexpr int N2 = SUB * R * GLC_RW / 2;           // double2 a stage
-    constexpr int PER = (N2 + SWEEP_TPB - 1) / SWEEP_TPB;
+    constexpr int PER = (N2 + GLC_TPB - 1) / GLC_TPB;
     static_assert(CHUNK % SUB == 0 && (SUB * R * GLC_RW) % 2 == 0, "stages inside a chunk, whole double2");
     __shared__ __attribute__((aligned(16))) double2 lut[2][N2];
     const uint32_t tid = threadIdx.x;
-    const uint32_t slot = blockIdx.x * SWEEP_TPB + tid;
+    const uint32_t slot = blockIdx.x * GLC_TPB + tid;
     const uint32_t seg = blockIdx.y;
     const bool live = slot < nslots && active[slot] != 0;
     if (!__syncthreads_or(live)) return;
@@ -1801,14 +1803,14 @@ __global__ __launch_bounds__(SWEEP_TPB, 2) void sweep_glm_codes(
         const double2* src = reinterpret_cast<const double2*>(lutg + (size_t)sb * R * GLC_RW);
 #pragma unroll
         for (int i = 0; i < PER; i++) {
-            const uint32_t e = tid + (uint32_t)i * SWEEP_TPB;
+            const uint32_t e = tid + (uint32_t)i * GLC_TPB;
             if (e < (uint32_t)N2) nx[i] = src[e];
         }
     };
     auto store = [&](int b) {
 #pragma unroll
         for (int i = 0; i < PER; i++) {
-            const uint32_t e = tid + (uint32_t)i * SWEEP_TPB;
+            const uint32_t e = tid + (uint32_t)i * GLC_TPB;
             if (e < (uint32_t)N2) lut[b][e] = nx[i];
         }
     };
@@ -3250,11 +3252,12 @@ hipError_t sweep_multi(int alleles, const uint32_t* geno, uint32_t row_words, co
                        double eps, double* partial, hipStream_t st, const float* table32, const uint32_t* codes,
                        const double* lut) {
     dim3 grid((nslots + SWEEP_TPB - 1) / SWEEP_TPB, nseg);
+    const dim3 gcodes((nslots + GLC_TPB - 1) / GLC_TPB, nseg);
     if (codes && !table32 && alleles == 3)  // hard calls: the genotype pairs' rows (the same bits)
-        hipLaunchKernelGGL(sweep_glm_codes<3>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, codes, lut,
+        hipLaunchKernelGGL(sweep_glm_codes<3>, gcodes, dim3(GLC_TPB), 0, st, geno, row_words, codes, lut,
                            seg_bounds, sx, sy, pt, active, nslots, v_pad, eps, partial);
     else if (codes && !table32 && alleles == 4)
-        hipLaunchKernelGGL(sweep_glm_codes<4>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, codes, lut,
+        hipLaunchKernelGGL(sweep_glm_codes<4>, gcodes, dim3(GLC_TPB), 0, st, geno, row_words, codes, lut,
                            seg_bounds, sx, sy, pt, active, nslots, v_pad, eps, partial);
     else if (table32 && alleles == 3)
         hipLaunchKernelGGL(sweep_glm_mixed<3>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, table32,
