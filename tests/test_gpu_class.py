@@ -164,3 +164,30 @@ def test_class_list_limit_keeps_records():
             co.evaluate(pairs, pts)
         co.set_option("class_list_mib", 0)
         assert_bitexact(co.fit_pairs(), f0)
+
+
+def test_last_fit_form_names_the_sweep():
+    """kestrel_last_fit_form: the class-sorted form for FP64 fits of 2-bit
+    rows, the site-table form with option "class_sweep" 0 or in the mixed
+    form, the wavefront form with "pair_lanes" 64; records of one form do not
+    depend on the others having run."""
+    say("case: last fit form, 40 x 3000")
+    from kestrel_amd import _lib
+    c, co, codes, afk = class_cohort(40, 3000, 41, 0.02)
+    with co:
+        a = co.fit_pairs()
+        assert co.last_fit_stats()["form"] == "class"
+        co.set_option("class_sweep", 0)
+        co.fit_pairs()
+        assert co.last_fit_stats()["form"] == "table"
+        co.set_option("class_sweep", 1)
+        co.set_option("pair_lanes", 64)
+        co.fit_pairs()
+        assert co.last_fit_stats()["form"] == "lanes"
+        co.set_option("pair_lanes", 1)
+        co.set_option("precision", _lib.MIXED)
+        co.fit_pairs()
+        assert co.last_fit_stats()["form"] == "table"
+        co.set_option("precision", _lib.FP64)
+        assert_bitexact(co.fit_pairs(), a)
+        assert co.last_fit_stats()["form"] == "class"
