@@ -941,13 +941,17 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
         DevBuf<unsigned long long> wrun;  // claims in runs: each workgroup's unclaimed run part
         RunClaim rc;
         ClassLists cls;                   // the class-sorted sweep's pattern lists
-        std::vector<hipEvent_t> ev;
+        std::vector<hipEvent_t> ev;       // per batch buffer: start and end of each sweep
         std::vector<int> hist_h;
+        hipEvent_t fin[2] = {nullptr, nullptr};  // per batch buffer: its live counts are on the host
+        bool queued[2] = {false, false};          // per batch buffer: this group has a batch in it
         int live_before = 0;
         hipGraph_t graph = nullptr;     // option "graph": the BATCH sweeps and steps, captured once
         hipGraphExec_t exec = nullptr;
         ~Group() {
             for (hipEvent_t x : ev)
+                if (x) (void)hipEventDestroy(x);
+            for (hipEvent_t x : fin)
                 if (x) (void)hipEventDestroy(x);
             if (exec) (void)hipGraphExecDestroy(exec);
             if (graph) (void)hipGraphDestroy(graph);
@@ -975,11 +979,12 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
             if (rc) return rc;
         }
         HIPCHK(G.active.alloc(ns));
-        HIPCHK(G.hist.alloc(BATCH));
+        HIPCHK(G.hist.alloc(2 * BATCH));
         HIPCHK(G.st.alloc(ns));
-        G.ev.assign(2 * BATCH, nullptr);
-        G.hist_h.assign(BATCH, 0);
+        G.ev.assign(4 * BATCH, nullptr);
+        G.hist_h.assign(2 * BATCH, 0);
         for (hipEvent_t& x : G.ev) HIPCHK(hipEventCreate(&x));
+        for (hipEvent_t& x : G.fin) HIPCHK(hipEventCreate(&x));
         HIPCHK(hipMemsetAsync(G.st.p, 0, (size_t)ns * sizeof(SlotState), G.stream));  // PH_IDLE
         HIPCHK(hipMemsetAsync(G.sx.p, 0, (size_t)ns * sizeof(uint32_t), G.stream));
         HIPCHK(hipMemsetAsync(G.sy.p, 0, (size_t)ns * sizeof(uint32_t), G.stream));
@@ -1037,83 +1042,108 @@ int kestrel_fit_pairs(kestrel_cohort* c, const kestrel_tune* tune, const uint32_
             if (grp[g].live_before > 0) return true;
         return false;
     };
-    while (any_live()) {
+    // Batches are double-buffered (hist, events): without a graph, batch k + 1
+    // is queued before the host waits for batch k, so the device does not drain
+    // at each host synchronisation.  A group is queued while its last known live
+    // count is positive; a batch queued after its group ran out sweeps and steps
+    // no live slot (nothing is claimed or written) and is not counted.
+    const bool pipe = !use_graph;
+    auto enqueue = [&](int p) -> int {
         if (launched > max_sweeps) return fail(KESTREL_EDEVICE, "pair fit did not terminate after %llu sweeps",
                                                (unsigned long long)launched);
-        for (int g = 0; g < ngroups; g++)
-            if (grp[g].live_before > 0)
-                HIPCHK(hipMemsetAsync(grp[g].hist.p, 0, BATCH * sizeof(int), grp[g].stream));
+        for (int g = 0; g < ngroups; g++) {
+            Group& G = grp[g];
+            G.queued[p] = G.live_before > 0;
+            if (G.queued[p]) HIPCHK(hipMemsetAsync(G.hist.p + p * BATCH, 0, BATCH * sizeof(int), G.stream));
+        }
         for (int g = 0; g < ngroups && use_graph; g++) {
             Group& G = grp[g];
-            if (G.live_before <= 0) continue;
-            HIPCHK(hipEventRecord(G.ev[0], G.stream));
+            if (!G.queued[p]) continue;
+            HIPCHK(hipEventRecord(G.ev[p * 2 * BATCH], G.stream));
             HIPCHK(hipGraphLaunch(G.exec, G.stream));
-            HIPCHK(hipEventRecord(G.ev[1], G.stream));
+            HIPCHK(hipEventRecord(G.ev[p * 2 * BATCH + 1], G.stream));
         }
         for (int i = 0; i < BATCH && !use_graph; i++)
             for (int g = 0; g < ngroups; g++) {
                 Group& G = grp[g];
-                if (G.live_before <= 0) continue;
+                if (!G.queued[p]) continue;
                 // (the class lists of newly claimed pairs are built outside the
                 // timed interval: it times the sweep kernel alone)
                 if (use_class)
                     HIPCHK(klaunch::class_build(c->geno.p, c->row_words, G.sx.p, G.sy.p, G.active.p, ns, 0, ns,
                                                 G.cls.nblk, G.cls.lists.p, G.cls.counts.p, G.cls.built.p, G.stream));
-                HIPCHK(hipEventRecord(G.ev[2 * i], G.stream));
+                HIPCHK(hipEventRecord(G.ev[p * 2 * BATCH + 2 * i], G.stream));
                 int rs = launch_sweeps(c, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, tune->epsilon, G.part.p, G.stream,
                                        use_runs, lanes, use_class ? &G.cls : nullptr, false);
                 if (rs) return rs;
-                HIPCHK(hipEventRecord(G.ev[2 * i + 1], G.stream));
+                HIPCHK(hipEventRecord(G.ev[p * 2 * BATCH + 2 * i + 1], G.stream));
                 HIPCHK(klaunch::step(G.st.p, G.sx.p, G.sy.p, G.pt.p, G.active.p, ns, G.part.p, nseg_fit, dpairs.p,
-                                     n_pairs, dorder_p, dnext.p, dout.p, dt, dlive.p, PairQueue{}, G.hist.p + i, G.stream,
-                                     G.rc));
+                                     n_pairs, dorder_p, dnext.p, dout.p, dt, dlive.p, PairQueue{},
+                                     G.hist.p + p * BATCH + i, G.stream, G.rc));
             }
         launched += BATCH;
         for (int g = 0; g < ngroups; g++) {
             Group& G = grp[g];
-            if (G.live_before <= 0) continue;
-            HIPCHK(hipMemcpyAsync(G.hist_h.data(), G.hist.p, BATCH * sizeof(int), hipMemcpyDeviceToHost, G.stream));
+            if (!G.queued[p]) continue;
+            HIPCHK(hipMemcpyAsync(G.hist_h.data() + p * BATCH, G.hist.p + p * BATCH, BATCH * sizeof(int),
+                                  hipMemcpyDeviceToHost, G.stream));
+            HIPCHK(hipEventRecord(G.fin[p], G.stream));
         }
+        return KESTREL_OK;
+    };
+    // waits for the batch in buffer p and counts (and times) its sweeps that
+    // ran with live slots
+    auto process = [&](int p) -> int {
         for (int g = 0; g < ngroups; g++)
-            if (grp[g].live_before > 0) HIPCHK(hipStreamSynchronize(grp[g].stream));
-        for (int g = 0; g < ngroups && use_graph; g++) {
+            if (grp[g].queued[p]) HIPCHK(hipEventSynchronize(grp[g].fin[p]));
+        for (int g = 0; g < ngroups; g++) {
             Group& G = grp[g];
-            if (G.live_before <= 0) continue;
-            float ms = 0, t_start = 0, t_end = 0;
-            HIPCHK(hipEventElapsedTime(&ms, G.ev[0], G.ev[1]));
-            HIPCHK(hipEventElapsedTime(&t_start, ref, G.ev[0]));
-            HIPCHK(hipEventElapsedTime(&t_end, ref, G.ev[1]));
-            sweep_ms += ms;  // the whole batch: sweeps, folds and steps
-            spans.emplace_back(t_start, t_end);
-            end_ms = std::max<double>(end_ms, t_end);
+            if (!G.queued[p]) continue;
+            const int* hist = G.hist_h.data() + p * BATCH;
+            const hipEvent_t* ev = G.ev.data() + p * 2 * BATCH;
             for (int i = 0; i < BATCH; i++) {
-                if (G.live_before > 0) {
-                    sweeps++;
-                    if (G.hist_h[i] < (int)ns && t_end < exhaust_ms) exhaust_ms = t_end;
-                }
-                G.live_before = G.hist_h[i];
-            }
-        }
-        for (int g = 0; g < ngroups && !use_graph; g++) {
-            Group& G = grp[g];
-            if (G.live_before <= 0) continue;
-            for (int i = 0; i < BATCH; i++) {
-                if (G.live_before > 0) {
+                if (G.live_before > 0 && (!use_graph || i == 0)) {
+                    // with a graph, one interval covers the whole batch
+                    // (sweeps, folds and steps)
                     float ms = 0, t_start = 0, t_end = 0;
-                    HIPCHK(hipEventElapsedTime(&ms, G.ev[2 * i], G.ev[2 * i + 1]));
-                    HIPCHK(hipEventElapsedTime(&t_start, ref, G.ev[2 * i]));
-                    HIPCHK(hipEventElapsedTime(&t_end, ref, G.ev[2 * i + 1]));
+                    const int j = use_graph ? 0 : 2 * i;
+                    HIPCHK(hipEventElapsedTime(&ms, ev[j], ev[j + 1]));
+                    HIPCHK(hipEventElapsedTime(&t_start, ref, ev[j]));
+                    HIPCHK(hipEventElapsedTime(&t_end, ref, ev[j + 1]));
                     sweep_ms += ms;
-                    sweeps++;
                     spans.emplace_back(t_start, t_end);
-                    // a slot goes idle only when no pair is left to claim
-                    if (G.hist_h[i] < (int)ns && t_end < exhaust_ms) exhaust_ms = t_end;
                     end_ms = std::max<double>(end_ms, t_end);
                 }
-                G.live_before = G.hist_h[i];
+                if (G.live_before > 0) {
+                    sweeps++;
+                    // a slot goes idle only when no pair is left to claim
+                    if (hist[i] < (int)ns) {
+                        float t_end = 0;
+                        HIPCHK(hipEventElapsedTime(&t_end, ref, ev[use_graph ? 1 : 2 * i + 1]));
+                        if (t_end < exhaust_ms) exhaust_ms = t_end;
+                    }
+                }
+                G.live_before = hist[i];
             }
         }
+        return KESTREL_OK;
+    };
+    {
+        const int rs = enqueue(0);
+        if (rs) return rs;
     }
+    for (int k = 0;; k++) {
+        if (pipe && any_live()) {
+            const int rs = enqueue((k + 1) & 1);
+            if (rs) return rs;
+        }
+        int rs = process(pipe ? k & 1 : 0);
+        if (rs) return rs;
+        if (!any_live()) break;
+        if (!pipe && (rs = enqueue(0))) return rs;
+    }
+    // a batch queued beyond the last live one
+    for (int g = 0; g < ngroups; g++) HIPCHK(hipStreamSynchronize(grp[g].stream));
     // device time with at least one sweep running (the launches of the two
     // groups overlap)
     std::sort(spans.begin(), spans.end());
