@@ -1753,11 +1753,12 @@ __global__ __launch_bounds__(256) void part_lut_kernel(const double* __restrict_
 // 2 G likelihoods and the mixture; the site update and the summation order
 // are sweep_glm's, so the records are the same bit for bit.  The block stages
 // the rows of SUB sites at a time in LDS, the next SUB sites' rows loaded into
-// registers while these sites are computed (two LDS buffers); GLC_TPB pairs a
-// block, one block a CU, so that a CU stages each site's rows once.
+// registers while these sites are computed (two LDS buffers); TPB pairs a
+// block: 512, one block a CU, so that a CU stages each site's rows once, or
+// 256 when blocks of 512 would leave CUs without one (few segments).
 constexpr int GLC_TPB = 512;
-template <int A>
-__global__ __launch_bounds__(GLC_TPB, 2) void sweep_glm_codes(
+template <int A, int TPB>
+__global__ __launch_bounds__(TPB, 2) void sweep_glm_codes(
     const uint32_t* __restrict__ mask, uint32_t row_words, const uint32_t* __restrict__ codes,
     const double* __restrict__ lutg, const uint32_t* __restrict__ seg_bounds, const uint32_t* __restrict__ sx,
     const uint32_t* __restrict__ sy, const double* __restrict__ pt, const int* __restrict__ active,
@@ -1765,11 +1766,12 @@ __global__ __launch_bounds__(GLC_TPB, 2) void sweep_glm_codes(
     constexpr int G = A * (A + 1) / 2, R = G * G + 1;  // rows a site
     constexpr int SUB = A == 3 ? 8 : 4;                // sites a stage
     constexpr int N2 = SUB * R * GLC_RW / 2;           // double2 a stage
-    constexpr int PER = (N2 + GLC_TPB - 1) / GLC_TPB;
-    static_assert(CHUNK % SUB == 0 && (SUB * R * GLC_RW) % 2 == 0, "stages inside a chunk, whole double2");
+    constexpr int PER = (N2 + TPB - 1) / TPB;
+    static_assert(8 % SUB == 0 && CHUNK % 8 == 0 && (SUB * R * GLC_RW) % 2 == 0,
+                  "stages inside a chunk and a code word, whole double2");
     __shared__ __attribute__((aligned(16))) double2 lut[2][N2];
     const uint32_t tid = threadIdx.x;
-    const uint32_t slot = blockIdx.x * GLC_TPB + tid;
+    const uint32_t slot = blockIdx.x * TPB + tid;
     const uint32_t seg = blockIdx.y;
     const bool live = slot < nslots && active[slot] != 0;
     if (!__syncthreads_or(live)) return;
@@ -1803,14 +1805,14 @@ __global__ __launch_bounds__(GLC_TPB, 2) void sweep_glm_codes(
         const double2* src = reinterpret_cast<const double2*>(lutg + (size_t)sb * R * GLC_RW);
 #pragma unroll
         for (int i = 0; i < PER; i++) {
-            const uint32_t e = tid + (uint32_t)i * GLC_TPB;
+            const uint32_t e = tid + (uint32_t)i * TPB;
             if (e < (uint32_t)N2) nx[i] = src[e];
         }
     };
     auto store = [&](int b) {
 #pragma unroll
         for (int i = 0; i < PER; i++) {
-            const uint32_t e = tid + (uint32_t)i * GLC_TPB;
+            const uint32_t e = tid + (uint32_t)i * TPB;
             if (e < (uint32_t)N2) lut[b][e] = nx[i];
         }
     };
@@ -1834,28 +1836,34 @@ __global__ __launch_bounds__(GLC_TPB, 2) void sweep_glm_codes(
     for (uint32_t sb = s0; sb < s1; sb += SUB) {
         if (sb + SUB < s1) fetch(sb + SUB);
         const double* lb = reinterpret_cast<const double*>(lut[b]);
-#pragma unroll 1
-        for (uint32_t t = sb; t < sb + SUB; t++) {
-            const int k = (int)((t - s0) & (CHUNK - 1));
-            if (k == 0) {
-                wx = nwx;
-                wy = nwy;
-                if (t + CHUNK < s1) {
-                    nwx = *reinterpret_cast<const uint2*>(rowx + (t + CHUNK) / 16);
-                    nwy = *reinterpret_cast<const uint2*>(rowy + (t + CHUNK) / 16);
-                }
-                const uint32_t ux = ~((wx.x & (wx.x >> 1)) | (wy.x & (wy.x >> 1))) & 0x55555555u;
-                const uint32_t uy = ~((wx.y & (wx.y >> 1)) | (wy.y & (wy.y >> 1))) & 0x55555555u;
-                N += (double)(__popc(ux) + __popc(uy));
+        // a stage lies inside one chunk and one code word (SUB divides 8, and
+        // stages start at chunk boundaries): the words move at its first site
+        const int k0 = (int)((sb - s0) & (CHUNK - 1));
+        if (k0 == 0) {
+            wx = nwx;
+            wy = nwy;
+            if (sb + CHUNK < s1) {
+                nwx = *reinterpret_cast<const uint2*>(rowx + (sb + CHUNK) / 16);
+                nwy = *reinterpret_cast<const uint2*>(rowy + (sb + CHUNK) / 16);
             }
-            if (k % 8 == 0) {
-                qx = nqx;
-                qy = nqy;
-                if (t + 8 < s1) {
-                    nqx = cx[(t + 8) / 8];
-                    nqy = cy[(t + 8) / 8];
-                }
+            const uint32_t ux = ~((wx.x & (wx.x >> 1)) | (wy.x & (wy.x >> 1))) & 0x55555555u;
+            const uint32_t uy = ~((wx.y & (wx.y >> 1)) | (wy.y & (wy.y >> 1))) & 0x55555555u;
+            N += (double)(__popc(ux) + __popc(uy));
+        }
+        if (k0 % 8 == 0) {
+            qx = nqx;
+            qy = nqy;
+            if (sb + 8 < s1) {
+                nqx = cx[(sb + 8) / 8];
+                nqy = cy[(sb + 8) / 8];
             }
+        }
+        // two sites a pass, so that a site's row is read from LDS while the
+        // previous site is computed
+#pragma unroll 2
+        for (int j = 0; j < SUB; j++) {
+            const uint32_t t = sb + (uint32_t)j;
+            const int k = k0 + j;
             const uint32_t ax = (k < 16 ? wx.x : wx.y) >> (2 * (k & 15));
             const uint32_t ay = (k < 16 ? wy.x : wy.y) >> (2 * (k & 15));
             const uint32_t gx = (qx >> (4 * (k & 7))) & 15u, gy = (qy >> (4 * (k & 7))) & 15u;
@@ -3252,12 +3260,28 @@ hipError_t sweep_multi(int alleles, const uint32_t* geno, uint32_t row_words, co
                        double eps, double* partial, hipStream_t st, const float* table32, const uint32_t* codes,
                        const double* lut) {
     dim3 grid((nslots + SWEEP_TPB - 1) / SWEEP_TPB, nseg);
-    const dim3 gcodes((nslots + GLC_TPB - 1) / GLC_TPB, nseg);
-    if (codes && !table32 && alleles == 3)  // hard calls: the genotype pairs' rows (the same bits)
-        hipLaunchKernelGGL(sweep_glm_codes<3>, gcodes, dim3(GLC_TPB), 0, st, geno, row_words, codes, lut,
+    // hard calls: the genotype pairs' rows (the same bits); blocks of 256
+    // pairs when blocks of 512 would not give every CU one
+    int dev = 0, cus = 0;
+    if (codes && !table32) {
+        const hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) return e;
+        const hipError_t e2 = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e2 != hipSuccess) return e2;
+    }
+    const bool half = (uint64_t)((nslots + GLC_TPB - 1) / GLC_TPB) * nseg < (uint64_t)cus;
+    const dim3 gcodes((nslots + GLC_TPB - 1) / GLC_TPB, nseg), ghalf((nslots + 255) / 256, nseg);
+    if (codes && !table32 && alleles == 3 && !half)
+        hipLaunchKernelGGL((sweep_glm_codes<3, GLC_TPB>), gcodes, dim3(GLC_TPB), 0, st, geno, row_words, codes, lut,
+                           seg_bounds, sx, sy, pt, active, nslots, v_pad, eps, partial);
+    else if (codes && !table32 && alleles == 3)
+        hipLaunchKernelGGL((sweep_glm_codes<3, 256>), ghalf, dim3(256), 0, st, geno, row_words, codes, lut,
+                           seg_bounds, sx, sy, pt, active, nslots, v_pad, eps, partial);
+    else if (codes && !table32 && alleles == 4 && !half)
+        hipLaunchKernelGGL((sweep_glm_codes<4, GLC_TPB>), gcodes, dim3(GLC_TPB), 0, st, geno, row_words, codes, lut,
                            seg_bounds, sx, sy, pt, active, nslots, v_pad, eps, partial);
     else if (codes && !table32 && alleles == 4)
-        hipLaunchKernelGGL(sweep_glm_codes<4>, gcodes, dim3(GLC_TPB), 0, st, geno, row_words, codes, lut,
+        hipLaunchKernelGGL((sweep_glm_codes<4, 256>), ghalf, dim3(256), 0, st, geno, row_words, codes, lut,
                            seg_bounds, sx, sy, pt, active, nslots, v_pad, eps, partial);
     else if (table32 && alleles == 3)
         hipLaunchKernelGGL(sweep_glm_mixed<3>, grid, dim3(SWEEP_TPB), 0, st, geno, row_words, gl, v_pad, table32,

@@ -228,3 +228,28 @@ def test_multi_onehot_class_form(O):
         t = co.fit_pairs()
         assert co.last_fit_stats()["form"] == "table"
         assert_bitexact(t[sel], O.multi_fit_pairs(parts, pairs[sel], order=O.ORDER_SEG, pads=pads, seg=seg))
+
+
+def test_multi_onehot_block_shapes_agree(O):
+    """The hard-call parts' sweep runs blocks of 512 pairs when they give
+    every CU one, and blocks of 256 otherwise; both shapes give the same bits.
+    140,000 evaluation points on one segment (274 blocks of 512, more than
+    the 256 CUs of an MI355X) against 40 of them evaluated alone (one block of
+    256), a few of which are checked against the oracle."""
+    say("case: hard-call parts in blocks of 512 and of 256, 48 x (1500, 300, 120), 140000 points")
+    b2, b3, b4 = synth.multi_onehot(synth.multi_genotypes(48, (1500, 300, 120), seed=71, missing=0.02))
+    parts, _ = O.multi_prepare({2: b2, 3: b3, 4: b4}, maf=0.0)
+    pads, seg = device_order(parts)
+    rng = np.random.default_rng(6)
+    n = 140_000
+    ep = rng.integers(0, 48, (n, 2))
+    pts = rng.dirichlet(np.ones(9), n)
+    idx = rng.choice(n, 40, replace=False)
+    with Cohort.gl_multi(b2, b3, b4, maf=0.0) as co:
+        big = co.evaluate(ep, pts)
+        small = co.evaluate(ep[idx], pts[idx])
+    for a, b in zip(small, big):
+        assert same(a, b[idx]).all()
+    for i in range(4):
+        fe, ge, he = O.multi_class_eval(parts, ep[idx[i], 0], ep[idx[i], 1], pads, seg, pts[idx[i]])
+        assert small[0][i] == fe and np.array_equal(small[1][i], ge) and np.array_equal(small[2][i], he), i
